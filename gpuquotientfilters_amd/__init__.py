@@ -1,0 +1,350 @@
+"""MI355X-native quotient filters (SQF + RSQF) -- Python binding of libgqf.so.
+
+The library is a C ABI (``include/gqf.h``) over hand-written HIP kernels for
+gfx950.  This module binds it with ctypes and mirrors the reference host API
+names so scripts read like the reference drivers:
+
+    qf = SQF.initFilterGPU(q, r)                 # quotientFilter.cuh:33
+    ms = qf.bulkBuildSegmentedLayouts(d_keys, n, NoDuplicates=True)
+    ms = qf.launchUnsortedLookups(d_keys, n, d_out)
+    ms = qf.superclusterDeletes(d_keys, n)
+
+    cqf = RSQF.initCQFGPU(q, r)                  # RSQF.cuh:41 (r is runtime here)
+    ms = cqf.insertGPU(d_keys, n, d_out)
+    ms = cqf.launchLookups(d_keys, n, d_out)     # sorted
+
+Device memory is a :class:`DeviceArray` (library-allocated, numpy round
+trips).  Nothing here computes on the CPU: if ``libgqf.so`` is missing the
+import of any entry point raises :class:`GQFError`.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgqf.so")
+
+NOT_FOUND = 0xFFFFFFFF
+OK, ERR_INVALID, ERR_CAPACITY, ERR_HIP, ERR_NOMEM, ERR_UNSUPPORTED = range(6)
+BUILD_SEGMENTED_LAYOUTS, BUILD_PARALLEL_MERGING, BUILD_SEQUENTIAL_SHIFTS = 0, 1, 2
+
+
+class GQFError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"gqf status {status}: {msg}")
+        self.status = status
+
+
+class CapacityError(GQFError):
+    pass
+
+
+class _SQF(C.Structure):
+    _fields_ = [("qbits", C.c_uint), ("rbits", C.c_uint), ("bytesPerElement", C.c_uint), ("table", C.c_void_p)]
+
+
+class _RSQF(C.Structure):
+    _fields_ = [("qbits", C.c_uint), ("numBlocks", C.c_uint), ("blocks", C.c_void_p), ("rbits", C.c_uint)]
+
+
+_st = C.c_int
+_vp = C.c_void_p
+_f = C.POINTER(C.c_float)
+_SIGS = {
+    "gqf_last_error": (C.c_char_p, []),
+    "gqf_status_string": (C.c_char_p, [_st]),
+    "gqf_version": (C.c_char_p, []),
+    "gqf_device_count": (_st, [C.POINTER(C.c_int)]),
+    "gqf_malloc": (_st, [C.POINTER(_vp), C.c_size_t]),
+    "gqf_free": (_st, [_vp]),
+    "gqf_memcpy_htod": (_st, [_vp, _vp, C.c_size_t]),
+    "gqf_memcpy_dtoh": (_st, [_vp, _vp, C.c_size_t]),
+    "gqf_memcpy_dtod": (_st, [_vp, _vp, C.c_size_t]),
+    "gqf_memset": (_st, [_vp, C.c_int, C.c_size_t]),
+    "gqf_synchronize": (_st, []),
+    "gqf_sqf_table_bytes": (C.c_size_t, [C.c_uint, C.c_uint]),
+    "gqf_sqf_num_slots": (C.c_uint32, [C.c_uint]),
+    "gqf_sqf_init": (_st, [C.POINTER(_SQF), C.c_uint, C.c_uint]),
+    "gqf_sqf_destroy": (_st, [C.POINTER(_SQF)]),
+    "gqf_sqf_bulk_build": (_st, [_SQF, C.c_int, C.c_int, _vp, C.c_int, _vp, _f]),
+    "gqf_sqf_insert": (_st, [_SQF, C.c_int, _vp, _vp, _f]),
+    "gqf_sqf_insert_via_merge": (_st, [_SQF, _vp, C.c_int, _vp, C.c_int, C.c_int, _vp, _f]),
+    "gqf_sqf_lookup": (_st, [_SQF, C.c_int, C.c_int, _vp, _vp, _vp, _f]),
+    "gqf_sqf_delete": (_st, [_SQF, C.c_int, _vp, _vp, _f]),
+    "gqf_sqf_merge": (_st, [_SQF, _SQF, C.c_int, _vp, _f]),
+    "gqf_rsqf_num_blocks": (C.c_uint32, [C.c_uint]),
+    "gqf_rsqf_table_bytes": (C.c_size_t, [C.c_uint, C.c_uint]),
+    "gqf_rsqf_ref_bytes": (C.c_size_t, [C.c_uint, C.c_uint]),
+    "gqf_rsqf_init": (_st, [C.POINTER(_RSQF), C.c_uint, C.c_uint]),
+    "gqf_rsqf_destroy": (_st, [C.POINTER(_RSQF)]),
+    "gqf_rsqf_bulk_build": (_st, [_RSQF, C.c_int, _vp, _vp, _f]),
+    "gqf_rsqf_insert": (_st, [_RSQF, C.c_int, _vp, _vp, _vp, _f]),
+    "gqf_rsqf_lookup": (_st, [_RSQF, C.c_int, C.c_int, _vp, _vp, _vp, _f]),
+    "gqf_rsqf_delete": (_st, [_RSQF, C.c_int, _vp, _vp, _f]),
+    "gqf_rsqf_dump": (_st, [_RSQF, _vp]),
+    "gqf_rsqf_load": (_st, [_RSQF, _vp]),
+}
+EXPORTED = tuple(_SIGS)
+
+_lib = None
+
+
+def lib():
+    """The loaded libgqf.so (raises GQFError if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise GQFError(ERR_UNSUPPORTED, f"{LIB_PATH} not built: run `python -m gpuquotientfilters_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status):
+    if status != OK:
+        msg = lib().gqf_last_error().decode(errors="replace")
+        if status == ERR_CAPACITY:
+            raise CapacityError(status, msg)
+        raise GQFError(status, msg)
+
+
+def device_count():
+    n = C.c_int(0)
+    st = lib().gqf_device_count(C.byref(n))
+    return n.value if st == OK else 0
+
+
+def synchronize():
+    check(lib().gqf_synchronize())
+
+
+class DeviceArray:
+    """A device allocation owned by Python (freed on garbage collection)."""
+
+    def __init__(self, nbytes, dtype=np.uint32, count=None):
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(nbytes)
+        self.count = int(count if count is not None else self.nbytes // self.dtype.itemsize)
+        p = C.c_void_p()
+        check(lib().gqf_malloc(C.byref(p), max(1, self.nbytes)))
+        self.ptr = p.value
+
+    @classmethod
+    def empty(cls, count, dtype=np.uint32):
+        return cls(int(count) * np.dtype(dtype).itemsize, dtype, count)
+
+    @classmethod
+    def from_host(cls, arr, dtype=None):
+        a = np.ascontiguousarray(arr, dtype=dtype)
+        d = cls(a.nbytes, a.dtype, a.size)
+        if a.nbytes:
+            check(lib().gqf_memcpy_htod(d.ptr, a.ctypes.data, a.nbytes))
+        return d
+
+    def to_host(self, count=None):
+        n = self.count if count is None else int(count)
+        out = np.empty(n, dtype=self.dtype)
+        if out.nbytes:
+            check(lib().gqf_memcpy_dtoh(out.ctypes.data, self.ptr, out.nbytes))
+        return out
+
+    def fill(self, byte):
+        check(lib().gqf_memset(self.ptr, int(byte), self.nbytes))
+
+    def __len__(self):
+        return self.count
+
+    def free(self):
+        if getattr(self, "ptr", None) and _lib is not None:
+            _lib.gqf_free(self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _ptr(x):
+    if x is None:
+        return None
+    if isinstance(x, DeviceArray):
+        return x.ptr
+    if isinstance(x, int):
+        return x
+    if hasattr(x, "data_ptr"):  # torch tensor on the device
+        return x.data_ptr()
+    raise TypeError(f"expected a device array, got {type(x)}")
+
+
+def _n(keys, n):
+    if n is not None:
+        return int(n)
+    return len(keys)
+
+
+class SQF:
+    """Standard quotient filter (quotientFilter.cuh:25-64 API)."""
+
+    def __init__(self, q, r):
+        self.h = _SQF()
+        check(lib().gqf_sqf_init(C.byref(self.h), q, r))
+        self.q, self.r = q, r
+
+    # reference names -------------------------------------------------
+    @classmethod
+    def initFilterGPU(cls, q, r):
+        return cls(q, r)
+
+    @staticmethod
+    def calcNumSlotsGPU(q, r):
+        """Table size in bytes (the reference function's name)."""
+        return lib().gqf_sqf_table_bytes(q, r)
+
+    @staticmethod
+    def num_slots(q):
+        return lib().gqf_sqf_num_slots(q)
+
+    def _ms(self, fn, *args):
+        ms = C.c_float(0)
+        check(fn(*args, C.byref(ms)))
+        return ms.value
+
+    def bulk_build(self, keys, n=None, NoDuplicates=True, method=BUILD_SEGMENTED_LAYOUTS, stream=None):
+        return self._ms(lib().gqf_sqf_bulk_build, self.h, method, _n(keys, n), _ptr(keys), int(bool(NoDuplicates)), stream)
+
+    def bulkBuildSegmentedLayouts(self, keys, n=None, NoDuplicates=True):
+        return self.bulk_build(keys, n, NoDuplicates, BUILD_SEGMENTED_LAYOUTS)
+
+    def bulkBuildParallelMerging(self, keys, n=None, NoDuplicates=True):
+        return self.bulk_build(keys, n, NoDuplicates, BUILD_PARALLEL_MERGING)
+
+    def bulkBuildSequentialShifts(self, keys, n=None, NoDuplicates=True):
+        return self.bulk_build(keys, n, NoDuplicates, BUILD_SEQUENTIAL_SHIFTS)
+
+    bulkBuildEarlyExit = bulkBuildSequentialShifts  # README.md:37 name
+
+    def insert(self, keys, n=None, stream=None):
+        return self._ms(lib().gqf_sqf_insert, self.h, _n(keys, n), _ptr(keys), stream)
+
+    def insertViaMerge(self, new_keys, n_new=None, NoDuplicates=True, old_keys=None, n_old=0, stream=None):
+        return self._ms(lib().gqf_sqf_insert_via_merge, self.h, _ptr(old_keys), int(n_old), _ptr(new_keys),
+                        _n(new_keys, n_new), int(bool(NoDuplicates)), stream)
+
+    def lookup(self, keys, out, n=None, sorted=False, stream=None):
+        return self._ms(lib().gqf_sqf_lookup, self.h, int(bool(sorted)), _n(keys, n), _ptr(keys), _ptr(out), stream)
+
+    def launchUnsortedLookups(self, keys, out, n=None):
+        return self.lookup(keys, out, n, sorted=False)
+
+    def launchSortedLookups(self, keys, out, n=None):
+        return self.lookup(keys, out, n, sorted=True)
+
+    def superclusterDeletes(self, keys, n=None, stream=None):
+        return self._ms(lib().gqf_sqf_delete, self.h, _n(keys, n), _ptr(keys), stream)
+
+    delete = superclusterDeletes
+
+    def mergeTwoFilters(self, other, NoDuplicates=True, stream=None):
+        return self._ms(lib().gqf_sqf_merge, self.h, other.h, int(bool(NoDuplicates)), stream)
+
+    # host views --------------------------------------------------------
+    def table_bytes(self):
+        nb = lib().gqf_sqf_table_bytes(self.q, self.r)
+        out = np.empty(nb, dtype=np.uint8)
+        check(lib().gqf_memcpy_dtoh(out.ctypes.data, self.h.table, nb))
+        return out
+
+    def load_table(self, table):
+        t = np.ascontiguousarray(table, dtype=np.uint8)
+        assert t.size == lib().gqf_sqf_table_bytes(self.q, self.r)
+        check(lib().gqf_memcpy_htod(self.h.table, t.ctypes.data, t.nbytes))
+
+    def clear(self):
+        check(lib().gqf_memset(self.h.table, 0, lib().gqf_sqf_table_bytes(self.q, self.r)))
+
+    def destroy(self):
+        if getattr(self, "h", None) is not None and self.h.table and _lib is not None:
+            _lib.gqf_sqf_destroy(C.byref(self.h))
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class RSQF:
+    """Rank-and-select quotient filter (RSQF.cuh:21-67 API, runtime r)."""
+
+    def __init__(self, q, r=5):
+        self.h = _RSQF()
+        check(lib().gqf_rsqf_init(C.byref(self.h), q, r))
+        self.q, self.r = q, r
+
+    @classmethod
+    def initCQFGPU(cls, q, r=5):
+        return cls(q, r)
+
+    @staticmethod
+    def calcNumBlocksGPU(q):
+        return lib().gqf_rsqf_num_blocks(q)
+
+    def _ms(self, fn, *args):
+        ms = C.c_float(0)
+        check(fn(*args, C.byref(ms)))
+        return ms.value
+
+    def bulk_build(self, keys, n=None, stream=None):
+        return self._ms(lib().gqf_rsqf_bulk_build, self.h, _n(keys, n), _ptr(keys), stream)
+
+    bulkBuild = bulk_build
+
+    def insertGPU(self, keys, out=None, n=None, stream=None):
+        return self._ms(lib().gqf_rsqf_insert, self.h, _n(keys, n), _ptr(keys), _ptr(out), stream)
+
+    insert = insertGPU
+
+    def lookup(self, keys, out, n=None, sorted=True, stream=None):
+        return self._ms(lib().gqf_rsqf_lookup, self.h, int(bool(sorted)), _n(keys, n), _ptr(keys), _ptr(out), stream)
+
+    def launchLookups(self, keys, out, n=None):
+        return self.lookup(keys, out, n, sorted=True)
+
+    def launchUnsortedLookups(self, keys, out, n=None):
+        return self.lookup(keys, out, n, sorted=False)
+
+    def deleteGPU(self, keys, n=None, stream=None):
+        return self._ms(lib().gqf_rsqf_delete, self.h, _n(keys, n), _ptr(keys), stream)
+
+    delete = deleteGPU
+
+    def dump(self):
+        """Reference packed-block layout (RSQF.cuh:24-30) as host bytes."""
+        out = np.empty(lib().gqf_rsqf_ref_bytes(self.q, self.r), dtype=np.uint8)
+        check(lib().gqf_rsqf_dump(self.h, out.ctypes.data))
+        return out
+
+    def load(self, ref_layout):
+        t = np.ascontiguousarray(ref_layout, dtype=np.uint8)
+        assert t.size == lib().gqf_rsqf_ref_bytes(self.q, self.r)
+        check(lib().gqf_rsqf_load(self.h, t.ctypes.data))
+
+    def clear(self):
+        check(lib().gqf_memset(self.h.blocks, 0, lib().gqf_rsqf_table_bytes(self.q, self.r)))
+
+    def destroy(self):
+        if getattr(self, "h", None) is not None and self.h.blocks and _lib is not None:
+            _lib.gqf_rsqf_destroy(C.byref(self.h))
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass

@@ -1,0 +1,89 @@
+// gqf_runtime.hpp -- host-side runtime of libgqf: error reporting, the
+// per-device workspace arena, and the stream/timing scaffold shared by every
+// entry point.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <mutex>
+
+#include "../../include/gqf.h"
+
+namespace gqf {
+
+// ---------------------------------------------------------------- errors
+void set_error(const char* fmt, ...);
+gqf_status hip_fail(hipError_t e, const char* what, const char* file, int line);
+
+#define GQF_HIP(x)                                                      \
+  do {                                                                  \
+    hipError_t _e = (x);                                                \
+    if (_e != hipSuccess) return ::gqf::hip_fail(_e, #x, __FILE__, __LINE__); \
+  } while (0)
+
+#define GQF_TRY(x)                      \
+  do {                                  \
+    gqf_status _s = (x);                \
+    if (_s != GQF_OK) return _s;        \
+  } while (0)
+
+// ---------------------------------------------------------------- device state
+// Small device counters used by the kernels (index constants below).
+enum Counter : int {
+  CTR_SORT_TILE = 0,   // 8 slots: dynamic tile ids of the radix passes
+  CTR_SCAN_TILE = 8,   // dynamic tile id of the layout scan
+  CTR_COUNT = 9,       // number of entries kept by the layout scan
+  CTR_ERR = 10,        // error flags raised on the device (bit 0: capacity)
+  CTR_MISC = 11,       // 5 general-purpose slots
+  CTR_TOTAL = 16
+};
+constexpr uint32_t DEV_ERR_CAPACITY = 1u;
+// Look-back words carry 30-bit counts: one call handles fewer entries.
+constexpr uint64_t GQF_MAX_ITEMS = (1ull << 30) - 4096;
+
+struct DeviceState {
+  int device = -1;
+  char* ws = nullptr;               // workspace arena (grow-only)
+  size_t ws_cap = 0;
+  unsigned long long* status = nullptr;  // look-back status words, zeroed at allocation
+  size_t status_cap = 0;                 // in words
+  uint32_t* counters = nullptr;     // CTR_TOTAL device counters
+  uint32_t* h_pinned = nullptr;     // pinned host mirror of the counters
+  uint32_t epoch = 0;               // radix-pass epoch (status words are tagged with it)
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+// Lock held by every entry point: calls on one process are serialized (the
+// reference API is single-threaded and blocking as well).
+std::mutex& api_mutex();
+gqf_status device_state(DeviceState** out);
+gqf_status ws_reserve(DeviceState& ds, size_t bytes);
+gqf_status status_reserve(DeviceState& ds, size_t words);
+
+// Bump allocator over the workspace of one call.
+struct Arena {
+  char* base;
+  size_t cap;
+  size_t used = 0;
+  Arena(char* b, size_t c) : base(b), cap(c) {}
+  template <typename T>
+  T* take(size_t count) {
+    size_t off = (used + 255) & ~size_t(255);
+    used = off + count * sizeof(T);
+    return reinterpret_cast<T*>(base + off);
+  }
+};
+__host__ __device__ inline size_t aligned_bytes(size_t bytes) { return (bytes + 255) & ~size_t(255); }
+
+// Timing/blocking scaffold: record start, run body, record stop, sync, read
+// the device error counter, report ms.
+struct CallScope {
+  DeviceState* ds;
+  hipStream_t stream;
+  gqf_status begin(gqf_stream s);
+  gqf_status end(float* ms, bool check_device_err);
+};
+
+}  // namespace gqf

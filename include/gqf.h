@@ -1,0 +1,185 @@
+/* gqf.h -- C ABI of the MI355X quotient-filter engine (libgqf.so).
+ *
+ * Plain C: device pointers, sizes and status codes; no HIP or torch types in
+ * any signature (a stream is passed as an opaque pointer; NULL = null stream).
+ * Every entry point replaces a reference host function, cited as
+ * file:line, where
+ *   quotientFilter.cu/.cuh = standardQuotientFilter/GPU_version/
+ *   RSQF.cu/.cuh           = rankSelectQuotientFilter/GPU_version/
+ *
+ * Conventions kept from the reference:
+ *   - a filter is one device allocation owned by the caller-visible handle
+ *     (freed with gqf_*_destroy, or hipFree on the pointer);
+ *   - an all-zero table is an empty filter;
+ *   - key arrays are caller-owned device buffers and are never modified;
+ *   - lookup results are slot indices, 0xFFFFFFFF (SQF UINT_MAX / RSQF -1) on
+ *     a miss;
+ *   - every call blocks until its work is done and reports its elapsed
+ *     device time in milliseconds through *ms (may be NULL).
+ * Differences: status codes instead of asserts/aborts; capacity overflow is
+ * reported (GQF_ERR_CAPACITY) and leaves the table unchanged instead of
+ * writing out of bounds; key 0xFFFFFFFF is an ordinary key.
+ */
+#ifndef GQF_H
+#define GQF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum gqf_status {
+  GQF_OK = 0,
+  GQF_ERR_INVALID = 1,   /* bad argument: q/r out of range, NULL pointer, n < 0 */
+  GQF_ERR_CAPACITY = 2,  /* the result does not fit the table; table left unchanged */
+  GQF_ERR_HIP = 3,       /* HIP runtime error (message in gqf_last_error) */
+  GQF_ERR_NOMEM = 4,     /* device allocation failed */
+  GQF_ERR_UNSUPPORTED = 5
+} gqf_status;
+
+#define GQF_NOT_FOUND 0xFFFFFFFFu
+
+/* Opaque stream handle (a hipStream_t); NULL selects the null stream. */
+typedef void* gqf_stream;
+
+/* Last error message of the calling thread (empty string if none). */
+const char* gqf_last_error(void);
+const char* gqf_status_string(gqf_status s);
+/* Library version string, e.g. "gqf 0.1 gfx950". */
+const char* gqf_version(void);
+
+/* ====================================================================== SQF
+ * Standard quotient filter.  Same layout as the reference struct
+ * quotient_filter (quotientFilter.cuh:25-31) and the same table format:
+ * floor(2^q * 1.1) slots of (r+3)/8 bytes, big-endian, element =
+ * remainder<<3 | occupied<<2 | continuation<<1 | shifted.
+ * Supported: (r + 3) % 8 == 0 (r = 5 or 13), q >= 1, q + r <= 32. */
+typedef struct gqf_sqf {
+  unsigned int qbits;
+  unsigned int rbits;
+  unsigned int bytesPerElement;
+  unsigned char* table;
+} gqf_sqf;
+
+enum gqf_sqf_build_method {
+  GQF_BUILD_SEGMENTED_LAYOUTS = 0,  /* bulkBuildSegmentedLayouts (quotientFilter.cu:1446) */
+  GQF_BUILD_PARALLEL_MERGING = 1,   /* bulkBuildParallelMerging  (quotientFilter.cu:737)  */
+  GQF_BUILD_SEQUENTIAL_SHIFTS = 2   /* bulkBuildSequentialShifts (quotientFilter.cu:930)
+                                       == README "bulkBuildEarlyExit" */
+};
+
+/* calcNumSlotsGPU (quotientFilter.cu:37-42): table size in BYTES. */
+size_t gqf_sqf_table_bytes(unsigned int q, unsigned int r);
+/* Number of slots: (1 << q) * 1.1 (quotientFilter.cu:174). */
+uint32_t gqf_sqf_num_slots(unsigned int q);
+
+/* initFilterGPU (quotientFilter.cu:44-55).  Allocates and zeroes the table. */
+gqf_status gqf_sqf_init(gqf_sqf* qf, unsigned int q, unsigned int r);
+/* hipFree of qf->table; sets it to NULL. */
+gqf_status gqf_sqf_destroy(gqf_sqf* qf);
+
+/* bulkBuild{SegmentedLayouts,ParallelMerging,SequentialShifts}
+ * (quotientFilter.cu:1446-1559, 737-878, 930-1058).  Replaces the table
+ * contents with the canonical layout of the n keys' fingerprints
+ * (duplicates removed when no_dup != 0). */
+gqf_status gqf_sqf_bulk_build(gqf_sqf qf, int method, int n, const uint32_t* d_keys, int no_dup,
+                              gqf_stream stream, float* ms);
+
+/* insert (quotientFilter.cu:533-634).  Deduplicating insert: a fingerprint
+ * already stored is not added again. */
+gqf_status gqf_sqf_insert(gqf_sqf qf, int n, const uint32_t* d_keys, gqf_stream stream, float* ms);
+
+/* insertViaMerge (quotientFilter.cu:1602-1742).  Extract the stored
+ * fingerprints, merge with the new keys', rebuild.  d_old/n_old are accepted
+ * for signature compatibility and not read (the reference does not read them
+ * either). */
+gqf_status gqf_sqf_insert_via_merge(gqf_sqf qf, const uint32_t* d_old, int n_old,
+                                    const uint32_t* d_new, int n_new, int no_dup,
+                                    gqf_stream stream, float* ms);
+
+/* launchUnsortedLookups (quotientFilter.cu:359-381) when sorted == 0,
+ * launchSortedLookups (quotientFilter.cu:272-318) when sorted != 0.  Both
+ * write, for each key, the slot of the first matching remainder of its run,
+ * or GQF_NOT_FOUND. */
+gqf_status gqf_sqf_lookup(gqf_sqf qf, int sorted, int n, const uint32_t* d_keys, uint32_t* d_out,
+                          gqf_stream stream, float* ms);
+
+/* superclusterDeletes (quotientFilter.cu:1202-1297).  Removes one stored
+ * copy per key whose fingerprint is present. */
+gqf_status gqf_sqf_delete(gqf_sqf qf, int n, const uint32_t* d_keys, gqf_stream stream, float* ms);
+
+/* mergeTwoFilters (quotientFilter.cu:1744-1891).  dst := merge of dst and
+ * src (same q and r); src unchanged. */
+gqf_status gqf_sqf_merge(gqf_sqf dst, gqf_sqf src, int no_dup, gqf_stream stream, float* ms);
+
+/* ===================================================================== RSQF
+ * Rank-and-select quotient filter.  The first three fields match the
+ * reference struct countingQuotientFilterGPU (RSQF.cuh:32-37); rbits is
+ * appended because the reference fixes RBITS at compile time.  The device
+ * layout is the project's (see gqf_rsqf_table_bytes); gqf_rsqf_dump/load
+ * convert to and from the reference packed block layout
+ * {u8 offset; u64 occupieds; u64 runEnds; u64 remainders[r]} (RSQF.cuh:24-30).
+ * Supported: 1 <= r <= 16, 6 <= q <= 30. */
+typedef struct gqf_rsqf {
+  unsigned int qbits;
+  unsigned int numBlocks;
+  void* blocks;
+  unsigned int rbits;
+} gqf_rsqf;
+
+/* calcNumBlocksGPU (RSQF.cu:27-30): ceil(2^q / 64) + 1. */
+uint32_t gqf_rsqf_num_blocks(unsigned int q);
+/* Device bytes of the project layout. */
+size_t gqf_rsqf_table_bytes(unsigned int q, unsigned int r);
+/* Bytes of the reference packed layout (what dump/load exchange). */
+size_t gqf_rsqf_ref_bytes(unsigned int q, unsigned int r);
+
+/* initCQFGPU (RSQF.cu:32-40): allocate and zero. */
+gqf_status gqf_rsqf_init(gqf_rsqf* cqf, unsigned int q, unsigned int r);
+gqf_status gqf_rsqf_destroy(gqf_rsqf* cqf);
+
+/* Bulk build (no reference counterpart; README.md:74 names it as missing).
+ * Replaces the contents with the canonical layout of the keys' fingerprints
+ * (duplicates kept, as the reference insert keeps them). */
+gqf_status gqf_rsqf_bulk_build(gqf_rsqf cqf, int n, const uint32_t* d_keys, gqf_stream stream,
+                               float* ms);
+
+/* insertGPU (RSQF.cu:882-988).  End state: canonical layout of the stored
+ * multiset plus the new fingerprints.  d_out (may be NULL) receives, per key,
+ * its slot in the resulting table (what a lookup of that key returns). */
+gqf_status gqf_rsqf_insert(gqf_rsqf cqf, int n, const uint32_t* d_keys, int32_t* d_out,
+                           gqf_stream stream, float* ms);
+
+/* launchUnsortedLookups (RSQF.cu:540-563) when sorted == 0, launchLookups
+ * (RSQF.cu:411-456) when sorted != 0: the LAST matching slot of the run,
+ * or -1. */
+gqf_status gqf_rsqf_lookup(gqf_rsqf cqf, int sorted, int n, const uint32_t* d_keys,
+                           int32_t* d_out, gqf_stream stream, float* ms);
+
+/* Delete (no reference counterpart): removes one stored copy per key whose
+ * fingerprint is present; result is the canonical layout of what remains. */
+gqf_status gqf_rsqf_delete(gqf_rsqf cqf, int n, const uint32_t* d_keys, gqf_stream stream,
+                           float* ms);
+
+/* Copy the table to host memory in the reference packed layout
+ * (gqf_rsqf_ref_bytes bytes), and back. */
+gqf_status gqf_rsqf_dump(gqf_rsqf cqf, void* host_ref_layout);
+gqf_status gqf_rsqf_load(gqf_rsqf cqf, const void* host_ref_layout);
+
+/* ================================================================ helpers
+ * Device memory for callers without a HIP toolchain (ctypes and the like). */
+gqf_status gqf_device_count(int* count);
+gqf_status gqf_malloc(void** d_ptr, size_t bytes);
+gqf_status gqf_free(void* d_ptr);
+gqf_status gqf_memcpy_htod(void* d_dst, const void* h_src, size_t bytes);
+gqf_status gqf_memcpy_dtoh(void* h_dst, const void* d_src, size_t bytes);
+gqf_status gqf_memcpy_dtod(void* d_dst, const void* d_src, size_t bytes);
+gqf_status gqf_memset(void* d_ptr, int value, size_t bytes);
+gqf_status gqf_synchronize(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GQF_H */

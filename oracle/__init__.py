@@ -29,6 +29,7 @@ _sz, _u, _i, _d = C.c_size_t, C.c_uint, C.c_int, C.c_double
 _SIGS = {
     "orc_mt19937": (None, [C.c_uint32, C.c_uint64, _sz, _u32p]),
     "orc_ap_rounds": (C.c_uint32, [C.c_uint32, C.c_uint32]),
+    "orc_fmix64": (C.c_uint64, [C.c_uint64]),
     "orc_fp_ap": (C.c_uint64, [C.c_uint32, _u, _u]),
     "orc_fp_fnv": (C.c_uint64, [C.c_uint32, _u, _u]),
     "orc_fingerprints": (None, [_u32p, _sz, _u, _u, _i, _u64p]),

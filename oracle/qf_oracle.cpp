@@ -146,14 +146,23 @@ uint32_t orc_ap_rounds(uint32_t key, uint32_t state) {
 
 // Fingerprint of width q+r.  q+r <= 31: AP(key) mod 2^(q+r) exactly as the
 // reference.  q+r == 32: AP(key).  q+r > 32 (undefined in the reference, whose
-// modulus 1<<(q+r) overflows): low 32 bits = AP(key), high bits = AP rounds from
-// state 0x55555555, truncated to q+r bits.
+// modulus 1<<(q+r) overflows): the MurmurHash3 64-bit finalizer (fmix64,
+// Appleby, public domain) of (AP(key) << 32 | key), truncated to q+r bits.
+// The input word is injective in the key, so the result is uniform; AP alone
+// (or any function of it) reaches only about a third of the 32-bit range.
+uint64_t orc_fmix64(uint64_t k) {
+  k ^= k >> 33;
+  k *= 0xff51afd7ed558ccdull;
+  k ^= k >> 33;
+  k *= 0xc4ceb9fe1a85ec53ull;
+  k ^= k >> 33;
+  return k;
+}
 uint64_t orc_fp_ap(uint32_t key, unsigned q, unsigned r) {
   unsigned bits = q + r;
   uint32_t lo = orc_ap_rounds(key, 0xAAAAAAAAu);
   if (bits <= 32) return lo & low_mask32(bits);
-  uint64_t hi = orc_ap_rounds(key, 0x55555555u);
-  return ((hi << 32) | lo) & low_mask64(bits);
+  return orc_fmix64(((uint64_t)lo << 32) | key) & low_mask64(bits);
 }
 
 // fnv_hash (quotientFilter.cpp:97-112): FNV-1, multiply then xor, big-endian bytes.

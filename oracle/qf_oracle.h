@@ -5,7 +5,7 @@
  * cpu_baseline leg use it, and only as the checker / CPU baseline.
  *
  * Contents (each function cites the reference code it restates):
- *   - hashes: Normal_APHash (+ the wide-fingerprint extension for q+r > 32),
+ *   - hashes: Normal_APHash (+ the project's wide fingerprint for q+r > 32),
  *     FNV-1 (CPU_version hash);
  *   - SQF byte-slot codec (GPU_version layout) and packed-word codec
  *     (CPU_version layout);
@@ -38,6 +38,7 @@ void orc_mt19937(uint32_t seed, uint64_t skip, size_t n, uint32_t* out);
 
 /* ---------------- hashes ---------------- */
 uint32_t orc_ap_rounds(uint32_t key, uint32_t state);
+uint64_t orc_fmix64(uint64_t k);
 uint64_t orc_fp_ap(uint32_t key, unsigned q, unsigned r);
 uint64_t orc_fp_fnv(uint32_t key, unsigned q, unsigned r);
 /* hash_kind: 0 = AP (wide rule above 32 bits), 1 = FNV */

@@ -1,0 +1,65 @@
+"""CPU-side checks of the C ABI: libgqf.so loads and exports every entry
+point include/gqf.h declares; geometry helpers agree with the oracle.  No
+compute call needs a GPU here."""
+import os
+import re
+
+import pytest
+
+from conftest import ROOT
+
+
+def _declared():
+    text = open(os.path.join(ROOT, "include", "gqf.h")).read()
+    return sorted(set(re.findall(r"\b(gqf_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_expected_entry_points():
+    names = _declared()
+    for must in ["gqf_sqf_init", "gqf_sqf_bulk_build", "gqf_sqf_insert", "gqf_sqf_insert_via_merge",
+                 "gqf_sqf_lookup", "gqf_sqf_delete", "gqf_sqf_merge", "gqf_rsqf_init", "gqf_rsqf_bulk_build",
+                 "gqf_rsqf_insert", "gqf_rsqf_lookup", "gqf_rsqf_delete", "gqf_rsqf_dump", "gqf_rsqf_load"]:
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    import gpuquotientfilters_amd as G
+
+    L = G.lib()
+    missing = [n for n in _declared() if not hasattr(L, n)]
+    assert not missing, missing
+    # and the binding's table covers the header
+    assert set(_declared()) <= set(G.EXPORTED)
+
+
+def test_geometry_matches_oracle(orc):
+    import gpuquotientfilters_amd as G
+
+    L = G.lib()
+    for q in (8, 12, 18, 20, 26):
+        for r in (5, 13):
+            if q + r <= 32:
+                assert L.gqf_sqf_table_bytes(q, r) == orc.sqf_table_bytes(q, r)
+        assert L.gqf_sqf_num_slots(q) == orc.sqf_num_slots(q)
+        assert L.gqf_rsqf_num_blocks(q) == orc.rsqf_num_blocks(q)
+        for r in (5, 8):
+            assert L.gqf_rsqf_ref_bytes(q, r) == orc.rsqf_table_bytes(q, r)
+
+
+def test_version_and_errors():
+    import gpuquotientfilters_amd as G
+
+    assert G.lib().gqf_version().decode().startswith("gqf")
+    assert G.lib().gqf_status_string(2).decode() == "capacity exceeded"
+
+
+def test_invalid_geometry_rejected_without_gpu():
+    import ctypes as C
+
+    import gpuquotientfilters_amd as G
+
+    h = G._SQF()
+    # (r + 3) % 8 != 0 is rejected before any device call
+    assert G.lib().gqf_sqf_init(C.byref(h), 10, 6) == G.ERR_INVALID
+    h2 = G._RSQF()
+    assert G.lib().gqf_rsqf_init(C.byref(h2), 10, 40) == G.ERR_INVALID

@@ -1,0 +1,167 @@
+"""RSQF on the GPU through the C ABI.
+
+Lookups must equal the reference lookup on any table -- the golden tables
+written by the compiled reference CPU insert (history-dependent layouts with
+holes) included.  Build / insert / delete must equal the canonical layout."""
+import numpy as np
+import pytest
+
+import gpuquotientfilters_amd as G
+import oracle as O
+from conftest import golden
+from gpu_helpers import dev, fresh_keys, require_gpu, rsqf_lookup, say
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    require_gpu()
+
+
+def canon(keys, q, r):
+    st, t = O.rsqf_canon(O.fingerprints(keys, q, r), q, r)
+    assert st == O.OK
+    return t
+
+
+@pytest.mark.parametrize("name", ["rsqf_r5_q10_a0.6.npz", "rsqf_r5_q12_a0.9.npz"])
+@pytest.mark.parametrize("sorted_", [False, True])
+def test_lookup_on_reference_tables(name, sorted_):
+    g = golden(name)
+    q, r = int(g["q"]), int(g["r"])
+    for tab, res in (("table_random", "lookup_random"), ("table_ascending", "lookup_ascending")):
+        say("rsqf golden lookup", name, tab, sorted_)
+        f = G.RSQF(q, r)
+        f.load(g[tab])
+        assert (f.dump() == g[tab]).all()
+        got = rsqf_lookup(f, g["lookup_keys"], sorted=sorted_)
+        assert (got == g[res]).all()
+
+
+CASES = [(10, 5, 0.6), (12, 5, 0.9), (14, 8, 0.9), (16, 8, 0.95), (18, 16, 0.9), (20, 5, 0.9), (20, 8, 0.8)]
+
+
+@pytest.mark.parametrize("q,r,alpha", CASES)
+def test_build_equals_canonical(q, r, alpha):
+    say("rsqf build", q, r, alpha)
+    n = O.key_count(alpha, q)
+    keys = O.mt19937(1, n)
+    keys[n - n // 8:] = keys[: n // 8]  # duplicates are stored
+    f = G.RSQF(q, r)
+    f.bulk_build(dev(keys))
+    want = canon(keys, q, r)
+    got = f.dump()
+    assert (got == want).all(), np.flatnonzero(got != want)[:8]
+
+
+@pytest.mark.parametrize("q,r,alpha", [(12, 5, 0.9), (16, 8, 0.9), (18, 16, 0.9), (20, 8, 0.9)])
+@pytest.mark.parametrize("sorted_", [False, True])
+def test_lookup_matches_oracle(q, r, alpha, sorted_):
+    say("rsqf lookup", q, r, alpha, sorted_)
+    n = O.key_count(alpha, q)
+    keys = O.mt19937(1, n)
+    t = canon(keys, q, r)
+    f = G.RSQF(q, r)
+    f.load(t)
+    probe = np.concatenate([keys, fresh_keys(3, n, keys)])
+    got = rsqf_lookup(f, probe, sorted=sorted_)
+    assert (got == O.rsqf_lookup_keys(t, q, r, probe)).all()
+    assert (got[:n] != G.NOT_FOUND).all()
+
+
+@pytest.mark.parametrize("q,r", [(12, 5), (16, 8), (18, 16)])
+def test_insert_batches_equal_canonical(q, r):
+    say("rsqf insert", q, r)
+    n = O.key_count(0.9, q)
+    keys = O.mt19937(2, n)
+    start = O.key_count(0.7, q)
+    f = G.RSQF(q, r)
+    f.bulk_build(dev(keys[:start]))
+    step = max(1, 1 << (q - 6))
+    for i in range(start, n, step):
+        b = keys[i:i + step]
+        out = G.DeviceArray.empty(len(b))
+        f.insertGPU(dev(b), out)
+    want = canon(keys, q, r)
+    assert (f.dump() == want).all()
+    # the per-key output is the key's slot in the final table
+    probe = keys[start:]
+    assert (rsqf_lookup(f, probe) == O.rsqf_lookup_keys(want, q, r, probe)).all()
+
+
+def test_insert_output_is_final_slot():
+    q, r = 12, 8
+    keys = O.mt19937(8, 3000)
+    f = G.RSQF(q, r)
+    f.bulk_build(dev(keys[:2000]))
+    out = G.DeviceArray.empty(1000)
+    f.insertGPU(dev(keys[2000:]), out)
+    t = canon(keys, q, r)
+    assert (out.to_host() == O.rsqf_lookup_keys(t, q, r, keys[2000:])).all()
+
+
+@pytest.mark.parametrize("q,r,frac", [(12, 5, 0.1), (16, 8, 0.5), (18, 16, 0.2)])
+def test_delete_equals_canonical(q, r, frac):
+    say("rsqf delete", q, r, frac)
+    n = O.key_count(0.9, q)
+    keys = O.mt19937(4, n)
+    base = np.concatenate([keys, keys[:100]])
+    f = G.RSQF(q, r)
+    f.bulk_build(dev(base))
+    d = np.concatenate([keys[: int(n * frac)], keys[:50], O.mt19937(44, 300)])
+    f.deleteGPU(dev(d))
+    fb = O.fingerprints(base, q, r)
+    vals, cnt = np.unique(fb, return_counts=True)
+    dv, dc = np.unique(O.fingerprints(d, q, r), return_counts=True)
+    dmap = dict(zip(dv.tolist(), dc.tolist()))
+    rest = np.repeat(vals, [max(0, c - dmap.get(v, 0)) for v, c in zip(vals.tolist(), cnt.tolist())])
+    st, want = O.rsqf_canon(rest, q, r)
+    assert (f.dump() == want).all()
+
+
+def test_edge_cases_and_capacity():
+    q, r = 8, 8
+    f = G.RSQF(q, r)
+    f.bulk_build(dev(np.zeros(0, np.uint32)), n=0)
+    assert (f.dump() == 0).all()
+    k = np.array([0xFFFFFFFF, 0, 7], np.uint32)
+    f.insertGPU(dev(k))
+    assert (f.dump() == canon(k, q, r)).all()
+    before = f.dump()
+    cap = O.rsqf_num_blocks(q) * 64
+    with pytest.raises(G.CapacityError):
+        f.insertGPU(dev(np.arange(cap + 5, dtype=np.uint32)))
+    assert (f.dump() == before).all()
+
+
+def test_full_size_q26_r8_properties():
+    """RSQF at q=26, r=8 (34-bit wide fingerprints), fill 0.9: no false
+    negatives, sorted == unsorted, FPR band, stored-entry count."""
+    q, r = 26, 8
+    n = O.key_count(0.9, q)
+    say("rsqf full size", q, r, n)
+    keys = O.mt19937(1, n)
+    f = G.RSQF(q, r)
+    f.bulk_build(dev(keys))
+    m = 1 << 20
+    probe = np.concatenate([keys[: m // 2], fresh_keys(5, m // 2, keys)])
+    u = rsqf_lookup(f, probe, sorted=False)
+    s = rsqf_lookup(f, probe, sorted=True)
+    assert (u == s).all()
+    assert (u[: m // 2] != G.NOT_FOUND).all()
+    # a quotient filter is exact on fingerprints: hit <=> fingerprint stored
+    fk = O.fingerprints(keys, q, r)
+    assert ((u != G.NOT_FOUND) == np.isin(O.fingerprints(probe, q, r), fk)).all()
+    # the wide fingerprint is uniform: FPR at alpha * 2^-r
+    p = n / 2.0 ** (q + r)
+    rate = (u[m // 2:] != G.NOT_FOUND).mean()
+    assert abs(rate - p) < 6 * np.sqrt(p * (1 - p) / (m // 2))
+    t = f.dump()
+    off, occ, ends, rem = O.rsqf_fields(t, q, r)
+    runs = int(sum(bin(int(x)).count("1") for x in ends[:: 97]))
+    occs = int(sum(bin(int(x)).count("1") for x in occ[:: 97]))
+    assert runs > 0 and occs > 0
+    total_runs = int(np.unpackbits(ends.view(np.uint8)).sum())
+    total_occ = int(np.unpackbits(occ.view(np.uint8)).sum())
+    assert total_runs == total_occ == len(np.unique(O.fingerprints(keys, q, r) >> np.uint64(r)))

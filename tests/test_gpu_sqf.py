@@ -1,0 +1,237 @@
+"""SQF on the GPU through the C ABI, bit-exact against the oracle.
+
+Tables must equal the canonical layout byte for byte; lookups must equal the
+oracle's restatement of hashAndLookUp (quotientFilter.cu:320-357)."""
+import numpy as np
+import pytest
+
+import gpuquotientfilters_amd as G
+import oracle as O
+from gpu_helpers import dev, fresh_keys, require_gpu, say, sqf_canon, sqf_lookup
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    require_gpu()
+
+
+BUILD_CASES = [(10, 5, 0.6), (14, 5, 0.8), (16, 5, 0.95), (12, 13, 0.8), (16, 13, 0.9), (20, 5, 0.8)]
+
+
+@pytest.mark.parametrize("q,r,alpha", BUILD_CASES)
+@pytest.mark.parametrize("no_dup", [True, False])
+@pytest.mark.parametrize("method", [G.BUILD_SEGMENTED_LAYOUTS, G.BUILD_PARALLEL_MERGING, G.BUILD_SEQUENTIAL_SHIFTS])
+def test_build_equals_canonical(q, r, alpha, no_dup, method):
+    say("sqf build", q, r, alpha, no_dup, method)
+    n = O.key_count(alpha, q)
+    keys = O.mt19937(1, n)
+    if not no_dup:  # make real duplicates appear
+        keys[n // 2:] = keys[: n - n // 2]
+    qf = G.SQF(q, r)
+    qf.bulk_build(dev(keys), NoDuplicates=no_dup, method=method)
+    want = sqf_canon(keys, q, r, no_dup)
+    got = qf.table_bytes()
+    assert (got == want).all(), f"first diff at byte {np.flatnonzero(got != want)[:5]}"
+
+
+@pytest.mark.parametrize("q,r,alpha", [(12, 5, 0.6), (16, 5, 0.9), (14, 13, 0.9), (20, 5, 0.8)])
+@pytest.mark.parametrize("sorted_", [False, True])
+def test_lookup_matches_oracle(q, r, alpha, sorted_):
+    say("sqf lookup", q, r, alpha, sorted_)
+    n = O.key_count(alpha, q)
+    keys = O.mt19937(1, n)
+    table = sqf_canon(keys, q, r, True)
+    qf = G.SQF(q, r)
+    qf.load_table(table)
+    probe = np.concatenate([keys, fresh_keys(2, n, keys)])
+    want = O.sqf_lookup_keys(table, q, r, probe)
+    got = sqf_lookup(qf, probe, sorted=sorted_)
+    assert (got == want).all()
+    assert (got[:n] != G.NOT_FOUND).all()  # no false negatives
+
+
+def test_lookup_on_reference_insert_table():
+    # a table produced by the faithful restatement of the reference insert loop
+    q, r = 13, 5
+    keys = O.mt19937(5, O.key_count(0.85, q))
+    t = np.zeros(O.sqf_table_bytes(q, r), np.uint8)
+    assert O.sqf_insert_keys(t, q, r, keys) == O.OK
+    qf = G.SQF(q, r)
+    qf.load_table(t)
+    probe = np.concatenate([keys, O.mt19937(6, 4000)])
+    assert (sqf_lookup(qf, probe) == O.sqf_lookup_keys(t, q, r, probe)).all()
+
+
+@pytest.mark.parametrize("q,r", [(14, 5), (16, 13), (18, 5)])
+def test_insert_batches_equal_sequential_reference_insert(q, r):
+    say("sqf insert", q, r)
+    n = O.key_count(0.85, q)
+    keys = O.mt19937(3, n)
+    half = O.key_count(0.5, q)
+    qf = G.SQF(q, r)
+    qf.bulk_build(dev(keys[:half]), NoDuplicates=True)
+    t = sqf_canon(keys[:half], q, r, True)
+    step = 1 << (q - 4)
+    for i in range(half, n, step):
+        batch = keys[i:i + step]
+        qf.insert(dev(batch))
+        assert O.sqf_insert_keys(t, q, r, batch) == O.OK
+    assert (qf.table_bytes() == t).all()
+
+
+def test_insert_dedups_against_dup_build():
+    q, r = 12, 5
+    keys = O.mt19937(9, 2000)
+    base = np.concatenate([keys[:800], keys[:300]])  # duplicates stored (Dup build)
+    qf = G.SQF(q, r)
+    qf.bulk_build(dev(base), NoDuplicates=False)
+    new = np.concatenate([keys[200:1200], keys[1000:1100]])
+    qf.insert(dev(new))
+    # per distinct fingerprint: max(stored count, [present in batch])
+    fb = O.fingerprints(base, q, r)
+    fn = O.fingerprints(new, q, r)
+    vals, cnt = np.unique(fb, return_counts=True)
+    extra = np.setdiff1d(np.unique(fn), vals)
+    want_multiset = np.concatenate([np.repeat(vals, cnt), extra])
+    st, want = O.sqf_canon(want_multiset, q, r, False)
+    assert (qf.table_bytes() == want).all()
+
+
+@pytest.mark.parametrize("q,r,frac", [(12, 5, 0.1), (16, 5, 0.5), (14, 13, 0.3), (18, 5, 0.1)])
+def test_delete_equals_reference_restatement(q, r, frac):
+    say("sqf delete", q, r, frac)
+    n = O.key_count(0.8, q)
+    keys = O.mt19937(1, n)
+    qf = G.SQF(q, r)
+    qf.bulk_build(dev(keys), NoDuplicates=True)
+    t = sqf_canon(keys, q, r, True)
+    d = np.concatenate([keys[: int(n * frac)], O.mt19937(77, 500)])
+    qf.superclusterDeletes(dev(d))
+    O.sqf_delete_keys(t, q, r, d)
+    assert (qf.table_bytes() == t).all()
+    got = sqf_lookup(qf, keys)
+    assert (got == O.sqf_lookup_keys(t, q, r, keys)).all()
+
+
+def test_delete_multiset_on_dup_table():
+    q, r = 12, 5
+    keys = O.mt19937(4, 1500)
+    base = np.concatenate([keys, keys[:400], keys[:100]])
+    qf = G.SQF(q, r)
+    qf.bulk_build(dev(base), NoDuplicates=False)
+    t = sqf_canon(base, q, r, False)
+    d = np.concatenate([keys[:50], keys[:50], keys[:250], keys[1400:]])
+    qf.delete(dev(d))
+    O.sqf_delete_keys(t, q, r, d)
+    assert (qf.table_bytes() == t).all()
+
+
+@pytest.mark.parametrize("no_dup", [True, False])
+def test_insert_via_merge(no_dup):
+    q, r = 15, 5
+    keys = O.mt19937(12, O.key_count(0.8, q))
+    old, new = keys[:15000], np.concatenate([keys[15000:], keys[:2000]])
+    qf = G.SQF(q, r)
+    qf.bulk_build(dev(old), NoDuplicates=no_dup)
+    qf.insertViaMerge(dev(new), NoDuplicates=no_dup)
+    want = sqf_canon(np.concatenate([old, new]), q, r, no_dup)
+    assert (qf.table_bytes() == want).all()
+
+
+@pytest.mark.parametrize("no_dup", [True, False])
+def test_merge_two_filters(no_dup):
+    q, r = 14, 5
+    keys = O.mt19937(13, O.key_count(0.8, q))
+    a, b = keys[:6000], keys[5000:]
+    f1, f2 = G.SQF(q, r), G.SQF(q, r)
+    f1.bulk_build(dev(a), NoDuplicates=True)
+    f2.bulk_build(dev(b), NoDuplicates=True)
+    before2 = f2.table_bytes()
+    f1.mergeTwoFilters(f2, NoDuplicates=no_dup)
+    fa = np.unique(O.fingerprints(a, q, r))
+    fb = np.unique(O.fingerprints(b, q, r))
+    st, want = O.sqf_canon(np.concatenate([fa, fb]), q, r, no_dup)
+    assert (f1.table_bytes() == want).all()
+    assert (f2.table_bytes() == before2).all()
+
+
+def test_edge_cases():
+    q, r = 10, 5
+    qf = G.SQF(q, r)
+    # empty input builds an empty table
+    qf.bulk_build(dev(np.zeros(0, np.uint32)), n=0)
+    assert (qf.table_bytes() == 0).all()
+    # 0xFFFFFFFF is an ordinary key (the reference's queue treats it as a sentinel)
+    k = np.array([0xFFFFFFFF, 0, 1, 0xFFFFFFFE], np.uint32)
+    qf.insert(dev(k))
+    assert (qf.table_bytes() == sqf_canon(k, q, r, True)).all()
+    assert (sqf_lookup(qf, k) != G.NOT_FOUND).all()
+    qf.delete(dev(k[:1]))
+    assert (qf.table_bytes() == sqf_canon(k[1:], q, r, True)).all()
+    # deleting absent keys changes nothing
+    before = qf.table_bytes()
+    qf.delete(dev(O.mt19937(99, 100)))
+    assert (qf.table_bytes() == before).all()
+
+
+def test_capacity_error_leaves_table_unchanged():
+    q, r = 8, 5
+    S = O.sqf_num_slots(q)
+    keys = O.mt19937(1, 200)
+    qf = G.SQF(q, r)
+    qf.bulk_build(dev(keys))
+    before = qf.table_bytes()
+    too_many = np.arange(S + 10, dtype=np.uint32) * 2654435761 % (1 << 32)
+    with pytest.raises(G.CapacityError):
+        qf.insert(dev(too_many.astype(np.uint32)))
+    assert (qf.table_bytes() == before).all()
+    with pytest.raises(G.CapacityError):
+        qf.bulk_build(dev(too_many.astype(np.uint32)), NoDuplicates=False)
+    assert (qf.table_bytes() == before).all()
+
+
+def test_deterministic_rebuild():
+    q, r = 18, 5
+    keys = O.mt19937(31, O.key_count(0.9, q))
+    a, b = G.SQF(q, r), G.SQF(q, r)
+    a.bulk_build(dev(keys))
+    b.bulk_build(dev(keys[::-1].copy()))
+    assert (a.table_bytes() == b.table_bytes()).all()
+
+
+def test_full_size_q26_properties():
+    """BASELINE config size (q=26, r=5, fill 0.8): checks that need no oracle
+    at this size -- no false negatives, FPR band, sorted == unsorted, extract
+    round trip count, and a delete+lookup round trip."""
+    q, r = 26, 5
+    n = O.key_count(0.8, q)
+    say("sqf full size", q, r, n)
+    keys = O.mt19937(1, n)
+    dk = dev(keys)
+    qf = G.SQF(q, r)
+    qf.bulkBuildSegmentedLayouts(dk, NoDuplicates=True)
+    m = 1 << 20
+    probe = np.concatenate([keys[: m // 2], fresh_keys(2, m // 2, keys)])
+    dp = dev(probe)
+    o1, o2 = G.DeviceArray.empty(m), G.DeviceArray.empty(m)
+    qf.launchUnsortedLookups(dp, o1)
+    qf.launchSortedLookups(dp, o2)
+    u, s = o1.to_host(), o2.to_host()
+    assert (u == s).all()
+    assert (u[: m // 2] != G.NOT_FOUND).all()
+    # exact on fingerprints: hit <=> the probe's fingerprint is stored (the FPR
+    # is then the reference AP hash's own collision rate at 31 bits)
+    fk = O.fingerprints(keys, q, r)
+    assert ((u != G.NOT_FOUND) == np.isin(O.fingerprints(probe, q, r), fk)).all()
+    nu = len(np.unique(fk))
+    # occupancy: number of non-empty slots == distinct fingerprints
+    t = qf.table_bytes()
+    assert int(((t & 7) != 0).sum()) == nu
+    # delete the first 2^20 keys, then they are absent unless a survivor shares the fingerprint
+    qf.delete(dev(keys[:m]))
+    t2 = qf.table_bytes()
+    fk = O.fingerprints(keys, q, r)
+    remaining = np.setdiff1d(np.unique(fk), np.unique(fk[:m]))
+    assert int(((t2 & 7) != 0).sum()) == len(remaining)

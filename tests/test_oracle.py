@@ -36,18 +36,49 @@ def test_ap_and_fnv_hash_golden(orc):
         assert (fnv == g["fnv"][i]).all(), bits
 
 
+def _fmix64_inverse(h):
+    """Inverse of the MurmurHash3 finalizer, from its definition (xor-shift by
+    33 is its own inverse on 64 bits; the multipliers are odd)."""
+    M = (1 << 64) - 1
+    inv1 = pow(0xFF51AFD7ED558CCD, -1, 1 << 64)
+    inv2 = pow(0xC4CEB9FE1A85EC53, -1, 1 << 64)
+    h ^= h >> 33
+    h = (h * inv2) & M
+    h ^= h >> 33
+    h = (h * inv1) & M
+    h ^= h >> 33
+    return h
+
+
 def test_wide_fingerprint_rule(orc):
     keys = orc.mt19937(5, 4096)
     lo = np.array([orc.lib().orc_ap_rounds(int(k), 0xAAAAAAAA) for k in keys], np.uint64)
-    hi = np.array([orc.lib().orc_ap_rounds(int(k), 0x55555555) for k in keys], np.uint64)
     # q + r == 32: the plain 32-bit AP value
     assert (orc.fingerprints(keys, 27, 5) == lo).all()
-    # q + r == 34: low 32 bits are AP, bits 32..33 from the second state
-    fp = orc.fingerprints(keys, 26, 8)
-    assert (fp & np.uint64(0xFFFFFFFF) == lo).all()
-    assert (fp >> np.uint64(32) == hi & np.uint64(3)).all()
     # below 32 bits: the reference modulus
     assert (orc.fingerprints(keys, 20, 5) == lo % np.uint64(1 << 25)).all()
+    # q + r > 32: fmix64(AP << 32 | key); inverting the finalizer recovers the input word
+    for k, a in zip(keys[:256], lo[:256]):
+        h = orc.lib().orc_fmix64((int(a) << 32) | int(k))
+        assert _fmix64_inverse(h) == (int(a) << 32) | int(k)
+        assert orc.fp_ap(int(k), 26, 8) == h & ((1 << 34) - 1)
+    # the published finalizer maps 0 to 0 and is not the identity elsewhere
+    assert orc.lib().orc_fmix64(0) == 0 and orc.lib().orc_fmix64(1) != 1
+
+
+def test_wide_fingerprint_is_uniform(orc):
+    # 34-bit fingerprints of 2^22 keys: distinct count matches the birthday
+    # bound of a uniform hash, and the top bits are balanced
+    n = 1 << 22
+    keys = np.unique(orc.mt19937(1, n))  # ~2^11 of the keys repeat; count distinct keys
+    k = len(keys)
+    fp = orc.fingerprints(keys, 26, 8)
+    d = len(np.unique(fp))
+    expect = k - k * (k - 1) / 2 / 2.0 ** 34
+    assert abs(d - expect) < 6 * np.sqrt(k * k / 2 / 2.0 ** 34)
+    top = np.bincount((fp >> np.uint64(26)).astype(np.int64), minlength=256)
+    chi2 = ((top - n / 256) ** 2 / (n / 256)).sum()
+    assert chi2 < 400  # 255 degrees of freedom
 
 
 def test_rank_select_golden_and_definition(orc):
