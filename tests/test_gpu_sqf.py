@@ -183,12 +183,13 @@ def test_capacity_error_leaves_table_unchanged():
     qf = G.SQF(q, r)
     qf.bulk_build(dev(keys))
     before = qf.table_bytes()
-    too_many = np.arange(S + 10, dtype=np.uint32) * 2654435761 % (1 << 32)
+    too_many = O.mt19937(7, S + 60)  # more distinct fingerprints than slots
+    assert len(np.unique(O.fingerprints(too_many, q, r))) > S
     with pytest.raises(G.CapacityError):
-        qf.insert(dev(too_many.astype(np.uint32)))
+        qf.insert(dev(too_many))
     assert (qf.table_bytes() == before).all()
     with pytest.raises(G.CapacityError):
-        qf.bulk_build(dev(too_many.astype(np.uint32)), NoDuplicates=False)
+        qf.bulk_build(dev(too_many), NoDuplicates=False)
     assert (qf.table_bytes() == before).all()
 
 
