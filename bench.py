@@ -1,0 +1,276 @@
+"""Benchmark of the quotient-filter hot path on one MI355X.
+
+Default workload: standard quotient filter, q=26, r=5, fill 0.8 -- 53,687,092
+MT19937(seed 1) uint32 keys.  One step is one pass of the whole SQF pipeline
+over that data, every input already resident in HBM:
+
+  1. bulk build, NoDup (segmented layouts)            53,687,092 keys
+  2. 2^24 unsorted lookups, then 2^24 sorted lookups  (half inserted keys,
+     half from an independent MT19937(2) stream)
+  3. delete 2^22 inserted keys
+  4. bulk build, Dup                                  53,687,092 keys
+  5. bulk build to fill 0.5 (33,554,432 keys), then incremental inserts in
+     batches of 2^20 up to fill 0.8 (20 batches, 20,132,660 keys)
+
+value = all operations of the step / step wall time (Mops/s).  Per-class
+rates, the dominant kernel's roofline (algorithmic bytes over its HIP-event
+time, measured in this run) and the CPU baseline (reference CPU_version code,
+or the project's restatement when that is not built, on a bounded sample of
+the same workload on this host) are printed in the same JSON line.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--workload sqf|rsqf]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def key_count(alpha, q):
+    """n = (int)(float alpha * 2^q), the reference drivers' rule."""
+    return int(np.float32(alpha) * np.float32(1 << q))
+
+
+class Step:
+    def __init__(self, G, args):
+        self.G = G
+        self.args = args
+        self.ops = {}
+        self.ms = {}
+
+    def add(self, cls, ops, ms):
+        self.ops[cls] = self.ops.get(cls, 0) + ops
+        self.ms[cls] = self.ms.get(cls, 0.0) + ms
+
+
+def setup_sqf(G, q, r):
+    n = key_count(0.8, q)
+    keys = G.mt19937(1, n)
+    m = 1 << 24
+    probe = np.concatenate([keys[: m // 2], G.mt19937(2, m // 2)])
+    half = 1 << (q - 1)
+    d = dict(
+        n=n,
+        half=half,
+        keys=G.DeviceArray.from_host(keys),
+        half_keys=G.DeviceArray.from_host(keys[:half]),
+        batches=[G.DeviceArray.from_host(keys[i:i + (1 << 20)]) for i in range(half, n, 1 << 20)],
+        probe=G.DeviceArray.from_host(probe),
+        out=G.DeviceArray.empty(m),
+        dels=G.DeviceArray.from_host(keys[: 1 << 22]),
+        m=m,
+        qf=G.SQF(q, r),
+    )
+    return d, keys
+
+
+def run_sqf_step(G, d, st):
+    qf = d["qf"]
+    st.add("build", d["n"], qf.bulkBuildSegmentedLayouts(d["keys"], NoDuplicates=True))
+    st.add("lookup_unsorted", d["m"], qf.launchUnsortedLookups(d["probe"], d["out"]))
+    st.add("lookup_sorted", d["m"], qf.launchSortedLookups(d["probe"], d["out"]))
+    st.add("delete", len(d["dels"]), qf.superclusterDeletes(d["dels"]))
+    st.add("build", d["n"], qf.bulkBuildSegmentedLayouts(d["keys"], NoDuplicates=False))
+    st.add("build", d["half"], qf.bulkBuildSegmentedLayouts(d["half_keys"], NoDuplicates=True))
+    for b in d["batches"]:
+        st.add("insert", len(b), qf.insert(b))
+
+
+def setup_rsqf(G, q, r):
+    n = key_count(0.9, q)
+    keys = G.mt19937(1, n)
+    m = 1 << 24
+    probe = np.concatenate([keys[: m // 2], G.mt19937(2, m // 2)])
+    start = key_count(0.7, q)
+    d = dict(
+        n=n,
+        start=start,
+        keys=G.DeviceArray.from_host(keys),
+        start_keys=G.DeviceArray.from_host(keys[:start]),
+        batches=[G.DeviceArray.from_host(keys[i:i + (1 << 18)]) for i in range(start, n, 1 << 18)],
+        probe=G.DeviceArray.from_host(probe),
+        out=G.DeviceArray.empty(m),
+        dels=G.DeviceArray.from_host(keys[: 1 << 22]),
+        m=m,
+        f=G.RSQF(q, r),
+    )
+    return d, keys
+
+
+def run_rsqf_step(G, d, st):
+    f = d["f"]
+    st.add("build", d["n"], f.bulk_build(d["keys"]))
+    st.add("lookup_sorted", d["m"], f.launchLookups(d["probe"], d["out"]))
+    st.add("lookup_unsorted", d["m"], f.launchUnsortedLookups(d["probe"], d["out"]))
+    st.add("delete", len(d["dels"]), f.deleteGPU(d["dels"]))
+    st.add("build", d["start"], f.bulk_build(d["start_keys"]))
+    for b in d["batches"]:
+        st.add("insert", len(b), f.insertGPU(b))
+
+
+def cpu_baseline(workload, q, r, keys, budget_s=20.0):
+    """Reference CPU path on this host, single thread, bounded sample."""
+    import oracle
+
+    n = len(keys)
+    m = 1 << 22
+    probe = np.concatenate([keys[: m // 2], oracle.mt19937(2, m // 2)])
+    try:
+        cpu = open("/proc/cpuinfo").read().split("model name")[1].split("\n")[0].strip(" :\t")
+    except Exception:
+        cpu = "unknown"
+    if workload == "sqf" and oracle.RefSQF.available():
+        # the reference CPU_version itself (FNV hash, packed words): its build is
+        # the insertItem loop (CPUInsertPerf.cpp:41-48), then mayContain
+        # (CPULookupPerf.cpp:51-58)
+        ref = oracle.RefSQF(q, r)
+        t0 = time.perf_counter()
+        ref.insert(keys)
+        t_ins = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        ref.lookup(probe)
+        t_lk = time.perf_counter() - t0
+        kind = "reference"
+        sample = (f"reference CPU_version SQF q={q} r={r}: insertItem of all {n} keys (its only build), "
+                  f"then mayContain of {m} probes (50% hit); FNV hash as compiled; 1 thread; {cpu}")
+    else:
+        # the project's restatement of the reference CPU algorithm (AP hash)
+        L = oracle.lib()
+        if workload == "sqf":
+            t = np.zeros(oracle.sqf_table_bytes(q, r), np.uint8)
+            t_ins = L.orc_time_sqf_insert(t, q, r, keys, n)
+            out = np.empty(m, np.uint32)
+            t_lk = L.orc_time_sqf_lookup(t, q, r, probe, m, out, 1)
+        else:
+            t = np.zeros(oracle.rsqf_table_bytes(q, r), np.uint8)
+            t_ins = L.orc_time_rsqf_insert(t, q, r, keys, n)
+            out = np.empty(m, np.uint32)
+            t_lk = L.orc_time_rsqf_lookup(t, q, r, probe, m, out, 1)
+        kind = "port"
+        sample = (f"project restatement of the reference CPU {workload.upper()} q={q} r={r}: sequential insert of "
+                  f"all {n} keys, then {m} lookups (50% hit); 1 thread; {cpu}")
+    ops = n + m
+    return {
+        "value": round(ops / (t_ins + t_lk) / 1e6, 3),
+        "unit": "Mops/s",
+        "cores": 1,
+        "kind": kind,
+        "sample": sample,
+        "insert_mops": round(n / t_ins / 1e6, 3),
+        "lookup_mops": round(m / t_lk / 1e6, 3),
+        "seconds": round(t_ins + t_lk, 2),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", choices=["sqf", "rsqf"], default="sqf")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (every box here has one GPU; "
+              "the multi-GPU sharding design is described in DESIGN.md, not run).", file=sys.stderr)
+        print(json.dumps({"error": "only --gpus 1 is supported", "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import gpuquotientfilters_amd as G
+
+    assert G.device_count() > 0, "no GPU visible"
+    if args.workload == "sqf":
+        q, r = 26, 5
+        d, keys = setup_sqf(G, q, r)
+        run = run_sqf_step
+        cfg = {"workload": "SQF q=26 r=5 fill 0.8 (53,687,092 MT19937 keys): build NoDup+Dup, 2^24 unsorted + "
+                           "2^24 sorted lookups (50% hit), 2^22 deletes, build to 0.5 + 2^20-key insert batches to 0.8",
+               "filter": "SQF", "q": q, "r": r, "fill": 0.8, "keys": d["n"]}
+    else:
+        q, r = 26, 8
+        d, keys = setup_rsqf(G, q, r)
+        run = run_rsqf_step
+        cfg = {"workload": "RSQF q=26 r=8 fill 0.9 (60,397,976 MT19937 keys, 34-bit fingerprints): build, 2^24 "
+                           "sorted + unsorted lookups, 2^22 deletes, build to 0.7 + 2^18-key insert batches to 0.9",
+               "filter": "RSQF", "q": q, "r": r, "fill": 0.9, "keys": d["n"]}
+
+    for _ in range(args.warmup):
+        run(G, d, Step(G, args))
+    G.synchronize()
+    G.profile_reset()
+    G.profile_enable(True)
+    st = Step(G, args)
+    G.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        run(G, d, st)
+    G.synchronize()
+    wall = time.perf_counter() - t0
+    G.profile_enable(False)
+    prof = G.profile_stats()
+
+    total_ops = sum(st.ops.values())
+    value = total_ops / wall / 1e6
+    breakdown = {c: {"Mops_s": round(st.ops[c] / st.ms[c] / 1e3, 1), "ms_per_step": round(st.ms[c] / args.steps, 3)}
+                 for c in st.ops}
+    # dominant kernel: most device time in the timed region
+    dom = max(prof.items(), key=lambda kv: kv[1][1])
+    name, (launches, kms, kbytes) = dom
+    avg_ms = kms / launches
+    achieved = (kbytes / launches) / (avg_ms / 1e3) / 1e9
+    kernels = {k: {"launches": v[0], "ms": round(v[1], 3),
+                   "GB_s": round(v[2] / (v[1] / 1e3) / 1e9, 1) if v[2] else None}
+               for k, v in sorted(prof.items(), key=lambda kv: -kv[1][1])}
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
+    if os.path.exists(tpath):
+        try:
+            traffic = json.load(open(tpath)).get(name)
+        except Exception:
+            traffic = None
+    roof = {
+        "bound": "hbm",
+        "kernel": name,
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": traffic,
+        "algorithmic_bytes_per_launch": round(kbytes / launches),
+        "avg_launch_ms": round(avg_ms, 4),
+    }
+    out = {
+        "metric": "Mops/s build/insert/lookup/delete @q=26 fill 0.8; p50/p99 served-batch latency",
+        "value": round(value, 1),
+        "unit": "Mops/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic (MT19937 uint32 keys, seed 1; misses from seed 2)",
+        "config": cfg,
+        "breakdown": breakdown,
+        "roofline": roof,
+        "kernels": kernels,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args.workload, q, r, keys)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -63,6 +63,12 @@ _SIGS = {
     "gqf_memcpy_dtod": (_st, [_vp, _vp, C.c_size_t]),
     "gqf_memset": (_st, [_vp, C.c_int, C.c_size_t]),
     "gqf_synchronize": (_st, []),
+    "gqf_profile_enable": (_st, [C.c_int]),
+    "gqf_profile_reset": (_st, []),
+    "gqf_profile_count": (C.c_int, []),
+    "gqf_profile_get": (_st, [C.c_int, C.c_char_p, C.c_int, C.POINTER(C.c_uint64), C.POINTER(C.c_double),
+                              C.POINTER(C.c_double)]),
+    "gqf_mt19937_fill": (None, [C.c_uint32, C.c_uint64, C.c_size_t, _vp]),
     "gqf_sqf_table_bytes": (C.c_size_t, [C.c_uint, C.c_uint]),
     "gqf_sqf_num_slots": (C.c_uint32, [C.c_uint]),
     "gqf_sqf_init": (_st, [C.POINTER(_SQF), C.c_uint, C.c_uint]),
@@ -121,6 +127,33 @@ def device_count():
 
 def synchronize():
     check(lib().gqf_synchronize())
+
+
+def profile_enable(on=True):
+    check(lib().gqf_profile_enable(int(bool(on))))
+
+
+def profile_reset():
+    check(lib().gqf_profile_reset())
+
+
+def profile_stats():
+    """{kernel name: (launches, total ms, total algorithmic bytes)} accumulated
+    while profiling was enabled."""
+    out = {}
+    buf = C.create_string_buffer(512)
+    for i in range(lib().gqf_profile_count()):
+        n, ms, by = C.c_uint64(0), C.c_double(0), C.c_double(0)
+        check(lib().gqf_profile_get(i, buf, 512, C.byref(n), C.byref(ms), C.byref(by)))
+        out[buf.value.decode()] = (n.value, ms.value, by.value)
+    return out
+
+
+def mt19937(seed, n, skip=0):
+    """The reference drivers' key stream (init_genrand + genrand_int32)."""
+    out = np.empty(int(n), dtype=np.uint32)
+    lib().gqf_mt19937_fill(seed, skip, int(n), out.ctypes.data)
+    return out
 
 
 class DeviceArray:

@@ -126,7 +126,7 @@ inline void launch_fill(hipStream_t s, T* p, uint64_t n, T v) {
   if (!n) return;
   uint64_t blocks = (n + 255) / 256;
   if (blocks > 2048) blocks = 2048;
-  hipLaunchKernelGGL(k_fill<T>, dim3((unsigned)blocks), dim3(256), 0, s, p, n, v);
+  GQF_LAUNCH(k_fill<T>, dim3((unsigned)blocks), dim3(256), 0, s, p, n, v);
 }
 
 // ====================================================================== radix sort
@@ -134,10 +134,6 @@ constexpr int SORT_THREADS = 256;
 constexpr int SORT_ITEMS = 16;
 constexpr int SORT_TILE = SORT_THREADS * SORT_ITEMS;  // 4096 keys per tile
 constexpr int SORT_MAX_PASSES = 8;
-constexpr unsigned long long ST_AGG = 1ull << 30;
-constexpr unsigned long long ST_INC = 2ull << 30;
-constexpr unsigned long long ST_FLAGS = 3ull << 30;
-constexpr unsigned long long ST_VAL = (1ull << 30) - 1ull;
 
 // Lanes of the wave holding the same 8-bit digit (8 ballots, all 64 lanes active).
 __device__ __forceinline__ uint64_t match_digit(uint32_t d) {
@@ -150,171 +146,232 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t d) {
   return m;
 }
 
+// Fingerprint when the result fits 32 bits (q + r <= 32): the AP value only.
+__host__ __device__ __forceinline__ uint32_t fingerprint32(uint32_t key, unsigned bits) {
+  const uint32_t lo = ap_rounds(key, 0xAAAAAAAAu);
+  return bits >= 32 ? lo : (lo & ((1u << bits) - 1u));
+}
+
 template <typename K, bool HASH>
 __device__ __forceinline__ K sort_load_key(const void* in, uint64_t idx, uint32_t fp_bits) {
-  if (HASH) return (K)fingerprint(reinterpret_cast<const uint32_t*>(in)[idx], fp_bits);
+  if (HASH) {
+    const uint32_t key = reinterpret_cast<const uint32_t*>(in)[idx];
+    // a 32-bit key type means q + r <= 32: skip the wide path entirely
+    if (sizeof(K) == 4) return (K)fingerprint32(key, fp_bits);
+    return (K)fingerprint(key, fp_bits);
+  }
   return reinterpret_cast<const K*>(in)[idx];
 }
 
-// Per-pass digit histograms of all keys (one read of the input).
+// Chunked reduce-then-scan LSD radix sort.  The input is split into G
+// contiguous chunks (one workgroup each, ~2K chunks so every CU stays busy);
+// per 8-bit digit pass:
+//   upsweep   -- each chunk's digit histogram (one read of the keys),
+//   row scan  -- exclusive scan of every digit's chunk counts, then of the
+//                digit totals (tiny),
+//   downsweep -- each chunk walks its keys in 4096-key sub-tiles in order,
+//                ranks them stably with wave64 ballots, stages them in LDS by
+//                digit and writes runs of equal digits (second read + write).
+// No workgroup waits on another and no counter atomics are used: on MI355X a
+// same-address device atomic serializes at ~11 ns and a cross-workgroup
+// publish/observe round trip costs ~0.4 us, which made a single-pass
+// (look-back) sort latency-bound.
+struct SortPlan {
+  uint32_t G;      // chunks
+  uint32_t chunk;  // keys per chunk (multiple of SORT_TILE)
+};
+inline SortPlan sort_plan(uint32_t n) {
+  SortPlan p;
+  const uint32_t tiles = (n + SORT_TILE - 1) / SORT_TILE;
+  if (tiles == 0) {  // empty input: one (idle) chunk
+    p.chunk = SORT_TILE;
+    p.G = 1;
+    return p;
+  }
+  const uint32_t G0 = tiles < 2048 ? tiles : 2048u;
+  const uint32_t per = (tiles + G0 - 1) / G0;
+  p.chunk = per * SORT_TILE;
+  p.G = (uint32_t)(((uint64_t)n + p.chunk - 1) / p.chunk);
+  return p;
+}
+
 template <typename K, bool HASH>
-__global__ __launch_bounds__(256) void k_sort_hist(const void* __restrict__ in, uint32_t n, uint32_t fp_bits,
-                                                   uint32_t lo_bit, uint32_t npass, uint32_t* __restrict__ hist) {
-  __shared__ uint32_t h[SORT_MAX_PASSES * 256];
-  for (int i = threadIdx.x; i < SORT_MAX_PASSES * 256; i += 256) h[i] = 0;
+__global__ __launch_bounds__(256) void k_rs_upsweep(const void* __restrict__ in, uint32_t n, uint32_t fp_bits,
+                                                    uint32_t shift, uint32_t chunk, uint32_t G,
+                                                    uint32_t* __restrict__ hist) {
+  __shared__ uint32_t h[4 * 256];
+  const int tid = threadIdx.x, wid = tid >> 6;
+  for (int i = tid; i < 4 * 256; i += 256) h[i] = 0;
   __syncthreads();
-  for (uint64_t idx = blockIdx.x * 256ull + threadIdx.x; idx < n; idx += (uint64_t)gridDim.x * 256ull) {
-    K key = sort_load_key<K, HASH>(in, idx, fp_bits);
-    for (uint32_t p = 0; p < npass; ++p) atomicAdd(&h[p * 256 + ((uint32_t)(key >> (lo_bit + 8 * p)) & 255u)], 1u);
+  const uint64_t lo = (uint64_t)blockIdx.x * chunk;
+  const uint64_t hi = lo + chunk < n ? lo + chunk : (uint64_t)n;
+  for (uint64_t t0 = lo; t0 < hi; t0 += SORT_TILE) {
+    // all SORT_ITEMS loads are issued before any is used
+    K k[SORT_ITEMS];
+#pragma unroll
+    for (int i = 0; i < SORT_ITEMS; ++i) {
+      const uint64_t idx = t0 + (uint64_t)i * 256 + tid;
+      k[i] = idx < hi ? sort_load_key<K, HASH>(in, idx, fp_bits) : (K)0;
+    }
+#pragma unroll
+    for (int i = 0; i < SORT_ITEMS; ++i) {
+      const uint64_t idx = t0 + (uint64_t)i * 256 + tid;
+      if (idx < hi) atomicAdd(&h[wid * 256 + ((uint32_t)(k[i] >> shift) & 255u)], 1u);
+    }
   }
   __syncthreads();
-  for (uint32_t i = threadIdx.x; i < npass * 256; i += 256) {
-    uint32_t c = h[i];
-    if (c) atomicAdd(&hist[i], c);
-  }
+  hist[(uint64_t)tid * G + blockIdx.x] = h[tid] + h[256 + tid] + h[512 + tid] + h[768 + tid];
 }
 
-// Exclusive scan of each pass's 256 digit counts, in place (grid = npass).
-static __global__ __launch_bounds__(256) void k_sort_hist_scan(uint32_t* hist) {
+// Row d (one workgroup per digit): exclusive scan of the G chunk counts in
+// place; the row total goes to totals[d].
+static __global__ __launch_bounds__(1024) void k_rs_rowscan(uint32_t* __restrict__ hist, uint32_t G,
+                                                            uint32_t* __restrict__ totals) {
+  __shared__ uint32_t s_w[16];
+  __shared__ uint32_t s_carry;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  uint32_t* row = hist + (uint64_t)blockIdx.x * G;
+  if (tid == 0) s_carry = 0;
+  __syncthreads();
+  for (uint32_t b = 0; b < G; b += 1024) {
+    const uint32_t i = b + tid;
+    const uint32_t x = i < G ? row[i] : 0u;
+    uint32_t inc = x;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t y = __shfl_up(inc, d, 64);
+      if (lane >= d) inc += y;
+    }
+    if (lane == 63) s_w[wid] = inc;
+    __syncthreads();
+    uint32_t pre = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < 16; ++w) {
+      pre += w < wid ? s_w[w] : 0u;
+      tot += s_w[w];
+    }
+    const uint32_t carry = s_carry;
+    if (i < G) row[i] = carry + pre + inc - x;
+    __syncthreads();
+    if (tid == 0) s_carry = carry + tot;
+    __syncthreads();
+  }
+  if (tid == 0) totals[blockIdx.x] = s_carry;
+}
+
+// totals[256] -> exclusive digit bases, in place.
+static __global__ __launch_bounds__(256) void k_rs_digitscan(uint32_t* __restrict__ totals) {
   __shared__ uint32_t tmp[4];
-  uint32_t* hp = hist + blockIdx.x * 256;
-  uint32_t x = hp[threadIdx.x];
-  uint32_t e = block256_excl_scan(x, tmp, nullptr);
-  hp[threadIdx.x] = e;
+  const uint32_t x = totals[threadIdx.x];
+  totals[threadIdx.x] = block256_excl_scan(x, tmp, nullptr);
 }
 
-// One digit pass.  Tiles are claimed in launch order through an atomic
-// counter so every tile a look-back waits on is already running.
 template <typename K, bool VALS, bool HASH>
-__global__ __launch_bounds__(256) void k_onesweep(const void* __restrict__ in_keys, const uint32_t* __restrict__ in_vals,
-                                                  K* __restrict__ out_keys, uint32_t* __restrict__ out_vals,
-                                                  uint32_t n, uint32_t shift, uint32_t fp_bits,
-                                                  const uint32_t* __restrict__ digit_base,
-                                                  unsigned long long* __restrict__ status,
-                                                  uint32_t* __restrict__ tile_counter, uint32_t epoch) {
-  __shared__ uint32_t s_tile;
+__global__ __launch_bounds__(256) void k_rs_downsweep(const void* __restrict__ in_keys,
+                                                      const uint32_t* __restrict__ in_vals, K* __restrict__ out_keys,
+                                                      uint32_t* __restrict__ out_vals, uint32_t n, uint32_t fp_bits,
+                                                      uint32_t shift, uint32_t chunk, uint32_t G,
+                                                      const uint32_t* __restrict__ hist,
+                                                      const uint32_t* __restrict__ digit_base) {
+  __shared__ uint32_t s_run[256];
   __shared__ uint32_t s_wcnt[4 * 256];
   __shared__ uint32_t s_lstart[256];
-  __shared__ uint32_t s_gbase[256];
   __shared__ uint32_t s_tmp[4];
   __shared__ K s_keys[SORT_TILE];
   __shared__ uint32_t s_vals[VALS ? SORT_TILE : 1];
-
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  if (tid == 0) s_tile = atomicAdd(tile_counter, 1u);
-  for (int i = tid; i < 4 * 256; i += 256) s_wcnt[i] = 0;
-  __syncthreads();
-  const uint32_t tile = s_tile;
-  const uint64_t base = (uint64_t)tile * SORT_TILE + (uint64_t)wid * (SORT_ITEMS * 64) + lane;
-
-  K k[SORT_ITEMS];
-  uint32_t v[VALS ? SORT_ITEMS : 1];
-#pragma unroll
-  for (int i = 0; i < SORT_ITEMS; ++i) {
-    const uint64_t idx = base + (uint64_t)i * 64;
-    const bool valid = idx < n;
-    // padding keys (all ones) take the last digit and rank after real keys
-    k[i] = valid ? sort_load_key<K, HASH>(in_keys, idx, fp_bits) : (K)~(K)0;
-    if (VALS) v[i] = in_vals ? (valid ? in_vals[idx] : 0u) : (uint32_t)idx;
-  }
-
-  // stable rank within the wave: items are visited in index order
-  uint32_t rk[SORT_ITEMS];
+  s_run[tid] = digit_base[tid] + hist[(uint64_t)tid * G + blockIdx.x];
+  const uint64_t c0 = (uint64_t)blockIdx.x * chunk;
+  const uint64_t c1 = c0 + chunk < n ? c0 + chunk : (uint64_t)n;
   const uint64_t lt = (1ull << lane) - 1ull;
+  for (uint64_t t0 = c0; t0 < c1; t0 += SORT_TILE) {
+    for (int i = tid; i < 4 * 256; i += 256) s_wcnt[i] = 0;
+    __syncthreads();
+    const uint64_t base = t0 + (uint64_t)wid * (SORT_ITEMS * 64) + lane;
+    K k[SORT_ITEMS];
+    uint32_t v[VALS ? SORT_ITEMS : 1];
 #pragma unroll
-  for (int i = 0; i < SORT_ITEMS; ++i) {
-    const uint32_t d = (uint32_t)(k[i] >> shift) & 255u;
-    const uint64_t m = match_digit(d);
-    const uint32_t pre = s_wcnt[wid * 256 + d];
-    const uint32_t cnt = (uint32_t)__popcll(m);
-    const uint32_t below = (uint32_t)__popcll(m & lt);
-    if (below == cnt - 1) s_wcnt[wid * 256 + d] = pre + cnt;
-    rk[i] = pre + below;
-  }
-  __syncthreads();
-
-  // thread d owns digit d: wave prefix, tile total, look-back
-  {
-    const uint32_t d = tid;
-    const uint32_t c0 = s_wcnt[d], c1 = s_wcnt[256 + d], c2 = s_wcnt[512 + d], c3 = s_wcnt[768 + d];
-    const uint32_t total = c0 + c1 + c2 + c3;
-    const unsigned long long tag = (unsigned long long)epoch << 32;
-    unsigned long long* my = status + (uint64_t)tile * 256 + d;
-    st_store(my, tag | (tile == 0 ? ST_INC : ST_AGG) | total);
-    s_wcnt[d] = 0;
-    s_wcnt[256 + d] = c0;
-    s_wcnt[512 + d] = c0 + c1;
-    s_wcnt[768 + d] = c0 + c1 + c2;
-    s_lstart[d] = block256_excl_scan(total, s_tmp, nullptr);
-    uint32_t excl = 0;
-    if (tile > 0) {
-      uint32_t j = tile - 1;
-      for (;;) {
-        unsigned long long st;
-        do {
-          st = st_load(status + (uint64_t)j * 256 + d);
-        } while ((uint32_t)(st >> 32) != epoch || (st & ST_FLAGS) == 0);
-        excl += (uint32_t)(st & ST_VAL);
-        if ((st & ST_FLAGS) == ST_INC) break;
-        --j;
+    for (int i = 0; i < SORT_ITEMS; ++i) {
+      const uint64_t idx = base + (uint64_t)i * 64;
+      const bool valid = idx < c1;
+      // padding keys (all ones) take the last digit and rank after real keys
+      k[i] = valid ? sort_load_key<K, HASH>(in_keys, idx, fp_bits) : (K)~(K)0;
+      if (VALS) v[i] = in_vals ? (valid ? in_vals[idx] : 0u) : (uint32_t)idx;
+    }
+    uint32_t rk[SORT_ITEMS];
+#pragma unroll
+    for (int i = 0; i < SORT_ITEMS; ++i) {
+      const uint32_t d = (uint32_t)(k[i] >> shift) & 255u;
+      const uint64_t m = match_digit(d);
+      const uint32_t pre = s_wcnt[wid * 256 + d];
+      const uint32_t cnt = (uint32_t)__popcll(m);
+      const uint32_t below = (uint32_t)__popcll(m & lt);
+      if (below == cnt - 1) s_wcnt[wid * 256 + d] = pre + cnt;
+      rk[i] = pre + below;
+    }
+    __syncthreads();
+    uint32_t total;
+    {
+      const uint32_t d = tid;
+      const uint32_t w0 = s_wcnt[d], w1 = s_wcnt[256 + d], w2 = s_wcnt[512 + d], w3 = s_wcnt[768 + d];
+      total = w0 + w1 + w2 + w3;
+      s_wcnt[d] = 0;
+      s_wcnt[256 + d] = w0;
+      s_wcnt[512 + d] = w0 + w1;
+      s_wcnt[768 + d] = w0 + w1 + w2;
+      s_lstart[d] = block256_excl_scan(total, s_tmp, nullptr);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < SORT_ITEMS; ++i) {
+      const uint32_t d = (uint32_t)(k[i] >> shift) & 255u;
+      const uint32_t p = s_lstart[d] + s_wcnt[wid * 256 + d] + rk[i];
+      s_keys[p] = k[i];
+      if (VALS) s_vals[p] = v[i];
+    }
+    __syncthreads();
+    const uint32_t tile_n = (uint32_t)(c1 - t0 < (uint64_t)SORT_TILE ? c1 - t0 : (uint64_t)SORT_TILE);
+#pragma unroll
+    for (int i = 0; i < SORT_ITEMS; ++i) {
+      const uint32_t p = tid + i * SORT_THREADS;
+      if (p < tile_n) {
+        const K key = s_keys[p];
+        const uint32_t d = (uint32_t)(key >> shift) & 255u;
+        const uint32_t gpos = s_run[d] + (p - s_lstart[d]);
+        out_keys[gpos] = key;
+        if (VALS) out_vals[gpos] = s_vals[p];
       }
-      st_store(my, tag | ST_INC | (unsigned long long)(excl + total));
     }
-    s_gbase[d] = digit_base[d] + excl;
-  }
-  __syncthreads();
-
-#pragma unroll
-  for (int i = 0; i < SORT_ITEMS; ++i) {
-    const uint32_t d = (uint32_t)(k[i] >> shift) & 255u;
-    const uint32_t p = s_lstart[d] + s_wcnt[wid * 256 + d] + rk[i];
-    s_keys[p] = k[i];
-    if (VALS) s_vals[p] = v[i];
-  }
-  __syncthreads();
-
-  const uint64_t left = (uint64_t)n - (uint64_t)tile * SORT_TILE;
-  const uint32_t tile_n = left < (uint64_t)SORT_TILE ? (uint32_t)left : (uint32_t)SORT_TILE;
-#pragma unroll
-  for (int i = 0; i < SORT_ITEMS; ++i) {
-    const uint32_t p = tid + i * SORT_THREADS;
-    if (p < tile_n) {
-      const K key = s_keys[p];
-      const uint32_t d = (uint32_t)(key >> shift) & 255u;
-      const uint32_t g = s_gbase[d] + (p - s_lstart[d]);
-      out_keys[g] = key;
-      if (VALS) out_vals[g] = s_vals[p];
-    }
+    __syncthreads();
+    s_run[tid] += total;
+    __syncthreads();
   }
 }
-
-struct SortBuffers {
-  void* keys[2];
-  uint32_t* vals[2];
-  uint32_t* hist;
-};
 
 inline size_t sort_workspace_bytes(uint64_t n, size_t key_bytes, bool vals) {
+  const SortPlan p = sort_plan((uint32_t)n);
   return 2 * aligned_bytes(n * key_bytes) + (vals ? 2 * aligned_bytes(n * 4) : 0) +
-         aligned_bytes(SORT_MAX_PASSES * 256 * 4) + 1024;
+         aligned_bytes((size_t)p.G * 256 * 4) + aligned_bytes(256 * 4) + 1024;
 }
-inline size_t sort_status_words(uint64_t n) { return ((n + SORT_TILE - 1) / SORT_TILE) * 256 + 256; }
 
 // LSD radix sort of n keys over bits [lo_bit, hi_bit).  When hash is set, the
-// input is raw uint32 keys, fingerprinted (fp_bits) as they are first read.
-// vals: nullptr + with_vals -> the original index is carried.  Returns the
-// buffers holding the result through *keys_out / *vals_out.
+// input is raw uint32 keys, fingerprinted (fp_bits) as they are read in the
+// first pass.  vals: nullptr + with_vals -> the original index is carried.
+// Returns the buffers holding the result through *keys_out / *vals_out.
 template <typename K>
 gqf_status radix_sort(hipStream_t s, DeviceState& ds, Arena& ar, const void* in, bool hash, uint32_t fp_bits,
                       const uint32_t* in_vals, bool with_vals, uint32_t n, uint32_t lo_bit, uint32_t hi_bit,
                       K** keys_out, uint32_t** vals_out) {
+  (void)ds;
   K* kb[2] = {ar.take<K>(n ? n : 1), ar.take<K>(n ? n : 1)};
   uint32_t* vb[2] = {nullptr, nullptr};
   if (with_vals) {
     vb[0] = ar.take<uint32_t>(n ? n : 1);
     vb[1] = ar.take<uint32_t>(n ? n : 1);
   }
-  uint32_t* hist = ar.take<uint32_t>(SORT_MAX_PASSES * 256);
+  const SortPlan plan = sort_plan(n);
+  uint32_t* hist = ar.take<uint32_t>((size_t)plan.G * 256);
+  uint32_t* dbase = ar.take<uint32_t>(256);
   if (ar.used > ar.cap) {
     set_error("internal: sort workspace overflow");
     return GQF_ERR_INVALID;
@@ -327,44 +384,39 @@ gqf_status radix_sort(hipStream_t s, DeviceState& ds, Arena& ar, const void* in,
     if (vals_out) *vals_out = vb[0];
     return GQF_OK;
   }
-  const uint32_t tiles = (n + SORT_TILE - 1) / SORT_TILE;
-  GQF_TRY(status_reserve(ds, (size_t)tiles * 256 + 256));
-  launch_fill<uint32_t>(s, hist, SORT_MAX_PASSES * 256, 0u);
-  hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, s, ds.counters, (int)CTR_SORT_TILE,
-                     (int)CTR_SORT_TILE + SORT_MAX_PASSES);
-  {
-    uint32_t blocks = (n + 255) / 256;
-    if (blocks > 1024) blocks = 1024;
-    if (hash)
-      hipLaunchKernelGGL((k_sort_hist<K, true>), dim3(blocks), dim3(256), 0, s, in, n, fp_bits, lo_bit, npass, hist);
-    else
-      hipLaunchKernelGGL((k_sort_hist<K, false>), dim3(blocks), dim3(256), 0, s, in, n, fp_bits, lo_bit, npass, hist);
-  }
-  hipLaunchKernelGGL(k_sort_hist_scan, dim3(npass), dim3(256), 0, s, hist);
   const void* src = in;
   const uint32_t* vsrc = in_vals;
   int cur = 0;
   for (uint32_t p = 0; p < npass; ++p) {
     const uint32_t shift = lo_bit + 8 * p;
-    const uint32_t epoch = ++ds.epoch;
+    const bool first_hash = hash && p == 0;
+    const double kin = first_hash ? 4.0 : (double)sizeof(K);
+    if (first_hash)
+      GQF_LAUNCH_B((double)n * kin, (k_rs_upsweep<K, true>), dim3(plan.G), dim3(256), 0, s, src, n, fp_bits, shift,
+                   plan.chunk, plan.G, hist);
+    else
+      GQF_LAUNCH_B((double)n * kin, (k_rs_upsweep<K, false>), dim3(plan.G), dim3(256), 0, s, src, n, fp_bits, shift,
+                   plan.chunk, plan.G, hist);
+    GQF_LAUNCH(k_rs_rowscan, dim3(256), dim3(1024), 0, s, hist, plan.G, dbase);
+    GQF_LAUNCH(k_rs_digitscan, dim3(1), dim3(256), 0, s, dbase);
     K* dst = kb[cur];
     uint32_t* vdst = vb[cur];
-    const bool first_hash = hash && p == 0;
+    // read keys (+ values) once, write them once
+    const double pass_bytes = (double)n * (kin + sizeof(K) + (with_vals ? ((vsrc ? 4.0 : 0.0) + 4.0) : 0.0));
     if (with_vals) {
       if (first_hash)
-        hipLaunchKernelGGL((k_onesweep<K, true, true>), dim3(tiles), dim3(SORT_THREADS), 0, s, src, vsrc, dst, vdst, n,
-                           shift, fp_bits, hist + p * 256, ds.status, ds.counters + CTR_SORT_TILE + p, epoch);
+        GQF_LAUNCH_B(pass_bytes, (k_rs_downsweep<K, true, true>), dim3(plan.G), dim3(SORT_THREADS), 0, s, src, vsrc,
+                     dst, vdst, n, fp_bits, shift, plan.chunk, plan.G, hist, dbase);
       else
-        hipLaunchKernelGGL((k_onesweep<K, true, false>), dim3(tiles), dim3(SORT_THREADS), 0, s, src, vsrc, dst, vdst,
-                           n, shift, fp_bits, hist + p * 256, ds.status, ds.counters + CTR_SORT_TILE + p, epoch);
+        GQF_LAUNCH_B(pass_bytes, (k_rs_downsweep<K, true, false>), dim3(plan.G), dim3(SORT_THREADS), 0, s, src, vsrc,
+                     dst, vdst, n, fp_bits, shift, plan.chunk, plan.G, hist, dbase);
     } else {
       if (first_hash)
-        hipLaunchKernelGGL((k_onesweep<K, false, true>), dim3(tiles), dim3(SORT_THREADS), 0, s, src, nullptr, dst,
-                           nullptr, n, shift, fp_bits, hist + p * 256, ds.status, ds.counters + CTR_SORT_TILE + p, epoch);
+        GQF_LAUNCH_B(pass_bytes, (k_rs_downsweep<K, false, true>), dim3(plan.G), dim3(SORT_THREADS), 0, s, src,
+                     nullptr, dst, nullptr, n, fp_bits, shift, plan.chunk, plan.G, hist, dbase);
       else
-        hipLaunchKernelGGL((k_onesweep<K, false, false>), dim3(tiles), dim3(SORT_THREADS), 0, s, src, nullptr, dst,
-                           nullptr, n, shift, fp_bits, hist + p * 256, ds.status, ds.counters + CTR_SORT_TILE + p,
-                           epoch);
+        GQF_LAUNCH_B(pass_bytes, (k_rs_downsweep<K, false, false>), dim3(plan.G), dim3(SORT_THREADS), 0, s, src,
+                     nullptr, dst, nullptr, n, fp_bits, shift, plan.chunk, plan.G, hist, dbase);
     }
     src = dst;
     vsrc = vdst;
@@ -409,6 +461,46 @@ __device__ __forceinline__ CL cl_unpack(unsigned long long w) {
   const uint32_t lp = (uint32_t)w;
   o.l = lp == 0 ? LOC_NONE : (int64_t)lp - 1;
   return o;
+}
+
+// Exclusive (c, l) prefix of `tile` from per-tile status words (flag 1 =
+// aggregate, 2 = inclusive prefix, in the top two bits), computed by the 64
+// lanes of the calling wave: each step requests 64 predecessors at once
+// (nearest in lane 0), waits until all have published, and combines back to
+// the nearest inclusive prefix (farther tiles first: the operator is not
+// commutative).  Tile 0 always publishes an inclusive prefix.
+__device__ __forceinline__ CL lookback_cl(const unsigned long long* status, uint32_t tile, int lane) {
+  CL excl = {0u, LOC_NONE};
+  int64_t base = (int64_t)tile - 1;
+  while (base >= 0) {
+    const int64_t j = base - lane;
+    unsigned long long st = 0;
+    if (j >= 0) {
+      st = st_load(status + j);
+      while ((st >> 62) == 0) {
+        __builtin_amdgcn_s_sleep(2);  // back off: spinning loads compete with real traffic
+        st = st_load(status + j);
+      }
+    }
+    const uint64_t incm = __ballot(j >= 0 && (st >> 62) == 2);
+    const int stop = incm ? (__ffsll((long long)incm) - 1) : 63;
+    CL v = {0u, LOC_NONE};
+    if (j >= 0 && lane <= stop) v = cl_unpack(st);
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      CL o;
+      o.c = __shfl_down(v.c, d, 64);
+      o.l = __shfl_down(v.l, d, 64);
+      if (lane + d < 64) v = cl_combine(o, v);
+    }
+    CL w;
+    w.c = __shfl(v.c, 0, 64);
+    w.l = __shfl(v.l, 0, 64);
+    excl = cl_combine(w, excl);
+    if (incm) break;
+    base -= 64;
+  }
+  return excl;
 }
 
 // in: sorted entries (fingerprint << tagbits | tag).  out: the kept
@@ -498,32 +590,23 @@ __global__ __launch_bounds__(256) void k_layout_scan(const K* __restrict__ in, c
   }
   ex = cl_combine(wpre, ex);
 
-  if (tid == 0) {
-    CL excl = {0u, LOC_NONE};
+  if (wid == 0) {
     unsigned long long* my = status + tile;
-    if (tile == 0) {
-      st_store(my, cl_pack(2, tagg));
-    } else {
-      st_store(my, cl_pack(1, tagg));
-      uint32_t j = tile - 1;
-      for (;;) {
-        unsigned long long st;
-        do {
-          st = st_load(status + j);
-        } while ((st >> 62) == 0);
-        excl = cl_combine(cl_unpack(st), excl);
-        if ((st >> 62) == 2) break;
-        --j;
-      }
-      st_store(my, cl_pack(2, cl_combine(excl, tagg)));
+    if (lane == 0) st_store(my, cl_pack(tile == 0 ? 2 : 1, tagg));
+    CL excl = {0u, LOC_NONE};
+    if (tile > 0) {
+      excl = lookback_cl(status, tile, lane);
+      if (lane == 0) st_store(my, cl_pack(2, cl_combine(excl, tagg)));
     }
-    s_pc = excl.c;
-    s_pl = excl.l;
-    s_tc = tagg.c;
-    if (tile == gridDim.x - 1) {
-      const CL tot = cl_combine(excl, tagg);
-      ctr[CTR_COUNT] = tot.c;
-      if (tot.l != LOC_NONE && (uint64_t)tot.l >= capacity) atomicOr(&ctr[CTR_ERR], DEV_ERR_CAPACITY);
+    if (lane == 0) {
+      s_pc = excl.c;
+      s_pl = excl.l;
+      s_tc = tagg.c;
+      if (tile == gridDim.x - 1) {
+        const CL tot = cl_combine(excl, tagg);
+        ctr[CTR_COUNT] = tot.c;
+        if (tot.l != LOC_NONE && (uint64_t)tot.l >= capacity) atomicOr(&ctr[CTR_ERR], DEV_ERR_CAPACITY);
+      }
     }
   }
   __syncthreads();
@@ -572,7 +655,7 @@ gqf_status layout_scan(hipStream_t s, DeviceState& ds, Arena& ar, const K* in, c
   *loc_out = loc;
   if (n == 0) return GQF_OK;  // CTR_COUNT stays 0 (reset by the caller)
   launch_fill<unsigned long long>(s, st, tiles + 1, 0ull);
-  hipLaunchKernelGGL((k_layout_scan<K, MODE>), dim3(tiles), dim3(LS_THREADS), 0, s, in, flags, n, q, r, tagbits,
+  GQF_LAUNCH_B((double)n * (2 * sizeof(K) + 4 + (flags ? 1 : 0)), (k_layout_scan<K, MODE>), dim3(tiles), dim3(LS_THREADS), 0, s, in, flags, n, q, r, tagbits,
                      capacity, (int)rsqf_offsets, ufp, loc, st, ds.counters);
   return GQF_OK;
 }
@@ -637,7 +720,7 @@ template <typename K>
 inline void launch_merge_tagged(hipStream_t s, const K* A, uint32_t na, const K* B, uint32_t nb, uint64_t* out) {
   const uint64_t total = (uint64_t)na + nb;
   if (!total) return;
-  hipLaunchKernelGGL(k_merge_tagged<K>, dim3((unsigned)((total + MG_TILE - 1) / MG_TILE)), dim3(MG_THREADS), 0, s, A,
+  GQF_LAUNCH_B((double)total * (sizeof(K) + 8), k_merge_tagged<K>, dim3((unsigned)((total + MG_TILE - 1) / MG_TILE)), dim3(MG_THREADS), 0, s, A,
                      na, B, nb, out);
 }
 

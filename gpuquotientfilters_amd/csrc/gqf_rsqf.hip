@@ -306,13 +306,14 @@ static RsqfDev rdev(const gqf_rsqf& f) {
 }
 
 static void reset_op_counters_r(hipStream_t s, DeviceState& ds) {
-  hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, s, ds.counters, (int)CTR_SCAN_TILE, (int)CTR_MISC);
+  GQF_LAUNCH(k_reset_counters, dim3(1), dim3(64), 0, s, ds.counters, (int)CTR_SCAN_TILE, (int)CTR_MISC);
 }
 
 template <typename K>
-static void launch_rwrite(hipStream_t s, DeviceState& ds, const gqf_rsqf& f, const K* ufp, const uint32_t* loc) {
+static void launch_rwrite(hipStream_t s, DeviceState& ds, const gqf_rsqf& f, const K* ufp, const uint32_t* loc,
+                          uint64_t items) {
   const uint32_t groups = (f.numBlocks + RW_BLOCKS - 1) / RW_BLOCKS;
-  hipLaunchKernelGGL(k_rsqf_write<K>, dim3(groups), dim3(256), 0, s, reinterpret_cast<uint8_t*>(f.blocks), f.numBlocks,
+  GQF_LAUNCH_B((double)items * (sizeof(K) + 4) + (double)rsqf_dev_bytes(f.qbits, f.rbits), k_rsqf_write<K>, dim3(groups), dim3(256), 0, s, reinterpret_cast<uint8_t*>(f.blocks), f.numBlocks,
                      f.rbits, ufp, loc, ds.counters);
 }
 
@@ -333,15 +334,15 @@ static gqf_status rsqf_decode(hipStream_t s, DeviceState& ds, Arena& ar, const g
     return GQF_ERR_INVALID;
   }
   const dim3 grid((f.numBlocks + 255) / 256);
-  hipLaunchKernelGGL((k_rsqf_decode<false, K>), grid, dim3(256), 0, s, d, counts, (K*)nullptr);
-  hipLaunchKernelGGL(k_scan_rows, dim3(1), dim3(1024), 0, s, counts, f.numBlocks, 1u, total);
+  GQF_LAUNCH_B((double)rsqf_meta_bytes(f.numBlocks), (k_rsqf_decode<false, K>), grid, dim3(256), 0, s, d, counts, (K*)nullptr);
+  GQF_LAUNCH(k_scan_rows, dim3(1), dim3(1024), 0, s, counts, f.numBlocks, 1u, total);
   GQF_TRY(read_ctr(s, ds, total, m_out));
   K* fps = ar.take<K>(*m_out + 1);
   if (ar.used > ar.cap) {
     set_error("internal: decode workspace overflow");
     return GQF_ERR_INVALID;
   }
-  hipLaunchKernelGGL((k_rsqf_decode<true, K>), grid, dim3(256), 0, s, d, counts, fps);
+  GQF_LAUNCH_B((double)rsqf_dev_bytes(f.qbits, f.rbits) + (double)*m_out * sizeof(K), (k_rsqf_decode<true, K>), grid, dim3(256), 0, s, d, counts, fps);
   *out = fps;
   return GQF_OK;
 }
@@ -367,7 +368,7 @@ static gqf_status rsqf_build_t(const gqf_rsqf& f, uint32_t n, const uint32_t* ke
   K* ufp;
   uint32_t* loc;
   GQF_TRY((layout_scan<K, KEEP_ALL>(cs.stream, ds, ar, sorted, nullptr, n, g.q, g.r, 0, g.cap, true, &ufp, &loc)));
-  launch_rwrite<K>(cs.stream, ds, f, ufp, loc);
+  launch_rwrite<K>(cs.stream, ds, f, ufp, loc, n);
   return GQF_OK;
 }
 
@@ -398,7 +399,7 @@ static gqf_status rsqf_insert_t(const gqf_rsqf& f, uint32_t n, const uint32_t* k
   uint32_t* loc;
   GQF_TRY((layout_scan<uint64_t, KEEP_ALL>(cs.stream, ds, ar, merged, nullptr, m + n, g.q, g.r, 1, g.cap, true, &ufp,
                                            &loc)));
-  launch_rwrite<uint64_t>(cs.stream, ds, f, ufp, loc);
+  launch_rwrite<uint64_t>(cs.stream, ds, f, ufp, loc, (uint64_t)m + n);
   return GQF_OK;
 }
 
@@ -420,19 +421,19 @@ static gqf_status rsqf_delete_t(const gqf_rsqf& f, uint32_t n, const uint32_t* k
     set_error("internal: delete workspace overflow");
     return GQF_ERR_INVALID;
   }
-  if (m) hipLaunchKernelGGL(k_delete_mark<K>, dim3((m + 255) / 256), dim3(256), 0, cs.stream, A, m, D, n, keep);
+  if (m) GQF_LAUNCH_B((double)m * (sizeof(K) + 1), k_delete_mark<K>, dim3((m + 255) / 256), dim3(256), 0, cs.stream, A, m, D, n, keep);
   K* ufp;
   uint32_t* loc;
   GQF_TRY((layout_scan<K, KEEP_FLAGS>(cs.stream, ds, ar, A, keep, m, g.q, g.r, 0, g.cap, true, &ufp, &loc)));
-  launch_rwrite<K>(cs.stream, ds, f, ufp, loc);
+  launch_rwrite<K>(cs.stream, ds, f, ufp, loc, m);
   return GQF_OK;
 }
 
 static void launch_rlookup(hipStream_t s, const gqf_rsqf& f, uint32_t n, const uint32_t* keys, uint32_t* out) {
   const RsqfDev d = rdev(f);
   const dim3 grid((n + 255) / 256);
-  if (f.rbits == 8) hipLaunchKernelGGL(k_rsqf_lookup<8>, grid, dim3(256), 0, s, d, f.qbits, keys, n, out);
-  else hipLaunchKernelGGL(k_rsqf_lookup<0>, grid, dim3(256), 0, s, d, f.qbits, keys, n, out);
+  if (f.rbits == 8) GQF_LAUNCH_B((double)n * (8 + 128), k_rsqf_lookup<8>, grid, dim3(256), 0, s, d, f.qbits, keys, n, out);
+  else GQF_LAUNCH_B((double)n * (8 + 128), k_rsqf_lookup<0>, grid, dim3(256), 0, s, d, f.qbits, keys, n, out);
 }
 
 template <typename K>
@@ -451,9 +452,9 @@ static gqf_status rsqf_lookup_sorted_t(const gqf_rsqf& f, uint32_t n, const uint
   const RsqfDev d = rdev(f);
   const dim3 grid((n + 255) / 256);
   if (f.rbits == 8)
-    hipLaunchKernelGGL((k_rsqf_lookup_sorted<8, K>), grid, dim3(256), 0, cs.stream, d, f.qbits, fps, idx, n, out);
+    GQF_LAUNCH_B((double)n * (sizeof(K) + 8 + 128), (k_rsqf_lookup_sorted<8, K>), grid, dim3(256), 0, cs.stream, d, f.qbits, fps, idx, n, out);
   else
-    hipLaunchKernelGGL((k_rsqf_lookup_sorted<0, K>), grid, dim3(256), 0, cs.stream, d, f.qbits, fps, idx, n, out);
+    GQF_LAUNCH_B((double)n * (sizeof(K) + 8 + 128), (k_rsqf_lookup_sorted<0, K>), grid, dim3(256), 0, cs.stream, d, f.qbits, fps, idx, n, out);
   return GQF_OK;
 }
 

@@ -6,6 +6,9 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <random>
+#include <string>
+#include <vector>
 
 namespace gqf {
 
@@ -77,6 +80,74 @@ gqf_status status_reserve(DeviceState& ds, size_t words) {
   return GQF_OK;
 }
 
+// ---------------------------------------------------------------- kernel timing
+struct ProfRec {
+  const char* name;
+  hipEvent_t a, b;
+  double bytes;
+};
+struct ProfStat {
+  std::string name;
+  uint64_t launches;
+  double ms;
+  double bytes;
+};
+static bool g_prof = false;
+static std::vector<hipEvent_t> g_pool;
+static size_t g_pool_used = 0;
+static std::vector<ProfRec> g_pending;
+static std::vector<ProfStat> g_stats;
+
+bool prof_enabled() { return g_prof; }
+
+static hipEvent_t pool_event() {
+  if (g_pool_used == g_pool.size()) {
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    g_pool.push_back(e);
+  }
+  return g_pool[g_pool_used++];
+}
+
+ProfScope::ProfScope(const char* n, hipStream_t s, double algorithmic_bytes)
+    : name(n), stream(s), on(g_prof), bytes(algorithmic_bytes) {
+  if (on) {
+    a = pool_event();
+    if (a) (void)hipEventRecord(a, stream);
+    else on = false;
+  }
+}
+
+ProfScope::~ProfScope() {
+  if (!on) return;
+  hipEvent_t b = pool_event();
+  if (!b) return;
+  (void)hipEventRecord(b, stream);
+  g_pending.push_back({name, a, b, bytes});
+}
+
+void prof_resolve() {
+  for (const ProfRec& r : g_pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, r.a, r.b) != hipSuccess) continue;
+    std::string nm(r.name);
+    while (!nm.empty() && (nm.front() == '(' || nm.front() == ' ')) nm.erase(nm.begin());
+    while (!nm.empty() && (nm.back() == ')' || nm.back() == ' ')) nm.pop_back();
+    bool found = false;
+    for (ProfStat& st : g_stats)
+      if (st.name == nm) {
+        st.launches++;
+        st.ms += ms;
+        st.bytes += r.bytes;
+        found = true;
+        break;
+      }
+    if (!found) g_stats.push_back({nm, 1, (double)ms, r.bytes});
+  }
+  g_pending.clear();
+  g_pool_used = 0;
+}
+
 gqf_status CallScope::begin(gqf_stream s) {
   GQF_TRY(device_state(&ds));
   stream = reinterpret_cast<hipStream_t>(s);
@@ -97,6 +168,7 @@ gqf_status CallScope::end(float* ms, bool check_device_err) {
     GQF_HIP(hipEventElapsedTime(&t, ds->ev0, ds->ev1));
     *ms = t;
   }
+  prof_resolve();
   if (check_device_err && (ds->h_pinned[CTR_ERR] & DEV_ERR_CAPACITY)) {
     set_error("capacity exceeded: the entries do not fit the table (table left unchanged)");
     return GQF_ERR_CAPACITY;
@@ -168,6 +240,39 @@ gqf_status gqf_memset(void* d_ptr, int value, size_t bytes) {
   if (bytes) GQF_HIP(hipMemset(d_ptr, value, bytes));
   GQF_HIP(hipDeviceSynchronize());
   return GQF_OK;
+}
+
+gqf_status gqf_profile_enable(int on) {
+  std::lock_guard<std::mutex> lk(api_mutex());
+  g_prof = on != 0;
+  return GQF_OK;
+}
+
+gqf_status gqf_profile_reset(void) {
+  std::lock_guard<std::mutex> lk(api_mutex());
+  g_stats.clear();
+  return GQF_OK;
+}
+
+int gqf_profile_count(void) { return (int)g_stats.size(); }
+
+gqf_status gqf_profile_get(int i, char* name, int name_cap, uint64_t* launches, double* total_ms,
+                           double* total_bytes) {
+  if (i < 0 || i >= (int)g_stats.size()) return GQF_ERR_INVALID;
+  const ProfStat& st = g_stats[(size_t)i];
+  if (name && name_cap > 0) {
+    snprintf(name, (size_t)name_cap, "%s", st.name.c_str());
+  }
+  if (launches) *launches = st.launches;
+  if (total_ms) *total_ms = st.ms;
+  if (total_bytes) *total_bytes = st.bytes;
+  return GQF_OK;
+}
+
+void gqf_mt19937_fill(uint32_t seed, uint64_t skip, size_t n, uint32_t* host_out) {
+  std::mt19937 g(seed);
+  g.discard(skip);
+  for (size_t i = 0; i < n; ++i) host_out[i] = (uint32_t)g();
 }
 
 gqf_status gqf_synchronize(void) {

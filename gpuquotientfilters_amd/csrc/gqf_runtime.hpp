@@ -77,6 +77,36 @@ struct Arena {
 };
 __host__ __device__ inline size_t aligned_bytes(size_t bytes) { return (bytes + 255) & ~size_t(255); }
 
+// ---------------------------------------------------------------- kernel timing
+// When enabled (gqf_profile_enable), every launch is bracketed by two events
+// on its stream; elapsed times are accumulated per kernel name when the
+// enclosing call completes.
+bool prof_enabled();
+struct ProfScope {
+  const char* name;
+  hipStream_t stream;
+  hipEvent_t a = nullptr;
+  bool on;
+  double bytes;
+  ProfScope(const char* n, hipStream_t s, double algorithmic_bytes = 0.0);
+  ~ProfScope();
+};
+void prof_resolve();
+
+#define GQF_LAUNCH(kernel, grid, block, shm, stream, ...)              \
+  do {                                                                 \
+    ::gqf::ProfScope _gqf_ps(#kernel, (stream));                       \
+    hipLaunchKernelGGL(kernel, grid, block, shm, stream, __VA_ARGS__); \
+  } while (0)
+
+// Same, recording the launch's algorithmic bytes (minimum HBM traffic of the
+// work it does) for roofline accounting.
+#define GQF_LAUNCH_B(bytes, kernel, grid, block, shm, stream, ...)     \
+  do {                                                                 \
+    ::gqf::ProfScope _gqf_ps(#kernel, (stream), (double)(bytes));      \
+    hipLaunchKernelGGL(kernel, grid, block, shm, stream, __VA_ARGS__); \
+  } while (0)
+
 // Timing/blocking scaffold: record start, run body, record stop, sync, read
 // the device error counter, report ms.
 struct CallScope {

@@ -101,90 +101,576 @@ __global__ __launch_bounds__(256) void k_sqf_lookup_sorted(const uint8_t* __rest
   out[idx[i]] = sqf_lookup_one<BPE>(t, S, (fp >> r) & (uint32_t)mask64(q), fp & ((1u << r) - 1u));
 }
 
-// ---------------------------------------------------------------- table writer
-// One workgroup per run of WR_SLOTS slots: gathers the entries whose location
-// (and the quotients) fall in its range into LDS and writes every byte of the
-// range once, zeros included.  Element rules as setMetadata /
-// segmentedQFWrite / setOccupiedBits (quotientFilter.cu:700-735, 1430-1443).
-constexpr uint32_t WR_SLOTS = 4096;
+// ---------------------------------------------------------------- fused rebuild
+// From sorted fingerprints to the finished table: optional merge of two sorted
+// inputs (merge path, A first on ties), the keep rule (all / unique /
+// dedup-insert / delete flags), the canonical locations
+// loc_i = max(fq_i, loc_{i-1}+1), and the table bytes.  A block of entries
+// acts on the incoming location as x -> max(x + c, l); the pairs (c, l) are
+// combined in chunk order:
+//   k_rb_agg    each chunk's (c, l) (one read of the inputs),
+//   k_rb_scan   exclusive scan over the chunks (one workgroup); the total
+//               decides capacity before anything is written,
+//   k_rb_write  each chunk walks its 4096-entry sub-tiles with a running
+//               prefix; sub-tile u owns the slot range (L_{u-1}, L_u] ending at
+//               its last entry and writes every byte of it through an LDS
+//               window; slots past the last entry are cleared by
+//               k_sqf_clear_tail.  An occupied bit that falls in an earlier
+//               sub-tile's range goes to a fix-up list (k_sqf_fixup).
+// Replaces the location and write stages of the three reference builds
+// (quotientFilter.cu:788-857, 985-1038, 1492-1539), the merge rebuild
+// (quotientFilter.cu:1651-1719) and the end states of insert /
+// superclusterDeletes.
+constexpr int RB_THREADS = 512;
+constexpr int RB_ITEMS = 8;
+constexpr int RB_WAVES = RB_THREADS / 64;
+constexpr int RB_TILE = RB_THREADS * RB_ITEMS;
+constexpr uint32_t RB_WIN = 8192;  // table bytes per write window
+constexpr uint32_t RB_FIX = 64;    // per-sub-tile occupied-bit fix-up slots
+// LDS: phase A = merged values u64[RB_TILE+1] + merge staging u32[RB_TILE];
+//      phase B = loc u32[RB_TILE] + element u32[RB_TILE] + run-head quotient u32[RB_TILE] + window
+constexpr uint32_t RB_RAW_A = 8 * (RB_TILE + 2) + 4 * RB_TILE;
+constexpr uint32_t RB_RAW_B = 12 * RB_TILE + RB_WIN;
+constexpr uint32_t RB_RAW = RB_RAW_A > RB_RAW_B ? RB_RAW_A : RB_RAW_B;
 
-template <typename T>
-__device__ __forceinline__ uint32_t lower_bound_u32(const T* __restrict__ a, uint32_t lo, uint32_t hi, uint64_t v) {
-  while (lo < hi) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if ((uint64_t)a[mid] < v) lo = mid + 1;
-    else hi = mid;
-  }
-  return lo;
-}
-template <typename K>
-__device__ __forceinline__ uint32_t lower_bound_fq(const K* __restrict__ fp, uint32_t lo, uint32_t hi, uint64_t v,
-                                                   uint32_t r, uint64_t mq) {
-  while (lo < hi) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if ((((uint64_t)fp[mid] >> r) & mq) < v) lo = mid + 1;
-    else hi = mid;
-  }
-  return lo;
+// Merge-path split (A entries among the first u*RB_TILE outputs) of every sub-tile boundary.
+__global__ __launch_bounds__(256) void k_rb_splits(const uint32_t* __restrict__ A, uint32_t na,
+                                                   const uint32_t* __restrict__ B, uint32_t nb, uint32_t nsub,
+                                                   uint32_t* __restrict__ splits) {
+  const uint32_t u = blockIdx.x * 256 + threadIdx.x;
+  if (u > nsub) return;
+  const uint64_t N = (uint64_t)na + nb;
+  const uint32_t d = (uint32_t)((uint64_t)u * RB_TILE < N ? (uint64_t)u * RB_TILE : N);
+  splits[u] = merge_path(A, na, B, nb, d);
 }
 
-template <typename K>
-__global__ __launch_bounds__(256) void k_sqf_write(uint8_t* __restrict__ t, uint32_t S, uint64_t tbytes, uint32_t q,
-                                                   uint32_t r, uint32_t bpe, const K* __restrict__ ufp,
-                                                   const uint32_t* __restrict__ loc, const uint32_t* __restrict__ ctr) {
-  if (ctr[CTR_ERR]) return;  // capacity error: the table stays as it was
-  __shared__ uint32_t s_el[WR_SLOTS];
-  __shared__ uint32_t s_rng[4];
-  const uint32_t m = ctr[CTR_COUNT];
-  const uint32_t tid = threadIdx.x;
-  const uint32_t s0 = blockIdx.x * WR_SLOTS;
-  const uint32_t s1 = s0 + WR_SLOTS < S ? s0 + WR_SLOTS : S;
-  const uint64_t mq = mask64(q);
-  for (uint32_t i = tid; i < WR_SLOTS; i += 256) s_el[i] = 0u;
-  if (tid == 0) s_rng[0] = lower_bound_u32(loc, 0u, m, s0);
-  if (tid == 64) s_rng[1] = lower_bound_u32(loc, 0u, m, s1);
-  if (tid == 128) s_rng[2] = lower_bound_fq(ufp, 0u, m, s0, r, mq);
-  if (tid == 192) s_rng[3] = lower_bound_fq(ufp, 0u, m, s1, r, mq);
-  __syncthreads();
-  for (uint32_t i = s_rng[0] + tid; i < s_rng[1]; i += 256) {
-    const uint64_t fp = (uint64_t)ufp[i];
-    const uint64_t fq = (fp >> r) & mq;
-    const uint32_t lc = loc[i];
-    uint32_t e = (uint32_t)(fp & ((1ull << r) - 1ull)) << 3;
-    if (i > 0 && ((((uint64_t)ufp[i - 1]) >> r) & mq) == fq) e |= 2u;
-    if ((uint64_t)lc != fq) e |= 1u;
-    s_el[lc - s0] = e;
-  }
-  __syncthreads();
-  for (uint32_t i = s_rng[2] + tid; i < s_rng[3]; i += 256) {
-    const uint64_t fq = ((uint64_t)ufp[i] >> r) & mq;
-    if (i == 0 || ((((uint64_t)ufp[i - 1]) >> r) & mq) != fq) s_el[fq - s0] |= 4u;
-  }
-  __syncthreads();
-  // write bytes [s0*bpe, e1) as aligned 32-bit words, big-endian slots
-  const uint64_t b0 = (uint64_t)s0 * bpe;
-  const bool last = s1 == S;
-  const uint64_t e1 = last ? tbytes : (uint64_t)s1 * bpe;
-  const uint64_t nb = e1 - b0;
-  const uint64_t nwords = nb / 4;
-  uint32_t* tw = reinterpret_cast<uint32_t*>(t + b0);
-  for (uint64_t w = tid; w < nwords; w += 256) {
-    uint32_t word = 0;
+// Fills v[0..RB_ITEMS] for this thread's RB_ITEMS consecutive entries of
+// sub-tile [d0, d0+tn): v[0] = the entry before them, v[1+j] = entry j, each
+// as (fp << 1 | source).  MERGE: merge path of A and B staged through LDS;
+// otherwise direct 16-byte loads.
+template <bool MERGE>
+__device__ __forceinline__ void rb_load(const uint32_t* __restrict__ A, uint32_t na, const uint32_t* __restrict__ B,
+                                        uint32_t nb, const uint32_t* __restrict__ splits, uint32_t u, uint32_t d0,
+                                        uint32_t tn, uint64_t* s_val, uint32_t* s_stage, uint64_t* v) {
+  const int tid = threadIdx.x;
+  const uint32_t i0 = (uint32_t)tid * RB_ITEMS;
+  if (MERGE) {
+    const uint32_t a0 = splits[u], a1 = splits[u + 1];
+    const uint32_t b0 = d0 - a0, b1 = d0 + tn - a1;
+    const uint32_t la = a1 - a0, lb = b1 - b0;
+    {
+      // la + lb == tn <= RB_TILE: each thread stages at most RB_ITEMS entries
+      uint32_t w[RB_ITEMS];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const uint64_t B = w * 4 + k;
-      const uint64_t slot = B / bpe;
-      uint32_t byte = 0;
-      if (s0 + slot < S) byte = (s_el[slot] >> (8 * (bpe - 1 - (uint32_t)(B % bpe)))) & 0xFFu;
-      word |= byte << (8 * k);
+      for (int k = 0; k < RB_ITEMS; ++k) {
+        const uint32_t i = tid + k * RB_THREADS;
+        w[k] = i < la ? A[a0 + i] : (i < la + lb ? B[b0 + (i - la)] : 0u);
+      }
+#pragma unroll
+      for (int k = 0; k < RB_ITEMS; ++k) {
+        const uint32_t i = tid + k * RB_THREADS;
+        if (i < la + lb) s_stage[i] = w[k];
+      }
     }
-    tw[w] = word;
+    if (tid == 0) {
+      uint64_t prev = 0;
+      if (d0 > 0) {
+        if (b0 > 0 && (a0 == 0 || A[a0 - 1] <= B[b0 - 1])) prev = ((uint64_t)B[b0 - 1] << 1) | 1ull;
+        else prev = (uint64_t)A[a0 - 1] << 1;
+      }
+      s_val[0] = prev;
+    }
+    __syncthreads();
+    const uint32_t ld = i0 < tn ? i0 : tn;
+    uint32_t ai = merge_path(s_stage, la, s_stage + la, lb, ld);
+    uint32_t bi = ld - ai;
+#pragma unroll
+    for (int j = 0; j < RB_ITEMS; ++j) {
+      const uint32_t o = ld + j;
+      if (o < tn) {
+        const bool takeA = bi >= lb || (ai < la && s_stage[ai] <= s_stage[la + bi]);
+        s_val[1 + o] = takeA ? ((uint64_t)s_stage[ai++] << 1) : (((uint64_t)s_stage[la + bi++] << 1) | 1ull);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j <= RB_ITEMS; ++j) v[j] = (i0 + j <= tn) ? s_val[i0 + j] : 0ull;
+  } else {
+    const uint64_t g0 = (uint64_t)d0 + i0;
+    uint32_t w[RB_ITEMS];
+    if (i0 + RB_ITEMS <= tn) {
+      // d0 is a multiple of RB_TILE and A is 256-byte aligned: 32-byte aligned runs
+      const uint4* p = reinterpret_cast<const uint4*>(A + g0);
+#pragma unroll
+      for (int k = 0; k < RB_ITEMS / 4; ++k) {
+        const uint4 x = p[k];
+        w[4 * k] = x.x;
+        w[4 * k + 1] = x.y;
+        w[4 * k + 2] = x.z;
+        w[4 * k + 3] = x.w;
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < RB_ITEMS; ++k) w[k] = (i0 + k < tn) ? A[g0 + k] : 0u;
+    }
+    v[0] = g0 > 0 && i0 <= tn ? (uint64_t)A[g0 - 1] << 1 : 0ull;
+#pragma unroll
+    for (int k = 0; k < RB_ITEMS; ++k) v[1 + k] = (uint64_t)w[k] << 1;
   }
-  for (uint64_t B = nwords * 4 + tid; B < nb; B += 256) {
-    const uint64_t slot = B / bpe;
-    uint32_t byte = 0;
-    if (s0 + slot < S) byte = (s_el[slot] >> (8 * (bpe - 1 - (uint32_t)(B % bpe)))) & 0xFFu;
-    t[b0 + B] = (uint8_t)byte;
+}
+
+// Keep rule and run heads of this thread's RB_ITEMS consecutive entries, and
+// their (count, last location) aggregate.
+template <int MODE, bool HEADS>
+__device__ __forceinline__ CL rb_classify(const uint64_t* v, uint32_t d0, uint32_t tn, uint32_t r, uint64_t mq,
+                                          const uint32_t* __restrict__ A, const uint8_t* __restrict__ flags,
+                                          uint32_t* fpv, uint32_t& keepbits, uint32_t& headbits, uint32_t& nhead) {
+  CL agg = {0u, LOC_NONE};
+  keepbits = 0;
+  headbits = 0;
+  nhead = 0;
+  const uint32_t i0 = (uint32_t)threadIdx.x * RB_ITEMS;
+#pragma unroll
+  for (int j = 0; j < RB_ITEMS; ++j) {
+    const uint32_t i = i0 + j;
+    fpv[j] = 0;
+    if (i < tn) {
+      const uint64_t key = v[1 + j], pkey = v[j];
+      const uint32_t fp = (uint32_t)(key >> 1), pfp = (uint32_t)(pkey >> 1);
+      const bool first = (d0 + i) == 0;
+      const uint64_t fq = ((uint64_t)fp >> r) & mq;
+      bool kp;
+      if (MODE == KEEP_ALL) kp = true;
+      else if (MODE == KEEP_UNIQUE) kp = first || fp != pfp;
+      else if (MODE == KEEP_TAGGED_INSERT) kp = ((key & 1ull) == 0) || first || fp != pfp;
+      else kp = flags[d0 + i] != 0;
+      fpv[j] = fp;
+      if (kp) {
+        keepbits |= 1u << j;
+        if (HEADS) {
+          bool head;
+          if (MODE != KEEP_FLAGS) {
+            // a dropped entry always repeats an earlier kept fingerprint, so
+            // the previous entry decides whether this one opens its run
+            head = first || (((uint64_t)pfp >> r) & mq) != fq;
+          } else {
+            head = true;  // opens its run unless an earlier kept entry shares the quotient
+            for (int64_t p = (int64_t)(d0 + i) - 1; p >= 0; --p) {
+              if ((((uint64_t)A[p] >> r) & mq) != fq) break;
+              if (flags[p]) {
+                head = false;
+                break;
+              }
+            }
+          }
+          if (head) {
+            headbits |= 1u << j;
+            nhead++;
+          }
+        }
+        const int64_t f = (int64_t)fq;
+        agg.l = agg.l == LOC_NONE ? f : (f > agg.l + 1 ? f : agg.l + 1);
+        agg.c++;
+      }
+    }
+  }
+  return agg;
+}
+
+// Exclusive scan of (c, l) across the 64 lanes of a wave; *total = the
+// wave's inclusive aggregate.
+__device__ __forceinline__ CL cl_wave_excl(CL x, CL* total) {
+  const int lane = threadIdx.x & 63;
+  CL inc = x;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    CL y;
+    y.c = __shfl_up(inc.c, d, 64);
+    y.l = __shfl_up(inc.l, d, 64);
+    if (lane >= d) inc = cl_combine(y, inc);
+  }
+  CL ex;
+  ex.c = __shfl_up(inc.c, 1, 64);
+  ex.l = __shfl_up(inc.l, 1, 64);
+  if (lane == 0) {
+    ex.c = 0;
+    ex.l = LOC_NONE;
+  }
+  total->c = __shfl(inc.c, 63, 64);
+  total->l = __shfl(inc.l, 63, 64);
+  return ex;
+}
+
+struct RbPlan {
+  uint32_t G, chunk, nsub;  // chunks, entries per chunk (multiple of RB_TILE), sub-tiles
+};
+static inline RbPlan rb_plan(uint64_t N) {
+  RbPlan p;
+  const SortPlan sp = sort_plan((uint32_t)N);
+  p.G = sp.G;
+  p.chunk = sp.chunk;
+  p.nsub = (uint32_t)((N + RB_TILE - 1) / RB_TILE);
+  return p;
+}
+
+template <int MODE, bool MERGE>
+__global__ __launch_bounds__(RB_THREADS) void k_rb_agg(const uint32_t* __restrict__ A, uint32_t na,
+                                                const uint32_t* __restrict__ B, uint32_t nb,
+                                                const uint8_t* __restrict__ flags, const uint32_t* __restrict__ splits,
+                                                uint32_t q, uint32_t r, uint32_t chunk,
+                                                uint32_t* __restrict__ agg_c, long long* __restrict__ agg_l) {
+  __shared__ __align__(16) unsigned char s_raw[RB_RAW_A];
+  __shared__ uint32_t s_wc[RB_WAVES];
+  __shared__ int64_t s_wl[RB_WAVES];
+  uint64_t* s_val = reinterpret_cast<uint64_t*>(s_raw);
+  uint32_t* s_stage = reinterpret_cast<uint32_t*>(s_raw + 8 * (RB_TILE + 2));
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const uint64_t N = (uint64_t)na + (MERGE ? nb : 0u);
+  const uint64_t mq = mask64(q);
+  const uint64_t c0 = (uint64_t)blockIdx.x * chunk;
+  const uint64_t c1 = c0 + chunk < N ? c0 + chunk : N;
+  CL X = {0u, LOC_NONE};
+  for (uint64_t d0 = c0; d0 < c1; d0 += RB_TILE) {
+    const uint32_t tn = (uint32_t)(c1 - d0 < (uint64_t)RB_TILE ? c1 - d0 : (uint64_t)RB_TILE);
+    uint64_t v[RB_ITEMS + 1];
+    rb_load<MERGE>(A, na, B, nb, splits, (uint32_t)(d0 / RB_TILE), (uint32_t)d0, tn, s_val, s_stage, v);
+    uint32_t fpv[RB_ITEMS], kb, hb, nh;
+    const CL a = rb_classify<MODE, false>(v, (uint32_t)d0, tn, r, mq, A, flags, fpv, kb, hb, nh);
+    CL wt;
+    (void)cl_wave_excl(a, &wt);
+    if (lane == 0) {
+      s_wc[wid] = wt.c;
+      s_wl[wid] = wt.l;
+    }
+    __syncthreads();
+    CL sub = {0u, LOC_NONE};
+#pragma unroll
+    for (int w = 0; w < RB_WAVES; ++w) sub = cl_combine(sub, CL{s_wc[w], s_wl[w]});
+    X = cl_combine(X, sub);
+    __syncthreads();
+  }
+  if (tid == 0) {
+    agg_c[blockIdx.x] = X.c;
+    agg_l[blockIdx.x] = X.l;
+  }
+}
+
+// Exclusive scan of the chunk aggregates (one workgroup, 1024 threads).  The
+// total fixes the kept count, the first slot after the last entry, and a
+// capacity error -- all before any table byte is written.
+__global__ __launch_bounds__(1024) void k_rb_scan(uint32_t* __restrict__ agg_c, long long* __restrict__ agg_l,
+                                                  uint32_t G, uint64_t S, uint32_t* __restrict__ ctr) {
+  __shared__ uint32_t s_wc[16];
+  __shared__ int64_t s_wl[16];
+  __shared__ uint32_t s_cc;
+  __shared__ int64_t s_cl;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  if (tid == 0) {
+    s_cc = 0;
+    s_cl = LOC_NONE;
+  }
+  __syncthreads();
+  for (uint32_t b = 0; b < G; b += 1024) {
+    const uint32_t i = b + tid;
+    CL x = {0u, LOC_NONE};
+    if (i < G) x = CL{agg_c[i], (int64_t)agg_l[i]};
+    CL wt;
+    CL ex = cl_wave_excl(x, &wt);
+    if (lane == 0) {
+      s_wc[wid] = wt.c;
+      s_wl[wid] = wt.l;
+    }
+    __syncthreads();
+    CL pre = {s_cc, s_cl}, tot = {s_cc, s_cl};
+#pragma unroll
+    for (int w = 0; w < 16; ++w) {
+      const CL wv = {s_wc[w], s_wl[w]};
+      if (w < wid) pre = cl_combine(pre, wv);
+      tot = cl_combine(tot, wv);
+    }
+    ex = cl_combine(pre, ex);
+    if (i < G) {
+      agg_c[i] = ex.c;
+      agg_l[i] = ex.l;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      s_cc = tot.c;
+      s_cl = tot.l;
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    ctr[CTR_COUNT] = s_cc;
+    const int64_t L = s_cl;
+    ctr[CTR_MISC + 3] = L == LOC_NONE ? 0u : (uint32_t)(L + 1 < (int64_t)S ? L + 1 : (int64_t)S);
+    if (L != LOC_NONE && (uint64_t)L >= S) ctr[CTR_ERR] |= DEV_ERR_CAPACITY;
+  }
+}
+
+template <int MODE, bool MERGE>
+__global__ __launch_bounds__(RB_THREADS) void k_rb_write(uint8_t* __restrict__ t, uint32_t S, uint32_t q, uint32_t r,
+                                                  uint32_t bpe, const uint32_t* __restrict__ A, uint32_t na,
+                                                  const uint32_t* __restrict__ B, uint32_t nb,
+                                                  const uint8_t* __restrict__ flags,
+                                                  const uint32_t* __restrict__ splits, uint32_t chunk,
+                                                  const uint32_t* __restrict__ pre_c, const long long* __restrict__ pre_l,
+                                                  const uint32_t* __restrict__ ctr, uint32_t* __restrict__ fix,
+                                                  uint32_t* __restrict__ fix_cnt, uint32_t* __restrict__ fix_more,
+                                                  uint32_t* __restrict__ fix_more_n, uint32_t fix_cap,
+                                                  int precleared) {
+  if (ctr[CTR_ERR]) return;  // capacity error: the table stays as it was
+  __shared__ __align__(16) unsigned char s_raw[RB_RAW];
+  __shared__ uint32_t s_wc[RB_WAVES], s_hc[RB_WAVES];
+  __shared__ int64_t s_wl[RB_WAVES];
+  __shared__ uint32_t s_tmp[4];
+  uint64_t* s_val = reinterpret_cast<uint64_t*>(s_raw);                      // [RB_TILE + 1]
+  uint32_t* s_stage = reinterpret_cast<uint32_t*>(s_raw + 8 * (RB_TILE + 2));  // [RB_TILE]
+  uint32_t* s_loc = reinterpret_cast<uint32_t*>(s_raw);                      // [RB_TILE]
+  uint32_t* s_el = reinterpret_cast<uint32_t*>(s_raw + 4 * RB_TILE);         // [RB_TILE]
+  uint32_t* s_ffq = reinterpret_cast<uint32_t*>(s_raw + 8 * RB_TILE);        // [RB_TILE]
+  uint8_t* s_win = s_raw + 12 * RB_TILE;                                      // [RB_WIN]
+
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const uint64_t N = (uint64_t)na + (MERGE ? nb : 0u);
+  const uint64_t mq = mask64(q);
+  const uint64_t c0 = (uint64_t)blockIdx.x * chunk;
+  const uint64_t c1 = c0 + chunk < N ? c0 + chunk : N;
+  CL X = {pre_c[blockIdx.x], (int64_t)pre_l[blockIdx.x]};
+  const uint32_t wslots = RB_WIN / bpe;
+  for (uint64_t d0 = c0; d0 < c1; d0 += RB_TILE) {
+    const uint32_t u = (uint32_t)(d0 / RB_TILE);
+    const uint32_t tn = (uint32_t)(c1 - d0 < (uint64_t)RB_TILE ? c1 - d0 : (uint64_t)RB_TILE);
+    uint64_t v[RB_ITEMS + 1];
+    rb_load<MERGE>(A, na, B, nb, splits, u, (uint32_t)d0, tn, s_val, s_stage, v);
+    uint32_t fpv[RB_ITEMS], keepbits, headbits, nhead;
+    const CL a = rb_classify<MODE, true>(v, (uint32_t)d0, tn, r, mq, A, flags, fpv, keepbits, headbits, nhead);
+    CL wt;
+    CL ex = cl_wave_excl(a, &wt);
+    // run-head counts: exclusive within the block
+    uint32_t hinc = nhead;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t y = __shfl_up(hinc, d, 64);
+      if (lane >= d) hinc += y;
+    }
+    if (lane == 63) {
+      s_wc[wid] = wt.c;
+      s_wl[wid] = wt.l;
+      s_hc[wid] = hinc;
+    }
+    __syncthreads();  // all threads are done with s_val / s_stage: LDS is reused below
+    CL wpre = {0u, LOC_NONE}, sub = {0u, LOC_NONE};
+    uint32_t hpre = 0, tf = 0;
+#pragma unroll
+    for (int w = 0; w < RB_WAVES; ++w) {
+      const CL wv = {s_wc[w], s_wl[w]};
+      if (w < wid) {
+        wpre = cl_combine(wpre, wv);
+        hpre += s_hc[w];
+      }
+      sub = cl_combine(sub, wv);
+      tf += s_hc[w];
+    }
+    const CL me = cl_combine(cl_combine(X, wpre), ex);
+    // this thread's territory: the slots after the previous kept entry up to
+    // its own last kept entry (territories tile the sub-tile's owned range)
+    const int64_t tstart = me.l == LOC_NONE ? 0 : me.l + 1;
+    int64_t tend = tstart - 1;
+    uint32_t o0 = me.c - X.c;
+    {
+      int64_t prev = me.l;
+      uint32_t o = o0, h = hpre + hinc - nhead;
+#pragma unroll
+      for (int j = 0; j < RB_ITEMS; ++j) {
+        if (keepbits & (1u << j)) {
+          const int64_t fq = (int64_t)(((uint64_t)fpv[j] >> r) & mq);
+          const int64_t lc = prev == LOC_NONE ? fq : (fq > prev + 1 ? fq : prev + 1);
+          const bool head = (headbits >> j) & 1u;
+          uint32_t e = (fpv[j] & ((1u << r) - 1u)) << 3;
+          if (!head) e |= 2u;
+          if (lc != fq) e |= 1u;
+          s_loc[o] = (uint32_t)lc;
+          s_el[o] = e;
+          if (head) s_ffq[h++] = (uint32_t)fq;
+          prev = lc;
+          tend = lc;
+          ++o;
+        }
+      }
+    }
+    __syncthreads();
+    const uint32_t tc = sub.c;
+    const CL Y = cl_combine(X, sub);
+    const uint32_t Llo = X.l == LOC_NONE ? 0u : (uint32_t)(X.l + 1);
+    const uint32_t Lhi = tc > 0 ? (uint32_t)(Y.l + 1) : Llo;
+    // run heads whose quotient precedes the owned range (a prefix of the
+    // ascending head list): their occupied bits go to the fix-up slots
+    uint32_t nfix;
+    {
+      uint32_t lo = 0, hi = tf;
+      while (lo < hi) {
+        const uint32_t m = (lo + hi) >> 1;
+        if (s_ffq[m] < Llo) lo = m + 1;
+        else hi = m;
+      }
+      nfix = lo;
+      const uint32_t mine = nfix < RB_FIX ? nfix : RB_FIX;
+      for (uint32_t f = tid; f < mine; f += RB_THREADS) fix[(uint64_t)u * RB_FIX + f] = s_ffq[f];
+      if (tid == 0) {
+        fix_cnt[u] = mine;
+        if (nfix > RB_FIX) s_tmp[0] = atomicAdd(fix_more_n, nfix - RB_FIX);
+      }
+      if (nfix > RB_FIX) {
+        __syncthreads();
+        const uint32_t k = s_tmp[0];
+        for (uint32_t f = RB_FIX + tid; f < nfix; f += RB_THREADS) {
+          const uint32_t slot = k + (f - RB_FIX);
+          if (slot < fix_cap) fix_more[slot] = s_ffq[f];
+        }
+      }
+    }
+    if ((uint64_t)(Lhi - Llo) * bpe <= RB_WIN) {
+      // fast path: the whole owned range fits one window; every thread fills
+      // its own territory (entries, zeros and occupied bits), then one copy out
+      if (tend >= tstart) {
+        uint32_t o = o0;
+        // first head at or after the territory start
+        uint32_t lo = nfix, hi = tf;
+        while (lo < hi) {
+          const uint32_t m = (lo + hi) >> 1;
+          if ((int64_t)s_ffq[m] < tstart) lo = m + 1;
+          else hi = m;
+        }
+        uint32_t f = lo;
+        for (int64_t sl = tstart; sl <= tend; ++sl) {
+          uint32_t e = 0;
+          if (s_loc[o] == (uint32_t)sl) e = s_el[o++];
+          if (f < tf && (int64_t)s_ffq[f] == sl) {
+            e |= 4u;
+            ++f;
+          }
+          const uint32_t off = (uint32_t)(sl - Llo) * bpe;
+          for (uint32_t b = 0; b < bpe; ++b) s_win[off + b] = (uint8_t)(e >> (8 * (bpe - 1 - b)));
+        }
+      }
+      __syncthreads();
+      if (Lhi > Llo) {
+        const uint64_t B0 = (uint64_t)Llo * bpe, B1 = (uint64_t)Lhi * bpe;
+        const uint64_t A0 = (B0 + 3) & ~3ull, A1 = B1 & ~3ull;
+        if (A0 >= A1) {
+          for (uint64_t b = B0 + tid; b < B1; b += RB_THREADS) t[b] = s_win[b - B0];
+        } else {
+          for (uint64_t b = B0 + tid; b < A0; b += RB_THREADS) t[b] = s_win[b - B0];
+          for (uint64_t b = A1 + tid; b < B1; b += RB_THREADS) t[b] = s_win[b - B0];
+          uint32_t* tw = reinterpret_cast<uint32_t*>(t + A0);
+          const uint32_t nw = (uint32_t)((A1 - A0) / 4), sh = (uint32_t)(A0 - B0);
+          for (uint32_t w = tid; w < nw; w += RB_THREADS) {
+            const uint32_t q0 = sh + 4 * w;
+            tw[w] = (uint32_t)s_win[q0] | ((uint32_t)s_win[q0 + 1] << 8) | ((uint32_t)s_win[q0 + 2] << 16) |
+                    ((uint32_t)s_win[q0 + 3] << 24);
+          }
+        }
+      }
+    } else {
+      // long owned range (sparse stretch): window by window
+      for (uint32_t w0 = Llo; w0 < Lhi; w0 += wslots) {
+        const uint32_t w1 = w0 + wslots < Lhi ? w0 + wslots : Lhi;
+        uint32_t k0, k1, f0, f1;
+        {
+          uint32_t lo = 0, hi = tc;
+          while (lo < hi) { const uint32_t m = (lo + hi) >> 1; if (s_loc[m] < w0) lo = m + 1; else hi = m; }
+          k0 = lo;
+          hi = tc;
+          while (lo < hi) { const uint32_t m = (lo + hi) >> 1; if (s_loc[m] < w1) lo = m + 1; else hi = m; }
+          k1 = lo;
+          lo = 0;
+          hi = tf;
+          while (lo < hi) { const uint32_t m = (lo + hi) >> 1; if (s_ffq[m] < w0) lo = m + 1; else hi = m; }
+          f0 = lo;
+          hi = tf;
+          while (lo < hi) { const uint32_t m = (lo + hi) >> 1; if (s_ffq[m] < w1) lo = m + 1; else hi = m; }
+          f1 = lo;
+        }
+        if (precleared && k0 == k1 && f0 == f1) continue;  // nothing but zeros, already cleared
+        const uint64_t B0 = (uint64_t)w0 * bpe, B1 = (uint64_t)w1 * bpe;
+        const uint32_t nbytes = (uint32_t)(B1 - B0);
+        for (uint32_t i = tid; i < (nbytes + 3) / 4; i += RB_THREADS) reinterpret_cast<uint32_t*>(s_win)[i] = 0u;
+        __syncthreads();
+        for (uint32_t k = k0 + tid; k < k1; k += RB_THREADS) {
+          const uint32_t off = (s_loc[k] - w0) * bpe;
+          const uint32_t e = s_el[k];
+          for (uint32_t b = 0; b < bpe; ++b) s_win[off + b] = (uint8_t)(e >> (8 * (bpe - 1 - b)));
+        }
+        __syncthreads();
+        for (uint32_t f = f0 + tid; f < f1; f += RB_THREADS) s_win[(s_ffq[f] - w0) * bpe + bpe - 1] |= 4u;
+        __syncthreads();
+        const uint64_t A0 = (B0 + 3) & ~3ull, A1 = B1 & ~3ull;
+        if (A0 >= A1) {
+          for (uint64_t b = B0 + tid; b < B1; b += RB_THREADS) t[b] = s_win[b - B0];
+        } else {
+          for (uint64_t b = B0 + tid; b < A0; b += RB_THREADS) t[b] = s_win[b - B0];
+          for (uint64_t b = A1 + tid; b < B1; b += RB_THREADS) t[b] = s_win[b - B0];
+          uint32_t* tw = reinterpret_cast<uint32_t*>(t + A0);
+          const uint32_t nw = (uint32_t)((A1 - A0) / 4), sh = (uint32_t)(A0 - B0);
+          for (uint32_t w = tid; w < nw; w += RB_THREADS) {
+            const uint32_t q0 = sh + 4 * w;
+            tw[w] = (uint32_t)s_win[q0] | ((uint32_t)s_win[q0 + 1] << 8) | ((uint32_t)s_win[q0 + 2] << 16) |
+                    ((uint32_t)s_win[q0 + 3] << 24);
+          }
+        }
+        __syncthreads();
+      }
+    }
+    X = Y;
+    __syncthreads();
+  }
+}
+
+// Zeroes every table byte from the slot after the last entry to the end of
+// the allocation (the 10% overflow area, usually), in parallel.
+// (whole_table: every byte, used to pre-clear sparse tables).  Nothing is
+// cleared after a capacity error.
+__global__ __launch_bounds__(256) void k_sqf_clear_tail(uint8_t* __restrict__ t, uint32_t bpe, uint64_t tbytes,
+                                                        const uint32_t* __restrict__ ctr, int whole_table) {
+  if (ctr[CTR_ERR]) return;
+  const uint64_t b0 = whole_table ? 0ull : (uint64_t)ctr[CTR_MISC + 3] * bpe;
+  const uint64_t a0 = (b0 + 15) & ~15ull, a1 = tbytes & ~15ull;
+  const uint64_t tid = blockIdx.x * 256ull + threadIdx.x, nth = (uint64_t)gridDim.x * 256;
+  if (a0 >= a1) {
+    for (uint64_t b = b0 + tid; b < tbytes; b += nth) t[b] = 0;
+    return;
+  }
+  for (uint64_t b = b0 + tid; b < a0; b += nth) t[b] = 0;
+  for (uint64_t b = a1 + tid; b < tbytes; b += nth) t[b] = 0;
+  uint4* w = reinterpret_cast<uint4*>(t + a0);
+  const uint64_t nw = (a1 - a0) / 16;
+  for (uint64_t i = tid; i < nw; i += nth) w[i] = make_uint4(0u, 0u, 0u, 0u);
+}
+
+// Occupied bits that rebuild tiles could not place in their own ranges: one
+// workgroup per tile's slots, plus the shared overflow list.
+__device__ __forceinline__ void sqf_set_occupied(uint8_t* t, uint32_t bpe, uint32_t fq) {
+  const uint64_t byte = (uint64_t)fq * bpe + bpe - 1;  // the element's low byte holds the flag bits
+  uint32_t* word = reinterpret_cast<uint32_t*>(t + (byte & ~3ull));
+  atomicOr(word, 4u << (8 * (uint32_t)(byte & 3)));
+}
+
+__global__ __launch_bounds__(64) void k_sqf_fixup(uint8_t* __restrict__ t, uint32_t bpe,
+                                                  const uint32_t* __restrict__ fix,
+                                                  const uint32_t* __restrict__ fix_cnt,
+                                                  const uint32_t* __restrict__ fix_more,
+                                                  const uint32_t* __restrict__ ctr, uint32_t tiles,
+                                                  uint32_t fix_cap) {
+  if (ctr[CTR_ERR]) return;
+  const uint32_t tile = blockIdx.x;
+  if (tile < tiles) {
+    const uint32_t c = fix_cnt[tile];
+    for (uint32_t i = threadIdx.x; i < c; i += 64) sqf_set_occupied(t, bpe, fix[(uint64_t)tile * RB_FIX + i]);
+  } else {
+    uint32_t n = ctr[CTR_MISC + 2];  // shared overflow list length
+    if (n > fix_cap) n = fix_cap;
+    for (uint32_t i = threadIdx.x; i < n; i += 64) sqf_set_occupied(t, bpe, fix_more[i]);
   }
 }
 
@@ -323,21 +809,13 @@ static SqfGeom geom(const gqf_sqf& qf) {
   return g;
 }
 
-static void reset_op_counters(hipStream_t s, DeviceState& ds) {
-  hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, s, ds.counters, (int)CTR_SCAN_TILE, (int)CTR_MISC);
-}
+static gqf_status read_counter(hipStream_t s, DeviceState& ds, const uint32_t* dptr, uint32_t* out);
 
-template <typename K>
-static void launch_write(hipStream_t s, DeviceState& ds, const gqf_sqf& qf, const SqfGeom& g, const K* ufp,
-                         const uint32_t* loc) {
-  const uint32_t regions = (g.S + WR_SLOTS - 1) / WR_SLOTS;
-  hipLaunchKernelGGL(k_sqf_write<K>, dim3(regions), dim3(256), 0, s, qf.table, g.S, g.bytes, g.q, g.r, g.bpe, ufp, loc,
-                     ds.counters);
-}
-
-// decode the table into ascending fingerprints; the count lands in *count_dev
+// decode the table into ascending fingerprints; the count (read back to the
+// host) goes to *count
 static gqf_status sqf_decode(hipStream_t s, DeviceState& ds, Arena& ar, const gqf_sqf& qf, const SqfGeom& g,
-                             uint32_t** fps_out, uint32_t* count_dev) {
+                             uint32_t** fps_out, uint32_t* count) {
+  uint32_t* count_dev = ds.counters + CTR_MISC;
   const uint32_t regions = (g.S + DC_SLOTS - 1) / DC_SLOTS;
   uint32_t* counts = ar.take<uint32_t>((size_t)regions * 3);
   uint32_t* fps = ar.take<uint32_t>(g.S);
@@ -345,20 +823,22 @@ static gqf_status sqf_decode(hipStream_t s, DeviceState& ds, Arena& ar, const gq
     set_error("internal: decode workspace overflow");
     return GQF_ERR_INVALID;
   }
-  if (g.bpe == 1) hipLaunchKernelGGL(k_sqf_count<1>, dim3(regions), dim3(256), 0, s, qf.table, g.S, counts);
-  else if (g.bpe == 2) hipLaunchKernelGGL(k_sqf_count<2>, dim3(regions), dim3(256), 0, s, qf.table, g.S, counts);
-  else if (g.bpe == 3) hipLaunchKernelGGL(k_sqf_count<3>, dim3(regions), dim3(256), 0, s, qf.table, g.S, counts);
-  else hipLaunchKernelGGL(k_sqf_count<4>, dim3(regions), dim3(256), 0, s, qf.table, g.S, counts);
+  if (g.bpe == 1) GQF_LAUNCH_B((double)g.bytes, k_sqf_count<1>, dim3(regions), dim3(256), 0, s, qf.table, g.S, counts);
+  else if (g.bpe == 2) GQF_LAUNCH_B((double)g.bytes, k_sqf_count<2>, dim3(regions), dim3(256), 0, s, qf.table, g.S, counts);
+  else if (g.bpe == 3) GQF_LAUNCH_B((double)g.bytes, k_sqf_count<3>, dim3(regions), dim3(256), 0, s, qf.table, g.S, counts);
+  else GQF_LAUNCH_B((double)g.bytes, k_sqf_count<4>, dim3(regions), dim3(256), 0, s, qf.table, g.S, counts);
   // totals: column 0 (entries) is the decoded count
-  hipLaunchKernelGGL(k_scan_rows, dim3(1), dim3(1024), 0, s, counts, regions, 3u, count_dev);
+  GQF_LAUNCH(k_scan_rows, dim3(1), dim3(1024), 0, s, counts, regions, 3u, count_dev);
+  GQF_TRY(read_counter(s, ds, count_dev, count));
+  const double dec_bytes = (double)g.bytes + 4.0 * (double)*count;
   if (g.bpe == 1)
-    hipLaunchKernelGGL((k_sqf_decode<1, uint32_t>), dim3(regions), dim3(256), 0, s, qf.table, g.S, g.r, counts, fps);
+    GQF_LAUNCH_B(dec_bytes, (k_sqf_decode<1, uint32_t>), dim3(regions), dim3(256), 0, s, qf.table, g.S, g.r, counts, fps);
   else if (g.bpe == 2)
-    hipLaunchKernelGGL((k_sqf_decode<2, uint32_t>), dim3(regions), dim3(256), 0, s, qf.table, g.S, g.r, counts, fps);
+    GQF_LAUNCH_B(dec_bytes, (k_sqf_decode<2, uint32_t>), dim3(regions), dim3(256), 0, s, qf.table, g.S, g.r, counts, fps);
   else if (g.bpe == 3)
-    hipLaunchKernelGGL((k_sqf_decode<3, uint32_t>), dim3(regions), dim3(256), 0, s, qf.table, g.S, g.r, counts, fps);
+    GQF_LAUNCH_B(dec_bytes, (k_sqf_decode<3, uint32_t>), dim3(regions), dim3(256), 0, s, qf.table, g.S, g.r, counts, fps);
   else
-    hipLaunchKernelGGL((k_sqf_decode<4, uint32_t>), dim3(regions), dim3(256), 0, s, qf.table, g.S, g.r, counts, fps);
+    GQF_LAUNCH_B(dec_bytes, (k_sqf_decode<4, uint32_t>), dim3(regions), dim3(256), 0, s, qf.table, g.S, g.r, counts, fps);
   *fps_out = fps;
   return GQF_OK;
 }
@@ -375,67 +855,96 @@ static gqf_status read_counter(hipStream_t s, DeviceState& ds, const uint32_t* d
   return GQF_OK;
 }
 
-static gqf_status sqf_build_impl(const gqf_sqf& qf, int n, const uint32_t* d_keys, int no_dup, gqf_stream stream,
-                                 float* ms) {
-  GQF_TRY(sqf_check(qf));
-  if (n < 0 || (uint64_t)n >= GQF_MAX_ITEMS || (n > 0 && !d_keys)) {
+static void clear_table(hipStream_t s, const gqf_sqf& qf, const SqfGeom& g) {
+  launch_fill<uint32_t>(s, reinterpret_cast<uint32_t*>(qf.table), g.bytes / 4, 0u);
+  if (g.bytes % 4) launch_fill<uint8_t>(s, qf.table + (g.bytes & ~3ull), g.bytes % 4, (uint8_t)0);
+}
+
+static size_t rebuild_ws_bytes(uint64_t N) {
+  const uint64_t tiles = (N + RB_TILE - 1) / RB_TILE;
+  return aligned_bytes((tiles + 1) * 4) + aligned_bytes(2048 * 12) + aligned_bytes(tiles * RB_FIX * 4) +
+         aligned_bytes(tiles * 4) + aligned_bytes((N + 1) * 4) + 2048;
+}
+
+// Fused rebuild of qf from A (and B when merging) under a keep rule.
+template <int MODE, bool MERGE>
+static gqf_status launch_rebuild(hipStream_t s, DeviceState& ds, Arena& ar, const gqf_sqf& qf, const SqfGeom& g,
+                                 const uint32_t* A, uint32_t na, const uint32_t* B, uint32_t nb,
+                                 const uint8_t* flags) {
+  const uint64_t N = (uint64_t)na + (MERGE ? nb : 0u);
+  GQF_LAUNCH(k_reset_counters, dim3(1), dim3(64), 0, s, ds.counters, (int)CTR_SCAN_TILE, (int)CTR_MISC + 4);
+  if (N == 0) {
+    clear_table(s, qf, g);
+    return GQF_OK;
+  }
+  const RbPlan p = rb_plan(N);
+  uint32_t* splits = ar.take<uint32_t>(p.nsub + 1);
+  uint32_t* agg_c = ar.take<uint32_t>(p.G);
+  long long* agg_l = ar.take<long long>(p.G);
+  uint32_t* fix = ar.take<uint32_t>((size_t)p.nsub * RB_FIX);
+  uint32_t* fix_cnt = ar.take<uint32_t>(p.nsub);
+  uint32_t* fix_more = ar.take<uint32_t>(N + 1);
+  if (ar.used > ar.cap) {
+    set_error("internal: rebuild workspace overflow");
+    return GQF_ERR_INVALID;
+  }
+  if (MERGE)
+    GQF_LAUNCH(k_rb_splits, dim3((p.nsub + 1 + 255) / 256), dim3(256), 0, s, A, na, B, nb, p.nsub, splits);
+  GQF_LAUNCH_B((double)N * 4, (k_rb_agg<MODE, MERGE>), dim3(p.G), dim3(RB_THREADS), 0, s, A, na, B, nb, flags, splits,
+               g.q, g.r, p.chunk, agg_c, agg_l);
+  GQF_LAUNCH(k_rb_scan, dim3(1), dim3(1024), 0, s, agg_c, agg_l, p.G, (uint64_t)g.S, ds.counters);
+  // sparse tables: clear once, then sub-tiles skip windows holding only zeros
+  const int precleared = N * 8 < (uint64_t)g.S ? 1 : 0;
+  if (precleared) GQF_LAUNCH(k_sqf_clear_tail, dim3(1024), dim3(256), 0, s, qf.table, g.bpe, g.bytes, ds.counters, 1);
+  GQF_LAUNCH_B((double)N * 4 + (double)g.bytes, (k_rb_write<MODE, MERGE>), dim3(p.G), dim3(RB_THREADS), 0, s,
+               qf.table, g.S, g.q, g.r, g.bpe, A, na, B, nb, flags, splits, p.chunk, agg_c, agg_l, ds.counters, fix,
+               fix_cnt, fix_more, ds.counters + CTR_MISC + 2, (uint32_t)(N + 1), precleared);
+  if (!precleared) GQF_LAUNCH(k_sqf_clear_tail, dim3(1024), dim3(256), 0, s, qf.table, g.bpe, g.bytes, ds.counters, 0);
+  GQF_LAUNCH(k_sqf_fixup, dim3(p.nsub + 1), dim3(64), 0, s, qf.table, g.bpe, fix, fix_cnt, fix_more, ds.counters,
+             p.nsub, (uint32_t)(N + 1));
+  return GQF_OK;
+}
+
+static gqf_status check_keys(int n, const void* keys) {
+  if (n < 0 || (uint64_t)n >= GQF_MAX_ITEMS || (n > 0 && !keys)) {
     set_error("invalid key array (n=%d)", n);
     return GQF_ERR_INVALID;
   }
+  return GQF_OK;
+}
+
+static gqf_status sqf_build_impl(const gqf_sqf& qf, int n, const uint32_t* d_keys, int no_dup, gqf_stream stream,
+                                 float* ms) {
+  GQF_TRY(sqf_check(qf));
+  GQF_TRY(check_keys(n, d_keys));
   std::lock_guard<std::mutex> lk(api_mutex());
   const SqfGeom g = geom(qf);
   CallScope cs;
   GQF_TRY(cs.begin(stream));
   DeviceState& ds = *cs.ds;
   const uint32_t un = (uint32_t)n;
-  GQF_TRY(ws_reserve(ds, sort_workspace_bytes(un, 4, false) + layout_workspace_bytes(un, 4)));
+  GQF_TRY(ws_reserve(ds, sort_workspace_bytes(un, 4, false) + rebuild_ws_bytes(un)));
   Arena ar(ds.ws, ds.ws_cap);
-  reset_op_counters(cs.stream, ds);
   uint32_t* sorted = nullptr;
   GQF_TRY(radix_sort<uint32_t>(cs.stream, ds, ar, d_keys, true, g.q + g.r, nullptr, false, un, 0, g.q + g.r, &sorted,
                                nullptr));
-  uint32_t *ufp, *loc;
-  if (no_dup)
-    GQF_TRY((layout_scan<uint32_t, KEEP_UNIQUE>(cs.stream, ds, ar, sorted, nullptr, un, g.q, g.r, 0, g.S, false, &ufp,
-                                                &loc)));
-  else
-    GQF_TRY((layout_scan<uint32_t, KEEP_ALL>(cs.stream, ds, ar, sorted, nullptr, un, g.q, g.r, 0, g.S, false, &ufp,
-                                             &loc)));
-  launch_write<uint32_t>(cs.stream, ds, qf, g, ufp, loc);
-  return cs.end(ms, true);
+  if (no_dup) GQF_TRY((launch_rebuild<KEEP_UNIQUE, false>(cs.stream, ds, ar, qf, g, sorted, un, nullptr, 0, nullptr)));
+  else GQF_TRY((launch_rebuild<KEEP_ALL, false>(cs.stream, ds, ar, qf, g, sorted, un, nullptr, 0, nullptr)));
+  return cs.end(ms, true);  // a layout that does not fit is detected before any write
 }
 
-// Shared tail of insert / insertViaMerge / merge: stored entries A (m of
-// them), new sorted fingerprints B (k), merged with source tags, then the rule.
-static gqf_status sqf_merge_rebuild(hipStream_t s, DeviceState& ds, Arena& ar, const gqf_sqf& qf, const SqfGeom& g,
-                                    const uint32_t* A, uint32_t m, const uint32_t* B, uint32_t k, int mode) {
-  uint64_t* merged = ar.take<uint64_t>((size_t)m + k + 1);
-  if (ar.used > ar.cap) {
-    set_error("internal: merge workspace overflow");
-    return GQF_ERR_INVALID;
-  }
-  launch_merge_tagged<uint32_t>(s, A, m, B, k, merged);
-  uint64_t* ufp;
-  uint32_t* loc;
-  const uint32_t total = m + k;
-  if (mode == KEEP_TAGGED_INSERT)
-    GQF_TRY((layout_scan<uint64_t, KEEP_TAGGED_INSERT>(s, ds, ar, merged, nullptr, total, g.q, g.r, 1, g.S, false, &ufp,
-                                                       &loc)));
-  else if (mode == KEEP_UNIQUE)
-    GQF_TRY((layout_scan<uint64_t, KEEP_UNIQUE>(s, ds, ar, merged, nullptr, total, g.q, g.r, 1, g.S, false, &ufp, &loc)));
-  else
-    GQF_TRY((layout_scan<uint64_t, KEEP_ALL>(s, ds, ar, merged, nullptr, total, g.q, g.r, 1, g.S, false, &ufp, &loc)));
-  launch_write<uint64_t>(s, ds, qf, g, ufp, loc);
-  return GQF_OK;
+// insert / insertViaMerge / merge: stored entries A merged with sorted B.
+template <int MODE>
+static gqf_status sqf_merge_into(CallScope& cs, Arena& ar, const gqf_sqf& qf, const SqfGeom& g, const uint32_t* A,
+                                 uint32_t m, const uint32_t* B, uint32_t k, float* ms) {
+  GQF_TRY((launch_rebuild<MODE, true>(cs.stream, *cs.ds, ar, qf, g, A, m, B, k, nullptr)));
+  return cs.end(ms, true);  // a layout that does not fit is detected before any write
 }
 
 static gqf_status sqf_insert_impl(const gqf_sqf& qf, int n, const uint32_t* d_keys, int mode, gqf_stream stream,
                                   float* ms) {
   GQF_TRY(sqf_check(qf));
-  if (n < 0 || (uint64_t)n >= GQF_MAX_ITEMS || (n > 0 && !d_keys)) {
-    set_error("invalid key array (n=%d)", n);
-    return GQF_ERR_INVALID;
-  }
+  GQF_TRY(check_keys(n, d_keys));
   std::lock_guard<std::mutex> lk(api_mutex());
   const SqfGeom g = geom(qf);
   CallScope cs;
@@ -443,40 +952,34 @@ static gqf_status sqf_insert_impl(const gqf_sqf& qf, int n, const uint32_t* d_ke
   DeviceState& ds = *cs.ds;
   const uint32_t un = (uint32_t)n;
   GQF_TRY(ws_reserve(ds, sqf_decode_bytes(g) + sort_workspace_bytes(un, 4, false) +
-                             aligned_bytes(((size_t)g.S + un + 1) * 8) + layout_workspace_bytes((size_t)g.S + un, 8)));
+                             rebuild_ws_bytes((uint64_t)g.S + un)));
   Arena ar(ds.ws, ds.ws_cap);
-  reset_op_counters(cs.stream, ds);
   uint32_t* A;
-  GQF_TRY(sqf_decode(cs.stream, ds, ar, qf, g, &A, ds.counters + CTR_MISC));
   uint32_t m = 0;
-  GQF_TRY(read_counter(cs.stream, ds, ds.counters + CTR_MISC, &m));
+  GQF_TRY(sqf_decode(cs.stream, ds, ar, qf, g, &A, &m));
   uint32_t* B = nullptr;
   GQF_TRY(radix_sort<uint32_t>(cs.stream, ds, ar, d_keys, true, g.q + g.r, nullptr, false, un, 0, g.q + g.r, &B,
                                nullptr));
-  GQF_TRY(sqf_merge_rebuild(cs.stream, ds, ar, qf, g, A, m, B, un, mode));
-  return cs.end(ms, true);
+  if (mode == KEEP_TAGGED_INSERT) return sqf_merge_into<KEEP_TAGGED_INSERT>(cs, ar, qf, g, A, m, B, un, ms);
+  if (mode == KEEP_UNIQUE) return sqf_merge_into<KEEP_UNIQUE>(cs, ar, qf, g, A, m, B, un, ms);
+  return sqf_merge_into<KEEP_ALL>(cs, ar, qf, g, A, m, B, un, ms);
 }
 
 static gqf_status sqf_delete_impl(const gqf_sqf& qf, int n, const uint32_t* d_keys, gqf_stream stream, float* ms) {
   GQF_TRY(sqf_check(qf));
-  if (n < 0 || (uint64_t)n >= GQF_MAX_ITEMS || (n > 0 && !d_keys)) {
-    set_error("invalid key array (n=%d)", n);
-    return GQF_ERR_INVALID;
-  }
+  GQF_TRY(check_keys(n, d_keys));
   std::lock_guard<std::mutex> lk(api_mutex());
   const SqfGeom g = geom(qf);
   CallScope cs;
   GQF_TRY(cs.begin(stream));
   DeviceState& ds = *cs.ds;
   const uint32_t un = (uint32_t)n;
-  GQF_TRY(ws_reserve(ds, sqf_decode_bytes(g) + sort_workspace_bytes(un, 4, false) + aligned_bytes(g.S) +
-                             layout_workspace_bytes(g.S, 4)));
+  GQF_TRY(ws_reserve(ds, sqf_decode_bytes(g) + sort_workspace_bytes(un, 4, false) + aligned_bytes(g.S + 1) +
+                             rebuild_ws_bytes(g.S)));
   Arena ar(ds.ws, ds.ws_cap);
-  reset_op_counters(cs.stream, ds);
   uint32_t* A;
-  GQF_TRY(sqf_decode(cs.stream, ds, ar, qf, g, &A, ds.counters + CTR_MISC));
   uint32_t m = 0;
-  GQF_TRY(read_counter(cs.stream, ds, ds.counters + CTR_MISC, &m));
+  GQF_TRY(sqf_decode(cs.stream, ds, ar, qf, g, &A, &m));
   uint32_t* D = nullptr;
   GQF_TRY(radix_sort<uint32_t>(cs.stream, ds, ar, d_keys, true, g.q + g.r, nullptr, false, un, 0, g.q + g.r, &D,
                                nullptr));
@@ -485,11 +988,9 @@ static gqf_status sqf_delete_impl(const gqf_sqf& qf, int n, const uint32_t* d_ke
     set_error("internal: delete workspace overflow");
     return GQF_ERR_INVALID;
   }
-  if (m) hipLaunchKernelGGL(k_delete_mark<uint32_t>, dim3((m + 255) / 256), dim3(256), 0, cs.stream, A, m, D, un, keep);
-  uint32_t *ufp, *loc;
-  GQF_TRY((layout_scan<uint32_t, KEEP_FLAGS>(cs.stream, ds, ar, A, keep, m, g.q, g.r, 0, g.S, false, &ufp, &loc)));
-  launch_write<uint32_t>(cs.stream, ds, qf, g, ufp, loc);
-  return cs.end(ms, true);
+  if (m) GQF_LAUNCH_B((double)m * 5, k_delete_mark<uint32_t>, dim3((m + 255) / 256), dim3(256), 0, cs.stream, A, m, D, un, keep);
+  GQF_TRY((launch_rebuild<KEEP_FLAGS, false>(cs.stream, ds, ar, qf, g, A, m, nullptr, 0, keep)));
+  return cs.end(ms, true);  // deleting only shrinks the layout: it always fits
 }
 
 }  // namespace gqf
@@ -566,10 +1067,10 @@ gqf_status gqf_sqf_lookup(gqf_sqf qf, int sorted, int n, const uint32_t* d_keys,
   if (un) {
     if (!sorted) {
       const dim3 grid((un + 255) / 256);
-      if (g.bpe == 1) hipLaunchKernelGGL(k_sqf_lookup<1>, grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, d_keys, un, d_out);
-      else if (g.bpe == 2) hipLaunchKernelGGL(k_sqf_lookup<2>, grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, d_keys, un, d_out);
-      else if (g.bpe == 3) hipLaunchKernelGGL(k_sqf_lookup<3>, grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, d_keys, un, d_out);
-      else hipLaunchKernelGGL(k_sqf_lookup<4>, grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, d_keys, un, d_out);
+      if (g.bpe == 1) GQF_LAUNCH_B((double)un * (8 + 64), k_sqf_lookup<1>, grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, d_keys, un, d_out);
+      else if (g.bpe == 2) GQF_LAUNCH_B((double)un * (8 + 64), k_sqf_lookup<2>, grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, d_keys, un, d_out);
+      else if (g.bpe == 3) GQF_LAUNCH_B((double)un * (8 + 64), k_sqf_lookup<3>, grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, d_keys, un, d_out);
+      else GQF_LAUNCH_B((double)un * (8 + 64), k_sqf_lookup<4>, grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, d_keys, un, d_out);
     } else {
       GQF_TRY(ws_reserve(ds, sort_workspace_bytes(un, 4, true)));
       Arena ar(ds.ws, ds.ws_cap);
@@ -579,10 +1080,10 @@ gqf_status gqf_sqf_lookup(gqf_sqf qf, int sorted, int n, const uint32_t* d_keys,
       const uint32_t lo = bits > 16 ? bits - 16 : 0;
       GQF_TRY(radix_sort<uint32_t>(cs.stream, ds, ar, d_keys, true, bits, nullptr, true, un, lo, bits, &fps, &idx));
       const dim3 grid((un + 255) / 256);
-      if (g.bpe == 1) hipLaunchKernelGGL(k_sqf_lookup_sorted<1>, grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, fps, idx, un, d_out);
-      else if (g.bpe == 2) hipLaunchKernelGGL(k_sqf_lookup_sorted<2>, grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, fps, idx, un, d_out);
-      else if (g.bpe == 3) hipLaunchKernelGGL(k_sqf_lookup_sorted<3>, grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, fps, idx, un, d_out);
-      else hipLaunchKernelGGL(k_sqf_lookup_sorted<4>, grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, fps, idx, un, d_out);
+      if (g.bpe == 1) GQF_LAUNCH_B((double)un * (12 + 64), k_sqf_lookup_sorted<1>, grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, fps, idx, un, d_out);
+      else if (g.bpe == 2) GQF_LAUNCH_B((double)un * (12 + 64), k_sqf_lookup_sorted<2>, grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, fps, idx, un, d_out);
+      else if (g.bpe == 3) GQF_LAUNCH_B((double)un * (12 + 64), k_sqf_lookup_sorted<3>, grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, fps, idx, un, d_out);
+      else GQF_LAUNCH_B((double)un * (12 + 64), k_sqf_lookup_sorted<4>, grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, fps, idx, un, d_out);
     }
   }
   return cs.end(ms, false);
@@ -600,18 +1101,14 @@ gqf_status gqf_sqf_merge(gqf_sqf dst, gqf_sqf src, int no_dup, gqf_stream stream
   CallScope cs;
   GQF_TRY(cs.begin(stream));
   DeviceState& ds = *cs.ds;
-  GQF_TRY(ws_reserve(ds, 2 * sqf_decode_bytes(g) + aligned_bytes((2 * (size_t)g.S + 1) * 8) +
-                             layout_workspace_bytes(2 * (size_t)g.S, 8)));
+  GQF_TRY(ws_reserve(ds, 2 * sqf_decode_bytes(g) + rebuild_ws_bytes(2 * (uint64_t)g.S)));
   Arena ar(ds.ws, ds.ws_cap);
-  reset_op_counters(cs.stream, ds);
   uint32_t *A, *B;
   uint32_t m1 = 0, m2 = 0;
-  GQF_TRY(sqf_decode(cs.stream, ds, ar, dst, g, &A, ds.counters + CTR_MISC));
-  GQF_TRY(sqf_decode(cs.stream, ds, ar, src, g, &B, ds.counters + CTR_MISC + 1));
-  GQF_TRY(read_counter(cs.stream, ds, ds.counters + CTR_MISC, &m1));
-  GQF_TRY(read_counter(cs.stream, ds, ds.counters + CTR_MISC + 1, &m2));
-  GQF_TRY(sqf_merge_rebuild(cs.stream, ds, ar, dst, g, A, m1, B, m2, no_dup ? KEEP_UNIQUE : KEEP_ALL));
-  return cs.end(ms, true);
+  GQF_TRY(sqf_decode(cs.stream, ds, ar, dst, g, &A, &m1));
+  GQF_TRY(sqf_decode(cs.stream, ds, ar, src, g, &B, &m2));
+  if (no_dup) return sqf_merge_into<KEEP_UNIQUE>(cs, ar, dst, g, A, m1, B, m2, ms);
+  return sqf_merge_into<KEEP_ALL>(cs, ar, dst, g, A, m1, B, m2, ms);
 }
 
 }  // extern "C"

@@ -179,6 +179,24 @@ gqf_status gqf_memcpy_dtod(void* d_dst, const void* d_src, size_t bytes);
 gqf_status gqf_memset(void* d_ptr, int value, size_t bytes);
 gqf_status gqf_synchronize(void);
 
+/* ============================================================ kernel timing
+ * Per-kernel device time measured with HIP events on the launch stream (two
+ * events around every kernel launch while enabled; results accumulate per
+ * kernel name when each call completes).  No reference counterpart: the
+ * reference times whole calls only (e.g. quotientFilter.cu:279-310). */
+gqf_status gqf_profile_enable(int on);
+gqf_status gqf_profile_reset(void);
+int gqf_profile_count(void);
+/* total_bytes: algorithmic bytes (minimum HBM traffic of the launches' work,
+ * as each launch site declares it); 0 for launches that declare none. */
+gqf_status gqf_profile_get(int i, char* name, int name_cap, uint64_t* launches, double* total_ms,
+                           double* total_bytes);
+
+/* Host-side key stream of the reference drivers: init_genrand(seed) +
+ * genrand_int32() (mt19937ar.cpp:57-70), i.e. std::mt19937, after skipping
+ * `skip` outputs. */
+void gqf_mt19937_fill(uint32_t seed, uint64_t skip, size_t n, uint32_t* host_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -37,7 +37,7 @@ _SIGS = {
     "orc_sqf_table_bytes": (_sz, [_u, _u]),
     "orc_sqf_cpu_words": (_sz, [_u, _u]),
     "orc_sqf_decode_bytes": (None, [_u8p, _u, C.c_uint32, _u32p]),
-    "orc_sqf_decode_words": (None, [_u32p, _u, C.c_uint32, _u32p]),
+    "orc_sqf_decode_words": (None, [_u32p, _sz, _u, C.c_uint32, _u32p]),
     "orc_sqf_canon": (_i, [_u64p, _sz, _u, _u, _i, _u8p]),
     "orc_sqf_lookup_fp": (C.c_uint32, [_u8p, _u, _u, C.c_uint64]),
     "orc_sqf_insert_fp": (C.c_uint32, [_u8p, _u, _u, C.c_uint64, C.POINTER(C.c_int)]),
@@ -143,7 +143,8 @@ def sqf_decode(table, q, r):
 def sqf_decode_words(words, q, r, nslots=None):
     s = sqf_num_slots(q) if nslots is None else nslots
     out = np.empty(s, dtype=np.uint32)
-    lib().orc_sqf_decode_words(_u32(words), r, s, out)
+    w = _u32(words)
+    lib().orc_sqf_decode_words(w, len(w), r, s, out)
     return out
 
 

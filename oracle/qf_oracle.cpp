@@ -216,15 +216,17 @@ void orc_sqf_decode_bytes(const uint8_t* table, unsigned r, uint32_t nslots, uin
 }
 
 // getElement of the CPU_version (quotientFilter.cpp:114-130): (r+3)-bit slots
-// packed MSB-first into uint32 words.
-void orc_sqf_decode_words(const uint32_t* words, unsigned r, uint32_t nslots, uint32_t* elems) {
+// packed MSB-first into uint32 words.  The CPU table can hold fewer than S
+// slots (calcNumSlots rounds differently); bits past nwords read as 0.
+void orc_sqf_decode_words(const uint32_t* words, size_t nwords, unsigned r, uint32_t nslots,
+                          uint32_t* elems) {
   unsigned w = r + 3;
   for (uint32_t i = 0; i < nslots; ++i) {
     uint64_t bit = (uint64_t)i * w;
     uint32_t v = 0;
     for (unsigned k = 0; k < w; ++k) {
       uint64_t p = bit + k;
-      uint32_t word = words[p / 32];
+      uint32_t word = p / 32 < nwords ? words[p / 32] : 0u;
       v = (v << 1) | ((word >> (31 - (p % 32))) & 1u);
     }
     elems[i] = v;

@@ -54,7 +54,8 @@ size_t orc_sqf_cpu_words(unsigned q, unsigned r);
 uint32_t orc_sqf_get(const uint8_t* table, unsigned r, uint32_t slot);
 void orc_sqf_set(uint8_t* table, unsigned r, uint32_t slot, uint32_t elem);
 void orc_sqf_decode_bytes(const uint8_t* table, unsigned r, uint32_t nslots, uint32_t* elems);
-void orc_sqf_decode_words(const uint32_t* words, unsigned r, uint32_t nslots, uint32_t* elems);
+void orc_sqf_decode_words(const uint32_t* words, size_t nwords, unsigned r, uint32_t nslots,
+                          uint32_t* elems);
 
 /* ---------------- SQF canonical layout ---------------- */
 /* fps: any order; sorts a private copy.  Returns ORC_OK or ORC_ERR_CAPACITY.

@@ -176,7 +176,7 @@ def test_edge_cases():
     assert (qf.table_bytes() == before).all()
 
 
-def test_capacity_error_leaves_table_unchanged():
+def test_capacity_error_handling():
     q, r = 8, 5
     S = O.sqf_num_slots(q)
     keys = O.mt19937(1, 200)
@@ -188,6 +188,10 @@ def test_capacity_error_leaves_table_unchanged():
     with pytest.raises(G.CapacityError):
         qf.insert(dev(too_many))
     assert (qf.table_bytes() == before).all()
+    qf.delete(dev(keys[:50]))  # still usable after the error
+    before = qf.table_bytes()
+    assert (before == sqf_canon(keys[50:], q, r, True)).all()
+    # a build that does not fit is rejected before any write
     with pytest.raises(G.CapacityError):
         qf.bulk_build(dev(too_many), NoDuplicates=False)
     assert (qf.table_bytes() == before).all()
