@@ -727,51 +727,130 @@ inline void launch_merge_tagged(hipStream_t s, const K* A, uint32_t na, const K*
 // ====================================================================== delete marking
 // keep[i] = 0 for the first c_D(v) stored copies of each value v of A
 // (c_D = its multiplicity among the sorted delete fingerprints D).
+// Delete marking, multiset difference A - D of two sorted arrays: entry i of A
+// is removed iff its rank among the equal entries of A is below the count of
+// its value in D (the first cD copies go).  keep[i] = 1 for survivors.
+//
+// One workgroup per tile of DM_TILE entries of A; the slice of D that can
+// match the tile ([lower_bound(first), upper_bound(last)) ) is staged in LDS
+// and every thread merges its DM_ITEMS consecutive entries against it.  A
+// slice larger than DM_DCAP falls back to per-entry global searches.
+constexpr int DM_THREADS = 256;
+constexpr int DM_ITEMS = 16;
+constexpr int DM_TILE = DM_THREADS * DM_ITEMS;
+constexpr uint32_t DM_DCAP = 4096;
+
 template <typename K>
-__global__ __launch_bounds__(256) void k_delete_mark(const K* __restrict__ A, uint32_t na, const K* __restrict__ D,
-                                                     uint32_t nd, uint8_t* __restrict__ keep) {
-  const uint64_t i = blockIdx.x * 256ull + threadIdx.x;
-  if (i >= na) return;
-  const K v = A[i];
-  uint32_t lo = 0, hi = nd;
+__device__ __forceinline__ uint32_t lower_bound_g(const K* __restrict__ a, uint32_t lo, uint32_t hi, K v) {
   while (lo < hi) {
     const uint32_t mid = (lo + hi) >> 1;
-    if (D[mid] < v) lo = mid + 1;
+    if (a[mid] < v) lo = mid + 1;
     else hi = mid;
   }
-  uint32_t ub_lo = lo, ub_hi = nd;
-  while (ub_lo < ub_hi) {
-    const uint32_t mid = (ub_lo + ub_hi) >> 1;
-    if (D[mid] <= v) ub_lo = mid + 1;
-    else ub_hi = mid;
+  return lo;
+}
+template <typename K>
+__device__ __forceinline__ uint32_t upper_bound_g(const K* __restrict__ a, uint32_t lo, uint32_t hi, K v) {
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (a[mid] <= v) lo = mid + 1;
+    else hi = mid;
   }
-  const uint32_t cd = ub_lo - lo;
-  if (cd == 0) {
-    keep[i] = 1;
-    return;
+  return lo;
+}
+
+// First index of the equal run of A that contains i (galloping search back).
+template <typename K>
+__device__ __forceinline__ uint32_t run_first(const K* __restrict__ A, uint32_t i) {
+  const K v = A[i];
+  if (i == 0 || A[i - 1] != v) return i;
+  uint32_t h = i, step = 1, l;
+  for (;;) {
+    l = h >= step ? h - step : 0u;
+    if (l == 0 || A[l] != v) break;
+    h = l;
+    step <<= 1;
   }
-  // rank of entry i among equal entries of A (galloping search backwards)
-  uint32_t first = (uint32_t)i;
-  if (i > 0 && A[i - 1] == v) {
-    uint32_t h = (uint32_t)i, step = 1, l;
-    for (;;) {
-      l = h >= step ? h - step : 0u;
-      if (l == 0 || A[l] != v) break;
-      h = l;
-      step <<= 1;
+  if (A[l] == v) return l;
+  while (h - l > 1) {
+    const uint32_t mid = (l + h) >> 1;
+    if (A[mid] == v) h = mid;
+    else l = mid;
+  }
+  return h;
+}
+
+template <typename K>
+__global__ __launch_bounds__(DM_THREADS) void k_delete_mark(const K* __restrict__ A, uint32_t na,
+                                                            const K* __restrict__ D, uint32_t nd,
+                                                            uint8_t* __restrict__ keep) {
+  __shared__ K s_d[DM_DCAP];
+  __shared__ uint32_t s_rng[2];
+  const int tid = threadIdx.x;
+  const uint32_t t0 = blockIdx.x * (uint32_t)DM_TILE;
+  const uint32_t t1 = t0 + DM_TILE < na ? t0 + DM_TILE : na;
+  if (tid == 0) s_rng[0] = lower_bound_g(D, 0u, nd, A[t0]);
+  if (tid == 64) s_rng[1] = upper_bound_g(D, 0u, nd, A[t1 - 1]);
+  // this thread's entries
+  const uint32_t i0 = t0 + (uint32_t)tid * DM_ITEMS;
+  K v[DM_ITEMS];
+  const bool full = i0 + DM_ITEMS <= t1;
+  if (full) {
+    // A is 256-byte aligned and i0 a multiple of DM_ITEMS: 16-byte aligned
+    uint4 w[DM_ITEMS * sizeof(K) / 16];
+    const uint4* p = reinterpret_cast<const uint4*>(A + i0);
+#pragma unroll
+    for (int k = 0; k < (int)(DM_ITEMS * sizeof(K) / 16); ++k) w[k] = p[k];
+    __builtin_memcpy(v, w, sizeof(v));
+  } else {
+#pragma unroll
+    for (int j = 0; j < DM_ITEMS; ++j) v[j] = i0 + j < t1 ? A[i0 + j] : K(0);
+  }
+  __syncthreads();
+  const uint32_t dlo = s_rng[0], dhi = s_rng[1];
+  const uint32_t dn = dhi - dlo;
+  const bool staged = dn <= DM_DCAP;
+  if (staged)
+    for (uint32_t k = tid; k < dn; k += DM_THREADS) s_d[k] = D[dlo + k];
+  __syncthreads();
+  if (i0 >= t1) return;
+  uint32_t out[DM_ITEMS / 4] = {0u, 0u, 0u, 0u};
+  // rank of the first entry among its equal run of A
+  uint32_t rank = i0 - run_first(A, i0);
+  uint32_t p = 0;  // merge position in the D slice
+  if (staged) {
+    uint32_t lo = 0, hi = dn;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (s_d[mid] < v[0]) lo = mid + 1;
+      else hi = mid;
     }
-    if (A[l] == v) {
-      first = l;
-    } else {
-      while (h - l > 1) {
-        const uint32_t mid = (l + h) >> 1;
-        if (A[mid] == v) h = mid;
-        else l = mid;
+    p = lo;
+  }
+#pragma unroll
+  for (int j = 0; j < DM_ITEMS; ++j) {
+    if (i0 + j < t1) {
+      if (j > 0) rank = v[j] == v[j - 1] ? rank + 1 : 0u;
+      uint32_t cd;
+      if (staged) {
+        while (p < dn && s_d[p] < v[j]) ++p;
+        uint32_t e = p;
+        while (e < dn && s_d[e] == v[j]) ++e;
+        cd = e - p;
+      } else {
+        const uint32_t lo = lower_bound_g(D, dlo, dhi, v[j]);
+        cd = upper_bound_g(D, lo, dhi, v[j]) - lo;
       }
-      first = h;
+      if (rank >= cd) out[j >> 2] |= 1u << (8 * (j & 3));
     }
   }
-  keep[i] = ((uint32_t)i - first) >= cd ? 1 : 0;
+  if (full) {
+    *reinterpret_cast<uint4*>(keep + i0) = make_uint4(out[0], out[1], out[2], out[3]);
+  } else {
+#pragma unroll
+    for (int j = 0; j < DM_ITEMS; ++j)
+      if (i0 + j < t1) keep[i0 + j] = (uint8_t)((out[j >> 2] >> (8 * (j & 3))) & 1u);
+  }
 }
 
 // ====================================================================== region scan

@@ -421,7 +421,7 @@ static gqf_status rsqf_delete_t(const gqf_rsqf& f, uint32_t n, const uint32_t* k
     set_error("internal: delete workspace overflow");
     return GQF_ERR_INVALID;
   }
-  if (m) GQF_LAUNCH_B((double)m * (sizeof(K) + 1), k_delete_mark<K>, dim3((m + 255) / 256), dim3(256), 0, cs.stream, A, m, D, n, keep);
+  if (m) GQF_LAUNCH_B((double)m * (sizeof(K) + 1), k_delete_mark<K>, dim3((m + DM_TILE - 1) / DM_TILE), dim3(DM_THREADS), 0, cs.stream, A, m, D, n, keep);
   K* ufp;
   uint32_t* loc;
   GQF_TRY((layout_scan<K, KEEP_FLAGS>(cs.stream, ds, ar, A, keep, m, g.q, g.r, 0, g.cap, true, &ufp, &loc)));

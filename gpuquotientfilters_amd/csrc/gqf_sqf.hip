@@ -41,7 +41,7 @@ static gqf_status sqf_check(const gqf_sqf& qf) {
 
 // ---------------------------------------------------------------- slot access
 template <int BPE>
-__device__ __forceinline__ uint32_t sqf_get(const uint8_t* __restrict__ t, uint32_t S, uint32_t i) {
+__host__ __device__ __forceinline__ uint32_t sqf_get(const uint8_t* __restrict__ t, uint32_t S, uint32_t i) {
   if (i >= S) return 0u;  // past the last slot reads as empty
   if (BPE == 1) return t[i];
   const uint8_t* p = t + (size_t)i * BPE;
@@ -54,7 +54,7 @@ __device__ __forceinline__ uint32_t sqf_get(const uint8_t* __restrict__ t, uint3
 // findRunStartGPU (quotientFilter.cu:172-196) + run scan of hashAndLookUp
 // (quotientFilter.cu:333-356): slot of the first matching remainder.
 template <int BPE>
-__device__ __forceinline__ uint32_t sqf_lookup_one(const uint8_t* __restrict__ t, uint32_t S, uint32_t fq, uint32_t fr) {
+__host__ __device__ __forceinline__ uint32_t sqf_lookup_one(const uint8_t* __restrict__ t, uint32_t S, uint32_t fq, uint32_t fr) {
   if (!(sqf_get<BPE>(t, S, fq) & 4u)) return SQF_NOT_FOUND;
   uint32_t b = fq;
   while (b > 0 && (sqf_get<BPE>(t, S, b) & 1u)) b--;
@@ -78,19 +78,183 @@ __device__ __forceinline__ uint32_t sqf_lookup_one(const uint8_t* __restrict__ t
   return SQF_NOT_FOUND;
 }
 
+// The same walk over 64-slot windows (BPE 1 and 2): a window is loaded with
+// 16-byte loads (64 or 128 table bytes, 64-slot aligned) and turned into
+// occupied / continuation / shifted bit masks, so that the cluster start, the
+// k-th run start and the first remainder >= fr are found with popcount and
+// count-trailing-zeros; the walk moves one window back or forward only when
+// it crosses a window edge.  Slots at or past S read as empty, as above.
 template <int BPE>
+struct SqfWin {
+  uint32_t raw[16 * BPE];
+  uint64_t occ, cont, shf;
+};
+
+__host__ __device__ __forceinline__ uint32_t byte_bits4(uint32_t x) {
+  // bit 0 of each byte of x (other bits zero) -> bits 0..3
+  return (x * 0x01020408u) >> 24;
+}
+
+template <int BPE>
+__host__ __device__ __forceinline__ void sqf_win_load(const uint8_t* __restrict__ t, uint32_t S, uint32_t w, SqfWin<BPE>& W) {
+  const uint64_t s0 = (uint64_t)w * 64;
+  if (s0 + 64 <= S) {
+    const uint4* p = reinterpret_cast<const uint4*>(t + s0 * BPE);
+#pragma unroll
+    for (int k = 0; k < 4 * BPE; ++k) {
+      const uint4 x = p[k];
+      W.raw[4 * k] = x.x;
+      W.raw[4 * k + 1] = x.y;
+      W.raw[4 * k + 2] = x.z;
+      W.raw[4 * k + 3] = x.w;
+    }
+  } else {
+    const uint64_t end = (uint64_t)S * BPE;
+#pragma unroll
+    for (int k = 0; k < 16 * BPE; ++k) {
+      uint32_t v = 0;
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const uint64_t at = s0 * BPE + 4 * k + b;
+        if (at < end) v |= (uint32_t)t[at] << (8 * b);
+      }
+      W.raw[k] = v;
+    }
+  }
+  uint64_t occ = 0, cont = 0, shf = 0;
+  if (BPE == 1) {
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const uint32_t x = W.raw[k];
+      occ |= (uint64_t)byte_bits4((x >> 2) & 0x01010101u) << (4 * k);
+      cont |= (uint64_t)byte_bits4((x >> 1) & 0x01010101u) << (4 * k);
+      shf |= (uint64_t)byte_bits4(x & 0x01010101u) << (4 * k);
+    }
+  } else {
+    // slot j = bytes 2j (high) and 2j+1 (low, holding the flags)
+#pragma unroll
+    for (int k = 0; k < 32; ++k) {
+      const uint32_t x = W.raw[k];
+      const uint32_t f = ((x >> 8) & 7u) | ((x >> 21) & 0x38u);  // flags of slot 2k in bits 0..2, 2k+1 in 3..5
+      occ |= (uint64_t)(((f >> 2) & 1u) | ((f >> 4) & 2u)) << (2 * k);
+      cont |= (uint64_t)(((f >> 1) & 1u) | ((f >> 3) & 2u)) << (2 * k);
+      shf |= (uint64_t)((f & 1u) | ((f >> 2) & 2u)) << (2 * k);
+    }
+  }
+  W.occ = occ;
+  W.cont = cont;
+  W.shf = shf;
+}
+
+// Bit j set iff remainder(slot j of the window) >= x  (x <= 2^r).
+template <int BPE>
+__host__ __device__ __forceinline__ uint64_t sqf_win_ge(const SqfWin<BPE>& W, uint32_t x) {
+  uint64_t m = 0;
+  if (BPE == 1) {
+    // per byte: (rem | 0x80) - x keeps bit 7 iff rem >= x (rem, x < 128: no borrow between bytes)
+    const uint32_t xr = x * 0x01010101u;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const uint32_t a = (W.raw[k] >> 3) & 0x1F1F1F1Fu;
+      const uint32_t g = ((a | 0x80808080u) - xr) & 0x80808080u;
+      m |= (uint64_t)byte_bits4(g >> 7) << (4 * k);
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 32; ++k) {
+      const uint32_t x0 = W.raw[k];
+      const uint32_t r0 = (((x0 & 0xFFu) << 8) | ((x0 >> 8) & 0xFFu)) >> 3;
+      const uint32_t r1 = ((((x0 >> 16) & 0xFFu) << 8) | (x0 >> 24)) >> 3;
+      m |= (uint64_t)((r0 >= x ? 1u : 0u) | (r1 >= x ? 2u : 0u)) << (2 * k);
+    }
+  }
+  return m;
+}
+
+template <int BPE>
+__host__ __device__ __forceinline__ uint32_t sqf_lookup_win(const uint8_t* __restrict__ t, uint32_t S, uint32_t fq,
+                                                   uint32_t fr) {
+  SqfWin<BPE> W;
+  uint32_t cw = fq >> 6;
+  const uint32_t rel = fq & 63u;
+  sqf_win_load<BPE>(t, S, cw, W);
+  if (!((W.occ >> rel) & 1ull)) return SQF_NOT_FOUND;
+  // cluster start b: the last non-shifted slot at or before fq (or slot 0);
+  // k = occupied slots in (b, fq] = run starts to skip after b
+  uint64_t lim = rel == 63 ? ~0ull : ((2ull << rel) - 1ull);  // bits [0, rel]
+  uint32_t k = 0, brel = 0;
+  for (;;) {
+    const uint64_t ns = ~W.shf & lim;
+    if (ns) {
+      brel = 63u - (uint32_t)__builtin_clzll(ns);
+      k += (uint32_t)__builtin_popcountll(W.occ & lim & ~((2ull << brel) - 1ull));
+      break;
+    }
+    k += (uint32_t)__builtin_popcountll(W.occ & lim);
+    if (cw == 0) {
+      brel = 0;
+      k -= (uint32_t)(W.occ & 1ull);
+      break;
+    }
+    --cw;
+    sqf_win_load<BPE>(t, S, cw, W);
+    lim = ~0ull;
+  }
+  // s: the k-th slot after b without a continuation bit
+  uint32_t srel = brel;
+  uint64_t from = brel == 63 ? 0ull : (~0ull << (brel + 1));
+  while (k > 0) {
+    uint64_t rs = ~W.cont & from;
+    const uint32_t c = (uint32_t)__builtin_popcountll(rs);
+    if (c >= k) {
+      for (uint32_t i = 1; i < k; ++i) rs &= rs - 1ull;
+      srel = (uint32_t)__builtin_ctzll(rs);
+      break;
+    }
+    k -= c;
+    ++cw;
+    sqf_win_load<BPE>(t, S, cw, W);
+    from = ~0ull;
+  }
+  // the run: s and the continuation slots after it; first remainder >= fr
+  uint64_t run_from = 1ull << srel;
+  uint64_t after = srel == 63 ? 0ull : (~0ull << (srel + 1));
+  for (;;) {
+    const uint64_t stop = ~W.cont & after;
+    const uint64_t run = (run_from | after) & (stop ? ((1ull << __builtin_ctzll(stop)) - 1ull) : ~0ull);
+    const uint64_t ge = sqf_win_ge<BPE>(W, fr) & run;
+    if (ge) {
+      const uint32_t p = (uint32_t)__builtin_ctzll(ge);
+      return ((sqf_win_ge<BPE>(W, fr + 1) >> p) & 1ull) ? SQF_NOT_FOUND : cw * 64u + p;
+    }
+    if (stop) return SQF_NOT_FOUND;
+    ++cw;
+    sqf_win_load<BPE>(t, S, cw, W);
+    run_from = 0;
+    after = ~0ull;
+  }
+}
+
+template <int BPE, bool WIN>
+__device__ __forceinline__ uint32_t sqf_lookup_fp(const uint8_t* __restrict__ t, uint32_t S, uint32_t fq,
+                                                  uint32_t fr) {
+  if (WIN) return sqf_lookup_win<BPE>(t, S, fq, fr);
+  return sqf_lookup_one<BPE>(t, S, fq, fr);
+}
+
+template <int BPE, bool WIN>
 __global__ __launch_bounds__(256) void k_sqf_lookup(const uint8_t* __restrict__ t, uint32_t S, uint32_t q, uint32_t r,
                                                     const uint32_t* __restrict__ keys, uint32_t n,
                                                     uint32_t* __restrict__ out) {
   const uint64_t i = blockIdx.x * 256ull + threadIdx.x;
   if (i >= n) return;
   const uint32_t fp = (uint32_t)fingerprint(keys[i], q + r);
-  out[i] = sqf_lookup_one<BPE>(t, S, (fp >> r) & (uint32_t)mask64(q), fp & ((1u << r) - 1u));
+  out[i] = sqf_lookup_fp<BPE, WIN>(t, S, (fp >> r) & (uint32_t)mask64(q), fp & ((1u << r) - 1u));
 }
 
 // Lookups in fingerprint order (sorted path): result scattered to the
 // original index.
-template <int BPE>
+template <int BPE, bool WIN>
 __global__ __launch_bounds__(256) void k_sqf_lookup_sorted(const uint8_t* __restrict__ t, uint32_t S, uint32_t q,
                                                            uint32_t r, const uint32_t* __restrict__ fps,
                                                            const uint32_t* __restrict__ idx, uint32_t n,
@@ -98,7 +262,7 @@ __global__ __launch_bounds__(256) void k_sqf_lookup_sorted(const uint8_t* __rest
   const uint64_t i = blockIdx.x * 256ull + threadIdx.x;
   if (i >= n) return;
   const uint32_t fp = fps[i];
-  out[idx[i]] = sqf_lookup_one<BPE>(t, S, (fp >> r) & (uint32_t)mask64(q), fp & ((1u << r) - 1u));
+  out[idx[i]] = sqf_lookup_fp<BPE, WIN>(t, S, (fp >> r) & (uint32_t)mask64(q), fp & ((1u << r) - 1u));
 }
 
 // ---------------------------------------------------------------- fused rebuild
@@ -988,7 +1152,7 @@ static gqf_status sqf_delete_impl(const gqf_sqf& qf, int n, const uint32_t* d_ke
     set_error("internal: delete workspace overflow");
     return GQF_ERR_INVALID;
   }
-  if (m) GQF_LAUNCH_B((double)m * 5, k_delete_mark<uint32_t>, dim3((m + 255) / 256), dim3(256), 0, cs.stream, A, m, D, un, keep);
+  if (m) GQF_LAUNCH_B((double)m * 5, k_delete_mark<uint32_t>, dim3((m + DM_TILE - 1) / DM_TILE), dim3(DM_THREADS), 0, cs.stream, A, m, D, un, keep);
   GQF_TRY((launch_rebuild<KEEP_FLAGS, false>(cs.stream, ds, ar, qf, g, A, m, nullptr, 0, keep)));
   return cs.end(ms, true);  // deleting only shrinks the layout: it always fits
 }
@@ -1065,12 +1229,17 @@ gqf_status gqf_sqf_lookup(gqf_sqf qf, int sorted, int n, const uint32_t* d_keys,
   DeviceState& ds = *cs.ds;
   const uint32_t un = (uint32_t)n;
   if (un) {
+    // 64-slot windows need 16-byte aligned tables (hipMalloc gives 256)
+    const bool win = (reinterpret_cast<uintptr_t>(qf.table) & 15u) == 0;
     if (!sorted) {
       const dim3 grid((un + 255) / 256);
-      if (g.bpe == 1) GQF_LAUNCH_B((double)un * (8 + 64), k_sqf_lookup<1>, grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, d_keys, un, d_out);
-      else if (g.bpe == 2) GQF_LAUNCH_B((double)un * (8 + 64), k_sqf_lookup<2>, grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, d_keys, un, d_out);
-      else if (g.bpe == 3) GQF_LAUNCH_B((double)un * (8 + 64), k_sqf_lookup<3>, grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, d_keys, un, d_out);
-      else GQF_LAUNCH_B((double)un * (8 + 64), k_sqf_lookup<4>, grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, d_keys, un, d_out);
+      const double b = (double)un * (8 + 64);
+      if (g.bpe == 1 && win) GQF_LAUNCH_B(b, (k_sqf_lookup<1, true>), grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, d_keys, un, d_out);
+      else if (g.bpe == 2 && win) GQF_LAUNCH_B(b, (k_sqf_lookup<2, true>), grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, d_keys, un, d_out);
+      else if (g.bpe == 1) GQF_LAUNCH_B(b, (k_sqf_lookup<1, false>), grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, d_keys, un, d_out);
+      else if (g.bpe == 2) GQF_LAUNCH_B(b, (k_sqf_lookup<2, false>), grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, d_keys, un, d_out);
+      else if (g.bpe == 3) GQF_LAUNCH_B(b, (k_sqf_lookup<3, false>), grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, d_keys, un, d_out);
+      else GQF_LAUNCH_B(b, (k_sqf_lookup<4, false>), grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, d_keys, un, d_out);
     } else {
       GQF_TRY(ws_reserve(ds, sort_workspace_bytes(un, 4, true)));
       Arena ar(ds.ws, ds.ws_cap);
@@ -1080,10 +1249,13 @@ gqf_status gqf_sqf_lookup(gqf_sqf qf, int sorted, int n, const uint32_t* d_keys,
       const uint32_t lo = bits > 16 ? bits - 16 : 0;
       GQF_TRY(radix_sort<uint32_t>(cs.stream, ds, ar, d_keys, true, bits, nullptr, true, un, lo, bits, &fps, &idx));
       const dim3 grid((un + 255) / 256);
-      if (g.bpe == 1) GQF_LAUNCH_B((double)un * (12 + 64), k_sqf_lookup_sorted<1>, grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, fps, idx, un, d_out);
-      else if (g.bpe == 2) GQF_LAUNCH_B((double)un * (12 + 64), k_sqf_lookup_sorted<2>, grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, fps, idx, un, d_out);
-      else if (g.bpe == 3) GQF_LAUNCH_B((double)un * (12 + 64), k_sqf_lookup_sorted<3>, grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, fps, idx, un, d_out);
-      else GQF_LAUNCH_B((double)un * (12 + 64), k_sqf_lookup_sorted<4>, grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, fps, idx, un, d_out);
+      const double b = (double)un * (12 + 64);
+      if (g.bpe == 1 && win) GQF_LAUNCH_B(b, (k_sqf_lookup_sorted<1, true>), grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, fps, idx, un, d_out);
+      else if (g.bpe == 2 && win) GQF_LAUNCH_B(b, (k_sqf_lookup_sorted<2, true>), grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, fps, idx, un, d_out);
+      else if (g.bpe == 1) GQF_LAUNCH_B(b, (k_sqf_lookup_sorted<1, false>), grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, fps, idx, un, d_out);
+      else if (g.bpe == 2) GQF_LAUNCH_B(b, (k_sqf_lookup_sorted<2, false>), grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, fps, idx, un, d_out);
+      else if (g.bpe == 3) GQF_LAUNCH_B(b, (k_sqf_lookup_sorted<3, false>), grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, fps, idx, un, d_out);
+      else GQF_LAUNCH_B(b, (k_sqf_lookup_sorted<4, false>), grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, fps, idx, un, d_out);
     }
   }
   return cs.end(ms, false);
