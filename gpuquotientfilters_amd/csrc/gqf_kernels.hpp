@@ -854,39 +854,84 @@ __global__ __launch_bounds__(DM_THREADS) void k_delete_mark(const K* __restrict_
 }
 
 // ====================================================================== region scan
-// Exclusive scan of `cols` interleaved uint32 columns over `rows` rows, in
-// place, by one block of 1024 threads (rows are region counts: small).
-static __global__ __launch_bounds__(1024) void k_scan_rows(uint32_t* __restrict__ data, uint32_t rows, uint32_t cols,
+// Exclusive scan of COLS interleaved uint32 columns over `rows` rows, in
+// place, by one block of 1024 threads, in pieces of 4096 rows: a piece is
+// staged through LDS with coalesced loads, each thread scans 4 consecutive
+// rows, one block-wide scan, coalesced stores; totals are carried between
+// pieces.  totals[c] = column sums (may be NULL).
+template <int COLS>
+__global__ __launch_bounds__(1024) void k_scan_rows(uint32_t* __restrict__ data, uint32_t rows,
                                                     uint32_t* __restrict__ totals) {
-  __shared__ uint32_t s_w[16];
-  __shared__ uint32_t s_carry;
+  constexpr uint32_t PIECE = 4096;
+  __shared__ uint32_t s_v[PIECE * COLS];
+  __shared__ uint32_t s_w[COLS][16];
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  for (uint32_t c = 0; c < cols; ++c) {
-    if (tid == 0) s_carry = 0;
+  uint32_t carry[COLS];
+#pragma unroll
+  for (int c = 0; c < COLS; ++c) carry[c] = 0;
+  for (uint32_t base = 0; base < rows; base += PIECE) {
+    const uint32_t n = rows - base < PIECE ? rows - base : PIECE;
+    const uint64_t g0 = (uint64_t)base * COLS;
+#pragma unroll
+    for (int k = 0; k < 4 * COLS; ++k) {
+      const uint32_t idx = tid + k * 1024;
+      if (idx < n * COLS) s_v[idx] = data[g0 + idx];
+    }
     __syncthreads();
-    for (uint32_t b = 0; b < rows; b += 1024) {
-      const uint32_t i = b + tid;
-      const uint32_t x = i < rows ? data[(uint64_t)i * cols + c] : 0u;
-      uint32_t inc = x;
+    uint32_t sum[COLS];
+#pragma unroll
+    for (int c = 0; c < COLS; ++c) {
+      sum[c] = 0;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const uint32_t row = tid * 4 + j;
+        sum[c] += row < n ? s_v[row * COLS + c] : 0u;
+      }
+    }
+    uint32_t pre[COLS];
+#pragma unroll
+    for (int c = 0; c < COLS; ++c) {
+      uint32_t inc = sum[c];
+#pragma unroll
       for (int d = 1; d < 64; d <<= 1) {
-        uint32_t y = __shfl_up(inc, d, 64);
+        const uint32_t y = __shfl_up(inc, d, 64);
         if (lane >= d) inc += y;
       }
-      if (lane == 63) s_w[wid] = inc;
-      __syncthreads();
-      uint32_t pre = 0, tot = 0;
-      for (int w = 0; w < 16; ++w) {
-        pre += w < wid ? s_w[w] : 0u;
-        tot += s_w[w];
-      }
-      const uint32_t carry = s_carry;
-      if (i < rows) data[(uint64_t)i * cols + c] = carry + pre + inc - x;
-      __syncthreads();
-      if (tid == 0) s_carry = carry + tot;
-      __syncthreads();
+      if (lane == 63) s_w[c][wid] = inc;
+      pre[c] = inc - sum[c];
     }
-    if (tid == 0 && totals) totals[c] = s_carry;
     __syncthreads();
+#pragma unroll
+    for (int c = 0; c < COLS; ++c) {
+      uint32_t wp = 0, tot = 0;
+#pragma unroll
+      for (int w = 0; w < 16; ++w) {
+        wp += w < wid ? s_w[c][w] : 0u;
+        tot += s_w[c][w];
+      }
+      uint32_t x = carry[c] + pre[c] + wp;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const uint32_t row = tid * 4 + j;
+        if (row < n) {
+          const uint32_t v = s_v[row * COLS + c];
+          s_v[row * COLS + c] = x;
+          x += v;
+        }
+      }
+      carry[c] += tot;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 4 * COLS; ++k) {
+      const uint32_t idx = tid + k * 1024;
+      if (idx < n * COLS) data[g0 + idx] = s_v[idx];
+    }
+    __syncthreads();
+  }
+  if (tid == 0 && totals) {
+#pragma unroll
+    for (int c = 0; c < COLS; ++c) totals[c] = carry[c];
   }
 }
 

@@ -335,7 +335,7 @@ static gqf_status rsqf_decode(hipStream_t s, DeviceState& ds, Arena& ar, const g
   }
   const dim3 grid((f.numBlocks + 255) / 256);
   GQF_LAUNCH_B((double)rsqf_meta_bytes(f.numBlocks), (k_rsqf_decode<false, K>), grid, dim3(256), 0, s, d, counts, (K*)nullptr);
-  GQF_LAUNCH(k_scan_rows, dim3(1), dim3(1024), 0, s, counts, f.numBlocks, 1u, total);
+  GQF_LAUNCH(k_scan_rows<1>, dim3(1), dim3(1024), 0, s, counts, f.numBlocks, total);
   GQF_TRY(read_ctr(s, ds, total, m_out));
   K* fps = ar.take<K>(*m_out + 1);
   if (ar.used > ar.cap) {

@@ -270,15 +270,17 @@ __global__ __launch_bounds__(256) void k_sqf_lookup_sorted(const uint8_t* __rest
 // inputs (merge path, A first on ties), the keep rule (all / unique /
 // dedup-insert / delete flags), the canonical locations
 // loc_i = max(fq_i, loc_{i-1}+1), and the table bytes.  A block of entries
-// acts on the incoming location as x -> max(x + c, l); the pairs (c, l) are
-// combined in chunk order:
-//   k_rb_agg    each chunk's (c, l) (one read of the inputs),
+// acts on the incoming location as x -> max(x + c, l); the states (c, l, f)
+// (f: quotient of the last kept entry, which decides run heads) are combined
+// in chunk order:
+//   k_rb_agg    each chunk's state (one read of the inputs),
 //   k_rb_scan   exclusive scan over the chunks (one workgroup); the total
 //               decides capacity before anything is written,
 //   k_rb_write  each chunk walks its 4096-entry sub-tiles with a running
 //               prefix; sub-tile u owns the slot range (L_{u-1}, L_u] ending at
 //               its last entry and writes every byte of it through an LDS
-//               window; slots past the last entry are cleared by
+//               window, each thread the slots up to its own last entry from
+//               registers; slots past the last entry are cleared by
 //               k_sqf_clear_tail.  An occupied bit that falls in an earlier
 //               sub-tile's range goes to a fix-up list (k_sqf_fixup).
 // Replaces the location and write stages of the three reference builds
@@ -291,11 +293,48 @@ constexpr int RB_WAVES = RB_THREADS / 64;
 constexpr int RB_TILE = RB_THREADS * RB_ITEMS;
 constexpr uint32_t RB_WIN = 8192;  // table bytes per write window
 constexpr uint32_t RB_FIX = 64;    // per-sub-tile occupied-bit fix-up slots
-// LDS: phase A = merged values u64[RB_TILE+1] + merge staging u32[RB_TILE];
-//      phase B = loc u32[RB_TILE] + element u32[RB_TILE] + run-head quotient u32[RB_TILE] + window
-constexpr uint32_t RB_RAW_A = 8 * (RB_TILE + 2) + 4 * RB_TILE;
-constexpr uint32_t RB_RAW_B = 12 * RB_TILE + RB_WIN;
-constexpr uint32_t RB_RAW = RB_RAW_A > RB_RAW_B ? RB_RAW_A : RB_RAW_B;
+constexpr uint32_t F_NONE = 0xFFFFFFFFu;
+
+// Scan state of a block of entries: kept count c, location map
+// x -> max(x + c, l), and the quotient f of its last kept entry (run heads are
+// the kept entries whose quotient differs from the previous kept entry's).
+struct RS {
+  uint32_t c;
+  int64_t l;
+  uint32_t f;
+};
+__device__ __forceinline__ RS rs_combine(RS a, RS b) {
+  RS o;
+  o.c = a.c + b.c;
+  const int64_t x = a.l == LOC_NONE ? LOC_NONE : a.l + (int64_t)b.c;
+  o.l = x > b.l ? x : b.l;
+  o.f = b.c ? b.f : a.f;
+  return o;
+}
+__device__ __forceinline__ RS rs_none() { return RS{0u, LOC_NONE, F_NONE}; }
+
+// Exclusive scan across the 64 lanes of a wave; *total = the wave aggregate.
+__device__ __forceinline__ RS rs_wave_excl(RS x, RS* total) {
+  const int lane = threadIdx.x & 63;
+  RS inc = x;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    RS y;
+    y.c = __shfl_up(inc.c, d, 64);
+    y.l = __shfl_up(inc.l, d, 64);
+    y.f = __shfl_up(inc.f, d, 64);
+    if (lane >= d) inc = rs_combine(y, inc);
+  }
+  RS ex;
+  ex.c = __shfl_up(inc.c, 1, 64);
+  ex.l = __shfl_up(inc.l, 1, 64);
+  ex.f = __shfl_up(inc.f, 1, 64);
+  if (lane == 0) ex = rs_none();
+  total->c = __shfl(inc.c, 63, 64);
+  total->l = __shfl(inc.l, 63, 64);
+  total->f = __shfl(inc.f, 63, 64);
+  return ex;
+}
 
 // Merge-path split (A entries among the first u*RB_TILE outputs) of every sub-tile boundary.
 __global__ __launch_bounds__(256) void k_rb_splits(const uint32_t* __restrict__ A, uint32_t na,
@@ -308,22 +347,25 @@ __global__ __launch_bounds__(256) void k_rb_splits(const uint32_t* __restrict__ 
   splits[u] = merge_path(A, na, B, nb, d);
 }
 
-// Fills v[0..RB_ITEMS] for this thread's RB_ITEMS consecutive entries of
-// sub-tile [d0, d0+tn): v[0] = the entry before them, v[1+j] = entry j, each
-// as (fp << 1 | source).  MERGE: merge path of A and B staged through LDS;
-// otherwise direct 16-byte loads.
-template <bool MERGE>
-__device__ __forceinline__ void rb_load(const uint32_t* __restrict__ A, uint32_t na, const uint32_t* __restrict__ B,
-                                        uint32_t nb, const uint32_t* __restrict__ splits, uint32_t u, uint32_t d0,
-                                        uint32_t tn, uint64_t* s_val, uint32_t* s_stage, uint64_t* v) {
+// This thread's RB_ITEMS consecutive entries of sub-tile [d0, d0+tn):
+// v[1+j] = entry j and v[0] = the entry before them; bit j of tags = entry j
+// comes from B; fl = delete flags of the entries (KEEP_FLAGS).  MERGE:
+// the sub-tile's A and B ranges are staged in LDS (s_stage) and each thread
+// merges from its own merge-path split; otherwise 16-byte loads.
+template <int MODE, bool MERGE>
+__device__ __forceinline__ void rb_load(const uint32_t* __restrict__ A, const uint32_t* __restrict__ B,
+                                        const uint8_t* __restrict__ flags, const uint32_t* __restrict__ splits,
+                                        uint32_t u, uint32_t d0, uint32_t tn, uint32_t* s_stage, uint32_t* v,
+                                        uint32_t& tags, uint32_t& fl) {
   const int tid = threadIdx.x;
   const uint32_t i0 = (uint32_t)tid * RB_ITEMS;
+  fl = 0;
+  tags = 0;
   if (MERGE) {
     const uint32_t a0 = splits[u], a1 = splits[u + 1];
     const uint32_t b0 = d0 - a0, b1 = d0 + tn - a1;
-    const uint32_t la = a1 - a0, lb = b1 - b0;
+    const uint32_t la = a1 - a0, lb = b1 - b0;  // la + lb == tn <= RB_TILE
     {
-      // la + lb == tn <= RB_TILE: each thread stages at most RB_ITEMS entries
       uint32_t w[RB_ITEMS];
 #pragma unroll
       for (int k = 0; k < RB_ITEMS; ++k) {
@@ -336,29 +378,32 @@ __device__ __forceinline__ void rb_load(const uint32_t* __restrict__ A, uint32_t
         if (i < la + lb) s_stage[i] = w[k];
       }
     }
-    if (tid == 0) {
-      uint64_t prev = 0;
-      if (d0 > 0) {
-        if (b0 > 0 && (a0 == 0 || A[a0 - 1] <= B[b0 - 1])) prev = ((uint64_t)B[b0 - 1] << 1) | 1ull;
-        else prev = (uint64_t)A[a0 - 1] << 1;
-      }
-      s_val[0] = prev;
-    }
     __syncthreads();
+    const uint32_t* sA = s_stage;
+    const uint32_t* sB = s_stage + la;
     const uint32_t ld = i0 < tn ? i0 : tn;
-    uint32_t ai = merge_path(s_stage, la, s_stage + la, lb, ld);
+    uint32_t ai = merge_path(sA, la, sB, lb, ld);
     uint32_t bi = ld - ai;
+    // the merged entry before position ld: the later (in merge order: value,
+    // then A before B) of the last A and the last B entry taken
+    const bool hasA = ai > 0 || a0 > 0, hasB = bi > 0 || b0 > 0;
+    const uint32_t pA = hasA ? (ai > 0 ? sA[ai - 1] : A[a0 - 1]) : 0u;
+    const uint32_t pB = hasB ? (bi > 0 ? sB[bi - 1] : B[b0 - 1]) : 0u;
+    v[0] = (hasB && (!hasA || pA <= pB)) ? pB : pA;
 #pragma unroll
     for (int j = 0; j < RB_ITEMS; ++j) {
-      const uint32_t o = ld + j;
-      if (o < tn) {
-        const bool takeA = bi >= lb || (ai < la && s_stage[ai] <= s_stage[la + bi]);
-        s_val[1 + o] = takeA ? ((uint64_t)s_stage[ai++] << 1) : (((uint64_t)s_stage[la + bi++] << 1) | 1ull);
+      uint32_t x = 0;
+      if (ld + j < tn) {
+        const bool takeA = bi >= lb || (ai < la && sA[ai] <= sB[bi]);
+        if (takeA) {
+          x = sA[ai++];
+        } else {
+          x = sB[bi++];
+          tags |= 1u << j;
+        }
       }
+      v[1 + j] = x;
     }
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j <= RB_ITEMS; ++j) v[j] = (i0 + j <= tn) ? s_val[i0 + j] : 0ull;
   } else {
     const uint64_t g0 = (uint64_t)d0 + i0;
     uint32_t w[RB_ITEMS];
@@ -373,419 +418,383 @@ __device__ __forceinline__ void rb_load(const uint32_t* __restrict__ A, uint32_t
         w[4 * k + 2] = x.z;
         w[4 * k + 3] = x.w;
       }
+      if (MODE == KEEP_FLAGS) {
+        const uint2 f = *reinterpret_cast<const uint2*>(flags + g0);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          fl |= ((f.x >> (8 * k)) & 1u) << k;
+          fl |= ((f.y >> (8 * k)) & 1u) << (4 + k);
+        }
+      }
     } else {
 #pragma unroll
-      for (int k = 0; k < RB_ITEMS; ++k) w[k] = (i0 + k < tn) ? A[g0 + k] : 0u;
+      for (int k = 0; k < RB_ITEMS; ++k) {
+        w[k] = (i0 + k < tn) ? A[g0 + k] : 0u;
+        if (MODE == KEEP_FLAGS && i0 + k < tn && flags[g0 + k]) fl |= 1u << k;
+      }
     }
-    v[0] = g0 > 0 && i0 <= tn ? (uint64_t)A[g0 - 1] << 1 : 0ull;
+    v[0] = g0 > 0 && i0 <= tn ? A[g0 - 1] : 0u;
 #pragma unroll
-    for (int k = 0; k < RB_ITEMS; ++k) v[1 + k] = (uint64_t)w[k] << 1;
+    for (int k = 0; k < RB_ITEMS; ++k) v[1 + k] = w[k];
   }
 }
 
-// Keep rule and run heads of this thread's RB_ITEMS consecutive entries, and
-// their (count, last location) aggregate.
-template <int MODE, bool HEADS>
-__device__ __forceinline__ CL rb_classify(const uint64_t* v, uint32_t d0, uint32_t tn, uint32_t r, uint64_t mq,
-                                          const uint32_t* __restrict__ A, const uint8_t* __restrict__ flags,
-                                          uint32_t* fpv, uint32_t& keepbits, uint32_t& headbits, uint32_t& nhead) {
-  CL agg = {0u, LOC_NONE};
-  keepbits = 0;
-  headbits = 0;
-  nhead = 0;
+// Keep rule of this thread's entries (bit j of the result) and their scan state.
+template <int MODE>
+__device__ __forceinline__ uint32_t rb_classify(const uint32_t* v, uint32_t tags, uint32_t fl, uint32_t d0,
+                                                uint32_t tn, uint32_t r, uint64_t mq, RS& agg) {
+  uint32_t keepbits = 0;
+  agg = rs_none();
   const uint32_t i0 = (uint32_t)threadIdx.x * RB_ITEMS;
 #pragma unroll
   for (int j = 0; j < RB_ITEMS; ++j) {
     const uint32_t i = i0 + j;
-    fpv[j] = 0;
     if (i < tn) {
-      const uint64_t key = v[1 + j], pkey = v[j];
-      const uint32_t fp = (uint32_t)(key >> 1), pfp = (uint32_t)(pkey >> 1);
+      const uint32_t fp = v[1 + j], pfp = v[j];
       const bool first = (d0 + i) == 0;
-      const uint64_t fq = ((uint64_t)fp >> r) & mq;
       bool kp;
       if (MODE == KEEP_ALL) kp = true;
       else if (MODE == KEEP_UNIQUE) kp = first || fp != pfp;
-      else if (MODE == KEEP_TAGGED_INSERT) kp = ((key & 1ull) == 0) || first || fp != pfp;
-      else kp = flags[d0 + i] != 0;
-      fpv[j] = fp;
+      else if (MODE == KEEP_TAGGED_INSERT) kp = ((tags >> j) & 1u) == 0 || first || fp != pfp;
+      else kp = (fl >> j) & 1u;
       if (kp) {
         keepbits |= 1u << j;
-        if (HEADS) {
-          bool head;
-          if (MODE != KEEP_FLAGS) {
-            // a dropped entry always repeats an earlier kept fingerprint, so
-            // the previous entry decides whether this one opens its run
-            head = first || (((uint64_t)pfp >> r) & mq) != fq;
-          } else {
-            head = true;  // opens its run unless an earlier kept entry shares the quotient
-            for (int64_t p = (int64_t)(d0 + i) - 1; p >= 0; --p) {
-              if ((((uint64_t)A[p] >> r) & mq) != fq) break;
-              if (flags[p]) {
-                head = false;
-                break;
-              }
-            }
-          }
-          if (head) {
-            headbits |= 1u << j;
-            nhead++;
-          }
-        }
+        const uint32_t fq = (uint32_t)(((uint64_t)fp >> r) & mq);
         const int64_t f = (int64_t)fq;
         agg.l = agg.l == LOC_NONE ? f : (f > agg.l + 1 ? f : agg.l + 1);
         agg.c++;
+        agg.f = fq;
       }
     }
   }
-  return agg;
-}
-
-// Exclusive scan of (c, l) across the 64 lanes of a wave; *total = the
-// wave's inclusive aggregate.
-__device__ __forceinline__ CL cl_wave_excl(CL x, CL* total) {
-  const int lane = threadIdx.x & 63;
-  CL inc = x;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    CL y;
-    y.c = __shfl_up(inc.c, d, 64);
-    y.l = __shfl_up(inc.l, d, 64);
-    if (lane >= d) inc = cl_combine(y, inc);
-  }
-  CL ex;
-  ex.c = __shfl_up(inc.c, 1, 64);
-  ex.l = __shfl_up(inc.l, 1, 64);
-  if (lane == 0) {
-    ex.c = 0;
-    ex.l = LOC_NONE;
-  }
-  total->c = __shfl(inc.c, 63, 64);
-  total->l = __shfl(inc.l, 63, 64);
-  return ex;
+  return keepbits;
 }
 
 struct RbPlan {
   uint32_t G, chunk, nsub;  // chunks, entries per chunk (multiple of RB_TILE), sub-tiles
 };
+constexpr uint32_t RB_MAX_CHUNKS = 16384;
 static inline RbPlan rb_plan(uint64_t N) {
+  // one sub-tile per workgroup (latency is hidden by occupancy, not by a
+  // sequential walk), several per chunk only beyond RB_MAX_CHUNKS sub-tiles
   RbPlan p;
-  const SortPlan sp = sort_plan((uint32_t)N);
-  p.G = sp.G;
-  p.chunk = sp.chunk;
   p.nsub = (uint32_t)((N + RB_TILE - 1) / RB_TILE);
+  const uint32_t per = (p.nsub + RB_MAX_CHUNKS - 1) / RB_MAX_CHUNKS;
+  p.chunk = (per ? per : 1) * RB_TILE;
+  p.G = (uint32_t)((N + p.chunk - 1) / p.chunk);
+  if (p.G == 0) p.G = 1;
   return p;
 }
 
+// Block-wide aggregate of the per-thread states; s_* hold one entry per wave.
+__device__ __forceinline__ RS rs_block(RS a, RS* ex_out, uint32_t* s_c, long long* s_l, uint32_t* s_f) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  RS wt;
+  const RS ex = rs_wave_excl(a, &wt);
+  if (lane == 0) {
+    s_c[wid] = wt.c;
+    s_l[wid] = wt.l;
+    s_f[wid] = wt.f;
+  }
+  __syncthreads();
+  RS wpre = rs_none(), sub = rs_none();
+#pragma unroll
+  for (int w = 0; w < RB_WAVES; ++w) {
+    const RS wv = {s_c[w], (int64_t)s_l[w], s_f[w]};
+    if (w < wid) wpre = rs_combine(wpre, wv);
+    sub = rs_combine(sub, wv);
+  }
+  *ex_out = rs_combine(wpre, ex);
+  return sub;
+}
+
 template <int MODE, bool MERGE>
-__global__ __launch_bounds__(RB_THREADS) void k_rb_agg(const uint32_t* __restrict__ A, uint32_t na,
+__global__ __launch_bounds__(RB_THREADS, 8) void k_rb_agg(const uint32_t* __restrict__ A, uint32_t na,
                                                 const uint32_t* __restrict__ B, uint32_t nb,
                                                 const uint8_t* __restrict__ flags, const uint32_t* __restrict__ splits,
                                                 uint32_t q, uint32_t r, uint32_t chunk,
-                                                uint32_t* __restrict__ agg_c, long long* __restrict__ agg_l) {
-  __shared__ __align__(16) unsigned char s_raw[RB_RAW_A];
-  __shared__ uint32_t s_wc[RB_WAVES];
-  __shared__ int64_t s_wl[RB_WAVES];
-  uint64_t* s_val = reinterpret_cast<uint64_t*>(s_raw);
-  uint32_t* s_stage = reinterpret_cast<uint32_t*>(s_raw + 8 * (RB_TILE + 2));
-  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+                                                uint32_t* __restrict__ agg_c, long long* __restrict__ agg_l,
+                                                uint32_t* __restrict__ agg_f) {
+  __shared__ uint32_t s_stage[MERGE ? RB_TILE : 1];
+  __shared__ uint32_t s_c[RB_WAVES], s_f[RB_WAVES];
+  __shared__ long long s_l[RB_WAVES];
   const uint64_t N = (uint64_t)na + (MERGE ? nb : 0u);
   const uint64_t mq = mask64(q);
   const uint64_t c0 = (uint64_t)blockIdx.x * chunk;
   const uint64_t c1 = c0 + chunk < N ? c0 + chunk : N;
-  CL X = {0u, LOC_NONE};
+  RS X = rs_none();
   for (uint64_t d0 = c0; d0 < c1; d0 += RB_TILE) {
     const uint32_t tn = (uint32_t)(c1 - d0 < (uint64_t)RB_TILE ? c1 - d0 : (uint64_t)RB_TILE);
-    uint64_t v[RB_ITEMS + 1];
-    rb_load<MERGE>(A, na, B, nb, splits, (uint32_t)(d0 / RB_TILE), (uint32_t)d0, tn, s_val, s_stage, v);
-    uint32_t fpv[RB_ITEMS], kb, hb, nh;
-    const CL a = rb_classify<MODE, false>(v, (uint32_t)d0, tn, r, mq, A, flags, fpv, kb, hb, nh);
-    CL wt;
-    (void)cl_wave_excl(a, &wt);
-    if (lane == 0) {
-      s_wc[wid] = wt.c;
-      s_wl[wid] = wt.l;
-    }
-    __syncthreads();
-    CL sub = {0u, LOC_NONE};
-#pragma unroll
-    for (int w = 0; w < RB_WAVES; ++w) sub = cl_combine(sub, CL{s_wc[w], s_wl[w]});
-    X = cl_combine(X, sub);
+    uint32_t v[RB_ITEMS + 1];
+    uint32_t tags, fl;
+    rb_load<MODE, MERGE>(A, B, flags, splits, (uint32_t)(d0 / RB_TILE), (uint32_t)d0, tn, s_stage, v, tags, fl);
+    RS a;
+    (void)rb_classify<MODE>(v, tags, fl, (uint32_t)d0, tn, r, mq, a);
+    RS ex;
+    const RS sub = rs_block(a, &ex, s_c, s_l, s_f);
+    X = rs_combine(X, sub);
     __syncthreads();
   }
-  if (tid == 0) {
+  if (threadIdx.x == 0) {
     agg_c[blockIdx.x] = X.c;
     agg_l[blockIdx.x] = X.l;
+    agg_f[blockIdx.x] = X.f;
   }
 }
 
-// Exclusive scan of the chunk aggregates (one workgroup, 1024 threads).  The
-// total fixes the kept count, the first slot after the last entry, and a
+// Exclusive scan of the chunk states (one workgroup of 1024 threads, in
+// pieces of 2048 states staged through LDS; 2 consecutive states per thread).
+// The total fixes the kept count, the first slot after the last entry, and a
 // capacity error -- all before any table byte is written.
 __global__ __launch_bounds__(1024) void k_rb_scan(uint32_t* __restrict__ agg_c, long long* __restrict__ agg_l,
-                                                  uint32_t G, uint64_t S, uint32_t* __restrict__ ctr) {
-  __shared__ uint32_t s_wc[16];
+                                                  uint32_t* __restrict__ agg_f, uint32_t G, uint64_t S,
+                                                  uint32_t* __restrict__ ctr) {
+  constexpr uint32_t PIECE = 2048;
+  __shared__ uint32_t s_c[PIECE], s_f[PIECE];
+  __shared__ long long s_l[PIECE];
+  __shared__ uint32_t s_wc[16], s_wf[16];
   __shared__ int64_t s_wl[16];
-  __shared__ uint32_t s_cc;
-  __shared__ int64_t s_cl;
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  if (tid == 0) {
-    s_cc = 0;
-    s_cl = LOC_NONE;
-  }
-  __syncthreads();
-  for (uint32_t b = 0; b < G; b += 1024) {
-    const uint32_t i = b + tid;
-    CL x = {0u, LOC_NONE};
-    if (i < G) x = CL{agg_c[i], (int64_t)agg_l[i]};
-    CL wt;
-    CL ex = cl_wave_excl(x, &wt);
+  RS carry = rs_none();
+  for (uint32_t base = 0; base < G; base += PIECE) {
+    const uint32_t n = G - base < PIECE ? G - base : PIECE;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const uint32_t i = tid + k * 1024;
+      if (i < n) {
+        s_c[i] = agg_c[base + i];
+        s_l[i] = agg_l[base + i];
+        s_f[i] = agg_f[base + i];
+      }
+    }
+    __syncthreads();
+    RS x0 = rs_none(), x1 = rs_none();
+    const uint32_t i0 = 2 * tid;
+    if (i0 < n) x0 = RS{s_c[i0], (int64_t)s_l[i0], s_f[i0]};
+    if (i0 + 1 < n) x1 = RS{s_c[i0 + 1], (int64_t)s_l[i0 + 1], s_f[i0 + 1]};
+    RS wt;
+    RS ex = rs_wave_excl(rs_combine(x0, x1), &wt);
     if (lane == 0) {
       s_wc[wid] = wt.c;
       s_wl[wid] = wt.l;
+      s_wf[wid] = wt.f;
     }
     __syncthreads();
-    CL pre = {s_cc, s_cl}, tot = {s_cc, s_cl};
+    RS pre = carry, tot = carry;
 #pragma unroll
     for (int w = 0; w < 16; ++w) {
-      const CL wv = {s_wc[w], s_wl[w]};
-      if (w < wid) pre = cl_combine(pre, wv);
-      tot = cl_combine(tot, wv);
+      const RS wv = {s_wc[w], s_wl[w], s_wf[w]};
+      if (w < wid) pre = rs_combine(pre, wv);
+      tot = rs_combine(tot, wv);
     }
-    ex = cl_combine(pre, ex);
-    if (i < G) {
-      agg_c[i] = ex.c;
-      agg_l[i] = ex.l;
+    ex = rs_combine(pre, ex);
+    if (i0 < n) {
+      s_c[i0] = ex.c;
+      s_l[i0] = ex.l;
+      s_f[i0] = ex.f;
+      ex = rs_combine(ex, x0);
     }
+    if (i0 + 1 < n) {
+      s_c[i0 + 1] = ex.c;
+      s_l[i0 + 1] = ex.l;
+      s_f[i0 + 1] = ex.f;
+    }
+    carry = tot;
     __syncthreads();
-    if (tid == 0) {
-      s_cc = tot.c;
-      s_cl = tot.l;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const uint32_t i = tid + k * 1024;
+      if (i < n) {
+        agg_c[base + i] = s_c[i];
+        agg_l[base + i] = s_l[i];
+        agg_f[base + i] = s_f[i];
+      }
     }
     __syncthreads();
   }
   if (tid == 0) {
-    ctr[CTR_COUNT] = s_cc;
-    const int64_t L = s_cl;
+    ctr[CTR_COUNT] = carry.c;
+    const int64_t L = carry.l;
     ctr[CTR_MISC + 3] = L == LOC_NONE ? 0u : (uint32_t)(L + 1 < (int64_t)S ? L + 1 : (int64_t)S);
     if (L != LOC_NONE && (uint64_t)L >= S) ctr[CTR_ERR] |= DEV_ERR_CAPACITY;
   }
 }
 
-template <int MODE, bool MERGE>
-__global__ __launch_bounds__(RB_THREADS) void k_rb_write(uint8_t* __restrict__ t, uint32_t S, uint32_t q, uint32_t r,
-                                                  uint32_t bpe, const uint32_t* __restrict__ A, uint32_t na,
+// Writes the element bytes of slot sl into the window (big-endian).
+template <int BPE>
+__device__ __forceinline__ void win_put(uint8_t* s_win, uint32_t off, uint32_t e) {
+#pragma unroll
+  for (int b = 0; b < BPE; ++b) s_win[off + b] = (uint8_t)(e >> (8 * (BPE - 1 - b)));
+}
+
+// Copies window bytes [B0, B1) of the table from LDS (dword stores inside,
+// byte stores at unaligned edges).
+__device__ __forceinline__ void win_out(uint8_t* __restrict__ t, const uint8_t* s_win, uint64_t B0, uint64_t B1) {
+  const int tid = threadIdx.x;
+  const uint64_t A0 = (B0 + 3) & ~3ull, A1 = B1 & ~3ull;
+  if (A0 >= A1) {
+    for (uint64_t b = B0 + tid; b < B1; b += RB_THREADS) t[b] = s_win[b - B0];
+    return;
+  }
+  for (uint64_t b = B0 + tid; b < A0; b += RB_THREADS) t[b] = s_win[b - B0];
+  for (uint64_t b = A1 + tid; b < B1; b += RB_THREADS) t[b] = s_win[b - B0];
+  uint32_t* tw = reinterpret_cast<uint32_t*>(t + A0);
+  const uint32_t nw = (uint32_t)((A1 - A0) / 4), sh = (uint32_t)(A0 - B0);
+  for (uint32_t w = tid; w < nw; w += RB_THREADS) {
+    const uint32_t q0 = sh + 4 * w;
+    tw[w] = (uint32_t)s_win[q0] | ((uint32_t)s_win[q0 + 1] << 8) | ((uint32_t)s_win[q0 + 2] << 16) |
+            ((uint32_t)s_win[q0 + 3] << 24);
+  }
+}
+
+// Each chunk walks its sub-tiles with the running state X.  Sub-tile u owns
+// the slots (X.l, Y.l] up to its last kept entry; each thread owns the slots
+// after the previous kept entry up to its own last one (its territory) and
+// writes them -- elements and the zeros between them -- into an LDS window
+// straight from registers.  Occupied bits are then set in the window (each
+// run head's quotient is a different slot, so plain byte updates do not
+// race); those whose quotient precedes the owned range go to the fix-up
+// list.  A long owned range (sparse table) is written window by window.
+template <int MODE, bool MERGE, int BPE>
+__global__ __launch_bounds__(RB_THREADS, 4) void k_rb_write(uint8_t* __restrict__ t, uint32_t S, uint32_t q, uint32_t r,
+                                                  const uint32_t* __restrict__ A, uint32_t na,
                                                   const uint32_t* __restrict__ B, uint32_t nb,
                                                   const uint8_t* __restrict__ flags,
                                                   const uint32_t* __restrict__ splits, uint32_t chunk,
                                                   const uint32_t* __restrict__ pre_c, const long long* __restrict__ pre_l,
+                                                  const uint32_t* __restrict__ pre_f,
                                                   const uint32_t* __restrict__ ctr, uint32_t* __restrict__ fix,
                                                   uint32_t* __restrict__ fix_cnt, uint32_t* __restrict__ fix_more,
                                                   uint32_t* __restrict__ fix_more_n, uint32_t fix_cap,
                                                   int precleared) {
   if (ctr[CTR_ERR]) return;  // capacity error: the table stays as it was
-  __shared__ __align__(16) unsigned char s_raw[RB_RAW];
-  __shared__ uint32_t s_wc[RB_WAVES], s_hc[RB_WAVES];
-  __shared__ int64_t s_wl[RB_WAVES];
-  __shared__ uint32_t s_tmp[4];
-  uint64_t* s_val = reinterpret_cast<uint64_t*>(s_raw);                      // [RB_TILE + 1]
-  uint32_t* s_stage = reinterpret_cast<uint32_t*>(s_raw + 8 * (RB_TILE + 2));  // [RB_TILE]
-  uint32_t* s_loc = reinterpret_cast<uint32_t*>(s_raw);                      // [RB_TILE]
-  uint32_t* s_el = reinterpret_cast<uint32_t*>(s_raw + 4 * RB_TILE);         // [RB_TILE]
-  uint32_t* s_ffq = reinterpret_cast<uint32_t*>(s_raw + 8 * RB_TILE);        // [RB_TILE]
-  uint8_t* s_win = s_raw + 12 * RB_TILE;                                      // [RB_WIN]
-
-  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  __shared__ uint32_t s_stage[MERGE ? RB_TILE : 1];
+  __shared__ __align__(16) uint8_t s_win[RB_WIN];
+  __shared__ uint32_t s_c[RB_WAVES], s_f[RB_WAVES];
+  __shared__ long long s_l[RB_WAVES];
+  __shared__ uint32_t s_nfix;
+  const int tid = threadIdx.x;
   const uint64_t N = (uint64_t)na + (MERGE ? nb : 0u);
   const uint64_t mq = mask64(q);
+  const uint32_t rmask = (1u << r) - 1u;
   const uint64_t c0 = (uint64_t)blockIdx.x * chunk;
   const uint64_t c1 = c0 + chunk < N ? c0 + chunk : N;
-  CL X = {pre_c[blockIdx.x], (int64_t)pre_l[blockIdx.x]};
-  const uint32_t wslots = RB_WIN / bpe;
+  RS X = {pre_c[blockIdx.x], (int64_t)pre_l[blockIdx.x], pre_f[blockIdx.x]};
+  constexpr uint32_t bpe = BPE;
+  const uint32_t wslots = RB_WIN / BPE;
   for (uint64_t d0 = c0; d0 < c1; d0 += RB_TILE) {
     const uint32_t u = (uint32_t)(d0 / RB_TILE);
     const uint32_t tn = (uint32_t)(c1 - d0 < (uint64_t)RB_TILE ? c1 - d0 : (uint64_t)RB_TILE);
-    uint64_t v[RB_ITEMS + 1];
-    rb_load<MERGE>(A, na, B, nb, splits, u, (uint32_t)d0, tn, s_val, s_stage, v);
-    uint32_t fpv[RB_ITEMS], keepbits, headbits, nhead;
-    const CL a = rb_classify<MODE, true>(v, (uint32_t)d0, tn, r, mq, A, flags, fpv, keepbits, headbits, nhead);
-    CL wt;
-    CL ex = cl_wave_excl(a, &wt);
-    // run-head counts: exclusive within the block
-    uint32_t hinc = nhead;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const uint32_t y = __shfl_up(hinc, d, 64);
-      if (lane >= d) hinc += y;
-    }
-    if (lane == 63) {
-      s_wc[wid] = wt.c;
-      s_wl[wid] = wt.l;
-      s_hc[wid] = hinc;
-    }
-    __syncthreads();  // all threads are done with s_val / s_stage: LDS is reused below
-    CL wpre = {0u, LOC_NONE}, sub = {0u, LOC_NONE};
-    uint32_t hpre = 0, tf = 0;
-#pragma unroll
-    for (int w = 0; w < RB_WAVES; ++w) {
-      const CL wv = {s_wc[w], s_wl[w]};
-      if (w < wid) {
-        wpre = cl_combine(wpre, wv);
-        hpre += s_hc[w];
-      }
-      sub = cl_combine(sub, wv);
-      tf += s_hc[w];
-    }
-    const CL me = cl_combine(cl_combine(X, wpre), ex);
-    // this thread's territory: the slots after the previous kept entry up to
-    // its own last kept entry (territories tile the sub-tile's owned range)
-    const int64_t tstart = me.l == LOC_NONE ? 0 : me.l + 1;
-    int64_t tend = tstart - 1;
-    uint32_t o0 = me.c - X.c;
+    uint32_t v[RB_ITEMS + 1];
+    uint32_t tags, fl;
+    rb_load<MODE, MERGE>(A, B, flags, splits, u, (uint32_t)d0, tn, s_stage, v, tags, fl);
+    RS a;
+    const uint32_t keepbits = rb_classify<MODE>(v, tags, fl, (uint32_t)d0, tn, r, mq, a);
+    if (tid == 0) s_nfix = 0;
+    RS ex;
+    const RS sub = rs_block(a, &ex, s_c, s_l, s_f);
+    const RS me = rs_combine(X, ex);  // state before this thread's entries
+    const RS Y = rs_combine(X, sub);
+    const uint32_t Llo = X.l == LOC_NONE ? 0u : (uint32_t)(X.l + 1);
+    const uint32_t Lhi = sub.c > 0 ? (uint32_t)(Y.l + 1) : Llo;
+    // per-thread location walk in 32 bits: -1 = no entry yet (fq >= 0)
+    const int32_t prev0 = me.l == LOC_NONE ? -1 : (int32_t)me.l;
+    const int32_t tstart = prev0 + 1;
+    // run heads whose quotient precedes the owned range: fix-up list
     {
-      int64_t prev = me.l;
-      uint32_t o = o0, h = hpre + hinc - nhead;
+      uint32_t lastf = me.f;
 #pragma unroll
       for (int j = 0; j < RB_ITEMS; ++j) {
         if (keepbits & (1u << j)) {
-          const int64_t fq = (int64_t)(((uint64_t)fpv[j] >> r) & mq);
-          const int64_t lc = prev == LOC_NONE ? fq : (fq > prev + 1 ? fq : prev + 1);
-          const bool head = (headbits >> j) & 1u;
-          uint32_t e = (fpv[j] & ((1u << r) - 1u)) << 3;
-          if (!head) e |= 2u;
-          if (lc != fq) e |= 1u;
-          s_loc[o] = (uint32_t)lc;
-          s_el[o] = e;
-          if (head) s_ffq[h++] = (uint32_t)fq;
-          prev = lc;
-          tend = lc;
-          ++o;
-        }
-      }
-    }
-    __syncthreads();
-    const uint32_t tc = sub.c;
-    const CL Y = cl_combine(X, sub);
-    const uint32_t Llo = X.l == LOC_NONE ? 0u : (uint32_t)(X.l + 1);
-    const uint32_t Lhi = tc > 0 ? (uint32_t)(Y.l + 1) : Llo;
-    // run heads whose quotient precedes the owned range (a prefix of the
-    // ascending head list): their occupied bits go to the fix-up slots
-    uint32_t nfix;
-    {
-      uint32_t lo = 0, hi = tf;
-      while (lo < hi) {
-        const uint32_t m = (lo + hi) >> 1;
-        if (s_ffq[m] < Llo) lo = m + 1;
-        else hi = m;
-      }
-      nfix = lo;
-      const uint32_t mine = nfix < RB_FIX ? nfix : RB_FIX;
-      for (uint32_t f = tid; f < mine; f += RB_THREADS) fix[(uint64_t)u * RB_FIX + f] = s_ffq[f];
-      if (tid == 0) {
-        fix_cnt[u] = mine;
-        if (nfix > RB_FIX) s_tmp[0] = atomicAdd(fix_more_n, nfix - RB_FIX);
-      }
-      if (nfix > RB_FIX) {
-        __syncthreads();
-        const uint32_t k = s_tmp[0];
-        for (uint32_t f = RB_FIX + tid; f < nfix; f += RB_THREADS) {
-          const uint32_t slot = k + (f - RB_FIX);
-          if (slot < fix_cap) fix_more[slot] = s_ffq[f];
-        }
-      }
-    }
-    if ((uint64_t)(Lhi - Llo) * bpe <= RB_WIN) {
-      // fast path: the whole owned range fits one window; every thread fills
-      // its own territory (entries, zeros and occupied bits), then one copy out
-      if (tend >= tstart) {
-        uint32_t o = o0;
-        // first head at or after the territory start
-        uint32_t lo = nfix, hi = tf;
-        while (lo < hi) {
-          const uint32_t m = (lo + hi) >> 1;
-          if ((int64_t)s_ffq[m] < tstart) lo = m + 1;
-          else hi = m;
-        }
-        uint32_t f = lo;
-        for (int64_t sl = tstart; sl <= tend; ++sl) {
-          uint32_t e = 0;
-          if (s_loc[o] == (uint32_t)sl) e = s_el[o++];
-          if (f < tf && (int64_t)s_ffq[f] == sl) {
-            e |= 4u;
-            ++f;
+          const uint32_t fp = v[1 + j];
+          const uint32_t fq = (fp >> r) & (uint32_t)mq;
+          if (fq != lastf && fq < Llo) {
+            const uint32_t k = atomicAdd(&s_nfix, 1u);
+            if (k < RB_FIX) {
+              fix[(uint64_t)u * RB_FIX + k] = fq;
+            } else {
+              const uint32_t slot = atomicAdd(fix_more_n, 1u);
+              if (slot < fix_cap) fix_more[slot] = fq;
+            }
           }
-          const uint32_t off = (uint32_t)(sl - Llo) * bpe;
-          for (uint32_t b = 0; b < bpe; ++b) s_win[off + b] = (uint8_t)(e >> (8 * (bpe - 1 - b)));
+          lastf = fq;
+        }
+      }
+    }
+    for (uint32_t w0 = Llo; w0 < Lhi; w0 += wslots) {
+      const uint32_t w1 = w0 + wslots < Lhi ? w0 + wslots : Lhi;
+      if (precleared) {
+        // the table is all zeros: windows holding no element and no
+        // occupied bit are skipped; the others start from a zeroed window
+        bool any = false;
+        int32_t prev = prev0;
+        uint32_t lastf = me.f;
+#pragma unroll
+        for (int j = 0; j < RB_ITEMS; ++j) {
+          if (keepbits & (1u << j)) {
+            const uint32_t fp = v[1 + j];
+            const int32_t fq = (int32_t)((fp >> r) & (uint32_t)mq);
+            const int32_t lc = fq > prev + 1 ? fq : prev + 1;
+            if ((uint32_t)lc >= w0 && (uint32_t)lc < w1) any = true;
+            if ((uint32_t)fq != lastf && (uint32_t)fq >= w0 && (uint32_t)fq < w1) any = true;
+            prev = lc;
+            lastf = (uint32_t)fq;
+          }
+        }
+        if (!__syncthreads_or(any)) continue;
+        const uint32_t nw = ((w1 - w0) * bpe + 3) / 4;
+        for (uint32_t k = tid; k < nw; k += RB_THREADS) reinterpret_cast<uint32_t*>(s_win)[k] = 0u;
+        __syncthreads();
+      }
+      // this thread's territory within the window: elements, and the zeros
+      // before each of them unless the window was zeroed above
+      {
+        int32_t prev = prev0;
+        uint32_t lastf = me.f;
+        int32_t cur = tstart > (int32_t)w0 ? tstart : (int32_t)w0;  // next slot to write
+#pragma unroll
+        for (int j = 0; j < RB_ITEMS; ++j) {
+          if (keepbits & (1u << j)) {
+            const uint32_t fp = v[1 + j];
+            const int32_t fq = (int32_t)((fp >> r) & (uint32_t)mq);
+            const int32_t lc = fq > prev + 1 ? fq : prev + 1;
+            const bool head = (uint32_t)fq != lastf;
+            if (!precleared) {
+              const int32_t zend = lc < (int32_t)w1 ? lc : (int32_t)w1;
+              for (; cur < zend; ++cur) win_put<BPE>(s_win, (uint32_t)(cur - (int32_t)w0) * BPE, 0u);
+            }
+            if ((uint32_t)lc >= w0 && (uint32_t)lc < w1) {
+              uint32_t e = (fp & rmask) << 3;
+              if (!head) e |= 2u;
+              if (lc != fq) e |= 1u;
+              win_put<BPE>(s_win, ((uint32_t)lc - w0) * BPE, e);
+            }
+            if (lc + 1 > cur) cur = lc + 1;
+            prev = lc;
+            lastf = (uint32_t)fq;
+          }
         }
       }
       __syncthreads();
-      if (Lhi > Llo) {
-        const uint64_t B0 = (uint64_t)Llo * bpe, B1 = (uint64_t)Lhi * bpe;
-        const uint64_t A0 = (B0 + 3) & ~3ull, A1 = B1 & ~3ull;
-        if (A0 >= A1) {
-          for (uint64_t b = B0 + tid; b < B1; b += RB_THREADS) t[b] = s_win[b - B0];
-        } else {
-          for (uint64_t b = B0 + tid; b < A0; b += RB_THREADS) t[b] = s_win[b - B0];
-          for (uint64_t b = A1 + tid; b < B1; b += RB_THREADS) t[b] = s_win[b - B0];
-          uint32_t* tw = reinterpret_cast<uint32_t*>(t + A0);
-          const uint32_t nw = (uint32_t)((A1 - A0) / 4), sh = (uint32_t)(A0 - B0);
-          for (uint32_t w = tid; w < nw; w += RB_THREADS) {
-            const uint32_t q0 = sh + 4 * w;
-            tw[w] = (uint32_t)s_win[q0] | ((uint32_t)s_win[q0 + 1] << 8) | ((uint32_t)s_win[q0 + 2] << 16) |
-                    ((uint32_t)s_win[q0 + 3] << 24);
+      // occupied bits of the run heads whose quotient is in this window
+      {
+        uint32_t lastf = me.f;
+#pragma unroll
+        for (int j = 0; j < RB_ITEMS; ++j) {
+          if (keepbits & (1u << j)) {
+            const uint32_t fp = v[1 + j];
+            const uint32_t fq = (fp >> r) & (uint32_t)mq;
+            if (fq != lastf && fq >= w0 && fq < w1) s_win[(fq - w0) * bpe + bpe - 1] |= 4u;
+            lastf = fq;
           }
         }
       }
-    } else {
-      // long owned range (sparse stretch): window by window
-      for (uint32_t w0 = Llo; w0 < Lhi; w0 += wslots) {
-        const uint32_t w1 = w0 + wslots < Lhi ? w0 + wslots : Lhi;
-        uint32_t k0, k1, f0, f1;
-        {
-          uint32_t lo = 0, hi = tc;
-          while (lo < hi) { const uint32_t m = (lo + hi) >> 1; if (s_loc[m] < w0) lo = m + 1; else hi = m; }
-          k0 = lo;
-          hi = tc;
-          while (lo < hi) { const uint32_t m = (lo + hi) >> 1; if (s_loc[m] < w1) lo = m + 1; else hi = m; }
-          k1 = lo;
-          lo = 0;
-          hi = tf;
-          while (lo < hi) { const uint32_t m = (lo + hi) >> 1; if (s_ffq[m] < w0) lo = m + 1; else hi = m; }
-          f0 = lo;
-          hi = tf;
-          while (lo < hi) { const uint32_t m = (lo + hi) >> 1; if (s_ffq[m] < w1) lo = m + 1; else hi = m; }
-          f1 = lo;
-        }
-        if (precleared && k0 == k1 && f0 == f1) continue;  // nothing but zeros, already cleared
-        const uint64_t B0 = (uint64_t)w0 * bpe, B1 = (uint64_t)w1 * bpe;
-        const uint32_t nbytes = (uint32_t)(B1 - B0);
-        for (uint32_t i = tid; i < (nbytes + 3) / 4; i += RB_THREADS) reinterpret_cast<uint32_t*>(s_win)[i] = 0u;
-        __syncthreads();
-        for (uint32_t k = k0 + tid; k < k1; k += RB_THREADS) {
-          const uint32_t off = (s_loc[k] - w0) * bpe;
-          const uint32_t e = s_el[k];
-          for (uint32_t b = 0; b < bpe; ++b) s_win[off + b] = (uint8_t)(e >> (8 * (bpe - 1 - b)));
-        }
-        __syncthreads();
-        for (uint32_t f = f0 + tid; f < f1; f += RB_THREADS) s_win[(s_ffq[f] - w0) * bpe + bpe - 1] |= 4u;
-        __syncthreads();
-        const uint64_t A0 = (B0 + 3) & ~3ull, A1 = B1 & ~3ull;
-        if (A0 >= A1) {
-          for (uint64_t b = B0 + tid; b < B1; b += RB_THREADS) t[b] = s_win[b - B0];
-        } else {
-          for (uint64_t b = B0 + tid; b < A0; b += RB_THREADS) t[b] = s_win[b - B0];
-          for (uint64_t b = A1 + tid; b < B1; b += RB_THREADS) t[b] = s_win[b - B0];
-          uint32_t* tw = reinterpret_cast<uint32_t*>(t + A0);
-          const uint32_t nw = (uint32_t)((A1 - A0) / 4), sh = (uint32_t)(A0 - B0);
-          for (uint32_t w = tid; w < nw; w += RB_THREADS) {
-            const uint32_t q0 = sh + 4 * w;
-            tw[w] = (uint32_t)s_win[q0] | ((uint32_t)s_win[q0 + 1] << 8) | ((uint32_t)s_win[q0 + 2] << 16) |
-                    ((uint32_t)s_win[q0 + 3] << 24);
-          }
-        }
-        __syncthreads();
-      }
+      __syncthreads();
+      win_out(t, s_win, (uint64_t)w0 * bpe, (uint64_t)w1 * bpe);
+      __syncthreads();
     }
+    __syncthreads();
+    if (tid == 0) fix_cnt[u] = s_nfix < RB_FIX ? s_nfix : RB_FIX;
     X = Y;
     __syncthreads();
   }
@@ -847,18 +856,50 @@ __global__ __launch_bounds__(64) void k_sqf_fixup(uint8_t* __restrict__ t, uint3
 constexpr uint32_t DC_SLOTS = 4096;
 constexpr uint32_t DC_PEND = 1024;
 
+// The 16 slots [s, s+16) of a thread: 16-byte loads when whole, else
+// bounded byte loads (past the last slot reads as empty).
+template <int BPE>
+__device__ __forceinline__ void sqf_load16(const uint8_t* __restrict__ t, uint32_t S, uint32_t s, uint32_t* e) {
+  if (s + 16 <= S && (reinterpret_cast<uintptr_t>(t) & 15u) == 0) {
+    uint32_t w[4 * BPE];
+    const uint4* p = reinterpret_cast<const uint4*>(t + (uint64_t)s * BPE);
+#pragma unroll
+    for (int k = 0; k < BPE; ++k) {
+      const uint4 x = p[k];
+      w[4 * k] = x.x;
+      w[4 * k + 1] = x.y;
+      w[4 * k + 2] = x.z;
+      w[4 * k + 3] = x.w;
+    }
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      uint32_t v = 0;
+#pragma unroll
+      for (int b = 0; b < BPE; ++b) {
+        const int at = j * BPE + b;
+        v = (v << 8) | ((w[at >> 2] >> (8 * (at & 3))) & 0xFFu);
+      }
+      e[j] = v;
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < 16; ++j) e[j] = sqf_get<BPE>(t, S, s + j);
+  }
+}
+
 template <int BPE>
 __global__ __launch_bounds__(256) void k_sqf_count(const uint8_t* __restrict__ t, uint32_t S,
                                                    uint32_t* __restrict__ counts) {
   __shared__ uint64_t s_tmp[4];
   const uint32_t s0 = blockIdx.x * DC_SLOTS + threadIdx.x * 16;
+  uint32_t e[16];
+  sqf_load16<BPE>(t, S, s0, e);
   uint32_t ci = 0, cr = 0, co = 0;
 #pragma unroll
   for (int j = 0; j < 16; ++j) {
-    const uint32_t e = sqf_get<BPE>(t, S, s0 + j);
-    ci += (e & 7u) != 0;
-    cr += ((e & 7u) != 0) && !(e & 2u);
-    co += (e & 4u) != 0;
+    ci += (e[j] & 7u) != 0;
+    cr += ((e[j] & 7u) != 0) && !(e[j] & 2u);
+    co += (e[j] & 4u) != 0;
   }
   uint64_t tot;
   block256_excl_scan64((uint64_t)ci | ((uint64_t)cr << 21) | ((uint64_t)co << 42), s_tmp, &tot);
@@ -895,21 +936,24 @@ __global__ __launch_bounds__(256) void k_sqf_decode(const uint8_t* __restrict__ 
                                                     const uint32_t* __restrict__ bases, K* __restrict__ out) {
   __shared__ uint16_t s_occpos[DC_SLOTS];
   __shared__ uint32_t s_pend[DC_PEND];
+  __shared__ K s_out[DC_SLOTS];
   __shared__ uint64_t s_tmp[4];
   __shared__ uint32_t s_tmp32[4];
   const uint32_t tid = threadIdx.x;
   const uint32_t s0 = blockIdx.x * DC_SLOTS;
   const uint32_t bI = bases[blockIdx.x * 3 + 0], bR = bases[blockIdx.x * 3 + 1], bO = bases[blockIdx.x * 3 + 2];
   uint32_t e[16];
+  sqf_load16<BPE>(t, S, s0 + tid * 16, e);
   uint32_t ci = 0, cr = 0, co = 0;
 #pragma unroll
   for (int j = 0; j < 16; ++j) {
-    e[j] = sqf_get<BPE>(t, S, s0 + tid * 16 + j);
     ci += (e[j] & 7u) != 0;
     cr += ((e[j] & 7u) != 0) && !(e[j] & 2u);
     co += (e[j] & 4u) != 0;
   }
-  const uint64_t ex = block256_excl_scan64((uint64_t)ci | ((uint64_t)cr << 21) | ((uint64_t)co << 42), s_tmp, nullptr);
+  uint64_t btot;
+  const uint64_t ex = block256_excl_scan64((uint64_t)ci | ((uint64_t)cr << 21) | ((uint64_t)co << 42), s_tmp, &btot);
+  const uint32_t nout = (uint32_t)(btot & 0x1FFFFF);
   uint32_t xi = (uint32_t)(ex & 0x1FFFFF), xr = (uint32_t)((ex >> 21) & 0x1FFFFF), xo = (uint32_t)(ex >> 42);
   {
     uint32_t o = xo;
@@ -953,9 +997,11 @@ __global__ __launch_bounds__(256) void k_sqf_decode(const uint8_t* __restrict__ 
     } else {
       fq = sqf_walk_quotient<BPE>(t, S, s0 + tid * 16 + j);
     }
-    out[bI + xi] = (K)(((uint64_t)fq << r) | (el >> 3));
+    s_out[xi] = (K)(((uint64_t)fq << r) | (el >> 3));
     xi++;
   }
+  __syncthreads();
+  for (uint32_t k = tid; k < nout; k += 256) out[bI + k] = s_out[k];
 }
 
 // ---------------------------------------------------------------- pipelines
@@ -992,7 +1038,7 @@ static gqf_status sqf_decode(hipStream_t s, DeviceState& ds, Arena& ar, const gq
   else if (g.bpe == 3) GQF_LAUNCH_B((double)g.bytes, k_sqf_count<3>, dim3(regions), dim3(256), 0, s, qf.table, g.S, counts);
   else GQF_LAUNCH_B((double)g.bytes, k_sqf_count<4>, dim3(regions), dim3(256), 0, s, qf.table, g.S, counts);
   // totals: column 0 (entries) is the decoded count
-  GQF_LAUNCH(k_scan_rows, dim3(1), dim3(1024), 0, s, counts, regions, 3u, count_dev);
+  GQF_LAUNCH(k_scan_rows<3>, dim3(1), dim3(1024), 0, s, counts, regions, count_dev);
   GQF_TRY(read_counter(s, ds, count_dev, count));
   const double dec_bytes = (double)g.bytes + 4.0 * (double)*count;
   if (g.bpe == 1)
@@ -1026,7 +1072,7 @@ static void clear_table(hipStream_t s, const gqf_sqf& qf, const SqfGeom& g) {
 
 static size_t rebuild_ws_bytes(uint64_t N) {
   const uint64_t tiles = (N + RB_TILE - 1) / RB_TILE;
-  return aligned_bytes((tiles + 1) * 4) + aligned_bytes(2048 * 12) + aligned_bytes(tiles * RB_FIX * 4) +
+  return aligned_bytes((tiles + 1) * 4) + aligned_bytes((size_t)RB_MAX_CHUNKS * 16) + 1024 + aligned_bytes(tiles * RB_FIX * 4) +
          aligned_bytes(tiles * 4) + aligned_bytes((N + 1) * 4) + 2048;
 }
 
@@ -1045,6 +1091,7 @@ static gqf_status launch_rebuild(hipStream_t s, DeviceState& ds, Arena& ar, cons
   uint32_t* splits = ar.take<uint32_t>(p.nsub + 1);
   uint32_t* agg_c = ar.take<uint32_t>(p.G);
   long long* agg_l = ar.take<long long>(p.G);
+  uint32_t* agg_f = ar.take<uint32_t>(p.G);
   uint32_t* fix = ar.take<uint32_t>((size_t)p.nsub * RB_FIX);
   uint32_t* fix_cnt = ar.take<uint32_t>(p.nsub);
   uint32_t* fix_more = ar.take<uint32_t>(N + 1);
@@ -1055,14 +1102,20 @@ static gqf_status launch_rebuild(hipStream_t s, DeviceState& ds, Arena& ar, cons
   if (MERGE)
     GQF_LAUNCH(k_rb_splits, dim3((p.nsub + 1 + 255) / 256), dim3(256), 0, s, A, na, B, nb, p.nsub, splits);
   GQF_LAUNCH_B((double)N * 4, (k_rb_agg<MODE, MERGE>), dim3(p.G), dim3(RB_THREADS), 0, s, A, na, B, nb, flags, splits,
-               g.q, g.r, p.chunk, agg_c, agg_l);
-  GQF_LAUNCH(k_rb_scan, dim3(1), dim3(1024), 0, s, agg_c, agg_l, p.G, (uint64_t)g.S, ds.counters);
+               g.q, g.r, p.chunk, agg_c, agg_l, agg_f);
+  GQF_LAUNCH(k_rb_scan, dim3(1), dim3(1024), 0, s, agg_c, agg_l, agg_f, p.G, (uint64_t)g.S, ds.counters);
   // sparse tables: clear once, then sub-tiles skip windows holding only zeros
   const int precleared = N * 8 < (uint64_t)g.S ? 1 : 0;
   if (precleared) GQF_LAUNCH(k_sqf_clear_tail, dim3(1024), dim3(256), 0, s, qf.table, g.bpe, g.bytes, ds.counters, 1);
-  GQF_LAUNCH_B((double)N * 4 + (double)g.bytes, (k_rb_write<MODE, MERGE>), dim3(p.G), dim3(RB_THREADS), 0, s,
-               qf.table, g.S, g.q, g.r, g.bpe, A, na, B, nb, flags, splits, p.chunk, agg_c, agg_l, ds.counters, fix,
-               fix_cnt, fix_more, ds.counters + CTR_MISC + 2, (uint32_t)(N + 1), precleared);
+#define GQF_RB_WRITE(BPE_)                                                                                       \
+  GQF_LAUNCH_B((double)N * 4 + (double)g.bytes, (k_rb_write<MODE, MERGE, BPE_>), dim3(p.G), dim3(RB_THREADS), 0, s, \
+               qf.table, g.S, g.q, g.r, A, na, B, nb, flags, splits, p.chunk, agg_c, agg_l, agg_f, ds.counters, fix,  \
+               fix_cnt, fix_more, ds.counters + CTR_MISC + 2, (uint32_t)(N + 1), precleared)
+  if (g.bpe == 1) GQF_RB_WRITE(1);
+  else if (g.bpe == 2) GQF_RB_WRITE(2);
+  else if (g.bpe == 3) GQF_RB_WRITE(3);
+  else GQF_RB_WRITE(4);
+#undef GQF_RB_WRITE
   if (!precleared) GQF_LAUNCH(k_sqf_clear_tail, dim3(1024), dim3(256), 0, s, qf.table, g.bpe, g.bytes, ds.counters, 0);
   GQF_LAUNCH(k_sqf_fixup, dim3(p.nsub + 1), dim3(64), 0, s, qf.table, g.bpe, fix, fix_cnt, fix_more, ds.counters,
              p.nsub, (uint32_t)(N + 1));
