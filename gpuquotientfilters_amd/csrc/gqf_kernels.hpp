@@ -64,15 +64,54 @@ __device__ __forceinline__ void st_store(unsigned long long* p, unsigned long lo
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// ---------------------------------------------------------------- wave64 scans
+// Inclusive scans across the 64 lanes of a wave with DPP moves: row_shr 1, 2,
+// 4, 8 inside each 16-lane row, then row_bcast:15 (rows 1, 3) and
+// row_bcast:31 (rows 2, 3); wave_shr:1 turns an inclusive scan into an
+// exclusive one.  A lane with no source lane receives the identity.  Every
+// lane of the wave must be active.
+constexpr int DPP_ROW_SHR = 0x110;  // | shift (1..15)
+constexpr int DPP_WAVE_SHR1 = 0x138;
+constexpr int DPP_ROW_BCAST15 = 0x142;
+constexpr int DPP_ROW_BCAST31 = 0x143;
+
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t ident, uint32_t x) {
+  return (uint32_t)__builtin_amdgcn_update_dpp((int)ident, (int)x, CTRL, ROW_MASK, 0xF, false);
+}
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ uint64_t dpp_u64(uint64_t ident, uint64_t x) {
+  const uint32_t lo = dpp_u32<CTRL, ROW_MASK>((uint32_t)ident, (uint32_t)x);
+  const uint32_t hi = dpp_u32<CTRL, ROW_MASK>((uint32_t)(ident >> 32), (uint32_t)(x >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ uint32_t wave_incl_add(uint32_t x) {
+  x += dpp_u32<DPP_ROW_SHR | 1>(0u, x);
+  x += dpp_u32<DPP_ROW_SHR | 2>(0u, x);
+  x += dpp_u32<DPP_ROW_SHR | 4>(0u, x);
+  x += dpp_u32<DPP_ROW_SHR | 8>(0u, x);
+  x += dpp_u32<DPP_ROW_BCAST15, 0xA>(0u, x);
+  x += dpp_u32<DPP_ROW_BCAST31, 0xC>(0u, x);
+  return x;
+}
+__device__ __forceinline__ uint64_t wave_incl_add64(uint64_t x) {
+  x += dpp_u64<DPP_ROW_SHR | 1>(0ull, x);
+  x += dpp_u64<DPP_ROW_SHR | 2>(0ull, x);
+  x += dpp_u64<DPP_ROW_SHR | 4>(0ull, x);
+  x += dpp_u64<DPP_ROW_SHR | 8>(0ull, x);
+  x += dpp_u64<DPP_ROW_BCAST15, 0xA>(0ull, x);
+  x += dpp_u64<DPP_ROW_BCAST31, 0xC>(0ull, x);
+  return x;
+}
+__device__ __forceinline__ uint32_t wave_read(uint32_t x, int lane) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)x, lane);
+}
+
 // Exclusive scan of one uint32 per thread over a 256-thread block.
 __device__ __forceinline__ uint32_t block256_excl_scan(uint32_t x, uint32_t* s_tmp4, uint32_t* total) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  uint32_t inc = x;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    uint32_t y = __shfl_up(inc, d, 64);
-    if (lane >= d) inc += y;
-  }
+  const uint32_t inc = wave_incl_add(x);
   if (lane == 63) s_tmp4[w] = inc;
   __syncthreads();
   uint32_t pre = 0, tot = 0;
@@ -90,12 +129,7 @@ __device__ __forceinline__ uint32_t block256_excl_scan(uint32_t x, uint32_t* s_t
 // Same for uint64 (used with packed counters).
 __device__ __forceinline__ uint64_t block256_excl_scan64(uint64_t x, uint64_t* s_tmp4, uint64_t* total) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  uint64_t inc = x;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    uint64_t y = __shfl_up(inc, d, 64);
-    if (lane >= d) inc += y;
-  }
+  const uint64_t inc = wave_incl_add64(x);
   if (lane == 63) s_tmp4[w] = inc;
   __syncthreads();
   uint64_t pre = 0, tot = 0;
@@ -236,12 +270,7 @@ static __global__ __launch_bounds__(1024) void k_rs_rowscan(uint32_t* __restrict
   for (uint32_t b = 0; b < G; b += 1024) {
     const uint32_t i = b + tid;
     const uint32_t x = i < G ? row[i] : 0u;
-    uint32_t inc = x;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const uint32_t y = __shfl_up(inc, d, 64);
-      if (lane >= d) inc += y;
-    }
+    const uint32_t inc = wave_incl_add(x);
     if (lane == 63) s_w[wid] = inc;
     __syncthreads();
     uint32_t pre = 0, tot = 0;
@@ -891,12 +920,7 @@ __global__ __launch_bounds__(1024) void k_scan_rows(uint32_t* __restrict__ data,
     uint32_t pre[COLS];
 #pragma unroll
     for (int c = 0; c < COLS; ++c) {
-      uint32_t inc = sum[c];
-#pragma unroll
-      for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t y = __shfl_up(inc, d, 64);
-        if (lane >= d) inc += y;
-      }
+      const uint32_t inc = wave_incl_add(sum[c]);
       if (lane == 63) s_w[c][wid] = inc;
       pre[c] = inc - sum[c];
     }

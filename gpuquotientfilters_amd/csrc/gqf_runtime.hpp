@@ -113,6 +113,8 @@ struct CallScope {
   DeviceState* ds;
   hipStream_t stream;
   gqf_status begin(gqf_stream s);
+  // Waits for the call's work; reports the device time since begin() and, when
+  // check_device_err, a capacity error raised by the kernels.
   gqf_status end(float* ms, bool check_device_err);
 };
 

@@ -312,28 +312,35 @@ __device__ __forceinline__ RS rs_combine(RS a, RS b) {
   return o;
 }
 __device__ __forceinline__ RS rs_none() { return RS{0u, LOC_NONE, F_NONE}; }
+// Stored form of a state: {c, l + 1 (0 = none), f, 0}.
+__device__ __forceinline__ uint4 rs_pack(RS x) {
+  return make_uint4(x.c, x.l == LOC_NONE ? 0u : (uint32_t)(x.l + 1), x.f, 0u);
+}
+__device__ __forceinline__ RS rs_unpack(uint4 w) { return RS{w.x, w.y == 0 ? LOC_NONE : (int64_t)w.y - 1, w.z}; }
 
-// Exclusive scan across the 64 lanes of a wave; *total = the wave aggregate.
+// Exclusive scan across the 64 lanes of a wave (DPP moves, see
+// wave_incl_add); *total = the wave aggregate.
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ RS rs_dpp(RS x) {
+  RS y;
+  y.c = dpp_u32<CTRL, ROW_MASK>(0u, x.c);
+  y.l = (int64_t)dpp_u64<CTRL, ROW_MASK>((uint64_t)LOC_NONE, (uint64_t)x.l);
+  y.f = dpp_u32<CTRL, ROW_MASK>(F_NONE, x.f);
+  return y;
+}
 __device__ __forceinline__ RS rs_wave_excl(RS x, RS* total) {
-  const int lane = threadIdx.x & 63;
   RS inc = x;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    RS y;
-    y.c = __shfl_up(inc.c, d, 64);
-    y.l = __shfl_up(inc.l, d, 64);
-    y.f = __shfl_up(inc.f, d, 64);
-    if (lane >= d) inc = rs_combine(y, inc);
-  }
-  RS ex;
-  ex.c = __shfl_up(inc.c, 1, 64);
-  ex.l = __shfl_up(inc.l, 1, 64);
-  ex.f = __shfl_up(inc.f, 1, 64);
-  if (lane == 0) ex = rs_none();
-  total->c = __shfl(inc.c, 63, 64);
-  total->l = __shfl(inc.l, 63, 64);
-  total->f = __shfl(inc.f, 63, 64);
-  return ex;
+  inc = rs_combine(rs_dpp<DPP_ROW_SHR | 1>(inc), inc);
+  inc = rs_combine(rs_dpp<DPP_ROW_SHR | 2>(inc), inc);
+  inc = rs_combine(rs_dpp<DPP_ROW_SHR | 4>(inc), inc);
+  inc = rs_combine(rs_dpp<DPP_ROW_SHR | 8>(inc), inc);
+  inc = rs_combine(rs_dpp<DPP_ROW_BCAST15, 0xA>(inc), inc);
+  inc = rs_combine(rs_dpp<DPP_ROW_BCAST31, 0xC>(inc), inc);
+  total->c = wave_read(inc.c, 63);
+  total->l = (int64_t)(((uint64_t)wave_read((uint32_t)((uint64_t)inc.l >> 32), 63) << 32) |
+                       wave_read((uint32_t)inc.l, 63));
+  total->f = wave_read(inc.f, 63);
+  return rs_dpp<DPP_WAVE_SHR1>(inc);
 }
 
 // Merge-path split (A entries among the first u*RB_TILE outputs) of every sub-tile boundary.
@@ -486,26 +493,25 @@ static inline RbPlan rb_plan(uint64_t N) {
   return p;
 }
 
-// Block-wide aggregate of the per-thread states; s_* hold one entry per wave.
-__device__ __forceinline__ RS rs_block(RS a, RS* ex_out, uint32_t* s_c, long long* s_l, uint32_t* s_f) {
+// Block-wide scan of the per-thread states: *ex_out = exclusive prefix of the
+// calling thread within the block, returns the block aggregate.  s_wave:
+// RB_WAVES + 1 entries (wave aggregates, then their prefixes and the total).
+__device__ __forceinline__ RS rs_block(RS a, RS* ex_out, uint4* s_wave) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   RS wt;
   const RS ex = rs_wave_excl(a, &wt);
-  if (lane == 0) {
-    s_c[wid] = wt.c;
-    s_l[wid] = wt.l;
-    s_f[wid] = wt.f;
+  if (lane == 0) s_wave[wid] = rs_pack(wt);
+  __syncthreads();
+  if (wid == 0) {
+    const RS x = lane < RB_WAVES ? rs_unpack(s_wave[lane]) : rs_none();
+    RS tot;
+    const RS xe = rs_wave_excl(x, &tot);
+    if (lane < RB_WAVES) s_wave[lane] = rs_pack(xe);
+    if (lane == 0) s_wave[RB_WAVES] = rs_pack(tot);
   }
   __syncthreads();
-  RS wpre = rs_none(), sub = rs_none();
-#pragma unroll
-  for (int w = 0; w < RB_WAVES; ++w) {
-    const RS wv = {s_c[w], (int64_t)s_l[w], s_f[w]};
-    if (w < wid) wpre = rs_combine(wpre, wv);
-    sub = rs_combine(sub, wv);
-  }
-  *ex_out = rs_combine(wpre, ex);
-  return sub;
+  *ex_out = rs_combine(rs_unpack(s_wave[wid]), ex);
+  return rs_unpack(s_wave[RB_WAVES]);
 }
 
 template <int MODE, bool MERGE>
@@ -513,11 +519,9 @@ __global__ __launch_bounds__(RB_THREADS, 8) void k_rb_agg(const uint32_t* __rest
                                                 const uint32_t* __restrict__ B, uint32_t nb,
                                                 const uint8_t* __restrict__ flags, const uint32_t* __restrict__ splits,
                                                 uint32_t q, uint32_t r, uint32_t chunk,
-                                                uint32_t* __restrict__ agg_c, long long* __restrict__ agg_l,
-                                                uint32_t* __restrict__ agg_f) {
+                                                uint4* __restrict__ agg) {
   __shared__ uint32_t s_stage[MERGE ? RB_TILE : 1];
-  __shared__ uint32_t s_c[RB_WAVES], s_f[RB_WAVES];
-  __shared__ long long s_l[RB_WAVES];
+  __shared__ uint4 s_wave[RB_WAVES + 1];
   const uint64_t N = (uint64_t)na + (MERGE ? nb : 0u);
   const uint64_t mq = mask64(q);
   const uint64_t c0 = (uint64_t)blockIdx.x * chunk;
@@ -531,85 +535,56 @@ __global__ __launch_bounds__(RB_THREADS, 8) void k_rb_agg(const uint32_t* __rest
     RS a;
     (void)rb_classify<MODE>(v, tags, fl, (uint32_t)d0, tn, r, mq, a);
     RS ex;
-    const RS sub = rs_block(a, &ex, s_c, s_l, s_f);
+    const RS sub = rs_block(a, &ex, s_wave);
     X = rs_combine(X, sub);
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    agg_c[blockIdx.x] = X.c;
-    agg_l[blockIdx.x] = X.l;
-    agg_f[blockIdx.x] = X.f;
-  }
+  if (threadIdx.x == 0) agg[blockIdx.x] = rs_pack(X);
 }
 
-// Exclusive scan of the chunk states (one workgroup of 1024 threads, in
-// pieces of 2048 states staged through LDS; 2 consecutive states per thread).
-// The total fixes the kept count, the first slot after the last entry, and a
-// capacity error -- all before any table byte is written.
-__global__ __launch_bounds__(1024) void k_rb_scan(uint32_t* __restrict__ agg_c, long long* __restrict__ agg_l,
-                                                  uint32_t* __restrict__ agg_f, uint32_t G, uint64_t S,
+// Exclusive scan of the chunk states by one workgroup of 1024 threads: in
+// pieces of 8192 states, each thread combines its 8 consecutive states, one
+// block-wide scan, then the thread rewrites its 8 states as exclusive
+// prefixes.  The total fixes the kept count, the first
+// slot after the last entry, and a capacity error -- all before any table
+// byte is written.
+__global__ __launch_bounds__(1024) void k_rb_scan(uint4* __restrict__ agg, uint32_t G, uint64_t S,
                                                   uint32_t* __restrict__ ctr) {
-  constexpr uint32_t PIECE = 2048;
-  __shared__ uint32_t s_c[PIECE], s_f[PIECE];
-  __shared__ long long s_l[PIECE];
-  __shared__ uint32_t s_wc[16], s_wf[16];
-  __shared__ int64_t s_wl[16];
+  constexpr uint32_t SEG = 8, PIECE = 1024 * SEG;
+  __shared__ uint4 s_wave[16];  // wave aggregates, then their exclusive prefixes
+  __shared__ uint4 s_tot;
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   RS carry = rs_none();
   for (uint32_t base = 0; base < G; base += PIECE) {
-    const uint32_t n = G - base < PIECE ? G - base : PIECE;
+    const uint32_t i0 = base + (uint32_t)tid * SEG;
+    uint4 w[SEG];
 #pragma unroll
-    for (int k = 0; k < 2; ++k) {
-      const uint32_t i = tid + k * 1024;
-      if (i < n) {
-        s_c[i] = agg_c[base + i];
-        s_l[i] = agg_l[base + i];
-        s_f[i] = agg_f[base + i];
-      }
-    }
-    __syncthreads();
-    RS x0 = rs_none(), x1 = rs_none();
-    const uint32_t i0 = 2 * tid;
-    if (i0 < n) x0 = RS{s_c[i0], (int64_t)s_l[i0], s_f[i0]};
-    if (i0 + 1 < n) x1 = RS{s_c[i0 + 1], (int64_t)s_l[i0 + 1], s_f[i0 + 1]};
+    for (uint32_t k = 0; k < SEG; ++k) w[k] = i0 + k < G ? agg[i0 + k] : make_uint4(0u, 0u, F_NONE, 0u);
+    RS seg = rs_none();
+#pragma unroll
+    for (uint32_t k = 0; k < SEG; ++k) seg = rs_combine(seg, rs_unpack(w[k]));
     RS wt;
-    RS ex = rs_wave_excl(rs_combine(x0, x1), &wt);
-    if (lane == 0) {
-      s_wc[wid] = wt.c;
-      s_wl[wid] = wt.l;
-      s_wf[wid] = wt.f;
+    RS ex = rs_wave_excl(seg, &wt);
+    if (lane == 0) s_wave[wid] = rs_pack(wt);
+    __syncthreads();
+    if (wid == 0) {
+      // wave 0 scans the 16 wave aggregates on top of the carry
+      RS x = lane < 16 ? rs_unpack(s_wave[lane]) : rs_none();
+      RS tot;
+      RS xe = rs_wave_excl(x, &tot);
+      if (lane < 16) s_wave[lane] = rs_pack(rs_combine(carry, xe));
+      if (lane == 0) s_tot = rs_pack(rs_combine(carry, tot));
     }
     __syncthreads();
-    RS pre = carry, tot = carry;
+    ex = rs_combine(rs_unpack(s_wave[wid]), ex);
 #pragma unroll
-    for (int w = 0; w < 16; ++w) {
-      const RS wv = {s_wc[w], s_wl[w], s_wf[w]};
-      if (w < wid) pre = rs_combine(pre, wv);
-      tot = rs_combine(tot, wv);
-    }
-    ex = rs_combine(pre, ex);
-    if (i0 < n) {
-      s_c[i0] = ex.c;
-      s_l[i0] = ex.l;
-      s_f[i0] = ex.f;
-      ex = rs_combine(ex, x0);
-    }
-    if (i0 + 1 < n) {
-      s_c[i0 + 1] = ex.c;
-      s_l[i0 + 1] = ex.l;
-      s_f[i0 + 1] = ex.f;
-    }
-    carry = tot;
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < 2; ++k) {
-      const uint32_t i = tid + k * 1024;
-      if (i < n) {
-        agg_c[base + i] = s_c[i];
-        agg_l[base + i] = s_l[i];
-        agg_f[base + i] = s_f[i];
+    for (uint32_t k = 0; k < SEG; ++k) {
+      if (i0 + k < G) {
+        agg[i0 + k] = rs_pack(ex);
+        ex = rs_combine(ex, rs_unpack(w[k]));
       }
     }
+    carry = rs_unpack(s_tot);
     __syncthreads();
   }
   if (tid == 0) {
@@ -655,44 +630,26 @@ __device__ __forceinline__ void win_out(uint8_t* __restrict__ t, const uint8_t* 
 // run head's quotient is a different slot, so plain byte updates do not
 // race); those whose quotient precedes the owned range go to the fix-up
 // list.  A long owned range (sparse table) is written window by window.
-template <int MODE, bool MERGE, int BPE>
-__global__ __launch_bounds__(RB_THREADS, 4) void k_rb_write(uint8_t* __restrict__ t, uint32_t S, uint32_t q, uint32_t r,
-                                                  const uint32_t* __restrict__ A, uint32_t na,
-                                                  const uint32_t* __restrict__ B, uint32_t nb,
-                                                  const uint8_t* __restrict__ flags,
-                                                  const uint32_t* __restrict__ splits, uint32_t chunk,
-                                                  const uint32_t* __restrict__ pre_c, const long long* __restrict__ pre_l,
-                                                  const uint32_t* __restrict__ pre_f,
-                                                  const uint32_t* __restrict__ ctr, uint32_t* __restrict__ fix,
-                                                  uint32_t* __restrict__ fix_cnt, uint32_t* __restrict__ fix_more,
-                                                  uint32_t* __restrict__ fix_more_n, uint32_t fix_cap,
-                                                  int precleared) {
-  if (ctr[CTR_ERR]) return;  // capacity error: the table stays as it was
-  __shared__ uint32_t s_stage[MERGE ? RB_TILE : 1];
-  __shared__ __align__(16) uint8_t s_win[RB_WIN];
-  __shared__ uint32_t s_c[RB_WAVES], s_f[RB_WAVES];
-  __shared__ long long s_l[RB_WAVES];
-  __shared__ uint32_t s_nfix;
+// Writes one sub-tile: block scan of the per-thread states (v, keepbits,
+// a) on top of the incoming state X, fix-up entries for run heads whose
+// quotient precedes the owned range, then the owned slot range window by
+// window.  Every thread of the block calls it; returns the outgoing state.
+template <int BPE>
+__device__ __forceinline__ RS rb_emit(uint8_t* __restrict__ t, uint32_t q, uint32_t r, const uint32_t* v,
+                                      uint32_t keepbits, RS a, RS X, uint32_t u, uint4* s_wave,
+                                      uint8_t* s_win, uint32_t* s_nfix_p,
+                                      uint32_t* __restrict__ fix, uint32_t* __restrict__ fix_cnt,
+                                      uint32_t* __restrict__ fix_more, uint32_t* __restrict__ fix_more_n,
+                                      uint32_t fix_cap, int precleared) {
   const int tid = threadIdx.x;
-  const uint64_t N = (uint64_t)na + (MERGE ? nb : 0u);
   const uint64_t mq = mask64(q);
   const uint32_t rmask = (1u << r) - 1u;
-  const uint64_t c0 = (uint64_t)blockIdx.x * chunk;
-  const uint64_t c1 = c0 + chunk < N ? c0 + chunk : N;
-  RS X = {pre_c[blockIdx.x], (int64_t)pre_l[blockIdx.x], pre_f[blockIdx.x]};
   constexpr uint32_t bpe = BPE;
   const uint32_t wslots = RB_WIN / BPE;
-  for (uint64_t d0 = c0; d0 < c1; d0 += RB_TILE) {
-    const uint32_t u = (uint32_t)(d0 / RB_TILE);
-    const uint32_t tn = (uint32_t)(c1 - d0 < (uint64_t)RB_TILE ? c1 - d0 : (uint64_t)RB_TILE);
-    uint32_t v[RB_ITEMS + 1];
-    uint32_t tags, fl;
-    rb_load<MODE, MERGE>(A, B, flags, splits, u, (uint32_t)d0, tn, s_stage, v, tags, fl);
-    RS a;
-    const uint32_t keepbits = rb_classify<MODE>(v, tags, fl, (uint32_t)d0, tn, r, mq, a);
+  uint32_t& s_nfix = *s_nfix_p;
     if (tid == 0) s_nfix = 0;
     RS ex;
-    const RS sub = rs_block(a, &ex, s_c, s_l, s_f);
+    const RS sub = rs_block(a, &ex, s_wave);
     const RS me = rs_combine(X, ex);  // state before this thread's entries
     const RS Y = rs_combine(X, sub);
     const uint32_t Llo = X.l == LOC_NONE ? 0u : (uint32_t)(X.l + 1);
@@ -795,18 +752,50 @@ __global__ __launch_bounds__(RB_THREADS, 4) void k_rb_write(uint8_t* __restrict_
     }
     __syncthreads();
     if (tid == 0) fix_cnt[u] = s_nfix < RB_FIX ? s_nfix : RB_FIX;
-    X = Y;
+  return Y;
+}
+
+template <int MODE, bool MERGE, int BPE>
+__global__ __launch_bounds__(RB_THREADS, 4) void k_rb_write(uint8_t* __restrict__ t, uint32_t S, uint32_t q, uint32_t r,
+                                                  const uint32_t* __restrict__ A, uint32_t na,
+                                                  const uint32_t* __restrict__ B, uint32_t nb,
+                                                  const uint8_t* __restrict__ flags,
+                                                  const uint32_t* __restrict__ splits, uint32_t chunk,
+                                                  const uint4* __restrict__ pre,
+                                                  const uint32_t* __restrict__ ctr, uint32_t* __restrict__ fix,
+                                                  uint32_t* __restrict__ fix_cnt, uint32_t* __restrict__ fix_more,
+                                                  uint32_t* __restrict__ fix_more_n, uint32_t fix_cap,
+                                                  int precleared) {
+  if (ctr[CTR_ERR]) return;  // capacity error: the table stays as it was
+  __shared__ uint32_t s_stage[MERGE ? RB_TILE : 1];
+  __shared__ __align__(16) uint8_t s_win[RB_WIN];
+  __shared__ uint4 s_wave[RB_WAVES + 1];
+  __shared__ uint32_t s_nfix;
+  const uint64_t N = (uint64_t)na + (MERGE ? nb : 0u);
+  const uint64_t mq = mask64(q);
+  const uint64_t c0 = (uint64_t)blockIdx.x * chunk;
+  const uint64_t c1 = c0 + chunk < N ? c0 + chunk : N;
+  RS X = rs_unpack(pre[blockIdx.x]);
+  for (uint64_t d0 = c0; d0 < c1; d0 += RB_TILE) {
+    const uint32_t u = (uint32_t)(d0 / RB_TILE);
+    const uint32_t tn = (uint32_t)(c1 - d0 < (uint64_t)RB_TILE ? c1 - d0 : (uint64_t)RB_TILE);
+    uint32_t v[RB_ITEMS + 1];
+    uint32_t tags, fl;
+    rb_load<MODE, MERGE>(A, B, flags, splits, u, (uint32_t)d0, tn, s_stage, v, tags, fl);
+    RS a;
+    const uint32_t keepbits = rb_classify<MODE>(v, tags, fl, (uint32_t)d0, tn, r, mq, a);
+    X = rb_emit<BPE>(t, q, r, v, keepbits, a, X, u, s_wave, s_win, &s_nfix, fix, fix_cnt, fix_more,
+                     fix_more_n, fix_cap, precleared);
     __syncthreads();
   }
 }
-
 // Zeroes every table byte from the slot after the last entry to the end of
 // the allocation (the 10% overflow area, usually), in parallel.
 // (whole_table: every byte, used to pre-clear sparse tables).  Nothing is
 // cleared after a capacity error.
 __global__ __launch_bounds__(256) void k_sqf_clear_tail(uint8_t* __restrict__ t, uint32_t bpe, uint64_t tbytes,
                                                         const uint32_t* __restrict__ ctr, int whole_table) {
-  if (ctr[CTR_ERR]) return;
+  if (ctr[CTR_ERR]) return;  // capacity error: the table stays as it was
   const uint64_t b0 = whole_table ? 0ull : (uint64_t)ctr[CTR_MISC + 3] * bpe;
   const uint64_t a0 = (b0 + 15) & ~15ull, a1 = tbytes & ~15ull;
   const uint64_t tid = blockIdx.x * 256ull + threadIdx.x, nth = (uint64_t)gridDim.x * 256;
@@ -835,7 +824,7 @@ __global__ __launch_bounds__(64) void k_sqf_fixup(uint8_t* __restrict__ t, uint3
                                                   const uint32_t* __restrict__ fix_more,
                                                   const uint32_t* __restrict__ ctr, uint32_t tiles,
                                                   uint32_t fix_cap) {
-  if (ctr[CTR_ERR]) return;
+  if (ctr[CTR_ERR]) return;  // capacity error: the table stays as it was
   const uint32_t tile = blockIdx.x;
   if (tile < tiles) {
     const uint32_t c = fix_cnt[tile];
@@ -1082,16 +1071,14 @@ static gqf_status launch_rebuild(hipStream_t s, DeviceState& ds, Arena& ar, cons
                                  const uint32_t* A, uint32_t na, const uint32_t* B, uint32_t nb,
                                  const uint8_t* flags) {
   const uint64_t N = (uint64_t)na + (MERGE ? nb : 0u);
-  GQF_LAUNCH(k_reset_counters, dim3(1), dim3(64), 0, s, ds.counters, (int)CTR_SCAN_TILE, (int)CTR_MISC + 4);
+  GQF_LAUNCH(k_reset_counters, dim3(1), dim3(64), 0, s, ds.counters, (int)CTR_SCAN_TILE, (int)CTR_TOTAL);
   if (N == 0) {
     clear_table(s, qf, g);
     return GQF_OK;
   }
   const RbPlan p = rb_plan(N);
   uint32_t* splits = ar.take<uint32_t>(p.nsub + 1);
-  uint32_t* agg_c = ar.take<uint32_t>(p.G);
-  long long* agg_l = ar.take<long long>(p.G);
-  uint32_t* agg_f = ar.take<uint32_t>(p.G);
+  uint4* agg = ar.take<uint4>(p.G);
   uint32_t* fix = ar.take<uint32_t>((size_t)p.nsub * RB_FIX);
   uint32_t* fix_cnt = ar.take<uint32_t>(p.nsub);
   uint32_t* fix_more = ar.take<uint32_t>(N + 1);
@@ -1102,14 +1089,14 @@ static gqf_status launch_rebuild(hipStream_t s, DeviceState& ds, Arena& ar, cons
   if (MERGE)
     GQF_LAUNCH(k_rb_splits, dim3((p.nsub + 1 + 255) / 256), dim3(256), 0, s, A, na, B, nb, p.nsub, splits);
   GQF_LAUNCH_B((double)N * 4, (k_rb_agg<MODE, MERGE>), dim3(p.G), dim3(RB_THREADS), 0, s, A, na, B, nb, flags, splits,
-               g.q, g.r, p.chunk, agg_c, agg_l, agg_f);
-  GQF_LAUNCH(k_rb_scan, dim3(1), dim3(1024), 0, s, agg_c, agg_l, agg_f, p.G, (uint64_t)g.S, ds.counters);
+               g.q, g.r, p.chunk, agg);
+  GQF_LAUNCH(k_rb_scan, dim3(1), dim3(1024), 0, s, agg, p.G, (uint64_t)g.S, ds.counters);
   // sparse tables: clear once, then sub-tiles skip windows holding only zeros
   const int precleared = N * 8 < (uint64_t)g.S ? 1 : 0;
   if (precleared) GQF_LAUNCH(k_sqf_clear_tail, dim3(1024), dim3(256), 0, s, qf.table, g.bpe, g.bytes, ds.counters, 1);
 #define GQF_RB_WRITE(BPE_)                                                                                       \
   GQF_LAUNCH_B((double)N * 4 + (double)g.bytes, (k_rb_write<MODE, MERGE, BPE_>), dim3(p.G), dim3(RB_THREADS), 0, s, \
-               qf.table, g.S, g.q, g.r, A, na, B, nb, flags, splits, p.chunk, agg_c, agg_l, agg_f, ds.counters, fix,  \
+               qf.table, g.S, g.q, g.r, A, na, B, nb, flags, splits, p.chunk, agg, ds.counters, fix,  \
                fix_cnt, fix_more, ds.counters + CTR_MISC + 2, (uint32_t)(N + 1), precleared)
   if (g.bpe == 1) GQF_RB_WRITE(1);
   else if (g.bpe == 2) GQF_RB_WRITE(2);
@@ -1168,15 +1155,15 @@ static gqf_status sqf_insert_impl(const gqf_sqf& qf, int n, const uint32_t* d_ke
   GQF_TRY(cs.begin(stream));
   DeviceState& ds = *cs.ds;
   const uint32_t un = (uint32_t)n;
-  GQF_TRY(ws_reserve(ds, sqf_decode_bytes(g) + sort_workspace_bytes(un, 4, false) +
+  GQF_TRY(ws_reserve(ds, sort_workspace_bytes(un, 4, false) + sqf_decode_bytes(g) +
                              rebuild_ws_bytes((uint64_t)g.S + un)));
   Arena ar(ds.ws, ds.ws_cap);
-  uint32_t* A;
-  uint32_t m = 0;
-  GQF_TRY(sqf_decode(cs.stream, ds, ar, qf, g, &A, &m));
   uint32_t* B = nullptr;
   GQF_TRY(radix_sort<uint32_t>(cs.stream, ds, ar, d_keys, true, g.q + g.r, nullptr, false, un, 0, g.q + g.r, &B,
                                nullptr));
+  uint32_t* A;
+  uint32_t m = 0;
+  GQF_TRY(sqf_decode(cs.stream, ds, ar, qf, g, &A, &m));
   if (mode == KEEP_TAGGED_INSERT) return sqf_merge_into<KEEP_TAGGED_INSERT>(cs, ar, qf, g, A, m, B, un, ms);
   if (mode == KEEP_UNIQUE) return sqf_merge_into<KEEP_UNIQUE>(cs, ar, qf, g, A, m, B, un, ms);
   return sqf_merge_into<KEEP_ALL>(cs, ar, qf, g, A, m, B, un, ms);
@@ -1191,15 +1178,15 @@ static gqf_status sqf_delete_impl(const gqf_sqf& qf, int n, const uint32_t* d_ke
   GQF_TRY(cs.begin(stream));
   DeviceState& ds = *cs.ds;
   const uint32_t un = (uint32_t)n;
-  GQF_TRY(ws_reserve(ds, sqf_decode_bytes(g) + sort_workspace_bytes(un, 4, false) + aligned_bytes(g.S + 1) +
+  GQF_TRY(ws_reserve(ds, sort_workspace_bytes(un, 4, false) + sqf_decode_bytes(g) + aligned_bytes(g.S + 1) +
                              rebuild_ws_bytes(g.S)));
   Arena ar(ds.ws, ds.ws_cap);
-  uint32_t* A;
-  uint32_t m = 0;
-  GQF_TRY(sqf_decode(cs.stream, ds, ar, qf, g, &A, &m));
   uint32_t* D = nullptr;
   GQF_TRY(radix_sort<uint32_t>(cs.stream, ds, ar, d_keys, true, g.q + g.r, nullptr, false, un, 0, g.q + g.r, &D,
                                nullptr));
+  uint32_t* A;
+  uint32_t m = 0;
+  GQF_TRY(sqf_decode(cs.stream, ds, ar, qf, g, &A, &m));
   uint8_t* keep = ar.take<uint8_t>(m + 1);
   if (ar.used > ar.cap) {
     set_error("internal: delete workspace overflow");

@@ -128,6 +128,46 @@ def test_delete_multiset_on_dup_table():
     assert (qf.table_bytes() == t).all()
 
 
+def test_skewed_tile_insert_and_delete():
+    # every fingerprint in the first 2048 quotients: one cluster several times
+    # longer than its quotient range, spanning many rebuild sub-tiles (long
+    # fix-up lists, owned ranges far from their quotients)
+    q, r = 14, 5
+    pool = O.mt19937(21, 120000)
+    fq = O.fingerprints(pool, q, r) >> r
+    skew = pool[fq < 2048][:6000]
+    assert len(skew) == 6000
+    say("sqf skewed tile", q, r, len(skew))
+    qf = G.SQF(q, r)
+    qf.bulk_build(dev(skew[:3000]), NoDuplicates=True)
+    qf.insert(dev(skew[3000:]))
+    want = sqf_canon(skew, q, r, True)
+    assert (qf.table_bytes() == want).all()
+    d = skew[1000:4000]
+    qf.superclusterDeletes(dev(d))
+    O.sqf_delete_keys(want, q, r, d)
+    assert (qf.table_bytes() == want).all()
+
+
+@pytest.mark.parametrize("q", [12, 16])
+def test_delete_heavy_duplicates(q):
+    # many copies per fingerprint (runs of equal entries inside one thread's
+    # slots and across threads); deletes remove up to, and past, every copy
+    r = 5
+    keys = O.mt19937(31, O.key_count(0.05, q))
+    reps = np.array([1, 2, 3, 5, 8])[np.arange(len(keys)) % 5]
+    base = np.repeat(keys, reps)
+    say("sqf delete duplicates", q, len(base))
+    qf = G.SQF(q, r)
+    qf.bulk_build(dev(base), NoDuplicates=False)
+    t = sqf_canon(base, q, r, False)
+    assert (qf.table_bytes() == t).all()
+    d = np.concatenate([np.repeat(keys[::2], 2), np.repeat(keys[1::3], 7), keys[::5]])
+    qf.delete(dev(d))
+    O.sqf_delete_keys(t, q, r, d)
+    assert (qf.table_bytes() == t).all()
+
+
 @pytest.mark.parametrize("no_dup", [True, False])
 def test_insert_via_merge(no_dup):
     q, r = 15, 5
