@@ -178,6 +178,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", choices=["sqf", "rsqf"], default="sqf")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="first copy BYTES device-to-device three times (a plain copy kernel of known size, "
+                         "to scale rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (every box here has one GPU; "
@@ -188,6 +191,15 @@ def main():
     import gpuquotientfilters_amd as G
 
     assert G.device_count() > 0, "no GPU visible"
+    if args.copy_calibration:
+        a = G.DeviceArray.empty(args.copy_calibration // 4)
+        b = G.DeviceArray.empty(args.copy_calibration // 4)
+        a.fill(1)
+        for _ in range(3):
+            G.check(G.lib().gqf_memcpy_dtod(b.ptr, a.ptr, args.copy_calibration))
+        G.synchronize()
+        a.free()
+        b.free()
     if args.workload == "sqf":
         q, r = 26, 5
         d, keys = setup_sqf(G, q, r)
