@@ -63,6 +63,8 @@ _SIGS = {
     "gqf_memcpy_dtod": (_st, [_vp, _vp, C.c_size_t]),
     "gqf_memset": (_st, [_vp, C.c_int, C.c_size_t]),
     "gqf_synchronize": (_st, []),
+    "gqf_stream_create": (_st, [C.POINTER(_vp)]),
+    "gqf_stream_destroy": (_st, [_vp]),
     "gqf_profile_enable": (_st, [C.c_int]),
     "gqf_profile_reset": (_st, []),
     "gqf_profile_count": (C.c_int, []),
@@ -220,6 +222,25 @@ def _n(keys, n):
     if n is not None:
         return int(n)
     return len(keys)
+
+
+class Stream:
+    """A non-blocking HIP stream (default priority).  Calls on different
+    streams, from different threads, run concurrently; pass it as `stream=`."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        check(lib().gqf_stream_create(C.byref(h)))
+        self.h = h.value
+
+    @property
+    def _as_parameter_(self):
+        return C.c_void_p(self.h)
+
+    def destroy(self):
+        if self.h is not None:
+            check(lib().gqf_stream_destroy(self.h))
+            self.h = None
 
 
 class SQF:

@@ -495,7 +495,6 @@ gqf_status gqf_rsqf_destroy(gqf_rsqf* cqf) {
 
 gqf_status gqf_rsqf_bulk_build(gqf_rsqf cqf, int n, const uint32_t* d_keys, gqf_stream stream, float* ms) {
   GQF_TRY(rsqf_args(cqf, n, d_keys));
-  std::lock_guard<std::mutex> lk(api_mutex());
   CallScope cs;
   GQF_TRY(cs.begin(stream));
   if (cqf.qbits + cqf.rbits <= 32) GQF_TRY(rsqf_build_t<uint32_t>(cqf, (uint32_t)n, d_keys, cs));
@@ -505,7 +504,6 @@ gqf_status gqf_rsqf_bulk_build(gqf_rsqf cqf, int n, const uint32_t* d_keys, gqf_
 
 gqf_status gqf_rsqf_insert(gqf_rsqf cqf, int n, const uint32_t* d_keys, int32_t* d_out, gqf_stream stream, float* ms) {
   GQF_TRY(rsqf_args(cqf, n, d_keys));
-  std::lock_guard<std::mutex> lk(api_mutex());
   CallScope cs;
   GQF_TRY(cs.begin(stream));
   if (cqf.qbits + cqf.rbits <= 32) GQF_TRY(rsqf_insert_t<uint32_t>(cqf, (uint32_t)n, d_keys, cs));
@@ -516,7 +514,6 @@ gqf_status gqf_rsqf_insert(gqf_rsqf cqf, int n, const uint32_t* d_keys, int32_t*
 
 gqf_status gqf_rsqf_delete(gqf_rsqf cqf, int n, const uint32_t* d_keys, gqf_stream stream, float* ms) {
   GQF_TRY(rsqf_args(cqf, n, d_keys));
-  std::lock_guard<std::mutex> lk(api_mutex());
   CallScope cs;
   GQF_TRY(cs.begin(stream));
   if (cqf.qbits + cqf.rbits <= 32) GQF_TRY(rsqf_delete_t<uint32_t>(cqf, (uint32_t)n, d_keys, cs));
@@ -531,7 +528,6 @@ gqf_status gqf_rsqf_lookup(gqf_rsqf cqf, int sorted, int n, const uint32_t* d_ke
     set_error("output array is NULL");
     return GQF_ERR_INVALID;
   }
-  std::lock_guard<std::mutex> lk(api_mutex());
   CallScope cs;
   GQF_TRY(cs.begin(stream));
   uint32_t* out = reinterpret_cast<uint32_t*>(d_out);

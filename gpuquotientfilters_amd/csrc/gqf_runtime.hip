@@ -8,6 +8,8 @@
 #include <cstring>
 #include <random>
 #include <string>
+#include <atomic>
+#include <cstdlib>
 #include <vector>
 
 namespace gqf {
@@ -26,37 +28,43 @@ gqf_status hip_fail(hipError_t e, const char* what, const char* file, int line) 
   return e == hipErrorOutOfMemory ? GQF_ERR_NOMEM : GQF_ERR_HIP;
 }
 
-std::mutex& api_mutex() {
-  static std::mutex m;
-  return m;
-}
+// Contexts by (device, stream); the table only grows.
+static std::mutex g_ctx_mu;
+struct CtxEntry {
+  int device;
+  hipStream_t stream;
+  DeviceState* ds;
+};
+static std::vector<CtxEntry>* g_ctx = nullptr;
 
-static DeviceState g_states[64];
-
-gqf_status device_state(DeviceState** out) {
+gqf_status device_state(hipStream_t stream, DeviceState** out) {
   int dev = 0;
   GQF_HIP(hipGetDevice(&dev));
-  if (dev < 0 || dev >= 64) {
-    set_error("device index %d out of range", dev);
-    return GQF_ERR_INVALID;
-  }
-  DeviceState& ds = g_states[dev];
-  if (ds.device < 0) {
-    GQF_HIP(hipMalloc((void**)&ds.counters, CTR_TOTAL * sizeof(uint32_t)));
-    GQF_HIP(hipMemset(ds.counters, 0, CTR_TOTAL * sizeof(uint32_t)));
-    GQF_HIP(hipHostMalloc((void**)&ds.h_pinned, CTR_TOTAL * sizeof(uint32_t), hipHostMallocDefault));
-    GQF_HIP(hipEventCreate(&ds.ev0));
-    GQF_HIP(hipEventCreate(&ds.ev1));
-    ds.device = dev;
-  }
-  *out = &ds;
+  std::lock_guard<std::mutex> lk(g_ctx_mu);
+  if (!g_ctx) g_ctx = new std::vector<CtxEntry>();
+  for (const CtxEntry& e : *g_ctx)
+    if (e.device == dev && e.stream == stream) {
+      *out = e.ds;
+      return GQF_OK;
+    }
+  DeviceState* ds = new DeviceState();
+  GQF_HIP(hipMalloc((void**)&ds->counters, CTR_TOTAL * sizeof(uint32_t)));
+  GQF_HIP(hipMemset(ds->counters, 0, CTR_TOTAL * sizeof(uint32_t)));
+  GQF_HIP(hipHostMalloc((void**)&ds->h_pinned, CTR_TOTAL * sizeof(uint32_t), hipHostMallocDefault));
+  GQF_HIP(hipEventCreate(&ds->ev0));
+  GQF_HIP(hipEventCreate(&ds->ev1));
+  GQF_HIP(hipDeviceSynchronize());
+  ds->device = dev;
+  ds->stream = stream;
+  g_ctx->push_back({dev, stream, ds});
+  *out = ds;
   return GQF_OK;
 }
 
 gqf_status ws_reserve(DeviceState& ds, size_t bytes) {
   if (bytes <= ds.ws_cap) return GQF_OK;
   size_t cap = bytes + bytes / 4 + (1u << 20);
-  GQF_HIP(hipDeviceSynchronize());
+  GQF_HIP(hipStreamSynchronize(ds.stream));
   if (ds.ws) GQF_HIP(hipFree(ds.ws));
   ds.ws = nullptr;
   ds.ws_cap = 0;
@@ -68,51 +76,52 @@ gqf_status ws_reserve(DeviceState& ds, size_t bytes) {
 gqf_status status_reserve(DeviceState& ds, size_t words) {
   if (words <= ds.status_cap) return GQF_OK;
   size_t cap = words + words / 4 + 4096;
-  GQF_HIP(hipDeviceSynchronize());
+  GQF_HIP(hipStreamSynchronize(ds.stream));
   if (ds.status) GQF_HIP(hipFree(ds.status));
   ds.status = nullptr;
   ds.status_cap = 0;
   GQF_HIP(hipMalloc((void**)&ds.status, cap * sizeof(unsigned long long)));
   // Tagged look-back words must never match a live epoch by accident.
-  GQF_HIP(hipMemset(ds.status, 0, cap * sizeof(unsigned long long)));
-  GQF_HIP(hipDeviceSynchronize());
+  GQF_HIP(hipMemsetAsync(ds.status, 0, cap * sizeof(unsigned long long), ds.stream));
+  GQF_HIP(hipStreamSynchronize(ds.stream));
   ds.status_cap = cap;
   return GQF_OK;
 }
 
 // ---------------------------------------------------------------- kernel timing
-struct ProfRec {
-  const char* name;
-  hipEvent_t a, b;
-  double bytes;
-};
 struct ProfStat {
   std::string name;
   uint64_t launches;
   double ms;
   double bytes;
 };
-static bool g_prof = false;
-static std::vector<hipEvent_t> g_pool;
-static size_t g_pool_used = 0;
-static std::vector<ProfRec> g_pending;
+static std::atomic<bool> g_prof{false};
+static std::mutex g_prof_mu;  // guards g_stats
 static std::vector<ProfStat> g_stats;
+static thread_local DeviceState* t_ctx = nullptr;  // context of the call running on this thread
 
-bool prof_enabled() { return g_prof; }
+bool prof_enabled() { return g_prof.load(std::memory_order_relaxed); }
 
-static hipEvent_t pool_event() {
-  if (g_pool_used == g_pool.size()) {
-    hipEvent_t e = nullptr;
-    if (hipEventCreate(&e) != hipSuccess) return nullptr;
-    g_pool.push_back(e);
+static hipEvent_t pool_event(DeviceState& ds) {
+  if (ds.pool_used == ds.pool_size) {
+    const size_t n = ds.pool_size ? 2 * ds.pool_size : 256;
+    hipEvent_t* p = static_cast<hipEvent_t*>(realloc(ds.pool, n * sizeof(hipEvent_t)));
+    if (!p) return nullptr;
+    ds.pool = p;
+    for (size_t i = ds.pool_size; i < n; ++i)
+      if (hipEventCreate(&ds.pool[i]) != hipSuccess) {
+        ds.pool_size = i;
+        return ds.pool_used < ds.pool_size ? ds.pool[ds.pool_used++] : nullptr;
+      }
+    ds.pool_size = n;
   }
-  return g_pool[g_pool_used++];
+  return ds.pool[ds.pool_used++];
 }
 
 ProfScope::ProfScope(const char* n, hipStream_t s, double algorithmic_bytes)
-    : name(n), stream(s), on(g_prof), bytes(algorithmic_bytes) {
+    : name(n), stream(s), on(prof_enabled() && t_ctx != nullptr), bytes(algorithmic_bytes) {
   if (on) {
-    a = pool_event();
+    a = pool_event(*t_ctx);
     if (a) (void)hipEventRecord(a, stream);
     else on = false;
   }
@@ -120,39 +129,61 @@ ProfScope::ProfScope(const char* n, hipStream_t s, double algorithmic_bytes)
 
 ProfScope::~ProfScope() {
   if (!on) return;
-  hipEvent_t b = pool_event();
+  DeviceState& ds = *t_ctx;
+  hipEvent_t b = pool_event(ds);
   if (!b) return;
   (void)hipEventRecord(b, stream);
-  g_pending.push_back({name, a, b, bytes});
+  if (ds.pending_used == ds.pending_size) {
+    const size_t n = ds.pending_size ? 2 * ds.pending_size : 256;
+    ProfRec* p = static_cast<ProfRec*>(realloc(ds.pending, n * sizeof(ProfRec)));
+    if (!p) return;
+    ds.pending = p;
+    ds.pending_size = n;
+  }
+  ds.pending[ds.pending_used++] = {name, a, b, bytes};
 }
 
-void prof_resolve() {
-  for (const ProfRec& r : g_pending) {
-    float ms = 0.f;
-    if (hipEventElapsedTime(&ms, r.a, r.b) != hipSuccess) continue;
-    std::string nm(r.name);
-    while (!nm.empty() && (nm.front() == '(' || nm.front() == ' ')) nm.erase(nm.begin());
-    while (!nm.empty() && (nm.back() == ')' || nm.back() == ' ')) nm.pop_back();
-    bool found = false;
-    for (ProfStat& st : g_stats)
-      if (st.name == nm) {
-        st.launches++;
-        st.ms += ms;
-        st.bytes += r.bytes;
-        found = true;
-        break;
-      }
-    if (!found) g_stats.push_back({nm, 1, (double)ms, r.bytes});
+void prof_resolve(DeviceState& ds) {
+  if (ds.pending_used) {
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    for (size_t i = 0; i < ds.pending_used; ++i) {
+      const ProfRec& r = ds.pending[i];
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, r.a, r.b) != hipSuccess) continue;
+      std::string nm(r.name);
+      while (!nm.empty() && (nm.front() == '(' || nm.front() == ' ')) nm.erase(nm.begin());
+      while (!nm.empty() && (nm.back() == ')' || nm.back() == ' ')) nm.pop_back();
+      bool found = false;
+      for (ProfStat& st : g_stats)
+        if (st.name == nm) {
+          st.launches++;
+          st.ms += ms;
+          st.bytes += r.bytes;
+          found = true;
+          break;
+        }
+      if (!found) g_stats.push_back({nm, 1, (double)ms, r.bytes});
+    }
   }
-  g_pending.clear();
-  g_pool_used = 0;
+  ds.pending_used = 0;
+  ds.pool_used = 0;
 }
 
 gqf_status CallScope::begin(gqf_stream s) {
-  GQF_TRY(device_state(&ds));
   stream = reinterpret_cast<hipStream_t>(s);
+  GQF_TRY(device_state(stream, &ds));
+  lock = std::unique_lock<std::mutex>(ds->mu);
+  t_ctx = ds;
   GQF_HIP(hipEventRecord(ds->ev0, stream));
   return GQF_OK;
+}
+
+CallScope::~CallScope() {
+  if (ds && t_ctx == ds) {
+    ds->pending_used = 0;  // a call that failed before end(): drop its timing records
+    ds->pool_used = 0;
+    t_ctx = nullptr;
+  }
 }
 
 gqf_status CallScope::end(float* ms, bool check_device_err) {
@@ -168,7 +199,7 @@ gqf_status CallScope::end(float* ms, bool check_device_err) {
     GQF_HIP(hipEventElapsedTime(&t, ds->ev0, ds->ev1));
     *ms = t;
   }
-  prof_resolve();
+  prof_resolve(*ds);
   if (check_device_err && (ds->h_pinned[CTR_ERR] & DEV_ERR_CAPACITY)) {
     set_error("capacity exceeded: the entries do not fit the table (table left unchanged)");
     return GQF_ERR_CAPACITY;
@@ -243,21 +274,24 @@ gqf_status gqf_memset(void* d_ptr, int value, size_t bytes) {
 }
 
 gqf_status gqf_profile_enable(int on) {
-  std::lock_guard<std::mutex> lk(api_mutex());
-  g_prof = on != 0;
+  g_prof.store(on != 0);
   return GQF_OK;
 }
 
 gqf_status gqf_profile_reset(void) {
-  std::lock_guard<std::mutex> lk(api_mutex());
+  std::lock_guard<std::mutex> lk(g_prof_mu);
   g_stats.clear();
   return GQF_OK;
 }
 
-int gqf_profile_count(void) { return (int)g_stats.size(); }
+int gqf_profile_count(void) {
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  return (int)g_stats.size();
+}
 
 gqf_status gqf_profile_get(int i, char* name, int name_cap, uint64_t* launches, double* total_ms,
                            double* total_bytes) {
+  std::lock_guard<std::mutex> lk(g_prof_mu);
   if (i < 0 || i >= (int)g_stats.size()) return GQF_ERR_INVALID;
   const ProfStat& st = g_stats[(size_t)i];
   if (name && name_cap > 0) {
@@ -277,6 +311,22 @@ void gqf_mt19937_fill(uint32_t seed, uint64_t skip, size_t n, uint32_t* host_out
 
 gqf_status gqf_synchronize(void) {
   GQF_HIP(hipDeviceSynchronize());
+  return GQF_OK;
+}
+
+gqf_status gqf_stream_create(gqf_stream* out) {
+  if (!out) return GQF_ERR_INVALID;
+  hipStream_t s = nullptr;
+  GQF_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  *out = reinterpret_cast<gqf_stream>(s);
+  return GQF_OK;
+}
+
+gqf_status gqf_stream_destroy(gqf_stream s) {
+  if (s) {
+    GQF_HIP(hipStreamSynchronize(reinterpret_cast<hipStream_t>(s)));
+    GQF_HIP(hipStreamDestroy(reinterpret_cast<hipStream_t>(s)));
+  }
   return GQF_OK;
 }
 

@@ -43,8 +43,20 @@ constexpr uint32_t DEV_ERR_CAPACITY = 1u;
 // Look-back words carry 30-bit counts: one call handles fewer entries.
 constexpr uint64_t GQF_MAX_ITEMS = (1ull << 30) - 4096;
 
+struct ProfRec {
+  const char* name;
+  hipEvent_t a, b;
+  double bytes;
+};
+
+// Execution context of one (device, stream) pair: workspace arena, device
+// counters and their pinned mirror, timing events.  Calls on the same stream
+// are serialized by the context's mutex; calls on different streams run
+// concurrently (each with its own workspace).
 struct DeviceState {
   int device = -1;
+  hipStream_t stream = nullptr;
+  std::mutex mu;
   char* ws = nullptr;               // workspace arena (grow-only)
   size_t ws_cap = 0;
   unsigned long long* status = nullptr;  // look-back status words, zeroed at allocation
@@ -53,12 +65,16 @@ struct DeviceState {
   uint32_t* h_pinned = nullptr;     // pinned host mirror of the counters
   uint32_t epoch = 0;               // radix-pass epoch (status words are tagged with it)
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // per-launch timing (gqf_profile_enable): event pool and launches to resolve
+  hipEvent_t* pool = nullptr;
+  size_t pool_size = 0, pool_used = 0;
+  ProfRec* pending = nullptr;
+  size_t pending_size = 0, pending_used = 0;
 };
 
-// Lock held by every entry point: calls on one process are serialized (the
-// reference API is single-threaded and blocking as well).
-std::mutex& api_mutex();
-gqf_status device_state(DeviceState** out);
+// The context of the calling thread's current device and `stream`, created
+// on first use (contexts live until the process ends).
+gqf_status device_state(hipStream_t stream, DeviceState** out);
 gqf_status ws_reserve(DeviceState& ds, size_t bytes);
 gqf_status status_reserve(DeviceState& ds, size_t words);
 
@@ -91,7 +107,7 @@ struct ProfScope {
   ProfScope(const char* n, hipStream_t s, double algorithmic_bytes = 0.0);
   ~ProfScope();
 };
-void prof_resolve();
+void prof_resolve(DeviceState& ds);
 
 #define GQF_LAUNCH(kernel, grid, block, shm, stream, ...)              \
   do {                                                                 \
@@ -107,15 +123,18 @@ void prof_resolve();
     hipLaunchKernelGGL(kernel, grid, block, shm, stream, __VA_ARGS__); \
   } while (0)
 
-// Timing/blocking scaffold: record start, run body, record stop, sync, read
-// the device error counter, report ms.
+// Timing/blocking scaffold of every entry point: takes the stream's context
+// (and its lock), records the start event; end() records the stop event,
+// waits, reads the device error counter and reports ms.
 struct CallScope {
-  DeviceState* ds;
-  hipStream_t stream;
+  DeviceState* ds = nullptr;
+  hipStream_t stream = nullptr;
+  std::unique_lock<std::mutex> lock;
   gqf_status begin(gqf_stream s);
   // Waits for the call's work; reports the device time since begin() and, when
   // check_device_err, a capacity error raised by the kernels.
   gqf_status end(float* ms, bool check_device_err);
+  ~CallScope();
 };
 
 }  // namespace gqf

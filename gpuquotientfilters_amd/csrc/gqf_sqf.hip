@@ -1121,7 +1121,6 @@ static gqf_status sqf_build_impl(const gqf_sqf& qf, int n, const uint32_t* d_key
                                  float* ms) {
   GQF_TRY(sqf_check(qf));
   GQF_TRY(check_keys(n, d_keys));
-  std::lock_guard<std::mutex> lk(api_mutex());
   const SqfGeom g = geom(qf);
   CallScope cs;
   GQF_TRY(cs.begin(stream));
@@ -1149,7 +1148,6 @@ static gqf_status sqf_insert_impl(const gqf_sqf& qf, int n, const uint32_t* d_ke
                                   float* ms) {
   GQF_TRY(sqf_check(qf));
   GQF_TRY(check_keys(n, d_keys));
-  std::lock_guard<std::mutex> lk(api_mutex());
   const SqfGeom g = geom(qf);
   CallScope cs;
   GQF_TRY(cs.begin(stream));
@@ -1172,7 +1170,6 @@ static gqf_status sqf_insert_impl(const gqf_sqf& qf, int n, const uint32_t* d_ke
 static gqf_status sqf_delete_impl(const gqf_sqf& qf, int n, const uint32_t* d_keys, gqf_stream stream, float* ms) {
   GQF_TRY(sqf_check(qf));
   GQF_TRY(check_keys(n, d_keys));
-  std::lock_guard<std::mutex> lk(api_mutex());
   const SqfGeom g = geom(qf);
   CallScope cs;
   GQF_TRY(cs.begin(stream));
@@ -1262,7 +1259,6 @@ gqf_status gqf_sqf_lookup(gqf_sqf qf, int sorted, int n, const uint32_t* d_keys,
     set_error("invalid key/output array (n=%d)", n);
     return GQF_ERR_INVALID;
   }
-  std::lock_guard<std::mutex> lk(api_mutex());
   const SqfGeom g = geom(qf);
   CallScope cs;
   GQF_TRY(cs.begin(stream));
@@ -1308,7 +1304,6 @@ gqf_status gqf_sqf_merge(gqf_sqf dst, gqf_sqf src, int no_dup, gqf_stream stream
     set_error("filters to merge must have the same q and r (quotientFilter.cu:1749)");
     return GQF_ERR_INVALID;
   }
-  std::lock_guard<std::mutex> lk(api_mutex());
   const SqfGeom g = geom(dst);
   CallScope cs;
   GQF_TRY(cs.begin(stream));

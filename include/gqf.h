@@ -178,6 +178,11 @@ gqf_status gqf_memcpy_dtoh(void* h_dst, const void* d_src, size_t bytes);
 gqf_status gqf_memcpy_dtod(void* d_dst, const void* d_src, size_t bytes);
 gqf_status gqf_memset(void* d_ptr, int value, size_t bytes);
 gqf_status gqf_synchronize(void);
+/* A non-blocking stream of default priority for concurrent callers: calls on
+ * different streams (from different host threads) run concurrently, each with
+ * its own workspace; calls on one stream are serialised. */
+gqf_status gqf_stream_create(gqf_stream* out);
+gqf_status gqf_stream_destroy(gqf_stream s);
 
 /* ============================================================ kernel timing
  * Per-kernel device time measured with HIP events on the launch stream (two

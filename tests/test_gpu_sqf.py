@@ -280,3 +280,49 @@ def test_full_size_q26_properties():
     fk = O.fingerprints(keys, q, r)
     remaining = np.setdiff1d(np.unique(fk), np.unique(fk[:m]))
     assert int(((t2 & 7) != 0).sum()) == len(remaining)
+
+
+def test_concurrent_streams_from_threads():
+    # four host threads, each with its own stream and filters, calling at once:
+    # every result must equal the single-threaded oracle result
+    import threading
+
+    say("sqf/rsqf concurrent streams", 4)
+    jobs = []
+    for t in range(4):
+        q = 14 + t
+        keys = O.mt19937(40 + t, O.key_count(0.7, q))
+        jobs.append((q, keys))
+    errors = []
+
+    def work(q, keys):
+        try:
+            s = G.Stream()
+            qf = G.SQF(q, 5)
+            for rep in range(3):
+                qf.bulk_build(dev(keys), NoDuplicates=True, stream=s)
+                qf.insert(dev(O.mt19937(99, 500)), stream=s)
+                qf.superclusterDeletes(dev(keys[:300]), stream=s)
+            f = G.RSQF(q, 8)
+            f.bulk_build(dev(keys), stream=s)
+            f.insertGPU(dev(O.mt19937(98, 400)), stream=s)
+            table, dumped = qf.table_bytes(), f.dump()
+            s.destroy()
+            jobs_out[q] = (table, dumped)
+        except Exception as e:  # noqa: BLE001
+            errors.append(e)
+
+    jobs_out = {}
+    th = [threading.Thread(target=work, args=j) for j in jobs]
+    for x in th:
+        x.start()
+    for x in th:
+        x.join()
+    assert not errors, errors
+    for q, keys in jobs:
+        want = sqf_canon(np.concatenate([keys, O.mt19937(99, 500)]), q, 5, True)
+        O.sqf_delete_keys(want, q, 5, keys[:300])
+        got_t, got_r = jobs_out[q]
+        assert (got_t == want).all()
+        st, rw = O.rsqf_canon(O.fingerprints(np.concatenate([keys, O.mt19937(98, 400)]), q, 8), q, 8)
+        assert (got_r == rw).all()
