@@ -57,13 +57,6 @@ __host__ __device__ __forceinline__ uint64_t fingerprint(uint32_t key, unsigned 
   return fmix64(((uint64_t)lo << 32) | key) & mask64(bits);
 }
 
-__device__ __forceinline__ unsigned long long st_load(const unsigned long long* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st_store(unsigned long long* p, unsigned long long v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 // ---------------------------------------------------------------- wave64 scans
 // Inclusive scans across the 64 lanes of a wave with DPP moves: row_shr 1, 2,
 // 4, 8 inside each 16-lane row, then row_bcast:15 (rows 1, 3) and
@@ -456,248 +449,14 @@ gqf_status radix_sort(hipStream_t s, DeviceState& ds, Arena& ar, const void* in,
   return GQF_OK;
 }
 
-// ====================================================================== layout scan
+// ====================================================================== locations
 // Canonical location of the i-th kept entry: loc_i = max(fq_i, loc_{i-1}+1).
-// A block of entries acts on the incoming location x as x -> max(x + c, l)
-// where c = entries kept and l = last location when nothing precedes; the
-// pair (c, l) composes associatively: (a then b) = (ca+cb, max(la+cb, lb)).
-constexpr int LS_THREADS = 256;
-constexpr int LS_ITEMS = 16;
-constexpr int LS_TILE = LS_THREADS * LS_ITEMS;
+// LOC_NONE: no entry yet.
 constexpr int64_t LOC_NONE = -(1ll << 40);
 
 enum KeepMode { KEEP_ALL = 0, KEEP_UNIQUE = 1, KEEP_TAGGED_INSERT = 2, KEEP_FLAGS = 3 };
 
-struct CL {
-  uint32_t c;
-  int64_t l;
-};
-__device__ __forceinline__ CL cl_combine(CL a, CL b) {
-  CL o;
-  o.c = a.c + b.c;
-  const int64_t x = a.l == LOC_NONE ? LOC_NONE : a.l + (int64_t)b.c;
-  o.l = x > b.l ? x : b.l;
-  return o;
-}
-__device__ __forceinline__ unsigned long long cl_pack(unsigned long long flag, CL x) {
-  unsigned long long lp = 0;
-  if (x.l != LOC_NONE) lp = (unsigned long long)(x.l + 1 < 0xFFFFFFFFll ? x.l + 1 : 0xFFFFFFFFll);
-  return (flag << 62) | ((unsigned long long)x.c << 32) | lp;
-}
-__device__ __forceinline__ CL cl_unpack(unsigned long long w) {
-  CL o;
-  o.c = (uint32_t)((w >> 32) & 0x3FFFFFFFull);
-  const uint32_t lp = (uint32_t)w;
-  o.l = lp == 0 ? LOC_NONE : (int64_t)lp - 1;
-  return o;
-}
-
-// Exclusive (c, l) prefix of `tile` from per-tile status words (flag 1 =
-// aggregate, 2 = inclusive prefix, in the top two bits), computed by the 64
-// lanes of the calling wave: each step requests 64 predecessors at once
-// (nearest in lane 0), waits until all have published, and combines back to
-// the nearest inclusive prefix (farther tiles first: the operator is not
-// commutative).  Tile 0 always publishes an inclusive prefix.
-__device__ __forceinline__ CL lookback_cl(const unsigned long long* status, uint32_t tile, int lane) {
-  CL excl = {0u, LOC_NONE};
-  int64_t base = (int64_t)tile - 1;
-  while (base >= 0) {
-    const int64_t j = base - lane;
-    unsigned long long st = 0;
-    if (j >= 0) {
-      st = st_load(status + j);
-      while ((st >> 62) == 0) {
-        __builtin_amdgcn_s_sleep(2);  // back off: spinning loads compete with real traffic
-        st = st_load(status + j);
-      }
-    }
-    const uint64_t incm = __ballot(j >= 0 && (st >> 62) == 2);
-    const int stop = incm ? (__ffsll((long long)incm) - 1) : 63;
-    CL v = {0u, LOC_NONE};
-    if (j >= 0 && lane <= stop) v = cl_unpack(st);
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      CL o;
-      o.c = __shfl_down(v.c, d, 64);
-      o.l = __shfl_down(v.l, d, 64);
-      if (lane + d < 64) v = cl_combine(o, v);
-    }
-    CL w;
-    w.c = __shfl(v.c, 0, 64);
-    w.l = __shfl(v.l, 0, 64);
-    excl = cl_combine(w, excl);
-    if (incm) break;
-    base -= 64;
-  }
-  return excl;
-}
-
-// in: sorted entries (fingerprint << tagbits | tag).  out: the kept
-// fingerprints (ufp) and their canonical locations (loc), compacted; the kept
-// count goes to ctr[CTR_COUNT]; overflow of capacity (or, for the RSQF, of
-// the 8-bit block offset) raises DEV_ERR_CAPACITY in ctr[CTR_ERR].
-template <typename K, int MODE>
-__global__ __launch_bounds__(256) void k_layout_scan(const K* __restrict__ in, const uint8_t* __restrict__ flags,
-                                                     uint32_t n, uint32_t q, uint32_t r, uint32_t tagbits,
-                                                     uint64_t capacity, int rsqf_offsets, K* __restrict__ ufp,
-                                                     uint32_t* __restrict__ loc, unsigned long long* __restrict__ status,
-                                                     uint32_t* __restrict__ ctr) {
-  __shared__ uint32_t s_tile;
-  __shared__ K s_buf[LS_TILE + 1];
-  __shared__ uint32_t s_loc[LS_TILE];
-  __shared__ uint32_t s_wc[4];
-  __shared__ int64_t s_wl[4];
-  __shared__ uint32_t s_pc;
-  __shared__ int64_t s_pl;
-  __shared__ uint32_t s_tc;
-
-  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  if (tid == 0) s_tile = atomicAdd(&ctr[CTR_SCAN_TILE], 1u);
-  __syncthreads();
-  const uint32_t tile = s_tile;
-  const uint64_t t0 = (uint64_t)tile * LS_TILE;
-  const uint32_t tn = (uint32_t)((uint64_t)n - t0 < (uint64_t)LS_TILE ? (uint64_t)n - t0 : (uint64_t)LS_TILE);
-  for (uint32_t i = tid; i < tn; i += LS_THREADS) s_buf[i + 1] = in[t0 + i];
-  if (tid == 0) s_buf[0] = t0 > 0 ? in[t0 - 1] : (K)0;
-  __syncthreads();
-
-  const uint64_t mq = mask64(q);
-  K x[LS_ITEMS];
-  uint32_t keepbits = 0;
-  CL agg = {0u, LOC_NONE};
-  const uint32_t i0 = (uint32_t)tid * LS_ITEMS;
-#pragma unroll
-  for (int j = 0; j < LS_ITEMS; ++j) {
-    const uint32_t i = i0 + j;
-    x[j] = 0;
-    if (i < tn) {
-      const K key = s_buf[i + 1];
-      const K fp = key >> tagbits;
-      const bool first = (t0 + i) == 0;
-      bool kp;
-      if (MODE == KEEP_ALL) kp = true;
-      else if (MODE == KEEP_UNIQUE) kp = first || fp != (s_buf[i] >> tagbits);
-      else if (MODE == KEEP_TAGGED_INSERT) kp = ((key & (K)1) == 0) || first || fp != (s_buf[i] >> 1);
-      else kp = flags[t0 + i] != 0;
-      x[j] = fp;
-      if (kp) {
-        keepbits |= 1u << j;
-        const int64_t fq = (int64_t)((uint64_t)(fp >> r) & mq);
-        agg.l = agg.l == LOC_NONE ? fq : (fq > agg.l + 1 ? fq : agg.l + 1);
-        agg.c++;
-      }
-    }
-  }
-
-  // block-wide exclusive scan of the per-thread (c, l)
-  CL inc = agg;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    CL y;
-    y.c = __shfl_up(inc.c, d, 64);
-    y.l = __shfl_up(inc.l, d, 64);
-    if (lane >= d) inc = cl_combine(y, inc);
-  }
-  CL ex;
-  ex.c = __shfl_up(inc.c, 1, 64);
-  ex.l = __shfl_up(inc.l, 1, 64);
-  if (lane == 0) {
-    ex.c = 0;
-    ex.l = LOC_NONE;
-  }
-  if (lane == 63) {
-    s_wc[wid] = inc.c;
-    s_wl[wid] = inc.l;
-  }
-  __syncthreads();
-  CL wpre = {0u, LOC_NONE}, tagg = {0u, LOC_NONE};
-#pragma unroll
-  for (int w = 0; w < 4; ++w) {
-    CL wv = {s_wc[w], s_wl[w]};
-    if (w < wid) wpre = cl_combine(wpre, wv);
-    tagg = cl_combine(tagg, wv);
-  }
-  ex = cl_combine(wpre, ex);
-
-  if (wid == 0) {
-    unsigned long long* my = status + tile;
-    if (lane == 0) st_store(my, cl_pack(tile == 0 ? 2 : 1, tagg));
-    CL excl = {0u, LOC_NONE};
-    if (tile > 0) {
-      excl = lookback_cl(status, tile, lane);
-      if (lane == 0) st_store(my, cl_pack(2, cl_combine(excl, tagg)));
-    }
-    if (lane == 0) {
-      s_pc = excl.c;
-      s_pl = excl.l;
-      s_tc = tagg.c;
-      if (tile == gridDim.x - 1) {
-        const CL tot = cl_combine(excl, tagg);
-        ctr[CTR_COUNT] = tot.c;
-        if (tot.l != LOC_NONE && (uint64_t)tot.l >= capacity) atomicOr(&ctr[CTR_ERR], DEV_ERR_CAPACITY);
-      }
-    }
-  }
-  __syncthreads();
-
-  // emit kept entries with their locations into LDS, then write coalesced
-  const CL pre = {s_pc, s_pl};
-  const CL me = cl_combine(pre, ex);
-  int64_t prev = me.l;
-  uint32_t o = me.c - s_pc;
-  bool off_overflow = false;
-#pragma unroll
-  for (int j = 0; j < LS_ITEMS; ++j) {
-    if (keepbits & (1u << j)) {
-      const int64_t fq = (int64_t)((uint64_t)(x[j] >> r) & mq);
-      const int64_t lc = prev == LOC_NONE ? fq : (fq > prev + 1 ? fq : prev + 1);
-      if (rsqf_offsets && lc + 1 - 64 * (fq / 64 + 1) > 255) off_overflow = true;
-      s_buf[o] = x[j];
-      s_loc[o] = (uint32_t)lc;
-      prev = lc;
-      ++o;
-    }
-  }
-  if (off_overflow) atomicOr(&ctr[CTR_ERR], DEV_ERR_CAPACITY);
-  __syncthreads();
-  const uint32_t tc = s_tc, pc = s_pc;
-  for (uint32_t i = tid; i < tc; i += LS_THREADS) {
-    ufp[pc + i] = s_buf[i];
-    loc[pc + i] = s_loc[i];
-  }
-}
-
-// Returns the layout outputs; kept count lands in ds.counters[CTR_COUNT].
-template <typename K, int MODE>
-gqf_status layout_scan(hipStream_t s, DeviceState& ds, Arena& ar, const K* in, const uint8_t* flags, uint32_t n,
-                       uint32_t q, uint32_t r, uint32_t tagbits, uint64_t capacity, bool rsqf_offsets, K** ufp_out,
-                       uint32_t** loc_out) {
-  const uint32_t tiles = (n + LS_TILE - 1) / LS_TILE;
-  K* ufp = ar.take<K>(n ? n : 1);
-  uint32_t* loc = ar.take<uint32_t>(n ? n : 1);
-  unsigned long long* st = ar.take<unsigned long long>(tiles + 1);
-  if (ar.used > ar.cap) {
-    set_error("internal: layout workspace overflow");
-    return GQF_ERR_INVALID;
-  }
-  *ufp_out = ufp;
-  *loc_out = loc;
-  if (n == 0) return GQF_OK;  // CTR_COUNT stays 0 (reset by the caller)
-  launch_fill<unsigned long long>(s, st, tiles + 1, 0ull);
-  GQF_LAUNCH_B((double)n * (2 * sizeof(K) + 4 + (flags ? 1 : 0)), (k_layout_scan<K, MODE>), dim3(tiles), dim3(LS_THREADS), 0, s, in, flags, n, q, r, tagbits,
-                     capacity, (int)rsqf_offsets, ufp, loc, st, ds.counters);
-  return GQF_OK;
-}
-inline size_t layout_workspace_bytes(uint64_t n, size_t key_bytes) {
-  return aligned_bytes(n * key_bytes) + aligned_bytes(n * 4) + aligned_bytes(((n + LS_TILE - 1) / LS_TILE + 1) * 8) +
-         768;
-}
-
 // ====================================================================== merge
-constexpr int MG_THREADS = 256;
-constexpr int MG_ITEMS = 8;
-constexpr int MG_TILE = MG_THREADS * MG_ITEMS;
-
 // Number of A entries among the first d outputs of the stable merge (A first on ties).
 template <typename K>
 __device__ __forceinline__ uint32_t merge_path(const K* A, uint32_t na, const K* B, uint32_t nb, uint32_t d) {
@@ -708,49 +467,6 @@ __device__ __forceinline__ uint32_t merge_path(const K* A, uint32_t na, const K*
     else hi = mid;
   }
   return lo;
-}
-
-// out = merge(A, B) as (fp << 1 | source): A entries carry tag 0 and precede
-// equal B entries (tag 1).
-template <typename K>
-__global__ __launch_bounds__(256) void k_merge_tagged(const K* __restrict__ A, uint32_t na, const K* __restrict__ B,
-                                                      uint32_t nb, uint64_t* __restrict__ out) {
-  __shared__ K sA[MG_TILE];
-  __shared__ K sB[MG_TILE];
-  __shared__ uint32_t s_split[2];
-  const uint32_t tid = threadIdx.x;
-  const uint64_t total = (uint64_t)na + nb;
-  const uint32_t d0 = blockIdx.x * MG_TILE;
-  const uint32_t d1 = (uint32_t)((uint64_t)d0 + MG_TILE < total ? (uint64_t)d0 + MG_TILE : total);
-  if (tid == 0) s_split[0] = merge_path(A, na, B, nb, d0);
-  if (tid == 1) s_split[1] = merge_path(A, na, B, nb, d1);
-  __syncthreads();
-  const uint32_t a0 = s_split[0], a1 = s_split[1];
-  const uint32_t b0 = d0 - a0, b1 = d1 - a1;
-  const uint32_t la = a1 - a0, lb = b1 - b0;
-  for (uint32_t i = tid; i < la; i += MG_THREADS) sA[i] = A[a0 + i];
-  for (uint32_t i = tid; i < lb; i += MG_THREADS) sB[i] = B[b0 + i];
-  __syncthreads();
-  const uint32_t len = la + lb;
-  const uint32_t ld = tid * MG_ITEMS < len ? tid * MG_ITEMS : len;
-  uint32_t ai = merge_path(sA, la, sB, lb, ld);
-  uint32_t bi = ld - ai;
-#pragma unroll
-  for (int j = 0; j < MG_ITEMS; ++j) {
-    const uint32_t o = ld + j;
-    if (o < len) {
-      const bool takeA = bi >= lb || (ai < la && sA[ai] <= sB[bi]);
-      out[(uint64_t)d0 + o] = takeA ? ((uint64_t)sA[ai++] << 1) : (((uint64_t)sB[bi++] << 1) | 1ull);
-    }
-  }
-}
-
-template <typename K>
-inline void launch_merge_tagged(hipStream_t s, const K* A, uint32_t na, const K* B, uint32_t nb, uint64_t* out) {
-  const uint64_t total = (uint64_t)na + nb;
-  if (!total) return;
-  GQF_LAUNCH_B((double)total * (sizeof(K) + 8), k_merge_tagged<K>, dim3((unsigned)((total + MG_TILE - 1) / MG_TILE)), dim3(MG_THREADS), 0, s, A,
-                     na, B, nb, out);
 }
 
 // ====================================================================== delete marking

@@ -1,0 +1,173 @@
+// gqf_rebuild.hpp -- the chunked location scan shared by the SQF and RSQF
+// rebuilds: from a sorted sequence of fingerprints (optionally the stable
+// merge of two, optionally filtered by a keep rule) to each kept entry's
+// canonical location loc_i = max(fq_i, loc_{i-1}+1) and run heads.
+//
+// Entries are cut into sub-tiles of RB_TILE and chunks of whole sub-tiles; the
+// scan runs reduce-then-scan over the chunks (no waits between workgroups):
+// a per-chunk aggregate, one exclusive scan of the aggregates, then the
+// writer walks each chunk again on top of its prefix.
+#pragma once
+
+#include "gqf_kernels.hpp"
+
+namespace gqf {
+
+constexpr int RB_THREADS = 512;
+constexpr int RB_ITEMS = 8;
+constexpr int RB_WAVES = RB_THREADS / 64;
+constexpr int RB_TILE = RB_THREADS * RB_ITEMS;
+constexpr uint32_t F_NONE = 0xFFFFFFFFu;
+
+// Scan state of a block of entries: kept count c, location map
+// x -> max(x + c, l), and the quotient f of its last kept entry (run heads are
+// the kept entries whose quotient differs from the previous kept entry's).
+struct RS {
+  uint32_t c;
+  int64_t l;
+  uint32_t f;
+};
+__device__ __forceinline__ RS rs_combine(RS a, RS b) {
+  RS o;
+  o.c = a.c + b.c;
+  const int64_t x = a.l == LOC_NONE ? LOC_NONE : a.l + (int64_t)b.c;
+  o.l = x > b.l ? x : b.l;
+  o.f = b.c ? b.f : a.f;
+  return o;
+}
+__device__ __forceinline__ RS rs_none() { return RS{0u, LOC_NONE, F_NONE}; }
+// Stored form of a state: {c, l + 1 (0 = none), f, 0}.
+__device__ __forceinline__ uint4 rs_pack(RS x) {
+  return make_uint4(x.c, x.l == LOC_NONE ? 0u : (uint32_t)(x.l + 1), x.f, 0u);
+}
+__device__ __forceinline__ RS rs_unpack(uint4 w) { return RS{w.x, w.y == 0 ? LOC_NONE : (int64_t)w.y - 1, w.z}; }
+
+// Exclusive scan across the 64 lanes of a wave (DPP moves, see
+// wave_incl_add); *total = the wave aggregate.
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ RS rs_dpp(RS x) {
+  RS y;
+  y.c = dpp_u32<CTRL, ROW_MASK>(0u, x.c);
+  y.l = (int64_t)dpp_u64<CTRL, ROW_MASK>((uint64_t)LOC_NONE, (uint64_t)x.l);
+  y.f = dpp_u32<CTRL, ROW_MASK>(F_NONE, x.f);
+  return y;
+}
+__device__ __forceinline__ RS rs_wave_excl(RS x, RS* total) {
+  RS inc = x;
+  inc = rs_combine(rs_dpp<DPP_ROW_SHR | 1>(inc), inc);
+  inc = rs_combine(rs_dpp<DPP_ROW_SHR | 2>(inc), inc);
+  inc = rs_combine(rs_dpp<DPP_ROW_SHR | 4>(inc), inc);
+  inc = rs_combine(rs_dpp<DPP_ROW_SHR | 8>(inc), inc);
+  inc = rs_combine(rs_dpp<DPP_ROW_BCAST15, 0xA>(inc), inc);
+  inc = rs_combine(rs_dpp<DPP_ROW_BCAST31, 0xC>(inc), inc);
+  total->c = wave_read(inc.c, 63);
+  total->l = (int64_t)(((uint64_t)wave_read((uint32_t)((uint64_t)inc.l >> 32), 63) << 32) |
+                       wave_read((uint32_t)inc.l, 63));
+  total->f = wave_read(inc.f, 63);
+  return rs_dpp<DPP_WAVE_SHR1>(inc);
+}
+
+// Merge-path split (A entries among the first u*RB_TILE outputs) of every sub-tile boundary.
+template <typename K>
+__global__ __launch_bounds__(256) void k_rb_splits(const K* __restrict__ A, uint32_t na, const K* __restrict__ B,
+                                                   uint32_t nb, uint32_t nsub, uint32_t* __restrict__ splits) {
+  const uint32_t u = blockIdx.x * 256 + threadIdx.x;
+  if (u > nsub) return;
+  const uint64_t N = (uint64_t)na + nb;
+  const uint32_t d = (uint32_t)((uint64_t)u * RB_TILE < N ? (uint64_t)u * RB_TILE : N);
+  splits[u] = merge_path(A, na, B, nb, d);
+}
+
+struct RbPlan {
+  uint32_t G, chunk, nsub;  // chunks, entries per chunk (multiple of RB_TILE), sub-tiles
+};
+constexpr uint32_t RB_MAX_CHUNKS = 16384;
+static inline RbPlan rb_plan(uint64_t N) {
+  // one sub-tile per workgroup (latency is hidden by occupancy, not by a
+  // sequential walk), several per chunk only beyond RB_MAX_CHUNKS sub-tiles
+  RbPlan p;
+  p.nsub = (uint32_t)((N + RB_TILE - 1) / RB_TILE);
+  const uint32_t per = (p.nsub + RB_MAX_CHUNKS - 1) / RB_MAX_CHUNKS;
+  p.chunk = (per ? per : 1) * RB_TILE;
+  p.G = (uint32_t)((N + p.chunk - 1) / p.chunk);
+  if (p.G == 0) p.G = 1;
+  return p;
+}
+
+// Block-wide scan of the per-thread states: *ex_out = exclusive prefix of the
+// calling thread within the block, returns the block aggregate.  s_wave:
+// RB_WAVES + 1 entries (wave aggregates, then their prefixes and the total).
+__device__ __forceinline__ RS rs_block(RS a, RS* ex_out, uint4* s_wave) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  RS wt;
+  const RS ex = rs_wave_excl(a, &wt);
+  if (lane == 0) s_wave[wid] = rs_pack(wt);
+  __syncthreads();
+  if (wid == 0) {
+    const RS x = lane < RB_WAVES ? rs_unpack(s_wave[lane]) : rs_none();
+    RS tot;
+    const RS xe = rs_wave_excl(x, &tot);
+    if (lane < RB_WAVES) s_wave[lane] = rs_pack(xe);
+    if (lane == 0) s_wave[RB_WAVES] = rs_pack(tot);
+  }
+  __syncthreads();
+  *ex_out = rs_combine(rs_unpack(s_wave[wid]), ex);
+  return rs_unpack(s_wave[RB_WAVES]);
+}
+
+// Exclusive scan of the chunk states by one workgroup of 1024 threads: in
+// pieces of 8192 states, each thread combines its 8 consecutive states, one
+// block-wide scan, then the thread rewrites its 8 states as exclusive
+// prefixes.  The total (also stored in *total when given) fixes the kept count, the first
+// slot after the last entry, and a capacity error -- all before any table
+// byte is written.
+static __global__ __launch_bounds__(1024) void k_rb_scan(uint4* __restrict__ agg, uint32_t G, uint64_t S,
+                                                         uint32_t* __restrict__ ctr, uint4* __restrict__ total) {
+  constexpr uint32_t SEG = 8, PIECE = 1024 * SEG;
+  __shared__ uint4 s_wave[16];  // wave aggregates, then their exclusive prefixes
+  __shared__ uint4 s_tot;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  RS carry = rs_none();
+  for (uint32_t base = 0; base < G; base += PIECE) {
+    const uint32_t i0 = base + (uint32_t)tid * SEG;
+    uint4 w[SEG];
+#pragma unroll
+    for (uint32_t k = 0; k < SEG; ++k) w[k] = i0 + k < G ? agg[i0 + k] : make_uint4(0u, 0u, F_NONE, 0u);
+    RS seg = rs_none();
+#pragma unroll
+    for (uint32_t k = 0; k < SEG; ++k) seg = rs_combine(seg, rs_unpack(w[k]));
+    RS wt;
+    RS ex = rs_wave_excl(seg, &wt);
+    if (lane == 0) s_wave[wid] = rs_pack(wt);
+    __syncthreads();
+    if (wid == 0) {
+      // wave 0 scans the 16 wave aggregates on top of the carry
+      RS x = lane < 16 ? rs_unpack(s_wave[lane]) : rs_none();
+      RS tot;
+      RS xe = rs_wave_excl(x, &tot);
+      if (lane < 16) s_wave[lane] = rs_pack(rs_combine(carry, xe));
+      if (lane == 0) s_tot = rs_pack(rs_combine(carry, tot));
+    }
+    __syncthreads();
+    ex = rs_combine(rs_unpack(s_wave[wid]), ex);
+#pragma unroll
+    for (uint32_t k = 0; k < SEG; ++k) {
+      if (i0 + k < G) {
+        agg[i0 + k] = rs_pack(ex);
+        ex = rs_combine(ex, rs_unpack(w[k]));
+      }
+    }
+    carry = rs_unpack(s_tot);
+    __syncthreads();
+  }
+  if (tid == 0) {
+    if (total) *total = rs_pack(carry);
+    ctr[CTR_COUNT] = carry.c;
+    const int64_t L = carry.l;
+    ctr[CTR_MISC + 3] = L == LOC_NONE ? 0u : (uint32_t)(L + 1 < (int64_t)S ? L + 1 : (int64_t)S);
+    if (L != LOC_NONE && (uint64_t)L >= S) ctr[CTR_ERR] |= DEV_ERR_CAPACITY;
+  }
+}
+
+
+}  // namespace gqf

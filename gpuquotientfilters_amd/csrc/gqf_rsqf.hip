@@ -14,7 +14,7 @@
 // reference on any table (including the history-dependent layouts its insert
 // leaves).  Build / insert / delete produce the canonical layout, which is
 // what the reference insert produces when fed fingerprints in ascending order.
-#include "gqf_kernels.hpp"
+#include "gqf_rebuild.hpp"
 
 #include <cstring>
 #include <vector>
@@ -147,139 +147,533 @@ __global__ __launch_bounds__(256) void k_rsqf_lookup_sorted(RsqfDev f, uint32_t 
   out[idx[i]] = rsqf_lookup_one<R>(f, (uint32_t)((fp >> f.r) & mask64(q)), (uint32_t)(fp & ((1u << f.r) - 1u)));
 }
 
-// ---------------------------------------------------------------- table writer
-// One workgroup per 64 blocks: occupieds / runEnds / remainders of the
-// canonical layout assembled in LDS, offsets from the last entry whose
-// quotient precedes each block, then every byte of the group written once.
-constexpr uint32_t RW_BLOCKS = 64;
-
-template <typename K>
-__device__ __forceinline__ uint32_t lb_loc(const uint32_t* __restrict__ a, uint32_t lo, uint32_t hi, uint64_t v) {
-  while (lo < hi) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if ((uint64_t)a[mid] < v) lo = mid + 1;
-    else hi = mid;
-  }
-  return lo;
-}
-template <typename K>
-__device__ __forceinline__ uint32_t lb_fq(const K* __restrict__ fp, uint32_t lo, uint32_t hi, uint64_t v, uint32_t r) {
-  while (lo < hi) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (((uint64_t)fp[mid] >> r) < v) lo = mid + 1;
-    else hi = mid;
-  }
-  return lo;
-}
-
-template <typename K>
-__global__ __launch_bounds__(256) void k_rsqf_write(uint8_t* __restrict__ base, uint32_t nb, uint32_t r,
-                                                    const K* __restrict__ ufp, const uint32_t* __restrict__ loc,
-                                                    const uint32_t* __restrict__ ctr) {
-  if (ctr[CTR_ERR]) return;
-  __shared__ unsigned long long s_occ[RW_BLOCKS];
-  __shared__ unsigned long long s_ends[RW_BLOCKS];
-  __shared__ uint32_t s_off[RW_BLOCKS];
-  __shared__ uint32_t s_rem[RW_BLOCKS * 16 * 2];  // 8*r bytes per block, r <= 16
-  __shared__ uint32_t s_rng[4];
-  const uint32_t m = ctr[CTR_COUNT];
-  const uint32_t tid = threadIdx.x;
-  const uint32_t j0 = blockIdx.x * RW_BLOCKS;
-  const uint32_t j1 = j0 + RW_BLOCKS < nb ? j0 + RW_BLOCKS : nb;
-  const uint64_t sl0 = (uint64_t)j0 * 64, sl1 = (uint64_t)j1 * 64;
-  const uint32_t remw = (j1 - j0) * 2 * r;  // 32-bit words of remainder bits in this group
-  for (uint32_t i = tid; i < RW_BLOCKS; i += 256) {
-    s_occ[i] = 0;
-    s_ends[i] = 0;
-    s_off[i] = 0;
-  }
-  for (uint32_t i = tid; i < remw; i += 256) s_rem[i] = 0;
-  if (tid == 0) s_rng[0] = lb_loc<K>(loc, 0u, m, sl0);
-  if (tid == 64) s_rng[1] = lb_loc<K>(loc, 0u, m, sl1);
-  if (tid == 128) s_rng[2] = lb_fq(ufp, 0u, m, sl0, r);
-  if (tid == 192) s_rng[3] = lb_fq(ufp, 0u, m, sl1, r);
-  __syncthreads();
-  const uint32_t f0 = s_rng[2], f1 = s_rng[3];
-  for (uint32_t i = s_rng[0] + tid; i < s_rng[1]; i += 256) {
-    const uint64_t fp = (uint64_t)ufp[i];
-    const uint64_t fq = fp >> r;
-    const uint32_t ls = (uint32_t)(loc[i] - sl0);  // slot within the group
-    const uint32_t fr = (uint32_t)(fp & ((1ull << r) - 1ull));
-    const uint32_t bit = ls * r;  // 64*r bits per block: group-relative bit position
-    atomicOr(&s_rem[bit >> 5], fr << (bit & 31));
-    if ((bit & 31) + r > 32) atomicOr(&s_rem[(bit >> 5) + 1], fr >> (32 - (bit & 31)));
-    const bool run_end = (i + 1 == m) || (((uint64_t)ufp[i + 1] >> r) != fq);
-    if (run_end) atomicOr(&s_ends[ls >> 6], 1ull << (ls & 63));
-  }
-  for (uint32_t i = f0 + tid; i < f1; i += 256) {
-    const uint64_t fq = ((uint64_t)ufp[i] >> r) - sl0;
-    atomicOr(&s_occ[fq >> 6], 1ull << (fq & 63));
-  }
-  // offset_j: entries with quotient < 64 j that sit in block j or later
-  for (uint32_t b = tid; b < j1 - j0; b += 256) {
-    const uint64_t lim = (uint64_t)(j0 + b) * 64;
-    const uint32_t lo = f0 > 0 ? f0 - 1 : 0u;
-    const uint32_t idx = lb_fq(ufp, lo, f1, lim, r);
-    if (idx > 0) {
-      const uint64_t E = loc[idx - 1];
-      if (E >= lim) s_off[b] = (uint32_t)(E - lim + 1) & 0xFFu;
-    }
-  }
-  __syncthreads();
-  uint64_t* meta = reinterpret_cast<uint64_t*>(base);
-  for (uint32_t i = tid; i < (j1 - j0) * 4; i += 256) {
-    const uint32_t b = i >> 2, w = i & 3;
-    const uint64_t v = w == 0 ? s_occ[b] : w == 1 ? s_ends[b] : w == 2 ? (uint64_t)s_off[b] : 0ull;
-    meta[(size_t)(j0 + b) * 4 + w] = v;
-  }
-  uint32_t* remg = reinterpret_cast<uint32_t*>(base + rsqf_meta_bytes(nb) + (size_t)j0 * 8 * r);
-  for (uint32_t i = tid; i < remw; i += 256) remg[i] = s_rem[i];
-}
-
 // ---------------------------------------------------------------- decode
-// Block j owns the runs of its occupied quotients; the first of them starts at
-// slot 64 j + offset_j (all earlier quotients' entries end before it), and the
-// k-th owned run ends at the k-th run end from there on.  Requires a canonical
-// table (no gaps between a run's predecessor and its quotient's first slot).
-__device__ __forceinline__ uint64_t next_run_end(const RsqfDev& f, uint64_t pos) {
-  uint32_t b = (uint32_t)(pos >> 6);
-  if (b >= f.nb) return ~0ull;
-  uint64_t w = f.ends(b) & ~low_mask_ll((uint32_t)(pos & 63));
-  while (w == 0) {
-    if (++b >= f.nb) return ~0ull;
-    w = f.ends(b);
+// In a valid RSQF the entries in slot order are the stored fingerprints in
+// ascending order, and the k-th run end (counting from slot 0) closes the run
+// of the k-th occupied quotient.  Slot s therefore holds an entry iff the
+// quotient Q of run k = 1 + #(run ends before s) satisfies Q <= s, and the
+// entry is Q << r | remainder(s).  Slots are decoded in windows of RD_SLOTS
+// (64 blocks):
+//   k_rd_pop      occupied and run-end counts per window,
+//   k_scan_rows   their exclusive prefixes O_w, E_w,
+//   k_rd_count    entries per window,
+//   k_scan_rows   output offsets,
+//   k_rd_emit     the fingerprints, staged in LDS, written coalesced.
+// Window w needs the quotients of runs E_w+1 ... E_w+1+#(run ends in w): a
+// list built in LDS from the occupied words, starting at the block that holds
+// the (E_w+1)-th occupied quotient (found through O).  Replaces the
+// reference's sequential walk of runs (RSQF.cpp:314-569 reads them back the
+// same way, one run at a time).
+constexpr uint32_t RD_BLOCKS = 64;
+constexpr uint32_t RD_SLOTS = RD_BLOCKS * 64;
+constexpr uint32_t RD_THREADS = 256;  // 16 slots per thread
+constexpr uint32_t RD_LIST = RD_SLOTS + 1;
+
+__global__ __launch_bounds__(256) void k_rd_pop(const uint64_t* __restrict__ meta, uint32_t nb, uint32_t W,
+                                                uint32_t* __restrict__ wp) {
+  const uint32_t w = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (w >= W) return;
+  const uint32_t b = w * RD_BLOCKS + lane;
+  uint32_t o = 0, e = 0;
+  if (b < nb) {
+    const uint4 x = reinterpret_cast<const uint4*>(meta)[2 * (size_t)b];
+    o = (uint32_t)__popc(x.x) + (uint32_t)__popc(x.y);
+    e = (uint32_t)__popc(x.z) + (uint32_t)__popc(x.w);
   }
-  return (uint64_t)b * 64 + (uint64_t)__ffsll((long long)w) - 1;
+  o = wave_incl_add(o);
+  e = wave_incl_add(e);
+  if (lane == 63) {
+    wp[2 * (size_t)w] = o;
+    wp[2 * (size_t)w + 1] = e;
+  }
 }
 
-template <bool EMIT, typename K>
-__global__ __launch_bounds__(256) void k_rsqf_decode(RsqfDev f, uint32_t* __restrict__ counts,
-                                                     K* __restrict__ out) {
-  const uint32_t j = blockIdx.x * 256 + threadIdx.x;
-  if (j >= f.nb) return;
-  uint64_t occw = f.occ(j);
-  uint64_t pos = (uint64_t)j * 64 + f.offset(j);
-  int64_t prevEnd = (int64_t)pos - 1;
-  uint32_t o = EMIT ? counts[j] : 0u;
-  uint32_t cnt = 0;
-  while (occw) {
-    const uint32_t bit = (uint32_t)__ffsll((long long)occw) - 1;
-    occw &= occw - 1;
-    const uint64_t fq = (uint64_t)j * 64 + bit;
-    const uint64_t e = next_run_end(f, pos);
-    if (e == ~0ull) break;
-    const int64_t start = (int64_t)fq > prevEnd + 1 ? (int64_t)fq : prevEnd + 1;
-    if (start <= (int64_t)e) {  // a run never ends before its quotient in a valid table
-      if (EMIT) {
-        for (int64_t s = start; s <= (int64_t)e; ++s)
-          out[o++] = (K)((fq << f.r) | f.get<0>((uint32_t)(s >> 6), (uint32_t)(s & 63)));
+// s_list[i] = quotient of the (E_w + 1 + i)-th occupied slot for i < RD_LIST,
+// RS_NOT_FOUND where that quotient lies past the window.
+__device__ void rd_build_list(const RsqfDev& f, const uint32_t* __restrict__ wp, uint32_t w, uint32_t* s_list,
+                              uint32_t* s_tmp4, uint32_t* s_v) {
+  const uint32_t tid = threadIdx.x;
+  for (uint32_t i = tid; i < RD_LIST; i += RD_THREADS) s_list[i] = RS_NOT_FOUND;
+  const uint32_t R0 = wp[2 * (size_t)w + 1] + 1;
+  if (tid == 0) {
+    // window v holding the R0-th occupied quotient: the last one with O_v < R0
+    uint32_t v = w;
+    if (wp[2 * (size_t)w] >= R0) {
+      uint32_t lo = 0, hi = w;  // O_lo < R0 <= O_hi
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (wp[2 * (size_t)mid] < R0) lo = mid;
+        else hi = mid;
       }
-      cnt += (uint32_t)((int64_t)e - start + 1);
+      v = lo;
     }
-    prevEnd = (int64_t)e;
-    pos = e + 1;
+    s_v[0] = v;
+    s_v[1] = wp[2 * (size_t)v];
   }
-  if (!EMIT) counts[j] = cnt;
+  __syncthreads();
+  const uint32_t v = s_v[0];
+  uint32_t base = s_v[1];  // occupied quotients before the current block row
+  const uint32_t bend = (w + 1) * RD_BLOCKS < f.nb ? (w + 1) * RD_BLOCKS : f.nb;
+  for (uint32_t bb = v * RD_BLOCKS; bb < bend && base < R0 + RD_LIST - 1; bb += RD_THREADS) {
+    const uint32_t b = bb + tid;
+    uint64_t occ = b < bend ? f.occ(b) : 0ull;
+    const uint32_t c = (uint32_t)__popcll(occ);
+    uint32_t tot;
+    const uint32_t pre = block256_excl_scan(c, s_tmp4, &tot);
+    uint32_t g = base + pre + 1;  // rank of this block's first occupied quotient
+    if (g + c > R0 && g < R0 + RD_LIST) {
+      while (occ) {
+        const uint32_t bit = (uint32_t)__ffsll((long long)occ) - 1;
+        occ &= occ - 1;
+        if (g >= R0 && g - R0 < RD_LIST) s_list[g - R0] = b * 64 + bit;
+        ++g;
+      }
+    }
+    base += tot;
+    __syncthreads();
+  }
+  __syncthreads();
+}
+
+// This thread's 16 slots: run ends (bit j = slot s0 + j) and the exclusive
+// count of run ends before them in the window.
+__device__ __forceinline__ uint32_t rd_ends16(const RsqfDev& f, uint32_t w, uint32_t* s0) {
+  const uint32_t t = threadIdx.x;
+  const uint32_t b = w * RD_BLOCKS + (t >> 2);
+  *s0 = w * RD_SLOTS + 16 * t;
+  return b < f.nb ? (uint32_t)(f.ends(b) >> (16 * (t & 3))) & 0xFFFFu : 0u;
+}
+
+// Bit j set: slot s0 + j holds an entry (its quotient in q[j]).
+__device__ __forceinline__ uint32_t rd_slots(const RsqfDev& f, uint32_t e16, uint32_t rank0, uint32_t s0,
+                                             const uint32_t* s_list, uint32_t* qv) {
+  uint32_t m = 0;
+  const bool in_table = (s0 >> 6) < f.nb;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const uint32_t tr = rank0 + (uint32_t)__popc(e16 & ((1u << j) - 1u));
+    const uint32_t Q = tr < RD_LIST ? s_list[tr] : RS_NOT_FOUND;
+    qv[j] = Q;
+    if (in_table && Q <= s0 + (uint32_t)j) m |= 1u << j;
+  }
+  return m;
+}
+
+__global__ __launch_bounds__(RD_THREADS) void k_rd_count(RsqfDev f, const uint32_t* __restrict__ wp,
+                                                         uint32_t* __restrict__ wc) {
+  __shared__ uint32_t s_list[RD_LIST];
+  __shared__ uint32_t s_tmp4[4];
+  __shared__ uint32_t s_v[2];
+  const uint32_t w = blockIdx.x;
+  rd_build_list(f, wp, w, s_list, s_tmp4, s_v);
+  uint32_t s0;
+  const uint32_t e16 = rd_ends16(f, w, &s0);
+  uint32_t tot;
+  const uint32_t rank0 = block256_excl_scan((uint32_t)__popc(e16), s_tmp4, &tot);
+  uint32_t qv[16];
+  const uint32_t m = rd_slots(f, e16, rank0, s0, s_list, qv);
+  __syncthreads();
+  uint32_t total;
+  (void)block256_excl_scan((uint32_t)__popc(m), s_tmp4, &total);
+  if (threadIdx.x == 0) wc[w] = total;
+}
+
+template <typename K, int R>
+__global__ __launch_bounds__(RD_THREADS) void k_rd_emit(RsqfDev f, const uint32_t* __restrict__ wp,
+                                                        const uint32_t* __restrict__ wc, K* __restrict__ out) {
+  // the quotient list, then (after every thread has read it) the window's entries
+  __shared__ __align__(16) K s_buf[RD_SLOTS * sizeof(K) >= RD_LIST * 4 ? RD_SLOTS : RD_LIST];
+  __shared__ uint32_t s_tmp4[4];
+  __shared__ uint32_t s_v[2];
+  uint32_t* s_list = reinterpret_cast<uint32_t*>(s_buf);
+  const uint32_t w = blockIdx.x;
+  const uint32_t rr = R ? (uint32_t)R : f.r;
+  rd_build_list(f, wp, w, s_list, s_tmp4, s_v);
+  uint32_t s0;
+  const uint32_t e16 = rd_ends16(f, w, &s0);
+  uint32_t tot;
+  const uint32_t rank0 = block256_excl_scan((uint32_t)__popc(e16), s_tmp4, &tot);
+  uint32_t qv[16];
+  const uint32_t m = rd_slots(f, e16, rank0, s0, s_list, qv);
+  // remainders of the 16 slots
+  uint32_t rem[16];
+  const uint32_t b = s0 >> 6;
+  if (m) {
+    if (R == 8) {
+      const uint4 x = *reinterpret_cast<const uint4*>(f.rem + s0);
+      const uint32_t wd[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+      for (int j = 0; j < 16; ++j) rem[j] = (wd[j >> 2] >> (8 * (j & 3))) & 0xFFu;
+    } else if (R == 16) {
+      const uint4* p = reinterpret_cast<const uint4*>(f.rem + 2 * (size_t)s0);
+      const uint4 x0 = p[0], x1 = p[1];
+      const uint32_t wd[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+#pragma unroll
+      for (int j = 0; j < 16; ++j) rem[j] = (wd[j >> 1] >> (16 * (j & 1))) & 0xFFFFu;
+    } else {
+#pragma unroll
+      for (int j = 0; j < 16; ++j) rem[j] = (m >> j) & 1u ? f.get<0>(b, (s0 & 63) + (uint32_t)j) : 0u;
+    }
+  }
+  uint32_t cnt;
+  const uint32_t pos = block256_excl_scan((uint32_t)__popc(m), s_tmp4, &cnt);
+  __syncthreads();  // every thread is done with s_list
+  uint32_t p = pos;
+#pragma unroll
+  for (int j = 0; j < 16; ++j)
+    if ((m >> j) & 1u) s_buf[p++] = ((K)qv[j] << rr) | (K)rem[j];
+  __syncthreads();
+  K* dst = out + wc[w];
+  for (uint32_t i = threadIdx.x; i < cnt; i += RD_THREADS) dst[i] = s_buf[i];
+}
+
+// ---------------------------------------------------------------- rebuild
+// Sorted fingerprints (build; the merge of the decoded table and a sorted
+// batch for insert; the decoded table with keep flags for delete) to the
+// canonical layout, through the chunked location scan (k_rr_agg, k_rb_scan)
+// and a writer that walks each sub-tile again on top of its prefix.
+//
+// The writer runs on a table cleared beforehand and leaves every field with
+// one owner:
+//   remainders   of slots [L_lo, L_hi) -- the sub-tile's own locations;
+//   run ends     of slots [X.l, Y.l) -- each run head marks the end of the
+//                previous kept entry's run (the last run end: k_rr_tail);
+//   occupieds    of its quotient blocks -- the first and last may be shared
+//                with the neighbouring sub-tiles;
+//   offset_j     by the first kept entry with quotient >= 64 j (the crossing
+//                entry; blocks after the last quotient: k_rr_tail).
+// A sub-tile assembles its fields in an LDS window of RW_WB blocks and
+// writes each field of a block with plain stores where it owns it whole,
+// atomic ORs where it shares it.  Replaces the write stage of insertGPU
+// (RSQF.cu:882-988) with the end state of inserting the fingerprints in
+// ascending order (RSQF.cpp:314-400).
+constexpr uint32_t RW_WB = 128;  // blocks per LDS window (8192 slots)
+
+template <typename K, int MODE, bool MERGE>
+__device__ __forceinline__ void rr_load(const K* __restrict__ A, const K* __restrict__ B,
+                                        const uint8_t* __restrict__ flags, const uint32_t* __restrict__ splits,
+                                        uint32_t u, uint32_t d0, uint32_t tn, K* s_stage, K* v, uint32_t& fl) {
+  const int tid = threadIdx.x;
+  const uint32_t i0 = (uint32_t)tid * RB_ITEMS;
+  fl = 0;
+  if (MERGE) {
+    const uint32_t a0 = splits[u], a1 = splits[u + 1];
+    const uint32_t b0 = d0 - a0, b1 = d0 + tn - a1;
+    const uint32_t la = a1 - a0, lb = b1 - b0;  // la + lb == tn <= RB_TILE
+    {
+      K w[RB_ITEMS];
+#pragma unroll
+      for (int k = 0; k < RB_ITEMS; ++k) {
+        const uint32_t i = tid + k * RB_THREADS;
+        w[k] = i < la ? A[a0 + i] : (i < la + lb ? B[b0 + (i - la)] : (K)0);
+      }
+#pragma unroll
+      for (int k = 0; k < RB_ITEMS; ++k) {
+        const uint32_t i = tid + k * RB_THREADS;
+        if (i < la + lb) s_stage[i] = w[k];
+      }
+    }
+    __syncthreads();
+    const K* sA = s_stage;
+    const K* sB = s_stage + la;
+    const uint32_t ld = i0 < tn ? i0 : tn;
+    uint32_t ai = merge_path(sA, la, sB, lb, ld);
+    uint32_t bi = ld - ai;
+#pragma unroll
+    for (int j = 0; j < RB_ITEMS; ++j) {
+      K x = 0;
+      if (ld + j < tn) {
+        const bool takeA = bi >= lb || (ai < la && sA[ai] <= sB[bi]);
+        x = takeA ? sA[ai++] : sB[bi++];
+      }
+      v[j] = x;
+    }
+  } else {
+    const uint64_t g0 = (uint64_t)d0 + i0;
+    if (i0 + RB_ITEMS <= tn) {
+      // d0 is a multiple of RB_TILE and A is 256-byte aligned
+      constexpr int PER = 16 / (int)sizeof(K);
+      const uint4* p = reinterpret_cast<const uint4*>(A + g0);
+#pragma unroll
+      for (int k = 0; k < RB_ITEMS / PER; ++k) {
+        const uint4 x = p[k];
+        if (sizeof(K) == 8) {
+          v[2 * k] = (K)(((uint64_t)x.y << 32) | x.x);
+          v[2 * k + 1] = (K)(((uint64_t)x.w << 32) | x.z);
+        } else {
+          v[4 * k] = (K)x.x;
+          v[4 * k + 1] = (K)x.y;
+          v[4 * k + 2] = (K)x.z;
+          v[4 * k + 3] = (K)x.w;
+        }
+      }
+      if (MODE == KEEP_FLAGS) {
+        const uint2 f = *reinterpret_cast<const uint2*>(flags + g0);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          fl |= ((f.x >> (8 * k)) & 1u) << k;
+          fl |= ((f.y >> (8 * k)) & 1u) << (4 + k);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < RB_ITEMS; ++k) {
+        v[k] = (i0 + k < tn) ? A[g0 + k] : (K)0;
+        if (MODE == KEEP_FLAGS && i0 + k < tn && flags[g0 + k]) fl |= 1u << k;
+      }
+    }
+  }
+}
+
+template <typename K, int MODE>
+__device__ __forceinline__ uint32_t rr_classify(const K* v, uint32_t fl, uint32_t tn, uint32_t r, uint32_t mq,
+                                                RS& agg) {
+  uint32_t keepbits = 0;
+  agg = rs_none();
+  const uint32_t i0 = (uint32_t)threadIdx.x * RB_ITEMS;
+#pragma unroll
+  for (int j = 0; j < RB_ITEMS; ++j) {
+    if (i0 + j < tn && (MODE == KEEP_ALL || ((fl >> j) & 1u))) {
+      keepbits |= 1u << j;
+      const uint32_t fq = (uint32_t)(v[j] >> r) & mq;
+      const int64_t fqi = (int64_t)fq;
+      agg.l = agg.l == LOC_NONE ? fqi : (fqi > agg.l + 1 ? fqi : agg.l + 1);
+      agg.c++;
+      agg.f = fq;
+    }
+  }
+  return keepbits;
+}
+
+template <typename K, int MODE, bool MERGE>
+__global__ __launch_bounds__(RB_THREADS, 8) void k_rr_agg(const K* __restrict__ A, uint32_t na,
+                                                          const K* __restrict__ B, uint32_t nbk,
+                                                          const uint8_t* __restrict__ flags,
+                                                          const uint32_t* __restrict__ splits, uint32_t q, uint32_t r,
+                                                          uint32_t chunk, uint4* __restrict__ agg) {
+  __shared__ K s_stage[MERGE ? RB_TILE : 1];
+  __shared__ uint4 s_wave[RB_WAVES + 1];
+  const uint64_t N = (uint64_t)na + (MERGE ? nbk : 0u);
+  const uint32_t mq = (uint32_t)mask64(q);
+  const uint64_t c0 = (uint64_t)blockIdx.x * chunk;
+  const uint64_t c1 = c0 + chunk < N ? c0 + chunk : N;
+  RS X = rs_none();
+  for (uint64_t d0 = c0; d0 < c1; d0 += RB_TILE) {
+    const uint32_t tn = (uint32_t)(c1 - d0 < (uint64_t)RB_TILE ? c1 - d0 : (uint64_t)RB_TILE);
+    K v[RB_ITEMS];
+    uint32_t fl;
+    rr_load<K, MODE, MERGE>(A, B, flags, splits, (uint32_t)(d0 / RB_TILE), (uint32_t)d0, tn, s_stage, v, fl);
+    RS a;
+    (void)rr_classify<K, MODE>(v, fl, tn, r, mq, a);
+    RS ex;
+    X = rs_combine(X, rs_block(a, &ex, s_wave));
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) agg[blockIdx.x] = rs_pack(X);
+}
+
+// Zeroes the table (meta and remainders), unless the scan raised a capacity error.
+__global__ __launch_bounds__(256) void k_rr_clear(uint8_t* __restrict__ t, uint64_t bytes,
+                                                  const uint32_t* __restrict__ ctr) {
+  if (ctr[CTR_ERR]) return;
+  const uint4 z = make_uint4(0u, 0u, 0u, 0u);
+  const uint64_t n16 = bytes / 16;
+  uint4* t16 = reinterpret_cast<uint4*>(t);
+  for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n16; i += (uint64_t)gridDim.x * 256) t16[i] = z;
+  if (blockIdx.x == 0 && threadIdx.x < bytes - 16 * n16) t[16 * n16 + threadIdx.x] = 0;
+}
+
+// One kept entry's contributions to the window [jw0, jw1) (block indices).
+// prev / lastf: location and quotient of the previous kept entry (-1 /
+// F_NONE when there is none).
+template <int R>
+__device__ __forceinline__ void rr_visit(uint32_t fq, uint32_t fr, int32_t lc, int32_t prev, uint32_t lastf,
+                                         uint32_t r, uint32_t jw0, uint32_t jw1, uint32_t* s_occ, uint32_t* s_end,
+                                         uint32_t* s_off, uint32_t* s_rem) {
+  const uint32_t sw0 = jw0 * 64;
+  const uint32_t lb = (uint32_t)lc >> 6;
+  if (lb >= jw0 && lb < jw1) {
+    const uint32_t s = (uint32_t)lc - sw0;
+    if (R == 8) {
+      reinterpret_cast<uint8_t*>(s_rem)[s] = (uint8_t)fr;
+    } else if (R == 16) {
+      reinterpret_cast<uint16_t*>(s_rem)[s] = (uint16_t)fr;
+    } else {
+      const uint32_t bit = s * r;
+      atomicOr(&s_rem[bit >> 5], fr << (bit & 31));
+      if ((bit & 31) + r > 32) atomicOr(&s_rem[(bit >> 5) + 1], fr >> (32 - (bit & 31)));
+    }
+  }
+  if (fq != lastf) {
+    const uint32_t qb = fq >> 6;
+    if (qb >= jw0 && qb < jw1) atomicOr(&s_occ[2 * (qb - jw0) + ((fq >> 5) & 1u)], 1u << (fq & 31));
+    if (prev >= 0) {
+      const uint32_t pb = (uint32_t)prev >> 6;
+      if (pb >= jw0 && pb < jw1)
+        atomicOr(&s_end[2 * (pb - jw0) + (((uint32_t)prev >> 5) & 1u)], 1u << ((uint32_t)prev & 31));
+      // blocks whose first slot lies in (lastf, min(fq, prev)]: offset = slots
+      // of block j taken by entries of quotients < 64 j
+      const uint32_t lim = fq < (uint32_t)prev ? fq : (uint32_t)prev;
+      uint32_t j = (lastf >> 6) + 1;
+      if (j < jw0) j = jw0;
+      const uint32_t je = (lim >> 6) + 1 < jw1 ? (lim >> 6) + 1 : jw1;
+      for (; j < je; ++j) s_off[j - jw0] = ((uint32_t)prev - 64 * j + 1) & 0xFFu;
+    }
+  }
+}
+
+template <typename K, int MODE, bool MERGE, int R>
+__global__ __launch_bounds__(RB_THREADS, 4) void k_rr_write(uint8_t* __restrict__ base, uint32_t nb, uint32_t q,
+                                                            uint32_t r, const K* __restrict__ A, uint32_t na,
+                                                            const K* __restrict__ B, uint32_t nbk,
+                                                            const uint8_t* __restrict__ flags,
+                                                            const uint32_t* __restrict__ splits, uint32_t chunk,
+                                                            const uint4* __restrict__ pre,
+                                                            const uint32_t* __restrict__ ctr) {
+  if (ctr[CTR_ERR]) return;  // capacity error: the table stays as it was
+  __shared__ K s_stage[MERGE ? RB_TILE : 1];
+  __shared__ __align__(16) uint32_t s_rem[RW_WB * 32];  // 2r words per block, r <= 16
+  __shared__ uint32_t s_occ[RW_WB * 2];
+  __shared__ uint32_t s_end[RW_WB * 2];
+  __shared__ uint32_t s_off[RW_WB];
+  __shared__ uint4 s_wave[RB_WAVES + 1];
+  __shared__ uint32_t s_b[2];
+  const int tid = threadIdx.x;
+  const uint32_t rr = R ? (uint32_t)R : r;
+  const uint32_t mq = (uint32_t)mask64(q);
+  const uint32_t rmask = (1u << rr) - 1u;
+  const uint64_t N = (uint64_t)na + (MERGE ? nbk : 0u);
+  const uint64_t c0 = (uint64_t)blockIdx.x * chunk;
+  const uint64_t c1 = c0 + chunk < N ? c0 + chunk : N;
+  uint64_t* meta = reinterpret_cast<uint64_t*>(base);
+  uint32_t* remg = reinterpret_cast<uint32_t*>(base + rsqf_meta_bytes(nb));
+  const uint32_t rw = 2 * rr;  // remainder words per block
+  RS X = rs_unpack(pre[blockIdx.x]);
+  for (uint64_t d0 = c0; d0 < c1; d0 += RB_TILE) {
+    const uint32_t u = (uint32_t)(d0 / RB_TILE);
+    const uint32_t tn = (uint32_t)(c1 - d0 < (uint64_t)RB_TILE ? c1 - d0 : (uint64_t)RB_TILE);
+    K v[RB_ITEMS];
+    uint32_t fl;
+    rr_load<K, MODE, MERGE>(A, B, flags, splits, u, (uint32_t)d0, tn, s_stage, v, fl);
+    RS a;
+    const uint32_t keepbits = rr_classify<K, MODE>(v, fl, tn, rr, mq, a);
+    if (tid == 0) s_b[0] = 0xFFFFFFFFu;
+    RS ex;
+    const RS sub = rs_block(a, &ex, s_wave);
+    const RS me = rs_combine(X, ex);
+    const RS Y = rs_combine(X, sub);
+    if (sub.c > 0) {
+      const int32_t prev0 = me.l == LOC_NONE ? -1 : (int32_t)me.l;
+      const uint32_t lastf0 = me.f;
+      // blocks this thread's contributions touch: [t_lo, t_hi]
+      uint32_t t_lo = 0xFFFFFFFFu, t_hi = 0;
+      if (keepbits) {
+        const int j0 = __builtin_ctz(keepbits);
+        const uint32_t fq = (uint32_t)(v[j0] >> rr) & mq;
+        t_lo = fq >> 6;
+        if (fq != lastf0 && prev0 >= 0) {
+          const uint32_t pb = (uint32_t)prev0 >> 6;
+          if (pb < t_lo) t_lo = pb;
+          const uint32_t lim = fq < (uint32_t)prev0 ? fq : (uint32_t)prev0;
+          const uint32_t oj = (lastf0 >> 6) + 1;
+          if (oj <= (lim >> 6) && oj < t_lo) t_lo = oj;
+        }
+        int32_t prev = prev0;
+#pragma unroll
+        for (int j = 0; j < RB_ITEMS; ++j) {
+          if (keepbits & (1u << j)) {
+            const int32_t f = (int32_t)((uint32_t)(v[j] >> rr) & mq);
+            prev = f > prev + 1 ? f : prev + 1;
+          }
+        }
+        t_hi = (uint32_t)prev >> 6;
+        atomicMin(&s_b[0], t_lo);
+        if (ex.c == 0) s_b[1] = fq >> 6;  // the sub-tile's first kept entry
+      }
+      __syncthreads();
+      const uint32_t B0 = s_b[0];
+      const uint32_t B1 = (uint32_t)(Y.l >> 6) + 1;
+      const uint32_t qfirst = s_b[1];      // quotient blocks of the first and last
+      const uint32_t qlast = Y.f >> 6;     // kept entries: occupied words shared
+      const uint32_t Llo = X.l == LOC_NONE ? 0u : (uint32_t)(X.l + 1);  // remainders: [Llo, Lhi)
+      const uint32_t Lhi = (uint32_t)(Y.l + 1);
+      const int64_t Elo = X.l == LOC_NONE ? -1 : X.l;  // run ends: [Elo, Ehi)
+      const int64_t Ehi = Y.l;
+      for (uint32_t jw0 = B0; jw0 < B1; jw0 += RW_WB) {
+        const uint32_t jw1 = jw0 + RW_WB < B1 ? jw0 + RW_WB : B1;
+        const uint32_t nblk = jw1 - jw0;
+        // several windows (sparse table): skip those without contributions
+        if (B1 - B0 > RW_WB && !__syncthreads_or(t_lo < jw1 && t_hi >= jw0)) continue;
+        for (uint32_t i = tid; i < nblk * rw; i += RB_THREADS) s_rem[i] = 0u;
+        for (uint32_t i = tid; i < nblk; i += RB_THREADS) {
+          s_occ[2 * i] = s_occ[2 * i + 1] = 0u;
+          s_end[2 * i] = s_end[2 * i + 1] = 0u;
+          s_off[i] = 0u;
+        }
+        __syncthreads();
+        {
+          int32_t prev = prev0;
+          uint32_t lastf = lastf0;
+#pragma unroll
+          for (int j = 0; j < RB_ITEMS; ++j) {
+            if (keepbits & (1u << j)) {
+              const uint32_t fq = (uint32_t)(v[j] >> rr) & mq;
+              const int32_t lc = (int32_t)fq > prev + 1 ? (int32_t)fq : prev + 1;
+              rr_visit<R>(fq, (uint32_t)v[j] & rmask, lc, prev, lastf, rr, jw0, jw1, s_occ, s_end, s_off, s_rem);
+              prev = lc;
+              lastf = fq;
+            }
+          }
+        }
+        __syncthreads();
+        // metadata: one thread per block and field
+        for (uint32_t i = tid; i < nblk; i += RB_THREADS) {
+          const uint32_t jb = jw0 + i;
+          const uint64_t occ = ((uint64_t)s_occ[2 * i + 1] << 32) | s_occ[2 * i];
+          const uint64_t end = ((uint64_t)s_end[2 * i + 1] << 32) | s_end[2 * i];
+          const uint32_t off = s_off[i];
+          uint64_t* mrec = meta + 4 * (size_t)jb;
+          if (occ) {
+            if (jb == qfirst || jb == qlast) atomicOr((unsigned long long*)&mrec[0], (unsigned long long)occ);
+            else mrec[0] = occ;
+          }
+          if (end) {
+            const bool own = 64 * (int64_t)jb >= Elo && 64 * (int64_t)jb + 64 <= Ehi;
+            if (own) mrec[1] = end;
+            else atomicOr((unsigned long long*)&mrec[1], (unsigned long long)end);
+          }
+          if (off) mrec[2] = off;
+        }
+        // remainders: whole blocks owned by this sub-tile are stored, shared
+        // ones ORed word by word
+        for (uint32_t i = tid; i < nblk * rw; i += RB_THREADS) {
+          const uint32_t jb = jw0 + i / rw;
+          const uint32_t x = s_rem[i];
+          const bool own = 64u * jb >= Llo && 64u * jb + 64 <= Lhi;
+          uint32_t* g = remg + (size_t)jw0 * rw + i;
+          if (own) *g = x;
+          else if (x) atomicOr(g, x);
+        }
+        __syncthreads();
+      }
+    }
+    X = Y;
+    __syncthreads();
+  }
+}
+
+// The last run end and the offsets of the blocks after the last quotient,
+// from the scan total {c, l = last location, f = last quotient}.
+__global__ __launch_bounds__(64) void k_rr_tail(uint8_t* __restrict__ base, const uint4* __restrict__ total,
+                                                const uint32_t* __restrict__ ctr) {
+  if (ctr[CTR_ERR]) return;
+  const RS t = rs_unpack(*total);
+  if (t.c == 0) return;
+  uint64_t* meta = reinterpret_cast<uint64_t*>(base);
+  const uint32_t L = (uint32_t)t.l, F = t.f;
+  if (threadIdx.x == 0) atomicOr((unsigned long long*)&meta[4 * (size_t)(L >> 6) + 1], 1ull << (L & 63));
+  for (uint32_t j = (F >> 6) + 1 + threadIdx.x; j <= (L >> 6); j += 64)
+    meta[4 * (size_t)j + 2] = (L - 64 * j + 1) & 0xFFu;
 }
 
 // ---------------------------------------------------------------- pipelines
@@ -305,18 +699,6 @@ static RsqfDev rdev(const gqf_rsqf& f) {
   return d;
 }
 
-static void reset_op_counters_r(hipStream_t s, DeviceState& ds) {
-  GQF_LAUNCH(k_reset_counters, dim3(1), dim3(64), 0, s, ds.counters, (int)CTR_SCAN_TILE, (int)CTR_MISC);
-}
-
-template <typename K>
-static void launch_rwrite(hipStream_t s, DeviceState& ds, const gqf_rsqf& f, const K* ufp, const uint32_t* loc,
-                          uint64_t items) {
-  const uint32_t groups = (f.numBlocks + RW_BLOCKS - 1) / RW_BLOCKS;
-  GQF_LAUNCH_B((double)items * (sizeof(K) + 4) + (double)rsqf_dev_bytes(f.qbits, f.rbits), k_rsqf_write<K>, dim3(groups), dim3(256), 0, s, reinterpret_cast<uint8_t*>(f.blocks), f.numBlocks,
-                     f.rbits, ufp, loc, ds.counters);
-}
-
 static gqf_status read_ctr(hipStream_t s, DeviceState& ds, const uint32_t* dptr, uint32_t* out) {
   GQF_HIP(hipMemcpyAsync(ds.h_pinned + CTR_MISC + 4, dptr, 4, hipMemcpyDeviceToHost, s));
   GQF_HIP(hipStreamSynchronize(s));
@@ -324,26 +706,92 @@ static gqf_status read_ctr(hipStream_t s, DeviceState& ds, const uint32_t* dptr,
   return GQF_OK;
 }
 
+static inline uint32_t rd_windows(uint32_t nb) { return (nb + RD_BLOCKS - 1) / RD_BLOCKS; }
+
+static size_t rsqf_decode_ws(const RsqfGeom& g, size_t kb) {
+  const uint32_t W = rd_windows(g.nb);
+  return aligned_bytes((size_t)W * 8) + aligned_bytes((size_t)W * 4) + 256 + aligned_bytes((g.cap + 1) * kb) + 256;
+}
+
+// The stored fingerprints in ascending order (*m_out of them).
 template <typename K>
-static gqf_status rsqf_decode(hipStream_t s, DeviceState& ds, Arena& ar, const gqf_rsqf& f, K** out, uint32_t* m_out) {
+static gqf_status rsqf_decode(hipStream_t s, DeviceState& ds, Arena& ar, const gqf_rsqf& f, K** out,
+                              uint32_t* m_out) {
   const RsqfDev d = rdev(f);
-  uint32_t* counts = ar.take<uint32_t>(f.numBlocks);
-  uint32_t* total = ar.take<uint32_t>(1);
+  const uint32_t W = rd_windows(f.numBlocks);
+  uint32_t* wp = ar.take<uint32_t>((size_t)W * 2);
+  uint32_t* wc = ar.take<uint32_t>(W);
+  uint32_t* tot = ar.take<uint32_t>(4);
   if (ar.used > ar.cap) {
     set_error("internal: decode workspace overflow");
     return GQF_ERR_INVALID;
   }
-  const dim3 grid((f.numBlocks + 255) / 256);
-  GQF_LAUNCH_B((double)rsqf_meta_bytes(f.numBlocks), (k_rsqf_decode<false, K>), grid, dim3(256), 0, s, d, counts, (K*)nullptr);
-  GQF_LAUNCH(k_scan_rows<1>, dim3(1), dim3(1024), 0, s, counts, f.numBlocks, total);
-  GQF_TRY(read_ctr(s, ds, total, m_out));
-  K* fps = ar.take<K>(*m_out + 1);
-  if (ar.used > ar.cap) {
+  const double meta = (double)f.numBlocks * 16;  // occupieds + run ends
+  GQF_LAUNCH_B(meta, k_rd_pop, dim3((W + 3) / 4), dim3(256), 0, s, d.meta, f.numBlocks, W, wp);
+  GQF_LAUNCH(k_scan_rows<2>, dim3(1), dim3(1024), 0, s, wp, W, tot);
+  GQF_LAUNCH_B(meta, k_rd_count, dim3(W), dim3(RD_THREADS), 0, s, d, wp, wc);
+  GQF_LAUNCH(k_scan_rows<1>, dim3(1), dim3(1024), 0, s, wc, W, tot + 2);
+  GQF_TRY(read_ctr(s, ds, tot + 2, m_out));
+  K* fps = ar.take<K>((size_t)*m_out + 1);
+  if (ar.used > ar.cap || *m_out > f.numBlocks * 64ull) {
     set_error("internal: decode workspace overflow");
     return GQF_ERR_INVALID;
   }
-  GQF_LAUNCH_B((double)rsqf_dev_bytes(f.qbits, f.rbits) + (double)*m_out * sizeof(K), (k_rsqf_decode<true, K>), grid, dim3(256), 0, s, d, counts, fps);
+  const double bytes = (double)rsqf_dev_bytes(f.qbits, f.rbits) - (double)f.numBlocks * 16 + (double)*m_out * sizeof(K);
+  if (f.rbits == 8)
+    GQF_LAUNCH_B(bytes, (k_rd_emit<K, 8>), dim3(W), dim3(RD_THREADS), 0, s, d, wp, wc, fps);
+  else if (f.rbits == 16)
+    GQF_LAUNCH_B(bytes, (k_rd_emit<K, 16>), dim3(W), dim3(RD_THREADS), 0, s, d, wp, wc, fps);
+  else
+    GQF_LAUNCH_B(bytes, (k_rd_emit<K, 0>), dim3(W), dim3(RD_THREADS), 0, s, d, wp, wc, fps);
   *out = fps;
+  return GQF_OK;
+}
+
+static size_t rsqf_rebuild_ws(uint64_t N) {
+  const RbPlan p = rb_plan(N ? N : 1);
+  return aligned_bytes((size_t)(p.nsub + 1) * 4) + aligned_bytes((size_t)p.G * 16) + 256 + 1024;
+}
+
+// Canonical layout of A (MERGE: merged with B; KEEP_FLAGS: the flagged
+// entries of A) written over the table.
+template <typename K, int MODE, bool MERGE>
+static gqf_status rsqf_rebuild(hipStream_t s, DeviceState& ds, Arena& ar, const gqf_rsqf& f, const K* A,
+                               uint32_t na, const K* B, uint32_t nbk, const uint8_t* flags) {
+  const RsqfGeom g = rgeom(f);
+  const uint64_t N = (uint64_t)na + (MERGE ? nbk : 0u);
+  const size_t tbytes = rsqf_dev_bytes(g.q, g.r);
+  GQF_LAUNCH(k_reset_counters, dim3(1), dim3(64), 0, s, ds.counters, (int)CTR_OP, (int)CTR_TOTAL);
+  if (N == 0) {
+    GQF_LAUNCH_B((double)tbytes, k_rr_clear, dim3(1024), dim3(256), 0, s, reinterpret_cast<uint8_t*>(f.blocks),
+                 (uint64_t)tbytes, ds.counters);
+    return GQF_OK;
+  }
+  const RbPlan p = rb_plan(N);
+  uint32_t* splits = ar.take<uint32_t>(p.nsub + 1);
+  uint4* agg = ar.take<uint4>(p.G);
+  uint4* total = ar.take<uint4>(1);
+  if (ar.used > ar.cap) {
+    set_error("internal: rebuild workspace overflow");
+    return GQF_ERR_INVALID;
+  }
+  if (MERGE)
+    GQF_LAUNCH(k_rb_splits<K>, dim3((p.nsub + 1 + 255) / 256), dim3(256), 0, s, A, na, B, nbk, p.nsub, splits);
+  GQF_LAUNCH_B((double)N * (sizeof(K) + (MODE == KEEP_FLAGS ? 1 : 0)), (k_rr_agg<K, MODE, MERGE>), dim3(p.G),
+               dim3(RB_THREADS), 0, s, A, na, B, nbk, flags, splits, g.q, g.r, p.chunk, agg);
+  GQF_LAUNCH(k_rb_scan, dim3(1), dim3(1024), 0, s, agg, p.G, g.cap, ds.counters, total);
+  GQF_LAUNCH_B((double)tbytes, k_rr_clear, dim3(1024), dim3(256), 0, s, reinterpret_cast<uint8_t*>(f.blocks),
+               (uint64_t)tbytes, ds.counters);
+  const double wbytes = (double)N * (sizeof(K) + (MODE == KEEP_FLAGS ? 1 : 0)) + (double)tbytes;
+#define GQF_RR_WRITE(R_)                                                                                         \
+  GQF_LAUNCH_B(wbytes, (k_rr_write<K, MODE, MERGE, R_>), dim3(p.G), dim3(RB_THREADS), 0, s,                     \
+               reinterpret_cast<uint8_t*>(f.blocks), g.nb, g.q, g.r, A, na, B, nbk, flags, splits, p.chunk, agg, \
+               ds.counters)
+  if (g.r == 8) GQF_RR_WRITE(8);
+  else if (g.r == 16) GQF_RR_WRITE(16);
+  else GQF_RR_WRITE(0);
+#undef GQF_RR_WRITE
+  GQF_LAUNCH(k_rr_tail, dim3(1), dim3(64), 0, s, reinterpret_cast<uint8_t*>(f.blocks), total, ds.counters);
   return GQF_OK;
 }
 
@@ -360,57 +808,48 @@ template <typename K>
 static gqf_status rsqf_build_t(const gqf_rsqf& f, uint32_t n, const uint32_t* keys, CallScope& cs) {
   DeviceState& ds = *cs.ds;
   const RsqfGeom g = rgeom(f);
-  GQF_TRY(ws_reserve(ds, sort_workspace_bytes(n, sizeof(K), false) + layout_workspace_bytes(n, sizeof(K))));
+  GQF_TRY(ws_reserve(ds, sort_workspace_bytes(n, sizeof(K), false) + rsqf_rebuild_ws(n)));
   Arena ar(ds.ws, ds.ws_cap);
-  reset_op_counters_r(cs.stream, ds);
   K* sorted;
   GQF_TRY(radix_sort<K>(cs.stream, ds, ar, keys, true, g.bits, nullptr, false, n, 0, g.bits, &sorted, nullptr));
-  K* ufp;
-  uint32_t* loc;
-  GQF_TRY((layout_scan<K, KEEP_ALL>(cs.stream, ds, ar, sorted, nullptr, n, g.q, g.r, 0, g.cap, true, &ufp, &loc)));
-  launch_rwrite<K>(cs.stream, ds, f, ufp, loc, n);
-  return GQF_OK;
+  return rsqf_rebuild<K, KEEP_ALL, false>(cs.stream, ds, ar, f, sorted, n, (const K*)nullptr, 0u, nullptr);
 }
 
-static size_t rsqf_decode_ws(const RsqfGeom& g, size_t kb) {
-  return aligned_bytes((size_t)g.nb * 4) + 256 + aligned_bytes((g.cap + 1) * kb) + 256;
+template <typename K>
+static size_t rsqf_insert_ws(const RsqfGeom& g, uint32_t n) {
+  return rsqf_decode_ws(g, sizeof(K)) + sort_workspace_bytes(n, sizeof(K), false) + rsqf_rebuild_ws(g.cap + n);
 }
 
 template <typename K>
 static gqf_status rsqf_insert_t(const gqf_rsqf& f, uint32_t n, const uint32_t* keys, CallScope& cs) {
   DeviceState& ds = *cs.ds;
   const RsqfGeom g = rgeom(f);
-  GQF_TRY(ws_reserve(ds, rsqf_decode_ws(g, sizeof(K)) + sort_workspace_bytes(n, sizeof(K), false) +
-                             aligned_bytes((g.cap + n + 1) * 8) + layout_workspace_bytes(g.cap + n, 8)));
+  GQF_TRY(ws_reserve(ds, rsqf_insert_ws<K>(g, n)));
   Arena ar(ds.ws, ds.ws_cap);
-  reset_op_counters_r(cs.stream, ds);
   K* A;
   uint32_t m = 0;
   GQF_TRY(rsqf_decode<K>(cs.stream, ds, ar, f, &A, &m));
-  K* B;
-  GQF_TRY(radix_sort<K>(cs.stream, ds, ar, keys, true, g.bits, nullptr, false, n, 0, g.bits, &B, nullptr));
-  uint64_t* merged = ar.take<uint64_t>((size_t)m + n + 1);
-  if (ar.used > ar.cap || (uint64_t)m + n >= GQF_MAX_ITEMS) {
-    set_error("table holds %u entries: decode exceeds the workspace (not a canonical table?)", m);
+  if ((uint64_t)m + n >= GQF_MAX_ITEMS) {
+    set_error("table holds %u entries: %u more exceed the per-call limit", m, n);
     return GQF_ERR_INVALID;
   }
-  launch_merge_tagged<K>(cs.stream, A, m, B, n, merged);
-  uint64_t* ufp;
-  uint32_t* loc;
-  GQF_TRY((layout_scan<uint64_t, KEEP_ALL>(cs.stream, ds, ar, merged, nullptr, m + n, g.q, g.r, 1, g.cap, true, &ufp,
-                                           &loc)));
-  launch_rwrite<uint64_t>(cs.stream, ds, f, ufp, loc, (uint64_t)m + n);
-  return GQF_OK;
+  K* B;
+  GQF_TRY(radix_sort<K>(cs.stream, ds, ar, keys, true, g.bits, nullptr, false, n, 0, g.bits, &B, nullptr));
+  return rsqf_rebuild<K, KEEP_ALL, true>(cs.stream, ds, ar, f, A, m, B, n, nullptr);
+}
+
+template <typename K>
+static size_t rsqf_delete_ws(const RsqfGeom& g, uint32_t n) {
+  return rsqf_decode_ws(g, sizeof(K)) + sort_workspace_bytes(n, sizeof(K), false) + aligned_bytes(g.cap + 1) +
+         rsqf_rebuild_ws(g.cap);
 }
 
 template <typename K>
 static gqf_status rsqf_delete_t(const gqf_rsqf& f, uint32_t n, const uint32_t* keys, CallScope& cs) {
   DeviceState& ds = *cs.ds;
   const RsqfGeom g = rgeom(f);
-  GQF_TRY(ws_reserve(ds, rsqf_decode_ws(g, sizeof(K)) + sort_workspace_bytes(n, sizeof(K), false) +
-                             aligned_bytes(g.cap + 1) + layout_workspace_bytes(g.cap, sizeof(K))));
+  GQF_TRY(ws_reserve(ds, rsqf_delete_ws<K>(g, n)));
   Arena ar(ds.ws, ds.ws_cap);
-  reset_op_counters_r(cs.stream, ds);
   K* A;
   uint32_t m = 0;
   GQF_TRY(rsqf_decode<K>(cs.stream, ds, ar, f, &A, &m));
@@ -421,12 +860,10 @@ static gqf_status rsqf_delete_t(const gqf_rsqf& f, uint32_t n, const uint32_t* k
     set_error("internal: delete workspace overflow");
     return GQF_ERR_INVALID;
   }
-  if (m) GQF_LAUNCH_B((double)m * (sizeof(K) + 1), k_delete_mark<K>, dim3((m + DM_TILE - 1) / DM_TILE), dim3(DM_THREADS), 0, cs.stream, A, m, D, n, keep);
-  K* ufp;
-  uint32_t* loc;
-  GQF_TRY((layout_scan<K, KEEP_FLAGS>(cs.stream, ds, ar, A, keep, m, g.q, g.r, 0, g.cap, true, &ufp, &loc)));
-  launch_rwrite<K>(cs.stream, ds, f, ufp, loc, m);
-  return GQF_OK;
+  if (m)
+    GQF_LAUNCH_B((double)m * (sizeof(K) + 1), k_delete_mark<K>, dim3((m + DM_TILE - 1) / DM_TILE), dim3(DM_THREADS),
+                 0, cs.stream, A, m, D, n, keep);
+  return rsqf_rebuild<K, KEEP_FLAGS, false>(cs.stream, ds, ar, f, A, m, (const K*)nullptr, 0u, keep);
 }
 
 static void launch_rlookup(hipStream_t s, const gqf_rsqf& f, uint32_t n, const uint32_t* keys, uint32_t* out) {

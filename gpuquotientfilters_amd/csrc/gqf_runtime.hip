@@ -73,21 +73,6 @@ gqf_status ws_reserve(DeviceState& ds, size_t bytes) {
   return GQF_OK;
 }
 
-gqf_status status_reserve(DeviceState& ds, size_t words) {
-  if (words <= ds.status_cap) return GQF_OK;
-  size_t cap = words + words / 4 + 4096;
-  GQF_HIP(hipStreamSynchronize(ds.stream));
-  if (ds.status) GQF_HIP(hipFree(ds.status));
-  ds.status = nullptr;
-  ds.status_cap = 0;
-  GQF_HIP(hipMalloc((void**)&ds.status, cap * sizeof(unsigned long long)));
-  // Tagged look-back words must never match a live epoch by accident.
-  GQF_HIP(hipMemsetAsync(ds.status, 0, cap * sizeof(unsigned long long), ds.stream));
-  GQF_HIP(hipStreamSynchronize(ds.stream));
-  ds.status_cap = cap;
-  return GQF_OK;
-}
-
 // ---------------------------------------------------------------- kernel timing
 struct ProfStat {
   std::string name;

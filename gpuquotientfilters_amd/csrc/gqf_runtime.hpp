@@ -32,15 +32,15 @@ gqf_status hip_fail(hipError_t e, const char* what, const char* file, int line);
 // ---------------------------------------------------------------- device state
 // Small device counters used by the kernels (index constants below).
 enum Counter : int {
-  CTR_SORT_TILE = 0,   // 8 slots: dynamic tile ids of the radix passes
-  CTR_SCAN_TILE = 8,   // dynamic tile id of the layout scan
-  CTR_COUNT = 9,       // number of entries kept by the layout scan
+  CTR_OP = 8,          // first counter reset by every rebuild
+  CTR_COUNT = 9,       // number of entries kept by the location scan
   CTR_ERR = 10,        // error flags raised on the device (bit 0: capacity)
   CTR_MISC = 11,       // 5 general-purpose slots
   CTR_TOTAL = 16
 };
 constexpr uint32_t DEV_ERR_CAPACITY = 1u;
-// Look-back words carry 30-bit counts: one call handles fewer entries.
+// Per-call entry limit (scan states hold 32-bit counts and locations; the
+// merged table + batch of an insert must stay below it).
 constexpr uint64_t GQF_MAX_ITEMS = (1ull << 30) - 4096;
 
 struct ProfRec {
@@ -59,11 +59,8 @@ struct DeviceState {
   std::mutex mu;
   char* ws = nullptr;               // workspace arena (grow-only)
   size_t ws_cap = 0;
-  unsigned long long* status = nullptr;  // look-back status words, zeroed at allocation
-  size_t status_cap = 0;                 // in words
   uint32_t* counters = nullptr;     // CTR_TOTAL device counters
   uint32_t* h_pinned = nullptr;     // pinned host mirror of the counters
-  uint32_t epoch = 0;               // radix-pass epoch (status words are tagged with it)
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // per-launch timing (gqf_profile_enable): event pool and launches to resolve
   hipEvent_t* pool = nullptr;
@@ -76,7 +73,6 @@ struct DeviceState {
 // on first use (contexts live until the process ends).
 gqf_status device_state(hipStream_t stream, DeviceState** out);
 gqf_status ws_reserve(DeviceState& ds, size_t bytes);
-gqf_status status_reserve(DeviceState& ds, size_t words);
 
 // Bump allocator over the workspace of one call.
 struct Arena {

@@ -11,7 +11,7 @@
 //   [decode table] -> [hash + radix sort keys] -> [merge / mark] ->
 //   layout scan (locations) -> table writer (every slot written once).
 // Lookups are the reference's run walk (quotientFilter.cu:172-196, 320-357).
-#include "gqf_kernels.hpp"
+#include "gqf_rebuild.hpp"
 
 namespace gqf {
 
@@ -287,72 +287,8 @@ __global__ __launch_bounds__(256) void k_sqf_lookup_sorted(const uint8_t* __rest
 // (quotientFilter.cu:788-857, 985-1038, 1492-1539), the merge rebuild
 // (quotientFilter.cu:1651-1719) and the end states of insert /
 // superclusterDeletes.
-constexpr int RB_THREADS = 512;
-constexpr int RB_ITEMS = 8;
-constexpr int RB_WAVES = RB_THREADS / 64;
-constexpr int RB_TILE = RB_THREADS * RB_ITEMS;
 constexpr uint32_t RB_WIN = 8192;  // table bytes per write window
 constexpr uint32_t RB_FIX = 64;    // per-sub-tile occupied-bit fix-up slots
-constexpr uint32_t F_NONE = 0xFFFFFFFFu;
-
-// Scan state of a block of entries: kept count c, location map
-// x -> max(x + c, l), and the quotient f of its last kept entry (run heads are
-// the kept entries whose quotient differs from the previous kept entry's).
-struct RS {
-  uint32_t c;
-  int64_t l;
-  uint32_t f;
-};
-__device__ __forceinline__ RS rs_combine(RS a, RS b) {
-  RS o;
-  o.c = a.c + b.c;
-  const int64_t x = a.l == LOC_NONE ? LOC_NONE : a.l + (int64_t)b.c;
-  o.l = x > b.l ? x : b.l;
-  o.f = b.c ? b.f : a.f;
-  return o;
-}
-__device__ __forceinline__ RS rs_none() { return RS{0u, LOC_NONE, F_NONE}; }
-// Stored form of a state: {c, l + 1 (0 = none), f, 0}.
-__device__ __forceinline__ uint4 rs_pack(RS x) {
-  return make_uint4(x.c, x.l == LOC_NONE ? 0u : (uint32_t)(x.l + 1), x.f, 0u);
-}
-__device__ __forceinline__ RS rs_unpack(uint4 w) { return RS{w.x, w.y == 0 ? LOC_NONE : (int64_t)w.y - 1, w.z}; }
-
-// Exclusive scan across the 64 lanes of a wave (DPP moves, see
-// wave_incl_add); *total = the wave aggregate.
-template <int CTRL, int ROW_MASK = 0xF>
-__device__ __forceinline__ RS rs_dpp(RS x) {
-  RS y;
-  y.c = dpp_u32<CTRL, ROW_MASK>(0u, x.c);
-  y.l = (int64_t)dpp_u64<CTRL, ROW_MASK>((uint64_t)LOC_NONE, (uint64_t)x.l);
-  y.f = dpp_u32<CTRL, ROW_MASK>(F_NONE, x.f);
-  return y;
-}
-__device__ __forceinline__ RS rs_wave_excl(RS x, RS* total) {
-  RS inc = x;
-  inc = rs_combine(rs_dpp<DPP_ROW_SHR | 1>(inc), inc);
-  inc = rs_combine(rs_dpp<DPP_ROW_SHR | 2>(inc), inc);
-  inc = rs_combine(rs_dpp<DPP_ROW_SHR | 4>(inc), inc);
-  inc = rs_combine(rs_dpp<DPP_ROW_SHR | 8>(inc), inc);
-  inc = rs_combine(rs_dpp<DPP_ROW_BCAST15, 0xA>(inc), inc);
-  inc = rs_combine(rs_dpp<DPP_ROW_BCAST31, 0xC>(inc), inc);
-  total->c = wave_read(inc.c, 63);
-  total->l = (int64_t)(((uint64_t)wave_read((uint32_t)((uint64_t)inc.l >> 32), 63) << 32) |
-                       wave_read((uint32_t)inc.l, 63));
-  total->f = wave_read(inc.f, 63);
-  return rs_dpp<DPP_WAVE_SHR1>(inc);
-}
-
-// Merge-path split (A entries among the first u*RB_TILE outputs) of every sub-tile boundary.
-__global__ __launch_bounds__(256) void k_rb_splits(const uint32_t* __restrict__ A, uint32_t na,
-                                                   const uint32_t* __restrict__ B, uint32_t nb, uint32_t nsub,
-                                                   uint32_t* __restrict__ splits) {
-  const uint32_t u = blockIdx.x * 256 + threadIdx.x;
-  if (u > nsub) return;
-  const uint64_t N = (uint64_t)na + nb;
-  const uint32_t d = (uint32_t)((uint64_t)u * RB_TILE < N ? (uint64_t)u * RB_TILE : N);
-  splits[u] = merge_path(A, na, B, nb, d);
-}
 
 // This thread's RB_ITEMS consecutive entries of sub-tile [d0, d0+tn):
 // v[1+j] = entry j and v[0] = the entry before them; bit j of tags = entry j
@@ -477,43 +413,6 @@ __device__ __forceinline__ uint32_t rb_classify(const uint32_t* v, uint32_t tags
   return keepbits;
 }
 
-struct RbPlan {
-  uint32_t G, chunk, nsub;  // chunks, entries per chunk (multiple of RB_TILE), sub-tiles
-};
-constexpr uint32_t RB_MAX_CHUNKS = 16384;
-static inline RbPlan rb_plan(uint64_t N) {
-  // one sub-tile per workgroup (latency is hidden by occupancy, not by a
-  // sequential walk), several per chunk only beyond RB_MAX_CHUNKS sub-tiles
-  RbPlan p;
-  p.nsub = (uint32_t)((N + RB_TILE - 1) / RB_TILE);
-  const uint32_t per = (p.nsub + RB_MAX_CHUNKS - 1) / RB_MAX_CHUNKS;
-  p.chunk = (per ? per : 1) * RB_TILE;
-  p.G = (uint32_t)((N + p.chunk - 1) / p.chunk);
-  if (p.G == 0) p.G = 1;
-  return p;
-}
-
-// Block-wide scan of the per-thread states: *ex_out = exclusive prefix of the
-// calling thread within the block, returns the block aggregate.  s_wave:
-// RB_WAVES + 1 entries (wave aggregates, then their prefixes and the total).
-__device__ __forceinline__ RS rs_block(RS a, RS* ex_out, uint4* s_wave) {
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  RS wt;
-  const RS ex = rs_wave_excl(a, &wt);
-  if (lane == 0) s_wave[wid] = rs_pack(wt);
-  __syncthreads();
-  if (wid == 0) {
-    const RS x = lane < RB_WAVES ? rs_unpack(s_wave[lane]) : rs_none();
-    RS tot;
-    const RS xe = rs_wave_excl(x, &tot);
-    if (lane < RB_WAVES) s_wave[lane] = rs_pack(xe);
-    if (lane == 0) s_wave[RB_WAVES] = rs_pack(tot);
-  }
-  __syncthreads();
-  *ex_out = rs_combine(rs_unpack(s_wave[wid]), ex);
-  return rs_unpack(s_wave[RB_WAVES]);
-}
-
 template <int MODE, bool MERGE>
 __global__ __launch_bounds__(RB_THREADS, 8) void k_rb_agg(const uint32_t* __restrict__ A, uint32_t na,
                                                 const uint32_t* __restrict__ B, uint32_t nb,
@@ -540,59 +439,6 @@ __global__ __launch_bounds__(RB_THREADS, 8) void k_rb_agg(const uint32_t* __rest
     __syncthreads();
   }
   if (threadIdx.x == 0) agg[blockIdx.x] = rs_pack(X);
-}
-
-// Exclusive scan of the chunk states by one workgroup of 1024 threads: in
-// pieces of 8192 states, each thread combines its 8 consecutive states, one
-// block-wide scan, then the thread rewrites its 8 states as exclusive
-// prefixes.  The total fixes the kept count, the first
-// slot after the last entry, and a capacity error -- all before any table
-// byte is written.
-__global__ __launch_bounds__(1024) void k_rb_scan(uint4* __restrict__ agg, uint32_t G, uint64_t S,
-                                                  uint32_t* __restrict__ ctr) {
-  constexpr uint32_t SEG = 8, PIECE = 1024 * SEG;
-  __shared__ uint4 s_wave[16];  // wave aggregates, then their exclusive prefixes
-  __shared__ uint4 s_tot;
-  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  RS carry = rs_none();
-  for (uint32_t base = 0; base < G; base += PIECE) {
-    const uint32_t i0 = base + (uint32_t)tid * SEG;
-    uint4 w[SEG];
-#pragma unroll
-    for (uint32_t k = 0; k < SEG; ++k) w[k] = i0 + k < G ? agg[i0 + k] : make_uint4(0u, 0u, F_NONE, 0u);
-    RS seg = rs_none();
-#pragma unroll
-    for (uint32_t k = 0; k < SEG; ++k) seg = rs_combine(seg, rs_unpack(w[k]));
-    RS wt;
-    RS ex = rs_wave_excl(seg, &wt);
-    if (lane == 0) s_wave[wid] = rs_pack(wt);
-    __syncthreads();
-    if (wid == 0) {
-      // wave 0 scans the 16 wave aggregates on top of the carry
-      RS x = lane < 16 ? rs_unpack(s_wave[lane]) : rs_none();
-      RS tot;
-      RS xe = rs_wave_excl(x, &tot);
-      if (lane < 16) s_wave[lane] = rs_pack(rs_combine(carry, xe));
-      if (lane == 0) s_tot = rs_pack(rs_combine(carry, tot));
-    }
-    __syncthreads();
-    ex = rs_combine(rs_unpack(s_wave[wid]), ex);
-#pragma unroll
-    for (uint32_t k = 0; k < SEG; ++k) {
-      if (i0 + k < G) {
-        agg[i0 + k] = rs_pack(ex);
-        ex = rs_combine(ex, rs_unpack(w[k]));
-      }
-    }
-    carry = rs_unpack(s_tot);
-    __syncthreads();
-  }
-  if (tid == 0) {
-    ctr[CTR_COUNT] = carry.c;
-    const int64_t L = carry.l;
-    ctr[CTR_MISC + 3] = L == LOC_NONE ? 0u : (uint32_t)(L + 1 < (int64_t)S ? L + 1 : (int64_t)S);
-    if (L != LOC_NONE && (uint64_t)L >= S) ctr[CTR_ERR] |= DEV_ERR_CAPACITY;
-  }
 }
 
 // Writes the element bytes of slot sl into the window (big-endian).
@@ -1071,7 +917,7 @@ static gqf_status launch_rebuild(hipStream_t s, DeviceState& ds, Arena& ar, cons
                                  const uint32_t* A, uint32_t na, const uint32_t* B, uint32_t nb,
                                  const uint8_t* flags) {
   const uint64_t N = (uint64_t)na + (MERGE ? nb : 0u);
-  GQF_LAUNCH(k_reset_counters, dim3(1), dim3(64), 0, s, ds.counters, (int)CTR_SCAN_TILE, (int)CTR_TOTAL);
+  GQF_LAUNCH(k_reset_counters, dim3(1), dim3(64), 0, s, ds.counters, (int)CTR_OP, (int)CTR_TOTAL);
   if (N == 0) {
     clear_table(s, qf, g);
     return GQF_OK;
@@ -1087,10 +933,10 @@ static gqf_status launch_rebuild(hipStream_t s, DeviceState& ds, Arena& ar, cons
     return GQF_ERR_INVALID;
   }
   if (MERGE)
-    GQF_LAUNCH(k_rb_splits, dim3((p.nsub + 1 + 255) / 256), dim3(256), 0, s, A, na, B, nb, p.nsub, splits);
+    GQF_LAUNCH(k_rb_splits<uint32_t>, dim3((p.nsub + 1 + 255) / 256), dim3(256), 0, s, A, na, B, nb, p.nsub, splits);
   GQF_LAUNCH_B((double)N * 4, (k_rb_agg<MODE, MERGE>), dim3(p.G), dim3(RB_THREADS), 0, s, A, na, B, nb, flags, splits,
                g.q, g.r, p.chunk, agg);
-  GQF_LAUNCH(k_rb_scan, dim3(1), dim3(1024), 0, s, agg, p.G, (uint64_t)g.S, ds.counters);
+  GQF_LAUNCH(k_rb_scan, dim3(1), dim3(1024), 0, s, agg, p.G, (uint64_t)g.S, ds.counters, (uint4*)nullptr);
   // sparse tables: clear once, then sub-tiles skip windows holding only zeros
   const int precleared = N * 8 < (uint64_t)g.S ? 1 : 0;
   if (precleared) GQF_LAUNCH(k_sqf_clear_tail, dim3(1024), dim3(256), 0, s, qf.table, g.bpe, g.bytes, ds.counters, 1);

@@ -165,3 +165,45 @@ def test_full_size_q26_r8_properties():
     total_runs = int(np.unpackbits(ends.view(np.uint8)).sum())
     total_occ = int(np.unpackbits(occ.view(np.uint8)).sum())
     assert total_runs == total_occ == len(np.unique(O.fingerprints(keys, q, r) >> np.uint64(r)))
+
+
+@pytest.mark.parametrize("q,r", [(14, 8), (14, 5), (15, 16)])
+def test_long_runs_across_windows(q, r):
+    """A key stored 5000 times: its run spans more than one 4096-slot decode
+    window and the following blocks' offsets pass 255 (stored mod 256).
+    Insert and delete decode such a table through run ranks, not offsets."""
+    say("rsqf long runs", q, r)
+    # a key whose quotient lies in the first quarter, so the run fits
+    cand = O.mt19937(5, 1000)
+    hot = cand[(O.fingerprints(cand, q, r) >> np.uint64(r)) < (1 << (q - 2))][0]
+    base = np.concatenate([O.mt19937(21, 6000), np.full(5000, hot, np.uint32)])
+    f = G.RSQF(q, r)
+    f.bulk_build(dev(base))
+    assert (f.dump() == canon(base, q, r)).all()
+    extra = np.concatenate([O.mt19937(22, 1500), np.full(300, hot, np.uint32)])
+    f.insertGPU(dev(extra))
+    allk = np.concatenate([base, extra])
+    assert (f.dump() == canon(allk, q, r)).all()
+    d = np.full(4000, hot, np.uint32)
+    f.deleteGPU(dev(d))
+    rest = np.concatenate([O.mt19937(21, 6000), np.full(1300, hot, np.uint32), O.mt19937(22, 1500)])
+    assert (f.dump() == canon(rest, q, r)).all()
+
+@pytest.mark.parametrize("q,r,alpha", [(25, 8, 0.85), (22, 13, 0.6)])
+def test_wide_insert_delete_equal_canonical(q, r, alpha):
+    """Fingerprints wider than 32 bits (q + r = 33, 35): insert batches and a
+    delete against the canonical layout."""
+    say("rsqf wide insert/delete", q, r, alpha)
+    n = O.key_count(alpha, q)
+    keys = O.mt19937(6, n)
+    start = O.key_count(alpha - 0.1, q)
+    f = G.RSQF(q, r)
+    f.bulk_build(dev(keys[:start]))
+    step = (n - start + 2) // 3
+    for i in range(start, n, step):
+        f.insertGPU(dev(keys[i:i + step]))
+    want = canon(keys, q, r)
+    assert (f.dump() == want).all()
+    d = keys[: n // 10]
+    f.deleteGPU(dev(d))
+    assert (f.dump() == canon(keys[n // 10:], q, r)).all()
