@@ -19,7 +19,15 @@ pair around every kernel) and the CPU baseline (reference CPU_version code,
 or the project's restatement when that is not built, on a bounded sample of
 the same workload on this host) are printed in the same JSON line.
 
+Serving leg (same JSON line, "serving"): eight resident RSQFs (r=8, q from
+24..27, fill 0.5) and a Poisson stream of 2000 requests (70/20/10
+lookup/insert/delete, sizes log-uniform 2^6..2^20) at each rate of
+--serve-qps, served with coalescing and again plainly from the same starting
+tables: Mops/s and p50/p99 latency per point, each mode's best Mops/s with
+p99 <= 10 ms, and whether end states and lookup results agree.
+
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--workload sqf|rsqf]
+                  [--serve-qps Q] [--serve-requests N]
 """
 import argparse
 import json
@@ -117,6 +125,115 @@ def run_rsqf_step(G, d, st):
         st.add("insert", len(b), f.insertGPU(b))
 
 
+SERVE_R = 8
+SERVE_BUDGET = 64 << 30
+
+
+def serve_setup(G, nreq, seed=1):
+    """Eight resident RSQFs (r=8, q drawn uniformly from 24..27, fill 0.5) and
+    a request stream: 70/20/10 lookup/insert/delete, sizes log-uniform
+    2^6..2^20, filter uniform.  Lookup and insert keys are uniform uint32;
+    deletes take keys the filter holds (each at most once), so they remove
+    entries.  Keys and results live in two device buffers."""
+    rng = np.random.default_rng(seed)
+    qs = [int(x) for x in rng.integers(24, 28, 8)]
+    init = [G.mt19937(100 + i, key_count(0.5, q)) for i, q in enumerate(qs)]
+    ops = rng.choice(3, size=nreq, p=[0.7, 0.2, 0.1])
+    sizes = (2.0 ** rng.uniform(6, 20, nreq)).astype(np.int64)
+    fil = rng.integers(0, 8, nreq)
+    gaps = rng.exponential(1.0, nreq)  # scaled by 1/QPS per run
+    unused = [rng.permutation(len(k)) for k in init]
+    taken = [0] * 8
+    parts, offs = [], []
+    off = 0
+    for i in range(nreq):
+        n = int(sizes[i])
+        if ops[i] == 2:
+            f = int(fil[i])
+            n = min(n, len(init[f]) - taken[f])
+            k = init[f][unused[f][taken[f]:taken[f] + n]]
+            taken[f] += n
+            sizes[i] = n
+        else:
+            k = rng.integers(0, 2 ** 32, n, dtype=np.uint32)
+        parts.append(k)
+        offs.append(off)
+        off += n
+    allk = np.concatenate(parts).astype(np.uint32)
+    keys = G.DeviceArray.from_host(allk)
+    out = G.DeviceArray.empty(max(off, 1))
+    return dict(qs=qs, init=[G.DeviceArray.from_host(k) for k in init], ops=ops, sizes=sizes, fil=fil,
+                gaps=gaps, offs=offs, keys=keys, out=out, total=off, nreq=nreq)
+
+
+def serve_requests(G, S, qps):
+    reqs = (G.Request * S["nreq"])()
+    t = 0.0
+    for i in range(S["nreq"]):
+        t += S["gaps"][i] * 1e6 / qps
+        r = reqs[i]
+        r.filter, r.op, r.n = int(S["fil"][i]), int(S["ops"][i]), int(S["sizes"][i])
+        r.d_keys = S["keys"].ptr + 4 * S["offs"][i]
+        r.d_out = S["out"].ptr + 4 * S["offs"][i]
+        r.arrive_us = t
+    return reqs
+
+
+def serve_filters(G, S):
+    fs = []
+    for q, k in zip(S["qs"], S["init"]):
+        f = G.RSQF(q, SERVE_R)
+        f.bulk_build(k)
+        fs.append(f)
+    G.synchronize()
+    return fs
+
+
+SERVE_SLO_MS = 10.0
+
+
+def serving_leg(G, qps_list, nreq, slo_ms=SERVE_SLO_MS):
+    """The same stream at each arrival rate, served with coalescing
+    (per-filter workers and streams) and plainly (one call per request, in
+    arrival order, one stream), each from the same starting tables.  Reports
+    every point and, per mode, the highest throughput whose p99 latency stays
+    within slo_ms.  At the highest rate the two end states (all eight tables)
+    and all lookup results are compared."""
+    S = serve_setup(G, nreq)
+    res = {"config": {"filters": "8 RSQF r=8, q from 24..27 (seed 1): " + ",".join(map(str, S["qs"])),
+                      "start_fill": 0.5, "requests": nreq, "mix": "70/20/10 lookup/insert/delete",
+                      "sizes": "log-uniform 2^6..2^20", "arrivals": "Poisson",
+                      "hbm_budget_bytes": SERVE_BUDGET, "keys_total": S["total"]},
+           "slo_p99_ms": slo_ms, "sweep": []}
+    top = max(qps_list)
+    dumps, outs = {}, {}
+    for qps in qps_list:
+        point = {"qps": qps, "offered_Mops_s": round(S["total"] * qps / nreq / 1e6, 1)}
+        for mode, coalesce in (("served", True), ("plain", False)):
+            fs = serve_filters(G, S)
+            reqs = serve_requests(G, S, qps)
+            st = G.serve(fs, reqs, coalesce=coalesce, hbm_budget=SERVE_BUDGET)
+            point[mode] = {"Mops_s": round(st.mops, 1), "p50_ms": round(st.p50_us / 1e3, 3),
+                           "p99_ms": round(st.p99_us / 1e3, 3), "mean_ms": round(st.mean_us / 1e3, 3),
+                           "calls": int(st.calls), "failed": sum(1 for r in reqs if r.status != G.OK)}
+            if qps == top:
+                dumps[mode] = [f.dump() for f in fs]
+                lk = S["out"].to_host()
+                outs[mode] = np.concatenate([lk[S["offs"][i]:S["offs"][i] + S["sizes"][i]]
+                                             for i in range(nreq) if S["ops"][i] == 0])
+            for f in fs:
+                f.destroy()
+        res["sweep"].append(point)
+    for mode in ("served", "plain"):
+        ok = [p for p in res["sweep"] if p[mode]["p99_ms"] <= slo_ms and p[mode]["failed"] == 0]
+        best = max(ok, key=lambda p: p[mode]["Mops_s"]) if ok else None
+        res[mode] = dict(best[mode], qps=best["qps"]) if best else None
+    res["checked_at_qps"] = top
+    res["end_state_equal"] = all((a == b).all() for a, b in zip(dumps["served"], dumps["plain"]))
+    res["lookups_equal"] = bool((outs["served"] == outs["plain"]).all())
+    return res
+
+
 def cpu_baseline(workload, q, r, keys, budget_s=20.0):
     """Reference CPU path on this host, single thread, bounded sample."""
     import oracle
@@ -178,6 +295,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", choices=["sqf", "rsqf"], default="sqf")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-qps", type=str, default="1000,2000,3000,4000,6000,8000",
+                    help="arrival rates of the serving leg, comma-separated ('' or 0: skip it)")
+    ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="first copy BYTES device-to-device three times (a plain copy kernel of known size, "
                          "to scale rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")
@@ -289,6 +409,11 @@ def main():
         "roofline": roof,
         "kernels": kernels,
     }
+    qps_list = [float(x) for x in args.serve_qps.split(",") if x.strip() and float(x) > 0]
+    if qps_list:
+        d = None  # free the step's buffers first
+        G.synchronize()
+        out["serving"] = serving_leg(G, qps_list, args.serve_requests)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.workload, q, r, keys)
     else:

@@ -48,6 +48,41 @@ class _RSQF(C.Structure):
     _fields_ = [("qbits", C.c_uint), ("numBlocks", C.c_uint), ("blocks", C.c_void_p), ("rbits", C.c_uint)]
 
 
+OP_LOOKUP, OP_INSERT, OP_DELETE = 0, 1, 2
+
+
+class Request(C.Structure):
+    """gqf_request (gqf.h): one served request."""
+    _fields_ = [("filter", C.c_int), ("op", C.c_int), ("n", C.c_int), ("d_keys", C.c_void_p),
+                ("d_out", C.c_void_p), ("arrive_us", C.c_double), ("start_us", C.c_double),
+                ("done_us", C.c_double), ("status", C.c_int), ("call", C.c_int)]
+
+
+class ServeConfig(C.Structure):
+    _fields_ = [("hbm_budget", C.c_size_t), ("coalesce", C.c_int), ("sorted_lookups", C.c_int),
+                ("max_batch", C.c_int)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [("calls", C.c_uint64), ("requests", C.c_uint64), ("rejected", C.c_uint64), ("keys", C.c_uint64),
+                ("span_us", C.c_double), ("p50_us", C.c_double), ("p99_us", C.c_double), ("mean_us", C.c_double),
+                ("mops", C.c_double), ("peak_bytes", C.c_size_t)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+GROW_FN = C.CFUNCTYPE(C.c_size_t, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_int)
+HELD_FN = C.CFUNCTYPE(C.c_size_t, C.c_void_p, C.c_int)
+TRIM_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_int)
+RUN_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.POINTER(C.POINTER(Request)), C.c_int)
+
+
+class ServeExecutor(C.Structure):
+    _fields_ = [("user", C.c_void_p), ("grow_bytes", GROW_FN), ("held_bytes", HELD_FN), ("trim", TRIM_FN),
+                ("run", RUN_FN)]
+
+
 _st = C.c_int
 _vp = C.c_void_p
 _f = C.POINTER(C.c_float)
@@ -65,6 +100,14 @@ _SIGS = {
     "gqf_synchronize": (_st, []),
     "gqf_stream_create": (_st, [C.POINTER(_vp)]),
     "gqf_stream_destroy": (_st, [_vp]),
+    "gqf_stream_workspace_bytes": (_st, [_vp, C.POINTER(C.c_size_t)]),
+    "gqf_stream_trim": (_st, [_vp]),
+    "gqf_workspace_alloc_bytes": (C.c_size_t, [C.c_size_t]),
+    "gqf_rsqf_workspace_bytes": (C.c_size_t, [_RSQF, C.c_int, C.c_int, C.c_int]),
+    "gqf_rsqf_serve": (_st, [C.POINTER(_RSQF), C.c_int, C.POINTER(Request), C.c_int, C.POINTER(ServeConfig),
+                             C.POINTER(ServeStats)]),
+    "gqf_serve_run": (_st, [C.c_int, C.c_size_t, C.POINTER(Request), C.c_int, C.POINTER(ServeConfig),
+                            C.POINTER(ServeExecutor), C.POINTER(ServeStats)]),
     "gqf_profile_enable": (_st, [C.c_int]),
     "gqf_profile_reset": (_st, []),
     "gqf_profile_count": (C.c_int, []),
@@ -402,3 +445,15 @@ class RSQF:
             self.destroy()
         except Exception:
             pass
+
+
+def serve(filters, requests, coalesce=True, hbm_budget=64 << 30, sorted_lookups=False, max_batch=0):
+    """Run a request stream over resident RSQFs (gqf_rsqf_serve).
+
+    ``requests`` is a ctypes array of :class:`Request` (filled in place with
+    start/done times, status and call ids).  Returns the :class:`ServeStats`."""
+    arr = (_RSQF * len(filters))(*[f.h for f in filters])
+    cfg = ServeConfig(int(hbm_budget), int(bool(coalesce)), int(bool(sorted_lookups)), int(max_batch))
+    st = ServeStats()
+    check(lib().gqf_rsqf_serve(arr, len(filters), requests, len(requests), C.byref(cfg), C.byref(st)))
+    return st

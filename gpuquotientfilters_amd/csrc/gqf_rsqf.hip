@@ -976,6 +976,18 @@ gqf_status gqf_rsqf_lookup(gqf_rsqf cqf, int sorted, int n, const uint32_t* d_ke
   return cs.end(ms, false);
 }
 
+size_t gqf_rsqf_workspace_bytes(gqf_rsqf cqf, int op, int n, int sorted) {
+  if (rsqf_check(cqf) != GQF_OK || n < 0) return 0;
+  const RsqfGeom g = rgeom(cqf);
+  const bool wide = g.bits > 32;
+  const uint32_t un = (uint32_t)n;
+  if (op == GQF_OP_LOOKUP)
+    return sorted && n > 0 ? sort_workspace_bytes(un, wide ? 8 : 4, true) : 0;
+  if (op == GQF_OP_INSERT) return wide ? rsqf_insert_ws<uint64_t>(g, un) : rsqf_insert_ws<uint32_t>(g, un);
+  if (op == GQF_OP_DELETE) return wide ? rsqf_delete_ws<uint64_t>(g, un) : rsqf_delete_ws<uint32_t>(g, un);
+  return 0;
+}
+
 // Device layout <-> reference packed blocks, converted on the host.
 gqf_status gqf_rsqf_dump(gqf_rsqf cqf, void* host_ref_layout) {
   GQF_TRY(rsqf_check(cqf));

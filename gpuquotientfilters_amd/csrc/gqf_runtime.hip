@@ -61,9 +61,52 @@ gqf_status device_state(hipStream_t stream, DeviceState** out) {
   return GQF_OK;
 }
 
+static DeviceState* find_state(hipStream_t stream, int dev) {
+  std::lock_guard<std::mutex> lk(g_ctx_mu);
+  if (g_ctx)
+    for (const CtxEntry& e : *g_ctx)
+      if (e.device == dev && e.stream == stream) return e.ds;
+  return nullptr;
+}
+
+// Frees the context of `stream` on the current device (its stream is about
+// to be destroyed); waits for the stream first.
+static gqf_status release_state(hipStream_t stream) {
+  int dev = 0;
+  GQF_HIP(hipGetDevice(&dev));
+  DeviceState* ds = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_ctx_mu);
+    if (!g_ctx) return GQF_OK;
+    for (size_t i = 0; i < g_ctx->size(); ++i)
+      if ((*g_ctx)[i].device == dev && (*g_ctx)[i].stream == stream) {
+        ds = (*g_ctx)[i].ds;
+        g_ctx->erase(g_ctx->begin() + (long)i);
+        break;
+      }
+  }
+  if (!ds) return GQF_OK;
+  {
+    std::lock_guard<std::mutex> lk(ds->mu);
+    GQF_HIP(hipStreamSynchronize(stream));
+    if (ds->ws) GQF_HIP(hipFree(ds->ws));
+    GQF_HIP(hipFree(ds->counters));
+    GQF_HIP(hipHostFree(ds->h_pinned));
+    GQF_HIP(hipEventDestroy(ds->ev0));
+    GQF_HIP(hipEventDestroy(ds->ev1));
+    for (size_t i = 0; i < ds->pool_size; ++i) (void)hipEventDestroy(ds->pool[i]);
+    free(ds->pool);
+    free(ds->pending);
+  }
+  delete ds;
+  return GQF_OK;
+}
+
+size_t ws_alloc_bytes(size_t bytes) { return bytes + bytes / 4 + (1u << 20); }
+
 gqf_status ws_reserve(DeviceState& ds, size_t bytes) {
   if (bytes <= ds.ws_cap) return GQF_OK;
-  size_t cap = bytes + bytes / 4 + (1u << 20);
+  size_t cap = ws_alloc_bytes(bytes);
   GQF_HIP(hipStreamSynchronize(ds.stream));
   if (ds.ws) GQF_HIP(hipFree(ds.ws));
   ds.ws = nullptr;
@@ -310,9 +353,38 @@ gqf_status gqf_stream_create(gqf_stream* out) {
 gqf_status gqf_stream_destroy(gqf_stream s) {
   if (s) {
     GQF_HIP(hipStreamSynchronize(reinterpret_cast<hipStream_t>(s)));
+    GQF_TRY(release_state(reinterpret_cast<hipStream_t>(s)));
     GQF_HIP(hipStreamDestroy(reinterpret_cast<hipStream_t>(s)));
   }
   return GQF_OK;
 }
+
+gqf_status gqf_stream_workspace_bytes(gqf_stream s, size_t* bytes) {
+  if (!bytes) return GQF_ERR_INVALID;
+  int dev = 0;
+  GQF_HIP(hipGetDevice(&dev));
+  DeviceState* ds = find_state(reinterpret_cast<hipStream_t>(s), dev);
+  *bytes = 0;
+  if (ds) {
+    std::lock_guard<std::mutex> lk(ds->mu);
+    *bytes = ds->ws_cap;
+  }
+  return GQF_OK;
+}
+
+gqf_status gqf_stream_trim(gqf_stream s) {
+  int dev = 0;
+  GQF_HIP(hipGetDevice(&dev));
+  DeviceState* ds = find_state(reinterpret_cast<hipStream_t>(s), dev);
+  if (!ds) return GQF_OK;
+  std::lock_guard<std::mutex> lk(ds->mu);
+  GQF_HIP(hipStreamSynchronize(ds->stream));
+  if (ds->ws) GQF_HIP(hipFree(ds->ws));
+  ds->ws = nullptr;
+  ds->ws_cap = 0;
+  return GQF_OK;
+}
+
+size_t gqf_workspace_alloc_bytes(size_t bytes) { return bytes ? ws_alloc_bytes(bytes) : 0; }
 
 }  // extern "C"

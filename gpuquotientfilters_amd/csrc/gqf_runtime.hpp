@@ -72,6 +72,8 @@ struct DeviceState {
 // The context of the calling thread's current device and `stream`, created
 // on first use (contexts live until the process ends).
 gqf_status device_state(hipStream_t stream, DeviceState** out);
+// Bytes a workspace grows to when a call needs `bytes` more than it holds.
+size_t ws_alloc_bytes(size_t bytes);
 gqf_status ws_reserve(DeviceState& ds, size_t bytes);
 
 // Bump allocator over the workspace of one call.

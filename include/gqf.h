@@ -163,6 +163,11 @@ gqf_status gqf_rsqf_lookup(gqf_rsqf cqf, int sorted, int n, const uint32_t* d_ke
 gqf_status gqf_rsqf_delete(gqf_rsqf cqf, int n, const uint32_t* d_keys, gqf_stream stream,
                            float* ms);
 
+/* Device workspace (bytes) a call of `op` (gqf_op) over n keys reserves on
+ * its stream's context; lookups only when sorted.  No reference counterpart
+ * (the reference allocates per call). */
+size_t gqf_rsqf_workspace_bytes(gqf_rsqf cqf, int op, int n, int sorted);
+
 /* Copy the table to host memory in the reference packed layout
  * (gqf_rsqf_ref_bytes bytes), and back. */
 gqf_status gqf_rsqf_dump(gqf_rsqf cqf, void* host_ref_layout);
@@ -182,7 +187,91 @@ gqf_status gqf_synchronize(void);
  * different streams (from different host threads) run concurrently, each with
  * its own workspace; calls on one stream are serialised. */
 gqf_status gqf_stream_create(gqf_stream* out);
+/* Waits for the stream, releases its workspace and destroys it. */
 gqf_status gqf_stream_destroy(gqf_stream s);
+/* Workspace currently held for calls on the stream (grow-only until trimmed),
+ * and its release. */
+gqf_status gqf_stream_workspace_bytes(gqf_stream s, size_t* bytes);
+gqf_status gqf_stream_trim(gqf_stream s);
+/* What a workspace grows to when a call needs `bytes` more than it holds. */
+size_t gqf_workspace_alloc_bytes(size_t bytes);
+
+/* ================================================================ serving
+ * A request loop over resident RSQFs (no reference counterpart; the
+ * reference has one blocking call per batch).  Requests carry device key
+ * buffers and a release time; the loop admits them against a device-memory
+ * budget and executes them:
+ *   coalesce != 0  one worker and stream per filter; a worker takes the
+ *                  longest run of same-kind requests at the head of its
+ *                  filter's queue (at most max_batch keys) and executes it as
+ *                  one call: lookups gathered and their results scattered
+ *                  back, inserts / deletes concatenated (the canonical layout
+ *                  of M + A + B, and M - A - B = M - (A + B) as multisets).
+ *                  Per filter, calls follow arrival order, so every result
+ *                  and the end state equal executing the requests one by one;
+ *                  different filters run concurrently on the device.
+ *   coalesce == 0  plain execution: one call per request, in arrival order,
+ *                  on one stream.
+ * Admission: a call starts when the filter tables, the workspaces the
+ * workers hold and the call's own need (workspace growth, gather buffers) fit
+ * hbm_budget; otherwise it waits for running calls to finish, releasing the
+ * workspaces of idle workers.  A request that cannot fit on its own is
+ * rejected (status GQF_ERR_NOMEM) and the loop goes on. */
+enum gqf_op { GQF_OP_LOOKUP = 0, GQF_OP_INSERT = 1, GQF_OP_DELETE = 2 };
+
+typedef struct gqf_request {
+  int filter;              /* index into the filter array */
+  int op;                  /* gqf_op */
+  int n;                   /* number of keys */
+  const uint32_t* d_keys;  /* device keys */
+  int32_t* d_out;          /* lookups: slot or -1 per key (device); else unused */
+  double arrive_us;        /* release time in microseconds after the loop starts; non-decreasing */
+  /* written by the loop */
+  double start_us;         /* its call started (admitted) */
+  double done_us;          /* its call completed */
+  int status;              /* gqf_status of that call (GQF_ERR_NOMEM: rejected) */
+  int call;                /* id of that call; coalesced requests share it */
+} gqf_request;
+
+typedef struct gqf_serve_config {
+  size_t hbm_budget;       /* bytes: tables + held workspaces + gather buffers */
+  int coalesce;            /* see above */
+  int sorted_lookups;      /* lookups through the sorted path */
+  int max_batch;           /* keys per coalesced call; 0 = 2^24 */
+} gqf_serve_config;
+
+typedef struct gqf_serve_stats {
+  uint64_t calls;          /* device calls made */
+  uint64_t requests;       /* requests served (not rejected) */
+  uint64_t rejected;
+  uint64_t keys;           /* keys of the served requests */
+  double span_us;          /* first release to last completion */
+  double p50_us, p99_us, mean_us;  /* latency: done_us - arrive_us over served requests */
+  double mops;             /* keys / span */
+  size_t peak_bytes;       /* highest accounted device bytes */
+} gqf_serve_stats;
+
+gqf_status gqf_rsqf_serve(gqf_rsqf* filters, int nfilters, gqf_request* reqs, int nreqs,
+                          const gqf_serve_config* cfg, gqf_serve_stats* stats);
+
+/* The same scheduler over caller-supplied operations (e.g. another filter
+ * type, or a host model of the device for testing the scheduling itself). */
+typedef struct gqf_serve_executor {
+  void* user;
+  /* device bytes worker `worker` must add to what it holds to run a call of
+   * `op` on `filter` over n keys (coalesced: the call gathers its keys) */
+  size_t (*grow_bytes)(void* user, int worker, int filter, int op, uint64_t n, int coalesced);
+  /* bytes the worker holds now; release them (called only while it is idle) */
+  size_t (*held_bytes)(void* user, int worker);
+  void (*trim)(void* user, int worker);
+  /* execute one call over the batch (same filter and op, arrival order);
+   * returns a gqf_status */
+  int (*run)(void* user, int worker, gqf_request** batch, int nbatch);
+} gqf_serve_executor;
+
+gqf_status gqf_serve_run(int nfilters, size_t resident_bytes, gqf_request* reqs, int nreqs,
+                         const gqf_serve_config* cfg, const gqf_serve_executor* ex,
+                         gqf_serve_stats* stats);
 
 /* ============================================================ kernel timing
  * Per-kernel device time measured with HIP events on the launch stream (two
