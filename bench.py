@@ -19,6 +19,11 @@ pair around every kernel) and the CPU baseline (reference CPU_version code,
 or the project's restatement when that is not built, on a bounded sample of
 the same workload on this host) are printed in the same JSON line.
 
+Batched leg (same JSON line, "batched"): 512 SQFs (r=5, q uniform 12..22,
+fill 0.3..0.9, Zipf(1.1) keys over 2^28, Dup) built by one graph launch and
+looked up (2^8..2^18 keys each) by another, against the per-filter calls;
+tables and lookup results compared.
+
 Serving leg (same JSON line, "serving"): eight resident RSQFs (r=8, q from
 24..27, fill 0.5) and a Poisson stream of 2000 requests (70/20/10
 lookup/insert/delete, sizes log-uniform 2^6..2^20) at each rate of
@@ -234,6 +239,106 @@ def serving_leg(G, qps_list, nreq, slo_ms=SERVE_SLO_MS):
     return res
 
 
+def zipf_keys(rng, n, s=1.1, universe=1 << 28):
+    """Zipf(s) ranks over [1, universe] (rejection of numpy's unbounded
+    sampler), as uint32 keys rank - 1: duplicates are frequent."""
+    out = np.empty(0, np.uint64)
+    while len(out) < n:
+        z = rng.zipf(s, 2 * (n - len(out)) + 16).astype(np.uint64)
+        out = np.concatenate([out, z[z <= universe]])
+    return (out[:n] - 1).astype(np.uint32)
+
+
+def batch_leg(G, F=512, reps=3, seed=7):
+    """Batched launches: F SQFs (r=5, q uniform over 12..22 -- log-uniform
+    table sizes -- fill uniform in 0.3..0.9, seed 7) built from Zipf(1.1) keys
+    over 2^28 in Dup mode, then a lookup batch per filter of log-uniform
+    2^8..2^18 keys (half stored keys).  One graph launch builds all filters,
+    one runs all lookups; compared with the per-filter calls
+    (gqf_sqf_bulk_build + unsorted gqf_sqf_lookup per filter, one stream).
+    Tables and lookup results of the two are compared bit for bit."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    r = 5
+    rng = np.random.default_rng(seed)
+    qs = [int(x) for x in rng.integers(12, 23, F)]
+    fills = rng.uniform(0.3, 0.9, F)
+    psz = (2.0 ** rng.uniform(8, 18, F)).astype(np.int64)
+    gens = [np.random.default_rng(x) for x in np.random.SeedSequence(seed).spawn(F)]
+
+    def gen(i):
+        k = zipf_keys(gens[i], key_count(float(fills[i]), qs[i]))
+        m = int(psz[i])
+        p = np.concatenate([gens[i].choice(k, m // 2), gens[i].integers(0, 2 ** 32, m - m // 2, dtype=np.uint32)])
+        return k, p.astype(np.uint32)
+
+    with ThreadPoolExecutor(16) as ex:
+        parts = list(ex.map(gen, range(F)))
+    keys = [k for k, _ in parts]
+    probe = [p for _, p in parts]
+    koff = np.concatenate([[0], np.cumsum([len(k) for k in keys])]).astype(np.uint64)
+    poff = np.concatenate([[0], np.cumsum([len(p) for p in probe])]).astype(np.uint64)
+    N, P = int(koff[-1]), int(poff[-1])
+    dk = G.DeviceArray.from_host(np.concatenate(keys))
+    dp = G.DeviceArray.from_host(np.concatenate(probe))
+    out_b = G.DeviceArray.empty(P)
+    out_p = G.DeviceArray.empty(P)
+    fb = [G.SQF(q, r) for q in qs]
+    fp = [G.SQF(q, r) for q in qs]
+    t0 = time.perf_counter()
+    batch = G.SQFBatch(fb, dk, koff, dp, poff, out_b, no_dup=False)
+    create_s = time.perf_counter() - t0
+    info = batch.info()
+    # batched: one graph launch per pipeline, wall time between synchronisations
+    tb = tl = 1e30
+    for _ in range(reps):
+        G.synchronize()
+        t0 = time.perf_counter()
+        _, st = batch.build(use_graph=True)
+        tb = min(tb, time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        batch.lookup(use_graph=True)
+        tl = min(tl, time.perf_counter() - t0)
+    # plain: the per-filter calls (a filter whose duplicate-heavy run spills
+    # past its last slot reports a capacity error and keeps its table, in
+    # both forms)
+    pb = pl = 1e30
+    for _ in range(reps):
+        G.synchronize()
+        pst = []
+        t0 = time.perf_counter()
+        for i, f in enumerate(fp):
+            pst.append(G.lib().gqf_sqf_bulk_build(f.h, 0, int(koff[i + 1] - koff[i]), dk.ptr + 4 * int(koff[i]), 0,
+                                                  None, None))
+        pb = min(pb, time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        for i, f in enumerate(fp):
+            n = int(poff[i + 1] - poff[i])
+            f.lookup(dp.ptr + 4 * int(poff[i]), out_p.ptr + 4 * int(poff[i]), n=n, sorted=False)
+        pl = min(pl, time.perf_counter() - t0)
+    tables_equal = all((a.table_bytes() == b.table_bytes()).all() for a, b in zip(fb, fp))
+    lookups_equal = bool((out_b.to_host() == out_p.to_host()).all())
+    res = {
+        "config": {"filters": F, "r": r, "q": "uniform 12..22 (seed 7)", "fill": "uniform 0.3..0.9",
+                   "keys": "Zipf(1.1) over 2^28, Dup", "build_keys": N,
+                   "lookups": "log-uniform 2^8..2^18 per filter, half stored keys", "lookup_keys": P},
+        "batched": {"build_ms": round(tb * 1e3, 3), "lookup_ms": round(tl * 1e3, 3),
+                    "build_Mops_s": round(N / tb / 1e6, 1), "lookup_Mops_s": round(P / tl / 1e6, 1),
+                    "Mops_s": round((N + P) / (tb + tl) / 1e6, 1)},
+        "plain": {"build_ms": round(pb * 1e3, 3), "lookup_ms": round(pl * 1e3, 3),
+                  "build_Mops_s": round(N / pb / 1e6, 1), "lookup_Mops_s": round(P / pl / 1e6, 1),
+                  "Mops_s": round((N + P) / (pb + pl) / 1e6, 1)},
+        "speedup": round((pb + pl) / (tb + tl), 2),
+        "tables_equal": tables_equal, "lookups_equal": lookups_equal, "statuses_equal": st == pst,
+        "capacity_errors": sum(1 for x in st if x != G.OK),
+        "plan": info, "create_s": round(create_s, 3),
+    }
+    batch.destroy()
+    for f in fb + fp:
+        f.destroy()
+    return res
+
+
 def cpu_baseline(workload, q, r, keys, budget_s=20.0):
     """Reference CPU path on this host, single thread, bounded sample."""
     import oracle
@@ -298,6 +403,7 @@ def main():
     ap.add_argument("--serve-qps", type=str, default="1000,2000,3000,4000,6000,8000",
                     help="arrival rates of the serving leg, comma-separated ('' or 0: skip it)")
     ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-launch leg (512 SQFs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="first copy BYTES device-to-device three times (a plain copy kernel of known size, "
                          "to scale rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")
@@ -409,6 +515,10 @@ def main():
         "roofline": roof,
         "kernels": kernels,
     }
+    if not args.no_batch:
+        d = None
+        G.synchronize()
+        out["batched"] = batch_leg(G)
     qps_list = [float(x) for x in args.serve_qps.split(",") if x.strip() and float(x) > 0]
     if qps_list:
         d = None  # free the step's buffers first

@@ -83,6 +83,21 @@ class ServeExecutor(C.Structure):
                 ("run", RUN_FN)]
 
 
+class BatchInfo(C.Structure):
+    """gqf_batch_info (gqf.h): the plan a batch chose."""
+    _fields_ = [("filters", C.c_int), ("build_keys", C.c_uint32), ("lookup_keys", C.c_uint32),
+                ("sort_passes", C.c_int), ("sort_chunks", C.c_int), ("sort_subtiles_per_chunk", C.c_int),
+                ("rebuild_subtiles", C.c_int), ("lookup_block", C.c_int), ("lookup_keys_per_thread", C.c_int),
+                ("build_graph_nodes", C.c_int), ("lookup_graph_nodes", C.c_int),
+                ("tune_build_ms", C.c_float * 4), ("tune_lookup_ms", C.c_float * 15)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_[:11]}
+        d["tune_build_ms"] = [round(x, 4) for x in self.tune_build_ms if x > 0]
+        d["tune_lookup_ms"] = [round(x, 4) for x in self.tune_lookup_ms if x > 0]
+        return d
+
+
 _st = C.c_int
 _vp = C.c_void_p
 _f = C.POINTER(C.c_float)
@@ -106,6 +121,12 @@ _SIGS = {
     "gqf_rsqf_workspace_bytes": (C.c_size_t, [_RSQF, C.c_int, C.c_int, C.c_int]),
     "gqf_rsqf_serve": (_st, [C.POINTER(_RSQF), C.c_int, C.POINTER(Request), C.c_int, C.POINTER(ServeConfig),
                              C.POINTER(ServeStats)]),
+    "gqf_sqf_batch_create": (_st, [C.POINTER(_vp), C.POINTER(_SQF), C.c_int, C.c_int, _vp, C.POINTER(C.c_uint64),
+                                   _vp, C.POINTER(C.c_uint64), _vp, _vp]),
+    "gqf_sqf_batch_build": (_st, [_vp, C.c_int, C.POINTER(C.c_int), _f]),
+    "gqf_sqf_batch_lookup": (_st, [_vp, C.c_int, _f]),
+    "gqf_sqf_batch_info": (_st, [_vp, C.POINTER(BatchInfo)]),
+    "gqf_sqf_batch_destroy": (_st, [_vp]),
     "gqf_serve_run": (_st, [C.c_int, C.c_size_t, C.POINTER(Request), C.c_int, C.POINTER(ServeConfig),
                             C.POINTER(ServeExecutor), C.POINTER(ServeStats)]),
     "gqf_profile_enable": (_st, [C.c_int]),
@@ -457,3 +478,53 @@ def serve(filters, requests, coalesce=True, hbm_budget=64 << 30, sorted_lookups=
     st = ServeStats()
     check(lib().gqf_rsqf_serve(arr, len(filters), requests, len(requests), C.byref(cfg), C.byref(st)))
     return st
+
+
+class SQFBatch:
+    """Many SQFs per launch (gqf_sqf_batch_*).
+
+    ``filters``: initialised :class:`SQF` objects of one r.  ``keys`` /
+    ``key_offsets``: every filter's build keys back to back in one
+    :class:`DeviceArray` (offsets: F + 1 host integers); ``probe`` /
+    ``probe_offsets`` / ``out`` likewise for lookups.  Creating the batch
+    autotunes and captures the pipelines (and leaves every table built)."""
+
+    def __init__(self, filters, keys=None, key_offsets=None, probe=None, probe_offsets=None, out=None,
+                 no_dup=False, stream=None):
+        self.filters = list(filters)
+        F = len(self.filters)
+        self._arr = (_SQF * F)(*[f.h for f in self.filters])
+        self._koff = (C.c_uint64 * (F + 1))(*[int(x) for x in key_offsets]) if key_offsets is not None else None
+        self._poff = (C.c_uint64 * (F + 1))(*[int(x) for x in probe_offsets]) if probe_offsets is not None else None
+        self._hold = (keys, probe, out)
+        self.h = _vp()
+        check(lib().gqf_sqf_batch_create(C.byref(self.h), self._arr, F, int(bool(no_dup)), _ptr(keys), self._koff,
+                                         _ptr(probe), self._poff, _ptr(out), stream))
+
+    def build(self, use_graph=True):
+        """Returns (ms, per-filter statuses)."""
+        st = (C.c_int * len(self.filters))()
+        ms = C.c_float(0)
+        check(lib().gqf_sqf_batch_build(self.h, int(bool(use_graph)), st, C.byref(ms)))
+        return ms.value, list(st)
+
+    def lookup(self, use_graph=True):
+        ms = C.c_float(0)
+        check(lib().gqf_sqf_batch_lookup(self.h, int(bool(use_graph)), C.byref(ms)))
+        return ms.value
+
+    def info(self):
+        i = BatchInfo()
+        check(lib().gqf_sqf_batch_info(self.h, C.byref(i)))
+        return i.as_dict()
+
+    def destroy(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.gqf_sqf_batch_destroy(self.h)
+            self.h = _vp()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass

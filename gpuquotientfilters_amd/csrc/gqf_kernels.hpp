@@ -143,7 +143,7 @@ __global__ void k_fill(T* __restrict__ p, uint64_t n, T v) {
     p[i] = v;
 }
 
-static __global__ void k_reset_counters(uint32_t* ctr, int from, int to) {
+[[maybe_unused]] static __global__ void k_reset_counters(uint32_t* ctr, int from, int to) {
   int i = from + threadIdx.x;
   if (i < to) ctr[i] = 0;
 }
@@ -252,7 +252,7 @@ __global__ __launch_bounds__(256) void k_rs_upsweep(const void* __restrict__ in,
 
 // Row d (one workgroup per digit): exclusive scan of the G chunk counts in
 // place; the row total goes to totals[d].
-static __global__ __launch_bounds__(1024) void k_rs_rowscan(uint32_t* __restrict__ hist, uint32_t G,
+[[maybe_unused]] static __global__ __launch_bounds__(1024) void k_rs_rowscan(uint32_t* __restrict__ hist, uint32_t G,
                                                             uint32_t* __restrict__ totals) {
   __shared__ uint32_t s_w[16];
   __shared__ uint32_t s_carry;
@@ -282,7 +282,7 @@ static __global__ __launch_bounds__(1024) void k_rs_rowscan(uint32_t* __restrict
 }
 
 // totals[256] -> exclusive digit bases, in place.
-static __global__ __launch_bounds__(256) void k_rs_digitscan(uint32_t* __restrict__ totals) {
+[[maybe_unused]] static __global__ __launch_bounds__(256) void k_rs_digitscan(uint32_t* __restrict__ totals) {
   __shared__ uint32_t tmp[4];
   const uint32_t x = totals[threadIdx.x];
   totals[threadIdx.x] = block256_excl_scan(x, tmp, nullptr);

@@ -121,7 +121,7 @@ __device__ __forceinline__ RS rs_block(RS a, RS* ex_out, uint4* s_wave) {
 // prefixes.  The total (also stored in *total when given) fixes the kept count, the first
 // slot after the last entry, and a capacity error -- all before any table
 // byte is written.
-static __global__ __launch_bounds__(1024) void k_rb_scan(uint4* __restrict__ agg, uint32_t G, uint64_t S,
+[[maybe_unused]] static __global__ __launch_bounds__(1024) void k_rb_scan(uint4* __restrict__ agg, uint32_t G, uint64_t S,
                                                          uint32_t* __restrict__ ctr, uint4* __restrict__ total) {
   constexpr uint32_t SEG = 8, PIECE = 1024 * SEG;
   __shared__ uint4 s_wave[16];  // wave aggregates, then their exclusive prefixes

@@ -11,833 +11,11 @@
 //   [decode table] -> [hash + radix sort keys] -> [merge / mark] ->
 //   layout scan (locations) -> table writer (every slot written once).
 // Lookups are the reference's run walk (quotientFilter.cu:172-196, 320-357).
-#include "gqf_rebuild.hpp"
+#include "gqf_sqf_dev.hpp"
+
+#include <cstring>
 
 namespace gqf {
-
-constexpr uint32_t SQF_NOT_FOUND = 0xFFFFFFFFu;
-
-// numElements (quotientFilter.cu:174): (1 << q) * 1.1 in double, truncated.
-static inline uint32_t sqf_slots(unsigned q) { return (uint32_t)((double)(1u << q) * 1.1); }
-// calcNumSlotsGPU (quotientFilter.cu:37-42): bytes.
-static inline size_t sqf_bytes(unsigned q, unsigned r) {
-  size_t tableBits = (size_t)(1u << q) * (r + 3);
-  return (size_t)((double)(tableBits / 8) * 1.1);
-}
-
-static gqf_status sqf_check(const gqf_sqf& qf) {
-  if (!qf.table) {
-    set_error("SQF table is NULL (call gqf_sqf_init)");
-    return GQF_ERR_INVALID;
-  }
-  if (qf.qbits < 1 || qf.rbits < 1 || (qf.rbits + 3) % 8 != 0 || qf.qbits + qf.rbits > 32 || qf.qbits > 28 ||
-      qf.bytesPerElement != (qf.rbits + 3) / 8) {
-    set_error("unsupported SQF geometry q=%u r=%u bytesPerElement=%u (need (r+3)%%8==0, q+r<=32, q<=28)", qf.qbits,
-              qf.rbits, qf.bytesPerElement);
-    return GQF_ERR_INVALID;
-  }
-  return GQF_OK;
-}
-
-// ---------------------------------------------------------------- slot access
-template <int BPE>
-__host__ __device__ __forceinline__ uint32_t sqf_get(const uint8_t* __restrict__ t, uint32_t S, uint32_t i) {
-  if (i >= S) return 0u;  // past the last slot reads as empty
-  if (BPE == 1) return t[i];
-  const uint8_t* p = t + (size_t)i * BPE;
-  uint32_t e = p[0];
-#pragma unroll
-  for (int k = 1; k < BPE; ++k) e = (e << 8) | p[k];
-  return e;
-}
-
-// findRunStartGPU (quotientFilter.cu:172-196) + run scan of hashAndLookUp
-// (quotientFilter.cu:333-356): slot of the first matching remainder.
-template <int BPE>
-__host__ __device__ __forceinline__ uint32_t sqf_lookup_one(const uint8_t* __restrict__ t, uint32_t S, uint32_t fq, uint32_t fr) {
-  if (!(sqf_get<BPE>(t, S, fq) & 4u)) return SQF_NOT_FOUND;
-  uint32_t b = fq;
-  while (b > 0 && (sqf_get<BPE>(t, S, b) & 1u)) b--;
-  uint32_t s = b;
-  while (b != fq) {
-    do {
-      s++;
-    } while ((sqf_get<BPE>(t, S, s) & 2u) && s < S);
-    do {
-      b++;
-    } while (!(sqf_get<BPE>(t, S, b) & 4u) && b < S);
-  }
-  uint32_t e = sqf_get<BPE>(t, S, s);
-  do {
-    const uint32_t rem = e >> 3;
-    if (rem == fr) return s;
-    if (rem > fr) return SQF_NOT_FOUND;
-    s++;
-    e = sqf_get<BPE>(t, S, s);
-  } while (e & 2u);
-  return SQF_NOT_FOUND;
-}
-
-// The same walk over 64-slot windows (BPE 1 and 2): a window is loaded with
-// 16-byte loads (64 or 128 table bytes, 64-slot aligned) and turned into
-// occupied / continuation / shifted bit masks, so that the cluster start, the
-// k-th run start and the first remainder >= fr are found with popcount and
-// count-trailing-zeros; the walk moves one window back or forward only when
-// it crosses a window edge.  Slots at or past S read as empty, as above.
-template <int BPE>
-struct SqfWin {
-  uint32_t raw[16 * BPE];
-  uint64_t occ, cont, shf;
-};
-
-__host__ __device__ __forceinline__ uint32_t byte_bits4(uint32_t x) {
-  // bit 0 of each byte of x (other bits zero) -> bits 0..3
-  return (x * 0x01020408u) >> 24;
-}
-
-template <int BPE>
-__host__ __device__ __forceinline__ void sqf_win_load(const uint8_t* __restrict__ t, uint32_t S, uint32_t w, SqfWin<BPE>& W) {
-  const uint64_t s0 = (uint64_t)w * 64;
-  if (s0 + 64 <= S) {
-    const uint4* p = reinterpret_cast<const uint4*>(t + s0 * BPE);
-#pragma unroll
-    for (int k = 0; k < 4 * BPE; ++k) {
-      const uint4 x = p[k];
-      W.raw[4 * k] = x.x;
-      W.raw[4 * k + 1] = x.y;
-      W.raw[4 * k + 2] = x.z;
-      W.raw[4 * k + 3] = x.w;
-    }
-  } else {
-    const uint64_t end = (uint64_t)S * BPE;
-#pragma unroll
-    for (int k = 0; k < 16 * BPE; ++k) {
-      uint32_t v = 0;
-#pragma unroll
-      for (int b = 0; b < 4; ++b) {
-        const uint64_t at = s0 * BPE + 4 * k + b;
-        if (at < end) v |= (uint32_t)t[at] << (8 * b);
-      }
-      W.raw[k] = v;
-    }
-  }
-  uint64_t occ = 0, cont = 0, shf = 0;
-  if (BPE == 1) {
-#pragma unroll
-    for (int k = 0; k < 16; ++k) {
-      const uint32_t x = W.raw[k];
-      occ |= (uint64_t)byte_bits4((x >> 2) & 0x01010101u) << (4 * k);
-      cont |= (uint64_t)byte_bits4((x >> 1) & 0x01010101u) << (4 * k);
-      shf |= (uint64_t)byte_bits4(x & 0x01010101u) << (4 * k);
-    }
-  } else {
-    // slot j = bytes 2j (high) and 2j+1 (low, holding the flags)
-#pragma unroll
-    for (int k = 0; k < 32; ++k) {
-      const uint32_t x = W.raw[k];
-      const uint32_t f = ((x >> 8) & 7u) | ((x >> 21) & 0x38u);  // flags of slot 2k in bits 0..2, 2k+1 in 3..5
-      occ |= (uint64_t)(((f >> 2) & 1u) | ((f >> 4) & 2u)) << (2 * k);
-      cont |= (uint64_t)(((f >> 1) & 1u) | ((f >> 3) & 2u)) << (2 * k);
-      shf |= (uint64_t)((f & 1u) | ((f >> 2) & 2u)) << (2 * k);
-    }
-  }
-  W.occ = occ;
-  W.cont = cont;
-  W.shf = shf;
-}
-
-// Bit j set iff remainder(slot j of the window) >= x  (x <= 2^r).
-template <int BPE>
-__host__ __device__ __forceinline__ uint64_t sqf_win_ge(const SqfWin<BPE>& W, uint32_t x) {
-  uint64_t m = 0;
-  if (BPE == 1) {
-    // per byte: (rem | 0x80) - x keeps bit 7 iff rem >= x (rem, x < 128: no borrow between bytes)
-    const uint32_t xr = x * 0x01010101u;
-#pragma unroll
-    for (int k = 0; k < 16; ++k) {
-      const uint32_t a = (W.raw[k] >> 3) & 0x1F1F1F1Fu;
-      const uint32_t g = ((a | 0x80808080u) - xr) & 0x80808080u;
-      m |= (uint64_t)byte_bits4(g >> 7) << (4 * k);
-    }
-  } else {
-#pragma unroll
-    for (int k = 0; k < 32; ++k) {
-      const uint32_t x0 = W.raw[k];
-      const uint32_t r0 = (((x0 & 0xFFu) << 8) | ((x0 >> 8) & 0xFFu)) >> 3;
-      const uint32_t r1 = ((((x0 >> 16) & 0xFFu) << 8) | (x0 >> 24)) >> 3;
-      m |= (uint64_t)((r0 >= x ? 1u : 0u) | (r1 >= x ? 2u : 0u)) << (2 * k);
-    }
-  }
-  return m;
-}
-
-template <int BPE>
-__host__ __device__ __forceinline__ uint32_t sqf_lookup_win(const uint8_t* __restrict__ t, uint32_t S, uint32_t fq,
-                                                   uint32_t fr) {
-  SqfWin<BPE> W;
-  uint32_t cw = fq >> 6;
-  const uint32_t rel = fq & 63u;
-  sqf_win_load<BPE>(t, S, cw, W);
-  if (!((W.occ >> rel) & 1ull)) return SQF_NOT_FOUND;
-  // cluster start b: the last non-shifted slot at or before fq (or slot 0);
-  // k = occupied slots in (b, fq] = run starts to skip after b
-  uint64_t lim = rel == 63 ? ~0ull : ((2ull << rel) - 1ull);  // bits [0, rel]
-  uint32_t k = 0, brel = 0;
-  for (;;) {
-    const uint64_t ns = ~W.shf & lim;
-    if (ns) {
-      brel = 63u - (uint32_t)__builtin_clzll(ns);
-      k += (uint32_t)__builtin_popcountll(W.occ & lim & ~((2ull << brel) - 1ull));
-      break;
-    }
-    k += (uint32_t)__builtin_popcountll(W.occ & lim);
-    if (cw == 0) {
-      brel = 0;
-      k -= (uint32_t)(W.occ & 1ull);
-      break;
-    }
-    --cw;
-    sqf_win_load<BPE>(t, S, cw, W);
-    lim = ~0ull;
-  }
-  // s: the k-th slot after b without a continuation bit
-  uint32_t srel = brel;
-  uint64_t from = brel == 63 ? 0ull : (~0ull << (brel + 1));
-  while (k > 0) {
-    uint64_t rs = ~W.cont & from;
-    const uint32_t c = (uint32_t)__builtin_popcountll(rs);
-    if (c >= k) {
-      for (uint32_t i = 1; i < k; ++i) rs &= rs - 1ull;
-      srel = (uint32_t)__builtin_ctzll(rs);
-      break;
-    }
-    k -= c;
-    ++cw;
-    sqf_win_load<BPE>(t, S, cw, W);
-    from = ~0ull;
-  }
-  // the run: s and the continuation slots after it; first remainder >= fr
-  uint64_t run_from = 1ull << srel;
-  uint64_t after = srel == 63 ? 0ull : (~0ull << (srel + 1));
-  for (;;) {
-    const uint64_t stop = ~W.cont & after;
-    const uint64_t run = (run_from | after) & (stop ? ((1ull << __builtin_ctzll(stop)) - 1ull) : ~0ull);
-    const uint64_t ge = sqf_win_ge<BPE>(W, fr) & run;
-    if (ge) {
-      const uint32_t p = (uint32_t)__builtin_ctzll(ge);
-      return ((sqf_win_ge<BPE>(W, fr + 1) >> p) & 1ull) ? SQF_NOT_FOUND : cw * 64u + p;
-    }
-    if (stop) return SQF_NOT_FOUND;
-    ++cw;
-    sqf_win_load<BPE>(t, S, cw, W);
-    run_from = 0;
-    after = ~0ull;
-  }
-}
-
-template <int BPE, bool WIN>
-__device__ __forceinline__ uint32_t sqf_lookup_fp(const uint8_t* __restrict__ t, uint32_t S, uint32_t fq,
-                                                  uint32_t fr) {
-  if (WIN) return sqf_lookup_win<BPE>(t, S, fq, fr);
-  return sqf_lookup_one<BPE>(t, S, fq, fr);
-}
-
-template <int BPE, bool WIN>
-__global__ __launch_bounds__(256) void k_sqf_lookup(const uint8_t* __restrict__ t, uint32_t S, uint32_t q, uint32_t r,
-                                                    const uint32_t* __restrict__ keys, uint32_t n,
-                                                    uint32_t* __restrict__ out) {
-  const uint64_t i = blockIdx.x * 256ull + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t fp = (uint32_t)fingerprint(keys[i], q + r);
-  out[i] = sqf_lookup_fp<BPE, WIN>(t, S, (fp >> r) & (uint32_t)mask64(q), fp & ((1u << r) - 1u));
-}
-
-// Lookups in fingerprint order (sorted path): result scattered to the
-// original index.
-template <int BPE, bool WIN>
-__global__ __launch_bounds__(256) void k_sqf_lookup_sorted(const uint8_t* __restrict__ t, uint32_t S, uint32_t q,
-                                                           uint32_t r, const uint32_t* __restrict__ fps,
-                                                           const uint32_t* __restrict__ idx, uint32_t n,
-                                                           uint32_t* __restrict__ out) {
-  const uint64_t i = blockIdx.x * 256ull + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t fp = fps[i];
-  out[idx[i]] = sqf_lookup_fp<BPE, WIN>(t, S, (fp >> r) & (uint32_t)mask64(q), fp & ((1u << r) - 1u));
-}
-
-// ---------------------------------------------------------------- fused rebuild
-// From sorted fingerprints to the finished table: optional merge of two sorted
-// inputs (merge path, A first on ties), the keep rule (all / unique /
-// dedup-insert / delete flags), the canonical locations
-// loc_i = max(fq_i, loc_{i-1}+1), and the table bytes.  A block of entries
-// acts on the incoming location as x -> max(x + c, l); the states (c, l, f)
-// (f: quotient of the last kept entry, which decides run heads) are combined
-// in chunk order:
-//   k_rb_agg    each chunk's state (one read of the inputs),
-//   k_rb_scan   exclusive scan over the chunks (one workgroup); the total
-//               decides capacity before anything is written,
-//   k_rb_write  each chunk walks its 4096-entry sub-tiles with a running
-//               prefix; sub-tile u owns the slot range (L_{u-1}, L_u] ending at
-//               its last entry and writes every byte of it through an LDS
-//               window, each thread the slots up to its own last entry from
-//               registers; slots past the last entry are cleared by
-//               k_sqf_clear_tail.  An occupied bit that falls in an earlier
-//               sub-tile's range goes to a fix-up list (k_sqf_fixup).
-// Replaces the location and write stages of the three reference builds
-// (quotientFilter.cu:788-857, 985-1038, 1492-1539), the merge rebuild
-// (quotientFilter.cu:1651-1719) and the end states of insert /
-// superclusterDeletes.
-constexpr uint32_t RB_WIN = 8192;  // table bytes per write window
-constexpr uint32_t RB_FIX = 64;    // per-sub-tile occupied-bit fix-up slots
-
-// This thread's RB_ITEMS consecutive entries of sub-tile [d0, d0+tn):
-// v[1+j] = entry j and v[0] = the entry before them; bit j of tags = entry j
-// comes from B; fl = delete flags of the entries (KEEP_FLAGS).  MERGE:
-// the sub-tile's A and B ranges are staged in LDS (s_stage) and each thread
-// merges from its own merge-path split; otherwise 16-byte loads.
-template <int MODE, bool MERGE>
-__device__ __forceinline__ void rb_load(const uint32_t* __restrict__ A, const uint32_t* __restrict__ B,
-                                        const uint8_t* __restrict__ flags, const uint32_t* __restrict__ splits,
-                                        uint32_t u, uint32_t d0, uint32_t tn, uint32_t* s_stage, uint32_t* v,
-                                        uint32_t& tags, uint32_t& fl) {
-  const int tid = threadIdx.x;
-  const uint32_t i0 = (uint32_t)tid * RB_ITEMS;
-  fl = 0;
-  tags = 0;
-  if (MERGE) {
-    const uint32_t a0 = splits[u], a1 = splits[u + 1];
-    const uint32_t b0 = d0 - a0, b1 = d0 + tn - a1;
-    const uint32_t la = a1 - a0, lb = b1 - b0;  // la + lb == tn <= RB_TILE
-    {
-      uint32_t w[RB_ITEMS];
-#pragma unroll
-      for (int k = 0; k < RB_ITEMS; ++k) {
-        const uint32_t i = tid + k * RB_THREADS;
-        w[k] = i < la ? A[a0 + i] : (i < la + lb ? B[b0 + (i - la)] : 0u);
-      }
-#pragma unroll
-      for (int k = 0; k < RB_ITEMS; ++k) {
-        const uint32_t i = tid + k * RB_THREADS;
-        if (i < la + lb) s_stage[i] = w[k];
-      }
-    }
-    __syncthreads();
-    const uint32_t* sA = s_stage;
-    const uint32_t* sB = s_stage + la;
-    const uint32_t ld = i0 < tn ? i0 : tn;
-    uint32_t ai = merge_path(sA, la, sB, lb, ld);
-    uint32_t bi = ld - ai;
-    // the merged entry before position ld: the later (in merge order: value,
-    // then A before B) of the last A and the last B entry taken
-    const bool hasA = ai > 0 || a0 > 0, hasB = bi > 0 || b0 > 0;
-    const uint32_t pA = hasA ? (ai > 0 ? sA[ai - 1] : A[a0 - 1]) : 0u;
-    const uint32_t pB = hasB ? (bi > 0 ? sB[bi - 1] : B[b0 - 1]) : 0u;
-    v[0] = (hasB && (!hasA || pA <= pB)) ? pB : pA;
-#pragma unroll
-    for (int j = 0; j < RB_ITEMS; ++j) {
-      uint32_t x = 0;
-      if (ld + j < tn) {
-        const bool takeA = bi >= lb || (ai < la && sA[ai] <= sB[bi]);
-        if (takeA) {
-          x = sA[ai++];
-        } else {
-          x = sB[bi++];
-          tags |= 1u << j;
-        }
-      }
-      v[1 + j] = x;
-    }
-  } else {
-    const uint64_t g0 = (uint64_t)d0 + i0;
-    uint32_t w[RB_ITEMS];
-    if (i0 + RB_ITEMS <= tn) {
-      // d0 is a multiple of RB_TILE and A is 256-byte aligned: 32-byte aligned runs
-      const uint4* p = reinterpret_cast<const uint4*>(A + g0);
-#pragma unroll
-      for (int k = 0; k < RB_ITEMS / 4; ++k) {
-        const uint4 x = p[k];
-        w[4 * k] = x.x;
-        w[4 * k + 1] = x.y;
-        w[4 * k + 2] = x.z;
-        w[4 * k + 3] = x.w;
-      }
-      if (MODE == KEEP_FLAGS) {
-        const uint2 f = *reinterpret_cast<const uint2*>(flags + g0);
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          fl |= ((f.x >> (8 * k)) & 1u) << k;
-          fl |= ((f.y >> (8 * k)) & 1u) << (4 + k);
-        }
-      }
-    } else {
-#pragma unroll
-      for (int k = 0; k < RB_ITEMS; ++k) {
-        w[k] = (i0 + k < tn) ? A[g0 + k] : 0u;
-        if (MODE == KEEP_FLAGS && i0 + k < tn && flags[g0 + k]) fl |= 1u << k;
-      }
-    }
-    v[0] = g0 > 0 && i0 <= tn ? A[g0 - 1] : 0u;
-#pragma unroll
-    for (int k = 0; k < RB_ITEMS; ++k) v[1 + k] = w[k];
-  }
-}
-
-// Keep rule of this thread's entries (bit j of the result) and their scan state.
-template <int MODE>
-__device__ __forceinline__ uint32_t rb_classify(const uint32_t* v, uint32_t tags, uint32_t fl, uint32_t d0,
-                                                uint32_t tn, uint32_t r, uint64_t mq, RS& agg) {
-  uint32_t keepbits = 0;
-  agg = rs_none();
-  const uint32_t i0 = (uint32_t)threadIdx.x * RB_ITEMS;
-#pragma unroll
-  for (int j = 0; j < RB_ITEMS; ++j) {
-    const uint32_t i = i0 + j;
-    if (i < tn) {
-      const uint32_t fp = v[1 + j], pfp = v[j];
-      const bool first = (d0 + i) == 0;
-      bool kp;
-      if (MODE == KEEP_ALL) kp = true;
-      else if (MODE == KEEP_UNIQUE) kp = first || fp != pfp;
-      else if (MODE == KEEP_TAGGED_INSERT) kp = ((tags >> j) & 1u) == 0 || first || fp != pfp;
-      else kp = (fl >> j) & 1u;
-      if (kp) {
-        keepbits |= 1u << j;
-        const uint32_t fq = (uint32_t)(((uint64_t)fp >> r) & mq);
-        const int64_t f = (int64_t)fq;
-        agg.l = agg.l == LOC_NONE ? f : (f > agg.l + 1 ? f : agg.l + 1);
-        agg.c++;
-        agg.f = fq;
-      }
-    }
-  }
-  return keepbits;
-}
-
-template <int MODE, bool MERGE>
-__global__ __launch_bounds__(RB_THREADS, 8) void k_rb_agg(const uint32_t* __restrict__ A, uint32_t na,
-                                                const uint32_t* __restrict__ B, uint32_t nb,
-                                                const uint8_t* __restrict__ flags, const uint32_t* __restrict__ splits,
-                                                uint32_t q, uint32_t r, uint32_t chunk,
-                                                uint4* __restrict__ agg) {
-  __shared__ uint32_t s_stage[MERGE ? RB_TILE : 1];
-  __shared__ uint4 s_wave[RB_WAVES + 1];
-  const uint64_t N = (uint64_t)na + (MERGE ? nb : 0u);
-  const uint64_t mq = mask64(q);
-  const uint64_t c0 = (uint64_t)blockIdx.x * chunk;
-  const uint64_t c1 = c0 + chunk < N ? c0 + chunk : N;
-  RS X = rs_none();
-  for (uint64_t d0 = c0; d0 < c1; d0 += RB_TILE) {
-    const uint32_t tn = (uint32_t)(c1 - d0 < (uint64_t)RB_TILE ? c1 - d0 : (uint64_t)RB_TILE);
-    uint32_t v[RB_ITEMS + 1];
-    uint32_t tags, fl;
-    rb_load<MODE, MERGE>(A, B, flags, splits, (uint32_t)(d0 / RB_TILE), (uint32_t)d0, tn, s_stage, v, tags, fl);
-    RS a;
-    (void)rb_classify<MODE>(v, tags, fl, (uint32_t)d0, tn, r, mq, a);
-    RS ex;
-    const RS sub = rs_block(a, &ex, s_wave);
-    X = rs_combine(X, sub);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) agg[blockIdx.x] = rs_pack(X);
-}
-
-// Writes the element bytes of slot sl into the window (big-endian).
-template <int BPE>
-__device__ __forceinline__ void win_put(uint8_t* s_win, uint32_t off, uint32_t e) {
-#pragma unroll
-  for (int b = 0; b < BPE; ++b) s_win[off + b] = (uint8_t)(e >> (8 * (BPE - 1 - b)));
-}
-
-// Copies window bytes [B0, B1) of the table from LDS (dword stores inside,
-// byte stores at unaligned edges).
-__device__ __forceinline__ void win_out(uint8_t* __restrict__ t, const uint8_t* s_win, uint64_t B0, uint64_t B1) {
-  const int tid = threadIdx.x;
-  const uint64_t A0 = (B0 + 3) & ~3ull, A1 = B1 & ~3ull;
-  if (A0 >= A1) {
-    for (uint64_t b = B0 + tid; b < B1; b += RB_THREADS) t[b] = s_win[b - B0];
-    return;
-  }
-  for (uint64_t b = B0 + tid; b < A0; b += RB_THREADS) t[b] = s_win[b - B0];
-  for (uint64_t b = A1 + tid; b < B1; b += RB_THREADS) t[b] = s_win[b - B0];
-  uint32_t* tw = reinterpret_cast<uint32_t*>(t + A0);
-  const uint32_t nw = (uint32_t)((A1 - A0) / 4), sh = (uint32_t)(A0 - B0);
-  for (uint32_t w = tid; w < nw; w += RB_THREADS) {
-    const uint32_t q0 = sh + 4 * w;
-    tw[w] = (uint32_t)s_win[q0] | ((uint32_t)s_win[q0 + 1] << 8) | ((uint32_t)s_win[q0 + 2] << 16) |
-            ((uint32_t)s_win[q0 + 3] << 24);
-  }
-}
-
-// Each chunk walks its sub-tiles with the running state X.  Sub-tile u owns
-// the slots (X.l, Y.l] up to its last kept entry; each thread owns the slots
-// after the previous kept entry up to its own last one (its territory) and
-// writes them -- elements and the zeros between them -- into an LDS window
-// straight from registers.  Occupied bits are then set in the window (each
-// run head's quotient is a different slot, so plain byte updates do not
-// race); those whose quotient precedes the owned range go to the fix-up
-// list.  A long owned range (sparse table) is written window by window.
-// Writes one sub-tile: block scan of the per-thread states (v, keepbits,
-// a) on top of the incoming state X, fix-up entries for run heads whose
-// quotient precedes the owned range, then the owned slot range window by
-// window.  Every thread of the block calls it; returns the outgoing state.
-template <int BPE>
-__device__ __forceinline__ RS rb_emit(uint8_t* __restrict__ t, uint32_t q, uint32_t r, const uint32_t* v,
-                                      uint32_t keepbits, RS a, RS X, uint32_t u, uint4* s_wave,
-                                      uint8_t* s_win, uint32_t* s_nfix_p,
-                                      uint32_t* __restrict__ fix, uint32_t* __restrict__ fix_cnt,
-                                      uint32_t* __restrict__ fix_more, uint32_t* __restrict__ fix_more_n,
-                                      uint32_t fix_cap, int precleared) {
-  const int tid = threadIdx.x;
-  const uint64_t mq = mask64(q);
-  const uint32_t rmask = (1u << r) - 1u;
-  constexpr uint32_t bpe = BPE;
-  const uint32_t wslots = RB_WIN / BPE;
-  uint32_t& s_nfix = *s_nfix_p;
-    if (tid == 0) s_nfix = 0;
-    RS ex;
-    const RS sub = rs_block(a, &ex, s_wave);
-    const RS me = rs_combine(X, ex);  // state before this thread's entries
-    const RS Y = rs_combine(X, sub);
-    const uint32_t Llo = X.l == LOC_NONE ? 0u : (uint32_t)(X.l + 1);
-    const uint32_t Lhi = sub.c > 0 ? (uint32_t)(Y.l + 1) : Llo;
-    // per-thread location walk in 32 bits: -1 = no entry yet (fq >= 0)
-    const int32_t prev0 = me.l == LOC_NONE ? -1 : (int32_t)me.l;
-    const int32_t tstart = prev0 + 1;
-    // run heads whose quotient precedes the owned range: fix-up list
-    {
-      uint32_t lastf = me.f;
-#pragma unroll
-      for (int j = 0; j < RB_ITEMS; ++j) {
-        if (keepbits & (1u << j)) {
-          const uint32_t fp = v[1 + j];
-          const uint32_t fq = (fp >> r) & (uint32_t)mq;
-          if (fq != lastf && fq < Llo) {
-            const uint32_t k = atomicAdd(&s_nfix, 1u);
-            if (k < RB_FIX) {
-              fix[(uint64_t)u * RB_FIX + k] = fq;
-            } else {
-              const uint32_t slot = atomicAdd(fix_more_n, 1u);
-              if (slot < fix_cap) fix_more[slot] = fq;
-            }
-          }
-          lastf = fq;
-        }
-      }
-    }
-    for (uint32_t w0 = Llo; w0 < Lhi; w0 += wslots) {
-      const uint32_t w1 = w0 + wslots < Lhi ? w0 + wslots : Lhi;
-      if (precleared) {
-        // the table is all zeros: windows holding no element and no
-        // occupied bit are skipped; the others start from a zeroed window
-        bool any = false;
-        int32_t prev = prev0;
-        uint32_t lastf = me.f;
-#pragma unroll
-        for (int j = 0; j < RB_ITEMS; ++j) {
-          if (keepbits & (1u << j)) {
-            const uint32_t fp = v[1 + j];
-            const int32_t fq = (int32_t)((fp >> r) & (uint32_t)mq);
-            const int32_t lc = fq > prev + 1 ? fq : prev + 1;
-            if ((uint32_t)lc >= w0 && (uint32_t)lc < w1) any = true;
-            if ((uint32_t)fq != lastf && (uint32_t)fq >= w0 && (uint32_t)fq < w1) any = true;
-            prev = lc;
-            lastf = (uint32_t)fq;
-          }
-        }
-        if (!__syncthreads_or(any)) continue;
-        const uint32_t nw = ((w1 - w0) * bpe + 3) / 4;
-        for (uint32_t k = tid; k < nw; k += RB_THREADS) reinterpret_cast<uint32_t*>(s_win)[k] = 0u;
-        __syncthreads();
-      }
-      // this thread's territory within the window: elements, and the zeros
-      // before each of them unless the window was zeroed above
-      {
-        int32_t prev = prev0;
-        uint32_t lastf = me.f;
-        int32_t cur = tstart > (int32_t)w0 ? tstart : (int32_t)w0;  // next slot to write
-#pragma unroll
-        for (int j = 0; j < RB_ITEMS; ++j) {
-          if (keepbits & (1u << j)) {
-            const uint32_t fp = v[1 + j];
-            const int32_t fq = (int32_t)((fp >> r) & (uint32_t)mq);
-            const int32_t lc = fq > prev + 1 ? fq : prev + 1;
-            const bool head = (uint32_t)fq != lastf;
-            if (!precleared) {
-              const int32_t zend = lc < (int32_t)w1 ? lc : (int32_t)w1;
-              for (; cur < zend; ++cur) win_put<BPE>(s_win, (uint32_t)(cur - (int32_t)w0) * BPE, 0u);
-            }
-            if ((uint32_t)lc >= w0 && (uint32_t)lc < w1) {
-              uint32_t e = (fp & rmask) << 3;
-              if (!head) e |= 2u;
-              if (lc != fq) e |= 1u;
-              win_put<BPE>(s_win, ((uint32_t)lc - w0) * BPE, e);
-            }
-            if (lc + 1 > cur) cur = lc + 1;
-            prev = lc;
-            lastf = (uint32_t)fq;
-          }
-        }
-      }
-      __syncthreads();
-      // occupied bits of the run heads whose quotient is in this window
-      {
-        uint32_t lastf = me.f;
-#pragma unroll
-        for (int j = 0; j < RB_ITEMS; ++j) {
-          if (keepbits & (1u << j)) {
-            const uint32_t fp = v[1 + j];
-            const uint32_t fq = (fp >> r) & (uint32_t)mq;
-            if (fq != lastf && fq >= w0 && fq < w1) s_win[(fq - w0) * bpe + bpe - 1] |= 4u;
-            lastf = fq;
-          }
-        }
-      }
-      __syncthreads();
-      win_out(t, s_win, (uint64_t)w0 * bpe, (uint64_t)w1 * bpe);
-      __syncthreads();
-    }
-    __syncthreads();
-    if (tid == 0) fix_cnt[u] = s_nfix < RB_FIX ? s_nfix : RB_FIX;
-  return Y;
-}
-
-template <int MODE, bool MERGE, int BPE>
-__global__ __launch_bounds__(RB_THREADS, 4) void k_rb_write(uint8_t* __restrict__ t, uint32_t S, uint32_t q, uint32_t r,
-                                                  const uint32_t* __restrict__ A, uint32_t na,
-                                                  const uint32_t* __restrict__ B, uint32_t nb,
-                                                  const uint8_t* __restrict__ flags,
-                                                  const uint32_t* __restrict__ splits, uint32_t chunk,
-                                                  const uint4* __restrict__ pre,
-                                                  const uint32_t* __restrict__ ctr, uint32_t* __restrict__ fix,
-                                                  uint32_t* __restrict__ fix_cnt, uint32_t* __restrict__ fix_more,
-                                                  uint32_t* __restrict__ fix_more_n, uint32_t fix_cap,
-                                                  int precleared) {
-  if (ctr[CTR_ERR]) return;  // capacity error: the table stays as it was
-  __shared__ uint32_t s_stage[MERGE ? RB_TILE : 1];
-  __shared__ __align__(16) uint8_t s_win[RB_WIN];
-  __shared__ uint4 s_wave[RB_WAVES + 1];
-  __shared__ uint32_t s_nfix;
-  const uint64_t N = (uint64_t)na + (MERGE ? nb : 0u);
-  const uint64_t mq = mask64(q);
-  const uint64_t c0 = (uint64_t)blockIdx.x * chunk;
-  const uint64_t c1 = c0 + chunk < N ? c0 + chunk : N;
-  RS X = rs_unpack(pre[blockIdx.x]);
-  for (uint64_t d0 = c0; d0 < c1; d0 += RB_TILE) {
-    const uint32_t u = (uint32_t)(d0 / RB_TILE);
-    const uint32_t tn = (uint32_t)(c1 - d0 < (uint64_t)RB_TILE ? c1 - d0 : (uint64_t)RB_TILE);
-    uint32_t v[RB_ITEMS + 1];
-    uint32_t tags, fl;
-    rb_load<MODE, MERGE>(A, B, flags, splits, u, (uint32_t)d0, tn, s_stage, v, tags, fl);
-    RS a;
-    const uint32_t keepbits = rb_classify<MODE>(v, tags, fl, (uint32_t)d0, tn, r, mq, a);
-    X = rb_emit<BPE>(t, q, r, v, keepbits, a, X, u, s_wave, s_win, &s_nfix, fix, fix_cnt, fix_more,
-                     fix_more_n, fix_cap, precleared);
-    __syncthreads();
-  }
-}
-// Zeroes every table byte from the slot after the last entry to the end of
-// the allocation (the 10% overflow area, usually), in parallel.
-// (whole_table: every byte, used to pre-clear sparse tables).  Nothing is
-// cleared after a capacity error.
-__global__ __launch_bounds__(256) void k_sqf_clear_tail(uint8_t* __restrict__ t, uint32_t bpe, uint64_t tbytes,
-                                                        const uint32_t* __restrict__ ctr, int whole_table) {
-  if (ctr[CTR_ERR]) return;  // capacity error: the table stays as it was
-  const uint64_t b0 = whole_table ? 0ull : (uint64_t)ctr[CTR_MISC + 3] * bpe;
-  const uint64_t a0 = (b0 + 15) & ~15ull, a1 = tbytes & ~15ull;
-  const uint64_t tid = blockIdx.x * 256ull + threadIdx.x, nth = (uint64_t)gridDim.x * 256;
-  if (a0 >= a1) {
-    for (uint64_t b = b0 + tid; b < tbytes; b += nth) t[b] = 0;
-    return;
-  }
-  for (uint64_t b = b0 + tid; b < a0; b += nth) t[b] = 0;
-  for (uint64_t b = a1 + tid; b < tbytes; b += nth) t[b] = 0;
-  uint4* w = reinterpret_cast<uint4*>(t + a0);
-  const uint64_t nw = (a1 - a0) / 16;
-  for (uint64_t i = tid; i < nw; i += nth) w[i] = make_uint4(0u, 0u, 0u, 0u);
-}
-
-// Occupied bits that rebuild tiles could not place in their own ranges: one
-// workgroup per tile's slots, plus the shared overflow list.
-__device__ __forceinline__ void sqf_set_occupied(uint8_t* t, uint32_t bpe, uint32_t fq) {
-  const uint64_t byte = (uint64_t)fq * bpe + bpe - 1;  // the element's low byte holds the flag bits
-  uint32_t* word = reinterpret_cast<uint32_t*>(t + (byte & ~3ull));
-  atomicOr(word, 4u << (8 * (uint32_t)(byte & 3)));
-}
-
-__global__ __launch_bounds__(64) void k_sqf_fixup(uint8_t* __restrict__ t, uint32_t bpe,
-                                                  const uint32_t* __restrict__ fix,
-                                                  const uint32_t* __restrict__ fix_cnt,
-                                                  const uint32_t* __restrict__ fix_more,
-                                                  const uint32_t* __restrict__ ctr, uint32_t tiles,
-                                                  uint32_t fix_cap) {
-  if (ctr[CTR_ERR]) return;  // capacity error: the table stays as it was
-  const uint32_t tile = blockIdx.x;
-  if (tile < tiles) {
-    const uint32_t c = fix_cnt[tile];
-    for (uint32_t i = threadIdx.x; i < c; i += 64) sqf_set_occupied(t, bpe, fix[(uint64_t)tile * RB_FIX + i]);
-  } else {
-    uint32_t n = ctr[CTR_MISC + 2];  // shared overflow list length
-    if (n > fix_cap) n = fix_cap;
-    for (uint32_t i = threadIdx.x; i < n; i += 64) sqf_set_occupied(t, bpe, fix_more[i]);
-  }
-}
-
-// ---------------------------------------------------------------- decode
-// Every stored fingerprint in slot (= ascending) order.  The k-th run start of
-// the table belongs to the k-th occupied quotient, so a region decodes from
-// three exclusive counts (entries, run starts, occupied bits) before it, plus
-// the occupied positions just before it that still own runs.  Replaces
-// extractQuotients + remove_if (quotientFilter.cu:1561-1600, 1627).
-constexpr uint32_t DC_SLOTS = 4096;
-constexpr uint32_t DC_PEND = 1024;
-
-// The 16 slots [s, s+16) of a thread: 16-byte loads when whole, else
-// bounded byte loads (past the last slot reads as empty).
-template <int BPE>
-__device__ __forceinline__ void sqf_load16(const uint8_t* __restrict__ t, uint32_t S, uint32_t s, uint32_t* e) {
-  if (s + 16 <= S && (reinterpret_cast<uintptr_t>(t) & 15u) == 0) {
-    uint32_t w[4 * BPE];
-    const uint4* p = reinterpret_cast<const uint4*>(t + (uint64_t)s * BPE);
-#pragma unroll
-    for (int k = 0; k < BPE; ++k) {
-      const uint4 x = p[k];
-      w[4 * k] = x.x;
-      w[4 * k + 1] = x.y;
-      w[4 * k + 2] = x.z;
-      w[4 * k + 3] = x.w;
-    }
-#pragma unroll
-    for (int j = 0; j < 16; ++j) {
-      uint32_t v = 0;
-#pragma unroll
-      for (int b = 0; b < BPE; ++b) {
-        const int at = j * BPE + b;
-        v = (v << 8) | ((w[at >> 2] >> (8 * (at & 3))) & 0xFFu);
-      }
-      e[j] = v;
-    }
-  } else {
-#pragma unroll
-    for (int j = 0; j < 16; ++j) e[j] = sqf_get<BPE>(t, S, s + j);
-  }
-}
-
-template <int BPE>
-__global__ __launch_bounds__(256) void k_sqf_count(const uint8_t* __restrict__ t, uint32_t S,
-                                                   uint32_t* __restrict__ counts) {
-  __shared__ uint64_t s_tmp[4];
-  const uint32_t s0 = blockIdx.x * DC_SLOTS + threadIdx.x * 16;
-  uint32_t e[16];
-  sqf_load16<BPE>(t, S, s0, e);
-  uint32_t ci = 0, cr = 0, co = 0;
-#pragma unroll
-  for (int j = 0; j < 16; ++j) {
-    ci += (e[j] & 7u) != 0;
-    cr += ((e[j] & 7u) != 0) && !(e[j] & 2u);
-    co += (e[j] & 4u) != 0;
-  }
-  uint64_t tot;
-  block256_excl_scan64((uint64_t)ci | ((uint64_t)cr << 21) | ((uint64_t)co << 42), s_tmp, &tot);
-  if (threadIdx.x == 0) {
-    counts[blockIdx.x * 3 + 0] = (uint32_t)(tot & 0x1FFFFF);
-    counts[blockIdx.x * 3 + 1] = (uint32_t)((tot >> 21) & 0x1FFFFF);
-    counts[blockIdx.x * 3 + 2] = (uint32_t)(tot >> 42);
-  }
-}
-
-// Quotient of the entry at slot idx by walking its cluster (extractQuotients,
-// quotientFilter.cu:1580-1599); fallback for very long pending chains.
-template <int BPE>
-__device__ uint32_t sqf_walk_quotient(const uint8_t* __restrict__ t, uint32_t S, uint32_t idx) {
-  uint32_t b = idx;
-  do {
-    b--;
-  } while (b > 0 && (sqf_get<BPE>(t, S, b) & 1u));
-  uint32_t s = b;
-  while (s <= idx) {
-    do {
-      s++;
-    } while (sqf_get<BPE>(t, S, s) & 2u);
-    if (s > idx) break;
-    do {
-      b++;
-    } while (!(sqf_get<BPE>(t, S, b) & 4u) && b < S);
-  }
-  return b;
-}
-
-template <int BPE, typename K>
-__global__ __launch_bounds__(256) void k_sqf_decode(const uint8_t* __restrict__ t, uint32_t S, uint32_t r,
-                                                    const uint32_t* __restrict__ bases, K* __restrict__ out) {
-  __shared__ uint16_t s_occpos[DC_SLOTS];
-  __shared__ uint32_t s_pend[DC_PEND];
-  __shared__ K s_out[DC_SLOTS];
-  __shared__ uint64_t s_tmp[4];
-  __shared__ uint32_t s_tmp32[4];
-  const uint32_t tid = threadIdx.x;
-  const uint32_t s0 = blockIdx.x * DC_SLOTS;
-  const uint32_t bI = bases[blockIdx.x * 3 + 0], bR = bases[blockIdx.x * 3 + 1], bO = bases[blockIdx.x * 3 + 2];
-  uint32_t e[16];
-  sqf_load16<BPE>(t, S, s0 + tid * 16, e);
-  uint32_t ci = 0, cr = 0, co = 0;
-#pragma unroll
-  for (int j = 0; j < 16; ++j) {
-    ci += (e[j] & 7u) != 0;
-    cr += ((e[j] & 7u) != 0) && !(e[j] & 2u);
-    co += (e[j] & 4u) != 0;
-  }
-  uint64_t btot;
-  const uint64_t ex = block256_excl_scan64((uint64_t)ci | ((uint64_t)cr << 21) | ((uint64_t)co << 42), s_tmp, &btot);
-  const uint32_t nout = (uint32_t)(btot & 0x1FFFFF);
-  uint32_t xi = (uint32_t)(ex & 0x1FFFFF), xr = (uint32_t)((ex >> 21) & 0x1FFFFF), xo = (uint32_t)(ex >> 42);
-  {
-    uint32_t o = xo;
-#pragma unroll
-    for (int j = 0; j < 16; ++j)
-      if (e[j] & 4u) s_occpos[o++] = (uint16_t)(tid * 16 + j);
-  }
-  // occupied slots before s0 whose runs start at or after s0 (nearest first):
-  // bO - bR of them, plus the owner of a run that continues into the region
-  __shared__ uint32_t s_first;
-  if (tid == 0) s_first = e[0];
-  __syncthreads();
-  const uint32_t want = (bO >= bR ? bO - bR : 0u) + ((s_first & 2u) ? 1u : 0u);
-  uint32_t got = 0;
-  for (uint32_t c = s0; got < want && got < DC_PEND && c > 0;) {
-    const uint32_t lo = c > 256 ? c - 256 : 0u;
-    const uint32_t slot = lo + tid;
-    const uint32_t isocc = (slot < c) && (sqf_get<BPE>(t, S, slot) & 4u) ? 1u : 0u;
-    uint32_t total;
-    const uint32_t below = block256_excl_scan(isocc, s_tmp32, &total);
-    if (isocc) {
-      const uint32_t fromEnd = got + (total - below - 1);
-      if (fromEnd < DC_PEND) s_pend[fromEnd] = slot;
-    }
-    got += total;
-    c = lo;
-  }
-  if (got > DC_PEND) got = DC_PEND;
-  __syncthreads();
-#pragma unroll
-  for (int j = 0; j < 16; ++j) {
-    const uint32_t el = e[j];
-    if (!(el & 7u)) continue;
-    if (!(el & 2u)) xr++;
-    const uint32_t k = bR + xr;  // 1-based run index of this entry
-    uint32_t fq;
-    if (k > bO) {
-      fq = s0 + s_occpos[k - bO - 1];
-    } else if (k >= 1 && bO - k < got) {
-      fq = s_pend[bO - k];
-    } else {
-      fq = sqf_walk_quotient<BPE>(t, S, s0 + tid * 16 + j);
-    }
-    s_out[xi] = (K)(((uint64_t)fq << r) | (el >> 3));
-    xi++;
-  }
-  __syncthreads();
-  for (uint32_t k = tid; k < nout; k += 256) out[bI + k] = s_out[k];
-}
 
 // ---------------------------------------------------------------- pipelines
 struct SqfGeom {

@@ -173,6 +173,46 @@ size_t gqf_rsqf_workspace_bytes(gqf_rsqf cqf, int op, int n, int sorted);
 gqf_status gqf_rsqf_dump(gqf_rsqf cqf, void* host_ref_layout);
 gqf_status gqf_rsqf_load(gqf_rsqf cqf, const void* host_ref_layout);
 
+/* ================================================================ batches
+ * Many SQFs per launch (no reference counterpart: the reference launches every
+ * kernel per filter).  A batch binds F initialised SQFs of one r (any q), the
+ * build keys of all of them back to back in d_keys (filter f owns
+ * [key_offsets[f], key_offsets[f+1]); host array of F + 1 offsets) and their
+ * lookup keys likewise (d_probe, probe_offsets), results to d_out in the
+ * lookup layout.  Either key set may be absent (NULL offsets).  Creating the
+ * batch autotunes its launch shapes on these inputs -- running the build
+ * (which leaves every table built) and the lookups -- then captures both
+ * pipelines as hipGraphs.  Buffers stay bound: refill them in place between
+ * launches.  With a NULL stream the batch runs on a stream of its own that
+ * waits for the null stream (graphs cannot be captured on the null stream).
+ * Results equal the per-filter calls: gqf_sqf_bulk_build
+ * (method irrelevant) with the same keys and no_dup, and unsorted
+ * gqf_sqf_lookup; a filter whose keys do not fit keeps its table and reports
+ * GQF_ERR_CAPACITY in filter_status. */
+typedef struct gqf_sqf_batch_s* gqf_sqf_batch;
+
+typedef struct gqf_batch_info {
+  int filters;
+  uint32_t build_keys, lookup_keys;
+  int sort_passes, sort_chunks, sort_subtiles_per_chunk, rebuild_subtiles;
+  int lookup_block, lookup_keys_per_thread;  /* chosen lookup shape */
+  int build_graph_nodes, lookup_graph_nodes;
+  float tune_build_ms[4];   /* build time per sort chunk size tried (1, 2, 4, 8 sub-tiles; 0 = skipped) */
+  float tune_lookup_ms[15]; /* lookup time per shape tried: blocks 64..1024 x keys per thread 1, 2, 4 */
+} gqf_batch_info;
+
+gqf_status gqf_sqf_batch_create(gqf_sqf_batch* out, const gqf_sqf* filters, int count, int no_dup,
+                                const uint32_t* d_keys, const uint64_t* key_offsets, const uint32_t* d_probe,
+                                const uint64_t* probe_offsets, uint32_t* d_out, gqf_stream stream);
+/* Rebuild every filter of the batch from its keys: one graph launch
+ * (use_graph != 0) or the same kernels launched one by one.  filter_status
+ * (F entries, may be NULL: then any failure returns GQF_ERR_CAPACITY). */
+gqf_status gqf_sqf_batch_build(gqf_sqf_batch b, int use_graph, int* filter_status, float* ms);
+/* All lookups of the batch (unsorted path): one graph launch or one kernel. */
+gqf_status gqf_sqf_batch_lookup(gqf_sqf_batch b, int use_graph, float* ms);
+gqf_status gqf_sqf_batch_info(gqf_sqf_batch b, gqf_batch_info* info);
+gqf_status gqf_sqf_batch_destroy(gqf_sqf_batch b);
+
 /* ================================================================ helpers
  * Device memory for callers without a HIP toolchain (ctypes and the like). */
 gqf_status gqf_device_count(int* count);
