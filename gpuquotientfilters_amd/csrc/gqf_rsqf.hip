@@ -523,7 +523,7 @@ __device__ __forceinline__ void rr_visit(uint32_t fq, uint32_t fr, int32_t lc, i
 }
 
 template <typename K, int MODE, bool MERGE, int R>
-__global__ __launch_bounds__(RB_THREADS, 4) void k_rr_write(uint8_t* __restrict__ base, uint32_t nb, uint32_t q,
+__global__ __launch_bounds__(RB_THREADS, 6) void k_rr_write(uint8_t* __restrict__ base, uint32_t nb, uint32_t q,
                                                             uint32_t r, const K* __restrict__ A, uint32_t na,
                                                             const K* __restrict__ B, uint32_t nbk,
                                                             const uint8_t* __restrict__ flags,

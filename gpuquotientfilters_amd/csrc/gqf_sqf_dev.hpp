@@ -594,7 +594,7 @@ __device__ __forceinline__ RS rb_emit(uint8_t* __restrict__ t, uint32_t q, uint3
 }
 
 template <int MODE, bool MERGE, int BPE>
-__global__ __launch_bounds__(RB_THREADS, 4) void k_rb_write(uint8_t* __restrict__ t, uint32_t S, uint32_t q, uint32_t r,
+__global__ __launch_bounds__(RB_THREADS, 6) void k_rb_write(uint8_t* __restrict__ t, uint32_t S, uint32_t q, uint32_t r,
                                                   const uint32_t* __restrict__ A, uint32_t na,
                                                   const uint32_t* __restrict__ B, uint32_t nb,
                                                   const uint8_t* __restrict__ flags,
