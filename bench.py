@@ -410,7 +410,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (every box here has one GPU; "
-              "the multi-GPU sharding design is described in DESIGN.md, not run).", file=sys.stderr)
+              "sharding across GPUs is a design, not run here).", file=sys.stderr)
         print(json.dumps({"error": "only --gpus 1 is supported", "n_gpus": args.gpus}))
         sys.exit(2)
 

@@ -247,7 +247,7 @@ def test_deterministic_rebuild():
 
 
 def test_full_size_q26_properties():
-    """BASELINE config size (q=26, r=5, fill 0.8): checks that need no oracle
+    """Full size (q=26, r=5, fill 0.8): checks that need no oracle
     at this size -- no false negatives, FPR band, sorted == unsorted, extract
     round trip count, and a delete+lookup round trip."""
     q, r = 26, 5
