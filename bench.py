@@ -409,8 +409,8 @@ def main():
                          "to scale rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (every box here has one GPU; "
-              "sharding across GPUs is a design, not run here).", file=sys.stderr)
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (sharding across GPUs is a "
+              "design, not run here).", file=sys.stderr)
         print(json.dumps({"error": "only --gpus 1 is supported", "n_gpus": args.gpus}))
         sys.exit(2)
 
@@ -477,6 +477,15 @@ def main():
     kernels = {k: {"launches": v[0], "ms": round(v[1], 3),
                    "GB_s": round(v[2] / (v[1] / 1e3) / 1e9, 1) if v[2] else None}
                for k, v in sorted(prof.items(), key=lambda kv: -kv[1][1])}
+    # lookups against the random-access peak measured here: 2^24 random
+    # 64-byte sector reads over a buffer the size of the table
+    tbytes = G.lib().gqf_sqf_table_bytes(q, r) if args.workload == "sqf" else G.lib().gqf_rsqf_table_bytes(q, r)
+    rpeak = G.gather_peak(tbytes, 1 << 24)
+    spl = 1 if args.workload == "sqf" else 2  # sectors a lookup touches (one window; metadata + remainders)
+    lookup_roof = {"R_peak_Gsectors_s": round(rpeak / 1e9, 2), "buffer_bytes": int(tbytes), "sectors_per_lookup": spl}
+    for cls in ("lookup_unsorted", "lookup_sorted"):
+        rate = st.ops[cls] / (st.ms[cls] / 1e3)
+        lookup_roof[cls] = {"G_lookups_s": round(rate / 1e9, 2), "frac": round(rate * spl / rpeak, 3)}
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
     if os.path.exists(tpath):
@@ -513,6 +522,7 @@ def main():
         "config": cfg,
         "breakdown": breakdown,
         "roofline": roof,
+        "lookup_roofline": lookup_roof,
         "kernels": kernels,
     }
     if not args.no_batch:

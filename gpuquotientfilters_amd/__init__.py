@@ -49,6 +49,7 @@ class _RSQF(C.Structure):
 
 
 OP_LOOKUP, OP_INSERT, OP_DELETE = 0, 1, 2
+HASH_AP, HASH_FNV = 0, 1
 
 
 class Request(C.Structure):
@@ -113,6 +114,8 @@ _SIGS = {
     "gqf_memcpy_dtod": (_st, [_vp, _vp, C.c_size_t]),
     "gqf_memset": (_st, [_vp, C.c_int, C.c_size_t]),
     "gqf_synchronize": (_st, []),
+    "gqf_sqf_set_hash": (_st, [C.c_int]),
+    "gqf_sqf_get_hash": (C.c_int, []),
     "gqf_stream_create": (_st, [C.POINTER(_vp)]),
     "gqf_stream_destroy": (_st, [_vp]),
     "gqf_stream_workspace_bytes": (_st, [_vp, C.POINTER(C.c_size_t)]),
@@ -135,6 +138,7 @@ _SIGS = {
     "gqf_profile_get": (_st, [C.c_int, C.c_char_p, C.c_int, C.POINTER(C.c_uint64), C.POINTER(C.c_double),
                               C.POINTER(C.c_double)]),
     "gqf_mt19937_fill": (None, [C.c_uint32, C.c_uint64, C.c_size_t, _vp]),
+    "gqf_gather_bench": (_st, [C.c_size_t, C.c_uint32, C.c_int, _f]),
     "gqf_sqf_table_bytes": (C.c_size_t, [C.c_uint, C.c_uint]),
     "gqf_sqf_num_slots": (C.c_uint32, [C.c_uint]),
     "gqf_sqf_init": (_st, [C.POINTER(_SQF), C.c_uint, C.c_uint]),
@@ -213,6 +217,24 @@ def profile_stats():
         check(lib().gqf_profile_get(i, buf, 512, C.byref(n), C.byref(ms), C.byref(by)))
         out[buf.value.decode()] = (n.value, ms.value, by.value)
     return out
+
+
+def set_sqf_hash(kind):
+    """Hash of this thread's SQF calls: HASH_AP (default, the reference GPU
+    code's) or HASH_FNV (the reference CPU_version's)."""
+    check(lib().gqf_sqf_set_hash(int(kind)))
+
+
+def sqf_hash():
+    return lib().gqf_sqf_get_hash()
+
+
+def gather_peak(buffer_bytes, n=1 << 24, reps=5):
+    """Random 64-byte sector reads per second over a buffer of buffer_bytes
+    (gqf_gather_bench): the peak lookups are measured against."""
+    ms = C.c_float(0)
+    check(lib().gqf_gather_bench(int(buffer_bytes), int(n), int(reps), C.byref(ms)))
+    return n / (ms.value / 1e3)
 
 
 def mt19937(seed, n, skip=0):

@@ -640,14 +640,14 @@ static gqf_status batch_create(SqfBatch* b, const gqf_sqf* filters, int count, i
     x.S = gqf_sqf_num_slots(q.qbits);
     x.q = q.qbits;
     x.r = q.rbits;
-    x.bits = q.qbits + q.rbits;
+    x.bits = (q.qbits + q.rbits) | sqf_hash_flag();  // width and hash of the fingerprints
     x.key0 = (uint32_t)b->koff[(size_t)f];
     x.nkeys = (uint32_t)(b->koff[(size_t)f + 1] - b->koff[(size_t)f]);
     x.probe0 = (uint32_t)b->poff[(size_t)f];
     x.nprobe = (uint32_t)(b->poff[(size_t)f + 1] - b->poff[(size_t)f]);
     x.more0 = x.key0 + (uint32_t)f;
     x.pad = 0;
-    maxbits = std::max(maxbits, x.bits);
+    maxbits = std::max(maxbits, q.qbits + q.rbits);
     for (uint32_t d0 = 0; d0 < x.nkeys; d0 += RB_TILE)
       tl.push_back({(uint32_t)f, d0, std::min<uint32_t>(RB_TILE, x.nkeys - d0), d0 == 0 ? 1u : 0u});
   }

@@ -173,10 +173,27 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t d) {
   return m;
 }
 
-// Fingerprint when the result fits 32 bits (q + r <= 32): the AP value only.
+// FNVhashGPU (quotientFilter.cu:117-132): FNV-1 over the key's big-endian
+// bytes (multiply by the FNV prime, then xor the byte), from the FNV offset
+// basis.  The SQF's alternative hash (GQF_HASH_FNV).
+__host__ __device__ __forceinline__ uint32_t fnv_rounds(uint32_t key) {
+  uint32_t h = 2166136261u;
+  h = (h * 16777619u) ^ (key >> 24);
+  h = (h * 16777619u) ^ ((key >> 16) & 0xFFu);
+  h = (h * 16777619u) ^ ((key >> 8) & 0xFFu);
+  h = (h * 16777619u) ^ (key & 0xFFu);
+  return h;
+}
+
+// Flag in a fingerprint width argument: hash with FNV instead of AP.
+constexpr uint32_t FP_FNV = 0x100u;
+
+// Fingerprint when the result fits 32 bits (q + r <= 32), bits = q + r
+// (| FP_FNV): the hash mod 2^bits, as the reference's `hash % tableSize`.
 __host__ __device__ __forceinline__ uint32_t fingerprint32(uint32_t key, unsigned bits) {
-  const uint32_t lo = ap_rounds(key, 0xAAAAAAAAu);
-  return bits >= 32 ? lo : (lo & ((1u << bits) - 1u));
+  const unsigned b = bits & 0xFFu;
+  const uint32_t h = (bits & FP_FNV) ? fnv_rounds(key) : ap_rounds(key, 0xAAAAAAAAu);
+  return b >= 32 ? h : (h & ((1u << b) - 1u));
 }
 
 template <typename K, bool HASH>
@@ -527,12 +544,15 @@ __device__ __forceinline__ uint32_t run_first(const K* __restrict__ A, uint32_t 
 
 template <typename K>
 __global__ __launch_bounds__(DM_THREADS) void k_delete_mark(const K* __restrict__ A, uint32_t na,
+                                                            const uint32_t* __restrict__ na_dev,
                                                             const K* __restrict__ D, uint32_t nd,
                                                             uint8_t* __restrict__ keep) {
   __shared__ K s_d[DM_DCAP];
   __shared__ uint32_t s_rng[2];
   const int tid = threadIdx.x;
+  if (na_dev) na = *na_dev;  // stored count on the device (the grid covers an upper bound)
   const uint32_t t0 = blockIdx.x * (uint32_t)DM_TILE;
+  if (t0 >= na) return;
   const uint32_t t1 = t0 + DM_TILE < na ? t0 + DM_TILE : na;
   if (tid == 0) s_rng[0] = lower_bound_g(D, 0u, nd, A[t0]);
   if (tid == 64) s_rng[1] = upper_bound_g(D, 0u, nd, A[t1 - 1]);

@@ -68,11 +68,14 @@ __device__ __forceinline__ RS rs_wave_excl(RS x, RS* total) {
 }
 
 // Merge-path split (A entries among the first u*RB_TILE outputs) of every sub-tile boundary.
+// na_dev (may be NULL): A's count on the device, overriding na.
 template <typename K>
-__global__ __launch_bounds__(256) void k_rb_splits(const K* __restrict__ A, uint32_t na, const K* __restrict__ B,
+__global__ __launch_bounds__(256) void k_rb_splits(const K* __restrict__ A, uint32_t na,
+                                                   const uint32_t* __restrict__ na_dev, const K* __restrict__ B,
                                                    uint32_t nb, uint32_t nsub, uint32_t* __restrict__ splits) {
   const uint32_t u = blockIdx.x * 256 + threadIdx.x;
   if (u > nsub) return;
+  if (na_dev) na = *na_dev;
   const uint64_t N = (uint64_t)na + nb;
   const uint32_t d = (uint32_t)((uint64_t)u * RB_TILE < N ? (uint64_t)u * RB_TILE : N);
   splits[u] = merge_path(A, na, B, nb, d);
@@ -81,7 +84,7 @@ __global__ __launch_bounds__(256) void k_rb_splits(const K* __restrict__ A, uint
 struct RbPlan {
   uint32_t G, chunk, nsub;  // chunks, entries per chunk (multiple of RB_TILE), sub-tiles
 };
-constexpr uint32_t RB_MAX_CHUNKS = 16384;
+constexpr uint32_t RB_MAX_CHUNKS = 65536;
 static inline RbPlan rb_plan(uint64_t N) {
   // one sub-tile per workgroup (latency is hidden by occupancy, not by a
   // sequential walk), several per chunk only beyond RB_MAX_CHUNKS sub-tiles

@@ -447,12 +447,14 @@ __device__ __forceinline__ uint32_t rr_classify(const K* v, uint32_t fl, uint32_
 
 template <typename K, int MODE, bool MERGE>
 __global__ __launch_bounds__(RB_THREADS, 8) void k_rr_agg(const K* __restrict__ A, uint32_t na,
+                                                          const uint32_t* __restrict__ na_dev,
                                                           const K* __restrict__ B, uint32_t nbk,
                                                           const uint8_t* __restrict__ flags,
                                                           const uint32_t* __restrict__ splits, uint32_t q, uint32_t r,
                                                           uint32_t chunk, uint4* __restrict__ agg) {
   __shared__ K s_stage[MERGE ? RB_TILE : 1];
   __shared__ uint4 s_wave[RB_WAVES + 1];
+  if (na_dev) na = *na_dev;  // stored count on the device (the grid covers an upper bound)
   const uint64_t N = (uint64_t)na + (MERGE ? nbk : 0u);
   const uint32_t mq = (uint32_t)mask64(q);
   const uint64_t c0 = (uint64_t)blockIdx.x * chunk;
@@ -525,6 +527,7 @@ __device__ __forceinline__ void rr_visit(uint32_t fq, uint32_t fr, int32_t lc, i
 template <typename K, int MODE, bool MERGE, int R>
 __global__ __launch_bounds__(RB_THREADS, 6) void k_rr_write(uint8_t* __restrict__ base, uint32_t nb, uint32_t q,
                                                             uint32_t r, const K* __restrict__ A, uint32_t na,
+                                                            const uint32_t* __restrict__ na_dev,
                                                             const K* __restrict__ B, uint32_t nbk,
                                                             const uint8_t* __restrict__ flags,
                                                             const uint32_t* __restrict__ splits, uint32_t chunk,
@@ -542,6 +545,7 @@ __global__ __launch_bounds__(RB_THREADS, 6) void k_rr_write(uint8_t* __restrict_
   const uint32_t rr = R ? (uint32_t)R : r;
   const uint32_t mq = (uint32_t)mask64(q);
   const uint32_t rmask = (1u << rr) - 1u;
+  if (na_dev) na = *na_dev;
   const uint64_t N = (uint64_t)na + (MERGE ? nbk : 0u);
   const uint64_t c0 = (uint64_t)blockIdx.x * chunk;
   const uint64_t c1 = c0 + chunk < N ? c0 + chunk : N;
@@ -699,13 +703,6 @@ static RsqfDev rdev(const gqf_rsqf& f) {
   return d;
 }
 
-static gqf_status read_ctr(hipStream_t s, DeviceState& ds, const uint32_t* dptr, uint32_t* out) {
-  GQF_HIP(hipMemcpyAsync(ds.h_pinned + CTR_MISC + 4, dptr, 4, hipMemcpyDeviceToHost, s));
-  GQF_HIP(hipStreamSynchronize(s));
-  *out = ds.h_pinned[CTR_MISC + 4];
-  return GQF_OK;
-}
-
 static inline uint32_t rd_windows(uint32_t nb) { return (nb + RD_BLOCKS - 1) / RD_BLOCKS; }
 
 static size_t rsqf_decode_ws(const RsqfGeom& g, size_t kb) {
@@ -713,10 +710,10 @@ static size_t rsqf_decode_ws(const RsqfGeom& g, size_t kb) {
   return aligned_bytes((size_t)W * 8) + aligned_bytes((size_t)W * 4) + 256 + aligned_bytes((g.cap + 1) * kb) + 256;
 }
 
-// The stored fingerprints in ascending order (*m_out of them).
+// The stored fingerprints in ascending order; their count is left in device
+// memory (*m_dev).
 template <typename K>
-static gqf_status rsqf_decode(hipStream_t s, DeviceState& ds, Arena& ar, const gqf_rsqf& f, K** out,
-                              uint32_t* m_out) {
+static gqf_status rsqf_decode(hipStream_t s, Arena& ar, const gqf_rsqf& f, K** out, const uint32_t** m_dev) {
   const RsqfDev d = rdev(f);
   const uint32_t W = rd_windows(f.numBlocks);
   uint32_t* wp = ar.take<uint32_t>((size_t)W * 2);
@@ -731,19 +728,22 @@ static gqf_status rsqf_decode(hipStream_t s, DeviceState& ds, Arena& ar, const g
   GQF_LAUNCH(k_scan_rows<2>, dim3(1), dim3(1024), 0, s, wp, W, tot);
   GQF_LAUNCH_B(meta, k_rd_count, dim3(W), dim3(RD_THREADS), 0, s, d, wp, wc);
   GQF_LAUNCH(k_scan_rows<1>, dim3(1), dim3(1024), 0, s, wc, W, tot + 2);
-  GQF_TRY(read_ctr(s, ds, tot + 2, m_out));
-  K* fps = ar.take<K>((size_t)*m_out + 1);
-  if (ar.used > ar.cap || *m_out > f.numBlocks * 64ull) {
+  // the count stays on the device (tot[2]): later kernels read it, the host
+  // plans for the capacity
+  *m_dev = tot + 2;
+  K* fps = ar.take<K>((size_t)f.numBlocks * 64 + 1);
+  if (ar.used > ar.cap) {
     set_error("internal: decode workspace overflow");
     return GQF_ERR_INVALID;
   }
-  const double bytes = (double)rsqf_dev_bytes(f.qbits, f.rbits) - (double)f.numBlocks * 16 + (double)*m_out * sizeof(K);
+  // table read (remainders + run ends) + the entries written (count on the device)
+  const double bytes = (double)rsqf_dev_bytes(f.qbits, f.rbits) - (double)f.numBlocks * 16;
   if (f.rbits == 8)
-    GQF_LAUNCH_B(bytes, (k_rd_emit<K, 8>), dim3(W), dim3(RD_THREADS), 0, s, d, wp, wc, fps);
+    GQF_LAUNCH_BC(bytes, *m_dev, sizeof(K), (k_rd_emit<K, 8>), dim3(W), dim3(RD_THREADS), 0, s, d, wp, wc, fps);
   else if (f.rbits == 16)
-    GQF_LAUNCH_B(bytes, (k_rd_emit<K, 16>), dim3(W), dim3(RD_THREADS), 0, s, d, wp, wc, fps);
+    GQF_LAUNCH_BC(bytes, *m_dev, sizeof(K), (k_rd_emit<K, 16>), dim3(W), dim3(RD_THREADS), 0, s, d, wp, wc, fps);
   else
-    GQF_LAUNCH_B(bytes, (k_rd_emit<K, 0>), dim3(W), dim3(RD_THREADS), 0, s, d, wp, wc, fps);
+    GQF_LAUNCH_BC(bytes, *m_dev, sizeof(K), (k_rd_emit<K, 0>), dim3(W), dim3(RD_THREADS), 0, s, d, wp, wc, fps);
   *out = fps;
   return GQF_OK;
 }
@@ -755,9 +755,12 @@ static size_t rsqf_rebuild_ws(uint64_t N) {
 
 // Canonical layout of A (MERGE: merged with B; KEEP_FLAGS: the flagged
 // entries of A) written over the table.
+// na_dev (may be NULL): A's count on the device; na is then an upper bound
+// that sizes the grids.
 template <typename K, int MODE, bool MERGE>
 static gqf_status rsqf_rebuild(hipStream_t s, DeviceState& ds, Arena& ar, const gqf_rsqf& f, const K* A,
-                               uint32_t na, const K* B, uint32_t nbk, const uint8_t* flags) {
+                               uint32_t na, const uint32_t* na_dev, const K* B, uint32_t nbk,
+                               const uint8_t* flags) {
   const RsqfGeom g = rgeom(f);
   const uint64_t N = (uint64_t)na + (MERGE ? nbk : 0u);
   const size_t tbytes = rsqf_dev_bytes(g.q, g.r);
@@ -775,18 +778,21 @@ static gqf_status rsqf_rebuild(hipStream_t s, DeviceState& ds, Arena& ar, const 
     set_error("internal: rebuild workspace overflow");
     return GQF_ERR_INVALID;
   }
+  const double per = (double)(sizeof(K) + (MODE == KEEP_FLAGS ? 1 : 0));  // bytes read per entry
+  const double nknown = na_dev ? (double)(MERGE ? nbk : 0u) : (double)N;   // entries the host knows of
   if (MERGE)
-    GQF_LAUNCH(k_rb_splits<K>, dim3((p.nsub + 1 + 255) / 256), dim3(256), 0, s, A, na, B, nbk, p.nsub, splits);
-  GQF_LAUNCH_B((double)N * (sizeof(K) + (MODE == KEEP_FLAGS ? 1 : 0)), (k_rr_agg<K, MODE, MERGE>), dim3(p.G),
-               dim3(RB_THREADS), 0, s, A, na, B, nbk, flags, splits, g.q, g.r, p.chunk, agg);
+    GQF_LAUNCH(k_rb_splits<K>, dim3((p.nsub + 1 + 255) / 256), dim3(256), 0, s, A, na, na_dev, B, nbk, p.nsub,
+               splits);
+  GQF_LAUNCH_BC(nknown * per, na_dev, per, (k_rr_agg<K, MODE, MERGE>), dim3(p.G), dim3(RB_THREADS), 0, s, A, na,
+                na_dev, B, nbk, flags, splits, g.q, g.r, p.chunk, agg);
   GQF_LAUNCH(k_rb_scan, dim3(1), dim3(1024), 0, s, agg, p.G, g.cap, ds.counters, total);
   GQF_LAUNCH_B((double)tbytes, k_rr_clear, dim3(1024), dim3(256), 0, s, reinterpret_cast<uint8_t*>(f.blocks),
                (uint64_t)tbytes, ds.counters);
-  const double wbytes = (double)N * (sizeof(K) + (MODE == KEEP_FLAGS ? 1 : 0)) + (double)tbytes;
-#define GQF_RR_WRITE(R_)                                                                                         \
-  GQF_LAUNCH_B(wbytes, (k_rr_write<K, MODE, MERGE, R_>), dim3(p.G), dim3(RB_THREADS), 0, s,                     \
-               reinterpret_cast<uint8_t*>(f.blocks), g.nb, g.q, g.r, A, na, B, nbk, flags, splits, p.chunk, agg, \
-               ds.counters)
+  const double wbytes = nknown * per + (double)tbytes;
+#define GQF_RR_WRITE(R_)                                                                                          \
+  GQF_LAUNCH_BC(wbytes, na_dev, per, (k_rr_write<K, MODE, MERGE, R_>), dim3(p.G), dim3(RB_THREADS), 0, s,        \
+                reinterpret_cast<uint8_t*>(f.blocks), g.nb, g.q, g.r, A, na, na_dev, B, nbk, flags, splits, p.chunk, \
+                agg, ds.counters)
   if (g.r == 8) GQF_RR_WRITE(8);
   else if (g.r == 16) GQF_RR_WRITE(16);
   else GQF_RR_WRITE(0);
@@ -812,7 +818,7 @@ static gqf_status rsqf_build_t(const gqf_rsqf& f, uint32_t n, const uint32_t* ke
   Arena ar(ds.ws, ds.ws_cap);
   K* sorted;
   GQF_TRY(radix_sort<K>(cs.stream, ds, ar, keys, true, g.bits, nullptr, false, n, 0, g.bits, &sorted, nullptr));
-  return rsqf_rebuild<K, KEEP_ALL, false>(cs.stream, ds, ar, f, sorted, n, (const K*)nullptr, 0u, nullptr);
+  return rsqf_rebuild<K, KEEP_ALL, false>(cs.stream, ds, ar, f, sorted, n, nullptr, (const K*)nullptr, 0u, nullptr);
 }
 
 template <typename K>
@@ -827,15 +833,12 @@ static gqf_status rsqf_insert_t(const gqf_rsqf& f, uint32_t n, const uint32_t* k
   GQF_TRY(ws_reserve(ds, rsqf_insert_ws<K>(g, n)));
   Arena ar(ds.ws, ds.ws_cap);
   K* A;
-  uint32_t m = 0;
-  GQF_TRY(rsqf_decode<K>(cs.stream, ds, ar, f, &A, &m));
-  if ((uint64_t)m + n >= GQF_MAX_ITEMS) {
-    set_error("table holds %u entries: %u more exceed the per-call limit", m, n);
-    return GQF_ERR_INVALID;
-  }
+  const uint32_t* m_dev = nullptr;
+  GQF_TRY(rsqf_decode<K>(cs.stream, ar, f, &A, &m_dev));
   K* B;
   GQF_TRY(radix_sort<K>(cs.stream, ds, ar, keys, true, g.bits, nullptr, false, n, 0, g.bits, &B, nullptr));
-  return rsqf_rebuild<K, KEEP_ALL, true>(cs.stream, ds, ar, f, A, m, B, n, nullptr);
+  // at most cap stored entries: the grids are sized for cap + n
+  return rsqf_rebuild<K, KEEP_ALL, true>(cs.stream, ds, ar, f, A, (uint32_t)g.cap, m_dev, B, n, nullptr);
 }
 
 template <typename K>
@@ -851,19 +854,19 @@ static gqf_status rsqf_delete_t(const gqf_rsqf& f, uint32_t n, const uint32_t* k
   GQF_TRY(ws_reserve(ds, rsqf_delete_ws<K>(g, n)));
   Arena ar(ds.ws, ds.ws_cap);
   K* A;
-  uint32_t m = 0;
-  GQF_TRY(rsqf_decode<K>(cs.stream, ds, ar, f, &A, &m));
+  const uint32_t* m_dev = nullptr;
+  GQF_TRY(rsqf_decode<K>(cs.stream, ar, f, &A, &m_dev));
   K* D;
   GQF_TRY(radix_sort<K>(cs.stream, ds, ar, keys, true, g.bits, nullptr, false, n, 0, g.bits, &D, nullptr));
-  uint8_t* keep = ar.take<uint8_t>(m + 1);
+  const uint32_t mcap = (uint32_t)g.cap;  // the grids cover the capacity; kernels read the count
+  uint8_t* keep = ar.take<uint8_t>((size_t)mcap + 1);
   if (ar.used > ar.cap) {
     set_error("internal: delete workspace overflow");
     return GQF_ERR_INVALID;
   }
-  if (m)
-    GQF_LAUNCH_B((double)m * (sizeof(K) + 1), k_delete_mark<K>, dim3((m + DM_TILE - 1) / DM_TILE), dim3(DM_THREADS),
-                 0, cs.stream, A, m, D, n, keep);
-  return rsqf_rebuild<K, KEEP_FLAGS, false>(cs.stream, ds, ar, f, A, m, (const K*)nullptr, 0u, keep);
+  GQF_LAUNCH_BC(0.0, m_dev, (double)(sizeof(K) + 1), k_delete_mark<K>, dim3((mcap + DM_TILE - 1) / DM_TILE),
+                dim3(DM_THREADS), 0, cs.stream, A, mcap, m_dev, D, n, keep);
+  return rsqf_rebuild<K, KEEP_FLAGS, false>(cs.stream, ds, ar, f, A, mcap, m_dev, (const K*)nullptr, 0u, keep);
 }
 
 static void launch_rlookup(hipStream_t s, const gqf_rsqf& f, uint32_t n, const uint32_t* keys, uint32_t* out) {

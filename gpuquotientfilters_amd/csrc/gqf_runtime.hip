@@ -15,6 +15,9 @@
 namespace gqf {
 
 static thread_local char g_err[512];
+static thread_local int t_sqf_hash = GQF_HASH_AP;
+
+uint32_t sqf_hash_flag() { return t_sqf_hash == GQF_HASH_FNV ? 0x100u : 0u; }
 
 void set_error(const char* fmt, ...) {
   va_list ap;
@@ -146,8 +149,10 @@ static hipEvent_t pool_event(DeviceState& ds) {
   return ds.pool[ds.pool_used++];
 }
 
-ProfScope::ProfScope(const char* n, hipStream_t s, double algorithmic_bytes)
-    : name(n), stream(s), on(prof_enabled() && t_ctx != nullptr), bytes(algorithmic_bytes) {
+ProfScope::ProfScope(const char* n, hipStream_t s, double algorithmic_bytes, const uint32_t* dev_count,
+                     double bytes_per_count)
+    : name(n), stream(s), on(prof_enabled() && t_ctx != nullptr), bytes(algorithmic_bytes), dcount(dev_count),
+      dper(bytes_per_count) {
   if (on) {
     a = pool_event(*t_ctx);
     if (a) (void)hipEventRecord(a, stream);
@@ -168,7 +173,7 @@ ProfScope::~ProfScope() {
     ds.pending = p;
     ds.pending_size = n;
   }
-  ds.pending[ds.pending_used++] = {name, a, b, bytes};
+  ds.pending[ds.pending_used++] = {name, a, b, bytes, dcount, dper};
 }
 
 void prof_resolve(DeviceState& ds) {
@@ -178,6 +183,11 @@ void prof_resolve(DeviceState& ds) {
       const ProfRec& r = ds.pending[i];
       float ms = 0.f;
       if (hipEventElapsedTime(&ms, r.a, r.b) != hipSuccess) continue;
+      double bytes = r.bytes;
+      if (r.dcount) {  // the call has completed: its device counts are final
+        uint32_t c = 0;
+        if (hipMemcpy(&c, r.dcount, 4, hipMemcpyDeviceToHost) == hipSuccess) bytes += (double)c * r.dper;
+      }
       std::string nm(r.name);
       while (!nm.empty() && (nm.front() == '(' || nm.front() == ' ')) nm.erase(nm.begin());
       while (!nm.empty() && (nm.back() == ')' || nm.back() == ' ')) nm.pop_back();
@@ -186,11 +196,11 @@ void prof_resolve(DeviceState& ds) {
         if (st.name == nm) {
           st.launches++;
           st.ms += ms;
-          st.bytes += r.bytes;
+          st.bytes += bytes;
           found = true;
           break;
         }
-      if (!found) g_stats.push_back({nm, 1, (double)ms, r.bytes});
+      if (!found) g_stats.push_back({nm, 1, (double)ms, bytes});
     }
   }
   ds.pending_used = 0;
@@ -256,6 +266,17 @@ const char* gqf_status_string(gqf_status s) {
 }
 
 const char* gqf_version(void) { return "gqf 0.1 gfx950"; }
+
+gqf_status gqf_sqf_set_hash(int kind) {
+  if (kind != GQF_HASH_AP && kind != GQF_HASH_FNV) {
+    set_error("unknown hash kind %d (GQF_HASH_AP or GQF_HASH_FNV)", kind);
+    return GQF_ERR_INVALID;
+  }
+  t_sqf_hash = kind;
+  return GQF_OK;
+}
+
+int gqf_sqf_get_hash(void) { return t_sqf_hash; }
 
 gqf_status gqf_device_count(int* count) {
   if (!count) return GQF_ERR_INVALID;

@@ -29,6 +29,10 @@ gqf_status hip_fail(hipError_t e, const char* what, const char* file, int line);
     if (_s != GQF_OK) return _s;        \
   } while (0)
 
+// SQF hash of the calling thread (gqf_sqf_set_hash) as a fingerprint-width
+// flag: 0 (AP) or FP_FNV.
+uint32_t sqf_hash_flag();
+
 // ---------------------------------------------------------------- device state
 // Small device counters used by the kernels (index constants below).
 enum Counter : int {
@@ -47,6 +51,8 @@ struct ProfRec {
   const char* name;
   hipEvent_t a, b;
   double bytes;
+  const uint32_t* dcount;  // device count whose value adds dper bytes each (may be NULL)
+  double dper;
 };
 
 // Execution context of one (device, stream) pair: workspace arena, device
@@ -102,7 +108,10 @@ struct ProfScope {
   hipEvent_t a = nullptr;
   bool on;
   double bytes;
-  ProfScope(const char* n, hipStream_t s, double algorithmic_bytes = 0.0);
+  const uint32_t* dcount;
+  double dper;
+  ProfScope(const char* n, hipStream_t s, double algorithmic_bytes = 0.0, const uint32_t* dev_count = nullptr,
+            double bytes_per_count = 0.0);
   ~ProfScope();
 };
 void prof_resolve(DeviceState& ds);
@@ -119,6 +128,15 @@ void prof_resolve(DeviceState& ds);
   do {                                                                 \
     ::gqf::ProfScope _gqf_ps(#kernel, (stream), (double)(bytes));      \
     hipLaunchKernelGGL(kernel, grid, block, shm, stream, __VA_ARGS__); \
+  } while (0)
+
+// Same, where part of the bytes depends on a count that only the device
+// knows at launch time: bytes + (*dev_count) * per_count, read back when the
+// call's timings are resolved.
+#define GQF_LAUNCH_BC(bytes, dev_count, per_count, kernel, grid, block, shm, stream, ...)       \
+  do {                                                                                        \
+    ::gqf::ProfScope _gqf_ps(#kernel, (stream), (double)(bytes), (dev_count), (double)(per_count)); \
+    hipLaunchKernelGGL(kernel, grid, block, shm, stream, __VA_ARGS__);                        \
   } while (0)
 
 // Timing/blocking scaffold of every entry point: takes the stream's context

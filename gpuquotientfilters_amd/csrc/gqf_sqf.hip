@@ -32,15 +32,14 @@ static SqfGeom geom(const gqf_sqf& qf) {
   return g;
 }
 
-static gqf_status read_counter(hipStream_t s, DeviceState& ds, const uint32_t* dptr, uint32_t* out);
-
-// decode the table into ascending fingerprints; the count (read back to the
-// host) goes to *count
-static gqf_status sqf_decode(hipStream_t s, DeviceState& ds, Arena& ar, const gqf_sqf& qf, const SqfGeom& g,
-                             uint32_t** fps_out, uint32_t* count) {
-  uint32_t* count_dev = ds.counters + CTR_MISC;
+// Decodes the table into ascending fingerprints (at most S of them); their
+// count stays in device memory (*count_dev): later kernels read it, and the
+// host sizes their grids for S.
+static gqf_status sqf_decode(hipStream_t s, Arena& ar, const gqf_sqf& qf, const SqfGeom& g, uint32_t** fps_out,
+                             const uint32_t** count_out) {
   const uint32_t regions = (g.S + DC_SLOTS - 1) / DC_SLOTS;
   uint32_t* counts = ar.take<uint32_t>((size_t)regions * 3);
+  uint32_t* count_dev = ar.take<uint32_t>(3);  // column totals; [0] = entries
   uint32_t* fps = ar.take<uint32_t>(g.S);
   if (ar.used > ar.cap) {
     set_error("internal: decode workspace overflow");
@@ -52,21 +51,22 @@ static gqf_status sqf_decode(hipStream_t s, DeviceState& ds, Arena& ar, const gq
   else GQF_LAUNCH_B((double)g.bytes, k_sqf_count<4>, dim3(regions), dim3(256), 0, s, qf.table, g.S, counts);
   // totals: column 0 (entries) is the decoded count
   GQF_LAUNCH(k_scan_rows<3>, dim3(1), dim3(1024), 0, s, counts, regions, count_dev);
-  GQF_TRY(read_counter(s, ds, count_dev, count));
-  const double dec_bytes = (double)g.bytes + 4.0 * (double)*count;
+  // table read + 4 B per entry written (the count is on the device)
+  const double tb = (double)g.bytes;
   if (g.bpe == 1)
-    GQF_LAUNCH_B(dec_bytes, (k_sqf_decode<1, uint32_t>), dim3(regions), dim3(256), 0, s, qf.table, g.S, g.r, counts, fps);
+    GQF_LAUNCH_BC(tb, count_dev, 4, (k_sqf_decode<1, uint32_t>), dim3(regions), dim3(256), 0, s, qf.table, g.S, g.r, counts, fps);
   else if (g.bpe == 2)
-    GQF_LAUNCH_B(dec_bytes, (k_sqf_decode<2, uint32_t>), dim3(regions), dim3(256), 0, s, qf.table, g.S, g.r, counts, fps);
+    GQF_LAUNCH_BC(tb, count_dev, 4, (k_sqf_decode<2, uint32_t>), dim3(regions), dim3(256), 0, s, qf.table, g.S, g.r, counts, fps);
   else if (g.bpe == 3)
-    GQF_LAUNCH_B(dec_bytes, (k_sqf_decode<3, uint32_t>), dim3(regions), dim3(256), 0, s, qf.table, g.S, g.r, counts, fps);
+    GQF_LAUNCH_BC(tb, count_dev, 4, (k_sqf_decode<3, uint32_t>), dim3(regions), dim3(256), 0, s, qf.table, g.S, g.r, counts, fps);
   else
-    GQF_LAUNCH_B(dec_bytes, (k_sqf_decode<4, uint32_t>), dim3(regions), dim3(256), 0, s, qf.table, g.S, g.r, counts, fps);
+    GQF_LAUNCH_BC(tb, count_dev, 4, (k_sqf_decode<4, uint32_t>), dim3(regions), dim3(256), 0, s, qf.table, g.S, g.r, counts, fps);
   *fps_out = fps;
+  *count_out = count_dev;
   return GQF_OK;
 }
 static size_t sqf_decode_bytes(const SqfGeom& g) {
-  return aligned_bytes((size_t)((g.S + DC_SLOTS - 1) / DC_SLOTS) * 12) + aligned_bytes((size_t)g.S * 4) + 512;
+  return aligned_bytes((size_t)((g.S + DC_SLOTS - 1) / DC_SLOTS) * 12) + 256 + aligned_bytes((size_t)g.S * 4) + 512;
 }
 
 // Reads a device counter into host memory (small synchronous readback, used
@@ -90,10 +90,12 @@ static size_t rebuild_ws_bytes(uint64_t N) {
 }
 
 // Fused rebuild of qf from A (and B when merging) under a keep rule.
+// na_dev (may be NULL): A's count on the device; na is then an upper bound
+// that sizes the grids, and the sparse-table choice is made on the device.
 template <int MODE, bool MERGE>
 static gqf_status launch_rebuild(hipStream_t s, DeviceState& ds, Arena& ar, const gqf_sqf& qf, const SqfGeom& g,
-                                 const uint32_t* A, uint32_t na, const uint32_t* B, uint32_t nb,
-                                 const uint8_t* flags) {
+                                 const uint32_t* A, uint32_t na, const uint32_t* na_dev, const uint32_t* B,
+                                 uint32_t nb, const uint8_t* flags) {
   const uint64_t N = (uint64_t)na + (MERGE ? nb : 0u);
   GQF_LAUNCH(k_reset_counters, dim3(1), dim3(64), 0, s, ds.counters, (int)CTR_OP, (int)CTR_TOTAL);
   if (N == 0) {
@@ -110,26 +112,36 @@ static gqf_status launch_rebuild(hipStream_t s, DeviceState& ds, Arena& ar, cons
     set_error("internal: rebuild workspace overflow");
     return GQF_ERR_INVALID;
   }
+  const uint32_t nadd = MERGE ? nb : 0u;
+  const double nknown = na_dev ? (double)nadd : (double)N;  // entries the host knows of; the rest: 4 B each on the device
   if (MERGE)
-    GQF_LAUNCH(k_rb_splits<uint32_t>, dim3((p.nsub + 1 + 255) / 256), dim3(256), 0, s, A, na, B, nb, p.nsub, splits);
-  GQF_LAUNCH_B((double)N * 4, (k_rb_agg<MODE, MERGE>), dim3(p.G), dim3(RB_THREADS), 0, s, A, na, B, nb, flags, splits,
-               g.q, g.r, p.chunk, agg);
+    GQF_LAUNCH(k_rb_splits<uint32_t>, dim3((p.nsub + 1 + 255) / 256), dim3(256), 0, s, A, na, na_dev, B, nb, p.nsub,
+               splits);
+  GQF_LAUNCH_BC(nknown * 4, na_dev, 4, (k_rb_agg<MODE, MERGE>), dim3(p.G), dim3(RB_THREADS), 0, s, A, na, na_dev, B,
+                nb, flags, splits, g.q, g.r, p.chunk, agg);
   GQF_LAUNCH(k_rb_scan, dim3(1), dim3(1024), 0, s, agg, p.G, (uint64_t)g.S, ds.counters, (uint4*)nullptr);
-  // sparse tables: clear once, then sub-tiles skip windows holding only zeros
-  const int precleared = N * 8 < (uint64_t)g.S ? 1 : 0;
-  if (precleared) GQF_LAUNCH(k_sqf_clear_tail, dim3(1024), dim3(256), 0, s, qf.table, g.bpe, g.bytes, ds.counters, 1);
-#define GQF_RB_WRITE(BPE_)                                                                                       \
-  GQF_LAUNCH_B((double)N * 4 + (double)g.bytes, (k_rb_write<MODE, MERGE, BPE_>), dim3(p.G), dim3(RB_THREADS), 0, s, \
-               qf.table, g.S, g.q, g.r, A, na, B, nb, flags, splits, p.chunk, agg, ds.counters, fix,  \
-               fix_cnt, fix_more, ds.counters + CTR_MISC + 2, (uint32_t)(N + 1), precleared)
+  // sparse tables (N * 8 < S): clear once, then sub-tiles skip windows holding
+  // only zeros; otherwise sub-tiles write every slot of their range and the
+  // tail is cleared after.  With the count on the device, the kernels decide.
+  const int precleared = na_dev ? -1 : (N * 8 < (uint64_t)g.S ? 1 : 0);
+  const int pre_mode = na_dev ? 2 : 1, post_mode = na_dev ? 3 : 0;
+  if (precleared != 0)
+    GQF_LAUNCH(k_sqf_clear_tail, dim3(1024), dim3(256), 0, s, qf.table, g.bpe, g.bytes, ds.counters, pre_mode, na_dev,
+               nadd, g.S);
+#define GQF_RB_WRITE(BPE_)                                                                                          \
+  GQF_LAUNCH_BC(nknown * 4 + (double)g.bytes, na_dev, 4, (k_rb_write<MODE, MERGE, BPE_>), dim3(p.G),               \
+                dim3(RB_THREADS), 0, s, qf.table, g.S, g.q, g.r, A, na, na_dev, B, nb, flags, splits, p.chunk, agg, \
+                ds.counters, fix, fix_cnt, fix_more, ds.counters + CTR_MISC + 2, (uint32_t)(N + 1), precleared)
   if (g.bpe == 1) GQF_RB_WRITE(1);
   else if (g.bpe == 2) GQF_RB_WRITE(2);
   else if (g.bpe == 3) GQF_RB_WRITE(3);
   else GQF_RB_WRITE(4);
 #undef GQF_RB_WRITE
-  if (!precleared) GQF_LAUNCH(k_sqf_clear_tail, dim3(1024), dim3(256), 0, s, qf.table, g.bpe, g.bytes, ds.counters, 0);
+  if (precleared != 1)
+    GQF_LAUNCH(k_sqf_clear_tail, dim3(1024), dim3(256), 0, s, qf.table, g.bpe, g.bytes, ds.counters, post_mode, na_dev,
+               nadd, g.S);
   GQF_LAUNCH(k_sqf_fixup, dim3(p.nsub + 1), dim3(64), 0, s, qf.table, g.bpe, fix, fix_cnt, fix_more, ds.counters,
-             p.nsub, (uint32_t)(N + 1));
+             p.nsub, (uint32_t)(N + 1), na_dev, nadd);
   return GQF_OK;
 }
 
@@ -153,18 +165,18 @@ static gqf_status sqf_build_impl(const gqf_sqf& qf, int n, const uint32_t* d_key
   GQF_TRY(ws_reserve(ds, sort_workspace_bytes(un, 4, false) + rebuild_ws_bytes(un)));
   Arena ar(ds.ws, ds.ws_cap);
   uint32_t* sorted = nullptr;
-  GQF_TRY(radix_sort<uint32_t>(cs.stream, ds, ar, d_keys, true, g.q + g.r, nullptr, false, un, 0, g.q + g.r, &sorted,
+  GQF_TRY(radix_sort<uint32_t>(cs.stream, ds, ar, d_keys, true, (g.q + g.r) | sqf_hash_flag(), nullptr, false, un, 0, g.q + g.r, &sorted,
                                nullptr));
-  if (no_dup) GQF_TRY((launch_rebuild<KEEP_UNIQUE, false>(cs.stream, ds, ar, qf, g, sorted, un, nullptr, 0, nullptr)));
-  else GQF_TRY((launch_rebuild<KEEP_ALL, false>(cs.stream, ds, ar, qf, g, sorted, un, nullptr, 0, nullptr)));
+  if (no_dup) GQF_TRY((launch_rebuild<KEEP_UNIQUE, false>(cs.stream, ds, ar, qf, g, sorted, un, nullptr, nullptr, 0, nullptr)));
+  else GQF_TRY((launch_rebuild<KEEP_ALL, false>(cs.stream, ds, ar, qf, g, sorted, un, nullptr, nullptr, 0, nullptr)));
   return cs.end(ms, true);  // a layout that does not fit is detected before any write
 }
 
 // insert / insertViaMerge / merge: stored entries A merged with sorted B.
 template <int MODE>
 static gqf_status sqf_merge_into(CallScope& cs, Arena& ar, const gqf_sqf& qf, const SqfGeom& g, const uint32_t* A,
-                                 uint32_t m, const uint32_t* B, uint32_t k, float* ms) {
-  GQF_TRY((launch_rebuild<MODE, true>(cs.stream, *cs.ds, ar, qf, g, A, m, B, k, nullptr)));
+                                 uint32_t m, const uint32_t* m_dev, const uint32_t* B, uint32_t k, float* ms) {
+  GQF_TRY((launch_rebuild<MODE, true>(cs.stream, *cs.ds, ar, qf, g, A, m, m_dev, B, k, nullptr)));
   return cs.end(ms, true);  // a layout that does not fit is detected before any write
 }
 
@@ -181,14 +193,15 @@ static gqf_status sqf_insert_impl(const gqf_sqf& qf, int n, const uint32_t* d_ke
                              rebuild_ws_bytes((uint64_t)g.S + un)));
   Arena ar(ds.ws, ds.ws_cap);
   uint32_t* B = nullptr;
-  GQF_TRY(radix_sort<uint32_t>(cs.stream, ds, ar, d_keys, true, g.q + g.r, nullptr, false, un, 0, g.q + g.r, &B,
+  GQF_TRY(radix_sort<uint32_t>(cs.stream, ds, ar, d_keys, true, (g.q + g.r) | sqf_hash_flag(), nullptr, false, un, 0, g.q + g.r, &B,
                                nullptr));
   uint32_t* A;
-  uint32_t m = 0;
-  GQF_TRY(sqf_decode(cs.stream, ds, ar, qf, g, &A, &m));
-  if (mode == KEEP_TAGGED_INSERT) return sqf_merge_into<KEEP_TAGGED_INSERT>(cs, ar, qf, g, A, m, B, un, ms);
-  if (mode == KEEP_UNIQUE) return sqf_merge_into<KEEP_UNIQUE>(cs, ar, qf, g, A, m, B, un, ms);
-  return sqf_merge_into<KEEP_ALL>(cs, ar, qf, g, A, m, B, un, ms);
+  const uint32_t* m_dev = nullptr;
+  GQF_TRY(sqf_decode(cs.stream, ar, qf, g, &A, &m_dev));
+  // at most S stored entries: grids sized for S + n, the count read on the device
+  if (mode == KEEP_TAGGED_INSERT) return sqf_merge_into<KEEP_TAGGED_INSERT>(cs, ar, qf, g, A, g.S, m_dev, B, un, ms);
+  if (mode == KEEP_UNIQUE) return sqf_merge_into<KEEP_UNIQUE>(cs, ar, qf, g, A, g.S, m_dev, B, un, ms);
+  return sqf_merge_into<KEEP_ALL>(cs, ar, qf, g, A, g.S, m_dev, B, un, ms);
 }
 
 static gqf_status sqf_delete_impl(const gqf_sqf& qf, int n, const uint32_t* d_keys, gqf_stream stream, float* ms) {
@@ -203,18 +216,19 @@ static gqf_status sqf_delete_impl(const gqf_sqf& qf, int n, const uint32_t* d_ke
                              rebuild_ws_bytes(g.S)));
   Arena ar(ds.ws, ds.ws_cap);
   uint32_t* D = nullptr;
-  GQF_TRY(radix_sort<uint32_t>(cs.stream, ds, ar, d_keys, true, g.q + g.r, nullptr, false, un, 0, g.q + g.r, &D,
+  GQF_TRY(radix_sort<uint32_t>(cs.stream, ds, ar, d_keys, true, (g.q + g.r) | sqf_hash_flag(), nullptr, false, un, 0, g.q + g.r, &D,
                                nullptr));
   uint32_t* A;
-  uint32_t m = 0;
-  GQF_TRY(sqf_decode(cs.stream, ds, ar, qf, g, &A, &m));
-  uint8_t* keep = ar.take<uint8_t>(m + 1);
+  const uint32_t* m_dev = nullptr;
+  GQF_TRY(sqf_decode(cs.stream, ar, qf, g, &A, &m_dev));
+  uint8_t* keep = ar.take<uint8_t>((size_t)g.S + 1);
   if (ar.used > ar.cap) {
     set_error("internal: delete workspace overflow");
     return GQF_ERR_INVALID;
   }
-  if (m) GQF_LAUNCH_B((double)m * 5, k_delete_mark<uint32_t>, dim3((m + DM_TILE - 1) / DM_TILE), dim3(DM_THREADS), 0, cs.stream, A, m, D, un, keep);
-  GQF_TRY((launch_rebuild<KEEP_FLAGS, false>(cs.stream, ds, ar, qf, g, A, m, nullptr, 0, keep)));
+  GQF_LAUNCH_BC(0.0, m_dev, 5, k_delete_mark<uint32_t>, dim3((g.S + DM_TILE - 1) / DM_TILE), dim3(DM_THREADS), 0,
+                cs.stream, A, g.S, m_dev, D, un, keep);
+  GQF_TRY((launch_rebuild<KEEP_FLAGS, false>(cs.stream, ds, ar, qf, g, A, g.S, m_dev, nullptr, 0, keep)));
   return cs.end(ms, true);  // deleting only shrinks the layout: it always fits
 }
 
@@ -294,12 +308,12 @@ gqf_status gqf_sqf_lookup(gqf_sqf qf, int sorted, int n, const uint32_t* d_keys,
     if (!sorted) {
       const dim3 grid((un + 255) / 256);
       const double b = (double)un * (8 + 64);
-      if (g.bpe == 1 && win) GQF_LAUNCH_B(b, (k_sqf_lookup<1, true>), grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, d_keys, un, d_out);
-      else if (g.bpe == 2 && win) GQF_LAUNCH_B(b, (k_sqf_lookup<2, true>), grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, d_keys, un, d_out);
-      else if (g.bpe == 1) GQF_LAUNCH_B(b, (k_sqf_lookup<1, false>), grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, d_keys, un, d_out);
-      else if (g.bpe == 2) GQF_LAUNCH_B(b, (k_sqf_lookup<2, false>), grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, d_keys, un, d_out);
-      else if (g.bpe == 3) GQF_LAUNCH_B(b, (k_sqf_lookup<3, false>), grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, d_keys, un, d_out);
-      else GQF_LAUNCH_B(b, (k_sqf_lookup<4, false>), grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, d_keys, un, d_out);
+      if (g.bpe == 1 && win) GQF_LAUNCH_B(b, (k_sqf_lookup<1, true>), grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, d_keys, un, d_out, sqf_hash_flag());
+      else if (g.bpe == 2 && win) GQF_LAUNCH_B(b, (k_sqf_lookup<2, true>), grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, d_keys, un, d_out, sqf_hash_flag());
+      else if (g.bpe == 1) GQF_LAUNCH_B(b, (k_sqf_lookup<1, false>), grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, d_keys, un, d_out, sqf_hash_flag());
+      else if (g.bpe == 2) GQF_LAUNCH_B(b, (k_sqf_lookup<2, false>), grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, d_keys, un, d_out, sqf_hash_flag());
+      else if (g.bpe == 3) GQF_LAUNCH_B(b, (k_sqf_lookup<3, false>), grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, d_keys, un, d_out, sqf_hash_flag());
+      else GQF_LAUNCH_B(b, (k_sqf_lookup<4, false>), grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, d_keys, un, d_out, sqf_hash_flag());
     } else {
       GQF_TRY(ws_reserve(ds, sort_workspace_bytes(un, 4, true)));
       Arena ar(ds.ws, ds.ws_cap);
@@ -307,7 +321,7 @@ gqf_status gqf_sqf_lookup(gqf_sqf qf, int sorted, int n, const uint32_t* d_keys,
       // order by the top 16 fingerprint bits: lookups then walk the table in order
       const uint32_t bits = g.q + g.r;
       const uint32_t lo = bits > 16 ? bits - 16 : 0;
-      GQF_TRY(radix_sort<uint32_t>(cs.stream, ds, ar, d_keys, true, bits, nullptr, true, un, lo, bits, &fps, &idx));
+      GQF_TRY(radix_sort<uint32_t>(cs.stream, ds, ar, d_keys, true, bits | sqf_hash_flag(), nullptr, true, un, lo, bits, &fps, &idx));
       const dim3 grid((un + 255) / 256);
       const double b = (double)un * (12 + 64);
       if (g.bpe == 1 && win) GQF_LAUNCH_B(b, (k_sqf_lookup_sorted<1, true>), grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, fps, idx, un, d_out);
@@ -335,11 +349,13 @@ gqf_status gqf_sqf_merge(gqf_sqf dst, gqf_sqf src, int no_dup, gqf_stream stream
   GQF_TRY(ws_reserve(ds, 2 * sqf_decode_bytes(g) + rebuild_ws_bytes(2 * (uint64_t)g.S)));
   Arena ar(ds.ws, ds.ws_cap);
   uint32_t *A, *B;
-  uint32_t m1 = 0, m2 = 0;
-  GQF_TRY(sqf_decode(cs.stream, ds, ar, dst, g, &A, &m1));
-  GQF_TRY(sqf_decode(cs.stream, ds, ar, src, g, &B, &m2));
-  if (no_dup) return sqf_merge_into<KEEP_UNIQUE>(cs, ar, dst, g, A, m1, B, m2, ms);
-  return sqf_merge_into<KEEP_ALL>(cs, ar, dst, g, A, m1, B, m2, ms);
+  const uint32_t *m1_dev = nullptr, *m2_dev = nullptr;
+  GQF_TRY(sqf_decode(cs.stream, ar, dst, g, &A, &m1_dev));
+  GQF_TRY(sqf_decode(cs.stream, ar, src, g, &B, &m2_dev));
+  uint32_t m2 = 0;  // the second operand's count sizes the merge on the host
+  GQF_TRY(read_counter(cs.stream, ds, m2_dev, &m2));
+  if (no_dup) return sqf_merge_into<KEEP_UNIQUE>(cs, ar, dst, g, A, g.S, m1_dev, B, m2, ms);
+  return sqf_merge_into<KEEP_ALL>(cs, ar, dst, g, A, g.S, m1_dev, B, m2, ms);
 }
 
 }  // extern "C"

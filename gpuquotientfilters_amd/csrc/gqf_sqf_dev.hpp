@@ -237,10 +237,10 @@ __device__ __forceinline__ uint32_t sqf_lookup_fp(const uint8_t* __restrict__ t,
 template <int BPE, bool WIN>
 __global__ __launch_bounds__(256) void k_sqf_lookup(const uint8_t* __restrict__ t, uint32_t S, uint32_t q, uint32_t r,
                                                     const uint32_t* __restrict__ keys, uint32_t n,
-                                                    uint32_t* __restrict__ out) {
+                                                    uint32_t* __restrict__ out, uint32_t hflag) {
   const uint64_t i = blockIdx.x * 256ull + threadIdx.x;
   if (i >= n) return;
-  const uint32_t fp = (uint32_t)fingerprint(keys[i], q + r);
+  const uint32_t fp = fingerprint32(keys[i], q + r + hflag);
   out[i] = sqf_lookup_fp<BPE, WIN>(t, S, (fp >> r) & (uint32_t)mask64(q), fp & ((1u << r) - 1u));
 }
 
@@ -407,12 +407,14 @@ __device__ __forceinline__ uint32_t rb_classify(const uint32_t* v, uint32_t tags
 
 template <int MODE, bool MERGE>
 __global__ __launch_bounds__(RB_THREADS, 8) void k_rb_agg(const uint32_t* __restrict__ A, uint32_t na,
+                                                const uint32_t* __restrict__ na_dev,
                                                 const uint32_t* __restrict__ B, uint32_t nb,
                                                 const uint8_t* __restrict__ flags, const uint32_t* __restrict__ splits,
                                                 uint32_t q, uint32_t r, uint32_t chunk,
                                                 uint4* __restrict__ agg) {
   __shared__ uint32_t s_stage[MERGE ? RB_TILE : 1];
   __shared__ uint4 s_wave[RB_WAVES + 1];
+  if (na_dev) na = *na_dev;
   const uint64_t N = (uint64_t)na + (MERGE ? nb : 0u);
   const uint64_t mq = mask64(q);
   const uint64_t c0 = (uint64_t)blockIdx.x * chunk;
@@ -596,6 +598,7 @@ __device__ __forceinline__ RS rb_emit(uint8_t* __restrict__ t, uint32_t q, uint3
 template <int MODE, bool MERGE, int BPE>
 __global__ __launch_bounds__(RB_THREADS, 6) void k_rb_write(uint8_t* __restrict__ t, uint32_t S, uint32_t q, uint32_t r,
                                                   const uint32_t* __restrict__ A, uint32_t na,
+                                                  const uint32_t* __restrict__ na_dev,
                                                   const uint32_t* __restrict__ B, uint32_t nb,
                                                   const uint8_t* __restrict__ flags,
                                                   const uint32_t* __restrict__ splits, uint32_t chunk,
@@ -609,7 +612,9 @@ __global__ __launch_bounds__(RB_THREADS, 6) void k_rb_write(uint8_t* __restrict_
   __shared__ __align__(16) uint8_t s_win[RB_WIN];
   __shared__ uint4 s_wave[RB_WAVES + 1];
   __shared__ uint32_t s_nfix;
+  if (na_dev) na = *na_dev;
   const uint64_t N = (uint64_t)na + (MERGE ? nb : 0u);
+  if (precleared < 0) precleared = N * 8 < (uint64_t)S ? 1 : 0;  // decided from the count on the device
   const uint64_t mq = mask64(q);
   const uint64_t c0 = (uint64_t)blockIdx.x * chunk;
   const uint64_t c1 = c0 + chunk < N ? c0 + chunk : N;
@@ -631,9 +636,20 @@ __global__ __launch_bounds__(RB_THREADS, 6) void k_rb_write(uint8_t* __restrict_
 // the allocation (the 10% overflow area, usually), in parallel.
 // (whole_table: every byte, used to pre-clear sparse tables).  Nothing is
 // cleared after a capacity error.
+// mode 0: the tail; 1: the whole table; 2: the whole table iff the rebuild
+// is sparse (N * 8 < S, then the writer skips windows of zeros); 3: the tail
+// iff it is not.  N = *n_dev (when given) + n_add.
 [[maybe_unused]] static __global__ __launch_bounds__(256) void k_sqf_clear_tail(uint8_t* __restrict__ t, uint32_t bpe, uint64_t tbytes,
-                                                        const uint32_t* __restrict__ ctr, int whole_table) {
+                                                        const uint32_t* __restrict__ ctr, int mode,
+                                                        const uint32_t* __restrict__ n_dev, uint32_t n_add,
+                                                        uint32_t S) {
   if (ctr[CTR_ERR]) return;  // capacity error: the table stays as it was
+  if (mode >= 2) {
+    const uint64_t N = (uint64_t)(n_dev ? *n_dev : 0u) + n_add;
+    const bool sparse = N * 8 < (uint64_t)S;
+    if (sparse != (mode == 2)) return;
+  }
+  const int whole_table = mode == 1 || mode == 2;
   const uint64_t b0 = whole_table ? 0ull : (uint64_t)ctr[CTR_MISC + 3] * bpe;
   const uint64_t a0 = (b0 + 15) & ~15ull, a1 = tbytes & ~15ull;
   const uint64_t tid = blockIdx.x * 256ull + threadIdx.x, nth = (uint64_t)gridDim.x * 256;
@@ -661,9 +677,15 @@ __device__ __forceinline__ void sqf_set_occupied(uint8_t* t, uint32_t bpe, uint3
                                                   const uint32_t* __restrict__ fix_cnt,
                                                   const uint32_t* __restrict__ fix_more,
                                                   const uint32_t* __restrict__ ctr, uint32_t tiles,
-                                                  uint32_t fix_cap) {
+                                                  uint32_t fix_cap, const uint32_t* __restrict__ n_dev,
+                                                  uint32_t n_add) {
   if (ctr[CTR_ERR]) return;  // capacity error: the table stays as it was
   const uint32_t tile = blockIdx.x;
+  if (n_dev) {  // the grid covers an upper bound: sub-tiles past the entries have no list
+    const uint64_t N = (uint64_t)*n_dev + n_add;
+    const uint64_t used = (N + RB_TILE - 1) / RB_TILE;
+    if (tile < tiles && tile >= used) return;
+  }
   if (tile < tiles) {
     const uint32_t c = fix_cnt[tile];
     for (uint32_t i = threadIdx.x; i < c; i += 64) sqf_set_occupied(t, bpe, fix[(uint64_t)tile * RB_FIX + i]);

@@ -63,6 +63,16 @@ typedef struct gqf_sqf {
   unsigned char* table;
 } gqf_sqf;
 
+/* Hash of the SQF calls made from the calling thread: AP (Normal_APHash,
+ * quotientFilter.cu:134-149; the reference GPU code's choice, default) or FNV
+ * (FNVhashGPU, quotientFilter.cu:117-132, present but unused there; the
+ * reference CPU_version's hash, so FNV tables equal the CPU_version's).  A
+ * filter must be built, updated and queried with one hash.  Batches take the
+ * hash current at gqf_sqf_batch_create.  The RSQF always uses AP. */
+enum gqf_hash { GQF_HASH_AP = 0, GQF_HASH_FNV = 1 };
+gqf_status gqf_sqf_set_hash(int kind);
+int gqf_sqf_get_hash(void);
+
 enum gqf_sqf_build_method {
   GQF_BUILD_SEGMENTED_LAYOUTS = 0,  /* bulkBuildSegmentedLayouts (quotientFilter.cu:1446) */
   GQF_BUILD_PARALLEL_MERGING = 1,   /* bulkBuildParallelMerging  (quotientFilter.cu:737)  */
@@ -312,6 +322,13 @@ typedef struct gqf_serve_executor {
 gqf_status gqf_serve_run(int nfilters, size_t resident_bytes, gqf_request* reqs, int nreqs,
                          const gqf_serve_config* cfg, const gqf_serve_executor* ex,
                          gqf_serve_stats* stats);
+
+/* Random-access peak (no reference counterpart): n independent uniform-random
+ * reads of aligned 64-byte sectors from a device buffer of buffer_bytes (one
+ * sector per thread, plus a 4-byte index read and 4-byte result write per
+ * thread, as a lookup's key and answer); best device time of reps timed runs
+ * after one warm-up.  Lookups are measured against n / best_ms sectors/s. */
+gqf_status gqf_gather_bench(size_t buffer_bytes, uint32_t n, int reps, float* best_ms);
 
 /* ============================================================ kernel timing
  * Per-kernel device time measured with HIP events on the launch stream (two

@@ -63,3 +63,14 @@ def test_invalid_geometry_rejected_without_gpu():
     assert G.lib().gqf_sqf_init(C.byref(h), 10, 6) == G.ERR_INVALID
     h2 = G._RSQF()
     assert G.lib().gqf_rsqf_init(C.byref(h2), 10, 40) == G.ERR_INVALID
+
+
+def test_sqf_hash_selector():
+    import gpuquotientfilters_amd as G
+    assert G.sqf_hash() == G.HASH_AP
+    G.set_sqf_hash(G.HASH_FNV)
+    assert G.sqf_hash() == G.HASH_FNV
+    G.set_sqf_hash(G.HASH_AP)
+    with __import__("pytest").raises(G.GQFError):
+        G.set_sqf_hash(7)
+    assert G.sqf_hash() == G.HASH_AP
