@@ -50,6 +50,7 @@ class _RSQF(C.Structure):
 
 OP_LOOKUP, OP_INSERT, OP_DELETE = 0, 1, 2
 HASH_AP, HASH_FNV = 0, 1
+IMAGE_SQF_GPU, IMAGE_SQF_CPU, IMAGE_RSQF = 1, 2, 3
 
 
 class Request(C.Structure):
@@ -114,6 +115,13 @@ _SIGS = {
     "gqf_memcpy_dtod": (_st, [_vp, _vp, C.c_size_t]),
     "gqf_memset": (_st, [_vp, C.c_int, C.c_size_t]),
     "gqf_synchronize": (_st, []),
+    "gqf_image_bytes": (C.c_size_t, [C.c_int, C.c_uint, C.c_uint]),
+    "gqf_image_info": (_st, [_vp, C.c_size_t, C.POINTER(C.c_int), C.POINTER(C.c_uint), C.POINTER(C.c_uint),
+                             C.POINTER(C.c_int)]),
+    "gqf_sqf_image_save": (_st, [_SQF, C.c_int, _vp, C.c_size_t]),
+    "gqf_sqf_image_load": (_st, [_SQF, _vp, C.c_size_t]),
+    "gqf_rsqf_image_save": (_st, [_RSQF, _vp, C.c_size_t]),
+    "gqf_rsqf_image_load": (_st, [_RSQF, _vp, C.c_size_t]),
     "gqf_sqf_set_hash": (_st, [C.c_int]),
     "gqf_sqf_get_hash": (C.c_int, []),
     "gqf_stream_create": (_st, [C.POINTER(_vp)]),
@@ -550,3 +558,32 @@ class SQFBatch:
             self.destroy()
         except Exception:
             pass
+
+
+def image_info(image):
+    """(layout, q, r, hash) of a table image (gqf_image_info)."""
+    b = np.frombuffer(bytes(image), dtype=np.uint8)
+    lay, hs = C.c_int(), C.c_int()
+    q, r = C.c_uint(), C.c_uint()
+    check(lib().gqf_image_info(b.ctypes.data, b.size, C.byref(lay), C.byref(q), C.byref(r), C.byref(hs)))
+    return lay.value, q.value, r.value, hs.value
+
+
+def image_save(f, layout=None):
+    """Versioned image of an SQF (layout IMAGE_SQF_GPU or IMAGE_SQF_CPU) or RSQF table, as bytes."""
+    if isinstance(f, RSQF):
+        out = np.empty(lib().gqf_image_bytes(IMAGE_RSQF, f.q, f.r), np.uint8)
+        check(lib().gqf_rsqf_image_save(f.h, out.ctypes.data, out.size))
+    else:
+        layout = IMAGE_SQF_GPU if layout is None else layout
+        out = np.empty(lib().gqf_image_bytes(layout, f.q, f.r), np.uint8)
+        check(lib().gqf_sqf_image_save(f.h, layout, out.ctypes.data, out.size))
+    return out.tobytes()
+
+
+def image_load(f, image):
+    b = np.frombuffer(bytes(image), dtype=np.uint8)
+    if isinstance(f, RSQF):
+        check(lib().gqf_rsqf_image_load(f.h, b.ctypes.data, b.size))
+    else:
+        check(lib().gqf_sqf_image_load(f.h, b.ctypes.data, b.size))

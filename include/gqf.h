@@ -183,6 +183,32 @@ size_t gqf_rsqf_workspace_bytes(gqf_rsqf cqf, int op, int n, int sorted);
 gqf_status gqf_rsqf_dump(gqf_rsqf cqf, void* host_ref_layout);
 gqf_status gqf_rsqf_load(gqf_rsqf cqf, const void* host_ref_layout);
 
+/* ============================================================ table images
+ * A versioned byte image of a table, for files, wire transfers, fixtures and
+ * snapshots (no reference counterpart: the reference only prints tables,
+ * quotientFilter.cu:383-405, RSQF.cu:127-172).  A 64-byte header (magic
+ * "GQFTABLE", version 1, layout, q, r, hash, payload size, FNV-1a 64 of the
+ * payload, slots or blocks; little-endian) followed by one of the reference
+ * layouts bit for bit:
+ *   GQF_IMAGE_SQF_GPU  the GPU table bytes (quotientFilter.cu:151-170);
+ *   GQF_IMAGE_SQF_CPU  the CPU_version's (r+3)-bit elements packed MSB-first
+ *                      in u32 words (quotientFilter.cpp:114-130); it holds a
+ *                      few slots fewer than the GPU table, saving a table that
+ *                      uses them fails with GQF_ERR_CAPACITY;
+ *   GQF_IMAGE_RSQF     packed reference blocks (RSQF.cuh:24-30).
+ * An SQF image records the calling thread's SQF hash at save time.  Loading
+ * checks magic, version, geometry and checksum before touching the device. */
+enum gqf_image_layout { GQF_IMAGE_SQF_GPU = 1, GQF_IMAGE_SQF_CPU = 2, GQF_IMAGE_RSQF = 3 };
+/* Image size in bytes (0 for an unknown layout). */
+size_t gqf_image_bytes(int layout, unsigned int q, unsigned int r);
+gqf_status gqf_image_info(const void* image, size_t bytes, int* layout, unsigned int* q, unsigned int* r,
+                          int* hash);
+gqf_status gqf_sqf_image_save(gqf_sqf qf, int layout, void* image, size_t cap);
+/* Either SQF layout into an initialised SQF of the same q and r. */
+gqf_status gqf_sqf_image_load(gqf_sqf qf, const void* image, size_t bytes);
+gqf_status gqf_rsqf_image_save(gqf_rsqf cqf, void* image, size_t cap);
+gqf_status gqf_rsqf_image_load(gqf_rsqf cqf, const void* image, size_t bytes);
+
 /* ================================================================ batches
  * Many SQFs per launch (no reference counterpart: the reference launches every
  * kernel per filter).  A batch binds F initialised SQFs of one r (any q), the
