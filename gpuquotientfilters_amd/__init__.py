@@ -166,6 +166,7 @@ _SIGS = {
     "gqf_rsqf_insert": (_st, [_RSQF, C.c_int, _vp, _vp, _vp, _f]),
     "gqf_rsqf_lookup": (_st, [_RSQF, C.c_int, C.c_int, _vp, _vp, _vp, _f]),
     "gqf_rsqf_delete": (_st, [_RSQF, C.c_int, _vp, _vp, _f]),
+    "gqf_rsqf_count": (_st, [_RSQF, C.c_int, _vp, _vp, _vp, _f]),
     "gqf_rsqf_dump": (_st, [_RSQF, _vp]),
     "gqf_rsqf_load": (_st, [_RSQF, _vp]),
 }
@@ -467,6 +468,10 @@ class RSQF:
 
     def launchUnsortedLookups(self, keys, out, n=None):
         return self.lookup(keys, out, n, sorted=False)
+
+    def count(self, keys, out, n=None, stream=None):
+        """Stored copies of each key's fingerprint (gqf_rsqf_count)."""
+        return self._ms(lib().gqf_rsqf_count, self.h, _n(keys, n), _ptr(keys), _ptr(out), stream)
 
     def deleteGPU(self, keys, n=None, stream=None):
         return self._ms(lib().gqf_rsqf_delete, self.h, _n(keys, n), _ptr(keys), stream)

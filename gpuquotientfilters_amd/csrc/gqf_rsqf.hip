@@ -128,6 +128,45 @@ __device__ __forceinline__ uint32_t rsqf_lookup_one(const RsqfDev& f, uint32_t f
   return RS_NOT_FOUND;
 }
 
+// Stored copies of fingerprint (fq, fr): the reference walk finds the last
+// match (rsqf_lookup_one); the copies are the equal remainders ending there,
+// since remainders ascend inside a run.  A counting query over the multiset
+// the filter keeps (the reference has no counters, README.md:74).
+template <int R>
+__device__ __forceinline__ uint32_t rsqf_count_one(const RsqfDev& f, uint32_t fq, uint32_t fr) {
+  const uint32_t last = rsqf_lookup_one<R>(f, fq, fr);
+  if (last == RS_NOT_FOUND) return 0u;
+  uint32_t blockNum = last >> 6, slot = last & 63u, cnt = 1;
+  uint64_t ends = f.ends(blockNum);
+  for (;;) {  // step back while the previous slot is in the same run and equal
+    uint32_t pb = blockNum, ps = slot;
+    if (ps > 0) {
+      ps--;
+    } else {
+      if (pb == 0) break;
+      pb--;
+      ps = 63;
+    }
+    const uint64_t pends = pb == blockNum ? ends : f.ends(pb);
+    if (((pends >> ps) & 1ull) || (pb * 64 + ps) < fq) break;  // previous run, or before the quotient
+    if (f.get<R>(pb, ps) != fr) break;
+    cnt++;
+    blockNum = pb;
+    slot = ps;
+    ends = pends;
+  }
+  return cnt;
+}
+
+template <int R>
+__global__ __launch_bounds__(256) void k_rsqf_count(RsqfDev f, uint32_t q, const uint32_t* __restrict__ keys,
+                                                    uint32_t n, uint32_t* __restrict__ out) {
+  const uint64_t i = blockIdx.x * 256ull + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t fp = fingerprint(keys[i], q + f.r);
+  out[i] = rsqf_count_one<R>(f, (uint32_t)((fp >> f.r) & mask64(q)), (uint32_t)(fp & ((1u << f.r) - 1u)));
+}
+
 template <int R>
 __global__ __launch_bounds__(256) void k_rsqf_lookup(RsqfDev f, uint32_t q, const uint32_t* __restrict__ keys,
                                                      uint32_t n, uint32_t* __restrict__ out) {
@@ -989,6 +1028,28 @@ size_t gqf_rsqf_workspace_bytes(gqf_rsqf cqf, int op, int n, int sorted) {
   if (op == GQF_OP_INSERT) return wide ? rsqf_insert_ws<uint64_t>(g, un) : rsqf_insert_ws<uint32_t>(g, un);
   if (op == GQF_OP_DELETE) return wide ? rsqf_delete_ws<uint64_t>(g, un) : rsqf_delete_ws<uint32_t>(g, un);
   return 0;
+}
+
+gqf_status gqf_rsqf_count(gqf_rsqf cqf, int n, const uint32_t* d_keys, uint32_t* d_counts, gqf_stream stream,
+                          float* ms) {
+  GQF_TRY(rsqf_args(cqf, n, d_keys));
+  if (n > 0 && !d_counts) {
+    set_error("output array is NULL");
+    return GQF_ERR_INVALID;
+  }
+  CallScope cs;
+  GQF_TRY(cs.begin(stream));
+  if (n > 0) {
+    const RsqfDev d = rdev(cqf);
+    const dim3 grid(((uint32_t)n + 255) / 256);
+    if (cqf.rbits == 8)
+      GQF_LAUNCH_B((double)n * (8 + 128), k_rsqf_count<8>, grid, dim3(256), 0, cs.stream, d, cqf.qbits, d_keys,
+                   (uint32_t)n, d_counts);
+    else
+      GQF_LAUNCH_B((double)n * (8 + 128), k_rsqf_count<0>, grid, dim3(256), 0, cs.stream, d, cqf.qbits, d_keys,
+                   (uint32_t)n, d_counts);
+  }
+  return cs.end(ms, false);
 }
 
 // Device layout <-> reference packed blocks, converted on the host.

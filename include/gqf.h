@@ -168,6 +168,15 @@ gqf_status gqf_rsqf_insert(gqf_rsqf cqf, int n, const uint32_t* d_keys, int32_t*
 gqf_status gqf_rsqf_lookup(gqf_rsqf cqf, int sorted, int n, const uint32_t* d_keys,
                            int32_t* d_out, gqf_stream stream, float* ms);
 
+/* Counting query (no reference counterpart; README.md:74 names counters as
+ * missing): per key, the number of stored copies of its fingerprint -- the
+ * RSQF here keeps a multiset, so the copies are the equal remainders that end
+ * at the slot launchLookups returns.  Like the lookups, it reads the 8-bit
+ * block offsets of the reference format, so it is exact while no run pushes
+ * an offset past 255. */
+gqf_status gqf_rsqf_count(gqf_rsqf cqf, int n, const uint32_t* d_keys, uint32_t* d_counts, gqf_stream stream,
+                          float* ms);
+
 /* Delete (no reference counterpart): removes one stored copy per key whose
  * fingerprint is present; result is the canonical layout of what remains. */
 gqf_status gqf_rsqf_delete(gqf_rsqf cqf, int n, const uint32_t* d_keys, gqf_stream stream,

@@ -207,3 +207,30 @@ def test_wide_insert_delete_equal_canonical(q, r, alpha):
     d = keys[: n // 10]
     f.deleteGPU(dev(d))
     assert (f.dump() == canon(keys[n // 10:], q, r)).all()
+
+
+@pytest.mark.parametrize("q,r", [(14, 8), (13, 5), (20, 14)])
+def test_count_query(q, r):
+    """gqf_rsqf_count == multiplicity of each key's fingerprint in the stored
+    multiset (duplicates, a run of 200 copies of one key, misses; 34-bit
+    fingerprints at q=20 r=14).  Counts follow the reference lookup walk,
+    which reads the 8-bit block offsets: a run that pushes an offset past
+    255 breaks lookups (and counts) in the reference format, so the long
+    run stays below that."""
+    say("rsqf count", q, r)
+    n = O.key_count(0.7, q)
+    keys = O.mt19937(31, n)
+    cand = O.mt19937(5, 1000)
+    hot = cand[(O.fingerprints(cand, q, r) >> np.uint64(r)) < (1 << (q - 2))][0]
+    stored = np.concatenate([keys, keys[:n // 5], keys[:n // 20], np.full(200, hot, np.uint32)])
+    f = G.RSQF(q, r)
+    f.bulk_build(dev(stored))
+    probe = np.concatenate([keys[: n // 2], [hot], fresh_keys(32, 2000, stored)]).astype(np.uint32)
+    out = G.DeviceArray.empty(len(probe))
+    f.count(dev(probe), out)
+    vals, cnt = np.unique(O.fingerprints(stored, q, r), return_counts=True)
+    want = dict(zip(vals.tolist(), cnt.tolist()))
+    exp = np.array([want.get(int(x), 0) for x in O.fingerprints(probe, q, r)], np.uint32)
+    got = out.to_host()
+    assert (got == exp).all(), np.flatnonzero(got != exp)[:8]
+    assert got[n // 2] >= 200
