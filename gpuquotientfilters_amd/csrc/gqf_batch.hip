@@ -216,10 +216,7 @@ __device__ __forceinline__ RSS rss_wave_excl(RSS x, RSS* total) {
   inc = rss_combine(rss_dpp<DPP_ROW_SHR | 8>(inc), inc);
   inc = rss_combine(rss_dpp<DPP_ROW_BCAST15, 0xA>(inc), inc);
   inc = rss_combine(rss_dpp<DPP_ROW_BCAST31, 0xC>(inc), inc);
-  total->s.c = wave_read(inc.s.c, 63);
-  total->s.l = (int64_t)(((uint64_t)wave_read((uint32_t)((uint64_t)inc.s.l >> 32), 63) << 32) |
-                         wave_read((uint32_t)inc.s.l, 63));
-  total->s.f = wave_read(inc.s.f, 63);
+  total->s = RS{wave_read(inc.s.c, 63), wave_read(inc.s.lp, 63), wave_read(inc.s.fq1, 63)};
   total->h = wave_read(inc.h, 63);
   return rss_dpp<DPP_WAVE_SHR1>(inc);
 }
@@ -245,7 +242,7 @@ __global__ __launch_bounds__(1024) void k_bb_scan(uint4* __restrict__ agg, uint3
     const uint32_t i0 = base + (uint32_t)tid * SEG;
     uint4 w[SEG];
 #pragma unroll
-    for (uint32_t k = 0; k < SEG; ++k) w[k] = i0 + k < G ? agg[i0 + k] : make_uint4(0u, 0u, F_NONE, 0u);
+    for (uint32_t k = 0; k < SEG; ++k) w[k] = i0 + k < G ? agg[i0 + k] : make_uint4(0u, 0u, 0u, 0u);
     RSS seg = rss_none();
 #pragma unroll
     for (uint32_t k = 0; k < SEG; ++k) seg = rss_combine(seg, rss_unpack(w[k]));
@@ -271,10 +268,9 @@ __global__ __launch_bounds__(1024) void k_bb_scan(uint4* __restrict__ agg, uint3
         run = rss_combine(run, e);
         if (i + 1 == G || tl[i + 1].head) {
           const uint32_t f = tl[i].f;
-          const int64_t L = run.s.l;
-          const uint32_t S = flt[f].S;
-          fend[f] = L == LOC_NONE ? 0u : (uint32_t)(L + 1 < (int64_t)S ? L + 1 : (int64_t)S);
-          if (L != LOC_NONE && (uint64_t)L >= S) ferr[f] = DEV_ERR_CAPACITY;
+          const uint32_t lp = run.s.lp, S = flt[f].S;
+          fend[f] = lp < S ? lp : S;  // the slot after the filter's last entry
+          if (lp > S) ferr[f] = DEV_ERR_CAPACITY;
         }
       }
     }
@@ -289,7 +285,7 @@ __global__ __launch_bounds__(RB_THREADS, 4) void k_bb_write(const uint32_t* __re
                                                             const uint32_t* __restrict__ ferr, uint32_t* __restrict__ fix,
                                                             uint32_t* __restrict__ fix_cnt, uint32_t* __restrict__ more,
                                                             uint32_t* __restrict__ more_n) {
-  __shared__ __align__(16) uint8_t s_win[RB_WIN];
+  __shared__ __align__(16) uint32_t s_win[rb_win_words<BPE>()];
   __shared__ uint4 s_wave[RB_WAVES + 1];
   __shared__ uint32_t s_nfix;
   const uint32_t u = blockIdx.x;

@@ -468,8 +468,6 @@ gqf_status radix_sort(hipStream_t s, DeviceState& ds, Arena& ar, const void* in,
 
 // ====================================================================== locations
 // Canonical location of the i-th kept entry: loc_i = max(fq_i, loc_{i-1}+1).
-// LOC_NONE: no entry yet.
-constexpr int64_t LOC_NONE = -(1ll << 40);
 
 enum KeepMode { KEEP_ALL = 0, KEEP_UNIQUE = 1, KEEP_TAGGED_INSERT = 2, KEEP_FLAGS = 3 };
 

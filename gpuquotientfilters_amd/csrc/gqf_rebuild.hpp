@@ -17,40 +17,44 @@ constexpr int RB_THREADS = 512;
 constexpr int RB_ITEMS = 8;
 constexpr int RB_WAVES = RB_THREADS / 64;
 constexpr int RB_TILE = RB_THREADS * RB_ITEMS;
-constexpr uint32_t F_NONE = 0xFFFFFFFFu;
 
-// Scan state of a block of entries: kept count c, location map
-// x -> max(x + c, l), and the quotient f of its last kept entry (run heads are
-// the kept entries whose quotient differs from the previous kept entry's).
+// Scan state of a block of entries.  Every field is 0 for a block with no
+// kept entry and combines by a monotone rule:
+//   c    kept count (add);
+//   lp   1 + location of the last kept entry when nothing precedes the block
+//        (0: none).  The kept entries take distinct locations >= 0, so
+//        lp >= c, and the block maps an incoming lp to max(lp + c, lp_block)
+//        -- for an incoming 0 as well, where that is lp_block.  The add
+//        saturates: a location past the table (a capacity error) never wraps;
+//   fq1  1 + quotient of the last kept entry (0: none); kept entries ascend,
+//        so it combines by max.  A kept entry starts a run iff its quotient
+//        differs from fq1 - 1 before it (0xFFFFFFFF: no entry before it).
 struct RS {
-  uint32_t c;
-  int64_t l;
-  uint32_t f;
+  uint32_t c, lp, fq1;
 };
+__device__ __forceinline__ uint32_t add_sat(uint32_t a, uint32_t b) { return __builtin_elementwise_add_sat(a, b); }
 __device__ __forceinline__ RS rs_combine(RS a, RS b) {
-  RS o;
-  o.c = a.c + b.c;
-  const int64_t x = a.l == LOC_NONE ? LOC_NONE : a.l + (int64_t)b.c;
-  o.l = x > b.l ? x : b.l;
-  o.f = b.c ? b.f : a.f;
-  return o;
+  const uint32_t x = add_sat(a.lp, b.c);
+  return RS{a.c + b.c, x > b.lp ? x : b.lp, a.fq1 > b.fq1 ? a.fq1 : b.fq1};
 }
-__device__ __forceinline__ RS rs_none() { return RS{0u, LOC_NONE, F_NONE}; }
-// Stored form of a state: {c, l + 1 (0 = none), f, 0}.
-__device__ __forceinline__ uint4 rs_pack(RS x) {
-  return make_uint4(x.c, x.l == LOC_NONE ? 0u : (uint32_t)(x.l + 1), x.f, 0u);
+__device__ __forceinline__ RS rs_none() { return RS{0u, 0u, 0u}; }
+// Appends one kept entry of quotient fq: its location is max(fq, last + 1).
+__device__ __forceinline__ void rs_push(RS& s, uint32_t fq) {
+  const uint32_t a = s.lp + 1u, b = fq + 1u;
+  s.lp = a > b ? a : b;
+  s.c++;
+  s.fq1 = b;
 }
-__device__ __forceinline__ RS rs_unpack(uint4 w) { return RS{w.x, w.y == 0 ? LOC_NONE : (int64_t)w.y - 1, w.z}; }
+// Stored form of a state: {c, lp, fq1, 0}.
+__device__ __forceinline__ uint4 rs_pack(RS x) { return make_uint4(x.c, x.lp, x.fq1, 0u); }
+__device__ __forceinline__ RS rs_unpack(uint4 w) { return RS{w.x, w.y, w.z}; }
 
 // Exclusive scan across the 64 lanes of a wave (DPP moves, see
-// wave_incl_add); *total = the wave aggregate.
+// wave_incl_add; lanes without a source read the empty state 0);
+// *total = the wave aggregate.
 template <int CTRL, int ROW_MASK = 0xF>
 __device__ __forceinline__ RS rs_dpp(RS x) {
-  RS y;
-  y.c = dpp_u32<CTRL, ROW_MASK>(0u, x.c);
-  y.l = (int64_t)dpp_u64<CTRL, ROW_MASK>((uint64_t)LOC_NONE, (uint64_t)x.l);
-  y.f = dpp_u32<CTRL, ROW_MASK>(F_NONE, x.f);
-  return y;
+  return RS{dpp_u32<CTRL, ROW_MASK>(0u, x.c), dpp_u32<CTRL, ROW_MASK>(0u, x.lp), dpp_u32<CTRL, ROW_MASK>(0u, x.fq1)};
 }
 __device__ __forceinline__ RS rs_wave_excl(RS x, RS* total) {
   RS inc = x;
@@ -60,10 +64,7 @@ __device__ __forceinline__ RS rs_wave_excl(RS x, RS* total) {
   inc = rs_combine(rs_dpp<DPP_ROW_SHR | 8>(inc), inc);
   inc = rs_combine(rs_dpp<DPP_ROW_BCAST15, 0xA>(inc), inc);
   inc = rs_combine(rs_dpp<DPP_ROW_BCAST31, 0xC>(inc), inc);
-  total->c = wave_read(inc.c, 63);
-  total->l = (int64_t)(((uint64_t)wave_read((uint32_t)((uint64_t)inc.l >> 32), 63) << 32) |
-                       wave_read((uint32_t)inc.l, 63));
-  total->f = wave_read(inc.f, 63);
+  *total = RS{wave_read(inc.c, 63), wave_read(inc.lp, 63), wave_read(inc.fq1, 63)};
   return rs_dpp<DPP_WAVE_SHR1>(inc);
 }
 
@@ -135,7 +136,7 @@ __device__ __forceinline__ RS rs_block(RS a, RS* ex_out, uint4* s_wave) {
     const uint32_t i0 = base + (uint32_t)tid * SEG;
     uint4 w[SEG];
 #pragma unroll
-    for (uint32_t k = 0; k < SEG; ++k) w[k] = i0 + k < G ? agg[i0 + k] : make_uint4(0u, 0u, F_NONE, 0u);
+    for (uint32_t k = 0; k < SEG; ++k) w[k] = i0 + k < G ? agg[i0 + k] : make_uint4(0u, 0u, 0u, 0u);
     RS seg = rs_none();
 #pragma unroll
     for (uint32_t k = 0; k < SEG; ++k) seg = rs_combine(seg, rs_unpack(w[k]));
@@ -166,9 +167,8 @@ __device__ __forceinline__ RS rs_block(RS a, RS* ex_out, uint4* s_wave) {
   if (tid == 0) {
     if (total) *total = rs_pack(carry);
     ctr[CTR_COUNT] = carry.c;
-    const int64_t L = carry.l;
-    ctr[CTR_MISC + 3] = L == LOC_NONE ? 0u : (uint32_t)(L + 1 < (int64_t)S ? L + 1 : (int64_t)S);
-    if (L != LOC_NONE && (uint64_t)L >= S) ctr[CTR_ERR] |= DEV_ERR_CAPACITY;
+    ctr[CTR_MISC + 3] = (uint32_t)(carry.lp < S ? carry.lp : S);  // the slot after the last entry
+    if (carry.lp > S) ctr[CTR_ERR] |= DEV_ERR_CAPACITY;
   }
 }
 

@@ -474,11 +474,7 @@ __device__ __forceinline__ uint32_t rr_classify(const K* v, uint32_t fl, uint32_
   for (int j = 0; j < RB_ITEMS; ++j) {
     if (i0 + j < tn && (MODE == KEEP_ALL || ((fl >> j) & 1u))) {
       keepbits |= 1u << j;
-      const uint32_t fq = (uint32_t)(v[j] >> r) & mq;
-      const int64_t fqi = (int64_t)fq;
-      agg.l = agg.l == LOC_NONE ? fqi : (fqi > agg.l + 1 ? fqi : agg.l + 1);
-      agg.c++;
-      agg.f = fq;
+      rs_push(agg, (uint32_t)(v[j] >> r) & mq);
     }
   }
   return keepbits;
@@ -526,7 +522,7 @@ __global__ __launch_bounds__(256) void k_rr_clear(uint8_t* __restrict__ t, uint6
 
 // One kept entry's contributions to the window [jw0, jw1) (block indices).
 // prev / lastf: location and quotient of the previous kept entry (-1 /
-// F_NONE when there is none).
+// 0xFFFFFFFF when there is none).
 template <int R>
 __device__ __forceinline__ void rr_visit(uint32_t fq, uint32_t fr, int32_t lc, int32_t prev, uint32_t lastf,
                                          uint32_t r, uint32_t jw0, uint32_t jw1, uint32_t* s_occ, uint32_t* s_end,
@@ -606,8 +602,8 @@ __global__ __launch_bounds__(RB_THREADS, 6) void k_rr_write(uint8_t* __restrict_
     const RS me = rs_combine(X, ex);
     const RS Y = rs_combine(X, sub);
     if (sub.c > 0) {
-      const int32_t prev0 = me.l == LOC_NONE ? -1 : (int32_t)me.l;
-      const uint32_t lastf0 = me.f;
+      const int32_t prev0 = (int32_t)me.lp - 1;  // -1: no entry before
+      const uint32_t lastf0 = me.fq1 - 1u;       // 0xFFFFFFFF: none
       // blocks this thread's contributions touch: [t_lo, t_hi]
       uint32_t t_lo = 0xFFFFFFFFu, t_hi = 0;
       if (keepbits) {
@@ -635,13 +631,12 @@ __global__ __launch_bounds__(RB_THREADS, 6) void k_rr_write(uint8_t* __restrict_
       }
       __syncthreads();
       const uint32_t B0 = s_b[0];
-      const uint32_t B1 = (uint32_t)(Y.l >> 6) + 1;
-      const uint32_t qfirst = s_b[1];      // quotient blocks of the first and last
-      const uint32_t qlast = Y.f >> 6;     // kept entries: occupied words shared
-      const uint32_t Llo = X.l == LOC_NONE ? 0u : (uint32_t)(X.l + 1);  // remainders: [Llo, Lhi)
-      const uint32_t Lhi = (uint32_t)(Y.l + 1);
-      const int64_t Elo = X.l == LOC_NONE ? -1 : X.l;  // run ends: [Elo, Ehi)
-      const int64_t Ehi = Y.l;
+      const uint32_t B1 = ((Y.lp - 1u) >> 6) + 1;
+      const uint32_t qfirst = s_b[1];            // quotient blocks of the first and last
+      const uint32_t qlast = (Y.fq1 - 1u) >> 6;  // kept entries: occupied words shared
+      const uint32_t Llo = X.lp, Lhi = Y.lp;     // remainders: [Llo, Lhi)
+      const int64_t Elo = (int64_t)X.lp - 1;     // run ends: [Elo, Ehi)
+      const int64_t Ehi = (int64_t)Y.lp - 1;
       for (uint32_t jw0 = B0; jw0 < B1; jw0 += RW_WB) {
         const uint32_t jw1 = jw0 + RW_WB < B1 ? jw0 + RW_WB : B1;
         const uint32_t nblk = jw1 - jw0;
@@ -706,14 +701,14 @@ __global__ __launch_bounds__(RB_THREADS, 6) void k_rr_write(uint8_t* __restrict_
 }
 
 // The last run end and the offsets of the blocks after the last quotient,
-// from the scan total {c, l = last location, f = last quotient}.
+// from the scan total (last location lp - 1, last quotient fq1 - 1).
 __global__ __launch_bounds__(64) void k_rr_tail(uint8_t* __restrict__ base, const uint4* __restrict__ total,
                                                 const uint32_t* __restrict__ ctr) {
   if (ctr[CTR_ERR]) return;
   const RS t = rs_unpack(*total);
   if (t.c == 0) return;
   uint64_t* meta = reinterpret_cast<uint64_t*>(base);
-  const uint32_t L = (uint32_t)t.l, F = t.f;
+  const uint32_t L = t.lp - 1u, F = t.fq1 - 1u;
   if (threadIdx.x == 0) atomicOr((unsigned long long*)&meta[4 * (size_t)(L >> 6) + 1], 1ull << (L & 63));
   for (uint32_t j = (F >> 6) + 1 + threadIdx.x; j <= (L >> 6); j += 64)
     meta[4 * (size_t)j + 2] = (L - 64 * j + 1) & 0xFFu;

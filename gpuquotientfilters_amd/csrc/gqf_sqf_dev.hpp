@@ -279,8 +279,14 @@ __global__ __launch_bounds__(256) void k_sqf_lookup_sorted(const uint8_t* __rest
 // (quotientFilter.cu:788-857, 985-1038, 1492-1539), the merge rebuild
 // (quotientFilter.cu:1651-1719) and the end states of insert /
 // superclusterDeletes.
-constexpr uint32_t RB_WIN = 8192;  // table bytes per write window
-constexpr uint32_t RB_FIX = 64;    // per-sub-tile occupied-bit fix-up slots
+constexpr uint32_t RB_WIN_SLOTS = 8192;  // slots per write window
+constexpr uint32_t RB_FIX = 64;          // per-sub-tile occupied-bit fix-up slots
+// LDS words of a write window: its table bytes plus 16, so that window word 0
+// sits on a 16-byte boundary of the table.
+template <int BPE>
+constexpr uint32_t rb_win_words() {
+  return (RB_WIN_SLOTS * BPE + 16) / 4;
+}
 
 // This thread's RB_ITEMS consecutive entries of sub-tile [d0, d0+tn):
 // v[1+j] = entry j and v[0] = the entry before them; bit j of tags = entry j
@@ -394,11 +400,7 @@ __device__ __forceinline__ uint32_t rb_classify(const uint32_t* v, uint32_t tags
       else kp = (fl >> j) & 1u;
       if (kp) {
         keepbits |= 1u << j;
-        const uint32_t fq = (uint32_t)(((uint64_t)fp >> r) & mq);
-        const int64_t f = (int64_t)fq;
-        agg.l = agg.l == LOC_NONE ? f : (f > agg.l + 1 ? f : agg.l + 1);
-        agg.c++;
-        agg.f = fq;
+        rs_push(agg, (uint32_t)(((uint64_t)fp >> r) & mq));
       }
     }
   }
@@ -435,163 +437,137 @@ __global__ __launch_bounds__(RB_THREADS, 8) void k_rb_agg(const uint32_t* __rest
   if (threadIdx.x == 0) agg[blockIdx.x] = rs_pack(X);
 }
 
-// Writes the element bytes of slot sl into the window (big-endian).
+// ORs element e (BPE bytes, big-endian) into the window at byte offset off.
 template <int BPE>
-__device__ __forceinline__ void win_put(uint8_t* s_win, uint32_t off, uint32_t e) {
-#pragma unroll
-  for (int b = 0; b < BPE; ++b) s_win[off + b] = (uint8_t)(e >> (8 * (BPE - 1 - b)));
+__device__ __forceinline__ void win_or(uint32_t* s_win, uint32_t off, uint32_t e) {
+  uint32_t m;  // the element's bytes in memory order, as a little-endian word
+  if (BPE == 1) m = e;
+  else if (BPE == 2) m = ((e & 0xFFu) << 8) | (e >> 8);
+  else if (BPE == 3) m = ((e & 0xFFu) << 16) | (e & 0xFF00u) | ((e >> 16) & 0xFFu);
+  else m = __builtin_bswap32(e);
+  const uint32_t sh = (off & 3u) * 8u;
+  atomicOr(&s_win[off >> 2], m << sh);
+  if (BPE == 3 && (off & 3u) > 1u) atomicOr(&s_win[(off >> 2) + 1], m >> (32u - sh));
 }
 
-// Copies window bytes [B0, B1) of the table from LDS (dword stores inside,
-// byte stores at unaligned edges).
-__device__ __forceinline__ void win_out(uint8_t* __restrict__ t, const uint8_t* s_win, uint64_t B0, uint64_t B1) {
-  const int tid = threadIdx.x;
-  const uint64_t A0 = (B0 + 3) & ~3ull, A1 = B1 & ~3ull;
-  if (A0 >= A1) {
-    for (uint64_t b = B0 + tid; b < B1; b += RB_THREADS) t[b] = s_win[b - B0];
-    return;
-  }
-  for (uint64_t b = B0 + tid; b < A0; b += RB_THREADS) t[b] = s_win[b - B0];
-  for (uint64_t b = A1 + tid; b < B1; b += RB_THREADS) t[b] = s_win[b - B0];
-  uint32_t* tw = reinterpret_cast<uint32_t*>(t + A0);
-  const uint32_t nw = (uint32_t)((A1 - A0) / 4), sh = (uint32_t)(A0 - B0);
-  for (uint32_t w = tid; w < nw; w += RB_THREADS) {
-    const uint32_t q0 = sh + 4 * w;
-    tw[w] = (uint32_t)s_win[q0] | ((uint32_t)s_win[q0 + 1] << 8) | ((uint32_t)s_win[q0 + 2] << 16) |
-            ((uint32_t)s_win[q0 + 3] << 24);
+// Copies table bytes [B0, B1) from the window, whose word 0 holds the table
+// bytes from the 16-byte boundary at or below B0 (absolute addresses): whole
+// 16-byte stores inside, byte stores in the two edge pieces.  skip_zero: the
+// table is all zeros already, so zero pieces are not stored.
+__device__ __forceinline__ void win_out(uint8_t* __restrict__ t, const uint32_t* s_win, uint64_t B0, uint64_t B1,
+                                        bool skip_zero) {
+  const uint32_t mis = (uint32_t)((reinterpret_cast<uintptr_t>(t) + B0) & 15u);
+  uint8_t* base = t + B0 - mis;  // 16-byte aligned; bytes before t + B0 are never stored
+  const uint32_t nb = (uint32_t)(B1 - B0) + mis;
+  const uint32_t nch = (nb + 15) / 16;
+  for (uint32_t k = threadIdx.x; k < nch; k += RB_THREADS) {
+    const uint4 x = reinterpret_cast<const uint4*>(s_win)[k];
+    const uint32_t lo = 16 * k, hi = 16 * k + 16;
+    if (lo >= mis && hi <= nb) {
+      if (!skip_zero || (x.x | x.y | x.z | x.w)) *reinterpret_cast<uint4*>(base + lo) = x;
+    } else {
+      const uint8_t* xb = reinterpret_cast<const uint8_t*>(s_win) + lo;
+      for (uint32_t i = lo < mis ? mis : lo; i < (hi < nb ? hi : nb); ++i) base[i] = xb[i - lo];
+    }
   }
 }
 
-// Each chunk walks its sub-tiles with the running state X.  Sub-tile u owns
-// the slots (X.l, Y.l] up to its last kept entry; each thread owns the slots
-// after the previous kept entry up to its own last one (its territory) and
-// writes them -- elements and the zeros between them -- into an LDS window
-// straight from registers.  Occupied bits are then set in the window (each
-// run head's quotient is a different slot, so plain byte updates do not
-// race); those whose quotient precedes the owned range go to the fix-up
-// list.  A long owned range (sparse table) is written window by window.
-// Writes one sub-tile: block scan of the per-thread states (v, keepbits,
-// a) on top of the incoming state X, fix-up entries for run heads whose
-// quotient precedes the owned range, then the owned slot range window by
-// window.  Every thread of the block calls it; returns the outgoing state.
+// Writes one sub-tile: block scan of the per-thread states (v, keepbits, a)
+// on top of the incoming state X; the sub-tile owns the slots [X.lp, Y.lp)
+// up to its last kept entry and writes every byte of them (elements and the
+// zeros between them) window by window: the window is zeroed in LDS, every
+// kept entry ORs its element into it and every run head its occupied bit
+// (ORs commute, so no ordering among threads), then the window is copied
+// out.  Run heads whose quotient precedes the owned range go to the fix-up
+// list.  precleared: the table is all zeros, so windows without an element
+// or occupied bit are skipped.  Every thread of the block calls it; returns
+// the outgoing state.
 template <int BPE>
 __device__ __forceinline__ RS rb_emit(uint8_t* __restrict__ t, uint32_t q, uint32_t r, const uint32_t* v,
                                       uint32_t keepbits, RS a, RS X, uint32_t u, uint4* s_wave,
-                                      uint8_t* s_win, uint32_t* s_nfix_p,
+                                      uint32_t* s_win, uint32_t* s_nfix_p,
                                       uint32_t* __restrict__ fix, uint32_t* __restrict__ fix_cnt,
                                       uint32_t* __restrict__ fix_more, uint32_t* __restrict__ fix_more_n,
                                       uint32_t fix_cap, int precleared) {
   const int tid = threadIdx.x;
-  const uint64_t mq = mask64(q);
+  const uint32_t mq = (uint32_t)mask64(q);
   const uint32_t rmask = (1u << r) - 1u;
   constexpr uint32_t bpe = BPE;
-  const uint32_t wslots = RB_WIN / BPE;
+  constexpr uint32_t nwin = rb_win_words<BPE>() / 4;  // 16-byte pieces
   uint32_t& s_nfix = *s_nfix_p;
-    if (tid == 0) s_nfix = 0;
-    RS ex;
-    const RS sub = rs_block(a, &ex, s_wave);
-    const RS me = rs_combine(X, ex);  // state before this thread's entries
-    const RS Y = rs_combine(X, sub);
-    const uint32_t Llo = X.l == LOC_NONE ? 0u : (uint32_t)(X.l + 1);
-    const uint32_t Lhi = sub.c > 0 ? (uint32_t)(Y.l + 1) : Llo;
-    // per-thread location walk in 32 bits: -1 = no entry yet (fq >= 0)
-    const int32_t prev0 = me.l == LOC_NONE ? -1 : (int32_t)me.l;
-    const int32_t tstart = prev0 + 1;
-    // run heads whose quotient precedes the owned range: fix-up list
-    {
-      uint32_t lastf = me.f;
+  if (tid == 0) s_nfix = 0;
+  RS ex;
+  const RS sub = rs_block(a, &ex, s_wave);
+  const RS me = rs_combine(X, ex);  // state before this thread's entries
+  const RS Y = rs_combine(X, sub);
+  const uint32_t Llo = X.lp, Lhi = Y.lp;
+  // per-thread location walk: prev = location of the previous kept entry
+  // (-1: none), lastf = its quotient (0xFFFFFFFF: none)
+  const int32_t prev0 = (int32_t)me.lp - 1;
+  const uint32_t lastf0 = me.fq1 - 1u;
+  for (uint32_t w0 = Llo; w0 < Lhi; w0 += RB_WIN_SLOTS) {
+    const uint32_t w1 = w0 + RB_WIN_SLOTS < Lhi ? w0 + RB_WIN_SLOTS : Lhi;
+    if (precleared) {
+      bool any = false;
+      int32_t prev = prev0;
+      uint32_t lastf = lastf0;
 #pragma unroll
       for (int j = 0; j < RB_ITEMS; ++j) {
         if (keepbits & (1u << j)) {
-          const uint32_t fp = v[1 + j];
-          const uint32_t fq = (fp >> r) & (uint32_t)mq;
-          if (fq != lastf && fq < Llo) {
-            const uint32_t k = atomicAdd(&s_nfix, 1u);
-            if (k < RB_FIX) {
-              fix[(uint64_t)u * RB_FIX + k] = fq;
-            } else {
-              const uint32_t slot = atomicAdd(fix_more_n, 1u);
-              if (slot < fix_cap) fix_more[slot] = fq;
-            }
+          const uint32_t fq = (v[1 + j] >> r) & mq;
+          const int32_t lc = (int32_t)fq > prev + 1 ? (int32_t)fq : prev + 1;
+          any |= ((uint32_t)lc - w0 < w1 - w0) || (fq != lastf && fq - w0 < w1 - w0);
+          prev = lc;
+          lastf = fq;
+        }
+      }
+      if (!__syncthreads_or(any)) continue;
+    }
+    const uint64_t B0 = (uint64_t)w0 * bpe;
+    const uint32_t mis = (uint32_t)((reinterpret_cast<uintptr_t>(t) + B0) & 15u);
+    for (uint32_t k = tid; k < nwin; k += RB_THREADS) reinterpret_cast<uint4*>(s_win)[k] = make_uint4(0u, 0u, 0u, 0u);
+    __syncthreads();
+    {
+      // window byte of slot s: (s - w0) * bpe + mis
+      int32_t prev = prev0;
+      uint32_t lastf = lastf0;
+#pragma unroll
+      for (int j = 0; j < RB_ITEMS; ++j) {
+        const uint32_t fp = v[1 + j];
+        const uint32_t fq = (fp >> r) & mq;
+        const bool kp = (keepbits >> j) & 1u;
+        const int32_t lc = (int32_t)fq > prev + 1 ? (int32_t)fq : prev + 1;
+        const bool head = fq != lastf;
+        const bool in_e = kp && (uint32_t)lc - w0 < w1 - w0;
+        const bool in_o = kp && head && fq - w0 < w1 - w0;
+        const uint32_t e = ((fp & rmask) << 3) | (head ? 0u : 2u) | ((uint32_t)lc != fq ? 1u : 0u);
+        const uint32_t eoff = ((uint32_t)lc - w0) * bpe + mis;
+        if (in_e) win_or<BPE>(s_win, eoff, e);
+        if (in_o) {
+          const uint32_t o = (fq - w0) * bpe + mis + bpe - 1;
+          atomicOr(&s_win[o >> 2], 4u << ((o & 3u) * 8u));
+        }
+        if (in_e && head && fq < Llo) {
+          // the run head's quotient precedes the owned range
+          const uint32_t k = atomicAdd(&s_nfix, 1u);
+          if (k < RB_FIX) {
+            fix[(uint64_t)u * RB_FIX + k] = fq;
+          } else {
+            const uint32_t slot = atomicAdd(fix_more_n, 1u);
+            if (slot < fix_cap) fix_more[slot] = fq;
           }
+        }
+        if (kp) {
+          prev = lc;
           lastf = fq;
         }
       }
     }
-    for (uint32_t w0 = Llo; w0 < Lhi; w0 += wslots) {
-      const uint32_t w1 = w0 + wslots < Lhi ? w0 + wslots : Lhi;
-      if (precleared) {
-        // the table is all zeros: windows holding no element and no
-        // occupied bit are skipped; the others start from a zeroed window
-        bool any = false;
-        int32_t prev = prev0;
-        uint32_t lastf = me.f;
-#pragma unroll
-        for (int j = 0; j < RB_ITEMS; ++j) {
-          if (keepbits & (1u << j)) {
-            const uint32_t fp = v[1 + j];
-            const int32_t fq = (int32_t)((fp >> r) & (uint32_t)mq);
-            const int32_t lc = fq > prev + 1 ? fq : prev + 1;
-            if ((uint32_t)lc >= w0 && (uint32_t)lc < w1) any = true;
-            if ((uint32_t)fq != lastf && (uint32_t)fq >= w0 && (uint32_t)fq < w1) any = true;
-            prev = lc;
-            lastf = (uint32_t)fq;
-          }
-        }
-        if (!__syncthreads_or(any)) continue;
-        const uint32_t nw = ((w1 - w0) * bpe + 3) / 4;
-        for (uint32_t k = tid; k < nw; k += RB_THREADS) reinterpret_cast<uint32_t*>(s_win)[k] = 0u;
-        __syncthreads();
-      }
-      // this thread's territory within the window: elements, and the zeros
-      // before each of them unless the window was zeroed above
-      {
-        int32_t prev = prev0;
-        uint32_t lastf = me.f;
-        int32_t cur = tstart > (int32_t)w0 ? tstart : (int32_t)w0;  // next slot to write
-#pragma unroll
-        for (int j = 0; j < RB_ITEMS; ++j) {
-          if (keepbits & (1u << j)) {
-            const uint32_t fp = v[1 + j];
-            const int32_t fq = (int32_t)((fp >> r) & (uint32_t)mq);
-            const int32_t lc = fq > prev + 1 ? fq : prev + 1;
-            const bool head = (uint32_t)fq != lastf;
-            if (!precleared) {
-              const int32_t zend = lc < (int32_t)w1 ? lc : (int32_t)w1;
-              for (; cur < zend; ++cur) win_put<BPE>(s_win, (uint32_t)(cur - (int32_t)w0) * BPE, 0u);
-            }
-            if ((uint32_t)lc >= w0 && (uint32_t)lc < w1) {
-              uint32_t e = (fp & rmask) << 3;
-              if (!head) e |= 2u;
-              if (lc != fq) e |= 1u;
-              win_put<BPE>(s_win, ((uint32_t)lc - w0) * BPE, e);
-            }
-            if (lc + 1 > cur) cur = lc + 1;
-            prev = lc;
-            lastf = (uint32_t)fq;
-          }
-        }
-      }
-      __syncthreads();
-      // occupied bits of the run heads whose quotient is in this window
-      {
-        uint32_t lastf = me.f;
-#pragma unroll
-        for (int j = 0; j < RB_ITEMS; ++j) {
-          if (keepbits & (1u << j)) {
-            const uint32_t fp = v[1 + j];
-            const uint32_t fq = (fp >> r) & (uint32_t)mq;
-            if (fq != lastf && fq >= w0 && fq < w1) s_win[(fq - w0) * bpe + bpe - 1] |= 4u;
-            lastf = fq;
-          }
-        }
-      }
-      __syncthreads();
-      win_out(t, s_win, (uint64_t)w0 * bpe, (uint64_t)w1 * bpe);
-      __syncthreads();
-    }
     __syncthreads();
-    if (tid == 0) fix_cnt[u] = s_nfix < RB_FIX ? s_nfix : RB_FIX;
+    win_out(t, s_win, B0, (uint64_t)w1 * bpe, precleared != 0);
+    __syncthreads();
+  }
+  if (tid == 0) fix_cnt[u] = s_nfix < RB_FIX ? s_nfix : RB_FIX;
   return Y;
 }
 
@@ -609,7 +585,7 @@ __global__ __launch_bounds__(RB_THREADS, 6) void k_rb_write(uint8_t* __restrict_
                                                   int precleared) {
   if (ctr[CTR_ERR]) return;  // capacity error: the table stays as it was
   __shared__ uint32_t s_stage[MERGE ? RB_TILE : 1];
-  __shared__ __align__(16) uint8_t s_win[RB_WIN];
+  __shared__ __align__(16) uint32_t s_win[rb_win_words<BPE>()];
   __shared__ uint4 s_wave[RB_WAVES + 1];
   __shared__ uint32_t s_nfix;
   if (na_dev) na = *na_dev;
