@@ -239,10 +239,12 @@ inline SortPlan sort_plan(uint32_t n) {
   return p;
 }
 
+// fp_out (HASH only, may be NULL): the fingerprints are also stored, so that
+// the pass's downsweep reads them instead of hashing again.
 template <typename K, bool HASH>
 __global__ __launch_bounds__(256) void k_rs_upsweep(const void* __restrict__ in, uint32_t n, uint32_t fp_bits,
                                                     uint32_t shift, uint32_t chunk, uint32_t G,
-                                                    uint32_t* __restrict__ hist) {
+                                                    uint32_t* __restrict__ hist, K* __restrict__ fp_out) {
   __shared__ uint32_t h[4 * 256];
   const int tid = threadIdx.x, wid = tid >> 6;
   for (int i = tid; i < 4 * 256; i += 256) h[i] = 0;
@@ -260,7 +262,10 @@ __global__ __launch_bounds__(256) void k_rs_upsweep(const void* __restrict__ in,
 #pragma unroll
     for (int i = 0; i < SORT_ITEMS; ++i) {
       const uint64_t idx = t0 + (uint64_t)i * 256 + tid;
-      if (idx < hi) atomicAdd(&h[wid * 256 + ((uint32_t)(k[i] >> shift) & 255u)], 1u);
+      if (idx < hi) {
+        atomicAdd(&h[wid * 256 + ((uint32_t)(k[i] >> shift) & 255u)], 1u);
+        if (HASH && fp_out) fp_out[idx] = k[i];
+      }
     }
   }
   __syncthreads();
@@ -428,14 +433,23 @@ gqf_status radix_sort(hipStream_t s, DeviceState& ds, Arena& ar, const void* in,
   int cur = 0;
   for (uint32_t p = 0; p < npass; ++p) {
     const uint32_t shift = lo_bit + 8 * p;
-    const bool first_hash = hash && p == 0;
+    bool first_hash = hash && p == 0;
+    if (first_hash && sizeof(K) == 4) {
+      // 32-bit fingerprints: hashed once, stored by the upsweep (into the
+      // buffer this pass's downsweep does not write) and read back by it
+      K* fps = kb[cur ^ 1];
+      GQF_LAUNCH_B((double)n * 8.0, (k_rs_upsweep<K, true>), dim3(plan.G), dim3(256), 0, s, src, n, fp_bits, shift,
+                   plan.chunk, plan.G, hist, fps);
+      src = fps;
+      first_hash = false;
+    } else if (first_hash) {
+      GQF_LAUNCH_B((double)n * 4.0, (k_rs_upsweep<K, true>), dim3(plan.G), dim3(256), 0, s, src, n, fp_bits, shift,
+                   plan.chunk, plan.G, hist, (K*)nullptr);
+    } else {
+      GQF_LAUNCH_B((double)n * sizeof(K), (k_rs_upsweep<K, false>), dim3(plan.G), dim3(256), 0, s, src, n, fp_bits,
+                   shift, plan.chunk, plan.G, hist, (K*)nullptr);
+    }
     const double kin = first_hash ? 4.0 : (double)sizeof(K);
-    if (first_hash)
-      GQF_LAUNCH_B((double)n * kin, (k_rs_upsweep<K, true>), dim3(plan.G), dim3(256), 0, s, src, n, fp_bits, shift,
-                   plan.chunk, plan.G, hist);
-    else
-      GQF_LAUNCH_B((double)n * kin, (k_rs_upsweep<K, false>), dim3(plan.G), dim3(256), 0, s, src, n, fp_bits, shift,
-                   plan.chunk, plan.G, hist);
     GQF_LAUNCH(k_rs_rowscan, dim3(256), dim3(1024), 0, s, hist, plan.G, dbase);
     GQF_LAUNCH(k_rs_digitscan, dim3(1), dim3(256), 0, s, dbase);
     K* dst = kb[cur];
@@ -618,74 +632,85 @@ __global__ __launch_bounds__(DM_THREADS) void k_delete_mark(const K* __restrict_
 
 // ====================================================================== region scan
 // Exclusive scan of COLS interleaved uint32 columns over `rows` rows, in
-// place, by one block of 1024 threads, in pieces of 4096 rows: a piece is
-// staged through LDS with coalesced loads, each thread scans 4 consecutive
-// rows, one block-wide scan, coalesced stores; totals are carried between
-// pieces.  totals[c] = column sums (may be NULL).
+// place, by one block of 1024 threads, in pieces of 4096 rows: each thread
+// holds 4 consecutive rows in registers (16-byte loads), one block-wide scan
+// per column, stores in place; totals are carried between pieces and the
+// next piece is loaded before the current one is scanned.  totals[c] =
+// column sums (may be NULL).  data must be 16-byte aligned.
+template <int COLS>
+__device__ __forceinline__ void scan_rows_load(const uint32_t* __restrict__ data, uint32_t rows, uint32_t row0,
+                                               uint32_t* v) {
+  constexpr int N = 4 * COLS;
+  if (row0 + 4 <= rows) {
+    const uint4* p = reinterpret_cast<const uint4*>(data + (uint64_t)row0 * COLS);
+#pragma unroll
+    for (int k = 0; k < N / 4; ++k) {
+      const uint4 x = p[k];
+      v[4 * k] = x.x;
+      v[4 * k + 1] = x.y;
+      v[4 * k + 2] = x.z;
+      v[4 * k + 3] = x.w;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < N; ++k) v[k] = row0 + k / COLS < rows ? data[(uint64_t)row0 * COLS + k] : 0u;
+  }
+}
 template <int COLS>
 __global__ __launch_bounds__(1024) void k_scan_rows(uint32_t* __restrict__ data, uint32_t rows,
                                                     uint32_t* __restrict__ totals) {
   constexpr uint32_t PIECE = 4096;
-  __shared__ uint32_t s_v[PIECE * COLS];
-  __shared__ uint32_t s_w[COLS][16];
+  constexpr int N = 4 * COLS;
+  __shared__ uint32_t s_w[2][COLS][16];
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   uint32_t carry[COLS];
 #pragma unroll
   for (int c = 0; c < COLS; ++c) carry[c] = 0;
+  uint32_t cur[N], nxt[N];
+  scan_rows_load<COLS>(data, rows, (uint32_t)tid * 4, cur);
+  int buf = 0;
   for (uint32_t base = 0; base < rows; base += PIECE) {
-    const uint32_t n = rows - base < PIECE ? rows - base : PIECE;
-    const uint64_t g0 = (uint64_t)base * COLS;
-#pragma unroll
-    for (int k = 0; k < 4 * COLS; ++k) {
-      const uint32_t idx = tid + k * 1024;
-      if (idx < n * COLS) s_v[idx] = data[g0 + idx];
-    }
-    __syncthreads();
-    uint32_t sum[COLS];
-#pragma unroll
-    for (int c = 0; c < COLS; ++c) {
-      sum[c] = 0;
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const uint32_t row = tid * 4 + j;
-        sum[c] += row < n ? s_v[row * COLS + c] : 0u;
-      }
-    }
+    const uint32_t row0 = base + (uint32_t)tid * 4;
+    if (base + PIECE < rows) scan_rows_load<COLS>(data, rows, row0 + PIECE, nxt);
     uint32_t pre[COLS];
 #pragma unroll
     for (int c = 0; c < COLS; ++c) {
-      const uint32_t inc = wave_incl_add(sum[c]);
-      if (lane == 63) s_w[c][wid] = inc;
-      pre[c] = inc - sum[c];
+      const uint32_t sum = cur[c] + cur[COLS + c] + cur[2 * COLS + c] + cur[3 * COLS + c];
+      const uint32_t inc = wave_incl_add(sum);
+      if (lane == 63) s_w[buf][c][wid] = inc;
+      pre[c] = inc - sum;
     }
-    __syncthreads();
+    __syncthreads();  // the other buffer of s_w is free again: nobody reads it past this point
 #pragma unroll
     for (int c = 0; c < COLS; ++c) {
       uint32_t wp = 0, tot = 0;
 #pragma unroll
       for (int w = 0; w < 16; ++w) {
-        wp += w < wid ? s_w[c][w] : 0u;
-        tot += s_w[c][w];
+        const uint32_t x = s_w[buf][c][w];
+        wp += w < wid ? x : 0u;
+        tot += x;
       }
       uint32_t x = carry[c] + pre[c] + wp;
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const uint32_t row = tid * 4 + j;
-        if (row < n) {
-          const uint32_t v = s_v[row * COLS + c];
-          s_v[row * COLS + c] = x;
-          x += v;
-        }
+        const uint32_t y = cur[j * COLS + c];
+        cur[j * COLS + c] = x;
+        x += y;
       }
       carry[c] += tot;
     }
-    __syncthreads();
+    if (row0 + 4 <= rows) {
+      uint4* p = reinterpret_cast<uint4*>(data + (uint64_t)row0 * COLS);
 #pragma unroll
-    for (int k = 0; k < 4 * COLS; ++k) {
-      const uint32_t idx = tid + k * 1024;
-      if (idx < n * COLS) data[g0 + idx] = s_v[idx];
+      for (int k = 0; k < N / 4; ++k) p[k] = make_uint4(cur[4 * k], cur[4 * k + 1], cur[4 * k + 2], cur[4 * k + 3]);
+    } else {
+#pragma unroll
+      for (int k = 0; k < N; ++k)
+        if (row0 + k / COLS < rows) data[(uint64_t)row0 * COLS + k] = cur[k];
     }
-    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < N; ++k) cur[k] = nxt[k];
+    buf ^= 1;
   }
   if (tid == 0 && totals) {
 #pragma unroll

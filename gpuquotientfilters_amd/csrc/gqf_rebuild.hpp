@@ -132,11 +132,18 @@ __device__ __forceinline__ RS rs_block(RS a, RS* ex_out, uint4* s_wave) {
   __shared__ uint4 s_tot;
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   RS carry = rs_none();
+  uint4 w[SEG], wn[SEG];  // this piece's states and the next piece's (loaded ahead)
+#pragma unroll
+  for (uint32_t k = 0; k < SEG; ++k) {
+    const uint32_t i = (uint32_t)tid * SEG + k;
+    w[k] = i < G ? agg[i] : make_uint4(0u, 0u, 0u, 0u);
+  }
   for (uint32_t base = 0; base < G; base += PIECE) {
     const uint32_t i0 = base + (uint32_t)tid * SEG;
-    uint4 w[SEG];
+    if (base + PIECE < G) {
 #pragma unroll
-    for (uint32_t k = 0; k < SEG; ++k) w[k] = i0 + k < G ? agg[i0 + k] : make_uint4(0u, 0u, 0u, 0u);
+      for (uint32_t k = 0; k < SEG; ++k) wn[k] = i0 + PIECE + k < G ? agg[i0 + PIECE + k] : make_uint4(0u, 0u, 0u, 0u);
+    }
     RS seg = rs_none();
 #pragma unroll
     for (uint32_t k = 0; k < SEG; ++k) seg = rs_combine(seg, rs_unpack(w[k]));
@@ -162,6 +169,8 @@ __device__ __forceinline__ RS rs_block(RS a, RS* ex_out, uint4* s_wave) {
       }
     }
     carry = rs_unpack(s_tot);
+#pragma unroll
+    for (uint32_t k = 0; k < SEG; ++k) w[k] = wn[k];
     __syncthreads();
   }
   if (tid == 0) {

@@ -234,42 +234,57 @@ __device__ void rd_build_list(const RsqfDev& f, const uint32_t* __restrict__ wp,
   const uint32_t tid = threadIdx.x;
   for (uint32_t i = tid; i < RD_LIST; i += RD_THREADS) s_list[i] = RS_NOT_FOUND;
   const uint32_t R0 = wp[2 * (size_t)w + 1] + 1;
-  if (tid == 0) {
-    // window v holding the R0-th occupied quotient: the last one with O_v < R0
-    uint32_t v = w;
-    if (wp[2 * (size_t)w] >= R0) {
-      uint32_t lo = 0, hi = w;  // O_lo < R0 <= O_hi
-      while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (wp[2 * (size_t)mid] < R0) lo = mid;
-        else hi = mid;
-      }
-      v = lo;
+  // window v holding the R0-th occupied quotient: the last one with O_v < R0
+  // (O_0 = 0, so there is one).  Usually w itself or a near predecessor: the
+  // block checks the 256 windows before w at once, thread t window w-1-t,
+  // and only a run longer than that falls back to a binary search.
+  if (wp[2 * (size_t)w] < R0) {
+    if (tid == 0) {
+      s_v[0] = w;
+      s_v[1] = wp[2 * (size_t)w];
     }
-    s_v[0] = v;
-    s_v[1] = wp[2 * (size_t)v];
+  } else {
+    if (tid == 0) s_v[0] = 0xFFFFFFFFu;
+    __syncthreads();
+    if (tid < w && wp[2 * (size_t)(w - 1 - tid)] < R0) atomicMin(&s_v[0], tid);
+    __syncthreads();
+    if (tid == 0) {
+      uint32_t v;
+      if (s_v[0] != 0xFFFFFFFFu) {
+        v = w - 1 - s_v[0];
+      } else {
+        uint32_t lo = 0, hi = w - RD_THREADS;  // O_lo < R0 <= O_hi
+        while (hi - lo > 1) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (wp[2 * (size_t)mid] < R0) lo = mid;
+          else hi = mid;
+        }
+        v = lo;
+      }
+      s_v[0] = v;
+      s_v[1] = wp[2 * (size_t)v];
+    }
   }
   __syncthreads();
   const uint32_t v = s_v[0];
-  uint32_t base = s_v[1];  // occupied quotients before the current block row
-  const uint32_t bend = (w + 1) * RD_BLOCKS < f.nb ? (w + 1) * RD_BLOCKS : f.nb;
-  for (uint32_t bb = v * RD_BLOCKS; bb < bend && base < R0 + RD_LIST - 1; bb += RD_THREADS) {
-    const uint32_t b = bb + tid;
-    uint64_t occ = b < bend ? f.occ(b) : 0ull;
-    const uint32_t c = (uint32_t)__popcll(occ);
+  uint32_t base = s_v[1];  // occupied quotients before the current window
+  // one window of 64 blocks per step: thread t holds 16 occupied bits of
+  // block t/4, ranks them by a block scan and places each set bit's quotient
+  for (uint32_t x = v; x <= w && base < R0 + RD_LIST - 1; ++x) {
+    const uint32_t b = x * RD_BLOCKS + (tid >> 2), sh = 16 * (tid & 3);
+    const uint32_t o16 = b < f.nb ? (uint32_t)(f.occ(b) >> sh) & 0xFFFFu : 0u;
     uint32_t tot;
-    const uint32_t pre = block256_excl_scan(c, s_tmp4, &tot);
-    uint32_t g = base + pre + 1;  // rank of this block's first occupied quotient
-    if (g + c > R0 && g < R0 + RD_LIST) {
-      while (occ) {
-        const uint32_t bit = (uint32_t)__ffsll((long long)occ) - 1;
-        occ &= occ - 1;
-        if (g >= R0 && g - R0 < RD_LIST) s_list[g - R0] = b * 64 + bit;
-        ++g;
+    uint32_t g = base + block256_excl_scan((uint32_t)__popc(o16), s_tmp4, &tot) + 1;  // rank of the first set bit
+    if (g + (uint32_t)__popc(o16) > R0 && g < R0 + RD_LIST) {
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        if ((o16 >> j) & 1u) {
+          if (g >= R0 && g - R0 < RD_LIST) s_list[g - R0] = b * 64 + sh + (uint32_t)j;
+          ++g;
+        }
       }
     }
     base += tot;
-    __syncthreads();
   }
   __syncthreads();
 }
@@ -298,23 +313,34 @@ __device__ __forceinline__ uint32_t rd_slots(const RsqfDev& f, uint32_t e16, uin
   return m;
 }
 
+// Entries per window without the quotient list: slot s holds an entry iff
+// the k-th occupied quotient (k = 1 + run ends before s) is <= s, i.e. iff
+// D(s) = #(occupied bits <= s) - #(run ends < s) > 0 -- the runs opened up
+// to s and not yet closed.  One block scan of the two popcounts per thread
+// (packed in the halves of a word: a window has 4096 slots).
 __global__ __launch_bounds__(RD_THREADS) void k_rd_count(RsqfDev f, const uint32_t* __restrict__ wp,
                                                          uint32_t* __restrict__ wc) {
-  __shared__ uint32_t s_list[RD_LIST];
   __shared__ uint32_t s_tmp4[4];
-  __shared__ uint32_t s_v[2];
-  const uint32_t w = blockIdx.x;
-  rd_build_list(f, wp, w, s_list, s_tmp4, s_v);
-  uint32_t s0;
-  const uint32_t e16 = rd_ends16(f, w, &s0);
-  uint32_t tot;
-  const uint32_t rank0 = block256_excl_scan((uint32_t)__popc(e16), s_tmp4, &tot);
-  uint32_t qv[16];
-  const uint32_t m = rd_slots(f, e16, rank0, s0, s_list, qv);
-  __syncthreads();
+  const uint32_t w = blockIdx.x, t = threadIdx.x;
+  const uint32_t b = w * RD_BLOCKS + (t >> 2), sh = 16 * (t & 3);
+  uint32_t o16 = 0, e16 = 0;
+  if (b < f.nb) {
+    o16 = (uint32_t)(f.occ(b) >> sh) & 0xFFFFu;
+    e16 = (uint32_t)(f.ends(b) >> sh) & 0xFFFFu;
+  }
+  const uint32_t pre = block256_excl_scan(((uint32_t)__popc(o16) << 16) | (uint32_t)__popc(e16), s_tmp4, nullptr);
+  int32_t D = (int32_t)(wp[2 * (size_t)w] - wp[2 * (size_t)w + 1]) + (int32_t)(pre >> 16) - (int32_t)(pre & 0xFFFFu);
+  uint32_t n = 0;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    D += (int32_t)((o16 >> j) & 1u);
+    n += D > 0 ? 1u : 0u;
+    D -= (int32_t)((e16 >> j) & 1u);
+  }
+  if (b >= f.nb) n = 0;
   uint32_t total;
-  (void)block256_excl_scan((uint32_t)__popc(m), s_tmp4, &total);
-  if (threadIdx.x == 0) wc[w] = total;
+  (void)block256_excl_scan(n, s_tmp4, &total);
+  if (t == 0) wc[w] = total;
 }
 
 template <typename K, int R>
@@ -327,32 +353,39 @@ __global__ __launch_bounds__(RD_THREADS) void k_rd_emit(RsqfDev f, const uint32_
   uint32_t* s_list = reinterpret_cast<uint32_t*>(s_buf);
   const uint32_t w = blockIdx.x;
   const uint32_t rr = R ? (uint32_t)R : f.r;
-  rd_build_list(f, wp, w, s_list, s_tmp4, s_v);
+  // this thread's run ends and (8- and 16-bit remainders) remainder words are
+  // loaded first, so that their latency overlaps the list build
   uint32_t s0;
   const uint32_t e16 = rd_ends16(f, w, &s0);
+  const uint32_t b = s0 >> 6;
+  uint4 x0 = make_uint4(0u, 0u, 0u, 0u), x1 = x0;
+  if (b < f.nb) {
+    if (R == 8) {
+      x0 = *reinterpret_cast<const uint4*>(f.rem + s0);
+    } else if (R == 16) {
+      const uint4* p = reinterpret_cast<const uint4*>(f.rem + 2 * (size_t)s0);
+      x0 = p[0];
+      x1 = p[1];
+    }
+  }
+  rd_build_list(f, wp, w, s_list, s_tmp4, s_v);
   uint32_t tot;
   const uint32_t rank0 = block256_excl_scan((uint32_t)__popc(e16), s_tmp4, &tot);
   uint32_t qv[16];
   const uint32_t m = rd_slots(f, e16, rank0, s0, s_list, qv);
   // remainders of the 16 slots
   uint32_t rem[16];
-  const uint32_t b = s0 >> 6;
-  if (m) {
-    if (R == 8) {
-      const uint4 x = *reinterpret_cast<const uint4*>(f.rem + s0);
-      const uint32_t wd[4] = {x.x, x.y, x.z, x.w};
+  if (R == 8) {
+    const uint32_t wd[4] = {x0.x, x0.y, x0.z, x0.w};
 #pragma unroll
-      for (int j = 0; j < 16; ++j) rem[j] = (wd[j >> 2] >> (8 * (j & 3))) & 0xFFu;
-    } else if (R == 16) {
-      const uint4* p = reinterpret_cast<const uint4*>(f.rem + 2 * (size_t)s0);
-      const uint4 x0 = p[0], x1 = p[1];
-      const uint32_t wd[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+    for (int j = 0; j < 16; ++j) rem[j] = (wd[j >> 2] >> (8 * (j & 3))) & 0xFFu;
+  } else if (R == 16) {
+    const uint32_t wd[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
 #pragma unroll
-      for (int j = 0; j < 16; ++j) rem[j] = (wd[j >> 1] >> (16 * (j & 1))) & 0xFFFFu;
-    } else {
+    for (int j = 0; j < 16; ++j) rem[j] = (wd[j >> 1] >> (16 * (j & 1))) & 0xFFFFu;
+  } else if (m) {
 #pragma unroll
-      for (int j = 0; j < 16; ++j) rem[j] = (m >> j) & 1u ? f.get<0>(b, (s0 & 63) + (uint32_t)j) : 0u;
-    }
+    for (int j = 0; j < 16; ++j) rem[j] = (m >> j) & 1u ? f.get<0>(b, (s0 & 63) + (uint32_t)j) : 0u;
   }
   uint32_t cnt;
   const uint32_t pos = block256_excl_scan((uint32_t)__popc(m), s_tmp4, &cnt);
