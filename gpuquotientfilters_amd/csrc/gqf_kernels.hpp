@@ -211,9 +211,11 @@ __device__ __forceinline__ K sort_load_key(const void* in, uint64_t idx, uint32_
 // contiguous chunks (one workgroup each, ~2K chunks so every CU stays busy);
 // per 8-bit digit pass:
 //   upsweep   -- each chunk's digit histogram (one read of the keys),
-//   row scan  -- exclusive scan of every digit's chunk counts, then of the
-//                digit totals (tiny),
-//   downsweep -- each chunk walks its keys in 4096-key sub-tiles in order,
+//   row scan  -- exclusive scan of every digit's chunk counts and the digit
+//                totals,
+//   downsweep -- the digit bases (a scan of the 256 totals, in every
+//                workgroup), then each chunk walks its keys in 4096-key
+//                sub-tiles in order,
 //                ranks them stably with wave64 ballots, stages them in LDS by
 //                digit and writes runs of equal digits (second read + write).
 // No workgroup waits on another and no counter atomics are used: on MI355X a
@@ -303,20 +305,13 @@ __global__ __launch_bounds__(256) void k_rs_upsweep(const void* __restrict__ in,
   if (tid == 0) totals[blockIdx.x] = s_carry;
 }
 
-// totals[256] -> exclusive digit bases, in place.
-[[maybe_unused]] static __global__ __launch_bounds__(256) void k_rs_digitscan(uint32_t* __restrict__ totals) {
-  __shared__ uint32_t tmp[4];
-  const uint32_t x = totals[threadIdx.x];
-  totals[threadIdx.x] = block256_excl_scan(x, tmp, nullptr);
-}
-
 template <typename K, bool VALS, bool HASH>
 __global__ __launch_bounds__(256) void k_rs_downsweep(const void* __restrict__ in_keys,
                                                       const uint32_t* __restrict__ in_vals, K* __restrict__ out_keys,
                                                       uint32_t* __restrict__ out_vals, uint32_t n, uint32_t fp_bits,
                                                       uint32_t shift, uint32_t chunk, uint32_t G,
                                                       const uint32_t* __restrict__ hist,
-                                                      const uint32_t* __restrict__ digit_base) {
+                                                      const uint32_t* __restrict__ digit_totals) {
   __shared__ uint32_t s_run[256];
   __shared__ uint32_t s_wcnt[4 * 256];
   __shared__ uint32_t s_lstart[256];
@@ -324,7 +319,9 @@ __global__ __launch_bounds__(256) void k_rs_downsweep(const void* __restrict__ i
   __shared__ K s_keys[SORT_TILE];
   __shared__ uint32_t s_vals[VALS ? SORT_TILE : 1];
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  s_run[tid] = digit_base[tid] + hist[(uint64_t)tid * G + blockIdx.x];
+  // digit bases: exclusive scan of the 256 digit totals (every workgroup
+  // does it; it saves a launch)
+  s_run[tid] = block256_excl_scan(digit_totals[tid], s_tmp, nullptr) + hist[(uint64_t)tid * G + blockIdx.x];
   const uint64_t c0 = (uint64_t)blockIdx.x * chunk;
   const uint64_t c1 = c0 + chunk < n ? c0 + chunk : (uint64_t)n;
   const uint64_t lt = (1ull << lane) - 1ull;
@@ -451,7 +448,6 @@ gqf_status radix_sort(hipStream_t s, DeviceState& ds, Arena& ar, const void* in,
     }
     const double kin = first_hash ? 4.0 : (double)sizeof(K);
     GQF_LAUNCH(k_rs_rowscan, dim3(256), dim3(1024), 0, s, hist, plan.G, dbase);
-    GQF_LAUNCH(k_rs_digitscan, dim3(1), dim3(256), 0, s, dbase);
     K* dst = kb[cur];
     uint32_t* vdst = vb[cur];
     // read keys (+ values) once, write them once

@@ -629,7 +629,6 @@ __global__ __launch_bounds__(RB_THREADS, 6) void k_rr_write(uint8_t* __restrict_
     rr_load<K, MODE, MERGE>(A, B, flags, splits, u, (uint32_t)d0, tn, s_stage, v, fl);
     RS a;
     const uint32_t keepbits = rr_classify<K, MODE>(v, fl, tn, rr, mq, a);
-    if (tid == 0) s_b[0] = 0xFFFFFFFFu;
     RS ex;
     const RS sub = rs_block(a, &ex, s_wave);
     const RS me = rs_combine(X, ex);
@@ -659,8 +658,13 @@ __global__ __launch_bounds__(RB_THREADS, 6) void k_rr_write(uint8_t* __restrict_
           }
         }
         t_hi = (uint32_t)prev >> 6;
-        atomicMin(&s_b[0], t_lo);
-        if (ex.c == 0) s_b[1] = fq >> 6;  // the sub-tile's first kept entry
+        if (ex.c == 0) {
+          // the thread holding the sub-tile's first kept entry: its quotient,
+          // prev0 and lastf0 are the smallest of the sub-tile, so its t_lo is
+          // the sub-tile's first block
+          s_b[0] = t_lo;
+          s_b[1] = fq >> 6;
+        }
       }
       __syncthreads();
       const uint32_t B0 = s_b[0];
