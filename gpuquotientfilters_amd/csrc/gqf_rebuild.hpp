@@ -119,67 +119,80 @@ __device__ __forceinline__ RS rs_block(RS a, RS* ex_out, uint4* s_wave) {
   return rs_unpack(s_wave[RB_WAVES]);
 }
 
-// Exclusive scan of the chunk states by one workgroup of 1024 threads: in
-// pieces of 8192 states, each thread combines its 8 consecutive states, one
-// block-wide scan, then the thread rewrites its 8 states as exclusive
-// prefixes.  The total (also stored in *total when given) fixes the kept count, the first
-// slot after the last entry, and a capacity error -- all before any table
-// byte is written.
-[[maybe_unused]] static __global__ __launch_bounds__(1024) void k_rb_scan(uint4* __restrict__ agg, uint32_t G, uint64_t S,
-                                                         uint32_t* __restrict__ ctr, uint4* __restrict__ total) {
-  constexpr uint32_t SEG = 8, PIECE = 1024 * SEG;
-  __shared__ uint4 s_wave[16];  // wave aggregates, then their exclusive prefixes
-  __shared__ uint4 s_tot;
+// Exclusive scan of the G chunk states (G <= RB_MAX_CHUNKS) in two launches
+// of ceil(G / 1024) workgroups, one state per thread with coalesced loads:
+//   k_rb_scan_reduce  the aggregate of each workgroup's 1024 states,
+//   k_rb_scan_apply   each workgroup combines the aggregates before it (at
+//                     most 64, one wave) and rewrites its states as
+//                     exclusive prefixes; the last one writes the total.
+// The total (also stored in *total when given) fixes the kept count, the
+// first slot after the last entry, and a capacity error -- all before any
+// table byte is written.
+constexpr uint32_t RBS_PER = 1024;
+static_assert(RB_MAX_CHUNKS <= 64 * RBS_PER, "the apply kernel combines at most 64 aggregates in one wave");
+
+[[maybe_unused]] static __global__ __launch_bounds__(1024) void k_rb_scan_reduce(const uint4* __restrict__ agg, uint32_t G,
+                                                                                 uint4* __restrict__ part) {
+  __shared__ uint4 s_wave[16];
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  RS carry = rs_none();
-  uint4 w[SEG], wn[SEG];  // this piece's states and the next piece's (loaded ahead)
-#pragma unroll
-  for (uint32_t k = 0; k < SEG; ++k) {
-    const uint32_t i = (uint32_t)tid * SEG + k;
-    w[k] = i < G ? agg[i] : make_uint4(0u, 0u, 0u, 0u);
-  }
-  for (uint32_t base = 0; base < G; base += PIECE) {
-    const uint32_t i0 = base + (uint32_t)tid * SEG;
-    if (base + PIECE < G) {
-#pragma unroll
-      for (uint32_t k = 0; k < SEG; ++k) wn[k] = i0 + PIECE + k < G ? agg[i0 + PIECE + k] : make_uint4(0u, 0u, 0u, 0u);
-    }
-    RS seg = rs_none();
-#pragma unroll
-    for (uint32_t k = 0; k < SEG; ++k) seg = rs_combine(seg, rs_unpack(w[k]));
-    RS wt;
-    RS ex = rs_wave_excl(seg, &wt);
-    if (lane == 0) s_wave[wid] = rs_pack(wt);
-    __syncthreads();
-    if (wid == 0) {
-      // wave 0 scans the 16 wave aggregates on top of the carry
-      RS x = lane < 16 ? rs_unpack(s_wave[lane]) : rs_none();
-      RS tot;
-      RS xe = rs_wave_excl(x, &tot);
-      if (lane < 16) s_wave[lane] = rs_pack(rs_combine(carry, xe));
-      if (lane == 0) s_tot = rs_pack(rs_combine(carry, tot));
-    }
-    __syncthreads();
-    ex = rs_combine(rs_unpack(s_wave[wid]), ex);
-#pragma unroll
-    for (uint32_t k = 0; k < SEG; ++k) {
-      if (i0 + k < G) {
-        agg[i0 + k] = rs_pack(ex);
-        ex = rs_combine(ex, rs_unpack(w[k]));
-      }
-    }
-    carry = rs_unpack(s_tot);
-#pragma unroll
-    for (uint32_t k = 0; k < SEG; ++k) w[k] = wn[k];
-    __syncthreads();
-  }
-  if (tid == 0) {
-    if (total) *total = rs_pack(carry);
-    ctr[CTR_COUNT] = carry.c;
-    ctr[CTR_MISC + 3] = (uint32_t)(carry.lp < S ? carry.lp : S);  // the slot after the last entry
-    if (carry.lp > S) ctr[CTR_ERR] |= DEV_ERR_CAPACITY;
+  const uint32_t i = blockIdx.x * RBS_PER + (uint32_t)tid;
+  RS t;
+  (void)rs_wave_excl(i < G ? rs_unpack(agg[i]) : rs_none(), &t);
+  if (lane == 0) s_wave[wid] = rs_pack(t);
+  __syncthreads();
+  if (wid == 0) {
+    RS tot;
+    (void)rs_wave_excl(lane < 16 ? rs_unpack(s_wave[lane]) : rs_none(), &tot);
+    if (lane == 0) part[blockIdx.x] = rs_pack(tot);
   }
 }
 
+[[maybe_unused]] static __global__ __launch_bounds__(1024) void k_rb_scan_apply(uint4* __restrict__ agg, uint32_t G,
+                                                                                const uint4* __restrict__ part,
+                                                                                uint64_t S, uint32_t* __restrict__ ctr,
+                                                                                uint4* __restrict__ total) {
+  __shared__ uint4 s_wave[17];  // wave aggregates, then their prefixes; [16] = carry
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const uint32_t i = blockIdx.x * RBS_PER + (uint32_t)tid;
+  const RS x = i < G ? rs_unpack(agg[i]) : rs_none();
+  if (wid == 0) {
+    RS carry;
+    (void)rs_wave_excl((uint32_t)lane < blockIdx.x ? rs_unpack(part[lane]) : rs_none(), &carry);
+    if (lane == 0) s_wave[16] = rs_pack(carry);
+  }
+  RS wt;
+  const RS ex = rs_wave_excl(x, &wt);
+  if (lane == 0) s_wave[wid] = rs_pack(wt);
+  __syncthreads();
+  if (wid == 0) {
+    const RS carry = rs_unpack(s_wave[16]);
+    RS tot;
+    const RS xe = rs_wave_excl(lane < 16 ? rs_unpack(s_wave[lane]) : rs_none(), &tot);
+    if (lane < 16) s_wave[lane] = rs_pack(rs_combine(carry, xe));
+    if (lane == 0 && blockIdx.x + 1 == gridDim.x) {
+      const RS all = rs_combine(carry, tot);
+      if (total) *total = rs_pack(all);
+      ctr[CTR_COUNT] = all.c;
+      ctr[CTR_MISC + 3] = (uint32_t)(all.lp < S ? all.lp : S);  // the slot after the last entry
+      if (all.lp > S) ctr[CTR_ERR] |= DEV_ERR_CAPACITY;
+    }
+  }
+  __syncthreads();
+  if (i < G) agg[i] = rs_pack(rs_combine(rs_unpack(s_wave[wid]), ex));
+}
+
+// Host side: the two launches, with their partials taken from the arena.
+static inline gqf_status rb_scan(hipStream_t s, Arena& ar, uint4* agg, uint32_t G, uint64_t S, uint32_t* ctr,
+                                 uint4* total) {
+  const uint32_t P = (G + RBS_PER - 1) / RBS_PER;
+  uint4* part = ar.take<uint4>(P ? P : 1);
+  if (ar.used > ar.cap) {
+    set_error("internal: rebuild workspace overflow");
+    return GQF_ERR_INVALID;
+  }
+  GQF_LAUNCH(k_rb_scan_reduce, dim3(P), dim3(RBS_PER), 0, s, agg, G, part);
+  GQF_LAUNCH(k_rb_scan_apply, dim3(P), dim3(RBS_PER), 0, s, agg, G, part, S, ctr, total);
+  return GQF_OK;
+}
 
 }  // namespace gqf

@@ -402,7 +402,7 @@ __global__ __launch_bounds__(RD_THREADS) void k_rd_emit(RsqfDev f, const uint32_
 // ---------------------------------------------------------------- rebuild
 // Sorted fingerprints (build; the merge of the decoded table and a sorted
 // batch for insert; the decoded table with keep flags for delete) to the
-// canonical layout, through the chunked location scan (k_rr_agg, k_rb_scan)
+// canonical layout, through the chunked location scan (k_rr_agg, rb_scan)
 // and a writer that walks each sub-tile again on top of its prefix.
 //
 // The writer runs on a table cleared beforehand and leaves every field with
@@ -856,7 +856,7 @@ static gqf_status rsqf_rebuild(hipStream_t s, DeviceState& ds, Arena& ar, const 
                splits);
   GQF_LAUNCH_BC(nknown * per, na_dev, per, (k_rr_agg<K, MODE, MERGE>), dim3(p.G), dim3(RB_THREADS), 0, s, A, na,
                 na_dev, B, nbk, flags, splits, g.q, g.r, p.chunk, agg);
-  GQF_LAUNCH(k_rb_scan, dim3(1), dim3(1024), 0, s, agg, p.G, g.cap, ds.counters, total);
+  GQF_TRY(rb_scan(s, ar, agg, p.G, g.cap, ds.counters, total));
   GQF_LAUNCH_B((double)tbytes, k_rr_clear, dim3(1024), dim3(256), 0, s, reinterpret_cast<uint8_t*>(f.blocks),
                (uint64_t)tbytes, ds.counters);
   const double wbytes = nknown * per + (double)tbytes;

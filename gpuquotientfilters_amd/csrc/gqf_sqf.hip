@@ -119,7 +119,7 @@ static gqf_status launch_rebuild(hipStream_t s, DeviceState& ds, Arena& ar, cons
                splits);
   GQF_LAUNCH_BC(nknown * 4, na_dev, 4, (k_rb_agg<MODE, MERGE>), dim3(p.G), dim3(RB_THREADS), 0, s, A, na, na_dev, B,
                 nb, flags, splits, g.q, g.r, p.chunk, agg);
-  GQF_LAUNCH(k_rb_scan, dim3(1), dim3(1024), 0, s, agg, p.G, (uint64_t)g.S, ds.counters, (uint4*)nullptr);
+  GQF_TRY(rb_scan(s, ar, agg, p.G, (uint64_t)g.S, ds.counters, nullptr));
   // sparse tables (N * 8 < S): clear once, then sub-tiles skip windows holding
   // only zeros; otherwise sub-tiles write every slot of their range and the
   // tail is cleared after.  With the count on the device, the kernels decide.

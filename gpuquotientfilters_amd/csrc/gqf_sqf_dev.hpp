@@ -266,7 +266,7 @@ __global__ __launch_bounds__(256) void k_sqf_lookup_sorted(const uint8_t* __rest
 // (f: quotient of the last kept entry, which decides run heads) are combined
 // in chunk order:
 //   k_rb_agg    each chunk's state (one read of the inputs),
-//   k_rb_scan   exclusive scan over the chunks (one workgroup); the total
+//   rb_scan     exclusive scan over the chunks (two small launches); the total
 //               decides capacity before anything is written,
 //   k_rb_write  each chunk walks its 4096-entry sub-tiles with a running
 //               prefix; sub-tile u owns the slot range (L_{u-1}, L_u] ending at
