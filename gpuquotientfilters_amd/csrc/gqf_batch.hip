@@ -181,7 +181,7 @@ __global__ __launch_bounds__(RB_THREADS, 8) void k_bb_agg(const uint32_t* __rest
   const BFilter fl = flt[t.f];
   uint32_t v[RB_ITEMS + 1];
   uint32_t tags, fl_bits;
-  rb_load<MODE, false>(A + fl.key0, nullptr, nullptr, nullptr, 0u, t.d0, t.tn, nullptr, v, tags, fl_bits);
+  rb_load<MODE>(A + fl.key0, nullptr, t.d0, t.tn, v, tags, fl_bits);
   RS a;
   (void)rb_classify<MODE>(v, tags, fl_bits, t.d0, t.tn, fl.r, mask64(fl.q), a);
   RS ex;
@@ -294,7 +294,7 @@ __global__ __launch_bounds__(RB_THREADS, 4) void k_bb_write(const uint32_t* __re
   const BFilter fl = flt[t.f];
   uint32_t v[RB_ITEMS + 1];
   uint32_t tags, fl_bits;
-  rb_load<MODE, false>(A + fl.key0, nullptr, nullptr, nullptr, 0u, t.d0, t.tn, nullptr, v, tags, fl_bits);
+  rb_load<MODE>(A + fl.key0, nullptr, t.d0, t.tn, v, tags, fl_bits);
   RS a;
   const uint32_t keepbits = rb_classify<MODE>(v, tags, fl_bits, t.d0, t.tn, fl.r, mask64(fl.q), a);
   (void)rb_emit<BPE>(fl.table, fl.q, fl.r, v, keepbits, a, rs_unpack(pre[u]), u, s_wave, s_win, &s_nfix, fix,

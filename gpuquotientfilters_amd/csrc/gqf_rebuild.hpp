@@ -86,13 +86,15 @@ struct RbPlan {
   uint32_t G, chunk, nsub;  // chunks, entries per chunk (multiple of RB_TILE), sub-tiles
 };
 constexpr uint32_t RB_MAX_CHUNKS = 65536;
-static inline RbPlan rb_plan(uint64_t N) {
-  // one sub-tile per workgroup (latency is hidden by occupancy, not by a
-  // sequential walk), several per chunk only beyond RB_MAX_CHUNKS sub-tiles
+static inline RbPlan rb_plan(uint64_t N, uint32_t min_per = 1) {
+  // min_per sub-tiles per workgroup (more where the next sub-tile's input is
+  // staged while the current one is processed), more only beyond
+  // RB_MAX_CHUNKS chunks
   RbPlan p;
   p.nsub = (uint32_t)((N + RB_TILE - 1) / RB_TILE);
-  const uint32_t per = (p.nsub + RB_MAX_CHUNKS - 1) / RB_MAX_CHUNKS;
-  p.chunk = (per ? per : 1) * RB_TILE;
+  uint32_t per = (p.nsub + RB_MAX_CHUNKS - 1) / RB_MAX_CHUNKS;
+  if (per < min_per) per = min_per;
+  p.chunk = per * RB_TILE;
   p.G = (uint32_t)((N + p.chunk - 1) / p.chunk);
   if (p.G == 0) p.G = 1;
   return p;

@@ -102,7 +102,7 @@ static gqf_status launch_rebuild(hipStream_t s, DeviceState& ds, Arena& ar, cons
     clear_table(s, qf, g);
     return GQF_OK;
   }
-  const RbPlan p = rb_plan(N);
+  const RbPlan p = rb_plan(N, MERGE ? RB_MERGE_PER : 1u);
   uint32_t* splits = ar.take<uint32_t>(p.nsub + 1);
   uint4* agg = ar.take<uint4>(p.G);
   uint32_t* fix = ar.take<uint32_t>((size_t)p.nsub * RB_FIX);

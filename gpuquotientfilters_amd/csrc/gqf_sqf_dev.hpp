@@ -279,73 +279,33 @@ __global__ __launch_bounds__(256) void k_sqf_lookup_sorted(const uint8_t* __rest
 // (quotientFilter.cu:788-857, 985-1038, 1492-1539), the merge rebuild
 // (quotientFilter.cu:1651-1719) and the end states of insert /
 // superclusterDeletes.
-constexpr uint32_t RB_WIN_SLOTS = 8192;  // slots per write window
-constexpr uint32_t RB_FIX = 64;          // per-sub-tile occupied-bit fix-up slots
+constexpr uint32_t RB_MERGE_PER = 2;  // sub-tiles per workgroup in merge rebuilds (staged ahead)
+constexpr uint32_t RB_FIX = 64;       // per-sub-tile occupied-bit fix-up slots
+// Slots per write window: 8192, 4096 for 3- and 4-byte elements (the
+// window and the two merge stages then fit three workgroups per CU).
+template <int BPE>
+constexpr uint32_t rb_win_slots() {
+  return BPE <= 2 ? 8192u : 4096u;
+}
 // LDS words of a write window: its table bytes plus 16, so that window word 0
 // sits on a 16-byte boundary of the table.
 template <int BPE>
 constexpr uint32_t rb_win_words() {
-  return (RB_WIN_SLOTS * BPE + 16) / 4;
+  return (rb_win_slots<BPE>() * BPE + 16) / 4;
 }
 
-// This thread's RB_ITEMS consecutive entries of sub-tile [d0, d0+tn):
-// v[1+j] = entry j and v[0] = the entry before them; bit j of tags = entry j
-// comes from B; fl = delete flags of the entries (KEEP_FLAGS).  MERGE:
-// the sub-tile's A and B ranges are staged in LDS (s_stage) and each thread
-// merges from its own merge-path split; otherwise 16-byte loads.
-template <int MODE, bool MERGE>
-__device__ __forceinline__ void rb_load(const uint32_t* __restrict__ A, const uint32_t* __restrict__ B,
-                                        const uint8_t* __restrict__ flags, const uint32_t* __restrict__ splits,
-                                        uint32_t u, uint32_t d0, uint32_t tn, uint32_t* s_stage, uint32_t* v,
-                                        uint32_t& tags, uint32_t& fl) {
+// This thread's RB_ITEMS consecutive entries of sub-tile [d0, d0+tn) of A
+// (no merge): v[1+j] = entry j and v[0] = the entry before them; fl =
+// delete flags of the entries (KEEP_FLAGS); 16-byte loads.  Merge rebuilds
+// stage their inputs instead (rb_stage_issue / rb_merge_staged).
+template <int MODE>
+__device__ __forceinline__ void rb_load(const uint32_t* __restrict__ A, const uint8_t* __restrict__ flags,
+                                        uint32_t d0, uint32_t tn, uint32_t* v, uint32_t& tags, uint32_t& fl) {
   const int tid = threadIdx.x;
   const uint32_t i0 = (uint32_t)tid * RB_ITEMS;
   fl = 0;
   tags = 0;
-  if (MERGE) {
-    const uint32_t a0 = splits[u], a1 = splits[u + 1];
-    const uint32_t b0 = d0 - a0, b1 = d0 + tn - a1;
-    const uint32_t la = a1 - a0, lb = b1 - b0;  // la + lb == tn <= RB_TILE
-    {
-      uint32_t w[RB_ITEMS];
-#pragma unroll
-      for (int k = 0; k < RB_ITEMS; ++k) {
-        const uint32_t i = tid + k * RB_THREADS;
-        w[k] = i < la ? A[a0 + i] : (i < la + lb ? B[b0 + (i - la)] : 0u);
-      }
-#pragma unroll
-      for (int k = 0; k < RB_ITEMS; ++k) {
-        const uint32_t i = tid + k * RB_THREADS;
-        if (i < la + lb) s_stage[i] = w[k];
-      }
-    }
-    __syncthreads();
-    const uint32_t* sA = s_stage;
-    const uint32_t* sB = s_stage + la;
-    const uint32_t ld = i0 < tn ? i0 : tn;
-    uint32_t ai = merge_path(sA, la, sB, lb, ld);
-    uint32_t bi = ld - ai;
-    // the merged entry before position ld: the later (in merge order: value,
-    // then A before B) of the last A and the last B entry taken
-    const bool hasA = ai > 0 || a0 > 0, hasB = bi > 0 || b0 > 0;
-    const uint32_t pA = hasA ? (ai > 0 ? sA[ai - 1] : A[a0 - 1]) : 0u;
-    const uint32_t pB = hasB ? (bi > 0 ? sB[bi - 1] : B[b0 - 1]) : 0u;
-    v[0] = (hasB && (!hasA || pA <= pB)) ? pB : pA;
-#pragma unroll
-    for (int j = 0; j < RB_ITEMS; ++j) {
-      uint32_t x = 0;
-      if (ld + j < tn) {
-        const bool takeA = bi >= lb || (ai < la && sA[ai] <= sB[bi]);
-        if (takeA) {
-          x = sA[ai++];
-        } else {
-          x = sB[bi++];
-          tags |= 1u << j;
-        }
-      }
-      v[1 + j] = x;
-    }
-  } else {
+  {
     const uint64_t g0 = (uint64_t)d0 + i0;
     uint32_t w[RB_ITEMS];
     if (i0 + RB_ITEMS <= tn) {
@@ -377,6 +337,63 @@ __device__ __forceinline__ void rb_load(const uint32_t* __restrict__ A, const ui
     v[0] = g0 > 0 && i0 <= tn ? A[g0 - 1] : 0u;
 #pragma unroll
     for (int k = 0; k < RB_ITEMS; ++k) v[1 + k] = w[k];
+  }
+}
+
+// Double-buffered staging of the merge inputs (insert / merge rebuilds):
+// sub-tile u's A range [a0, a1) and B range are copied into an LDS buffer
+// with global->LDS loads, which take no VGPRs and run on while the
+// workgroup works on the previous sub-tile; the entries just before the
+// ranges (the first thread's previous entry) go to prev2[0] (A) and
+// prev2[1] (B).  rb_stage_wait() and a barrier make a buffer readable.
+__device__ __forceinline__ void rb_stage_issue(const uint32_t* __restrict__ A, const uint32_t* __restrict__ B,
+                                               uint32_t a0, uint32_t a1, uint32_t d0, uint32_t tn, uint32_t* buf,
+                                               uint32_t* prev2) {
+  const uint32_t la = a1 - a0, b0 = d0 - a0;
+  const uint32_t wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+  for (int k = 0; k < RB_ITEMS; ++k) {
+    // wave wid, step k: the 64 consecutive positions from base (lane l -> buf[base + l])
+    const uint32_t base = ((uint32_t)k * RB_WAVES + wid) * 64;
+    const uint32_t pos = base + lane;
+    if (pos < tn) {
+      const uint32_t* src = pos < la ? A + (a0 + pos) : B + (b0 + (pos - la));
+      __builtin_amdgcn_global_load_lds(src, buf + base, 4, 0, 0);
+    }
+  }
+  if (wid == 0 && lane < 2 && (lane == 0 ? a0 > 0 : b0 > 0))
+    __builtin_amdgcn_global_load_lds(lane == 0 ? A + (a0 - 1) : B + (b0 - 1), prev2, 4, 0, 0);
+}
+__device__ __forceinline__ void rb_stage_wait() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
+// This thread's RB_ITEMS merged entries of a staged sub-tile (as rb_load).
+__device__ __forceinline__ void rb_merge_staged(const uint32_t* buf, const uint32_t* prev2, uint32_t a0, uint32_t a1,
+                                                uint32_t d0, uint32_t tn, uint32_t* v, uint32_t& tags) {
+  const uint32_t i0 = (uint32_t)threadIdx.x * RB_ITEMS;
+  const uint32_t b0 = d0 - a0, la = a1 - a0, lb = tn - la;
+  const uint32_t* sA = buf;
+  const uint32_t* sB = buf + la;
+  const uint32_t ld = i0 < tn ? i0 : tn;
+  uint32_t ai = merge_path(sA, la, sB, lb, ld);
+  uint32_t bi = ld - ai;
+  const bool hasA = ai > 0 || a0 > 0, hasB = bi > 0 || b0 > 0;
+  const uint32_t pA = hasA ? (ai > 0 ? sA[ai - 1] : prev2[0]) : 0u;
+  const uint32_t pB = hasB ? (bi > 0 ? sB[bi - 1] : prev2[1]) : 0u;
+  v[0] = (hasB && (!hasA || pA <= pB)) ? pB : pA;
+  tags = 0;
+#pragma unroll
+  for (int j = 0; j < RB_ITEMS; ++j) {
+    uint32_t x = 0;
+    if (ld + j < tn) {
+      const bool takeA = bi >= lb || (ai < la && sA[ai] <= sB[bi]);
+      if (takeA) {
+        x = sA[ai++];
+      } else {
+        x = sB[bi++];
+        tags |= 1u << j;
+      }
+    }
+    v[1 + j] = x;
   }
 }
 
@@ -414,7 +431,8 @@ __global__ __launch_bounds__(RB_THREADS, 8) void k_rb_agg(const uint32_t* __rest
                                                 const uint8_t* __restrict__ flags, const uint32_t* __restrict__ splits,
                                                 uint32_t q, uint32_t r, uint32_t chunk,
                                                 uint4* __restrict__ agg) {
-  __shared__ uint32_t s_stage[MERGE ? RB_TILE : 1];
+  __shared__ uint32_t s_stage[MERGE ? 2 : 1][MERGE ? RB_TILE : 1];
+  __shared__ uint32_t s_prev[2][2];
   __shared__ uint4 s_wave[RB_WAVES + 1];
   if (na_dev) na = *na_dev;
   const uint64_t N = (uint64_t)na + (MERGE ? nb : 0u);
@@ -422,11 +440,42 @@ __global__ __launch_bounds__(RB_THREADS, 8) void k_rb_agg(const uint32_t* __rest
   const uint64_t c0 = (uint64_t)blockIdx.x * chunk;
   const uint64_t c1 = c0 + chunk < N ? c0 + chunk : N;
   RS X = rs_none();
+  // MERGE: the next sub-tile's inputs are staged while this one is processed
+  // splits: ca* of the current sub-tile, na* of the next one (loaded a
+  // sub-tile ahead, so that issuing its stage does not wait on them)
+  uint32_t ca0 = 0, ca1 = 0, na0 = 0, na1 = 0;
+  if (MERGE && c0 < c1) {
+    const uint32_t u0 = (uint32_t)(c0 / RB_TILE);
+    ca0 = splits[u0];
+    ca1 = splits[u0 + 1];
+    if (c0 + RB_TILE < c1) na1 = splits[u0 + 2];
+    na0 = ca1;
+    rb_stage_issue(A, B, ca0, ca1, (uint32_t)c0, (uint32_t)(c1 - c0 < RB_TILE ? c1 - c0 : RB_TILE), s_stage[0],
+                   s_prev[0]);
+  }
+  int buf = 0;
   for (uint64_t d0 = c0; d0 < c1; d0 += RB_TILE) {
     const uint32_t tn = (uint32_t)(c1 - d0 < (uint64_t)RB_TILE ? c1 - d0 : (uint64_t)RB_TILE);
     uint32_t v[RB_ITEMS + 1];
-    uint32_t tags, fl;
-    rb_load<MODE, MERGE>(A, B, flags, splits, (uint32_t)(d0 / RB_TILE), (uint32_t)d0, tn, s_stage, v, tags, fl);
+    uint32_t tags, fl = 0;
+    if (MERGE) {
+      const uint32_t u = (uint32_t)(d0 / RB_TILE);
+      const uint64_t dn = d0 + RB_TILE;
+      rb_stage_wait();
+      __syncthreads();
+      if (dn < c1)
+        rb_stage_issue(A, B, na0, na1, (uint32_t)dn, (uint32_t)(c1 - dn < RB_TILE ? c1 - dn : RB_TILE),
+                       s_stage[buf ^ 1], s_prev[buf ^ 1]);
+      const uint32_t nn1 = dn + RB_TILE < c1 ? splits[u + 3] : 0u;
+      rb_merge_staged(s_stage[buf], s_prev[buf], ca0, ca1, (uint32_t)d0, tn, v, tags);
+      ca0 = na0;
+      ca1 = na1;
+      na0 = na1;
+      na1 = nn1;
+      buf ^= 1;
+    } else {
+      rb_load<MODE>(A, flags, (uint32_t)d0, tn, v, tags, fl);
+    }
     RS a;
     (void)rb_classify<MODE>(v, tags, fl, (uint32_t)d0, tn, r, mq, a);
     RS ex;
@@ -505,8 +554,9 @@ __device__ __forceinline__ RS rb_emit(uint8_t* __restrict__ t, uint32_t q, uint3
   // (-1: none), lastf = its quotient (0xFFFFFFFF: none)
   const int32_t prev0 = (int32_t)me.lp - 1;
   const uint32_t lastf0 = me.fq1 - 1u;
-  for (uint32_t w0 = Llo; w0 < Lhi; w0 += RB_WIN_SLOTS) {
-    const uint32_t w1 = w0 + RB_WIN_SLOTS < Lhi ? w0 + RB_WIN_SLOTS : Lhi;
+  constexpr uint32_t WS = rb_win_slots<BPE>();
+  for (uint32_t w0 = Llo; w0 < Lhi; w0 += WS) {
+    const uint32_t w1 = w0 + WS < Lhi ? w0 + WS : Lhi;
     if (precleared) {
       bool any = false;
       int32_t prev = prev0;
@@ -584,7 +634,8 @@ __global__ __launch_bounds__(RB_THREADS, 6) void k_rb_write(uint8_t* __restrict_
                                                   uint32_t* __restrict__ fix_more_n, uint32_t fix_cap,
                                                   int precleared) {
   if (ctr[CTR_ERR]) return;  // capacity error: the table stays as it was
-  __shared__ uint32_t s_stage[MERGE ? RB_TILE : 1];
+  __shared__ uint32_t s_stage[MERGE ? 2 : 1][MERGE ? RB_TILE : 1];
+  __shared__ uint32_t s_prev[2][2];
   __shared__ __align__(16) uint32_t s_win[rb_win_words<BPE>()];
   __shared__ uint4 s_wave[RB_WAVES + 1];
   __shared__ uint32_t s_nfix;
@@ -595,12 +646,41 @@ __global__ __launch_bounds__(RB_THREADS, 6) void k_rb_write(uint8_t* __restrict_
   const uint64_t c0 = (uint64_t)blockIdx.x * chunk;
   const uint64_t c1 = c0 + chunk < N ? c0 + chunk : N;
   RS X = rs_unpack(pre[blockIdx.x]);
+  // splits: ca* of the current sub-tile, na* of the next one (loaded a
+  // sub-tile ahead, so that issuing its stage does not wait on them)
+  uint32_t ca0 = 0, ca1 = 0, na0 = 0, na1 = 0;
+  if (MERGE && c0 < c1) {
+    const uint32_t u0 = (uint32_t)(c0 / RB_TILE);
+    ca0 = splits[u0];
+    ca1 = splits[u0 + 1];
+    if (c0 + RB_TILE < c1) na1 = splits[u0 + 2];
+    na0 = ca1;
+    rb_stage_issue(A, B, ca0, ca1, (uint32_t)c0, (uint32_t)(c1 - c0 < RB_TILE ? c1 - c0 : RB_TILE), s_stage[0],
+                   s_prev[0]);
+  }
+  int buf = 0;
   for (uint64_t d0 = c0; d0 < c1; d0 += RB_TILE) {
     const uint32_t u = (uint32_t)(d0 / RB_TILE);
     const uint32_t tn = (uint32_t)(c1 - d0 < (uint64_t)RB_TILE ? c1 - d0 : (uint64_t)RB_TILE);
     uint32_t v[RB_ITEMS + 1];
-    uint32_t tags, fl;
-    rb_load<MODE, MERGE>(A, B, flags, splits, u, (uint32_t)d0, tn, s_stage, v, tags, fl);
+    uint32_t tags, fl = 0;
+    if (MERGE) {
+      const uint64_t dn = d0 + RB_TILE;
+      rb_stage_wait();
+      __syncthreads();
+      if (dn < c1)
+        rb_stage_issue(A, B, na0, na1, (uint32_t)dn, (uint32_t)(c1 - dn < RB_TILE ? c1 - dn : RB_TILE),
+                       s_stage[buf ^ 1], s_prev[buf ^ 1]);
+      const uint32_t nn1 = dn + RB_TILE < c1 ? splits[u + 3] : 0u;
+      rb_merge_staged(s_stage[buf], s_prev[buf], ca0, ca1, (uint32_t)d0, tn, v, tags);
+      ca0 = na0;
+      ca1 = na1;
+      na0 = na1;
+      na1 = nn1;
+      buf ^= 1;
+    } else {
+      rb_load<MODE>(A, flags, (uint32_t)d0, tn, v, tags, fl);
+    }
     RS a;
     const uint32_t keepbits = rb_classify<MODE>(v, tags, fl, (uint32_t)d0, tn, r, mq, a);
     X = rb_emit<BPE>(t, q, r, v, keepbits, a, X, u, s_wave, s_win, &s_nfix, fix, fix_cnt, fix_more,
