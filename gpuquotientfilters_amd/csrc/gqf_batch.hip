@@ -42,7 +42,7 @@ struct BFilter {
   uint32_t key0, nkeys;     // build keys [key0, key0 + nkeys) of the batch key array
   uint32_t probe0, nprobe;  // lookup keys
   uint32_t more0;           // this filter's fix-up overflow list (nkeys + 1 entries)
-  uint32_t pad;
+  uint32_t idx0;            // this filter's lookup index: ceil(S / 64) + 1 entries from here
 };
 struct BChunk {
   uint32_t begin, end;  // batch key indices
@@ -345,14 +345,77 @@ __global__ __launch_bounds__(64) void k_bb_fixup(const BTile* __restrict__ tl, c
   }
 }
 
+// ---------------------------------------------------------------- lookup index
+// Per 64-slot window of every filter: {occupied bits, run starts} in the
+// window, then an exclusive scan per filter, so that entry w holds the
+// counts before slot 64 w and the filter's last entry the totals
+// (sqf_lookup_indexed).  Thread i handles index entry i; its filter is the
+// last f with woff[f] <= i.
+template <int BPE>
+__global__ __launch_bounds__(256) void k_bi_count(const BFilter* __restrict__ flt, const uint32_t* __restrict__ woff,
+                                                  uint32_t F, uint2* __restrict__ idx) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= woff[F]) return;
+  uint32_t lo = 0, hi = F;  // woff[lo] <= i < woff[hi]
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (woff[mid] <= i) lo = mid;
+    else hi = mid;
+  }
+  const BFilter fl = flt[lo];
+  const uint32_t w = i - woff[lo];
+  uint32_t o = 0, r = 0;
+  const uint32_t s0 = w * 64, s1 = s0 + 64 < fl.S ? s0 + 64 : fl.S;  // the totals entry (s0 >= S) counts nothing
+  for (uint32_t s = s0; s < s1; ++s) {
+    const uint32_t e = sqf_get<BPE>(fl.table, fl.S, s);
+    o += (e >> 2) & 1u;
+    r += ((e & 7u) != 0 && !(e & 2u)) ? 1u : 0u;
+  }
+  idx[i] = make_uint2(o, r);
+}
+
+// Filter blockIdx.x: exclusive scan of its entries in place, 1024 at a time.
+__global__ __launch_bounds__(1024) void k_bi_scan(const uint32_t* __restrict__ woff, uint2* __restrict__ idx) {
+  __shared__ uint32_t s_w[2][16];
+  const uint32_t f = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const uint32_t b0 = woff[f], b1 = woff[f + 1];
+  uint32_t co = 0, cr = 0;  // carries
+  for (uint32_t base = b0; base < b1; base += 1024) {
+    const uint32_t i = base + tid;
+    const uint2 x = i < b1 ? idx[i] : make_uint2(0u, 0u);
+    const uint32_t io = wave_incl_add(x.x), ir = wave_incl_add(x.y);
+    if (lane == 63) {
+      s_w[0][wid] = io;
+      s_w[1][wid] = ir;
+    }
+    __syncthreads();
+    uint32_t po = 0, pr = 0, to = 0, tr = 0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const uint32_t a = s_w[0][k], c = s_w[1][k];
+      po += (uint32_t)k < wid ? a : 0u;
+      pr += (uint32_t)k < wid ? c : 0u;
+      to += a;
+      tr += c;
+    }
+    if (i < b1) idx[i] = make_uint2(co + po + io - x.x, cr + pr + ir - x.y);
+    co += to;
+    cr += tr;
+    __syncthreads();
+  }
+}
+
 // ---------------------------------------------------------------- lookups
 // Key i of the batch lookup array: its filter is the last f with
 // poff[f] <= i (binary search over F + 1 offsets), then the per-filter
-// lookup (unsorted path).  IPT keys per thread, strided by the workgroup.
+// lookup (unsorted path).  IPT keys per thread, strided by the workgroup.  A
+// walk that needs more than BB_WALK_BUDGET windows (a long cluster: hot keys
+// of a skewed batch) is finished through the filter's window index.
+constexpr uint32_t BB_WALK_BUDGET = 8;
 template <int BPE, bool WIN, int IPT>
 __global__ __launch_bounds__(1024) void k_bb_lookup(const BFilter* __restrict__ flt, const uint32_t* __restrict__ poff,
                                                     uint32_t F, const uint32_t* __restrict__ keys, uint32_t n,
-                                                    uint32_t* __restrict__ out) {
+                                                    uint32_t* __restrict__ out, const uint2* __restrict__ idx) {
   const uint32_t base = blockIdx.x * blockDim.x * IPT + threadIdx.x;
   for (int j = 0; j < IPT; ++j) {
     const uint32_t i = base + (uint32_t)j * blockDim.x;
@@ -365,8 +428,10 @@ __global__ __launch_bounds__(1024) void k_bb_lookup(const BFilter* __restrict__ 
       }
       const BFilter fl = flt[lo];
       const uint32_t fp = fingerprint32(keys[i], fl.bits);
-      out[i] = sqf_lookup_fp<BPE, WIN>(fl.table, fl.S, (fp >> fl.r) & (uint32_t)mask64(fl.q),
-                                       fp & ((1u << fl.r) - 1u));
+      const uint32_t fq = (fp >> fl.r) & (uint32_t)mask64(fl.q), fr = fp & ((1u << fl.r) - 1u);
+      uint32_t res = sqf_lookup_fp<BPE, WIN>(fl.table, fl.S, fq, fr, BB_WALK_BUDGET);
+      if (res == SQF_GIVE_UP) res = sqf_lookup_indexed<BPE>(fl.table, fl.S, idx + fl.idx0, fq, fr);
+      out[i] = res;
     }
   }
 }
@@ -394,6 +459,9 @@ struct SqfBatch {
   uint4* d_agg = nullptr;
   uint32_t *d_fix = nullptr, *d_fixcnt = nullptr, *d_more = nullptr, *d_moren = nullptr;
   uint32_t *d_ferr = nullptr, *d_fend = nullptr;
+  uint32_t* d_woff = nullptr;  // lookup index offsets (F + 1)
+  uint2* d_idx = nullptr;      // lookup index
+  uint32_t nidx = 0;
   // plan
   uint32_t G = 0;       // sort chunks of the chosen shape
   int sort_per = 1;     // sub-tiles per sort chunk
@@ -440,6 +508,17 @@ static gqf_status batch_upload_chunks(SqfBatch& b, int per) {
     }                             \
   } while (0)
 
+// The lookup index of every filter, from its finished table.
+static void batch_index_launch(SqfBatch& b) {
+  hipStream_t s = b.stream;
+  const dim3 grid((b.nidx + 255) / 256);
+#define GQF_BI_COUNT(BPE_) \
+  GQF_LAUNCH_B((double)b.nidx * 64 * (BPE_) + (double)b.nidx * 8, k_bi_count<BPE_>, grid, dim3(256), 0, s, b.d_flt, b.d_woff, (uint32_t)b.F, b.d_idx)
+  GQF_BB_BPE(b.bpe, GQF_BI_COUNT);
+#undef GQF_BI_COUNT
+  GQF_LAUNCH(k_bi_scan, dim3((unsigned)b.F), dim3(1024), 0, s, b.d_woff, b.d_idx);
+}
+
 // The build pipeline: kernel launches only (no host synchronisation), so it
 // can be captured.
 static void batch_build_launch(SqfBatch& b) {
@@ -448,6 +527,7 @@ static void batch_build_launch(SqfBatch& b) {
   GQF_LAUNCH(k_bb_reset, dim3(64), dim3(256), 0, s, b.d_ferr, 3 * F);  // ferr, fend, more_n
   if (b.N == 0) {
     GQF_LAUNCH(k_bb_clear_tail, dim3(64, F), dim3(256), 0, s, b.d_flt, b.d_ferr, b.d_fend, b.bpe);
+    batch_index_launch(b);
     return;
   }
   const uint32_t* src = b.d_keys;
@@ -497,6 +577,7 @@ static void batch_build_launch(SqfBatch& b) {
   GQF_LAUNCH(k_bb_clear_tail, dim3(64, F), dim3(256), 0, s, b.d_flt, b.d_ferr, b.d_fend, b.bpe);
   GQF_LAUNCH(k_bb_fixup, dim3(b.nsub + F), dim3(64), 0, s, b.d_tl, b.d_flt, b.d_ferr, b.d_fix, b.d_fixcnt, b.d_more,
              b.d_moren, b.nsub, b.bpe);
+  batch_index_launch(b);
 }
 
 static void batch_lookup_launch(SqfBatch& b) {
@@ -506,7 +587,7 @@ static void batch_lookup_launch(SqfBatch& b) {
   const double bytes = (double)b.P * (8 + 64);
 #define GQF_BB_LK(BPE_, WIN_, IPT_)                                                                                 \
   GQF_LAUNCH_B(bytes, (k_bb_lookup<BPE_, WIN_, IPT_>), grid, block, 0, b.stream, b.d_flt, b.d_poff, (uint32_t)b.F, \
-               b.d_probe, b.P, b.d_out)
+               b.d_probe, b.P, b.d_out, b.d_idx)
 #define GQF_BB_LK_IPT(BPE_, WIN_)           \
   do {                                      \
     if (b.lk_ipt == 1) GQF_BB_LK(BPE_, WIN_, 1); \
@@ -626,6 +707,8 @@ static gqf_status batch_create(SqfBatch* b, const gqf_sqf* filters, int count, i
   b->d_probe = d_probe;
   b->d_out = d_out;
   uint32_t maxbits = 1;
+  uint64_t nidx = 0;
+  std::vector<uint32_t> woff(1, 0u);
   std::vector<BTile> tl;
   std::vector<BFilter> hf((size_t)count);
   for (int f = 0; f < count; ++f) {
@@ -642,7 +725,9 @@ static gqf_status batch_create(SqfBatch* b, const gqf_sqf* filters, int count, i
     x.probe0 = (uint32_t)b->poff[(size_t)f];
     x.nprobe = (uint32_t)(b->poff[(size_t)f + 1] - b->poff[(size_t)f]);
     x.more0 = x.key0 + (uint32_t)f;
-    x.pad = 0;
+    x.idx0 = (uint32_t)nidx;
+    nidx += (x.S + 63) / 64 + 1;
+    woff.push_back((uint32_t)nidx);
     maxbits = std::max(maxbits, q.qbits + q.rbits);
     for (uint32_t d0 = 0; d0 < x.nkeys; d0 += RB_TILE)
       tl.push_back({(uint32_t)f, d0, std::min<uint32_t>(RB_TILE, x.nkeys - d0), d0 == 0 ? 1u : 0u});
@@ -665,7 +750,8 @@ static gqf_status batch_create(SqfBatch* b, const gqf_sqf* filters, int count, i
                o_dbase = take((size_t)4 * 256 * count), o_k0 = take((size_t)4 * b->N), o_k1 = take((size_t)4 * b->N),
                o_agg = take((size_t)16 * b->nsub), o_fix = take((size_t)4 * RB_FIX * b->nsub),
                o_fixcnt = take((size_t)4 * b->nsub), o_more = take((size_t)4 * ((size_t)b->N + count)),
-               o_state = take((size_t)4 * 3 * count);
+               o_state = take((size_t)4 * 3 * count), o_woff = take(4 * ((size_t)count + 1)),
+               o_idx = take((size_t)8 * nidx);
   GQF_HIP(hipMalloc((void**)&b->mem, off));
   char* m = b->mem;
   b->d_flt = reinterpret_cast<BFilter*>(m + o_flt);
@@ -685,12 +771,16 @@ static gqf_status batch_create(SqfBatch* b, const gqf_sqf* filters, int count, i
   b->d_ferr = reinterpret_cast<uint32_t*>(m + o_state);
   b->d_fend = b->d_ferr + count;
   b->d_moren = b->d_fend + count;
+  b->d_woff = reinterpret_cast<uint32_t*>(m + o_woff);
+  b->d_idx = reinterpret_cast<uint2*>(m + o_idx);
+  b->nidx = (uint32_t)nidx;
   std::vector<uint32_t> poff32((size_t)count + 1);
   for (int f = 0; f <= count; ++f) poff32[(size_t)f] = (uint32_t)b->poff[(size_t)f];
   GQF_HIP(hipMemcpyAsync(b->d_flt, hf.data(), hf.size() * sizeof(BFilter), hipMemcpyHostToDevice, b->stream));
   if (!tl.empty())
     GQF_HIP(hipMemcpyAsync(b->d_tl, tl.data(), tl.size() * sizeof(BTile), hipMemcpyHostToDevice, b->stream));
   GQF_HIP(hipMemcpyAsync(b->d_poff, poff32.data(), poff32.size() * 4, hipMemcpyHostToDevice, b->stream));
+  GQF_HIP(hipMemcpyAsync(b->d_woff, woff.data(), woff.size() * 4, hipMemcpyHostToDevice, b->stream));
   b->h_err.assign((size_t)count, 0);
   // autotune the sort chunk size on the build (the tables are rebuilt from
   // the batch keys each time: the last run leaves them built)

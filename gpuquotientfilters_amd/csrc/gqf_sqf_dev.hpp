@@ -43,20 +43,33 @@ __host__ __device__ __forceinline__ uint32_t sqf_get(const uint8_t* __restrict__
   return e;
 }
 
+// Returned by a walk that ran out of its budget (never a slot index: slots
+// are below 2^29).
+constexpr uint32_t SQF_GIVE_UP = 0xFFFFFFFEu;
+
 // findRunStartGPU (quotientFilter.cu:172-196) + run scan of hashAndLookUp
 // (quotientFilter.cu:333-356): slot of the first matching remainder.
+// budget: at most 64 * budget slot steps, then SQF_GIVE_UP.
 template <int BPE>
-__host__ __device__ __forceinline__ uint32_t sqf_lookup_one(const uint8_t* __restrict__ t, uint32_t S, uint32_t fq, uint32_t fr) {
+__host__ __device__ __forceinline__ uint32_t sqf_lookup_one(const uint8_t* __restrict__ t, uint32_t S, uint32_t fq,
+                                                           uint32_t fr, uint32_t budget = 0xFFFFFFFFu) {
   if (!(sqf_get<BPE>(t, S, fq) & 4u)) return SQF_NOT_FOUND;
+  const uint64_t lim = (uint64_t)budget * 64;
+  uint64_t steps = 0;
   uint32_t b = fq;
-  while (b > 0 && (sqf_get<BPE>(t, S, b) & 1u)) b--;
+  while (b > 0 && (sqf_get<BPE>(t, S, b) & 1u)) {
+    b--;
+    if (++steps > lim) return SQF_GIVE_UP;
+  }
   uint32_t s = b;
   while (b != fq) {
     do {
       s++;
+      if (++steps > lim) return SQF_GIVE_UP;
     } while ((sqf_get<BPE>(t, S, s) & 2u) && s < S);
     do {
       b++;
+      if (++steps > lim) return SQF_GIVE_UP;
     } while (!(sqf_get<BPE>(t, S, b) & 4u) && b < S);
   }
   uint32_t e = sqf_get<BPE>(t, S, s);
@@ -66,6 +79,7 @@ __host__ __device__ __forceinline__ uint32_t sqf_lookup_one(const uint8_t* __res
     if (rem > fr) return SQF_NOT_FOUND;
     s++;
     e = sqf_get<BPE>(t, S, s);
+    if (++steps > lim) return SQF_GIVE_UP;
   } while (e & 2u);
   return SQF_NOT_FOUND;
 }
@@ -163,9 +177,11 @@ __host__ __device__ __forceinline__ uint64_t sqf_win_ge(const SqfWin<BPE>& W, ui
   return m;
 }
 
+// budget: at most that many window loads after the first, then SQF_GIVE_UP.
 template <int BPE>
 __host__ __device__ __forceinline__ uint32_t sqf_lookup_win(const uint8_t* __restrict__ t, uint32_t S, uint32_t fq,
-                                                   uint32_t fr) {
+                                                   uint32_t fr, uint32_t budget = 0xFFFFFFFFu) {
+  uint32_t loads = 0;
   SqfWin<BPE> W;
   uint32_t cw = fq >> 6;
   const uint32_t rel = fq & 63u;
@@ -188,6 +204,7 @@ __host__ __device__ __forceinline__ uint32_t sqf_lookup_win(const uint8_t* __res
       k -= (uint32_t)(W.occ & 1ull);
       break;
     }
+    if (++loads > budget) return SQF_GIVE_UP;
     --cw;
     sqf_win_load<BPE>(t, S, cw, W);
     lim = ~0ull;
@@ -204,6 +221,7 @@ __host__ __device__ __forceinline__ uint32_t sqf_lookup_win(const uint8_t* __res
       break;
     }
     k -= c;
+    if (++loads > budget) return SQF_GIVE_UP;
     ++cw;
     sqf_win_load<BPE>(t, S, cw, W);
     from = ~0ull;
@@ -220,6 +238,7 @@ __host__ __device__ __forceinline__ uint32_t sqf_lookup_win(const uint8_t* __res
       return ((sqf_win_ge<BPE>(W, fr + 1) >> p) & 1ull) ? SQF_NOT_FOUND : cw * 64u + p;
     }
     if (stop) return SQF_NOT_FOUND;
+    if (++loads > budget) return SQF_GIVE_UP;
     ++cw;
     sqf_win_load<BPE>(t, S, cw, W);
     run_from = 0;
@@ -229,9 +248,64 @@ __host__ __device__ __forceinline__ uint32_t sqf_lookup_win(const uint8_t* __res
 
 template <int BPE, bool WIN>
 __device__ __forceinline__ uint32_t sqf_lookup_fp(const uint8_t* __restrict__ t, uint32_t S, uint32_t fq,
-                                                  uint32_t fr) {
-  if (WIN) return sqf_lookup_win<BPE>(t, S, fq, fr);
-  return sqf_lookup_one<BPE>(t, S, fq, fr);
+                                                  uint32_t fr, uint32_t budget = 0xFFFFFFFFu) {
+  if (WIN) return sqf_lookup_win<BPE>(t, S, fq, fr, budget);
+  return sqf_lookup_one<BPE>(t, S, fq, fr, budget);
+}
+
+// Slot of the k-th run start (1-based, counting from slot 0) through the
+// window index, searching windows from vmin on; S when the table holds fewer
+// than k runs (or the index disagrees with the table).
+template <int BPE>
+__device__ uint32_t sqf_select_run(const uint8_t* __restrict__ t, uint32_t S, const uint2* __restrict__ idx,
+                                   uint32_t k, uint32_t vmin) {
+  const uint32_t nwin = (S + 63) / 64;
+  if (k == 0 || idx[nwin].y < k || idx[vmin].y >= k) return S;
+  uint32_t lo = vmin, hi = nwin;  // idx[lo].y < k <= idx[hi].y
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (idx[mid].y < k) lo = mid;
+    else hi = mid;
+  }
+  uint32_t need = k - idx[lo].y;
+  const uint32_t end = lo * 64 + 64 < S ? lo * 64 + 64 : S;
+  for (uint32_t s = lo * 64; s < end; ++s) {
+    const uint32_t e = sqf_get<BPE>(t, S, s);
+    if ((e & 7u) != 0 && !(e & 2u) && --need == 0) return s;  // occupied and not a continuation
+  }
+  return S;
+}
+
+// The same result as the walk, through a window index, for walks through
+// long clusters.  idx[w] = {occupied quotients, run starts} in the slots
+// before 64 w, for w = 0 .. ceil(S / 64) (the last entry: the totals).  In a
+// canonical table the k-th run start (from slot 0) begins the run of the
+// k-th occupied quotient, so fq's run [s, N) is bracketed by two index
+// searches (N: the next run start).  Over (s, N] the predicate "no
+// continuation bit, or remainder >= fr" is monotone (the run's remainders
+// ascend, and only empty slots lie between a run's end and the next run), so
+// the first remainder >= fr is found by a binary search.  fq must be
+// occupied.
+template <int BPE>
+__device__ uint32_t sqf_lookup_indexed(const uint8_t* __restrict__ t, uint32_t S, const uint2* __restrict__ idx,
+                                       uint32_t fq, uint32_t fr) {
+  const uint32_t w = fq >> 6;
+  uint32_t k = idx[w].x;  // 1-based rank of fq among the occupied quotients
+  for (uint32_t s = w * 64; s <= fq; ++s) k += (sqf_get<BPE>(t, S, s) >> 2) & 1u;
+  // a run of a quotient >= 64 w starts at or after slot 64 w
+  const uint32_t s = sqf_select_run<BPE>(t, S, idx, k, w);
+  if (s >= S) return SQF_NOT_FOUND;
+  const uint32_t e0 = sqf_get<BPE>(t, S, s);
+  if ((e0 >> 3) >= fr) return (e0 >> 3) == fr ? s : SQF_NOT_FOUND;
+  uint32_t a = s, b = sqf_select_run<BPE>(t, S, idx, k + 1, s >> 6);  // P(a) false, P(b) true
+  while (b - a > 1) {
+    const uint32_t m = a + (b - a) / 2;
+    const uint32_t e = sqf_get<BPE>(t, S, m);
+    if (!(e & 2u) || (e >> 3) >= fr) b = m;
+    else a = m;
+  }
+  const uint32_t e = sqf_get<BPE>(t, S, b);  // slot S reads as empty
+  return ((e & 2u) && (e >> 3) == fr) ? b : SQF_NOT_FOUND;
 }
 
 template <int BPE, bool WIN>

@@ -156,3 +156,49 @@ def test_batch_capacity_empty_and_wide_slots():
         assert (t == want[i][1]).all(), i
         assert (o[int(po[i]):int(po[i + 1])] == O.sqf_lookup_keys(want[i][1], q, r, p)).all(), i
     b.destroy()
+
+
+def _hot_batch(F, q, r, seed):
+    """Per filter: uniform keys at fill 0.45 plus one hot key repeated 0.3 S
+    times (Dup mode), the hot quotient in the first half of the table so the
+    run fits.  Probes: the hot key, stored keys whose quotients lie in the
+    run's shadow (their walk skips the whole hot run), other stored keys and
+    misses."""
+    rng = np.random.default_rng(seed)
+    S = G.SQF.num_slots(q)
+    keys, probe = [], []
+    for _ in range(F):
+        base = rng.integers(0, 2 ** 32, int(0.45 * S), dtype=np.uint32)
+        cand = rng.integers(0, 2 ** 32, 64, dtype=np.uint32)
+        cq = (O.fingerprints(cand, q, r).astype(np.uint64) >> r).astype(np.int64)
+        hot = cand[np.argmax(cq < (1 << (q - 1)))]
+        hq = int(O.fingerprints(np.array([hot], np.uint32), q, r)[0]) >> r
+        k = np.concatenate([base, np.full(int(0.3 * S), hot, np.uint32)])
+        bq = (O.fingerprints(base, q, r).astype(np.uint64) >> r).astype(np.int64)
+        shadow = base[(bq > hq) & (bq < hq + int(0.3 * S) + 64)]
+        probe.append(np.concatenate([np.full(8, hot, np.uint32), shadow, rng.choice(base, 200),
+                                     rng.integers(0, 2 ** 32, 200, dtype=np.uint32)]))
+        keys.append(k)
+    return [q] * F, keys, probe
+
+
+@pytest.mark.parametrize("q,r", [(14, 5), (13, 13), (11, 21)])
+def test_batch_long_clusters_lookup_through_the_index(q, r):
+    """Lookups whose walk would cross a run of thousands of slots finish
+    through the batch's window index; the results equal the per-filter calls
+    (plain walk) and the oracle."""
+    qs, keys, probe = _hot_batch(6, q, r, seed=11 + q)
+    say("sqf batch long clusters", q, r, [len(p) for p in probe])
+    filters = [G.SQF(q, r) for q in qs]
+    dk, dp = dev(flat(keys)), dev(flat(probe))
+    out = G.DeviceArray.empty(len(flat(probe)))
+    b = G.SQFBatch(filters, dk, offsets(keys), dp, offsets(probe), out, no_dup=False)
+    ref_tables, ref_looks, ref_status = per_filter(qs, r, keys, probe, False)
+    assert all(s == G.OK for s in ref_status)
+    for use_graph in (True, False):
+        out.fill(0)
+        ms, st = b.build(use_graph=use_graph)
+        assert st == ref_status
+        b.lookup(use_graph=use_graph)
+        check_batch(qs, r, keys, probe, False, filters, out, ref_tables, ref_looks, ref_status)
+    b.destroy()
