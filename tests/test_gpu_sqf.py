@@ -246,6 +246,32 @@ def test_deterministic_rebuild():
     assert (a.table_bytes() == b.table_bytes()).all()
 
 
+def test_large_rebuilds_equal_canonical():
+    """q=24 (up to 13 M entries, more than 1024 rebuild chunks, so the chunk
+    scan combines several workgroups): build, two merge inserts of 2^21
+    keys and a delete, each against the oracle's canonical layout."""
+    q, r = 24, 5
+    n = O.key_count(0.8, q)
+    keys = O.mt19937(41, n)
+    half = O.key_count(0.5, q)
+    say("sqf large rebuilds", q, r, n)
+    qf = G.SQF(q, r)
+    qf.bulk_build(dev(keys[:half]), NoDuplicates=True)
+    assert (qf.table_bytes() == sqf_canon(keys[:half], q, r, True)).all()
+    end = half
+    for _ in range(2):
+        qf.insert(dev(keys[end:end + (1 << 21)]))
+        end += 1 << 21
+    assert (qf.table_bytes() == sqf_canon(keys[:end], q, r, True)).all()
+    # NoDup table: one entry per distinct fingerprint; deleting a key removes
+    # its fingerprint's entry
+    d = keys[: 1 << 20]
+    qf.delete(dev(d))
+    left = np.setdiff1d(np.unique(O.fingerprints(keys[:end], q, r)), O.fingerprints(d, q, r))
+    st, want = O.sqf_canon(left, q, r, True)
+    assert st == O.OK and (qf.table_bytes() == want).all()
+
+
 def test_full_size_q26_properties():
     """Full size (q=26, r=5, fill 0.8): checks that need no oracle
     at this size -- no false negatives, FPR band, sorted == unsorted, extract
