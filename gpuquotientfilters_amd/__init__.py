@@ -50,6 +50,7 @@ class _RSQF(C.Structure):
 
 OP_LOOKUP, OP_INSERT, OP_DELETE = 0, 1, 2
 HASH_AP, HASH_FNV = 0, 1
+INSERT_AUTO, INSERT_REBUILD, INSERT_LOCAL = 0, 1, 2
 IMAGE_SQF_GPU, IMAGE_SQF_CPU, IMAGE_RSQF = 1, 2, 3
 
 
@@ -124,6 +125,8 @@ _SIGS = {
     "gqf_rsqf_image_load": (_st, [_RSQF, _vp, C.c_size_t]),
     "gqf_sqf_set_hash": (_st, [C.c_int]),
     "gqf_sqf_get_hash": (C.c_int, []),
+    "gqf_sqf_set_insert_path": (_st, [C.c_int]),
+    "gqf_sqf_get_insert_path": (C.c_int, []),
     "gqf_stream_create": (_st, [C.POINTER(_vp)]),
     "gqf_stream_destroy": (_st, [_vp]),
     "gqf_stream_workspace_bytes": (_st, [_vp, C.POINTER(C.c_size_t)]),
@@ -232,6 +235,17 @@ def set_sqf_hash(kind):
     """Hash of this thread's SQF calls: HASH_AP (default, the reference GPU
     code's) or HASH_FNV (the reference CPU_version's)."""
     check(lib().gqf_sqf_set_hash(int(kind)))
+
+
+def set_sqf_insert_path(path):
+    """How this thread's SQF inserts update the table: INSERT_AUTO (default),
+    INSERT_REBUILD (every slot) or INSERT_LOCAL (only the regions a batch
+    changes).  The resulting tables are the same."""
+    check(lib().gqf_sqf_set_insert_path(int(path)))
+
+
+def sqf_insert_path():
+    return lib().gqf_sqf_get_insert_path()
 
 
 def sqf_hash():

@@ -13,6 +13,8 @@
 // Lookups are the reference's run walk (quotientFilter.cu:172-196, 320-357).
 #include "gqf_sqf_dev.hpp"
 
+#include <algorithm>
+
 #include <cstring>
 
 namespace gqf {
@@ -180,6 +182,60 @@ static gqf_status sqf_merge_into(CallScope& cs, Arena& ar, const gqf_sqf& qf, co
   return cs.end(ms, true);  // a layout that does not fit is detected before any write
 }
 
+// Workspace of the localized insert of nb keys.
+static size_t sqf_local_bytes(const SqfGeom& g, uint32_t nb) {
+  const uint64_t R = (g.S + PI_REGION - 1) / PI_REGION, W = ((uint64_t)g.S + 63) / 64;
+  const uint64_t G = ((uint64_t)nb + RB_TILE - 1) / RB_TILE;
+  return aligned_bytes(W) + aligned_bytes((R + 1) * 4) + 256 + 4 * aligned_bytes((uint64_t)nb * 4) +
+         aligned_bytes(nb) + aligned_bytes(G * 16) + aligned_bytes(CTR_TOTAL * 4) + 256 + aligned_bytes((uint64_t)g.S * 4) +
+         aligned_bytes(((G + RBS_PER - 1) / RBS_PER) * 16) + 2048;
+}
+
+// The localized insert (gqf_sqf_dev.hpp, "localized insert") of the sorted
+// fingerprints B.  *done: the table was written; otherwise nothing was
+// written and the call needs the full rebuild (a region too long, or a key
+// past the last empty slot, where the full rebuild decides capacity).
+template <int BPE>
+static gqf_status sqf_insert_local(CallScope& cs, Arena& ar, const gqf_sqf& qf, const SqfGeom& g, const uint32_t* B,
+                                   uint32_t nb, bool* done) {
+  hipStream_t s = cs.stream;
+  DeviceState& ds = *cs.ds;
+  *done = false;
+  const uint32_t R = (g.S + PI_REGION - 1) / PI_REGION, W = (g.S + 63) / 64, G = (nb + RB_TILE - 1) / RB_TILE;
+  uint8_t* wcnt = ar.take<uint8_t>(W);
+  uint32_t* rcnt = ar.take<uint32_t>(R + 1);
+  uint32_t* tot = ar.take<uint32_t>(4);
+  uint32_t* E = ar.take<uint32_t>(nb);
+  uint32_t* s0 = ar.take<uint32_t>(nb);
+  uint32_t* P = ar.take<uint32_t>(nb);
+  uint32_t* rend = ar.take<uint32_t>(nb);
+  uint8_t* head = ar.take<uint8_t>(nb);
+  uint4* agg = ar.take<uint4>(G);
+  uint32_t* sctr = ar.take<uint32_t>(CTR_TOTAL);
+  uint32_t* flags = ar.take<uint32_t>(4);
+  uint32_t* scratch = ar.take<uint32_t>(g.S);
+  if (ar.used > ar.cap) {
+    set_error("internal: localized insert workspace overflow");
+    return GQF_ERR_INVALID;
+  }
+  const dim3 kg((nb + 255) / 256);
+  GQF_LAUNCH(k_pi_init, dim3(1), dim3(64), 0, s, rcnt + R, flags);
+  GQF_LAUNCH_B((double)g.bytes, k_pi_count<BPE>, dim3(R), dim3(256), 0, s, qf.table, g.S, wcnt, rcnt);
+  GQF_LAUNCH(k_scan_rows<1>, dim3(1), dim3(1024), 0, s, rcnt, R + 1, tot);
+  GQF_LAUNCH(k_pi_keys<BPE>, kg, dim3(256), 0, s, qf.table, g.S, g.q, g.r, B, nb, wcnt, rcnt, E, s0, flags);
+  GQF_LAUNCH(k_pi_jagg, dim3(G), dim3(RB_THREADS), 0, s, E, nb, agg);
+  GQF_TRY(rb_scan(s, ar, agg, G, ~0ull, sctr, nullptr));
+  GQF_LAUNCH(k_pi_select<BPE>, dim3(G), dim3(RB_THREADS), 0, s, qf.table, g.S, E, s0, nb, agg, wcnt, rcnt, R, P, head,
+             flags);
+  GQF_LAUNCH(k_pi_regions, kg, dim3(256), 0, s, s0, P, head, nb, rend, flags);
+  GQF_HIP(hipMemcpyAsync(ds.h_pinned, flags, 4, hipMemcpyDeviceToHost, s));
+  GQF_HIP(hipStreamSynchronize(s));
+  if (ds.h_pinned[0] != 0) return GQF_OK;  // the full rebuild takes over; nothing was written
+  GQF_LAUNCH(k_pi_write<BPE>, kg, dim3(256), 0, s, qf.table, g.S, g.r, B, nb, s0, P, head, rend, scratch, flags);
+  *done = true;
+  return GQF_OK;
+}
+
 static gqf_status sqf_insert_impl(const gqf_sqf& qf, int n, const uint32_t* d_keys, int mode, gqf_stream stream,
                                   float* ms) {
   GQF_TRY(sqf_check(qf));
@@ -189,12 +245,26 @@ static gqf_status sqf_insert_impl(const gqf_sqf& qf, int n, const uint32_t* d_ke
   GQF_TRY(cs.begin(stream));
   DeviceState& ds = *cs.ds;
   const uint32_t un = (uint32_t)n;
-  GQF_TRY(ws_reserve(ds, sort_workspace_bytes(un, 4, false) + sqf_decode_bytes(g) +
-                             rebuild_ws_bytes((uint64_t)g.S + un)));
+  const int path = sqf_insert_path();
+  const bool local = mode == KEEP_TAGGED_INSERT && un > 0 && path != GQF_INSERT_REBUILD &&
+                     (path == GQF_INSERT_LOCAL || (un <= PI_AUTO_MAX && (uint64_t)un * PI_MIN_RATIO <= g.S));
+  const size_t full = sqf_decode_bytes(g) + rebuild_ws_bytes((uint64_t)g.S + un);
+  GQF_TRY(ws_reserve(ds, sort_workspace_bytes(un, 4, false) + std::max(full, local ? sqf_local_bytes(g, un) : 0)));
   Arena ar(ds.ws, ds.ws_cap);
   uint32_t* B = nullptr;
   GQF_TRY(radix_sort<uint32_t>(cs.stream, ds, ar, d_keys, true, (g.q + g.r) | sqf_hash_flag(), nullptr, false, un, 0, g.q + g.r, &B,
                                nullptr));
+  if (local) {
+    const size_t mark = ar.used;
+    bool done = false;
+    GQF_LAUNCH(k_reset_counters, dim3(1), dim3(64), 0, cs.stream, ds.counters, (int)CTR_OP, (int)CTR_TOTAL);
+    if (g.bpe == 1) GQF_TRY(sqf_insert_local<1>(cs, ar, qf, g, B, un, &done));
+    else if (g.bpe == 2) GQF_TRY(sqf_insert_local<2>(cs, ar, qf, g, B, un, &done));
+    else if (g.bpe == 3) GQF_TRY(sqf_insert_local<3>(cs, ar, qf, g, B, un, &done));
+    else GQF_TRY(sqf_insert_local<4>(cs, ar, qf, g, B, un, &done));
+    if (done) return cs.end(ms, true);
+    ar.used = mark;
+  }
   uint32_t* A;
   const uint32_t* m_dev = nullptr;
   GQF_TRY(sqf_decode(cs.stream, ar, qf, g, &A, &m_dev));

@@ -73,6 +73,20 @@ enum gqf_hash { GQF_HASH_AP = 0, GQF_HASH_FNV = 1 };
 gqf_status gqf_sqf_set_hash(int kind);
 int gqf_sqf_get_hash(void);
 
+/* How gqf_sqf_insert (insert, quotientFilter.cu:1184) of the calling thread
+ * updates the table.  No reference counterpart: the results are the same
+ * either way; only the work differs.
+ *   AUTO     localized for batches of at most 1024 keys (and S/256), else
+ *            the rebuild;
+ *   REBUILD  decode the table, merge the sorted batch, rebuild every slot;
+ *   LOCAL    rewrite only the regions the batch changes, between the last
+ *            empty-or-unshifted slot before each key and the last slot its
+ *            keys take; a region longer than 2^16 slots, or a batch that
+ *            does not fit, falls back to the rebuild. */
+enum gqf_insert_path { GQF_INSERT_AUTO = 0, GQF_INSERT_REBUILD = 1, GQF_INSERT_LOCAL = 2 };
+gqf_status gqf_sqf_set_insert_path(int path);
+int gqf_sqf_get_insert_path(void);
+
 enum gqf_sqf_build_method {
   GQF_BUILD_SEGMENTED_LAYOUTS = 0,  /* bulkBuildSegmentedLayouts (quotientFilter.cu:1446) */
   GQF_BUILD_PARALLEL_MERGING = 1,   /* bulkBuildParallelMerging  (quotientFilter.cu:737)  */

@@ -352,3 +352,91 @@ def test_concurrent_streams_from_threads():
         assert (got_t == want).all()
         st, rw = O.rsqf_canon(O.fingerprints(np.concatenate([keys, O.mt19937(98, 400)]), q, 8), q, 8)
         assert (got_r == rw).all()
+
+
+def _insert_both_paths(q, r, base, batches, no_dup=True):
+    """Apply the insert batches to two filters built from base, one with the
+    localized path, one with the full rebuild; the tables (and statuses) must
+    be equal after every batch."""
+    out = []
+    for path in (G.INSERT_LOCAL, G.INSERT_REBUILD):
+        G.set_sqf_insert_path(path)
+        try:
+            f = G.SQF(q, r)
+            if len(base):
+                f.bulk_build(dev(base), NoDuplicates=no_dup)
+            tables, status = [], []
+            for b in batches:
+                try:
+                    f.insert(dev(b))
+                    status.append(G.OK)
+                except G.CapacityError:
+                    status.append(G.ERR_CAPACITY)
+                tables.append(f.table_bytes())
+            out.append((tables, status))
+            f.destroy()
+        finally:
+            G.set_sqf_insert_path(G.INSERT_AUTO)
+    (tl, sl), (tr, sr) = out
+    assert sl == sr
+    for i, (a, b) in enumerate(zip(tl, tr)):
+        assert (a == b).all(), ("batch", i, np.flatnonzero(a != b)[:5])
+    return tl, sl
+
+
+@pytest.mark.parametrize("q,r,a0,a1,nb", [(14, 5, 0.5, 0.85, 6), (16, 13, 0.6, 0.9, 5), (11, 21, 0.3, 0.8, 4),
+                                          (18, 5, 0.05, 0.5, 3), (12, 5, 0.0, 0.6, 4)])
+def test_insert_local_equals_rebuild_and_reference(q, r, a0, a1, nb):
+    """Localized inserts (regions around each key) equal the full rebuild and
+    the reference's sequential insert, batch by batch."""
+    say("sqf localized insert", q, r, a0, a1)
+    n0, n1 = O.key_count(a0, q), O.key_count(a1, q)
+    keys = O.mt19937(51 + q, n1)
+    step = max(1, (n1 - n0) // nb)
+    batches = [keys[i:i + step] for i in range(n0, n1, step)]
+    tables, status = _insert_both_paths(q, r, keys[:n0], batches)
+    assert all(s == G.OK for s in status)
+    t = sqf_canon(keys[:n0], q, r, True) if n0 else np.zeros(O.sqf_table_bytes(q, r), np.uint8)
+    for b, got in zip(batches, tables):
+        assert O.sqf_insert_keys(t, q, r, b) == O.OK
+        assert (got == t).all()
+
+
+def test_insert_local_duplicates_hot_keys_and_edges():
+    """Duplicates in a batch and against a Dup-mode table, keys whose quotients
+    sit at both ends of the table, and 3000 copies of one key (one region
+    over thousands of slots, of which one entry is new)."""
+    q, r = 13, 5
+    S = G.SQF.num_slots(q)
+    say("sqf localized insert edges", q, r)
+    base = O.mt19937(61, O.key_count(0.4, q))
+    base = np.concatenate([base, base[:300]])  # stored twice (Dup build)
+    fp = O.fingerprints(np.arange(200000, dtype=np.uint32), q, r)
+    qs = fp >> r
+    low = np.arange(200000, dtype=np.uint32)[qs < 8][:20]
+    high = np.arange(200000, dtype=np.uint32)[qs >= (1 << q) - 8][:20]
+    hot = np.full(3000, 12345, np.uint32)
+    batches = [np.concatenate([base[:100], base[:100], O.mt19937(62, 200), O.mt19937(62, 200)]),
+               np.concatenate([low, high]), hot, O.mt19937(63, 500)]
+    _insert_both_paths(q, r, base, batches, no_dup=False)
+
+
+def test_insert_local_near_capacity():
+    """A batch that does not fit: the localized path hands over to the
+    rebuild, which reports the capacity error and leaves the table as it was."""
+    q, r = 12, 5
+    S = G.SQF.num_slots(q)
+    base = O.mt19937(71, int(0.97 * S))
+    extra = O.mt19937(72, 400)
+    say("sqf localized insert capacity", q, r)
+    tables, status = _insert_both_paths(q, r, base, [extra[:50], extra])
+    assert status[-1] == G.ERR_CAPACITY and (tables[-1] == tables[-2]).all()
+
+
+def test_insert_local_long_region_falls_back():
+    """70,000 copies of one key: the localized path's region would pass 2^16
+    slots, so the call runs the rebuild; the result is the same."""
+    q, r = 18, 5
+    say("sqf localized insert fallback", q, r)
+    base = O.mt19937(81, O.key_count(0.3, q))
+    _insert_both_paths(q, r, base, [np.full(70000, 777, np.uint32), O.mt19937(82, 3000)])
