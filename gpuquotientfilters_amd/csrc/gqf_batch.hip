@@ -351,7 +351,7 @@ __global__ __launch_bounds__(64) void k_bb_fixup(const BTile* __restrict__ tl, c
 // counts before slot 64 w and the filter's last entry the totals
 // (sqf_lookup_indexed).  Thread i handles index entry i; its filter is the
 // last f with woff[f] <= i.
-template <int BPE>
+template <int BPE, bool WIN>
 __global__ __launch_bounds__(256) void k_bi_count(const BFilter* __restrict__ flt, const uint32_t* __restrict__ woff,
                                                   uint32_t F, uint2* __restrict__ idx) {
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
@@ -366,10 +366,19 @@ __global__ __launch_bounds__(256) void k_bi_count(const BFilter* __restrict__ fl
   const uint32_t w = i - woff[lo];
   uint32_t o = 0, r = 0;
   const uint32_t s0 = w * 64, s1 = s0 + 64 < fl.S ? s0 + 64 : fl.S;  // the totals entry (s0 >= S) counts nothing
-  for (uint32_t s = s0; s < s1; ++s) {
-    const uint32_t e = sqf_get<BPE>(fl.table, fl.S, s);
-    o += (e >> 2) & 1u;
-    r += ((e & 7u) != 0 && !(e & 2u)) ? 1u : 0u;
+  if (WIN) {
+    if (s0 < fl.S) {  // 16-byte loads and bit masks (slots past S read as empty)
+      SqfWin<BPE> W;
+      sqf_win_load<BPE>(fl.table, fl.S, w, W);
+      o = (uint32_t)__popcll(W.occ);
+      r = (uint32_t)__popcll((W.occ | W.cont | W.shf) & ~W.cont);
+    }
+  } else {
+    for (uint32_t s = s0; s < s1; ++s) {
+      const uint32_t e = sqf_get<BPE>(fl.table, fl.S, s);
+      o += (e >> 2) & 1u;
+      r += ((e & 7u) != 0 && !(e & 2u)) ? 1u : 0u;
+    }
   }
   idx[i] = make_uint2(o, r);
 }
@@ -508,13 +517,19 @@ static gqf_status batch_upload_chunks(SqfBatch& b, int per) {
     }                             \
   } while (0)
 
-// The lookup index of every filter, from its finished table.
+// The lookup index of every filter, from its table.
 static void batch_index_launch(SqfBatch& b) {
   hipStream_t s = b.stream;
   const dim3 grid((b.nidx + 255) / 256);
-#define GQF_BI_COUNT(BPE_) \
-  GQF_LAUNCH_B((double)b.nidx * 64 * (BPE_) + (double)b.nidx * 8, k_bi_count<BPE_>, grid, dim3(256), 0, s, b.d_flt, b.d_woff, (uint32_t)b.F, b.d_idx)
-  GQF_BB_BPE(b.bpe, GQF_BI_COUNT);
+#define GQF_BI_COUNT(BPE_, WIN_)                                                                            \
+  GQF_LAUNCH_B((double)b.nidx * 64 * (BPE_) + (double)b.nidx * 8, (k_bi_count<BPE_, WIN_>), grid, dim3(256), 0, s, \
+               b.d_flt, b.d_woff, (uint32_t)b.F, b.d_idx)
+  if (b.win && b.bpe == 1) GQF_BI_COUNT(1, true);
+  else if (b.win && b.bpe == 2) GQF_BI_COUNT(2, true);
+  else if (b.bpe == 1) GQF_BI_COUNT(1, false);
+  else if (b.bpe == 2) GQF_BI_COUNT(2, false);
+  else if (b.bpe == 3) GQF_BI_COUNT(3, false);
+  else GQF_BI_COUNT(4, false);
 #undef GQF_BI_COUNT
   GQF_LAUNCH(k_bi_scan, dim3((unsigned)b.F), dim3(1024), 0, s, b.d_woff, b.d_idx);
 }
@@ -527,7 +542,6 @@ static void batch_build_launch(SqfBatch& b) {
   GQF_LAUNCH(k_bb_reset, dim3(64), dim3(256), 0, s, b.d_ferr, 3 * F);  // ferr, fend, more_n
   if (b.N == 0) {
     GQF_LAUNCH(k_bb_clear_tail, dim3(64, F), dim3(256), 0, s, b.d_flt, b.d_ferr, b.d_fend, b.bpe);
-    batch_index_launch(b);
     return;
   }
   const uint32_t* src = b.d_keys;
@@ -577,11 +591,13 @@ static void batch_build_launch(SqfBatch& b) {
   GQF_LAUNCH(k_bb_clear_tail, dim3(64, F), dim3(256), 0, s, b.d_flt, b.d_ferr, b.d_fend, b.bpe);
   GQF_LAUNCH(k_bb_fixup, dim3(b.nsub + F), dim3(64), 0, s, b.d_tl, b.d_flt, b.d_ferr, b.d_fix, b.d_fixcnt, b.d_more,
              b.d_moren, b.nsub, b.bpe);
-  batch_index_launch(b);
 }
 
+// The lookups: first the window index of every table as it is now (a
+// filter may have changed since the batch build), then one lookup launch.
 static void batch_lookup_launch(SqfBatch& b) {
   if (b.P == 0) return;
+  batch_index_launch(b);
   const uint32_t per = (uint32_t)(b.lk_block * b.lk_ipt);
   const dim3 grid((b.P + per - 1) / per), block((unsigned)b.lk_block);
   const double bytes = (double)b.P * (8 + 64);

@@ -267,7 +267,11 @@ gqf_status gqf_sqf_batch_create(gqf_sqf_batch* out, const gqf_sqf* filters, int 
  * (use_graph != 0) or the same kernels launched one by one.  filter_status
  * (F entries, may be NULL: then any failure returns GQF_ERR_CAPACITY). */
 gqf_status gqf_sqf_batch_build(gqf_sqf_batch b, int use_graph, int* filter_status, float* ms);
-/* All lookups of the batch (unsorted path): one graph launch or one kernel. */
+/* All lookups of the batch (unsorted path): one graph launch, or the same
+ * kernels one by one.  The launch first indexes every table as it is (the
+ * occupied quotients and run starts before each 64-slot window), so a walk
+ * through a long cluster finishes by binary searches; results equal the
+ * per-filter walk, also after a filter was changed since the batch build. */
 gqf_status gqf_sqf_batch_lookup(gqf_sqf_batch b, int use_graph, float* ms);
 gqf_status gqf_sqf_batch_info(gqf_sqf_batch b, gqf_batch_info* info);
 gqf_status gqf_sqf_batch_destroy(gqf_sqf_batch b);

@@ -96,7 +96,7 @@ def test_batch_equals_per_filter_calls(no_dup):
     b = G.SQFBatch(filters, dk, offsets(keys), dp, offsets(probe), out, no_dup=no_dup)
     info = b.info()
     say("batch plan", info)
-    assert info["build_graph_nodes"] > 0 and info["lookup_graph_nodes"] == 1
+    assert info["build_graph_nodes"] > 0 and info["lookup_graph_nodes"] == 3  # index count + scan, lookups
     ref_tables, ref_looks, ref_status = per_filter(qs, r, keys, probe, no_dup)
     # graph replays with eager work (per-filter calls, resets) in between; every replay checked
     for rep in range(3):
@@ -201,4 +201,32 @@ def test_batch_long_clusters_lookup_through_the_index(q, r):
         assert st == ref_status
         b.lookup(use_graph=use_graph)
         check_batch(qs, r, keys, probe, False, filters, out, ref_tables, ref_looks, ref_status)
+    b.destroy()
+
+
+def test_batch_lookup_sees_tables_changed_after_the_build():
+    """A filter changed by a plain call after the batch build (here: a hot key
+    inserted into a Dup filter, making a run of thousands of slots): the
+    batch lookup indexes the tables as they are, so its results equal the
+    per-filter walk on the changed table."""
+    q, r = 14, 5
+    qs, keys, probe = _hot_batch(4, q, r, seed=5)
+    say("sqf batch lookup after a plain change", q, r)
+    filters = [G.SQF(q, r) for q in qs]
+    dk, dp = dev(flat(keys)), dev(flat(probe))
+    out = G.DeviceArray.empty(len(flat(probe)))
+    b = G.SQFBatch(filters, dk, offsets(keys), dp, offsets(probe), out, no_dup=False)
+    assert b.build(use_graph=True)[1] == [G.OK] * 4
+    # filter 2: rebuilt from its keys plus a second hot key (Dup build)
+    extra = np.concatenate([keys[2][: len(keys[2]) * 2 // 3], np.full(int(0.2 * G.SQF.num_slots(q)), 424242, np.uint32)])
+    st, want = O.sqf_canon(O.fingerprints(extra, q, r), q, r, False)
+    assert st == O.OK
+    filters[2].load_table(want)
+    probe2 = probe[2]
+    for use_graph in (True, False):
+        out.fill(0)
+        b.lookup(use_graph=use_graph)
+        po = offsets(probe)
+        got = out.to_host()[int(po[2]):int(po[3])]
+        assert (got == O.sqf_lookup_keys(want, q, r, probe2)).all()
     b.destroy()
