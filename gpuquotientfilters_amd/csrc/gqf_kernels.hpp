@@ -389,6 +389,86 @@ __global__ __launch_bounds__(256) void k_rs_downsweep(const void* __restrict__ i
   }
 }
 
+// Every pass of a sort of at most SORT_TILE keys in one workgroup: the keys
+// (and values) stay in registers and LDS between passes; each pass ranks them
+// stably as k_rs_downsweep does (ballots within a wave, then waves in order)
+// and scatters them within LDS.  Key j sits at position wid*1024 + i*64 +
+// lane of thread (wid, lane), item i, before and after every pass; padding
+// keys (all ones) take the last digit and rank after every real key.
+template <typename K, bool VALS, bool HASH>
+__global__ __launch_bounds__(SORT_THREADS) void k_rs_small(const void* __restrict__ in_keys,
+                                                           const uint32_t* __restrict__ in_vals,
+                                                           K* __restrict__ out_keys, uint32_t* __restrict__ out_vals,
+                                                           uint32_t n, uint32_t fp_bits, uint32_t lo_bit,
+                                                           uint32_t npass) {
+  __shared__ uint32_t s_wcnt[4 * 256];
+  __shared__ uint32_t s_lstart[256];
+  __shared__ uint32_t s_tmp[4];
+  __shared__ K s_keys[SORT_TILE];
+  __shared__ uint32_t s_vals[VALS ? SORT_TILE : 1];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const uint64_t lt = (1ull << lane) - 1ull;
+  K k[SORT_ITEMS];
+  uint32_t v[VALS ? SORT_ITEMS : 1];
+#pragma unroll
+  for (int i = 0; i < SORT_ITEMS; ++i) {
+    const uint32_t idx = (uint32_t)wid * (SORT_ITEMS * 64) + (uint32_t)i * 64 + lane;
+    const bool valid = idx < n;
+    k[i] = valid ? sort_load_key<K, HASH>(in_keys, idx, fp_bits) : (K)~(K)0;
+    if (VALS) v[i] = in_vals ? (valid ? in_vals[idx] : 0u) : idx;
+  }
+  for (uint32_t p = 0; p < npass; ++p) {
+    const uint32_t shift = lo_bit + 8 * p;
+    for (int i = tid; i < 4 * 256; i += SORT_THREADS) s_wcnt[i] = 0;
+    __syncthreads();
+    uint32_t rk[SORT_ITEMS];
+#pragma unroll
+    for (int i = 0; i < SORT_ITEMS; ++i) {
+      const uint32_t d = (uint32_t)(k[i] >> shift) & 255u;
+      const uint64_t m = match_digit(d);
+      const uint32_t pre = s_wcnt[wid * 256 + d];
+      const uint32_t cnt = (uint32_t)__popcll(m);
+      const uint32_t below = (uint32_t)__popcll(m & lt);
+      if (below == cnt - 1) s_wcnt[wid * 256 + d] = pre + cnt;
+      rk[i] = pre + below;
+    }
+    __syncthreads();
+    {
+      const uint32_t d = tid;
+      const uint32_t w0 = s_wcnt[d], w1 = s_wcnt[256 + d], w2 = s_wcnt[512 + d], w3 = s_wcnt[768 + d];
+      s_wcnt[d] = 0;
+      s_wcnt[256 + d] = w0;
+      s_wcnt[512 + d] = w0 + w1;
+      s_wcnt[768 + d] = w0 + w1 + w2;
+      s_lstart[d] = block256_excl_scan(w0 + w1 + w2 + w3, s_tmp, nullptr);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < SORT_ITEMS; ++i) {
+      const uint32_t d = (uint32_t)(k[i] >> shift) & 255u;
+      const uint32_t pos = s_lstart[d] + s_wcnt[wid * 256 + d] + rk[i];
+      s_keys[pos] = k[i];
+      if (VALS) s_vals[pos] = v[i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < SORT_ITEMS; ++i) {
+      const uint32_t pos = (uint32_t)wid * (SORT_ITEMS * 64) + (uint32_t)i * 64 + lane;
+      k[i] = s_keys[pos];
+      if (VALS) v[i] = s_vals[pos];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < SORT_ITEMS; ++i) {
+    const uint32_t idx = (uint32_t)wid * (SORT_ITEMS * 64) + (uint32_t)i * 64 + lane;
+    if (idx < n) {
+      out_keys[idx] = k[i];
+      if (VALS) out_vals[idx] = v[i];
+    }
+  }
+}
+
 inline size_t sort_workspace_bytes(uint64_t n, size_t key_bytes, bool vals) {
   const SortPlan p = sort_plan((uint32_t)n);
   return 2 * aligned_bytes(n * key_bytes) + (vals ? 2 * aligned_bytes(n * 4) : 0) +
@@ -421,6 +501,22 @@ gqf_status radix_sort(hipStream_t s, DeviceState& ds, Arena& ar, const void* in,
   uint32_t npass = (hi_bit - lo_bit + 7) / 8;
   if (npass > SORT_MAX_PASSES) npass = SORT_MAX_PASSES;
   if (n == 0) {
+    *keys_out = kb[0];
+    if (vals_out) *vals_out = vb[0];
+    return GQF_OK;
+  }
+  if (n <= (uint32_t)SORT_TILE) {
+    // one workgroup, every pass in LDS, one launch
+    const double bytes = (double)n * ((hash ? 4.0 : (double)sizeof(K)) + sizeof(K) +
+                                      (with_vals ? ((in_vals ? 4.0 : 0.0) + 4.0) : 0.0));
+#define GQF_RS_SMALL(V_, H_)                                                                                    \
+  GQF_LAUNCH_B(bytes, (k_rs_small<K, V_, H_>), dim3(1), dim3(SORT_THREADS), 0, s, in, in_vals, kb[0], vb[0], n, \
+               fp_bits, lo_bit, npass)
+    if (with_vals && hash) GQF_RS_SMALL(true, true);
+    else if (with_vals) GQF_RS_SMALL(true, false);
+    else if (hash) GQF_RS_SMALL(false, true);
+    else GQF_RS_SMALL(false, false);
+#undef GQF_RS_SMALL
     *keys_out = kb[0];
     if (vals_out) *vals_out = vb[0];
     return GQF_OK;
