@@ -354,12 +354,12 @@ def test_concurrent_streams_from_threads():
         assert (got_r == rw).all()
 
 
-def _insert_both_paths(q, r, base, batches, no_dup=True):
-    """Apply the insert batches to two filters built from base, one with the
-    localized path, one with the full rebuild; the tables (and statuses) must
-    be equal after every batch."""
+def _insert_both_paths(q, r, base, batches, no_dup=True, paths=(G.INSERT_LOCAL, G.INSERT_REBUILD)):
+    """Apply the insert batches to filters built from base, one per insert
+    path (localized, full rebuild); the tables (and statuses) must be equal
+    after every batch."""
     out = []
-    for path in (G.INSERT_LOCAL, G.INSERT_REBUILD):
+    for path in paths:
         G.set_sqf_insert_path(path)
         try:
             f = G.SQF(q, r)
@@ -377,10 +377,11 @@ def _insert_both_paths(q, r, base, batches, no_dup=True):
             f.destroy()
         finally:
             G.set_sqf_insert_path(G.INSERT_AUTO)
-    (tl, sl), (tr, sr) = out
-    assert sl == sr
-    for i, (a, b) in enumerate(zip(tl, tr)):
-        assert (a == b).all(), ("batch", i, np.flatnonzero(a != b)[:5])
+    tl, sl = out[0]
+    for p, (tr, sr) in zip(paths[1:], out[1:]):
+        assert sl == sr, ("path", p)
+        for i, (a, b) in enumerate(zip(tl, tr)):
+            assert (a == b).all(), ("path", p, "batch", i, np.flatnonzero(a != b)[:5])
     return tl, sl
 
 
@@ -440,3 +441,58 @@ def test_insert_local_long_region_falls_back():
     say("sqf localized insert fallback", q, r)
     base = O.mt19937(81, O.key_count(0.3, q))
     _insert_both_paths(q, r, base, [np.full(70000, 777, np.uint32), O.mt19937(82, 3000)])
+
+
+@pytest.mark.parametrize("q,r,a0,a1,nb", [(20, 5, 0.0005, 0.004, 3), (3, 29, 0.0, 0.8, 3), (9, 13, 0.2, 0.95, 3),
+                                          (11, 21, 0.4, 0.9, 2), (21, 5, 0.7, 0.9, 2)])
+def test_insert_sparse_tiny_and_wide_tables(q, r, a0, a1, nb):
+    """Sparse tables (most 4096-slot decode regions empty), a table smaller
+    than one region, every element width, and full ones: both insert paths
+    against the reference's sequential insert."""
+    say("sqf insert sparse/tiny/wide", q, r, a0, a1)
+    n0, n1 = O.key_count(a0, q), O.key_count(a1, q)
+    keys = O.mt19937(91 + q, n1)
+    step = max(1, (n1 - n0) // nb)
+    batches = [keys[i:i + step] for i in range(n0, n1, step)]
+    tables, status = _insert_both_paths(q, r, keys[:n0], batches)
+    assert all(s == G.OK for s in status)
+    t = sqf_canon(keys[:n0], q, r, True) if n0 else np.zeros(O.sqf_table_bytes(q, r), np.uint8)
+    for b, got in zip(batches, tables):
+        assert O.sqf_insert_keys(t, q, r, b) == O.OK
+        assert (got == t).all()
+
+
+def test_insert_long_pending_chains():
+    """12,000 fingerprints in the first 2048 quotients: at the second decode
+    region's start more than 1024 occupied quotients still wait for their
+    runs, so decoding walks clusters for some quotients; hot keys put more
+    than a sub-tile of new entries between two stored ones."""
+    q, r = 14, 5
+    pool = O.mt19937(23, 200000)
+    fq = O.fingerprints(pool, q, r) >> r
+    skew = np.unique(pool[fq < 2048])[:12000]
+    assert len(skew) == 12000
+    say("sqf insert pending chains", q, r)
+    hot = np.concatenate([np.full(5000, skew[7], np.uint32), skew[6000:9000], np.full(3000, 4242, np.uint32)])
+    batches = [skew[4000:6000], hot, skew[9000:]]
+    tables, status = _insert_both_paths(q, r, skew[:4000], batches, no_dup=False)
+    assert all(s == G.OK for s in status)
+    t = sqf_canon(skew[:4000], q, r, False)
+    for b, got in zip(batches, tables):
+        assert O.sqf_insert_keys(t, q, r, b) == O.OK
+        assert (got == t).all()
+
+
+@pytest.mark.parametrize("no_dup", [True, False])
+def test_insert_via_merge_heavy_duplicates(no_dup):
+    """insertViaMerge on a table holding every key three times, with new keys
+    repeated four times: keep-all keeps every copy, no-dup one of each."""
+    q, r = 13, 5
+    keys = O.mt19937(94, 1500)
+    base = np.repeat(keys, 3)
+    new = np.concatenate([np.repeat(keys[::7], 4), O.mt19937(95, 900)])
+    f = G.SQF(q, r)
+    f.bulk_build(dev(base), NoDuplicates=False)
+    f.insertViaMerge(dev(new), NoDuplicates=no_dup)
+    want = sqf_canon(np.concatenate([base, new]), q, r, no_dup)
+    assert (f.table_bytes() == want).all()
