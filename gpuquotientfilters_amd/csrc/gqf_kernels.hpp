@@ -577,10 +577,45 @@ gqf_status radix_sort(hipStream_t s, DeviceState& ds, Arena& ar, const void* in,
 
 enum KeepMode { KEEP_ALL = 0, KEEP_UNIQUE = 1, KEEP_TAGGED_INSERT = 2, KEEP_FLAGS = 3 };
 
+// ====================================================================== packed fingerprints
+// Read access to an ascending fingerprint array.  Packed form (PK, 64-bit
+// values of at most 34 bits): the array holds the low 32 bits, and the high
+// word, which only steps up along the array, is the number of the bounds
+// h0 <= h1 <= h2 at or below the index (0xFFFFFFFF: a high word not reached).
+template <typename K, bool PK>
+struct FpArray {
+  const void* p;
+  uint32_t h0, h1, h2;
+  __device__ __forceinline__ K operator[](uint32_t i) const {
+    if constexpr (PK) {
+      const uint32_t hi = (uint32_t)(i >= h0) + (uint32_t)(i >= h1) + (uint32_t)(i >= h2);
+      return (K)(((uint64_t)hi << 32) | reinterpret_cast<const uint32_t*>(p)[i]);
+    } else {
+      return reinterpret_cast<const K*>(p)[i];
+    }
+  }
+  // the value of entry i from its low word lo (PK)
+  __device__ __forceinline__ K widen(uint32_t i, uint32_t lo) const {
+    const uint32_t hi = (uint32_t)(i >= h0) + (uint32_t)(i >= h1) + (uint32_t)(i >= h2);
+    return (K)(((uint64_t)hi << 32) | lo);
+  }
+};
+// hb: the three bounds in device memory (PK only).
+template <typename K, bool PK>
+__device__ __forceinline__ FpArray<K, PK> fp_array(const void* p, const uint32_t* __restrict__ hb) {
+  FpArray<K, PK> a;
+  a.p = p;
+  a.h0 = PK ? hb[0] : 0u;
+  a.h1 = PK ? hb[1] : 0u;
+  a.h2 = PK ? hb[2] : 0u;
+  return a;
+}
+
 // ====================================================================== merge
-// Number of A entries among the first d outputs of the stable merge (A first on ties).
-template <typename K>
-__device__ __forceinline__ uint32_t merge_path(const K* A, uint32_t na, const K* B, uint32_t nb, uint32_t d) {
+// Number of A entries among the first d outputs of the stable merge (A first
+// on ties).  A and B: pointers or FpArray views.
+template <typename AA, typename BB>
+__device__ __forceinline__ uint32_t merge_path(AA A, uint32_t na, BB B, uint32_t nb, uint32_t d) {
   uint32_t lo = d > nb ? d - nb : 0u, hi = d < na ? d : na;
   while (lo < hi) {
     const uint32_t mid = (lo + hi) >> 1;
@@ -626,9 +661,9 @@ __device__ __forceinline__ uint32_t upper_bound_g(const K* __restrict__ a, uint3
 }
 
 // First index of the equal run of A that contains i (galloping search back).
-template <typename K>
-__device__ __forceinline__ uint32_t run_first(const K* __restrict__ A, uint32_t i) {
-  const K v = A[i];
+template <typename AA>
+__device__ __forceinline__ uint32_t run_first(AA A, uint32_t i) {
+  const auto v = A[i];
   if (i == 0 || A[i - 1] != v) return i;
   uint32_t h = i, step = 1, l;
   for (;;) {
@@ -646,14 +681,17 @@ __device__ __forceinline__ uint32_t run_first(const K* __restrict__ A, uint32_t 
   return h;
 }
 
-template <typename K>
-__global__ __launch_bounds__(DM_THREADS) void k_delete_mark(const K* __restrict__ A, uint32_t na,
+// PK: A is packed (FpArray), with its bounds at hb.
+template <typename K, bool PK>
+__global__ __launch_bounds__(DM_THREADS) void k_delete_mark(const K* __restrict__ Araw, uint32_t na,
                                                             const uint32_t* __restrict__ na_dev,
                                                             const K* __restrict__ D, uint32_t nd,
-                                                            uint8_t* __restrict__ keep) {
+                                                            uint8_t* __restrict__ keep,
+                                                            const uint32_t* __restrict__ hb) {
   __shared__ K s_d[DM_DCAP];
   __shared__ uint32_t s_rng[2];
   const int tid = threadIdx.x;
+  const FpArray<K, PK> A = fp_array<K, PK>(Araw, hb);
   if (na_dev) na = *na_dev;  // stored count on the device (the grid covers an upper bound)
   const uint32_t t0 = blockIdx.x * (uint32_t)DM_TILE;
   if (t0 >= na) return;
@@ -664,10 +702,21 @@ __global__ __launch_bounds__(DM_THREADS) void k_delete_mark(const K* __restrict_
   const uint32_t i0 = t0 + (uint32_t)tid * DM_ITEMS;
   K v[DM_ITEMS];
   const bool full = i0 + DM_ITEMS <= t1;
-  if (full) {
+  if (full && PK) {
+    // 16 low words, 16-byte aligned
+    const uint4* p = reinterpret_cast<const uint4*>(reinterpret_cast<const uint32_t*>(Araw) + i0);
+#pragma unroll
+    for (int k = 0; k < DM_ITEMS / 4; ++k) {
+      const uint4 x = p[k];
+      v[4 * k] = A.widen(i0 + 4 * k, x.x);
+      v[4 * k + 1] = A.widen(i0 + 4 * k + 1, x.y);
+      v[4 * k + 2] = A.widen(i0 + 4 * k + 2, x.z);
+      v[4 * k + 3] = A.widen(i0 + 4 * k + 3, x.w);
+    }
+  } else if (full) {
     // A is 256-byte aligned and i0 a multiple of DM_ITEMS: 16-byte aligned
     uint4 w[DM_ITEMS * sizeof(K) / 16];
-    const uint4* p = reinterpret_cast<const uint4*>(A + i0);
+    const uint4* p = reinterpret_cast<const uint4*>(Araw + i0);
 #pragma unroll
     for (int k = 0; k < (int)(DM_ITEMS * sizeof(K) / 16); ++k) w[k] = p[k];
     __builtin_memcpy(v, w, sizeof(v));

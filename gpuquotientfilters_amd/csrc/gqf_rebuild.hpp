@@ -69,17 +69,19 @@ __device__ __forceinline__ RS rs_wave_excl(RS x, RS* total) {
 }
 
 // Merge-path split (A entries among the first u*RB_TILE outputs) of every sub-tile boundary.
-// na_dev (may be NULL): A's count on the device, overriding na.
-template <typename K>
+// na_dev (may be NULL): A's count on the device, overriding na.  PK: A is
+// packed (FpArray) with its bounds at hb.
+template <typename K, bool PK>
 __global__ __launch_bounds__(256) void k_rb_splits(const K* __restrict__ A, uint32_t na,
                                                    const uint32_t* __restrict__ na_dev, const K* __restrict__ B,
-                                                   uint32_t nb, uint32_t nsub, uint32_t* __restrict__ splits) {
+                                                   uint32_t nb, uint32_t nsub, uint32_t* __restrict__ splits,
+                                                   const uint32_t* __restrict__ hb) {
   const uint32_t u = blockIdx.x * 256 + threadIdx.x;
   if (u > nsub) return;
   if (na_dev) na = *na_dev;
   const uint64_t N = (uint64_t)na + nb;
   const uint32_t d = (uint32_t)((uint64_t)u * RB_TILE < N ? (uint64_t)u * RB_TILE : N);
-  splits[u] = merge_path(A, na, B, nb, d);
+  splits[u] = merge_path(fp_array<K, PK>(A, hb), na, B, nb, d);
 }
 
 struct RbPlan {

@@ -208,9 +208,12 @@ constexpr uint32_t RD_SLOTS = RD_BLOCKS * 64;
 constexpr uint32_t RD_THREADS = 256;  // 16 slots per thread
 constexpr uint32_t RD_LIST = RD_SLOTS + 1;
 
+// hb (may be NULL): the bounds of a packed decode, reset here (nhv of them
+// counted by k_rd_emit, the others never reached).
 __global__ __launch_bounds__(256) void k_rd_pop(const uint64_t* __restrict__ meta, uint32_t nb, uint32_t W,
-                                                uint32_t* __restrict__ wp) {
+                                                uint32_t* __restrict__ wp, uint32_t* __restrict__ hb, uint32_t nhv) {
   const uint32_t w = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (hb && blockIdx.x == 0 && threadIdx.x < 3) hb[threadIdx.x] = threadIdx.x < nhv ? 0u : 0xFFFFFFFFu;
   if (w >= W) return;
   const uint32_t b = w * RD_BLOCKS + lane;
   uint32_t o = 0, e = 0;
@@ -343,9 +346,14 @@ __global__ __launch_bounds__(RD_THREADS) void k_rd_count(RsqfDev f, const uint32
   if (t == 0) wc[w] = total;
 }
 
-template <typename K, int R>
+// PK: out receives the low 32 bits of each 64-bit fingerprint, and the
+// window holding quotient k << (32 - r) (k = 1 .. nhv) -- or a later one
+// whose first entries still have smaller quotients -- adds the entries below
+// k << 32 to hb[k - 1], which ends as the index where the high word reaches k.
+template <typename K, int R, bool PK>
 __global__ __launch_bounds__(RD_THREADS) void k_rd_emit(RsqfDev f, const uint32_t* __restrict__ wp,
-                                                        const uint32_t* __restrict__ wc, K* __restrict__ out) {
+                                                        const uint32_t* __restrict__ wc, K* __restrict__ out,
+                                                        uint32_t* __restrict__ hb, uint32_t nhv) {
   // the quotient list, then (after every thread has read it) the window's entries
   __shared__ __align__(16) K s_buf[RD_SLOTS * sizeof(K) >= RD_LIST * 4 ? RD_SLOTS : RD_LIST];
   __shared__ uint32_t s_tmp4[4];
@@ -395,8 +403,29 @@ __global__ __launch_bounds__(RD_THREADS) void k_rd_emit(RsqfDev f, const uint32_
   for (int j = 0; j < 16; ++j)
     if ((m >> j) & 1u) s_buf[p++] = ((K)qv[j] << rr) | (K)rem[j];
   __syncthreads();
-  K* dst = out + wc[w];
-  for (uint32_t i = threadIdx.x; i < cnt; i += RD_THREADS) dst[i] = s_buf[i];
+  if constexpr (PK) {
+    uint32_t* dst = reinterpret_cast<uint32_t*>(out) + wc[w];
+    for (uint32_t i = threadIdx.x; i < cnt; i += RD_THREADS) dst[i] = (uint32_t)s_buf[i];
+    if (threadIdx.x == 0) {
+      for (uint32_t k = 1; k <= nhv; ++k) {
+        const uint64_t wk = ((uint64_t)k << (32 - rr)) / RD_SLOTS;  // window of quotient k << (32 - r)
+        if (w < wk) break;
+        const uint64_t thr = (uint64_t)k << 32;
+        if (w > wk && (cnt == 0 || (uint64_t)s_buf[0] >= thr)) continue;
+        uint32_t lo = 0, hi = cnt;  // entries below thr: a prefix of the window's
+        while (lo < hi) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if ((uint64_t)s_buf[mid] < thr) lo = mid + 1;
+          else hi = mid;
+        }
+        const uint32_t add = lo + (w == wk ? wc[w] : 0u);
+        if (add) atomicAdd(&hb[k - 1], add);
+      }
+    }
+  } else {
+    K* dst = out + wc[w];
+    for (uint32_t i = threadIdx.x; i < cnt; i += RD_THREADS) dst[i] = s_buf[i];
+  }
 }
 
 // ---------------------------------------------------------------- rebuild
@@ -421,8 +450,8 @@ __global__ __launch_bounds__(RD_THREADS) void k_rd_emit(RsqfDev f, const uint32_
 // ascending order (RSQF.cpp:314-400).
 constexpr uint32_t RW_WB = 128;  // blocks per LDS window (8192 slots)
 
-template <typename K, int MODE, bool MERGE>
-__device__ __forceinline__ void rr_load(const K* __restrict__ A, const K* __restrict__ B,
+template <typename K, int MODE, bool MERGE, bool PK>
+__device__ __forceinline__ void rr_load(const FpArray<K, PK>& A, const K* __restrict__ B,
                                         const uint8_t* __restrict__ flags, const uint32_t* __restrict__ splits,
                                         uint32_t u, uint32_t d0, uint32_t tn, K* s_stage, K* v, uint32_t& fl) {
   const int tid = threadIdx.x;
@@ -464,12 +493,19 @@ __device__ __forceinline__ void rr_load(const K* __restrict__ A, const K* __rest
     const uint64_t g0 = (uint64_t)d0 + i0;
     if (i0 + RB_ITEMS <= tn) {
       // d0 is a multiple of RB_TILE and A is 256-byte aligned
-      constexpr int PER = 16 / (int)sizeof(K);
-      const uint4* p = reinterpret_cast<const uint4*>(A + g0);
+      constexpr int PER = PK ? 4 : 16 / (int)sizeof(K);
+      const uint4* p = reinterpret_cast<const uint4*>(reinterpret_cast<const K*>(A.p) + g0);
+      if (PK) p = reinterpret_cast<const uint4*>(reinterpret_cast<const uint32_t*>(A.p) + g0);
 #pragma unroll
       for (int k = 0; k < RB_ITEMS / PER; ++k) {
         const uint4 x = p[k];
-        if (sizeof(K) == 8) {
+        if (PK) {
+          const uint32_t i = (uint32_t)g0 + 4 * k;
+          v[4 * k] = A.widen(i, x.x);
+          v[4 * k + 1] = A.widen(i + 1, x.y);
+          v[4 * k + 2] = A.widen(i + 2, x.z);
+          v[4 * k + 3] = A.widen(i + 3, x.w);
+        } else if (sizeof(K) == 8) {
           v[2 * k] = (K)(((uint64_t)x.y << 32) | x.x);
           v[2 * k + 1] = (K)(((uint64_t)x.w << 32) | x.z);
         } else {
@@ -490,7 +526,7 @@ __device__ __forceinline__ void rr_load(const K* __restrict__ A, const K* __rest
     } else {
 #pragma unroll
       for (int k = 0; k < RB_ITEMS; ++k) {
-        v[k] = (i0 + k < tn) ? A[g0 + k] : (K)0;
+        v[k] = (i0 + k < tn) ? A[(uint32_t)g0 + k] : (K)0;
         if (MODE == KEEP_FLAGS && i0 + k < tn && flags[g0 + k]) fl |= 1u << k;
       }
     }
@@ -513,15 +549,18 @@ __device__ __forceinline__ uint32_t rr_classify(const K* v, uint32_t fl, uint32_
   return keepbits;
 }
 
-template <typename K, int MODE, bool MERGE>
-__global__ __launch_bounds__(RB_THREADS, 8) void k_rr_agg(const K* __restrict__ A, uint32_t na,
+// PK: A is packed (FpArray) with its bounds at hb.
+template <typename K, int MODE, bool MERGE, bool PK>
+__global__ __launch_bounds__(RB_THREADS, 8) void k_rr_agg(const K* __restrict__ Araw, uint32_t na,
                                                           const uint32_t* __restrict__ na_dev,
                                                           const K* __restrict__ B, uint32_t nbk,
                                                           const uint8_t* __restrict__ flags,
                                                           const uint32_t* __restrict__ splits, uint32_t q, uint32_t r,
-                                                          uint32_t chunk, uint4* __restrict__ agg) {
+                                                          uint32_t chunk, uint4* __restrict__ agg,
+                                                          const uint32_t* __restrict__ hb) {
   __shared__ K s_stage[MERGE ? RB_TILE : 1];
   __shared__ uint4 s_wave[RB_WAVES + 1];
+  const FpArray<K, PK> A = fp_array<K, PK>(Araw, hb);
   if (na_dev) na = *na_dev;  // stored count on the device (the grid covers an upper bound)
   const uint64_t N = (uint64_t)na + (MERGE ? nbk : 0u);
   const uint32_t mq = (uint32_t)mask64(q);
@@ -532,7 +571,7 @@ __global__ __launch_bounds__(RB_THREADS, 8) void k_rr_agg(const K* __restrict__ 
     const uint32_t tn = (uint32_t)(c1 - d0 < (uint64_t)RB_TILE ? c1 - d0 : (uint64_t)RB_TILE);
     K v[RB_ITEMS];
     uint32_t fl;
-    rr_load<K, MODE, MERGE>(A, B, flags, splits, (uint32_t)(d0 / RB_TILE), (uint32_t)d0, tn, s_stage, v, fl);
+    rr_load<K, MODE, MERGE, PK>(A, B, flags, splits, (uint32_t)(d0 / RB_TILE), (uint32_t)d0, tn, s_stage, v, fl);
     RS a;
     (void)rr_classify<K, MODE>(v, fl, tn, r, mq, a);
     RS ex;
@@ -592,16 +631,18 @@ __device__ __forceinline__ void rr_visit(uint32_t fq, uint32_t fr, int32_t lc, i
   }
 }
 
-template <typename K, int MODE, bool MERGE, int R>
+template <typename K, int MODE, bool MERGE, int R, bool PK>
 __global__ __launch_bounds__(RB_THREADS, 6) void k_rr_write(uint8_t* __restrict__ base, uint32_t nb, uint32_t q,
-                                                            uint32_t r, const K* __restrict__ A, uint32_t na,
+                                                            uint32_t r, const K* __restrict__ Araw, uint32_t na,
                                                             const uint32_t* __restrict__ na_dev,
                                                             const K* __restrict__ B, uint32_t nbk,
                                                             const uint8_t* __restrict__ flags,
                                                             const uint32_t* __restrict__ splits, uint32_t chunk,
                                                             const uint4* __restrict__ pre,
-                                                            const uint32_t* __restrict__ ctr) {
+                                                            const uint32_t* __restrict__ ctr,
+                                                            const uint32_t* __restrict__ hb) {
   if (ctr[CTR_ERR]) return;  // capacity error: the table stays as it was
+  const FpArray<K, PK> A = fp_array<K, PK>(Araw, hb);
   __shared__ K s_stage[MERGE ? RB_TILE : 1];
   __shared__ __align__(16) uint32_t s_rem[RW_WB * 32];  // 2r words per block, r <= 16
   __shared__ uint32_t s_occ[RW_WB * 2];
@@ -626,7 +667,7 @@ __global__ __launch_bounds__(RB_THREADS, 6) void k_rr_write(uint8_t* __restrict_
     const uint32_t tn = (uint32_t)(c1 - d0 < (uint64_t)RB_TILE ? c1 - d0 : (uint64_t)RB_TILE);
     K v[RB_ITEMS];
     uint32_t fl;
-    rr_load<K, MODE, MERGE>(A, B, flags, splits, u, (uint32_t)d0, tn, s_stage, v, fl);
+    rr_load<K, MODE, MERGE, PK>(A, B, flags, splits, u, (uint32_t)d0, tn, s_stage, v, fl);
     RS a;
     const uint32_t keepbits = rr_classify<K, MODE>(v, fl, tn, rr, mq, a);
     RS ex;
@@ -776,46 +817,67 @@ static RsqfDev rdev(const gqf_rsqf& f) {
 
 static inline uint32_t rd_windows(uint32_t nb) { return (nb + RD_BLOCKS - 1) / RD_BLOCKS; }
 
+// Decoded copies of 64-bit fingerprints of at most 34 bits are packed (low
+// words plus the three indices where the high word steps up; FpArray).
+static inline bool rsqf_packed(const RsqfGeom& g, size_t kb) { return kb == 8 && g.bits <= 34; }
+
 static size_t rsqf_decode_ws(const RsqfGeom& g, size_t kb) {
   const uint32_t W = rd_windows(g.nb);
-  return aligned_bytes((size_t)W * 8) + aligned_bytes((size_t)W * 4) + 256 + aligned_bytes((g.cap + 1) * kb) + 256;
+  const size_t eb = rsqf_packed(g, kb) ? 4 : kb;
+  return aligned_bytes((size_t)W * 8) + aligned_bytes((size_t)W * 4) + 256 + aligned_bytes((g.cap + 1) * eb) + 512;
 }
 
 // The stored fingerprints in ascending order; their count is left in device
-// memory (*m_dev).
+// memory (*m_dev).  *hb: the bounds of the packed form (rsqf_packed), else
+// NULL.
 template <typename K>
-static gqf_status rsqf_decode(hipStream_t s, Arena& ar, const gqf_rsqf& f, K** out, const uint32_t** m_dev) {
+static gqf_status rsqf_decode(hipStream_t s, Arena& ar, const gqf_rsqf& f, K** out, const uint32_t** m_dev,
+                              const uint32_t** hb_out) {
   const RsqfDev d = rdev(f);
+  const RsqfGeom g = rgeom(f);
+  const bool pk = rsqf_packed(g, sizeof(K));
+  const uint32_t nhv = pk ? (1u << (g.bits - 32)) - 1u : 0u;
   const uint32_t W = rd_windows(f.numBlocks);
   uint32_t* wp = ar.take<uint32_t>((size_t)W * 2);
   uint32_t* wc = ar.take<uint32_t>(W);
   uint32_t* tot = ar.take<uint32_t>(4);
+  uint32_t* hb = pk ? ar.take<uint32_t>(4) : nullptr;
   if (ar.used > ar.cap) {
     set_error("internal: decode workspace overflow");
     return GQF_ERR_INVALID;
   }
   const double meta = (double)f.numBlocks * 16;  // occupieds + run ends
-  GQF_LAUNCH_B(meta, k_rd_pop, dim3((W + 3) / 4), dim3(256), 0, s, d.meta, f.numBlocks, W, wp);
+  GQF_LAUNCH_B(meta, k_rd_pop, dim3((W + 3) / 4), dim3(256), 0, s, d.meta, f.numBlocks, W, wp, hb, nhv);
   GQF_LAUNCH(k_scan_rows<2>, dim3(1), dim3(1024), 0, s, wp, W, tot);
   GQF_LAUNCH_B(meta, k_rd_count, dim3(W), dim3(RD_THREADS), 0, s, d, wp, wc);
   GQF_LAUNCH(k_scan_rows<1>, dim3(1), dim3(1024), 0, s, wc, W, tot + 2);
   // the count stays on the device (tot[2]): later kernels read it, the host
   // plans for the capacity
   *m_dev = tot + 2;
-  K* fps = ar.take<K>((size_t)f.numBlocks * 64 + 1);
+  // packed: 4 bytes per entry (K* typed, holding 32-bit words)
+  K* fps = pk ? reinterpret_cast<K*>(ar.take<uint32_t>((size_t)f.numBlocks * 64 + 1))
+              : ar.take<K>((size_t)f.numBlocks * 64 + 1);
   if (ar.used > ar.cap) {
     set_error("internal: decode workspace overflow");
     return GQF_ERR_INVALID;
   }
   // table read (remainders + run ends) + the entries written (count on the device)
   const double bytes = (double)rsqf_dev_bytes(f.qbits, f.rbits) - (double)f.numBlocks * 16;
-  if (f.rbits == 8)
-    GQF_LAUNCH_BC(bytes, *m_dev, sizeof(K), (k_rd_emit<K, 8>), dim3(W), dim3(RD_THREADS), 0, s, d, wp, wc, fps);
-  else if (f.rbits == 16)
-    GQF_LAUNCH_BC(bytes, *m_dev, sizeof(K), (k_rd_emit<K, 16>), dim3(W), dim3(RD_THREADS), 0, s, d, wp, wc, fps);
-  else
-    GQF_LAUNCH_BC(bytes, *m_dev, sizeof(K), (k_rd_emit<K, 0>), dim3(W), dim3(RD_THREADS), 0, s, d, wp, wc, fps);
+  const double per = pk ? 4.0 : (double)sizeof(K);
+#define GQF_RD_EMIT(R_, PK_) \
+  GQF_LAUNCH_BC(bytes, *m_dev, per, (k_rd_emit<K, R_, PK_>), dim3(W), dim3(RD_THREADS), 0, s, d, wp, wc, fps, hb, nhv)
+  if (pk) {
+    if (f.rbits == 8) GQF_RD_EMIT(8, true);
+    else if (f.rbits == 16) GQF_RD_EMIT(16, true);
+    else GQF_RD_EMIT(0, true);
+  } else {
+    if (f.rbits == 8) GQF_RD_EMIT(8, false);
+    else if (f.rbits == 16) GQF_RD_EMIT(16, false);
+    else GQF_RD_EMIT(0, false);
+  }
+#undef GQF_RD_EMIT
   *out = fps;
+  *hb_out = hb;
   return GQF_OK;
 }
 
@@ -827,11 +889,11 @@ static size_t rsqf_rebuild_ws(uint64_t N) {
 // Canonical layout of A (MERGE: merged with B; KEEP_FLAGS: the flagged
 // entries of A) written over the table.
 // na_dev (may be NULL): A's count on the device; na is then an upper bound
-// that sizes the grids.
-template <typename K, int MODE, bool MERGE>
-static gqf_status rsqf_rebuild(hipStream_t s, DeviceState& ds, Arena& ar, const gqf_rsqf& f, const K* A,
-                               uint32_t na, const uint32_t* na_dev, const K* B, uint32_t nbk,
-                               const uint8_t* flags) {
+// that sizes the grids.  PK: A is a packed decode with its bounds at hb.
+template <typename K, int MODE, bool MERGE, bool PK>
+static gqf_status rsqf_rebuild_t(hipStream_t s, DeviceState& ds, Arena& ar, const gqf_rsqf& f, const K* A,
+                                 uint32_t na, const uint32_t* na_dev, const K* B, uint32_t nbk,
+                                 const uint8_t* flags, const uint32_t* hb) {
   const RsqfGeom g = rgeom(f);
   const uint64_t N = (uint64_t)na + (MERGE ? nbk : 0u);
   const size_t tbytes = rsqf_dev_bytes(g.q, g.r);
@@ -849,27 +911,37 @@ static gqf_status rsqf_rebuild(hipStream_t s, DeviceState& ds, Arena& ar, const 
     set_error("internal: rebuild workspace overflow");
     return GQF_ERR_INVALID;
   }
-  const double per = (double)(sizeof(K) + (MODE == KEEP_FLAGS ? 1 : 0));  // bytes read per entry
-  const double nknown = na_dev ? (double)(MERGE ? nbk : 0u) : (double)N;   // entries the host knows of
+  // bytes read per stored entry (the host-known entries are new ones, full width)
+  const double per = (double)((PK ? 4 : sizeof(K)) + (MODE == KEEP_FLAGS ? 1 : 0));
+  const double nknown = na_dev ? (double)(MERGE ? nbk : 0u) : (double)N;  // entries the host knows of
+  const double kper = na_dev ? (double)sizeof(K) : per;
   if (MERGE)
-    GQF_LAUNCH(k_rb_splits<K>, dim3((p.nsub + 1 + 255) / 256), dim3(256), 0, s, A, na, na_dev, B, nbk, p.nsub,
-               splits);
-  GQF_LAUNCH_BC(nknown * per, na_dev, per, (k_rr_agg<K, MODE, MERGE>), dim3(p.G), dim3(RB_THREADS), 0, s, A, na,
-                na_dev, B, nbk, flags, splits, g.q, g.r, p.chunk, agg);
+    GQF_LAUNCH((k_rb_splits<K, PK>), dim3((p.nsub + 1 + 255) / 256), dim3(256), 0, s, A, na, na_dev, B, nbk, p.nsub,
+               splits, hb);
+  GQF_LAUNCH_BC(nknown * kper, na_dev, per, (k_rr_agg<K, MODE, MERGE, PK>), dim3(p.G), dim3(RB_THREADS), 0, s, A,
+                na, na_dev, B, nbk, flags, splits, g.q, g.r, p.chunk, agg, hb);
   GQF_TRY(rb_scan(s, ar, agg, p.G, g.cap, ds.counters, total));
   GQF_LAUNCH_B((double)tbytes, k_rr_clear, dim3(1024), dim3(256), 0, s, reinterpret_cast<uint8_t*>(f.blocks),
                (uint64_t)tbytes, ds.counters);
-  const double wbytes = nknown * per + (double)tbytes;
+  const double wbytes = nknown * kper + (double)tbytes;
 #define GQF_RR_WRITE(R_)                                                                                          \
-  GQF_LAUNCH_BC(wbytes, na_dev, per, (k_rr_write<K, MODE, MERGE, R_>), dim3(p.G), dim3(RB_THREADS), 0, s,        \
+  GQF_LAUNCH_BC(wbytes, na_dev, per, (k_rr_write<K, MODE, MERGE, R_, PK>), dim3(p.G), dim3(RB_THREADS), 0, s,    \
                 reinterpret_cast<uint8_t*>(f.blocks), g.nb, g.q, g.r, A, na, na_dev, B, nbk, flags, splits, p.chunk, \
-                agg, ds.counters)
+                agg, ds.counters, hb)
   if (g.r == 8) GQF_RR_WRITE(8);
   else if (g.r == 16) GQF_RR_WRITE(16);
   else GQF_RR_WRITE(0);
 #undef GQF_RR_WRITE
   GQF_LAUNCH(k_rr_tail, dim3(1), dim3(64), 0, s, reinterpret_cast<uint8_t*>(f.blocks), total, ds.counters);
   return GQF_OK;
+}
+template <typename K, int MODE, bool MERGE>
+static gqf_status rsqf_rebuild(hipStream_t s, DeviceState& ds, Arena& ar, const gqf_rsqf& f, const K* A,
+                               uint32_t na, const uint32_t* na_dev, const K* B, uint32_t nbk,
+                               const uint8_t* flags, const uint32_t* hb = nullptr) {
+  if constexpr (sizeof(K) == 8)
+    if (hb) return rsqf_rebuild_t<K, MODE, MERGE, true>(s, ds, ar, f, A, na, na_dev, B, nbk, flags, hb);
+  return rsqf_rebuild_t<K, MODE, MERGE, false>(s, ds, ar, f, A, na, na_dev, B, nbk, flags, nullptr);
 }
 
 static gqf_status rsqf_args(const gqf_rsqf& f, int n, const uint32_t* keys) {
@@ -905,11 +977,12 @@ static gqf_status rsqf_insert_t(const gqf_rsqf& f, uint32_t n, const uint32_t* k
   Arena ar(ds.ws, ds.ws_cap);
   K* A;
   const uint32_t* m_dev = nullptr;
-  GQF_TRY(rsqf_decode<K>(cs.stream, ar, f, &A, &m_dev));
+  const uint32_t* hb = nullptr;
+  GQF_TRY(rsqf_decode<K>(cs.stream, ar, f, &A, &m_dev, &hb));
   K* B;
   GQF_TRY(radix_sort<K>(cs.stream, ds, ar, keys, true, g.bits, nullptr, false, n, 0, g.bits, &B, nullptr));
   // at most cap stored entries: the grids are sized for cap + n
-  return rsqf_rebuild<K, KEEP_ALL, true>(cs.stream, ds, ar, f, A, (uint32_t)g.cap, m_dev, B, n, nullptr);
+  return rsqf_rebuild<K, KEEP_ALL, true>(cs.stream, ds, ar, f, A, (uint32_t)g.cap, m_dev, B, n, nullptr, hb);
 }
 
 template <typename K>
@@ -926,7 +999,8 @@ static gqf_status rsqf_delete_t(const gqf_rsqf& f, uint32_t n, const uint32_t* k
   Arena ar(ds.ws, ds.ws_cap);
   K* A;
   const uint32_t* m_dev = nullptr;
-  GQF_TRY(rsqf_decode<K>(cs.stream, ar, f, &A, &m_dev));
+  const uint32_t* hb = nullptr;
+  GQF_TRY(rsqf_decode<K>(cs.stream, ar, f, &A, &m_dev, &hb));
   K* D;
   GQF_TRY(radix_sort<K>(cs.stream, ds, ar, keys, true, g.bits, nullptr, false, n, 0, g.bits, &D, nullptr));
   const uint32_t mcap = (uint32_t)g.cap;  // the grids cover the capacity; kernels read the count
@@ -935,9 +1009,16 @@ static gqf_status rsqf_delete_t(const gqf_rsqf& f, uint32_t n, const uint32_t* k
     set_error("internal: delete workspace overflow");
     return GQF_ERR_INVALID;
   }
-  GQF_LAUNCH_BC(0.0, m_dev, (double)(sizeof(K) + 1), k_delete_mark<K>, dim3((mcap + DM_TILE - 1) / DM_TILE),
-                dim3(DM_THREADS), 0, cs.stream, A, mcap, m_dev, D, n, keep);
-  return rsqf_rebuild<K, KEEP_FLAGS, false>(cs.stream, ds, ar, f, A, mcap, m_dev, (const K*)nullptr, 0u, keep);
+  const double dper = (double)((hb ? 4 : sizeof(K)) + 1);
+  if constexpr (sizeof(K) == 8) {
+    if (hb)
+      GQF_LAUNCH_BC(0.0, m_dev, dper, (k_delete_mark<K, true>), dim3((mcap + DM_TILE - 1) / DM_TILE), dim3(DM_THREADS),
+                    0, cs.stream, A, mcap, m_dev, D, n, keep, hb);
+  }
+  if (!hb)
+    GQF_LAUNCH_BC(0.0, m_dev, dper, (k_delete_mark<K, false>), dim3((mcap + DM_TILE - 1) / DM_TILE), dim3(DM_THREADS),
+                  0, cs.stream, A, mcap, m_dev, D, n, keep, (const uint32_t*)nullptr);
+  return rsqf_rebuild<K, KEEP_FLAGS, false>(cs.stream, ds, ar, f, A, mcap, m_dev, (const K*)nullptr, 0u, keep, hb);
 }
 
 static void launch_rlookup(hipStream_t s, const gqf_rsqf& f, uint32_t n, const uint32_t* keys, uint32_t* out) {

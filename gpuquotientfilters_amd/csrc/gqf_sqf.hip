@@ -117,8 +117,8 @@ static gqf_status launch_rebuild(hipStream_t s, DeviceState& ds, Arena& ar, cons
   const uint32_t nadd = MERGE ? nb : 0u;
   const double nknown = na_dev ? (double)nadd : (double)N;  // entries the host knows of; the rest: 4 B each on the device
   if (MERGE)
-    GQF_LAUNCH(k_rb_splits<uint32_t>, dim3((p.nsub + 1 + 255) / 256), dim3(256), 0, s, A, na, na_dev, B, nb, p.nsub,
-               splits);
+    GQF_LAUNCH((k_rb_splits<uint32_t, false>), dim3((p.nsub + 1 + 255) / 256), dim3(256), 0, s, A, na, na_dev, B, nb,
+               p.nsub, splits, (const uint32_t*)nullptr);
   GQF_LAUNCH_BC(nknown * 4, na_dev, 4, (k_rb_agg<MODE, MERGE>), dim3(p.G), dim3(RB_THREADS), 0, s, A, na, na_dev, B,
                 nb, flags, splits, g.q, g.r, p.chunk, agg);
   GQF_TRY(rb_scan(s, ar, agg, p.G, (uint64_t)g.S, ds.counters, nullptr));
@@ -296,8 +296,8 @@ static gqf_status sqf_delete_impl(const gqf_sqf& qf, int n, const uint32_t* d_ke
     set_error("internal: delete workspace overflow");
     return GQF_ERR_INVALID;
   }
-  GQF_LAUNCH_BC(0.0, m_dev, 5, k_delete_mark<uint32_t>, dim3((g.S + DM_TILE - 1) / DM_TILE), dim3(DM_THREADS), 0,
-                cs.stream, A, g.S, m_dev, D, un, keep);
+  GQF_LAUNCH_BC(0.0, m_dev, 5, (k_delete_mark<uint32_t, false>), dim3((g.S + DM_TILE - 1) / DM_TILE), dim3(DM_THREADS),
+                0, cs.stream, A, g.S, m_dev, D, un, keep, (const uint32_t*)nullptr);
   GQF_TRY((launch_rebuild<KEEP_FLAGS, false>(cs.stream, ds, ar, qf, g, A, g.S, m_dev, nullptr, 0, keep)));
   return cs.end(ms, true);  // deleting only shrinks the layout: it always fits
 }

@@ -234,3 +234,26 @@ def test_count_query(q, r):
     got = out.to_host()
     assert (got == exp).all(), np.flatnonzero(got != exp)[:8]
     assert got[n // 2] >= 200
+
+
+@pytest.mark.parametrize("q,r,alpha", [(24, 10, 0.2), (26, 8, 0.05)])
+def test_wide_decode_bounds_crossed_by_a_long_run(q, r, alpha):
+    """34-bit fingerprints decode into 32-bit low words plus the indices where
+    the high word steps up.  6000 copies of a key whose quotient lies just
+    below the first step form a run that reaches into later 4096-slot decode
+    windows; insert and delete against the canonical layout."""
+    say("rsqf wide decode bounds", q, r)
+    step = 1 << (32 - r)  # quotient where the high word becomes 1
+    cand = O.mt19937(7, 1 << 21)
+    fq = O.fingerprints(cand, q, r) >> np.uint64(r)
+    hot = cand[(fq >= step - 300) & (fq < step - 100)][0]
+    rnd = O.mt19937(8, O.key_count(alpha, q))
+    base = np.concatenate([rnd, np.full(6000, hot, np.uint32)])
+    f = G.RSQF(q, r)
+    f.bulk_build(dev(base))
+    extra = np.concatenate([O.mt19937(9, 50000), np.full(500, hot, np.uint32)])
+    f.insertGPU(dev(extra))
+    assert (f.dump() == canon(np.concatenate([base, extra]), q, r)).all()
+    f.deleteGPU(dev(np.concatenate([np.full(3000, hot, np.uint32), rnd[:20000]])))
+    rest = np.concatenate([rnd[20000:], extra[:50000], np.full(3500, hot, np.uint32)])
+    assert (f.dump() == canon(rest, q, r)).all()
