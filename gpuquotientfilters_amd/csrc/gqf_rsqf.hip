@@ -978,12 +978,9 @@ static gqf_status rsqf_insert_t(const gqf_rsqf& f, uint32_t n, const uint32_t* k
   K* A;
   const uint32_t* m_dev = nullptr;
   const uint32_t* hb = nullptr;
-  K* B;
-  // the batch is sorted on the side stream while the table is decoded
-  GQF_TRY(fork_side(ds, cs.stream));
-  GQF_TRY(radix_sort<K>(ds.side, ds, ar, keys, true, g.bits, nullptr, false, n, 0, g.bits, &B, nullptr));
   GQF_TRY(rsqf_decode<K>(cs.stream, ar, f, &A, &m_dev, &hb));
-  GQF_TRY(join_side(ds, cs.stream));
+  K* B;
+  GQF_TRY(radix_sort<K>(cs.stream, ds, ar, keys, true, g.bits, nullptr, false, n, 0, g.bits, &B, nullptr));
   // at most cap stored entries: the grids are sized for cap + n
   return rsqf_rebuild<K, KEEP_ALL, true>(cs.stream, ds, ar, f, A, (uint32_t)g.cap, m_dev, B, n, nullptr, hb);
 }
@@ -1003,12 +1000,9 @@ static gqf_status rsqf_delete_t(const gqf_rsqf& f, uint32_t n, const uint32_t* k
   K* A;
   const uint32_t* m_dev = nullptr;
   const uint32_t* hb = nullptr;
-  K* D;
-  // the deletes are sorted on the side stream while the table is decoded
-  GQF_TRY(fork_side(ds, cs.stream));
-  GQF_TRY(radix_sort<K>(ds.side, ds, ar, keys, true, g.bits, nullptr, false, n, 0, g.bits, &D, nullptr));
   GQF_TRY(rsqf_decode<K>(cs.stream, ar, f, &A, &m_dev, &hb));
-  GQF_TRY(join_side(ds, cs.stream));
+  K* D;
+  GQF_TRY(radix_sort<K>(cs.stream, ds, ar, keys, true, g.bits, nullptr, false, n, 0, g.bits, &D, nullptr));
   const uint32_t mcap = (uint32_t)g.cap;  // the grids cover the capacity; kernels read the count
   uint8_t* keep = ar.take<uint8_t>((size_t)mcap + 1);
   if (ar.used > ar.cap) {

@@ -94,12 +94,6 @@ static gqf_status release_state(hipStream_t stream) {
   {
     std::lock_guard<std::mutex> lk(ds->mu);
     GQF_HIP(hipStreamSynchronize(stream));
-    if (ds->side) {
-      GQF_HIP(hipStreamSynchronize(ds->side));
-      GQF_HIP(hipStreamDestroy(ds->side));
-      GQF_HIP(hipEventDestroy(ds->ev_fork));
-      GQF_HIP(hipEventDestroy(ds->ev_join));
-    }
     if (ds->ws) GQF_HIP(hipFree(ds->ws));
     GQF_HIP(hipFree(ds->counters));
     GQF_HIP(hipHostFree(ds->h_pinned));
@@ -115,28 +109,10 @@ static gqf_status release_state(hipStream_t stream) {
 
 size_t ws_alloc_bytes(size_t bytes) { return bytes + bytes / 4 + (1u << 20); }
 
-gqf_status fork_side(DeviceState& ds, hipStream_t main) {
-  if (!ds.side) {
-    GQF_HIP(hipStreamCreateWithFlags(&ds.side, hipStreamNonBlocking));
-    GQF_HIP(hipEventCreateWithFlags(&ds.ev_fork, hipEventDisableTiming));
-    GQF_HIP(hipEventCreateWithFlags(&ds.ev_join, hipEventDisableTiming));
-  }
-  GQF_HIP(hipEventRecord(ds.ev_fork, main));
-  GQF_HIP(hipStreamWaitEvent(ds.side, ds.ev_fork, 0));
-  return GQF_OK;
-}
-
-gqf_status join_side(DeviceState& ds, hipStream_t main) {
-  GQF_HIP(hipEventRecord(ds.ev_join, ds.side));
-  GQF_HIP(hipStreamWaitEvent(main, ds.ev_join, 0));
-  return GQF_OK;
-}
-
 gqf_status ws_reserve(DeviceState& ds, size_t bytes) {
   if (bytes <= ds.ws_cap) return GQF_OK;
   size_t cap = ws_alloc_bytes(bytes);
   GQF_HIP(hipStreamSynchronize(ds.stream));
-  if (ds.side) GQF_HIP(hipStreamSynchronize(ds.side));
   if (ds.ws) GQF_HIP(hipFree(ds.ws));
   ds.ws = nullptr;
   ds.ws_cap = 0;

@@ -75,20 +75,11 @@ struct DeviceState {
   size_t pool_size = 0, pool_used = 0;
   ProfRec* pending = nullptr;
   size_t pending_size = 0, pending_used = 0;
-  // a second stream (default priority) for stages of a call that do not
-  // depend on each other, and the events that fork and join it
-  hipStream_t side = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
 
 // The context of the calling thread's current device and `stream`, created
 // on first use (contexts live until the process ends).
 gqf_status device_state(hipStream_t stream, DeviceState** out);
-// Work launched on ds.side after fork_side() starts after everything already
-// queued on `main`; after join_side(), work on `main` starts after everything
-// queued on ds.side.  The side stream is created on first use.
-gqf_status fork_side(DeviceState& ds, hipStream_t main);
-gqf_status join_side(DeviceState& ds, hipStream_t main);
 // Bytes a workspace grows to when a call needs `bytes` more than it holds.
 size_t ws_alloc_bytes(size_t bytes);
 gqf_status ws_reserve(DeviceState& ds, size_t bytes);

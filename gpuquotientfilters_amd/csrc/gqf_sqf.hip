@@ -252,11 +252,9 @@ static gqf_status sqf_insert_impl(const gqf_sqf& qf, int n, const uint32_t* d_ke
   GQF_TRY(ws_reserve(ds, sort_workspace_bytes(un, 4, false) + std::max(full, local ? sqf_local_bytes(g, un) : 0)));
   Arena ar(ds.ws, ds.ws_cap);
   uint32_t* B = nullptr;
-  uint32_t* A;
-  const uint32_t* m_dev = nullptr;
+  GQF_TRY(radix_sort<uint32_t>(cs.stream, ds, ar, d_keys, true, (g.q + g.r) | sqf_hash_flag(), nullptr, false, un, 0, g.q + g.r, &B,
+                               nullptr));
   if (local) {
-    GQF_TRY(radix_sort<uint32_t>(cs.stream, ds, ar, d_keys, true, (g.q + g.r) | sqf_hash_flag(), nullptr, false, un, 0,
-                                 g.q + g.r, &B, nullptr));
     const size_t mark = ar.used;
     bool done = false;
     GQF_LAUNCH(k_reset_counters, dim3(1), dim3(64), 0, cs.stream, ds.counters, (int)CTR_OP, (int)CTR_TOTAL);
@@ -266,15 +264,10 @@ static gqf_status sqf_insert_impl(const gqf_sqf& qf, int n, const uint32_t* d_ke
     else GQF_TRY(sqf_insert_local<4>(cs, ar, qf, g, B, un, &done));
     if (done) return cs.end(ms, true);
     ar.used = mark;
-    GQF_TRY(sqf_decode(cs.stream, ar, qf, g, &A, &m_dev));
-  } else {
-    // the batch is sorted on the side stream while the table is decoded
-    GQF_TRY(fork_side(ds, cs.stream));
-    GQF_TRY(radix_sort<uint32_t>(ds.side, ds, ar, d_keys, true, (g.q + g.r) | sqf_hash_flag(), nullptr, false, un, 0,
-                                 g.q + g.r, &B, nullptr));
-    GQF_TRY(sqf_decode(cs.stream, ar, qf, g, &A, &m_dev));
-    GQF_TRY(join_side(ds, cs.stream));
   }
+  uint32_t* A;
+  const uint32_t* m_dev = nullptr;
+  GQF_TRY(sqf_decode(cs.stream, ar, qf, g, &A, &m_dev));
   // at most S stored entries: grids sized for S + n, the count read on the device
   if (mode == KEEP_TAGGED_INSERT) return sqf_merge_into<KEEP_TAGGED_INSERT>(cs, ar, qf, g, A, g.S, m_dev, B, un, ms);
   if (mode == KEEP_UNIQUE) return sqf_merge_into<KEEP_UNIQUE>(cs, ar, qf, g, A, g.S, m_dev, B, un, ms);
@@ -293,14 +286,11 @@ static gqf_status sqf_delete_impl(const gqf_sqf& qf, int n, const uint32_t* d_ke
                              rebuild_ws_bytes(g.S)));
   Arena ar(ds.ws, ds.ws_cap);
   uint32_t* D = nullptr;
+  GQF_TRY(radix_sort<uint32_t>(cs.stream, ds, ar, d_keys, true, (g.q + g.r) | sqf_hash_flag(), nullptr, false, un, 0, g.q + g.r, &D,
+                               nullptr));
   uint32_t* A;
   const uint32_t* m_dev = nullptr;
-  // the deletes are sorted on the side stream while the table is decoded
-  GQF_TRY(fork_side(ds, cs.stream));
-  GQF_TRY(radix_sort<uint32_t>(ds.side, ds, ar, d_keys, true, (g.q + g.r) | sqf_hash_flag(), nullptr, false, un, 0,
-                               g.q + g.r, &D, nullptr));
   GQF_TRY(sqf_decode(cs.stream, ar, qf, g, &A, &m_dev));
-  GQF_TRY(join_side(ds, cs.stream));
   uint8_t* keep = ar.take<uint8_t>((size_t)g.S + 1);
   if (ar.used > ar.cap) {
     set_error("internal: delete workspace overflow");
