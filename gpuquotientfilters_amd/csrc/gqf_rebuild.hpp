@@ -68,6 +68,10 @@ __device__ __forceinline__ RS rs_wave_excl(RS x, RS* total) {
   return rs_dpp<DPP_WAVE_SHR1>(inc);
 }
 
+// Waits for every vector memory operation of the thread, global->LDS stage
+// loads among them; with a barrier after it, staged LDS data is readable.
+__device__ __forceinline__ void rb_stage_wait() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
 // Merge-path split (A entries among the first u*RB_TILE outputs) of every sub-tile boundary.
 // na_dev (may be NULL): A's count on the device, overriding na.  PK: A is
 // packed (FpArray) with its bounds at hb.

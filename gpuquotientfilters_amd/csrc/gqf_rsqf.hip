@@ -449,47 +449,93 @@ __global__ __launch_bounds__(RD_THREADS) void k_rd_emit(RsqfDev f, const uint32_
 // (RSQF.cu:882-988) with the end state of inserting the fingerprints in
 // ascending order (RSQF.cpp:314-400).
 constexpr uint32_t RW_WB = 128;  // blocks per LDS window (8192 slots)
+constexpr uint32_t RR_MERGE_PER = 2;  // sub-tiles per workgroup in merge rebuilds (the next one staged)
 
-template <typename K, int MODE, bool MERGE, bool PK>
-__device__ __forceinline__ void rr_load(const FpArray<K, PK>& A, const K* __restrict__ B,
-                                        const uint8_t* __restrict__ flags, const uint32_t* __restrict__ splits,
-                                        uint32_t u, uint32_t d0, uint32_t tn, K* s_stage, K* v, uint32_t& fl) {
+// Merge rebuilds stage a sub-tile's inputs in LDS with global->LDS loads,
+// which take no VGPRs and run on while the workgroup processes the previous
+// sub-tile: 32-bit words, the low words in s_lo and, for 64-bit values, the
+// high words in s_hi (a packed A takes its high words from the index).
+// Positions [0, a1 - a0) hold A[a0 ..], the rest B[d0 - a0 ..].
+template <typename K, bool PK>
+__device__ __forceinline__ void rr_stage_issue(const FpArray<K, PK>& A, const K* __restrict__ B, uint32_t a0,
+                                               uint32_t a1, uint32_t d0, uint32_t tn, uint32_t* s_lo,
+                                               uint32_t* s_hi) {
+  constexpr bool WIDE = sizeof(K) == 8;
+  const uint32_t la = a1 - a0, b0 = d0 - a0;
+  const uint32_t wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const uint32_t* A32 = reinterpret_cast<const uint32_t*>(A.p);
+  const uint32_t* B32 = reinterpret_cast<const uint32_t*>(B);
+#pragma unroll
+  for (int k = 0; k < RB_ITEMS; ++k) {
+    // wave wid, step k: the 64 consecutive positions from base (lane l -> base + l)
+    const uint32_t base = ((uint32_t)k * RB_WAVES + wid) * 64;
+    const uint32_t pos = base + lane;
+    if (pos < tn) {
+      const bool inA = pos < la;
+      const uint64_t w = inA ? (uint64_t)a0 + pos : (uint64_t)b0 + (pos - la);  // entry index
+      const bool two = WIDE && (!PK || !inA);                                   // 64-bit words in memory
+      const uint32_t* src = (inA ? A32 : B32) + (two ? 2 * w : w);
+      __builtin_amdgcn_global_load_lds(src, s_lo + base, 4, 0, 0);
+      if (two) __builtin_amdgcn_global_load_lds(src + 1, s_hi + base, 4, 0, 0);
+    }
+  }
+}
+
+// Views of a staged sub-tile's A and B parts.
+template <typename K, bool PK>
+struct StagedA {
+  const uint32_t* lo;
+  const uint32_t* hi;
+  FpArray<K, PK> A;
+  uint32_t a0;
+  __device__ __forceinline__ K operator[](uint32_t i) const {
+    if constexpr (PK) return A.widen(a0 + i, lo[i]);
+    else if constexpr (sizeof(K) == 8) return (K)(((uint64_t)hi[i] << 32) | lo[i]);
+    else return (K)lo[i];
+  }
+};
+template <typename K>
+struct StagedB {
+  const uint32_t* lo;
+  const uint32_t* hi;
+  __device__ __forceinline__ K operator[](uint32_t j) const {
+    if constexpr (sizeof(K) == 8) return (K)(((uint64_t)hi[j] << 32) | lo[j]);
+    else return (K)lo[j];
+  }
+};
+
+// This thread's RB_ITEMS entries of the stable merge (A first on ties) of a
+// staged sub-tile with A range [a0, a1) and tn entries.
+template <typename K, bool PK>
+__device__ __forceinline__ void rr_merge_staged(const FpArray<K, PK>& A, const uint32_t* s_lo, const uint32_t* s_hi,
+                                                uint32_t a0, uint32_t a1, uint32_t tn, K* v) {
+  const uint32_t la = a1 - a0, lb = tn - la;
+  const StagedA<K, PK> sA{s_lo, s_hi, A, a0};
+  const StagedB<K> sB{s_lo + la, s_hi + la};
+  const uint32_t i0 = (uint32_t)threadIdx.x * RB_ITEMS;
+  const uint32_t ld = i0 < tn ? i0 : tn;
+  uint32_t ai = merge_path(sA, la, sB, lb, ld);
+  uint32_t bi = ld - ai;
+#pragma unroll
+  for (int j = 0; j < RB_ITEMS; ++j) {
+    K x = 0;
+    if (ld + j < tn) {
+      const bool takeA = bi >= lb || (ai < la && sA[ai] <= sB[bi]);
+      x = takeA ? sA[ai++] : sB[bi++];
+    }
+    v[j] = x;
+  }
+}
+
+// This thread's RB_ITEMS consecutive entries of sub-tile [d0, d0+tn) of A
+// (no merge) and, under KEEP_FLAGS, their flags.
+template <typename K, int MODE, bool PK>
+__device__ __forceinline__ void rr_load(const FpArray<K, PK>& A, const uint8_t* __restrict__ flags, uint32_t d0,
+                                        uint32_t tn, K* v, uint32_t& fl) {
   const int tid = threadIdx.x;
   const uint32_t i0 = (uint32_t)tid * RB_ITEMS;
   fl = 0;
-  if (MERGE) {
-    const uint32_t a0 = splits[u], a1 = splits[u + 1];
-    const uint32_t b0 = d0 - a0, b1 = d0 + tn - a1;
-    const uint32_t la = a1 - a0, lb = b1 - b0;  // la + lb == tn <= RB_TILE
-    {
-      K w[RB_ITEMS];
-#pragma unroll
-      for (int k = 0; k < RB_ITEMS; ++k) {
-        const uint32_t i = tid + k * RB_THREADS;
-        w[k] = i < la ? A[a0 + i] : (i < la + lb ? B[b0 + (i - la)] : (K)0);
-      }
-#pragma unroll
-      for (int k = 0; k < RB_ITEMS; ++k) {
-        const uint32_t i = tid + k * RB_THREADS;
-        if (i < la + lb) s_stage[i] = w[k];
-      }
-    }
-    __syncthreads();
-    const K* sA = s_stage;
-    const K* sB = s_stage + la;
-    const uint32_t ld = i0 < tn ? i0 : tn;
-    uint32_t ai = merge_path(sA, la, sB, lb, ld);
-    uint32_t bi = ld - ai;
-#pragma unroll
-    for (int j = 0; j < RB_ITEMS; ++j) {
-      K x = 0;
-      if (ld + j < tn) {
-        const bool takeA = bi >= lb || (ai < la && sA[ai] <= sB[bi]);
-        x = takeA ? sA[ai++] : sB[bi++];
-      }
-      v[j] = x;
-    }
-  } else {
+  {
     const uint64_t g0 = (uint64_t)d0 + i0;
     if (i0 + RB_ITEMS <= tn) {
       // d0 is a multiple of RB_TILE and A is 256-byte aligned
@@ -558,7 +604,8 @@ __global__ __launch_bounds__(RB_THREADS, 8) void k_rr_agg(const K* __restrict__ 
                                                           const uint32_t* __restrict__ splits, uint32_t q, uint32_t r,
                                                           uint32_t chunk, uint4* __restrict__ agg,
                                                           const uint32_t* __restrict__ hb) {
-  __shared__ K s_stage[MERGE ? RB_TILE : 1];
+  __shared__ uint32_t s_lo[MERGE ? RB_TILE : 1];
+  __shared__ uint32_t s_hi[MERGE && sizeof(K) == 8 ? RB_TILE : 1];
   __shared__ uint4 s_wave[RB_WAVES + 1];
   const FpArray<K, PK> A = fp_array<K, PK>(Araw, hb);
   if (na_dev) na = *na_dev;  // stored count on the device (the grid covers an upper bound)
@@ -567,11 +614,38 @@ __global__ __launch_bounds__(RB_THREADS, 8) void k_rr_agg(const K* __restrict__ 
   const uint64_t c0 = (uint64_t)blockIdx.x * chunk;
   const uint64_t c1 = c0 + chunk < N ? c0 + chunk : N;
   RS X = rs_none();
+  // MERGE: splits of the current sub-tile (ca0, ca1) and the next one's end
+  // (na1), loaded a sub-tile ahead
+  uint32_t ca0 = 0, ca1 = 0, na1 = 0;
+  if (MERGE && c0 < c1) {
+    const uint32_t u0 = (uint32_t)(c0 / RB_TILE);
+    ca0 = splits[u0];
+    ca1 = splits[u0 + 1];
+    if (c0 + RB_TILE < c1) na1 = splits[u0 + 2];
+    rr_stage_issue<K, PK>(A, B, ca0, ca1, (uint32_t)c0, (uint32_t)(c1 - c0 < RB_TILE ? c1 - c0 : RB_TILE), s_lo,
+                          s_hi);
+  }
   for (uint64_t d0 = c0; d0 < c1; d0 += RB_TILE) {
     const uint32_t tn = (uint32_t)(c1 - d0 < (uint64_t)RB_TILE ? c1 - d0 : (uint64_t)RB_TILE);
     K v[RB_ITEMS];
-    uint32_t fl;
-    rr_load<K, MODE, MERGE, PK>(A, B, flags, splits, (uint32_t)(d0 / RB_TILE), (uint32_t)d0, tn, s_stage, v, fl);
+    uint32_t fl = 0;
+    if (MERGE) {
+      const uint32_t u = (uint32_t)(d0 / RB_TILE);
+      const uint64_t dn = d0 + RB_TILE;
+      rb_stage_wait();
+      __syncthreads();
+      rr_merge_staged<K, PK>(A, s_lo, s_hi, ca0, ca1, tn, v);
+      const uint32_t nn1 = dn + RB_TILE < c1 ? splits[u + 3] : 0u;
+      __syncthreads();  // every thread holds its entries: the stage is free
+      if (dn < c1)
+        rr_stage_issue<K, PK>(A, B, ca1, na1, (uint32_t)dn, (uint32_t)(c1 - dn < RB_TILE ? c1 - dn : RB_TILE), s_lo,
+                              s_hi);
+      ca0 = ca1;
+      ca1 = na1;
+      na1 = nn1;
+    } else {
+      rr_load<K, MODE, PK>(A, flags, (uint32_t)d0, tn, v, fl);
+    }
     RS a;
     (void)rr_classify<K, MODE>(v, fl, tn, r, mq, a);
     RS ex;
@@ -643,7 +717,8 @@ __global__ __launch_bounds__(RB_THREADS, 6) void k_rr_write(uint8_t* __restrict_
                                                             const uint32_t* __restrict__ hb) {
   if (ctr[CTR_ERR]) return;  // capacity error: the table stays as it was
   const FpArray<K, PK> A = fp_array<K, PK>(Araw, hb);
-  __shared__ K s_stage[MERGE ? RB_TILE : 1];
+  __shared__ uint32_t s_lo[MERGE ? RB_TILE : 1];
+  __shared__ uint32_t s_hi[MERGE && sizeof(K) == 8 ? RB_TILE : 1];
   __shared__ __align__(16) uint32_t s_rem[RW_WB * 32];  // 2r words per block, r <= 16
   __shared__ uint32_t s_occ[RW_WB * 2];
   __shared__ uint32_t s_end[RW_WB * 2];
@@ -662,12 +737,36 @@ __global__ __launch_bounds__(RB_THREADS, 6) void k_rr_write(uint8_t* __restrict_
   uint32_t* remg = reinterpret_cast<uint32_t*>(base + rsqf_meta_bytes(nb));
   const uint32_t rw = 2 * rr;  // remainder words per block
   RS X = rs_unpack(pre[blockIdx.x]);
+  uint32_t ca0 = 0, ca1 = 0, na1 = 0;  // as in k_rr_agg
+  if (MERGE && c0 < c1) {
+    const uint32_t u0 = (uint32_t)(c0 / RB_TILE);
+    ca0 = splits[u0];
+    ca1 = splits[u0 + 1];
+    if (c0 + RB_TILE < c1) na1 = splits[u0 + 2];
+    rr_stage_issue<K, PK>(A, B, ca0, ca1, (uint32_t)c0, (uint32_t)(c1 - c0 < RB_TILE ? c1 - c0 : RB_TILE), s_lo,
+                          s_hi);
+  }
   for (uint64_t d0 = c0; d0 < c1; d0 += RB_TILE) {
     const uint32_t u = (uint32_t)(d0 / RB_TILE);
     const uint32_t tn = (uint32_t)(c1 - d0 < (uint64_t)RB_TILE ? c1 - d0 : (uint64_t)RB_TILE);
     K v[RB_ITEMS];
-    uint32_t fl;
-    rr_load<K, MODE, MERGE, PK>(A, B, flags, splits, u, (uint32_t)d0, tn, s_stage, v, fl);
+    uint32_t fl = 0;
+    if (MERGE) {
+      const uint64_t dn = d0 + RB_TILE;
+      rb_stage_wait();
+      __syncthreads();
+      rr_merge_staged<K, PK>(A, s_lo, s_hi, ca0, ca1, tn, v);
+      const uint32_t nn1 = dn + RB_TILE < c1 ? splits[u + 3] : 0u;
+      __syncthreads();  // every thread holds its entries: the stage is free
+      if (dn < c1)
+        rr_stage_issue<K, PK>(A, B, ca1, na1, (uint32_t)dn, (uint32_t)(c1 - dn < RB_TILE ? c1 - dn : RB_TILE), s_lo,
+                              s_hi);
+      ca0 = ca1;
+      ca1 = na1;
+      na1 = nn1;
+    } else {
+      rr_load<K, MODE, PK>(A, flags, (uint32_t)d0, tn, v, fl);
+    }
     RS a;
     const uint32_t keepbits = rr_classify<K, MODE>(v, fl, tn, rr, mq, a);
     RS ex;
@@ -903,7 +1002,7 @@ static gqf_status rsqf_rebuild_t(hipStream_t s, DeviceState& ds, Arena& ar, cons
                  (uint64_t)tbytes, ds.counters);
     return GQF_OK;
   }
-  const RbPlan p = rb_plan(N);
+  const RbPlan p = rb_plan(N, MERGE ? RR_MERGE_PER : 1u);
   uint32_t* splits = ar.take<uint32_t>(p.nsub + 1);
   uint4* agg = ar.take<uint4>(p.G);
   uint4* total = ar.take<uint4>(1);

@@ -438,7 +438,6 @@ __device__ __forceinline__ void rb_stage_issue(const uint32_t* __restrict__ A, c
   if (wid == 0 && lane < 2 && (lane == 0 ? a0 > 0 : b0 > 0))
     __builtin_amdgcn_global_load_lds(lane == 0 ? A + (a0 - 1) : B + (b0 - 1), prev2, 4, 0, 0);
 }
-__device__ __forceinline__ void rb_stage_wait() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
 // This thread's RB_ITEMS merged entries of a staged sub-tile (as rb_load).
 __device__ __forceinline__ void rb_merge_staged(const uint32_t* buf, const uint32_t* prev2, uint32_t a0, uint32_t a1,
