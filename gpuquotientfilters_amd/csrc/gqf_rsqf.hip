@@ -505,24 +505,49 @@ struct StagedB {
 };
 
 // This thread's RB_ITEMS entries of the stable merge (A first on ties) of a
-// staged sub-tile with A range [a0, a1) and tn entries.
+// staged sub-tile with A range [a0, a1) and tn entries.  A batch adds few
+// entries to a sub-tile, so instead of a merge-path search per thread every
+// B entry finds its output position (its index + the A entries <= it) and
+// marks it in the bitmap s_bmp (RB_TILE bits, zero on entry); a prefix count
+// of the bitmap words (s_wpre) then gives each thread its B and A offsets.
+// Every thread of the block calls it (two barriers inside).
 template <typename K, bool PK>
 __device__ __forceinline__ void rr_merge_staged(const FpArray<K, PK>& A, const uint32_t* s_lo, const uint32_t* s_hi,
-                                                uint32_t a0, uint32_t a1, uint32_t tn, K* v) {
+                                                uint32_t a0, uint32_t a1, uint32_t tn, K* v, uint32_t* s_bmp,
+                                                uint32_t* s_wpre) {
   const uint32_t la = a1 - a0, lb = tn - la;
   const StagedA<K, PK> sA{s_lo, s_hi, A, a0};
   const StagedB<K> sB{s_lo + la, s_hi + la};
-  const uint32_t i0 = (uint32_t)threadIdx.x * RB_ITEMS;
-  const uint32_t ld = i0 < tn ? i0 : tn;
-  uint32_t ai = merge_path(sA, la, sB, lb, ld);
-  uint32_t bi = ld - ai;
+  for (uint32_t j = threadIdx.x; j < lb; j += RB_THREADS) {
+    const K x = sB[j];
+    uint32_t lo = 0, hi = la;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (sA[mid] <= x) lo = mid + 1;
+      else hi = mid;
+    }
+    const uint32_t p = j + lo;
+    atomicOr(&s_bmp[p >> 5], 1u << (p & 31));
+  }
+  __syncthreads();
+  static_assert(RB_TILE == 64 * 64, "one wave counts the bitmap, two words per lane");
+  if (threadIdx.x < 64) {
+    const uint32_t l = threadIdx.x;
+    const uint32_t c0 = (uint32_t)__popc(s_bmp[2 * l]), c1 = (uint32_t)__popc(s_bmp[2 * l + 1]);
+    const uint32_t inc = wave_incl_add(c0 + c1);
+    s_wpre[2 * l] = inc - c0 - c1;
+    s_wpre[2 * l + 1] = inc - c1;
+  }
+  __syncthreads();
+  const uint32_t i0 = (uint32_t)threadIdx.x * RB_ITEMS;  // RB_ITEMS positions inside one bitmap word
+  const uint32_t w = s_bmp[i0 >> 5];
+  uint32_t bi = s_wpre[i0 >> 5] + (uint32_t)__popc(w & ((1u << (i0 & 31)) - 1u));
+  uint32_t ai = i0 - bi;  // bi counts marked positions below i0
+  const uint32_t bits = w >> (i0 & 31);
 #pragma unroll
   for (int j = 0; j < RB_ITEMS; ++j) {
     K x = 0;
-    if (ld + j < tn) {
-      const bool takeA = bi >= lb || (ai < la && sA[ai] <= sB[bi]);
-      x = takeA ? sA[ai++] : sB[bi++];
-    }
+    if (i0 + j < tn) x = ((bits >> j) & 1u) ? sB[bi++] : sA[ai++];
     v[j] = x;
   }
 }
@@ -606,8 +631,10 @@ __global__ __launch_bounds__(RB_THREADS, 8) void k_rr_agg(const K* __restrict__ 
                                                           const uint32_t* __restrict__ hb) {
   __shared__ uint32_t s_lo[MERGE ? RB_TILE : 1];
   __shared__ uint32_t s_hi[MERGE && sizeof(K) == 8 ? RB_TILE : 1];
+  __shared__ uint32_t s_bmp[MERGE ? RB_TILE / 32 : 1], s_wpre[MERGE ? RB_TILE / 32 : 1];
   __shared__ uint4 s_wave[RB_WAVES + 1];
   const FpArray<K, PK> A = fp_array<K, PK>(Araw, hb);
+  if (MERGE && threadIdx.x < RB_TILE / 32) s_bmp[threadIdx.x] = 0u;
   if (na_dev) na = *na_dev;  // stored count on the device (the grid covers an upper bound)
   const uint64_t N = (uint64_t)na + (MERGE ? nbk : 0u);
   const uint32_t mq = (uint32_t)mask64(q);
@@ -634,9 +661,10 @@ __global__ __launch_bounds__(RB_THREADS, 8) void k_rr_agg(const K* __restrict__ 
       const uint64_t dn = d0 + RB_TILE;
       rb_stage_wait();
       __syncthreads();
-      rr_merge_staged<K, PK>(A, s_lo, s_hi, ca0, ca1, tn, v);
+      rr_merge_staged<K, PK>(A, s_lo, s_hi, ca0, ca1, tn, v, s_bmp, s_wpre);
       const uint32_t nn1 = dn + RB_TILE < c1 ? splits[u + 3] : 0u;
-      __syncthreads();  // every thread holds its entries: the stage is free
+      __syncthreads();  // every thread holds its entries: the stage and the bitmap are free
+      if (threadIdx.x < RB_TILE / 32) s_bmp[threadIdx.x] = 0u;
       if (dn < c1)
         rr_stage_issue<K, PK>(A, B, ca1, na1, (uint32_t)dn, (uint32_t)(c1 - dn < RB_TILE ? c1 - dn : RB_TILE), s_lo,
                               s_hi);
@@ -719,6 +747,8 @@ __global__ __launch_bounds__(RB_THREADS, 6) void k_rr_write(uint8_t* __restrict_
   const FpArray<K, PK> A = fp_array<K, PK>(Araw, hb);
   __shared__ uint32_t s_lo[MERGE ? RB_TILE : 1];
   __shared__ uint32_t s_hi[MERGE && sizeof(K) == 8 ? RB_TILE : 1];
+  __shared__ uint32_t s_bmp[MERGE ? RB_TILE / 32 : 1], s_wpre[MERGE ? RB_TILE / 32 : 1];
+  if (MERGE && threadIdx.x < RB_TILE / 32) s_bmp[threadIdx.x] = 0u;
   __shared__ __align__(16) uint32_t s_rem[RW_WB * 32];  // 2r words per block, r <= 16
   __shared__ uint32_t s_occ[RW_WB * 2];
   __shared__ uint32_t s_end[RW_WB * 2];
@@ -755,9 +785,10 @@ __global__ __launch_bounds__(RB_THREADS, 6) void k_rr_write(uint8_t* __restrict_
       const uint64_t dn = d0 + RB_TILE;
       rb_stage_wait();
       __syncthreads();
-      rr_merge_staged<K, PK>(A, s_lo, s_hi, ca0, ca1, tn, v);
+      rr_merge_staged<K, PK>(A, s_lo, s_hi, ca0, ca1, tn, v, s_bmp, s_wpre);
       const uint32_t nn1 = dn + RB_TILE < c1 ? splits[u + 3] : 0u;
-      __syncthreads();  // every thread holds its entries: the stage is free
+      __syncthreads();  // every thread holds its entries: the stage and the bitmap are free
+      if (threadIdx.x < RB_TILE / 32) s_bmp[threadIdx.x] = 0u;
       if (dn < c1)
         rr_stage_issue<K, PK>(A, B, ca1, na1, (uint32_t)dn, (uint32_t)(c1 - dn < RB_TILE ? c1 - dn : RB_TILE), s_lo,
                               s_hi);

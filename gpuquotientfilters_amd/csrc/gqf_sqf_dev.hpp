@@ -439,31 +439,66 @@ __device__ __forceinline__ void rb_stage_issue(const uint32_t* __restrict__ A, c
     __builtin_amdgcn_global_load_lds(lane == 0 ? A + (a0 - 1) : B + (b0 - 1), prev2, 4, 0, 0);
 }
 
-// This thread's RB_ITEMS merged entries of a staged sub-tile (as rb_load).
+// This thread's RB_ITEMS merged entries of a staged sub-tile (as rb_load;
+// tags bit j: entry j came from B).  A batch adds few entries to a sub-tile,
+// so instead of a merge-path search per thread every B entry finds its
+// output position (its index + the A entries <= it: A first on ties) and
+// marks it in the bitmap s_bmp (RB_TILE bits, zero on entry); a prefix count
+// of the bitmap words (s_wpre) gives each thread its A and B offsets.  Every
+// thread of the block calls it (two barriers inside).
 __device__ __forceinline__ void rb_merge_staged(const uint32_t* buf, const uint32_t* prev2, uint32_t a0, uint32_t a1,
-                                                uint32_t d0, uint32_t tn, uint32_t* v, uint32_t& tags) {
-  const uint32_t i0 = (uint32_t)threadIdx.x * RB_ITEMS;
+                                                uint32_t d0, uint32_t tn, uint32_t* v, uint32_t& tags,
+                                                uint32_t* s_bmp, uint32_t* s_wpre) {
   const uint32_t b0 = d0 - a0, la = a1 - a0, lb = tn - la;
   const uint32_t* sA = buf;
   const uint32_t* sB = buf + la;
-  const uint32_t ld = i0 < tn ? i0 : tn;
-  uint32_t ai = merge_path(sA, la, sB, lb, ld);
-  uint32_t bi = ld - ai;
-  const bool hasA = ai > 0 || a0 > 0, hasB = bi > 0 || b0 > 0;
-  const uint32_t pA = hasA ? (ai > 0 ? sA[ai - 1] : prev2[0]) : 0u;
-  const uint32_t pB = hasB ? (bi > 0 ? sB[bi - 1] : prev2[1]) : 0u;
-  v[0] = (hasB && (!hasA || pA <= pB)) ? pB : pA;
+  for (uint32_t j = threadIdx.x; j < lb; j += RB_THREADS) {
+    const uint32_t x = sB[j];
+    uint32_t lo = 0, hi = la;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (sA[mid] <= x) lo = mid + 1;
+      else hi = mid;
+    }
+    const uint32_t p = j + lo;
+    atomicOr(&s_bmp[p >> 5], 1u << (p & 31));
+  }
+  __syncthreads();
+  static_assert(RB_TILE == 64 * 64, "one wave counts the bitmap, two words per lane");
+  if (threadIdx.x < 64) {
+    const uint32_t l = threadIdx.x;
+    const uint32_t c0 = (uint32_t)__popc(s_bmp[2 * l]), c1 = (uint32_t)__popc(s_bmp[2 * l + 1]);
+    const uint32_t inc = wave_incl_add(c0 + c1);
+    s_wpre[2 * l] = inc - c0 - c1;
+    s_wpre[2 * l + 1] = inc - c1;
+  }
+  __syncthreads();
+  const uint32_t i0 = (uint32_t)threadIdx.x * RB_ITEMS;  // RB_ITEMS positions inside one bitmap word
+  const uint32_t w = s_bmp[i0 >> 5];
+  uint32_t bi = s_wpre[i0 >> 5] + (uint32_t)__popc(w & ((1u << (i0 & 31)) - 1u));
+  uint32_t ai = i0 - bi;  // bi counts marked positions below i0
+  // v[0]: the entry before position i0 (the sub-tile's predecessor for i0 = 0)
+  if (i0 == 0) {
+    const bool hasA = a0 > 0, hasB = b0 > 0;
+    const uint32_t pA = hasA ? prev2[0] : 0u, pB = hasB ? prev2[1] : 0u;
+    v[0] = (hasB && (!hasA || pA <= pB)) ? pB : pA;
+  } else if (i0 <= tn) {
+    const bool pb = (s_bmp[(i0 - 1) >> 5] >> ((i0 - 1) & 31)) & 1u;
+    v[0] = pb ? sB[bi - 1] : sA[ai - 1];
+  } else {
+    v[0] = 0u;
+  }
+  const uint32_t bits = w >> (i0 & 31);
   tags = 0;
 #pragma unroll
   for (int j = 0; j < RB_ITEMS; ++j) {
     uint32_t x = 0;
-    if (ld + j < tn) {
-      const bool takeA = bi >= lb || (ai < la && sA[ai] <= sB[bi]);
-      if (takeA) {
-        x = sA[ai++];
-      } else {
+    if (i0 + j < tn) {
+      if ((bits >> j) & 1u) {
         x = sB[bi++];
         tags |= 1u << j;
+      } else {
+        x = sA[ai++];
       }
     }
     v[1 + j] = x;
@@ -506,6 +541,7 @@ __global__ __launch_bounds__(RB_THREADS, 8) void k_rb_agg(const uint32_t* __rest
                                                 uint4* __restrict__ agg) {
   __shared__ uint32_t s_stage[MERGE ? 2 : 1][MERGE ? RB_TILE : 1];
   __shared__ uint32_t s_prev[2][2];
+  __shared__ uint32_t s_bmp[MERGE ? RB_TILE / 32 : 1], s_wpre[MERGE ? RB_TILE / 32 : 1];
   __shared__ uint4 s_wave[RB_WAVES + 1];
   if (na_dev) na = *na_dev;
   const uint64_t N = (uint64_t)na + (MERGE ? nb : 0u);
@@ -535,12 +571,14 @@ __global__ __launch_bounds__(RB_THREADS, 8) void k_rb_agg(const uint32_t* __rest
       const uint32_t u = (uint32_t)(d0 / RB_TILE);
       const uint64_t dn = d0 + RB_TILE;
       rb_stage_wait();
+      // the previous merge's bitmap reads ended before the barriers since
+      if (threadIdx.x < RB_TILE / 32) s_bmp[threadIdx.x] = 0u;
       __syncthreads();
       if (dn < c1)
         rb_stage_issue(A, B, na0, na1, (uint32_t)dn, (uint32_t)(c1 - dn < RB_TILE ? c1 - dn : RB_TILE),
                        s_stage[buf ^ 1], s_prev[buf ^ 1]);
       const uint32_t nn1 = dn + RB_TILE < c1 ? splits[u + 3] : 0u;
-      rb_merge_staged(s_stage[buf], s_prev[buf], ca0, ca1, (uint32_t)d0, tn, v, tags);
+      rb_merge_staged(s_stage[buf], s_prev[buf], ca0, ca1, (uint32_t)d0, tn, v, tags, s_bmp, s_wpre);
       ca0 = na0;
       ca1 = na1;
       na0 = na1;
@@ -709,6 +747,7 @@ __global__ __launch_bounds__(RB_THREADS, 6) void k_rb_write(uint8_t* __restrict_
   if (ctr[CTR_ERR]) return;  // capacity error: the table stays as it was
   __shared__ uint32_t s_stage[MERGE ? 2 : 1][MERGE ? RB_TILE : 1];
   __shared__ uint32_t s_prev[2][2];
+  __shared__ uint32_t s_bmp[MERGE ? RB_TILE / 32 : 1], s_wpre[MERGE ? RB_TILE / 32 : 1];
   __shared__ __align__(16) uint32_t s_win[rb_win_words<BPE>()];
   __shared__ uint4 s_wave[RB_WAVES + 1];
   __shared__ uint32_t s_nfix;
@@ -740,12 +779,14 @@ __global__ __launch_bounds__(RB_THREADS, 6) void k_rb_write(uint8_t* __restrict_
     if (MERGE) {
       const uint64_t dn = d0 + RB_TILE;
       rb_stage_wait();
+      // the previous merge's bitmap reads ended before the barriers since
+      if (threadIdx.x < RB_TILE / 32) s_bmp[threadIdx.x] = 0u;
       __syncthreads();
       if (dn < c1)
         rb_stage_issue(A, B, na0, na1, (uint32_t)dn, (uint32_t)(c1 - dn < RB_TILE ? c1 - dn : RB_TILE),
                        s_stage[buf ^ 1], s_prev[buf ^ 1]);
       const uint32_t nn1 = dn + RB_TILE < c1 ? splits[u + 3] : 0u;
-      rb_merge_staged(s_stage[buf], s_prev[buf], ca0, ca1, (uint32_t)d0, tn, v, tags);
+      rb_merge_staged(s_stage[buf], s_prev[buf], ca0, ca1, (uint32_t)d0, tn, v, tags, s_bmp, s_wpre);
       ca0 = na0;
       ca1 = na1;
       na0 = na1;
