@@ -16,6 +16,8 @@
 // what the reference insert produces when fed fingerprints in ascending order.
 #include "gqf_rebuild.hpp"
 
+#include <type_traits>
+
 #include <cstring>
 #include <vector>
 
@@ -346,18 +348,21 @@ __global__ __launch_bounds__(RD_THREADS) void k_rd_count(RsqfDev f, const uint32
   if (t == 0) wc[w] = total;
 }
 
-// PK: out receives the low 32 bits of each 64-bit fingerprint, and the
-// window holding quotient k << (32 - r) (k = 1 .. nhv) -- or a later one
-// whose first entries still have smaller quotients -- adds the entries below
-// k << 32 to hb[k - 1], which ends as the index where the high word reaches k.
+// PK: out receives the low 32 bits of each 64-bit fingerprint (so the LDS
+// stage holds 32-bit words too), and the window holding quotient
+// Qk = k << (32 - r) (k = 1 .. nhv) -- or a later one whose first entries
+// still have smaller quotients -- adds its entries of quotient below Qk to
+// hb[k - 1], which ends as the index where the high word reaches k.
 template <typename K, int R, bool PK>
 __global__ __launch_bounds__(RD_THREADS) void k_rd_emit(RsqfDev f, const uint32_t* __restrict__ wp,
                                                         const uint32_t* __restrict__ wc, K* __restrict__ out,
                                                         uint32_t* __restrict__ hb, uint32_t nhv) {
   // the quotient list, then (after every thread has read it) the window's entries
-  __shared__ __align__(16) K s_buf[RD_SLOTS * sizeof(K) >= RD_LIST * 4 ? RD_SLOTS : RD_LIST];
+  using EK = typename std::conditional<PK, uint32_t, K>::type;
+  __shared__ __align__(16) EK s_buf[RD_SLOTS * sizeof(EK) >= RD_LIST * 4 ? RD_SLOTS : RD_LIST];
   __shared__ uint32_t s_tmp4[4];
   __shared__ uint32_t s_v[2];
+  __shared__ uint32_t s_fq;  // PK: quotient of the window's first entry
   uint32_t* s_list = reinterpret_cast<uint32_t*>(s_buf);
   const uint32_t w = blockIdx.x;
   const uint32_t rr = R ? (uint32_t)R : f.r;
@@ -401,26 +406,25 @@ __global__ __launch_bounds__(RD_THREADS) void k_rd_emit(RsqfDev f, const uint32_
   uint32_t p = pos;
 #pragma unroll
   for (int j = 0; j < 16; ++j)
-    if ((m >> j) & 1u) s_buf[p++] = ((K)qv[j] << rr) | (K)rem[j];
+    if ((m >> j) & 1u) s_buf[p++] = (EK)(((K)qv[j] << rr) | (K)rem[j]);
+  if (PK && m && pos == 0) s_fq = qv[__builtin_ctz(m)];  // the thread holding the window's first entry
   __syncthreads();
   if constexpr (PK) {
     uint32_t* dst = reinterpret_cast<uint32_t*>(out) + wc[w];
-    for (uint32_t i = threadIdx.x; i < cnt; i += RD_THREADS) dst[i] = (uint32_t)s_buf[i];
-    if (threadIdx.x == 0) {
-      for (uint32_t k = 1; k <= nhv; ++k) {
-        const uint64_t wk = ((uint64_t)k << (32 - rr)) / RD_SLOTS;  // window of quotient k << (32 - r)
-        if (w < wk) break;
-        const uint64_t thr = (uint64_t)k << 32;
-        if (w > wk && (cnt == 0 || (uint64_t)s_buf[0] >= thr)) continue;
-        uint32_t lo = 0, hi = cnt;  // entries below thr: a prefix of the window's
-        while (lo < hi) {
-          const uint32_t mid = (lo + hi) >> 1;
-          if ((uint64_t)s_buf[mid] < thr) lo = mid + 1;
-          else hi = mid;
-        }
-        const uint32_t add = lo + (w == wk ? wc[w] : 0u);
-        if (add) atomicAdd(&hb[k - 1], add);
-      }
+    for (uint32_t i = threadIdx.x; i < cnt; i += RD_THREADS) dst[i] = s_buf[i];
+    // (w, cnt and s_fq are the same in every thread: the branches are uniform)
+    for (uint32_t k = 1; k <= nhv; ++k) {
+      const uint64_t Qk = (uint64_t)k << (32 - rr);
+      const uint64_t wk = Qk / RD_SLOTS;  // window of quotient Qk
+      if (w < wk) break;
+      if (w > wk && (cnt == 0 || s_fq >= Qk)) continue;
+      uint32_t c = 0;  // this thread's entries of quotient below Qk
+#pragma unroll
+      for (int j = 0; j < 16; ++j) c += ((m >> j) & 1u) && qv[j] < Qk ? 1u : 0u;
+      uint32_t below;
+      (void)block256_excl_scan(c, s_tmp4, &below);
+      const uint32_t add = below + (w == wk ? wc[w] : 0u);
+      if (threadIdx.x == 0 && add) atomicAdd(&hb[k - 1], add);
     }
   } else {
     K* dst = out + wc[w];
