@@ -873,7 +873,9 @@ __device__ __forceinline__ void sqf_set_occupied(uint8_t* t, uint32_t bpe, uint3
 // the occupied positions just before it that still own runs.  Replaces
 // extractQuotients + remove_if (quotientFilter.cu:1561-1600, 1627).
 constexpr uint32_t DC_SLOTS = 4096;
-constexpr uint32_t DC_PEND = 1024;
+// Pending quotients kept in LDS (the rest are found by walking the cluster);
+// 512 keeps k_sqf_decode within 6 workgroups per CU.
+constexpr uint32_t DC_PEND = 512;
 
 // The 16 slots [s, s+16) of a thread: 16-byte loads when whole, else
 // bounded byte loads (past the last slot reads as empty).
