@@ -291,6 +291,7 @@ __global__ __launch_bounds__(RB_THREADS, 4) void k_bb_write(const uint32_t* __re
   const uint32_t u = blockIdx.x;
   const BTile t = tl[u];
   if (ferr[t.f]) return;  // this filter keeps its table (capacity)
+  win_clear<BPE>(s_win);  // (the barriers before the first scatter order it)
   const BFilter fl = flt[t.f];
   uint32_t v[RB_ITEMS + 1];
   uint32_t tags, fl_bits;

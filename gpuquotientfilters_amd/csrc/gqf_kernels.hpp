@@ -306,10 +306,12 @@ __global__ __launch_bounds__(256) void k_rs_upsweep(const void* __restrict__ in,
 }
 
 // Occupancy: its registers hold a sub-tile's keys and ranks; the bounds give
-// 5 waves per SIMD to the 32-bit key-only sort (96 VGPRs) and 4 to the rest
-// (their LDS allows no more), each at the cost of one spilled register.
+// 5 waves per SIMD to the 32-bit key-only sort (96 VGPRs) and 4 to the
+// 32-bit sort with values and the 64-bit key-only one (their LDS allows no
+// more), each at the cost of one spilled register; the LDS of the 64-bit
+// sort with values allows 2.
 template <typename K, bool VALS, bool HASH>
-__global__ __launch_bounds__(256, (sizeof(K) == 4 && !VALS) ? 5 : 4) void k_rs_downsweep(const void* __restrict__ in_keys,
+__global__ __launch_bounds__(256, sizeof(K) == 4 ? (VALS ? 4 : 5) : (VALS ? 2 : 4)) void k_rs_downsweep(const void* __restrict__ in_keys,
                                                       const uint32_t* __restrict__ in_vals, K* __restrict__ out_keys,
                                                       uint32_t* __restrict__ out_vals, uint32_t n, uint32_t fp_bits,
                                                       uint32_t shift, uint32_t chunk, uint32_t G,

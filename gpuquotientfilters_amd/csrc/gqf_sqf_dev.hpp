@@ -613,8 +613,9 @@ __device__ __forceinline__ void win_or(uint32_t* s_win, uint32_t off, uint32_t e
 // Copies table bytes [B0, B1) from the window, whose word 0 holds the table
 // bytes from the 16-byte boundary at or below B0 (absolute addresses): whole
 // 16-byte stores inside, byte stores in the two edge pieces.  skip_zero: the
-// table is all zeros already, so zero pieces are not stored.
-__device__ __forceinline__ void win_out(uint8_t* __restrict__ t, const uint32_t* s_win, uint64_t B0, uint64_t B1,
+// table is all zeros already, so zero pieces are not stored.  Each piece is
+// cleared once copied, so the window is all zeros again afterwards.
+__device__ __forceinline__ void win_out(uint8_t* __restrict__ t, uint32_t* s_win, uint64_t B0, uint64_t B1,
                                         bool skip_zero) {
   const uint32_t mis = (uint32_t)((reinterpret_cast<uintptr_t>(t) + B0) & 15u);
   uint8_t* base = t + B0 - mis;  // 16-byte aligned; bytes before t + B0 are never stored
@@ -629,16 +630,25 @@ __device__ __forceinline__ void win_out(uint8_t* __restrict__ t, const uint32_t*
       const uint8_t* xb = reinterpret_cast<const uint8_t*>(s_win) + lo;
       for (uint32_t i = lo < mis ? mis : lo; i < (hi < nb ? hi : nb); ++i) base[i] = xb[i - lo];
     }
+    reinterpret_cast<uint4*>(s_win)[k] = make_uint4(0u, 0u, 0u, 0u);
   }
+}
+
+// Clears a write window (each thread its 16-byte pieces); rb_emit expects a
+// cleared window and leaves it cleared.
+template <int BPE>
+__device__ __forceinline__ void win_clear(uint32_t* s_win) {
+  for (uint32_t k = threadIdx.x; k < rb_win_words<BPE>() / 4; k += RB_THREADS)
+    reinterpret_cast<uint4*>(s_win)[k] = make_uint4(0u, 0u, 0u, 0u);
 }
 
 // Writes one sub-tile: block scan of the per-thread states (v, keepbits, a)
 // on top of the incoming state X; the sub-tile owns the slots [X.lp, Y.lp)
 // up to its last kept entry and writes every byte of them (elements and the
-// zeros between them) window by window: the window is zeroed in LDS, every
-// kept entry ORs its element into it and every run head its occupied bit
-// (ORs commute, so no ordering among threads), then the window is copied
-// out.  Run heads whose quotient precedes the owned range go to the fix-up
+// zeros between them) window by window: every kept entry ORs its element
+// into the LDS window (all zeros on entry) and every run head its occupied
+// bit (ORs commute, so no ordering among threads), then the window is copied
+// out and cleared again.  Run heads whose quotient precedes the owned range go to the fix-up
 // list.  precleared: the table is all zeros, so windows without an element
 // or occupied bit are skipped.  Every thread of the block calls it; returns
 // the outgoing state.
@@ -653,7 +663,6 @@ __device__ __forceinline__ RS rb_emit(uint8_t* __restrict__ t, uint32_t q, uint3
   const uint32_t mq = (uint32_t)mask64(q);
   const uint32_t rmask = (1u << r) - 1u;
   constexpr uint32_t bpe = BPE;
-  constexpr uint32_t nwin = rb_win_words<BPE>() / 4;  // 16-byte pieces
   uint32_t& s_nfix = *s_nfix_p;
   if (tid == 0) s_nfix = 0;
   RS ex;
@@ -686,8 +695,6 @@ __device__ __forceinline__ RS rb_emit(uint8_t* __restrict__ t, uint32_t q, uint3
     }
     const uint64_t B0 = (uint64_t)w0 * bpe;
     const uint32_t mis = (uint32_t)((reinterpret_cast<uintptr_t>(t) + B0) & 15u);
-    for (uint32_t k = tid; k < nwin; k += RB_THREADS) reinterpret_cast<uint4*>(s_win)[k] = make_uint4(0u, 0u, 0u, 0u);
-    __syncthreads();
     {
       // window byte of slot s: (s - w0) * bpe + mis
       int32_t prev = prev0;
@@ -751,6 +758,7 @@ __global__ __launch_bounds__(RB_THREADS, 6) void k_rb_write(uint8_t* __restrict_
   __shared__ __align__(16) uint32_t s_win[rb_win_words<BPE>()];
   __shared__ uint4 s_wave[RB_WAVES + 1];
   __shared__ uint32_t s_nfix;
+  win_clear<BPE>(s_win);  // (the barriers before the first scatter order it)
   if (na_dev) na = *na_dev;
   const uint64_t N = (uint64_t)na + (MERGE ? nb : 0u);
   if (precleared < 0) precleared = N * 8 < (uint64_t)S ? 1 : 0;  // decided from the count on the device
