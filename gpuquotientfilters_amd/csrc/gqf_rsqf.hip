@@ -759,6 +759,13 @@ __global__ __launch_bounds__(RB_THREADS, 6) void k_rr_write(uint8_t* __restrict_
   __shared__ uint32_t s_off[RW_WB];
   __shared__ uint4 s_wave[RB_WAVES + 1];
   __shared__ uint32_t s_b[2];
+  // the window fields start cleared (the barriers before the first visit order it)
+  for (uint32_t i = threadIdx.x; i < RW_WB * 32; i += RB_THREADS) s_rem[i] = 0u;
+  for (uint32_t i = threadIdx.x; i < RW_WB; i += RB_THREADS) {
+    s_occ[2 * i] = s_occ[2 * i + 1] = 0u;
+    s_end[2 * i] = s_end[2 * i + 1] = 0u;
+    s_off[i] = 0u;
+  }
   const int tid = threadIdx.x;
   const uint32_t rr = R ? (uint32_t)R : r;
   const uint32_t mq = (uint32_t)mask64(q);
@@ -854,13 +861,8 @@ __global__ __launch_bounds__(RB_THREADS, 6) void k_rr_write(uint8_t* __restrict_
         const uint32_t nblk = jw1 - jw0;
         // several windows (sparse table): skip those without contributions
         if (B1 - B0 > RW_WB && !__syncthreads_or(t_lo < jw1 && t_hi >= jw0)) continue;
-        for (uint32_t i = tid; i < nblk * rw; i += RB_THREADS) s_rem[i] = 0u;
-        for (uint32_t i = tid; i < nblk; i += RB_THREADS) {
-          s_occ[2 * i] = s_occ[2 * i + 1] = 0u;
-          s_end[2 * i] = s_end[2 * i + 1] = 0u;
-          s_off[i] = 0u;
-        }
-        __syncthreads();
+        // (the window fields are all zeros here: cleared at the start and by
+        // each copy-out below)
         {
           int32_t prev = prev0;
           uint32_t lastf = lastf0;
@@ -882,6 +884,9 @@ __global__ __launch_bounds__(RB_THREADS, 6) void k_rr_write(uint8_t* __restrict_
           const uint64_t occ = ((uint64_t)s_occ[2 * i + 1] << 32) | s_occ[2 * i];
           const uint64_t end = ((uint64_t)s_end[2 * i + 1] << 32) | s_end[2 * i];
           const uint32_t off = s_off[i];
+          s_occ[2 * i] = s_occ[2 * i + 1] = 0u;
+          s_end[2 * i] = s_end[2 * i + 1] = 0u;
+          s_off[i] = 0u;
           uint64_t* mrec = meta + 4 * (size_t)jb;
           if (occ) {
             if (jb == qfirst || jb == qlast) atomicOr((unsigned long long*)&mrec[0], (unsigned long long)occ);
@@ -899,6 +904,7 @@ __global__ __launch_bounds__(RB_THREADS, 6) void k_rr_write(uint8_t* __restrict_
         for (uint32_t i = tid; i < nblk * rw; i += RB_THREADS) {
           const uint32_t jb = jw0 + i / rw;
           const uint32_t x = s_rem[i];
+          s_rem[i] = 0u;
           const bool own = 64u * jb >= Llo && 64u * jb + 64 <= Lhi;
           uint32_t* g = remg + (size_t)jw0 * rw + i;
           if (own) *g = x;
