@@ -115,15 +115,17 @@ __host__ __device__ __forceinline__ void sqf_win_load(const uint8_t* __restrict_
       W.raw[4 * k + 3] = x.w;
     }
   } else {
+    // the table's last window (slots past S read as empty): 4-byte loads of
+    // the words inside the table, byte loads for the one word that crosses
+    // its end
     const uint64_t end = (uint64_t)S * BPE;
 #pragma unroll
     for (int k = 0; k < 16 * BPE; ++k) {
+      const uint64_t at = s0 * BPE + 4 * (uint64_t)k;
       uint32_t v = 0;
-#pragma unroll
-      for (int b = 0; b < 4; ++b) {
-        const uint64_t at = s0 * BPE + 4 * k + b;
-        if (at < end) v |= (uint32_t)t[at] << (8 * b);
-      }
+      if (at + 4 <= end) v = *reinterpret_cast<const uint32_t*>(t + at);
+      else
+        for (uint64_t b = at; b < end; ++b) v |= (uint32_t)t[b] << (8 * (uint32_t)(b - at));
       W.raw[k] = v;
     }
   }
