@@ -802,22 +802,34 @@ __device__ __forceinline__ void scan_rows_load(const uint32_t* __restrict__ data
     for (int k = 0; k < N; ++k) v[k] = row0 + k / COLS < rows ? data[(uint64_t)row0 * COLS + k] : 0u;
   }
 }
+constexpr uint32_t SCAN_PIECE = 4096;  // rows per piece (1024 threads x 4 rows)
+
+// part (may be NULL): column sums of every piece (k_scan_rows_sum); workgroup
+// g then scans piece g alone on top of the sums of the pieces before it, and
+// the last one writes the totals.  Without it one workgroup scans them all.
 template <int COLS>
 __global__ __launch_bounds__(1024) void k_scan_rows(uint32_t* __restrict__ data, uint32_t rows,
-                                                    uint32_t* __restrict__ totals) {
-  constexpr uint32_t PIECE = 4096;
+                                                    uint32_t* __restrict__ totals,
+                                                    const uint32_t* __restrict__ part) {
+  constexpr uint32_t PIECE = SCAN_PIECE;
   constexpr int N = 4 * COLS;
   __shared__ uint32_t s_w[2][COLS][16];
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const uint32_t base0 = part ? blockIdx.x * PIECE : 0u;
+  const uint32_t end = part && base0 + PIECE < rows ? base0 + PIECE : rows;
   uint32_t carry[COLS];
 #pragma unroll
-  for (int c = 0; c < COLS; ++c) carry[c] = 0;
+  for (int c = 0; c < COLS; ++c) {
+    carry[c] = 0;
+    if (part)
+      for (uint32_t g = 0; g < blockIdx.x; ++g) carry[c] += part[g * COLS + c];
+  }
   uint32_t cur[N], nxt[N];
-  scan_rows_load<COLS>(data, rows, (uint32_t)tid * 4, cur);
+  scan_rows_load<COLS>(data, rows, base0 + (uint32_t)tid * 4, cur);
   int buf = 0;
-  for (uint32_t base = 0; base < rows; base += PIECE) {
+  for (uint32_t base = base0; base < end; base += PIECE) {
     const uint32_t row0 = base + (uint32_t)tid * 4;
-    if (base + PIECE < rows) scan_rows_load<COLS>(data, rows, row0 + PIECE, nxt);
+    if (base + PIECE < end) scan_rows_load<COLS>(data, rows, row0 + PIECE, nxt);
     uint32_t pre[COLS];
 #pragma unroll
     for (int c = 0; c < COLS; ++c) {
@@ -858,10 +870,58 @@ __global__ __launch_bounds__(1024) void k_scan_rows(uint32_t* __restrict__ data,
     for (int k = 0; k < N; ++k) cur[k] = nxt[k];
     buf ^= 1;
   }
-  if (tid == 0 && totals) {
+  if (tid == 0 && totals && (!part || blockIdx.x + 1 == gridDim.x)) {
 #pragma unroll
     for (int c = 0; c < COLS; ++c) totals[c] = carry[c];
   }
+}
+
+// Column sums of piece blockIdx.x into part[blockIdx.x * COLS ..].
+template <int COLS>
+__global__ __launch_bounds__(1024) void k_scan_rows_sum(const uint32_t* __restrict__ data, uint32_t rows,
+                                                        uint32_t* __restrict__ part) {
+  constexpr int N = 4 * COLS;
+  __shared__ uint32_t s_w[COLS][16];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  uint32_t v[N];
+  scan_rows_load<COLS>(data, rows, blockIdx.x * SCAN_PIECE + (uint32_t)tid * 4, v);
+#pragma unroll
+  for (int c = 0; c < COLS; ++c) {
+    const uint32_t sum = v[c] + v[COLS + c] + v[2 * COLS + c] + v[3 * COLS + c];
+    const uint32_t inc = wave_incl_add(sum);
+    if (lane == 63) s_w[c][wid] = inc;
+  }
+  __syncthreads();
+  if (tid < COLS) {
+    uint32_t t = 0;
+#pragma unroll
+    for (int w = 0; w < 16; ++w) t += s_w[tid][w];
+    part[blockIdx.x * COLS + tid] = t;
+  }
+}
+
+// Exclusive scan of COLS interleaved columns over `rows` rows in place;
+// totals (may be NULL) = column sums.  Several pieces: a sum pass and a scan
+// pass of one workgroup per piece (the piece sums from the arena).
+template <int COLS>
+static inline gqf_status scan_rows(hipStream_t s, Arena& ar, uint32_t* data, uint32_t rows, uint32_t* totals) {
+  const uint32_t G = (rows + SCAN_PIECE - 1) / SCAN_PIECE;
+  if (G <= 1) {
+    GQF_LAUNCH(k_scan_rows<COLS>, dim3(1), dim3(1024), 0, s, data, rows, totals, (const uint32_t*)nullptr);
+    return GQF_OK;
+  }
+  uint32_t* part = ar.take<uint32_t>((size_t)G * COLS);
+  if (ar.used > ar.cap) {
+    set_error("internal: scan workspace overflow");
+    return GQF_ERR_INVALID;
+  }
+  GQF_LAUNCH(k_scan_rows_sum<COLS>, dim3(G), dim3(1024), 0, s, data, rows, part);
+  GQF_LAUNCH(k_scan_rows<COLS>, dim3(G), dim3(1024), 0, s, data, rows, totals, (const uint32_t*)part);
+  return GQF_OK;
+}
+// Workspace bytes of scan_rows.
+static inline size_t scan_rows_bytes(uint32_t rows, int cols) {
+  return aligned_bytes((size_t)((rows + SCAN_PIECE - 1) / SCAN_PIECE) * cols * 4) + 256;
 }
 
 }  // namespace gqf

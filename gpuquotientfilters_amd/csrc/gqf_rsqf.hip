@@ -964,7 +964,8 @@ static inline bool rsqf_packed(const RsqfGeom& g, size_t kb) { return kb == 8 &&
 static size_t rsqf_decode_ws(const RsqfGeom& g, size_t kb) {
   const uint32_t W = rd_windows(g.nb);
   const size_t eb = rsqf_packed(g, kb) ? 4 : kb;
-  return aligned_bytes((size_t)W * 8) + aligned_bytes((size_t)W * 4) + 256 + aligned_bytes((g.cap + 1) * eb) + 512;
+  return aligned_bytes((size_t)W * 8) + aligned_bytes((size_t)W * 4) + 256 + aligned_bytes((g.cap + 1) * eb) + 512 +
+         scan_rows_bytes(W, 2) + scan_rows_bytes(W, 1);
 }
 
 // The stored fingerprints in ascending order; their count is left in device
@@ -988,9 +989,9 @@ static gqf_status rsqf_decode(hipStream_t s, Arena& ar, const gqf_rsqf& f, K** o
   }
   const double meta = (double)f.numBlocks * 16;  // occupieds + run ends
   GQF_LAUNCH_B(meta, k_rd_pop, dim3((W + 3) / 4), dim3(256), 0, s, d.meta, f.numBlocks, W, wp, hb, nhv);
-  GQF_LAUNCH(k_scan_rows<2>, dim3(1), dim3(1024), 0, s, wp, W, tot);
+  GQF_TRY(scan_rows<2>(s, ar, wp, W, tot));
   GQF_LAUNCH_B(meta, k_rd_count, dim3(W), dim3(RD_THREADS), 0, s, d, wp, wc);
-  GQF_LAUNCH(k_scan_rows<1>, dim3(1), dim3(1024), 0, s, wc, W, tot + 2);
+  GQF_TRY(scan_rows<1>(s, ar, wc, W, tot + 2));
   // the count stays on the device (tot[2]): later kernels read it, the host
   // plans for the capacity
   *m_dev = tot + 2;

@@ -52,7 +52,7 @@ static gqf_status sqf_decode(hipStream_t s, Arena& ar, const gqf_sqf& qf, const 
   else if (g.bpe == 3) GQF_LAUNCH_B((double)g.bytes, k_sqf_count<3>, dim3(regions), dim3(256), 0, s, qf.table, g.S, counts);
   else GQF_LAUNCH_B((double)g.bytes, k_sqf_count<4>, dim3(regions), dim3(256), 0, s, qf.table, g.S, counts);
   // totals: column 0 (entries) is the decoded count
-  GQF_LAUNCH(k_scan_rows<3>, dim3(1), dim3(1024), 0, s, counts, regions, count_dev);
+  GQF_TRY(scan_rows<3>(s, ar, counts, regions, count_dev));
   // table read + 4 B per entry written (the count is on the device)
   const double tb = (double)g.bytes;
   if (g.bpe == 1)
@@ -68,7 +68,8 @@ static gqf_status sqf_decode(hipStream_t s, Arena& ar, const gqf_sqf& qf, const 
   return GQF_OK;
 }
 static size_t sqf_decode_bytes(const SqfGeom& g) {
-  return aligned_bytes((size_t)((g.S + DC_SLOTS - 1) / DC_SLOTS) * 12) + 256 + aligned_bytes((size_t)g.S * 4) + 512;
+  const uint32_t regions = (g.S + DC_SLOTS - 1) / DC_SLOTS;
+  return aligned_bytes((size_t)regions * 12) + 256 + aligned_bytes((size_t)g.S * 4) + 512 + scan_rows_bytes(regions, 3);
 }
 
 // Reads a device counter into host memory (small synchronous readback, used
@@ -188,7 +189,7 @@ static size_t sqf_local_bytes(const SqfGeom& g, uint32_t nb) {
   const uint64_t G = ((uint64_t)nb + RB_TILE - 1) / RB_TILE;
   return aligned_bytes(W) + aligned_bytes((R + 1) * 4) + 256 + 4 * aligned_bytes((uint64_t)nb * 4) +
          aligned_bytes(nb) + aligned_bytes(G * 16) + aligned_bytes(CTR_TOTAL * 4) + 256 + aligned_bytes((uint64_t)g.S * 4) +
-         aligned_bytes(((G + RBS_PER - 1) / RBS_PER) * 16) + 2048;
+         aligned_bytes(((G + RBS_PER - 1) / RBS_PER) * 16) + 2048 + scan_rows_bytes((uint32_t)R + 1, 1);
 }
 
 // The localized insert (gqf_sqf_dev.hpp, "localized insert") of the sorted
@@ -221,7 +222,7 @@ static gqf_status sqf_insert_local(CallScope& cs, Arena& ar, const gqf_sqf& qf, 
   const dim3 kg((nb + 255) / 256);
   GQF_LAUNCH(k_pi_init, dim3(1), dim3(64), 0, s, rcnt + R, flags);
   GQF_LAUNCH_B((double)g.bytes, k_pi_count<BPE>, dim3(R), dim3(256), 0, s, qf.table, g.S, wcnt, rcnt);
-  GQF_LAUNCH(k_scan_rows<1>, dim3(1), dim3(1024), 0, s, rcnt, R + 1, tot);
+  GQF_TRY(scan_rows<1>(s, ar, rcnt, R + 1, tot));
   GQF_LAUNCH(k_pi_keys<BPE>, kg, dim3(256), 0, s, qf.table, g.S, g.q, g.r, B, nb, wcnt, rcnt, E, s0, flags);
   GQF_LAUNCH(k_pi_jagg, dim3(G), dim3(RB_THREADS), 0, s, E, nb, agg);
   GQF_TRY(rb_scan(s, ar, agg, G, ~0ull, sctr, nullptr));
