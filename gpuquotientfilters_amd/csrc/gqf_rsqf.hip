@@ -632,7 +632,9 @@ __global__ __launch_bounds__(RB_THREADS, 8) void k_rr_agg(const K* __restrict__ 
                                                           const uint8_t* __restrict__ flags,
                                                           const uint32_t* __restrict__ splits, uint32_t q, uint32_t r,
                                                           uint32_t chunk, uint4* __restrict__ agg,
-                                                          const uint32_t* __restrict__ hb) {
+                                                          const uint32_t* __restrict__ hb, uint32_t* __restrict__ ctr) {
+  // the rebuild's counters start at zero (nothing reads them before the scan)
+  if (blockIdx.x == 0 && threadIdx.x >= CTR_OP && threadIdx.x < CTR_TOTAL) ctr[threadIdx.x] = 0u;
   __shared__ uint32_t s_lo[MERGE ? RB_TILE : 1];
   __shared__ uint32_t s_hi[MERGE && sizeof(K) == 8 ? RB_TILE : 1];
   __shared__ uint32_t s_bmp[MERGE ? RB_TILE / 32 : 1], s_wpre[MERGE ? RB_TILE / 32 : 1];
@@ -1038,8 +1040,8 @@ static gqf_status rsqf_rebuild_t(hipStream_t s, DeviceState& ds, Arena& ar, cons
   const RsqfGeom g = rgeom(f);
   const uint64_t N = (uint64_t)na + (MERGE ? nbk : 0u);
   const size_t tbytes = rsqf_dev_bytes(g.q, g.r);
-  GQF_LAUNCH(k_reset_counters, dim3(1), dim3(64), 0, s, ds.counters, (int)CTR_OP, (int)CTR_TOTAL);
   if (N == 0) {
+    GQF_LAUNCH(k_reset_counters, dim3(1), dim3(64), 0, s, ds.counters, (int)CTR_OP, (int)CTR_TOTAL);
     GQF_LAUNCH_B((double)tbytes, k_rr_clear, dim3(1024), dim3(256), 0, s, reinterpret_cast<uint8_t*>(f.blocks),
                  (uint64_t)tbytes, ds.counters);
     return GQF_OK;
@@ -1060,7 +1062,7 @@ static gqf_status rsqf_rebuild_t(hipStream_t s, DeviceState& ds, Arena& ar, cons
     GQF_LAUNCH((k_rb_splits<K, PK>), dim3((p.nsub + 1 + 255) / 256), dim3(256), 0, s, A, na, na_dev, B, nbk, p.nsub,
                splits, hb);
   GQF_LAUNCH_BC(nknown * kper, na_dev, per, (k_rr_agg<K, MODE, MERGE, PK>), dim3(p.G), dim3(RB_THREADS), 0, s, A,
-                na, na_dev, B, nbk, flags, splits, g.q, g.r, p.chunk, agg, hb);
+                na, na_dev, B, nbk, flags, splits, g.q, g.r, p.chunk, agg, hb, ds.counters);
   GQF_TRY(rb_scan(s, ar, agg, p.G, g.cap, ds.counters, total));
   GQF_LAUNCH_B((double)tbytes, k_rr_clear, dim3(1024), dim3(256), 0, s, reinterpret_cast<uint8_t*>(f.blocks),
                (uint64_t)tbytes, ds.counters);

@@ -100,8 +100,8 @@ static gqf_status launch_rebuild(hipStream_t s, DeviceState& ds, Arena& ar, cons
                                  const uint32_t* A, uint32_t na, const uint32_t* na_dev, const uint32_t* B,
                                  uint32_t nb, const uint8_t* flags) {
   const uint64_t N = (uint64_t)na + (MERGE ? nb : 0u);
-  GQF_LAUNCH(k_reset_counters, dim3(1), dim3(64), 0, s, ds.counters, (int)CTR_OP, (int)CTR_TOTAL);
   if (N == 0) {
+    GQF_LAUNCH(k_reset_counters, dim3(1), dim3(64), 0, s, ds.counters, (int)CTR_OP, (int)CTR_TOTAL);
     clear_table(s, qf, g);
     return GQF_OK;
   }
@@ -121,7 +121,7 @@ static gqf_status launch_rebuild(hipStream_t s, DeviceState& ds, Arena& ar, cons
     GQF_LAUNCH((k_rb_splits<uint32_t, false>), dim3((p.nsub + 1 + 255) / 256), dim3(256), 0, s, A, na, na_dev, B, nb,
                p.nsub, splits, (const uint32_t*)nullptr);
   GQF_LAUNCH_BC(nknown * 4, na_dev, 4, (k_rb_agg<MODE, MERGE>), dim3(p.G), dim3(RB_THREADS), 0, s, A, na, na_dev, B,
-                nb, flags, splits, g.q, g.r, p.chunk, agg);
+                nb, flags, splits, g.q, g.r, p.chunk, agg, ds.counters);
   GQF_TRY(rb_scan(s, ar, agg, p.G, (uint64_t)g.S, ds.counters, nullptr));
   // sparse tables (N * 8 < S): clear once, then sub-tiles skip windows holding
   // only zeros; otherwise sub-tiles write every slot of their range and the

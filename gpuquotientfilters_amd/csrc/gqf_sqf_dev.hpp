@@ -540,7 +540,9 @@ __global__ __launch_bounds__(RB_THREADS, 8) void k_rb_agg(const uint32_t* __rest
                                                 const uint32_t* __restrict__ B, uint32_t nb,
                                                 const uint8_t* __restrict__ flags, const uint32_t* __restrict__ splits,
                                                 uint32_t q, uint32_t r, uint32_t chunk,
-                                                uint4* __restrict__ agg) {
+                                                uint4* __restrict__ agg, uint32_t* __restrict__ ctr) {
+  // the rebuild's counters start at zero (nothing reads them before the scan)
+  if (blockIdx.x == 0 && threadIdx.x >= CTR_OP && threadIdx.x < CTR_TOTAL) ctr[threadIdx.x] = 0u;
   __shared__ uint32_t s_stage[MERGE ? 2 : 1][MERGE ? RB_TILE : 1];
   __shared__ uint32_t s_prev[2][2];
   __shared__ uint32_t s_bmp[MERGE ? RB_TILE / 32 : 1], s_wpre[MERGE ? RB_TILE / 32 : 1];
