@@ -305,18 +305,30 @@ __global__ __launch_bounds__(256) void k_rs_upsweep(const void* __restrict__ in,
   if (tid == 0) totals[blockIdx.x] = s_carry;
 }
 
-// The keys [c0, c1) of one chunk, in 4096-key sub-tiles in order: each
-// sub-tile is ranked stably (ballots within a wave, then waves in order),
-// staged in LDS by digit and written as runs of equal digits, digit d's run
-// starting at s_run[d] (advanced past the sub-tile's keys).  s_keys holds at
-// least SORT_TILE keys.
+// Occupancy: its registers hold a sub-tile's keys and ranks; the bounds give
+// 5 waves per SIMD to the 32-bit key-only sort (96 VGPRs) and 4 to the
+// 32-bit sort with values and the 64-bit key-only one (their LDS allows no
+// more), each at the cost of one spilled register; the LDS of the 64-bit
+// sort with values allows 2.
 template <typename K, bool VALS, bool HASH>
-__device__ __forceinline__ void rs_scatter_chunk(const void* __restrict__ in_keys, const uint32_t* __restrict__ in_vals,
-                                                 K* __restrict__ out_keys, uint32_t* __restrict__ out_vals,
-                                                 uint32_t fp_bits, uint32_t shift, uint64_t c0, uint64_t c1,
-                                                 uint32_t* s_run, uint32_t* s_wcnt, uint32_t* s_lstart,
-                                                 uint32_t* s_tmp, K* s_keys, uint32_t* s_vals) {
+__global__ __launch_bounds__(256, sizeof(K) == 4 ? (VALS ? 4 : 5) : (VALS ? 2 : 4)) void k_rs_downsweep(const void* __restrict__ in_keys,
+                                                      const uint32_t* __restrict__ in_vals, K* __restrict__ out_keys,
+                                                      uint32_t* __restrict__ out_vals, uint32_t n, uint32_t fp_bits,
+                                                      uint32_t shift, uint32_t chunk, uint32_t G,
+                                                      const uint32_t* __restrict__ hist,
+                                                      const uint32_t* __restrict__ digit_totals) {
+  __shared__ uint32_t s_run[256];
+  __shared__ uint32_t s_wcnt[4 * 256];
+  __shared__ uint32_t s_lstart[256];
+  __shared__ uint32_t s_tmp[4];
+  __shared__ K s_keys[SORT_TILE];
+  __shared__ uint32_t s_vals[VALS ? SORT_TILE : 1];
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  // digit bases: exclusive scan of the 256 digit totals (every workgroup
+  // does it; it saves a launch)
+  s_run[tid] = block256_excl_scan(digit_totals[tid], s_tmp, nullptr) + hist[(uint64_t)tid * G + blockIdx.x];
+  const uint64_t c0 = (uint64_t)blockIdx.x * chunk;
+  const uint64_t c1 = c0 + chunk < n ? c0 + chunk : (uint64_t)n;
   const uint64_t lt = (1ull << lane) - 1ull;
   for (uint64_t t0 = c0; t0 < c1; t0 += SORT_TILE) {
     for (int i = tid; i < 4 * 256; i += 256) s_wcnt[i] = 0;
@@ -382,64 +394,48 @@ __device__ __forceinline__ void rs_scatter_chunk(const void* __restrict__ in_key
   }
 }
 
-// Occupancy: its registers hold a sub-tile's keys and ranks; the bounds give
-// 5 waves per SIMD to the 32-bit key-only sort (96 VGPRs) and 4 to the
-// 32-bit sort with values and the 64-bit key-only one (their LDS allows no
-// more), each at the cost of one spilled register; the LDS of the 64-bit
-// sort with values allows 2.
+// Every pass of a sort of at most SORT_TILE keys in one workgroup: the keys
+// (and values) stay in registers and LDS between passes; each pass ranks them
+// stably as k_rs_downsweep does (ballots within a wave, then waves in order)
+// and scatters them within LDS.  Key j sits at position wid*1024 + i*64 +
+// lane of thread (wid, lane), item i, before and after every pass; padding
+// keys (all ones) take the last digit and rank after every real key.
 template <typename K, bool VALS, bool HASH>
-__global__ __launch_bounds__(256, sizeof(K) == 4 ? (VALS ? 4 : 5) : (VALS ? 2 : 4)) void k_rs_downsweep(const void* __restrict__ in_keys,
-                                                      const uint32_t* __restrict__ in_vals, K* __restrict__ out_keys,
-                                                      uint32_t* __restrict__ out_vals, uint32_t n, uint32_t fp_bits,
-                                                      uint32_t shift, uint32_t chunk, uint32_t G,
-                                                      const uint32_t* __restrict__ hist,
-                                                      const uint32_t* __restrict__ digit_totals) {
-  __shared__ uint32_t s_run[256];
+__global__ __launch_bounds__(SORT_THREADS) void k_rs_small(const void* __restrict__ in_keys,
+                                                           const uint32_t* __restrict__ in_vals,
+                                                           K* __restrict__ out_keys, uint32_t* __restrict__ out_vals,
+                                                           uint32_t n, uint32_t fp_bits, uint32_t lo_bit,
+                                                           uint32_t npass) {
   __shared__ uint32_t s_wcnt[4 * 256];
   __shared__ uint32_t s_lstart[256];
   __shared__ uint32_t s_tmp[4];
   __shared__ K s_keys[SORT_TILE];
   __shared__ uint32_t s_vals[VALS ? SORT_TILE : 1];
-  const int tid = threadIdx.x;
-  // digit bases: exclusive scan of the 256 digit totals (every workgroup
-  // does it; it saves a launch)
-  s_run[tid] = block256_excl_scan(digit_totals[tid], s_tmp, nullptr) + hist[(uint64_t)tid * G + blockIdx.x];
-  const uint64_t c0 = (uint64_t)blockIdx.x * chunk;
-  const uint64_t c1 = c0 + chunk < n ? c0 + chunk : (uint64_t)n;
-  rs_scatter_chunk<K, VALS, HASH>(in_keys, in_vals, out_keys, out_vals, fp_bits, shift, c0, c1, s_run, s_wcnt,
-                                  s_lstart, s_tmp, s_keys, s_vals);
-}
-
-// npass 8-bit passes from lo_bit over the 256 * rpw keys (rpw <= ITEMS rows
-// of 64 per wave) held in registers: key j sits at position
-// wid * rpw * 64 + i * 64 + lane of thread (wid, lane), item i < rpw, before
-// and after every pass; items i >= rpw take no part.  Each pass ranks the
-// keys stably as rs_scatter_chunk does (ballots within a wave, then waves in
-// order) and scatters them within LDS (s_keys, s_vals: 256 * rpw entries).
-// Padding keys (all ones) at the end take the last digit and rank after
-// every real key.
-template <typename K, bool VALS, int ITEMS>
-__device__ __forceinline__ void rs_lds_passes(K (&k)[ITEMS], uint32_t (&v)[VALS ? ITEMS : 1], uint32_t rpw,
-                                              uint32_t lo_bit, uint32_t npass, uint32_t* s_wcnt,
-                                              uint32_t* s_lstart, uint32_t* s_tmp, K* s_keys, uint32_t* s_vals) {
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const uint64_t lt = (1ull << lane) - 1ull;
+  K k[SORT_ITEMS];
+  uint32_t v[VALS ? SORT_ITEMS : 1];
+#pragma unroll
+  for (int i = 0; i < SORT_ITEMS; ++i) {
+    const uint32_t idx = (uint32_t)wid * (SORT_ITEMS * 64) + (uint32_t)i * 64 + lane;
+    const bool valid = idx < n;
+    k[i] = valid ? sort_load_key<K, HASH>(in_keys, idx, fp_bits) : (K)~(K)0;
+    if (VALS) v[i] = in_vals ? (valid ? in_vals[idx] : 0u) : idx;
+  }
   for (uint32_t p = 0; p < npass; ++p) {
     const uint32_t shift = lo_bit + 8 * p;
     for (int i = tid; i < 4 * 256; i += SORT_THREADS) s_wcnt[i] = 0;
     __syncthreads();
-    uint32_t rk[ITEMS];
+    uint32_t rk[SORT_ITEMS];
 #pragma unroll
-    for (int i = 0; i < ITEMS; ++i) {
-      if ((uint32_t)i < rpw) {
-        const uint32_t d = (uint32_t)(k[i] >> shift) & 255u;
-        const uint64_t m = match_digit(d);
-        const uint32_t pre = s_wcnt[wid * 256 + d];
-        const uint32_t cnt = (uint32_t)__popcll(m);
-        const uint32_t below = (uint32_t)__popcll(m & lt);
-        if (below == cnt - 1) s_wcnt[wid * 256 + d] = pre + cnt;
-        rk[i] = pre + below;
-      }
+    for (int i = 0; i < SORT_ITEMS; ++i) {
+      const uint32_t d = (uint32_t)(k[i] >> shift) & 255u;
+      const uint64_t m = match_digit(d);
+      const uint32_t pre = s_wcnt[wid * 256 + d];
+      const uint32_t cnt = (uint32_t)__popcll(m);
+      const uint32_t below = (uint32_t)__popcll(m & lt);
+      if (below == cnt - 1) s_wcnt[wid * 256 + d] = pre + cnt;
+      rk[i] = pre + below;
     }
     __syncthreads();
     {
@@ -453,138 +449,27 @@ __device__ __forceinline__ void rs_lds_passes(K (&k)[ITEMS], uint32_t (&v)[VALS 
     }
     __syncthreads();
 #pragma unroll
-    for (int i = 0; i < ITEMS; ++i) {
-      if ((uint32_t)i < rpw) {
-        const uint32_t d = (uint32_t)(k[i] >> shift) & 255u;
-        const uint32_t pos = s_lstart[d] + s_wcnt[wid * 256 + d] + rk[i];
-        s_keys[pos] = k[i];
-        if (VALS) s_vals[pos] = v[i];
-      }
+    for (int i = 0; i < SORT_ITEMS; ++i) {
+      const uint32_t d = (uint32_t)(k[i] >> shift) & 255u;
+      const uint32_t pos = s_lstart[d] + s_wcnt[wid * 256 + d] + rk[i];
+      s_keys[pos] = k[i];
+      if (VALS) s_vals[pos] = v[i];
     }
     __syncthreads();
 #pragma unroll
-    for (int i = 0; i < ITEMS; ++i) {
-      if ((uint32_t)i < rpw) {
-        const uint32_t pos = (uint32_t)wid * (rpw * 64) + (uint32_t)i * 64 + lane;
-        k[i] = s_keys[pos];
-        if (VALS) v[i] = s_vals[pos];
-      }
+    for (int i = 0; i < SORT_ITEMS; ++i) {
+      const uint32_t pos = (uint32_t)wid * (SORT_ITEMS * 64) + (uint32_t)i * 64 + lane;
+      k[i] = s_keys[pos];
+      if (VALS) v[i] = s_vals[pos];
     }
     __syncthreads();
   }
-}
-
-// Every pass of a sort of at most SORT_TILE keys in one workgroup: the keys
-// (and values) stay in registers and LDS between passes (rs_lds_passes).
-template <typename K, bool VALS, bool HASH>
-__global__ __launch_bounds__(SORT_THREADS) void k_rs_small(const void* __restrict__ in_keys,
-                                                           const uint32_t* __restrict__ in_vals,
-                                                           K* __restrict__ out_keys, uint32_t* __restrict__ out_vals,
-                                                           uint32_t n, uint32_t fp_bits, uint32_t lo_bit,
-                                                           uint32_t npass) {
-  __shared__ uint32_t s_wcnt[4 * 256];
-  __shared__ uint32_t s_lstart[256];
-  __shared__ uint32_t s_tmp[4];
-  __shared__ K s_keys[SORT_TILE];
-  __shared__ uint32_t s_vals[VALS ? SORT_TILE : 1];
-  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  K k[SORT_ITEMS];
-  uint32_t v[VALS ? SORT_ITEMS : 1];
-#pragma unroll
-  for (int i = 0; i < SORT_ITEMS; ++i) {
-    const uint32_t idx = (uint32_t)wid * (SORT_ITEMS * 64) + (uint32_t)i * 64 + lane;
-    const bool valid = idx < n;
-    k[i] = valid ? sort_load_key<K, HASH>(in_keys, idx, fp_bits) : (K)~(K)0;
-    if (VALS) v[i] = in_vals ? (valid ? in_vals[idx] : 0u) : idx;
-  }
-  rs_lds_passes<K, VALS, SORT_ITEMS>(k, v, SORT_ITEMS, lo_bit, npass, s_wcnt, s_lstart, s_tmp, s_keys, s_vals);
 #pragma unroll
   for (int i = 0; i < SORT_ITEMS; ++i) {
     const uint32_t idx = (uint32_t)wid * (SORT_ITEMS * 64) + (uint32_t)i * 64 + lane;
     if (idx < n) {
       out_keys[idx] = k[i];
       if (VALS) out_vals[idx] = v[i];
-    }
-  }
-}
-
-// Second launch of the bucket sort (batches of up to 2^20 keys): bucket b
-// (one workgroup each) holds the keys whose top digit is b, in input order,
-// after the first scatter; the workgroup sorts it by the npass digits from
-// lo_bit.  A bucket of at most 256 * ITEMS keys is sorted in registers and
-// LDS and written to kb; a larger one (many equal keys) is sorted by LSD
-// passes over global memory between ka and kb within the workgroup, each a
-// digit count then rs_scatter_chunk, and ends in kb.  totals: the 256 bucket
-// sizes.
-template <typename K, bool VALS, int ITEMS>
-__global__ __launch_bounds__(SORT_THREADS) void k_rs_bucket_finish(K* __restrict__ ka, uint32_t* __restrict__ va,
-                                                                   K* __restrict__ kb, uint32_t* __restrict__ vb,
-                                                                   const uint32_t* __restrict__ totals,
-                                                                   uint32_t lo_bit, uint32_t npass) {
-  constexpr uint32_t CAP = SORT_THREADS * ITEMS;
-  __shared__ uint32_t s_run[256];
-  __shared__ uint32_t s_wcnt[4 * 256];
-  __shared__ uint32_t s_lstart[256];
-  __shared__ uint32_t s_tmp[4];
-  __shared__ K s_keys[CAP];
-  __shared__ uint32_t s_vals[VALS ? CAP : 1];
-  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  const uint32_t b = blockIdx.x;
-  s_lstart[tid] = block256_excl_scan(totals[tid], s_tmp, nullptr);
-  __syncthreads();
-  const uint32_t beg = s_lstart[b], nb = totals[b];
-  __syncthreads();
-  if (nb == 0) return;
-  if (nb <= CAP) {
-    // rows of 64 keys per wave: the bucket's rows spread evenly over the 4 waves
-    const uint32_t rpw = (nb + 255) / 256;
-    K k[ITEMS];
-    uint32_t v[VALS ? ITEMS : 1];
-#pragma unroll
-    for (int i = 0; i < ITEMS; ++i) {
-      const uint32_t idx = (uint32_t)wid * (rpw * 64) + (uint32_t)i * 64 + lane;
-      const bool valid = (uint32_t)i < rpw && idx < nb;
-      k[i] = valid ? ka[beg + idx] : (K)~(K)0;
-      if (VALS) v[i] = valid ? va[beg + idx] : 0u;
-    }
-    rs_lds_passes<K, VALS, ITEMS>(k, v, rpw, lo_bit, npass, s_wcnt, s_lstart, s_tmp, s_keys, s_vals);
-#pragma unroll
-    for (int i = 0; i < ITEMS; ++i) {
-      const uint32_t idx = (uint32_t)wid * (rpw * 64) + (uint32_t)i * 64 + lane;
-      if ((uint32_t)i < rpw && idx < nb) {
-        kb[beg + idx] = k[i];
-        if (VALS) vb[beg + idx] = v[i];
-      }
-    }
-    return;
-  }
-  const uint64_t c0 = beg, c1 = (uint64_t)beg + nb;
-  K* kin = ka;
-  K* kout = kb;
-  uint32_t* vin = va;
-  uint32_t* vout = vb;
-  for (uint32_t p = 0; p < npass; ++p) {
-    const uint32_t shift = lo_bit + 8 * p;
-    for (int i = tid; i < 4 * 256; i += SORT_THREADS) s_wcnt[i] = 0;
-    __syncthreads();
-    for (uint64_t i = c0 + tid; i < c1; i += SORT_THREADS)
-      atomicAdd(&s_wcnt[wid * 256 + ((uint32_t)(kin[i] >> shift) & 255u)], 1u);
-    __syncthreads();
-    const uint32_t cnt = s_wcnt[tid] + s_wcnt[256 + tid] + s_wcnt[512 + tid] + s_wcnt[768 + tid];
-    s_run[tid] = beg + block256_excl_scan(cnt, s_tmp, nullptr);
-    rs_scatter_chunk<K, VALS, false>(kin, vin, kout, vout, 0u, shift, c0, c1, s_run, s_wcnt, s_lstart, s_tmp,
-                                     s_keys, s_vals);
-    K* kt = kin;
-    kin = kout;
-    kout = kt;
-    uint32_t* vt = vin;
-    vin = vout;
-    vout = vt;
-  }
-  if (kin != kb) {  // an even number of passes ends in ka
-    for (uint64_t i = c0 + tid; i < c1; i += SORT_THREADS) {
-      kb[i] = ka[i];
-      if (VALS) vb[i] = va[i];
     }
   }
 }
@@ -641,21 +526,11 @@ gqf_status radix_sort(hipStream_t s, DeviceState& ds, Arena& ar, const void* in,
     if (vals_out) *vals_out = vb[0];
     return GQF_OK;
   }
-  // Bucket sort for batches of up to 2^20 32-bit keys (2^19 64-bit ones) and
-  // at least two digits: one pass scatters by the top 8 bits
-  // [hi_bit - 8, hi_bit), then one workgroup per bucket sorts it by the
-  // digits below (k_rs_bucket_finish): four launches instead of three per
-  // digit.  The buckets of hashed keys hold n / 256 keys on average, at most
-  // half of a workgroup's capacity (8192 32-bit keys, 4096 64-bit ones).
-  int bucket_items = 0;
-  if (npass >= 2 && sizeof(K) == 4 && n <= (1u << 20)) bucket_items = 32;
-  else if (npass >= 2 && sizeof(K) == 8 && n <= (1u << 19)) bucket_items = 16;
-  const uint32_t scatter_passes = bucket_items ? 1u : npass;
   const void* src = in;
   const uint32_t* vsrc = in_vals;
   int cur = 0;
-  for (uint32_t p = 0; p < scatter_passes; ++p) {
-    const uint32_t shift = bucket_items ? hi_bit - 8 : lo_bit + 8 * p;
+  for (uint32_t p = 0; p < npass; ++p) {
+    const uint32_t shift = lo_bit + 8 * p;
     bool first_hash = hash && p == 0;
     if (first_hash && sizeof(K) == 4) {
       // 32-bit fingerprints: hashed once, stored by the upsweep (into the
@@ -696,25 +571,6 @@ gqf_status radix_sort(hipStream_t s, DeviceState& ds, Arena& ar, const void* in,
     src = dst;
     vsrc = vdst;
     cur ^= 1;
-  }
-  if (bucket_items) {
-    // src = kb[0] (vb[0]): the keys by bucket; sorted into kb[1] (vb[1])
-    const uint32_t npass2 = (hi_bit - 8 - lo_bit + 7) / 8;
-    const double fbytes = (double)n * (2.0 * sizeof(K) + (with_vals ? 8.0 : 0.0));
-#define GQF_RS_FINISH(V_, I_)                                                                                     \
-  GQF_LAUNCH_B(fbytes, (k_rs_bucket_finish<K, V_, I_>), dim3(256), dim3(SORT_THREADS), 0, s, kb[0], vb[0], kb[1], \
-               vb[1], dbase, lo_bit, npass2)
-    if constexpr (sizeof(K) == 8) {
-      if (with_vals) GQF_RS_FINISH(true, 16);
-      else GQF_RS_FINISH(false, 16);
-    } else {
-      if (with_vals) GQF_RS_FINISH(true, 32);
-      else GQF_RS_FINISH(false, 32);
-    }
-#undef GQF_RS_FINISH
-    *keys_out = kb[1];
-    if (vals_out) *vals_out = vb[1];
-    return GQF_OK;
   }
   *keys_out = (K*)src;
   if (vals_out) *vals_out = (uint32_t*)vsrc;
