@@ -96,9 +96,13 @@ struct SqfWin {
   uint64_t occ, cont, shf;
 };
 
-__host__ __device__ __forceinline__ uint32_t byte_bits4(uint32_t x) {
-  // bit 0 of each byte of x (other bits zero) -> bits 0..3
-  return (x * 0x01020408u) >> 24;
+// Bit 0 of each byte of a, then of b (other bits zero) -> bits 0..7, one
+// multiply: with b's bits moved to bit 4 of each byte, the product's partial
+// terms (byte j, bit t, multiplier bit s in {3, 10, 17, 24}) sit at distinct
+// positions 8j + t + s, so no carries occur, and bits 24..31 hold exactly
+// a's bits (t = 0) then b's (t = 4).
+__host__ __device__ __forceinline__ uint32_t byte_bits8(uint32_t a, uint32_t b) {
+  return ((a | (b << 4)) * 0x01020408u) >> 24;
 }
 
 template <int BPE>
@@ -132,11 +136,11 @@ __host__ __device__ __forceinline__ void sqf_win_load(const uint8_t* __restrict_
   uint64_t occ = 0, cont = 0, shf = 0;
   if (BPE == 1) {
 #pragma unroll
-    for (int k = 0; k < 16; ++k) {
-      const uint32_t x = W.raw[k];
-      occ |= (uint64_t)byte_bits4((x >> 2) & 0x01010101u) << (4 * k);
-      cont |= (uint64_t)byte_bits4((x >> 1) & 0x01010101u) << (4 * k);
-      shf |= (uint64_t)byte_bits4(x & 0x01010101u) << (4 * k);
+    for (int k = 0; k < 16; k += 2) {
+      const uint32_t x = W.raw[k], y = W.raw[k + 1];
+      occ |= (uint64_t)byte_bits8((x >> 2) & 0x01010101u, (y >> 2) & 0x01010101u) << (4 * k);
+      cont |= (uint64_t)byte_bits8((x >> 1) & 0x01010101u, (y >> 1) & 0x01010101u) << (4 * k);
+      shf |= (uint64_t)byte_bits8(x & 0x01010101u, y & 0x01010101u) << (4 * k);
     }
   } else {
     // slot j = bytes 2j (high) and 2j+1 (low, holding the flags)
@@ -162,10 +166,11 @@ __host__ __device__ __forceinline__ uint64_t sqf_win_ge(const SqfWin<BPE>& W, ui
     // per byte: (rem | 0x80) - x keeps bit 7 iff rem >= x (rem, x < 128: no borrow between bytes)
     const uint32_t xr = x * 0x01010101u;
 #pragma unroll
-    for (int k = 0; k < 16; ++k) {
-      const uint32_t a = (W.raw[k] >> 3) & 0x1F1F1F1Fu;
-      const uint32_t g = ((a | 0x80808080u) - xr) & 0x80808080u;
-      m |= (uint64_t)byte_bits4(g >> 7) << (4 * k);
+    for (int k = 0; k < 16; k += 2) {
+      const uint32_t a = (W.raw[k] >> 3) & 0x1F1F1F1Fu, b = (W.raw[k + 1] >> 3) & 0x1F1F1F1Fu;
+      const uint32_t ga = ((a | 0x80808080u) - xr) & 0x80808080u;
+      const uint32_t gb = ((b | 0x80808080u) - xr) & 0x80808080u;
+      m |= (uint64_t)byte_bits8(ga >> 7, gb >> 7) << (4 * k);
     }
   } else {
 #pragma unroll
