@@ -1,8 +1,8 @@
 """Benchmark of the quotient-filter hot path on one MI355X.
 
-Default workload: standard quotient filter, q=26, r=5, fill 0.8 -- 53,687,092
-MT19937(seed 1) uint32 keys.  One step is one pass of the whole SQF pipeline
-over that data, every input already resident in HBM:
+Headline workload (value): standard quotient filter, q=26, r=5, fill 0.8 --
+53,687,092 MT19937(seed 1) uint32 keys.  One step is one pass of the whole
+SQF pipeline over that data, every input already resident in HBM:
 
   1. bulk build, NoDup (segmented layouts)            53,687,092 keys
   2. 2^24 unsorted lookups, then 2^24 sorted lookups  (half inserted keys,
@@ -15,25 +15,40 @@ over that data, every input already resident in HBM:
 value = all operations of the step / step wall time (Mops/s).  Per-class
 rates, the roofline of the dominant kernel (the one moving the most
 algorithmic bytes; those bytes over its HIP-event time, measured in this run
-on a second pass of the same steps with one event pair around every kernel)
-and the CPU baseline (reference CPU_version code,
-or the project's restatement when that is not built, on a bounded sample of
-the same workload on this host) are printed in the same JSON line.
+on a second pass of the same steps with one event pair around every kernel),
+the full-size build against HBM peak (contract bytes n(4 + 2w + 2wP) + T),
+the lookups against the random-access peak measured in the same run, and the
+CPU baseline (reference CPU_version code, or the project's restatement when
+that is not built, on a bounded sample of the same workload on this host;
+1 core, plus all-core read-only lookups and the delete loop) are printed in
+the same JSON line.
 
-Batched leg (same JSON line, "batched"): 512 SQFs (r=5, q uniform 12..22,
-fill 0.3..0.9, Zipf(1.1) keys over 2^28, Dup) built by one graph launch and
-looked up (2^8..2^18 keys each) by another, against the per-filter calls;
-tables and lookup results compared.
+RSQF step ("rsqf", same line, same fields): q=26, r=8 (34-bit
+fingerprints), fill 0.9 -- 60,397,976 keys: bulk build; 2^24 sorted and
+2^24 unsorted lookups; 2^22 deletes; build to fill 0.7 and 52 insert
+batches of 2^18 up to 0.9.
 
-Serving leg (same JSON line, "serving"): eight resident RSQFs (r=8, q from
-24..27, fill 0.5) and a Poisson stream of 2000 requests (70/20/10
+Streaming peak ("stream_peak_GBs"): read, copy and write passes over 4 GiB
+buffers (gqf_stream_bench), the practical HBM ceiling beside the 8 TB/s
+spec peak.
+
+Batched leg ("batched"): 512 SQFs (r=5, q uniform 12..22, fill 0.3..0.9,
+Zipf(1.1) keys over 2^28, Dup) built by one graph launch and looked up
+(2^8..2^18 keys each) by another, against the per-filter calls; tables and
+lookup results compared.  Also the walk-only batched lookups (the per-filter
+algorithm in one launch: the like-for-like ratio), rates over the filters
+whose keys fit, and the build graph's launch overhead.
+
+Serving leg ("serving"): eight resident RSQFs (r=8, q from 24..27, fill
+0.5) and a Poisson stream of 12,000 requests (70/20/10
 lookup/insert/delete, sizes log-uniform 2^6..2^20) at each rate of
 --serve-qps, served with coalescing and again plainly from the same starting
-tables: Mops/s and p50/p99 latency per point, each mode's best Mops/s with
-p99 <= 10 ms, and whether end states and lookup results agree.
+tables; statistics over the requests after the first 2,000: Mops/s and
+p50/p99 latency per point, each mode's best Mops/s with p99 <= 10 ms (run
+twice: both p99s reported), and whether end states and lookup results agree.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--workload sqf|rsqf]
-                  [--serve-qps Q] [--serve-requests N]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--workload both|sqf|rsqf]
+                  [--serve-qps Q] [--serve-requests N] [--serve-warmup N] [--no-batch] [--no-stream]
 """
 import argparse
 import json
@@ -198,42 +213,76 @@ def serve_filters(G, S):
 SERVE_SLO_MS = 10.0
 
 
-def serving_leg(G, qps_list, nreq, slo_ms=SERVE_SLO_MS):
+def window_stats(S, reqs, warm):
+    """Latency percentiles and throughput over the requests after the first
+    `warm` (the warm-up: the loop's workers, streams and workspaces start
+    cold)."""
+    lat, keys, t_first, t_last = [], 0, None, 0.0
+    for i in range(warm, S["nreq"]):
+        q = reqs[i]
+        if q.status != 0:
+            continue
+        lat.append(q.done_us - q.arrive_us)
+        keys += int(S["sizes"][i])
+        t_first = q.arrive_us if t_first is None else min(t_first, q.arrive_us)
+        t_last = max(t_last, q.done_us)
+    lat = np.sort(np.array(lat))
+    span = max(t_last - (t_first or 0.0), 1e-9)
+    return {"Mops_s": round(keys / span, 1), "p50_ms": round(float(np.percentile(lat, 50)) / 1e3, 3),
+            "p99_ms": round(float(np.percentile(lat, 99)) / 1e3, 3), "mean_ms": round(float(lat.mean()) / 1e3, 3),
+            "requests": len(lat)}
+
+
+def serving_leg(G, qps_list, nreq, warm, slo_ms=SERVE_SLO_MS):
     """The same stream at each arrival rate, served with coalescing
     (per-filter workers and streams) and plainly (one call per request, in
-    arrival order, one stream), each from the same starting tables.  Reports
-    every point and, per mode, the highest throughput whose p99 latency stays
-    within slo_ms.  At the highest rate the two end states (all eight tables)
-    and all lookup results are compared."""
+    arrival order, one stream), each from the same starting tables.  The
+    statistics leave out the first `warm` requests.  Reports every point and,
+    per mode, the highest throughput whose p99 latency stays within slo_ms;
+    that point is run a second time and both p99s are reported (run-to-run
+    spread).  At the highest rate the two end states (all eight tables) and
+    all lookup results are compared."""
     S = serve_setup(G, nreq)
     res = {"config": {"filters": "8 RSQF r=8, q from 24..27 (seed 1): " + ",".join(map(str, S["qs"])),
-                      "start_fill": 0.5, "requests": nreq, "mix": "70/20/10 lookup/insert/delete",
-                      "sizes": "log-uniform 2^6..2^20", "arrivals": "Poisson",
-                      "hbm_budget_bytes": SERVE_BUDGET, "keys_total": S["total"]},
+                      "start_fill": 0.5, "requests": nreq, "warmup_requests_excluded": warm,
+                      "mix": "70/20/10 lookup/insert/delete", "sizes": "log-uniform 2^6..2^20",
+                      "arrivals": "Poisson", "hbm_budget_bytes": SERVE_BUDGET, "keys_total": S["total"]},
            "slo_p99_ms": slo_ms, "sweep": []}
     top = max(qps_list)
     dumps, outs = {}, {}
+
+    def one(qps, coalesce, keep=False, mode=None):
+        fs = serve_filters(G, S)
+        reqs = serve_requests(G, S, qps)
+        st = G.serve(fs, reqs, coalesce=coalesce, hbm_budget=SERVE_BUDGET)
+        w = window_stats(S, reqs, warm)
+        w["calls"] = int(st.calls)
+        w["failed"] = sum(1 for q in reqs if q.status != G.OK)
+        if keep:
+            dumps[mode] = [f.dump() for f in fs]
+            lk = S["out"].to_host()
+            outs[mode] = np.concatenate([lk[S["offs"][i]:S["offs"][i] + S["sizes"][i]]
+                                         for i in range(nreq) if S["ops"][i] == 0])
+        for f in fs:
+            f.destroy()
+        return w
+
     for qps in qps_list:
         point = {"qps": qps, "offered_Mops_s": round(S["total"] * qps / nreq / 1e6, 1)}
         for mode, coalesce in (("served", True), ("plain", False)):
-            fs = serve_filters(G, S)
-            reqs = serve_requests(G, S, qps)
-            st = G.serve(fs, reqs, coalesce=coalesce, hbm_budget=SERVE_BUDGET)
-            point[mode] = {"Mops_s": round(st.mops, 1), "p50_ms": round(st.p50_us / 1e3, 3),
-                           "p99_ms": round(st.p99_us / 1e3, 3), "mean_ms": round(st.mean_us / 1e3, 3),
-                           "calls": int(st.calls), "failed": sum(1 for r in reqs if r.status != G.OK)}
-            if qps == top:
-                dumps[mode] = [f.dump() for f in fs]
-                lk = S["out"].to_host()
-                outs[mode] = np.concatenate([lk[S["offs"][i]:S["offs"][i] + S["sizes"][i]]
-                                             for i in range(nreq) if S["ops"][i] == 0])
-            for f in fs:
-                f.destroy()
+            point[mode] = one(qps, coalesce, keep=qps == top, mode=mode)
         res["sweep"].append(point)
-    for mode in ("served", "plain"):
+    for mode, coalesce in (("served", True), ("plain", False)):
         ok = [p for p in res["sweep"] if p[mode]["p99_ms"] <= slo_ms and p[mode]["failed"] == 0]
         best = max(ok, key=lambda p: p[mode]["Mops_s"]) if ok else None
-        res[mode] = dict(best[mode], qps=best["qps"]) if best else None
+        if best:
+            again = one(best["qps"], coalesce)
+            res[mode] = dict(best[mode], qps=best["qps"], p99_ms_runs=[best[mode]["p99_ms"], again["p99_ms"]],
+                             Mops_s_runs=[best[mode]["Mops_s"], again["Mops_s"]])
+        else:
+            res[mode] = None
+    if res["served"] and res["plain"]:
+        res["served_over_plain_Mops"] = round(res["served"]["Mops_s"] / res["plain"]["Mops_s"], 3)
     res["checked_at_qps"] = top
     res["end_state_equal"] = all((a == b).all() for a, b in zip(dumps["served"], dumps["plain"]))
     res["lookups_equal"] = bool((outs["served"] == outs["plain"]).all())
@@ -291,7 +340,7 @@ def batch_leg(G, F=512, reps=3, seed=7):
     create_s = time.perf_counter() - t0
     info = batch.info()
     # batched: one graph launch per pipeline, wall time between synchronisations
-    tb = tl = 1e30
+    tb = tl = tw = te = 1e30
     for _ in range(reps):
         G.synchronize()
         t0 = time.perf_counter()
@@ -300,6 +349,30 @@ def batch_leg(G, F=512, reps=3, seed=7):
         t0 = time.perf_counter()
         batch.lookup(use_graph=True)
         tl = min(tl, time.perf_counter() - t0)
+    walk_equal = True
+    ref_out = out_b.to_host()
+    for _ in range(reps):
+        # like for like with the per-filter calls: the same walk, no index
+        G.synchronize()
+        t0 = time.perf_counter()
+        batch.lookup_walk()
+        tw = min(tw, time.perf_counter() - t0)
+    walk_equal = bool((out_b.to_host() == ref_out).all())
+    # launch overhead of the build graph: the same kernels launched one by one,
+    # and their summed device time (per-kernel events)
+    for _ in range(reps):
+        G.synchronize()
+        t0 = time.perf_counter()
+        batch.build(use_graph=False)
+        te = min(te, time.perf_counter() - t0)
+    G.profile_reset()
+    G.profile_enable(True)
+    batch.build(use_graph=False)
+    G.synchronize()
+    G.profile_enable(False)
+    ksum = sum(v[1] for v in G.profile_stats().values())
+    batch.lookup(use_graph=True)
+    G.synchronize()
     # plain: the per-filter calls (a filter whose duplicate-heavy run spills
     # past its last slot reports a capacity error and keeps its table, in
     # both forms)
@@ -319,19 +392,36 @@ def batch_leg(G, F=512, reps=3, seed=7):
         pl = min(pl, time.perf_counter() - t0)
     tables_equal = all((a.table_bytes() == b.table_bytes()).all() for a, b in zip(fb, fp))
     lookups_equal = bool((out_b.to_host() == out_p.to_host()).all())
+    built = [i for i, x in enumerate(st) if x == G.OK]
+    nbuilt = int(sum(int(koff[i + 1] - koff[i]) for i in built))
     res = {
         "config": {"filters": F, "r": r, "q": "uniform 12..22 (seed 7)", "fill": "uniform 0.3..0.9",
                    "keys": "Zipf(1.1) over 2^28, Dup", "build_keys": N,
                    "lookups": "log-uniform 2^8..2^18 per filter, half stored keys", "lookup_keys": P},
         "batched": {"build_ms": round(tb * 1e3, 3), "lookup_ms": round(tl * 1e3, 3),
                     "build_Mops_s": round(N / tb / 1e6, 1), "lookup_Mops_s": round(P / tl / 1e6, 1),
-                    "Mops_s": round((N + P) / (tb + tl) / 1e6, 1)},
+                    "Mops_s": round((N + P) / (tb + tl) / 1e6, 1),
+                    "lookup_walk_ms": round(tw * 1e3, 3), "lookup_walk_Mops_s": round(P / tw / 1e6, 1),
+                    "build_eager_ms": round(te * 1e3, 3), "build_kernel_sum_ms": round(ksum, 3)},
         "plain": {"build_ms": round(pb * 1e3, 3), "lookup_ms": round(pl * 1e3, 3),
                   "build_Mops_s": round(N / pb / 1e6, 1), "lookup_Mops_s": round(P / pl / 1e6, 1),
                   "Mops_s": round((N + P) / (pb + pl) / 1e6, 1)},
+        # a filter whose keys do not fit keeps its table and reports a
+        # capacity error; its keys were sorted and scanned but not written
+        "filters_built": len(built), "capacity_errors": F - len(built),
+        "built_filters_keys": nbuilt,
+        "build_Mops_s_built_filters": {"batched": round(nbuilt / tb / 1e6, 1), "plain": round(nbuilt / pb / 1e6, 1)},
         "speedup": round((pb + pl) / (tb + tl), 2),
-        "tables_equal": tables_equal, "lookups_equal": lookups_equal, "statuses_equal": st == pst,
-        "capacity_errors": sum(1 for x in st if x != G.OK),
+        "speedup_build": round(pb / tb, 2),
+        # the batched lookups finish long walks through a window index only
+        # the batch has; the walk-only launch is the same algorithm as the
+        # per-filter calls
+        "speedup_lookup_with_index": round(pl / tl, 2),
+        "speedup_lookup_like_for_like": round(pl / tw, 2),
+        "speedup_like_for_like": round((pb + pl) / (tb + tw), 2),
+        "build_graph_launch_overhead_ms": round(tb * 1e3 - ksum, 3),
+        "tables_equal": tables_equal, "lookups_equal": lookups_equal, "walk_lookups_equal": walk_equal,
+        "statuses_equal": st == pst,
         "plan": info, "create_s": round(create_s, 3),
     }
     batch.destroy()
@@ -340,8 +430,21 @@ def batch_leg(G, F=512, reps=3, seed=7):
     return res
 
 
-def cpu_baseline(workload, q, r, keys, budget_s=20.0):
-    """Reference CPU path on this host, single thread, bounded sample."""
+def host_cores():
+    """Cores this process may run on (its affinity set), capped at 16, the
+    host share of one GPU on these machines."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except Exception:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def cpu_baseline(workload, q, r, keys):
+    """The reference CPU path on this host on a bounded sample of the same
+    workload (about 10-30 s): primary figure on 1 core, as the reference runs;
+    plus a read-only lookup run over all the cores this process may use, and
+    the restated delete loop."""
     import oracle
 
     n = len(keys)
@@ -351,6 +454,8 @@ def cpu_baseline(workload, q, r, keys, budget_s=20.0):
         cpu = open("/proc/cpuinfo").read().split("model name")[1].split("\n")[0].strip(" :\t")
     except Exception:
         cpu = "unknown"
+    cores = host_cores()
+    L = oracle.lib()
     if workload == "sqf" and oracle.RefSQF.available():
         # the reference CPU_version itself (FNV hash, packed words): its build is
         # the insertItem loop (CPUInsertPerf.cpp:41-48), then mayContain
@@ -364,10 +469,9 @@ def cpu_baseline(workload, q, r, keys, budget_s=20.0):
         t_lk = time.perf_counter() - t0
         kind = "reference"
         sample = (f"reference CPU_version SQF q={q} r={r}: insertItem of all {n} keys (its only build), "
-                  f"then mayContain of {m} probes (50% hit); FNV hash as compiled; 1 thread; {cpu}")
+                  f"then mayContain of {m} probes (50% hit); FNV hash as compiled; 1 thread")
     else:
         # the project's restatement of the reference CPU algorithm (AP hash)
-        L = oracle.lib()
         if workload == "sqf":
             t = np.zeros(oracle.sqf_table_bytes(q, r), np.uint8)
             t_ins = L.orc_time_sqf_insert(t, q, r, keys, n)
@@ -380,18 +484,155 @@ def cpu_baseline(workload, q, r, keys, budget_s=20.0):
             t_lk = L.orc_time_rsqf_lookup(t, q, r, probe, m, out, 1)
         kind = "port"
         sample = (f"project restatement of the reference CPU {workload.upper()} q={q} r={r}: sequential insert of "
-                  f"all {n} keys, then {m} lookups (50% hit); 1 thread; {cpu}")
-    ops = n + m
-    return {
-        "value": round(ops / (t_ins + t_lk) / 1e6, 3),
+                  f"all {n} keys, then {m} lookups (50% hit); 1 thread")
+    res = {
+        "value": round((n + m) / (t_ins + t_lk) / 1e6, 3),
         "unit": "Mops/s",
         "cores": 1,
         "kind": kind,
         "sample": sample,
+        "cpu_model": cpu,
+        "host_cores_available": cores,
         "insert_mops": round(n / t_ins / 1e6, 3),
         "lookup_mops": round(m / t_lk / 1e6, 3),
         "seconds": round(t_ins + t_lk, 2),
     }
+    # read-only lookups over every available core (std::thread per core), and
+    # the delete loop (the reference has no CPU delete: the restated
+    # deleteItemGPU loop), on the restatement's canonical table (AP hash)
+    out = np.empty(m, np.uint32)
+    if workload == "sqf":
+        st, t = oracle.sqf_canon(oracle.fingerprints(keys, q, r), q, r, True)
+        t_all = L.orc_time_sqf_lookup(t, q, r, probe, m, out, cores)
+        t_del = L.orc_time_sqf_delete(t, q, r, keys[: m], m)
+        res["delete_mops"] = round(m / t_del / 1e6, 3)
+        res["delete_sample"] = f"restated deleteItemGPU loop over {m} stored keys, 1 thread"
+    else:
+        st, t = oracle.rsqf_canon(oracle.fingerprints(keys, q, r), q, r)
+        t_all = L.orc_time_rsqf_lookup(t, q, r, probe, m, out, cores)
+    res["lookup_mops_all_cores"] = round(m / t_all / 1e6, 3)
+    res["all_cores_sample"] = f"{m} lookups (50% hit) split over {cores} threads, project restatement (AP hash)"
+    return res
+
+
+def timed_steps(G, run, d, steps, warmup):
+    """W untimed steps, then K timed steps (wall clock between two device
+    synchronizations), then the same K steps again with per-kernel HIP events
+    on the launch stream (kernel durations and algorithmic bytes for the
+    roofline), kept out of the timed region because the extra event records
+    add gaps."""
+    for _ in range(warmup):
+        run(G, d, Step(G, None))
+    st = Step(G, None)
+    G.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        run(G, d, st)
+    G.synchronize()
+    wall = time.perf_counter() - t0
+    G.profile_reset()
+    G.profile_enable(True)
+    G.synchronize()
+    tp = time.perf_counter()
+    for _ in range(steps):
+        run(G, d, Step(G, None))
+    G.synchronize()
+    wall_prof = time.perf_counter() - tp
+    G.profile_enable(False)
+    return st, wall, wall_prof, G.profile_stats()
+
+
+# expected random lines a lookup reads.  SQF: its 64-slot window (one line;
+# walks that leave it read more, so this is a floor).  RSQF: the block's
+# 32-byte metadata record, plus the remainder line when the quotient is
+# occupied -- every hit, and a miss with probability 1 - exp(-fill) for
+# uniform quotients -- at 50% hits: 1 + 0.5 + 0.5 * (1 - exp(-fill)).
+def rsqf_expected_lines(fill, hit=0.5):
+    return 1.0 + hit + (1.0 - hit) * (1.0 - float(np.exp(-fill)))
+
+
+def step_report(G, workload, q, r, fill, st, wall, wall_prof, prof, steps):
+    total_ops = sum(st.ops.values())
+    breakdown = {c: {"Mops_s": round(st.ops[c] / st.ms[c] / 1e3, 1), "ms_per_step": round(st.ms[c] / steps, 3)}
+                 for c in st.ops}
+    # dominant kernel: the one that moves the most algorithmic bytes over the
+    # step.  Its event time ranks it too, but an event pair adds a few us to
+    # every launch, so many short launches would outrank fewer long ones
+    # that take as much device time.
+    dom = max(((k, v) for k, v in prof.items() if v[2] > 0), key=lambda kv: kv[1][2])
+    name, (launches, kms, kbytes) = dom
+    avg_ms = kms / launches
+    achieved = (kbytes / launches) / (avg_ms / 1e3) / 1e9
+    kernels = {k: {"launches": v[0], "ms": round(v[1], 3),
+                   "GB_s": round(v[2] / (v[1] / 1e3) / 1e9, 1) if v[2] else None}
+               for k, v in sorted(prof.items(), key=lambda kv: -kv[1][1])}
+    # lookups against the random-access peak measured here: 2^24 random
+    # 64-byte sector reads over a buffer the size of the table
+    tbytes = G.lib().gqf_sqf_table_bytes(q, r) if workload == "sqf" else G.lib().gqf_rsqf_table_bytes(q, r)
+    rpeak = G.gather_peak(tbytes, 1 << 24)
+    if workload == "sqf":
+        per = {"contract": 1.0, "floor": 1.0}
+        note = "SQF: one 64-slot window per lookup (a floor: walks that leave the line read another)"
+    else:
+        per = {"contract": 2.0, "floor": 1.0, "expected": round(rsqf_expected_lines(fill), 3)}
+        note = ("RSQF: contract 2 sectors (the reference's 81-byte block: header + remainder word); this layout "
+                "reads the 32-byte metadata record always and the remainder line only when the quotient is "
+                "occupied: 'expected' = 1 + 0.5 + 0.5 * (1 - exp(-fill)) at 50% hits")
+    lookup_roof = {"R_peak_Gsectors_s": round(rpeak / 1e9, 2), "buffer_bytes": int(tbytes),
+                   "sectors_per_lookup": per, "note": note}
+    for cls in ("lookup_unsorted", "lookup_sorted"):
+        rate = st.ops[cls] / (st.ms[cls] / 1e3)
+        lookup_roof[cls] = {"G_lookups_s": round(rate / 1e9, 2)}
+        for k, v in per.items():
+            lookup_roof[cls]["frac_" + k] = round(rate * v / rpeak, 3)
+    traffic, tsrc = None, None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    if os.path.exists(tpath):
+        try:
+            tj = json.load(open(tpath))
+            traffic = tj.get(name)
+            if traffic is not None:
+                tsrc = (f"profiles/traffic_{workload}.json from {tj.get('source', '?')}: rocprofv3 --pmc FETCH_SIZE "
+                        "and WRITE_SIZE runs of this bench's step, per launch of this kernel, scaled by a copy "
+                        "kernel of known size (not measured in this run)")
+        except Exception:
+            traffic = None
+    roof = {
+        "bound": "hbm",
+        "kernel": name,
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": traffic,
+        "traffic_source": tsrc,
+        "algorithmic_bytes_per_launch": round(kbytes / launches),
+        "avg_launch_ms": round(avg_ms, 4),
+        "launches": launches,
+        "profiled_ms_per_step": round(wall_prof / steps * 1e3, 3),
+    }
+    return {
+        "value": round(total_ops / wall / 1e6, 1),
+        "ms_per_step": round(wall / steps * 1e3, 3),
+        "breakdown": breakdown,
+        "roofline": roof,
+        "lookup_roofline": lookup_roof,
+        "kernels": kernels,
+    }
+
+
+def build_roofline(G, report, workload, n, q, r, builds):
+    """The bulk build against HBM peak: contract bytes B_build = n (4 + 2w +
+    2wP) + T (w = fingerprint bytes, P = 8-bit radix passes, T = table bytes)
+    over the measured time of one full-size build."""
+    w = 4 if q + r <= 32 else 8
+    P = (q + r + 7) // 8
+    T = G.lib().gqf_sqf_table_bytes(q, r) if workload == "sqf" else G.lib().gqf_rsqf_table_bytes(q, r)
+    B = n * (4 + 2 * w + 2 * w * P) + T
+    ms = min(builds)
+    return {"keys": n, "contract_bytes": int(B), "best_ms": round(ms, 4),
+            "achieved_GB_s": round(B / (ms / 1e3) / 1e9, 1), "frac": round(B / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+            "target_ms_at_half_peak": round(B / (HBM_PEAK_GBS * 0.5e9) * 1e3, 4)}
 
 
 def main():
@@ -399,12 +640,15 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["sqf", "rsqf"], default="sqf")
+    ap.add_argument("--workload", choices=["both", "sqf", "rsqf"], default="both",
+                    help="both: the SQF step gives value, the RSQF step its own object")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--serve-qps", type=str, default="1000,2000,3000,4000,5000,6000,7000,8000",
+    ap.add_argument("--serve-qps", type=str, default="2000,3000,4000,5000,6000,7000,8000",
                     help="arrival rates of the serving leg, comma-separated ('' or 0: skip it)")
-    ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--serve-requests", type=int, default=12000)
+    ap.add_argument("--serve-warmup", type=int, default=2000, help="leading requests left out of the statistics")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-launch leg (512 SQFs)")
+    ap.add_argument("--no-stream", action="store_true", help="skip the streaming-bandwidth measurement")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="first copy BYTES device-to-device three times (a plain copy kernel of known size, "
                          "to scale rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")
@@ -427,123 +671,77 @@ def main():
         G.synchronize()
         a.free()
         b.free()
-    if args.workload == "sqf":
-        q, r = 26, 5
-        d, keys = setup_sqf(G, q, r)
-        run = run_sqf_step
-        cfg = {"workload": "SQF q=26 r=5 fill 0.8 (53,687,092 MT19937 keys): build NoDup+Dup, 2^24 unsorted + "
-                           "2^24 sorted lookups (50% hit), 2^22 deletes, build to 0.5 + 2^20-key insert batches to 0.8",
-               "filter": "SQF", "q": q, "r": r, "fill": 0.8, "keys": d["n"]}
-    else:
-        q, r = 26, 8
-        d, keys = setup_rsqf(G, q, r)
-        run = run_rsqf_step
-        cfg = {"workload": "RSQF q=26 r=8 fill 0.9 (60,397,976 MT19937 keys, 34-bit fingerprints): build, 2^24 "
-                           "sorted + unsorted lookups, 2^22 deletes, build to 0.7 + 2^18-key insert batches to 0.9",
-               "filter": "RSQF", "q": q, "r": r, "fill": 0.9, "keys": d["n"]}
 
-    for _ in range(args.warmup):
-        run(G, d, Step(G, args))
-    # timed region: K steps, wall clock between two device synchronizations
-    st = Step(G, args)
-    G.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        run(G, d, st)
-    G.synchronize()
-    wall = time.perf_counter() - t0
-    # the same K steps again with per-kernel HIP events on the launch stream
-    # (kernel durations and algorithmic bytes for the roofline); kept out of
-    # the timed region above because the extra event records add gaps
-    G.profile_reset()
-    G.profile_enable(True)
-    G.synchronize()
-    tp = time.perf_counter()
-    for _ in range(args.steps):
-        run(G, d, Step(G, args))
-    G.synchronize()
-    wall_prof = time.perf_counter() - tp
-    G.profile_enable(False)
-    prof = G.profile_stats()
+    workloads = ["sqf", "rsqf"] if args.workload == "both" else [args.workload]
+    reports, keysets = {}, {}
+    for wl in workloads:
+        if wl == "sqf":
+            q, r, fill = 26, 5, 0.8
+            d, keys = setup_sqf(G, q, r)
+            run = run_sqf_step
+            cfg = {"workload": "SQF q=26 r=5 fill 0.8 (53,687,092 MT19937 keys): build NoDup+Dup, 2^24 unsorted + "
+                               "2^24 sorted lookups (50% hit), 2^22 deletes, build to 0.5 + 2^20-key insert batches "
+                               "to 0.8", "filter": "SQF", "q": q, "r": r, "fill": fill, "keys": d["n"]}
+            full = lambda: d["qf"].bulkBuildSegmentedLayouts(d["keys"], NoDuplicates=True)  # noqa: E731
+        else:
+            q, r, fill = 26, 8, 0.9
+            d, keys = setup_rsqf(G, q, r)
+            run = run_rsqf_step
+            cfg = {"workload": "RSQF q=26 r=8 fill 0.9 (60,397,976 MT19937 keys, 34-bit fingerprints): build, 2^24 "
+                               "sorted + unsorted lookups (50% hit), 2^22 deletes, build to 0.7 + 2^18-key insert "
+                               "batches to 0.9", "filter": "RSQF", "q": q, "r": r, "fill": fill, "keys": d["n"]}
+            full = lambda: d["f"].bulk_build(d["keys"])  # noqa: E731
+        st, wall, wall_prof, prof = timed_steps(G, run, d, args.steps, args.warmup)
+        rep = step_report(G, wl, q, r, fill, st, wall, wall_prof, prof, args.steps)
+        rep["config"] = cfg
+        # the full-size build alone (NoDup for the SQF), timed per call on the device
+        rep["build_roofline"] = build_roofline(G, rep, wl, d["n"], q, r, [full() for _ in range(5)])
+        reports[wl] = rep
+        keysets[wl] = (q, r, keys)
+        d = None
+        G.synchronize()
 
-    total_ops = sum(st.ops.values())
-    value = total_ops / wall / 1e6
-    breakdown = {c: {"Mops_s": round(st.ops[c] / st.ms[c] / 1e3, 1), "ms_per_step": round(st.ms[c] / args.steps, 3)}
-                 for c in st.ops}
-    # dominant kernel: the one that moves the most algorithmic bytes over the
-    # step.  Its event time ranks it too, but an event pair adds a few us to
-    # every launch, so many short launches would outrank fewer long ones
-    # that take as much device time.
-    dom = max(((k, v) for k, v in prof.items() if v[2] > 0), key=lambda kv: kv[1][2])
-    name, (launches, kms, kbytes) = dom
-    avg_ms = kms / launches
-    achieved = (kbytes / launches) / (avg_ms / 1e3) / 1e9
-    kernels = {k: {"launches": v[0], "ms": round(v[1], 3),
-                   "GB_s": round(v[2] / (v[1] / 1e3) / 1e9, 1) if v[2] else None}
-               for k, v in sorted(prof.items(), key=lambda kv: -kv[1][1])}
-    # lookups against the random-access peak measured here: 2^24 random
-    # 64-byte sector reads over a buffer the size of the table
-    tbytes = G.lib().gqf_sqf_table_bytes(q, r) if args.workload == "sqf" else G.lib().gqf_rsqf_table_bytes(q, r)
-    rpeak = G.gather_peak(tbytes, 1 << 24)
-    # the floor of sectors a lookup touches: one (the SQF window; the RSQF
-    # block's metadata).  Walks that leave the 128-byte line, and RSQF lookups
-    # whose quotient is occupied (they read the remainders too), touch more,
-    # so the fractions below are lower bounds.
-    spl = 1
-    lookup_roof = {"R_peak_Gsectors_s": round(rpeak / 1e9, 2), "buffer_bytes": int(tbytes), "sectors_per_lookup": spl}
-    for cls in ("lookup_unsorted", "lookup_sorted"):
-        rate = st.ops[cls] / (st.ms[cls] / 1e3)
-        lookup_roof[cls] = {"G_lookups_s": round(rate / 1e9, 2), "frac": round(rate * spl / rpeak, 3)}
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
-    if os.path.exists(tpath):
-        try:
-            traffic = json.load(open(tpath)).get(name)
-        except Exception:
-            traffic = None
-    roof = {
-        "bound": "hbm",
-        "kernel": name,
-        "achieved": round(achieved, 1),
-        "peak": HBM_PEAK_GBS,
-        "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": traffic,
-        "algorithmic_bytes_per_launch": round(kbytes / launches),
-        "avg_launch_ms": round(avg_ms, 4),
-        "launches": launches,
-        "profiled_ms_per_step": round(wall_prof / args.steps * 1e3, 3),
-    }
+    head = reports[workloads[0]]
     out = {
         "metric": "Mops/s build/insert/lookup/delete @q=26 fill 0.8; p50/p99 served-batch latency",
-        "value": round(value, 1),
+        "value": head["value"],
         "unit": "Mops/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(wall / args.steps * 1e3, 3),
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u32",
         "data": "synthetic (MT19937 uint32 keys, seed 1; misses from seed 2)",
-        "config": cfg,
-        "breakdown": breakdown,
-        "roofline": roof,
-        "lookup_roofline": lookup_roof,
-        "kernels": kernels,
+        "config": head["config"],
+        "breakdown": head["breakdown"],
+        "roofline": head["roofline"],
+        "build_roofline": head["build_roofline"],
+        "lookup_roofline": head["lookup_roofline"],
+        "kernels": head["kernels"],
     }
+    if len(workloads) > 1:
+        out["rsqf"] = reports["rsqf"]
+    if not args.no_stream:
+        # streaming HBM bandwidth over 4 GiB buffers (beyond the 256 MB MALL)
+        sp = G.stream_peak(4 << 30, 5)
+        out["stream_peak_GBs"] = {k: round(v, 1) if isinstance(v, float) else v for k, v in sp.items()}
     if not args.no_batch:
-        d = None
-        G.synchronize()
         out["batched"] = batch_leg(G)
     qps_list = [float(x) for x in args.serve_qps.split(",") if x.strip() and float(x) > 0]
     if qps_list:
-        d = None  # free the step's buffers first
         G.synchronize()
-        out["serving"] = serving_leg(G, qps_list, args.serve_requests)
+        out["serving"] = serving_leg(G, qps_list, args.serve_requests, args.serve_warmup)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(args.workload, q, r, keys)
+        for wl in workloads:
+            q, r, keys = keysets[wl]
+            cb = cpu_baseline(wl, q, r, keys)
+            if wl == workloads[0]:
+                out["cpu_baseline"] = cb
+            else:
+                out[wl]["cpu_baseline"] = cb
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out))

@@ -139,6 +139,7 @@ _SIGS = {
                                    _vp, C.POINTER(C.c_uint64), _vp, _vp]),
     "gqf_sqf_batch_build": (_st, [_vp, C.c_int, C.POINTER(C.c_int), _f]),
     "gqf_sqf_batch_lookup": (_st, [_vp, C.c_int, _f]),
+    "gqf_sqf_batch_lookup_walk": (_st, [_vp, _f]),
     "gqf_sqf_batch_info": (_st, [_vp, C.POINTER(BatchInfo)]),
     "gqf_sqf_batch_destroy": (_st, [_vp]),
     "gqf_serve_run": (_st, [C.c_int, C.c_size_t, C.POINTER(Request), C.c_int, C.POINTER(ServeConfig),
@@ -150,6 +151,7 @@ _SIGS = {
                               C.POINTER(C.c_double)]),
     "gqf_mt19937_fill": (None, [C.c_uint32, C.c_uint64, C.c_size_t, _vp]),
     "gqf_gather_bench": (_st, [C.c_size_t, C.c_uint32, C.c_int, _f]),
+    "gqf_stream_bench": (_st, [C.c_size_t, C.c_int, _f, _f, _f]),
     "gqf_sqf_table_bytes": (C.c_size_t, [C.c_uint, C.c_uint]),
     "gqf_sqf_num_slots": (C.c_uint32, [C.c_uint]),
     "gqf_sqf_init": (_st, [C.POINTER(_SQF), C.c_uint, C.c_uint]),
@@ -258,6 +260,15 @@ def gather_peak(buffer_bytes, n=1 << 24, reps=5):
     ms = C.c_float(0)
     check(lib().gqf_gather_bench(int(buffer_bytes), int(n), int(reps), C.byref(ms)))
     return n / (ms.value / 1e3)
+
+
+def stream_peak(nbytes=2 << 30, reps=5):
+    """Streaming HBM bandwidth in GB/s over buffers of nbytes (gqf_stream_bench):
+    read, copy (read + write bytes counted) and write."""
+    r, c, w = C.c_float(0), C.c_float(0), C.c_float(0)
+    check(lib().gqf_stream_bench(int(nbytes), int(reps), C.byref(r), C.byref(c), C.byref(w)))
+    return {"read": nbytes / (r.value / 1e3) / 1e9, "copy": 2 * nbytes / (c.value / 1e3) / 1e9,
+            "write": nbytes / (w.value / 1e3) / 1e9, "bytes": int(nbytes), "reps": int(reps)}
 
 
 def mt19937(seed, n, skip=0):
@@ -560,6 +571,13 @@ class SQFBatch:
     def lookup(self, use_graph=True):
         ms = C.c_float(0)
         check(lib().gqf_sqf_batch_lookup(self.h, int(bool(use_graph)), C.byref(ms)))
+        return ms.value
+
+    def lookup_walk(self):
+        """The same lookups with every walk run to its end (no window index):
+        the per-filter algorithm in one launch."""
+        ms = C.c_float(0)
+        check(lib().gqf_sqf_batch_lookup_walk(self.h, C.byref(ms)))
         return ms.value
 
     def info(self):

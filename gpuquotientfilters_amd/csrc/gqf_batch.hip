@@ -420,12 +420,15 @@ __global__ __launch_bounds__(1024) void k_bi_scan(const uint32_t* __restrict__ w
 // poff[f] <= i (binary search over F + 1 offsets), then the per-filter
 // lookup (unsorted path).  IPT keys per thread, strided by the workgroup.  A
 // walk that needs more than BB_WALK_BUDGET windows (a long cluster: hot keys
-// of a skewed batch) is finished through the filter's window index.
+// of a skewed batch) is finished through the filter's window index; with
+// budget 0xFFFFFFFF every walk runs to its end (the per-filter algorithm, no
+// index needed).
 constexpr uint32_t BB_WALK_BUDGET = 8;
 template <int BPE, bool WIN, int IPT>
 __global__ __launch_bounds__(1024) void k_bb_lookup(const BFilter* __restrict__ flt, const uint32_t* __restrict__ poff,
                                                     uint32_t F, const uint32_t* __restrict__ keys, uint32_t n,
-                                                    uint32_t* __restrict__ out, const uint2* __restrict__ idx) {
+                                                    uint32_t* __restrict__ out, const uint2* __restrict__ idx,
+                                                    uint32_t budget) {
   const uint32_t base = blockIdx.x * blockDim.x * IPT + threadIdx.x;
   for (int j = 0; j < IPT; ++j) {
     const uint32_t i = base + (uint32_t)j * blockDim.x;
@@ -439,7 +442,7 @@ __global__ __launch_bounds__(1024) void k_bb_lookup(const BFilter* __restrict__ 
       const BFilter fl = flt[lo];
       const uint32_t fp = fingerprint32(keys[i], fl.bits);
       const uint32_t fq = (fp >> fl.r) & (uint32_t)mask64(fl.q), fr = fp & ((1u << fl.r) - 1u);
-      uint32_t res = sqf_lookup_fp<BPE, WIN>(fl.table, fl.S, fq, fr, BB_WALK_BUDGET);
+      uint32_t res = sqf_lookup_fp<BPE, WIN>(fl.table, fl.S, fq, fr, budget);
       if (res == SQF_GIVE_UP) res = sqf_lookup_indexed<BPE>(fl.table, fl.S, idx + fl.idx0, fq, fr);
       out[i] = res;
     }
@@ -596,15 +599,17 @@ static void batch_build_launch(SqfBatch& b) {
 
 // The lookups: first the window index of every table as it is now (a
 // filter may have changed since the batch build), then one lookup launch.
-static void batch_lookup_launch(SqfBatch& b) {
+// walk_only: no index; every walk runs to its end.
+static void batch_lookup_launch_as(SqfBatch& b, bool walk_only) {
   if (b.P == 0) return;
-  batch_index_launch(b);
+  if (!walk_only) batch_index_launch(b);
+  const uint32_t budget = walk_only ? 0xFFFFFFFFu : BB_WALK_BUDGET;
   const uint32_t per = (uint32_t)(b.lk_block * b.lk_ipt);
   const dim3 grid((b.P + per - 1) / per), block((unsigned)b.lk_block);
   const double bytes = (double)b.P * (8 + 64);
 #define GQF_BB_LK(BPE_, WIN_, IPT_)                                                                                 \
   GQF_LAUNCH_B(bytes, (k_bb_lookup<BPE_, WIN_, IPT_>), grid, block, 0, b.stream, b.d_flt, b.d_poff, (uint32_t)b.F, \
-               b.d_probe, b.P, b.d_out, b.d_idx)
+               b.d_probe, b.P, b.d_out, b.d_idx, budget)
 #define GQF_BB_LK_IPT(BPE_, WIN_)           \
   do {                                      \
     if (b.lk_ipt == 1) GQF_BB_LK(BPE_, WIN_, 1); \
@@ -620,6 +625,7 @@ static void batch_lookup_launch(SqfBatch& b) {
 #undef GQF_BB_LK_IPT
 #undef GQF_BB_LK
 }
+static void batch_lookup_launch(SqfBatch& b) { batch_lookup_launch_as(b, false); }
 
 static gqf_status time_on_stream(SqfBatch& b, void (*fn)(SqfBatch&), int reps, float* best) {
   hipEvent_t e0, e1;
@@ -902,6 +908,15 @@ gqf_status gqf_sqf_batch_lookup(gqf_sqf_batch h, int use_graph, float* ms) {
     if (use_graph) GQF_HIP(hipGraphLaunch(b.g_lookup, b.stream));
     else batch_lookup_launch(b);
   }
+  return cs.end(ms, false);
+}
+
+gqf_status gqf_sqf_batch_lookup_walk(gqf_sqf_batch h, float* ms) {
+  if (!h) return GQF_ERR_INVALID;
+  SqfBatch& b = h->b;
+  CallScope cs;
+  GQF_TRY(cs.begin(reinterpret_cast<gqf_stream>(b.stream)));
+  if (b.P) batch_lookup_launch_as(b, true);
   return cs.end(ms, false);
 }
 

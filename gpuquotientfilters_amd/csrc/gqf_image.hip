@@ -214,6 +214,13 @@ gqf_status gqf_sqf_image_load(gqf_sqf qf, const void* image, size_t bytes) {
     set_error("image (layout %d, q %u, r %u) does not fit this SQF (q %u, r %u)", lay, q, r, qf.qbits, qf.rbits);
     return GQF_ERR_INVALID;
   }
+  // the table's fingerprints were made with the image's hash; queries of the
+  // calling thread use its own, so a mismatch would give silent misses
+  if (hs != gqf_sqf_get_hash()) {
+    set_error("image was saved with SQF hash %d but the calling thread uses hash %d (gqf_sqf_set_hash)", hs,
+              gqf_sqf_get_hash());
+    return GQF_ERR_INVALID;
+  }
   const size_t tb = gqf_sqf_table_bytes(q, r);
   if (lay == GQF_IMAGE_SQF_GPU) {
     GQF_HIP(hipMemcpy(qf.table, payload, tb, hipMemcpyHostToDevice));
@@ -261,6 +268,10 @@ gqf_status gqf_rsqf_image_load(gqf_rsqf cqf, const void* image, size_t bytes) {
   GQF_TRY(read_header(image, bytes, &lay, &q, &r, &hs, &payload));
   if (lay != GQF_IMAGE_RSQF || q != cqf.qbits || r != cqf.rbits) {
     set_error("image (layout %d, q %u, r %u) does not fit this RSQF (q %u, r %u)", lay, q, r, cqf.qbits, cqf.rbits);
+    return GQF_ERR_INVALID;
+  }
+  if (hs != GQF_HASH_AP) {
+    set_error("RSQF image records hash %d; the RSQF hashes with AP only", hs);
     return GQF_ERR_INVALID;
   }
   return gqf_rsqf_load(cqf, payload);

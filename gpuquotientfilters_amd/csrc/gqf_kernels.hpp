@@ -2,19 +2,19 @@
 //
 //   * fingerprint(): Normal_APHash (quotientFilter.cu:134-149 / RSQF.cu:174-189)
 //     with the wide extension for q + r > 32;
-//   * an LDS-staged LSD radix sort ("onesweep": one read and one write of the
-//     keys per 8-bit digit, wave64 ballot ranking, decoupled look-back over
-//     tiles), optionally hashing raw keys on the fly and carrying a uint32
-//     index -- replaces thrust::sort / sort_by_key (quotientFilter.cu:295,
-//     304, 770, 963, 1475; RSQF.cu:433, 442, 913);
-//   * the canonical-layout scan: loc_i = max(fq_i, loc_{i-1} + 1) as a
-//     single-pass decoupled look-back scan over (count, last location) pairs,
-//     fused with duplicate removal / merge rules / delete flags -- replaces the
-//     location computations of the three SQF builds (quotientFilter.cu:788-841,
-//     985-1022, 1516-1530) and thrust::unique;
-//   * merge-path merge with a source tag -- replaces mgpu::merge
+//   * an LDS-staged LSD radix sort, chunked reduce-then-scan (per pass: chunk
+//     digit histograms, a row scan of them, then a downsweep that ranks each
+//     4096-key sub-tile with wave64 ballots and writes runs of equal digits;
+//     no inter-workgroup waits), optionally hashing raw keys on the fly and
+//     carrying a uint32 index -- replaces thrust::sort / sort_by_key
+//     (quotientFilter.cu:295, 304, 770, 963, 1475; RSQF.cu:433, 442, 913);
+//   * the row scans the decoders use for per-region counts;
+//   * merge-path splits with a source tag -- replaces mgpu::merge
 //     (quotientFilter.cu:1651, 1800);
 //   * delete marking (multiset difference, one stored copy per delete key).
+// The canonical-layout scan (loc_i = max(fq_i, loc_{i-1} + 1), a
+// reduce-then-scan over chunk states fused with the keep rules) lives in
+// gqf_rebuild.hpp.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -305,92 +305,287 @@ __global__ __launch_bounds__(256) void k_rs_upsweep(const void* __restrict__ in,
   if (tid == 0) totals[blockIdx.x] = s_carry;
 }
 
-// Occupancy: its registers hold a sub-tile's keys and ranks; the bounds give
-// 5 waves per SIMD to the 32-bit key-only sort (96 VGPRs) and 4 to the
-// 32-bit sort with values and the 64-bit key-only one (their LDS allows no
-// more), each at the cost of one spilled register; the LDS of the 64-bit
-// sort with values allows 2.
+// ---------------------------------------------------------------- LDS tile sort
+// A sub-tile of SORT_TILE entries is sorted stably by an 8-bit digit in LDS
+// with two 4-bit counting passes (low nibble, then high nibble), in blocked
+// order: thread t holds tile positions 16t .. 16t+15.  Per pass:
+//   1. each thread counts its 16 nibbles in one 64-bit register of 4-bit
+//      fields; the field's value before an entry is the entry's rank among
+//      the thread's entries of that nibble (a field reaches 16 only when all
+//      16 entries share the nibble, which the 16th entry's rank of 15 shows);
+//   2. the 16 x 256 counts, nibble-major and thread-minor, are scanned;
+//   3. every entry goes to prefix(nibble, t) + rank.
+// Positions of equal keys never swap, so padding entries (all-ones keys at
+// the end of a short tile) keep the last positions of the last digit.  This
+// replaces per-entry matching of the 8 digit bits with wave ballots, whose
+// ~90 VALU instructions per entry made the downsweep ALU-bound.
+//
+// In LDS the 16-byte chunks of every 16 consecutive entries are XOR-permuted
+// (ts_swz) so that threads reading their own 16 entries -- 64 or 128 bytes
+// apart -- hit different banks; rows read by consecutive lanes stay whole.
+template <int E>
+__device__ __forceinline__ uint32_t ts_swz(uint32_t p) {
+  static_assert(E == 4 || E == 8, "4- or 8-byte entries");
+  if (E == 4) return p ^ (((p >> 6) & 3u) << 2);
+  return p ^ (((p >> 5) & 7u) << 1);
+}
+
+// Thread t's 16 consecutive entries [16t, 16t+16) of the sub-tile at t0
+// (tile_n of them valid; the rest become all-ones padding).  vec: the array
+// is 16-byte aligned, so whole groups load as 16-byte words.
+template <typename K, bool HASH>
+__device__ __forceinline__ void ts_load_keys(const void* __restrict__ in, uint64_t t0, uint32_t tile_n,
+                                             uint32_t fp_bits, bool vec, K (&k)[SORT_ITEMS]) {
+  const uint32_t i0 = threadIdx.x * SORT_ITEMS;
+  if (HASH || sizeof(K) == 4) {
+    const uint32_t* p = reinterpret_cast<const uint32_t*>(in) + t0 + i0;
+    uint32_t w[SORT_ITEMS];
+    if (vec && i0 + SORT_ITEMS <= tile_n) {
+#pragma unroll
+      for (int c = 0; c < SORT_ITEMS / 4; ++c) {
+        const uint4 x = reinterpret_cast<const uint4*>(p)[c];
+        w[4 * c] = x.x;
+        w[4 * c + 1] = x.y;
+        w[4 * c + 2] = x.z;
+        w[4 * c + 3] = x.w;
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < SORT_ITEMS; ++i) w[i] = i0 + i < tile_n ? p[i] : 0u;
+    }
+#pragma unroll
+    for (int i = 0; i < SORT_ITEMS; ++i) {
+      K x;
+      if (HASH) x = sizeof(K) == 4 ? (K)fingerprint32(w[i], fp_bits) : (K)fingerprint(w[i], fp_bits);
+      else x = (K)w[i];
+      k[i] = i0 + i < tile_n ? x : (K)~(K)0;
+    }
+  } else {
+    const K* p = reinterpret_cast<const K*>(in) + t0 + i0;
+    if (vec && i0 + SORT_ITEMS <= tile_n) {
+#pragma unroll
+      for (int c = 0; c < SORT_ITEMS / 2; ++c) {
+        const uint4 x = reinterpret_cast<const uint4*>(p)[c];
+        k[2 * c] = (K)(((uint64_t)x.y << 32) | x.x);
+        k[2 * c + 1] = (K)(((uint64_t)x.w << 32) | x.z);
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < SORT_ITEMS; ++i) k[i] = i0 + i < tile_n ? p[i] : (K)~(K)0;
+    }
+  }
+}
+
+// Values of the same positions; without an input array, the input index.
+__device__ __forceinline__ void ts_load_vals(const uint32_t* __restrict__ in_vals, uint64_t t0, uint32_t tile_n,
+                                             bool vec, uint32_t (&v)[SORT_ITEMS]) {
+  const uint32_t i0 = threadIdx.x * SORT_ITEMS;
+  if (!in_vals) {
+#pragma unroll
+    for (int i = 0; i < SORT_ITEMS; ++i) v[i] = (uint32_t)(t0 + i0 + i);
+    return;
+  }
+  const uint32_t* p = in_vals + t0 + i0;
+  if (vec && i0 + SORT_ITEMS <= tile_n) {
+#pragma unroll
+    for (int c = 0; c < SORT_ITEMS / 4; ++c) {
+      const uint4 x = reinterpret_cast<const uint4*>(p)[c];
+      v[4 * c] = x.x;
+      v[4 * c + 1] = x.y;
+      v[4 * c + 2] = x.z;
+      v[4 * c + 3] = x.w;
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < SORT_ITEMS; ++i) v[i] = i0 + i < tile_n ? p[i] : 0u;
+  }
+}
+
+// Step 1 of a counting pass over the nibble at bit sh: the entries' ranks
+// within the thread (4 bits each, entry i at bits 4i of *rk) and the thread's
+// 16 counts, written to s_cnt[j*256 + t].
+template <typename K>
+__device__ __forceinline__ void ts_count(const K (&k)[SORT_ITEMS], uint32_t sh, uint64_t& rk, uint16_t* s_cnt) {
+  uint64_t h = 0;
+  uint32_t last = 0, r = 0;
+  rk = 0;
+#pragma unroll
+  for (int i = 0; i < SORT_ITEMS; ++i) {
+    const uint32_t d = (uint32_t)(k[i] >> sh) & 15u;
+    r = (uint32_t)(h >> (4 * d)) & 15u;
+    rk |= (uint64_t)r << (4 * i);
+    h += 1ull << (4 * d);
+    last = d;
+  }
+  const bool full = r == 15u;  // the 16th entry saw 15 before it: all 16 share nibble `last`
+  const uint32_t t = threadIdx.x;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const uint32_t c = full ? (j == (int)last ? 16u : 0u) : (uint32_t)(h >> (4 * j)) & 15u;
+    s_cnt[j * SORT_THREADS + t] = (uint16_t)c;
+  }
+}
+
+// Step 2: thread t scans the 16 counts s_cnt[16t .. 16t+15] (extra, a value
+// scanned alongside in the high half of a 32-bit word: both halves stay below
+// 2^16) and writes exclusive prefixes back.  Returns the exclusive prefix of
+// `extra` over the block.  Contains two barriers.
+__device__ __forceinline__ uint32_t ts_scan(uint16_t* s_cnt, uint32_t extra, uint32_t* s_tmp) {
+  const uint32_t t = threadIdx.x;
+  uint4* row = reinterpret_cast<uint4*>(s_cnt + 16 * t);
+  uint4 a = row[0], b = row[1];
+  uint32_t w[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+  uint32_t sum = 0;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) sum += (w[j] & 0xFFFFu) + (w[j] >> 16);
+  const uint32_t ex = block256_excl_scan(sum | (extra << 16), s_tmp, nullptr);
+  uint32_t run = ex & 0xFFFFu;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const uint32_t lo = run;
+    run += w[j] & 0xFFFFu;
+    const uint32_t hi = run;
+    run += w[j] >> 16;
+    w[j] = lo | (hi << 16);
+  }
+  row[0] = make_uint4(w[0], w[1], w[2], w[3]);
+  row[1] = make_uint4(w[4], w[5], w[6], w[7]);
+  return ex >> 16;
+}
+
+// Step 3: every entry to its position (keys and values, swizzled).
+template <typename K, bool VALS>
+__device__ __forceinline__ void ts_scatter(const K (&k)[SORT_ITEMS], const uint32_t (&v)[SORT_ITEMS], uint32_t sh,
+                                           uint64_t rk, const uint16_t* s_cnt, K* s_keys, uint32_t* s_vals) {
+  const uint32_t t = threadIdx.x;
+#pragma unroll
+  for (int i = 0; i < SORT_ITEMS; ++i) {
+    const uint32_t d = (uint32_t)(k[i] >> sh) & 15u;
+    const uint32_t pos = (uint32_t)s_cnt[d * SORT_THREADS + t] + ((uint32_t)(rk >> (4 * i)) & 15u);
+    s_keys[ts_swz<sizeof(K)>(pos)] = k[i];
+    if (VALS) s_vals[ts_swz<4>(pos)] = v[i];
+  }
+}
+
+// Thread t's 16 sorted entries [16t, 16t+16) back from LDS.
+template <typename K, bool VALS>
+__device__ __forceinline__ void ts_read_blocked(const K* s_keys, const uint32_t* s_vals, K (&k)[SORT_ITEMS],
+                                                uint32_t (&v)[SORT_ITEMS]) {
+  const uint32_t b = threadIdx.x * SORT_ITEMS;
+  if (sizeof(K) == 4) {
+#pragma unroll
+    for (int c = 0; c < SORT_ITEMS / 4; ++c) {
+      const uint4 x = *reinterpret_cast<const uint4*>(s_keys + ts_swz<4>(b + 4 * c));
+      k[4 * c] = (K)x.x;
+      k[4 * c + 1] = (K)x.y;
+      k[4 * c + 2] = (K)x.z;
+      k[4 * c + 3] = (K)x.w;
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < SORT_ITEMS / 2; ++c) {
+      const uint4 x = *reinterpret_cast<const uint4*>(s_keys + ts_swz<8>(b + 2 * c));
+      k[2 * c] = (K)(((uint64_t)x.y << 32) | x.x);
+      k[2 * c + 1] = (K)(((uint64_t)x.w << 32) | x.z);
+    }
+  }
+  if (VALS) {
+#pragma unroll
+    for (int c = 0; c < SORT_ITEMS / 4; ++c) {
+      const uint4 x = *reinterpret_cast<const uint4*>(s_vals + ts_swz<4>(b + 4 * c));
+      v[4 * c] = x.x;
+      v[4 * c + 1] = x.y;
+      v[4 * c + 2] = x.z;
+      v[4 * c + 3] = x.w;
+    }
+  }
+}
+
+// Downsweep of one 8-bit pass: each chunk walks its sub-tiles in order,
+// sorts each by the digit in LDS (the tile sort above) and writes every
+// digit's run to the digit base + the chunk's offset + the runs of earlier
+// sub-tiles.  The sub-tile's digit counts come from LDS atomics on per-wave
+// histograms during the first counting pass.  Occupancy: the 32-bit
+// key-only sort takes 30 KB of LDS and 128 VGPRs (four workgroups per CU;
+// at five its loop-invariant addresses spill); with values or 64-bit keys,
+// 46 KB (three per CU), both: 62 KB (two).
 template <typename K, bool VALS, bool HASH>
-__global__ __launch_bounds__(256, sizeof(K) == 4 ? (VALS ? 4 : 5) : (VALS ? 2 : 4)) void k_rs_downsweep(const void* __restrict__ in_keys,
-                                                      const uint32_t* __restrict__ in_vals, K* __restrict__ out_keys,
-                                                      uint32_t* __restrict__ out_vals, uint32_t n, uint32_t fp_bits,
-                                                      uint32_t shift, uint32_t chunk, uint32_t G,
-                                                      const uint32_t* __restrict__ hist,
-                                                      const uint32_t* __restrict__ digit_totals) {
+__global__ __launch_bounds__(256, sizeof(K) == 4 ? (VALS ? 3 : 4) : (VALS ? 2 : 3)) void k_rs_downsweep(
+    const void* __restrict__ in_keys, const uint32_t* __restrict__ in_vals, K* __restrict__ out_keys,
+    uint32_t* __restrict__ out_vals, uint32_t n, uint32_t fp_bits, uint32_t shift, uint32_t chunk, uint32_t G,
+    const uint32_t* __restrict__ hist, const uint32_t* __restrict__ digit_totals) {
+  __shared__ __align__(16) K s_keys[SORT_TILE];
+  __shared__ __align__(16) uint32_t s_vals[VALS ? SORT_TILE : 4];
+  __shared__ __align__(16) uint16_t s_cnt[16 * SORT_THREADS];
+  __shared__ uint32_t s_hist[4 * 256];
   __shared__ uint32_t s_run[256];
-  __shared__ uint32_t s_wcnt[4 * 256];
-  __shared__ uint32_t s_lstart[256];
+  __shared__ uint32_t s_base[256];
   __shared__ uint32_t s_tmp[4];
-  __shared__ K s_keys[SORT_TILE];
-  __shared__ uint32_t s_vals[VALS ? SORT_TILE : 1];
-  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const uint32_t tid = threadIdx.x, wid = tid >> 6;
+#pragma unroll
+  for (int w = 0; w < 4; ++w) s_hist[w * 256 + tid] = 0;
   // digit bases: exclusive scan of the 256 digit totals (every workgroup
-  // does it; it saves a launch)
+  // does it; it saves a launch), plus this chunk's offset
   s_run[tid] = block256_excl_scan(digit_totals[tid], s_tmp, nullptr) + hist[(uint64_t)tid * G + blockIdx.x];
   const uint64_t c0 = (uint64_t)blockIdx.x * chunk;
   const uint64_t c1 = c0 + chunk < n ? c0 + chunk : (uint64_t)n;
-  const uint64_t lt = (1ull << lane) - 1ull;
+  const bool kvec = (reinterpret_cast<uintptr_t>(in_keys) & 15u) == 0;
+  const bool vvec = (reinterpret_cast<uintptr_t>(in_vals) & 15u) == 0;
+  K k[SORT_ITEMS];
+  uint32_t v[SORT_ITEMS];
+  uint64_t rk;
+  if (c0 < c1) {
+    const uint32_t tn = (uint32_t)(c1 - c0 < (uint64_t)SORT_TILE ? c1 - c0 : (uint64_t)SORT_TILE);
+    ts_load_keys<K, HASH>(in_keys, c0, tn, fp_bits, kvec, k);
+    if (VALS) ts_load_vals(in_vals, c0, tn, vvec, v);
+  }
   for (uint64_t t0 = c0; t0 < c1; t0 += SORT_TILE) {
-    for (int i = tid; i < 4 * 256; i += 256) s_wcnt[i] = 0;
-    __syncthreads();
-    const uint64_t base = t0 + (uint64_t)wid * (SORT_ITEMS * 64) + lane;
-    K k[SORT_ITEMS];
-    uint32_t v[VALS ? SORT_ITEMS : 1];
-#pragma unroll
-    for (int i = 0; i < SORT_ITEMS; ++i) {
-      const uint64_t idx = base + (uint64_t)i * 64;
-      const bool valid = idx < c1;
-      // padding keys (all ones) take the last digit and rank after real keys
-      k[i] = valid ? sort_load_key<K, HASH>(in_keys, idx, fp_bits) : (K)~(K)0;
-      if (VALS) v[i] = in_vals ? (valid ? in_vals[idx] : 0u) : (uint32_t)idx;
-    }
-    uint32_t rk[SORT_ITEMS];
-#pragma unroll
-    for (int i = 0; i < SORT_ITEMS; ++i) {
-      const uint32_t d = (uint32_t)(k[i] >> shift) & 255u;
-      const uint64_t m = match_digit(d);
-      const uint32_t pre = s_wcnt[wid * 256 + d];
-      const uint32_t cnt = (uint32_t)__popcll(m);
-      const uint32_t below = (uint32_t)__popcll(m & lt);
-      if (below == cnt - 1) s_wcnt[wid * 256 + d] = pre + cnt;
-      rk[i] = pre + below;
-    }
-    __syncthreads();
-    uint32_t total;
-    {
-      const uint32_t d = tid;
-      const uint32_t w0 = s_wcnt[d], w1 = s_wcnt[256 + d], w2 = s_wcnt[512 + d], w3 = s_wcnt[768 + d];
-      total = w0 + w1 + w2 + w3;
-      s_wcnt[d] = 0;
-      s_wcnt[256 + d] = w0;
-      s_wcnt[512 + d] = w0 + w1;
-      s_wcnt[768 + d] = w0 + w1 + w2;
-      s_lstart[d] = block256_excl_scan(total, s_tmp, nullptr);
-    }
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < SORT_ITEMS; ++i) {
-      const uint32_t d = (uint32_t)(k[i] >> shift) & 255u;
-      const uint32_t p = s_lstart[d] + s_wcnt[wid * 256 + d] + rk[i];
-      s_keys[p] = k[i];
-      if (VALS) s_vals[p] = v[i];
-    }
-    __syncthreads();
     const uint32_t tile_n = (uint32_t)(c1 - t0 < (uint64_t)SORT_TILE ? c1 - t0 : (uint64_t)SORT_TILE);
+    // pass 1 (low nibble) and the sub-tile's digit histogram
+    ts_count(k, shift, rk, s_cnt);
 #pragma unroll
-    for (int i = 0; i < SORT_ITEMS; ++i) {
-      const uint32_t p = tid + i * SORT_THREADS;
+    for (int i = 0; i < SORT_ITEMS; ++i) atomicAdd(&s_hist[wid * 256 + ((uint32_t)(k[i] >> shift) & 255u)], 1u);
+    __syncthreads();
+    uint32_t total = 0;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      total += s_hist[w * 256 + tid];
+      s_hist[w * 256 + tid] = 0;  // the next sub-tile's counts start after several barriers
+    }
+    const uint32_t lstart = ts_scan(s_cnt, total, s_tmp);
+    s_base[tid] = s_run[tid] - lstart;  // output of sorted position p with digit tid: s_base + p
+    __syncthreads();
+    ts_scatter<K, VALS>(k, v, shift, rk, s_cnt, s_keys, s_vals);
+    __syncthreads();
+    ts_read_blocked<K, VALS>(s_keys, s_vals, k, v);
+    // pass 2 (high nibble)
+    ts_count(k, shift + 4, rk, s_cnt);
+    __syncthreads();
+    (void)ts_scan(s_cnt, 0u, s_tmp);
+    __syncthreads();
+    ts_scatter<K, VALS>(k, v, shift + 4, rk, s_cnt, s_keys, s_vals);
+    __syncthreads();
+    // the next sub-tile's loads go out before this one's stores
+    const uint64_t tn0 = t0 + SORT_TILE;
+    if (tn0 < c1) {
+      const uint32_t tn = (uint32_t)(c1 - tn0 < (uint64_t)SORT_TILE ? c1 - tn0 : (uint64_t)SORT_TILE);
+      ts_load_keys<K, HASH>(in_keys, tn0, tn, fp_bits, kvec, k);
+      if (VALS) ts_load_vals(in_vals, tn0, tn, vvec, v);
+    }
+    // runs of equal digits to their outputs; consecutive lanes, consecutive positions
+#pragma unroll 4
+    for (int j = 0; j < SORT_ITEMS; ++j) {
+      const uint32_t p = tid + (uint32_t)j * SORT_THREADS;
       if (p < tile_n) {
-        const K key = s_keys[p];
-        const uint32_t d = (uint32_t)(key >> shift) & 255u;
-        const uint32_t gpos = s_run[d] + (p - s_lstart[d]);
+        const K key = s_keys[ts_swz<sizeof(K)>(p)];
+        const uint32_t gpos = s_base[(uint32_t)(key >> shift) & 255u] + p;
         out_keys[gpos] = key;
-        if (VALS) out_vals[gpos] = s_vals[p];
+        if (VALS) out_vals[gpos] = s_vals[ts_swz<4>(p)];
       }
     }
-    __syncthreads();
     s_run[tid] += total;
-    __syncthreads();
+    // no barrier: the next sub-tile writes s_base and s_keys only after its
+    // first barrier, which every thread reaches after these reads
   }
 }
 
@@ -485,10 +680,9 @@ inline size_t sort_workspace_bytes(uint64_t n, size_t key_bytes, bool vals) {
 // first pass.  vals: nullptr + with_vals -> the original index is carried.
 // Returns the buffers holding the result through *keys_out / *vals_out.
 template <typename K>
-gqf_status radix_sort(hipStream_t s, DeviceState& ds, Arena& ar, const void* in, bool hash, uint32_t fp_bits,
+gqf_status radix_sort(hipStream_t s, Arena& ar, const void* in, bool hash, uint32_t fp_bits,
                       const uint32_t* in_vals, bool with_vals, uint32_t n, uint32_t lo_bit, uint32_t hi_bit,
                       K** keys_out, uint32_t** vals_out) {
-  (void)ds;
   K* kb[2] = {ar.take<K>(n ? n : 1), ar.take<K>(n ? n : 1)};
   uint32_t* vb[2] = {nullptr, nullptr};
   if (with_vals) {

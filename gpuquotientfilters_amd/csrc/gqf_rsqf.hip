@@ -1103,7 +1103,7 @@ static gqf_status rsqf_build_t(const gqf_rsqf& f, uint32_t n, const uint32_t* ke
   GQF_TRY(ws_reserve(ds, sort_workspace_bytes(n, sizeof(K), false) + rsqf_rebuild_ws(n)));
   Arena ar(ds.ws, ds.ws_cap);
   K* sorted;
-  GQF_TRY(radix_sort<K>(cs.stream, ds, ar, keys, true, g.bits, nullptr, false, n, 0, g.bits, &sorted, nullptr));
+  GQF_TRY(radix_sort<K>(cs.stream, ar, keys, true, g.bits, nullptr, false, n, 0, g.bits, &sorted, nullptr));
   return rsqf_rebuild<K, KEEP_ALL, false>(cs.stream, ds, ar, f, sorted, n, nullptr, (const K*)nullptr, 0u, nullptr);
 }
 
@@ -1123,7 +1123,7 @@ static gqf_status rsqf_insert_t(const gqf_rsqf& f, uint32_t n, const uint32_t* k
   const uint32_t* hb = nullptr;
   GQF_TRY(rsqf_decode<K>(cs.stream, ar, f, &A, &m_dev, &hb));
   K* B;
-  GQF_TRY(radix_sort<K>(cs.stream, ds, ar, keys, true, g.bits, nullptr, false, n, 0, g.bits, &B, nullptr));
+  GQF_TRY(radix_sort<K>(cs.stream, ar, keys, true, g.bits, nullptr, false, n, 0, g.bits, &B, nullptr));
   // at most cap stored entries: the grids are sized for cap + n
   return rsqf_rebuild<K, KEEP_ALL, true>(cs.stream, ds, ar, f, A, (uint32_t)g.cap, m_dev, B, n, nullptr, hb);
 }
@@ -1145,7 +1145,7 @@ static gqf_status rsqf_delete_t(const gqf_rsqf& f, uint32_t n, const uint32_t* k
   const uint32_t* hb = nullptr;
   GQF_TRY(rsqf_decode<K>(cs.stream, ar, f, &A, &m_dev, &hb));
   K* D;
-  GQF_TRY(radix_sort<K>(cs.stream, ds, ar, keys, true, g.bits, nullptr, false, n, 0, g.bits, &D, nullptr));
+  GQF_TRY(radix_sort<K>(cs.stream, ar, keys, true, g.bits, nullptr, false, n, 0, g.bits, &D, nullptr));
   const uint32_t mcap = (uint32_t)g.cap;  // the grids cover the capacity; kernels read the count
   uint8_t* keep = ar.take<uint8_t>((size_t)mcap + 1);
   if (ar.used > ar.cap) {
@@ -1183,7 +1183,7 @@ static gqf_status rsqf_lookup_sorted_t(const gqf_rsqf& f, uint32_t n, const uint
   // order by the top 16 fingerprint bits (quotient prefix): blocks are then
   // visited in increasing order
   const uint32_t lo = g.bits > 16 ? g.bits - 16 : 0;
-  GQF_TRY(radix_sort<K>(cs.stream, ds, ar, keys, true, g.bits, nullptr, true, n, lo, g.bits, &fps, &idx));
+  GQF_TRY(radix_sort<K>(cs.stream, ar, keys, true, g.bits, nullptr, true, n, lo, g.bits, &fps, &idx));
   const RsqfDev d = rdev(f);
   const dim3 grid((n + 255) / 256);
   if (f.rbits == 8)

@@ -14,7 +14,6 @@
 #include "gqf_sqf_dev.hpp"
 
 #include <algorithm>
-
 #include <cstring>
 
 namespace gqf {
@@ -96,12 +95,12 @@ static size_t rebuild_ws_bytes(uint64_t N) {
 // na_dev (may be NULL): A's count on the device; na is then an upper bound
 // that sizes the grids, and the sparse-table choice is made on the device.
 template <int MODE, bool MERGE>
-static gqf_status launch_rebuild(hipStream_t s, DeviceState& ds, Arena& ar, const gqf_sqf& qf, const SqfGeom& g,
+static gqf_status launch_rebuild(hipStream_t s, uint32_t* counters, Arena& ar, const gqf_sqf& qf, const SqfGeom& g,
                                  const uint32_t* A, uint32_t na, const uint32_t* na_dev, const uint32_t* B,
                                  uint32_t nb, const uint8_t* flags) {
   const uint64_t N = (uint64_t)na + (MERGE ? nb : 0u);
   if (N == 0) {
-    GQF_LAUNCH(k_reset_counters, dim3(1), dim3(64), 0, s, ds.counters, (int)CTR_OP, (int)CTR_TOTAL);
+    GQF_LAUNCH(k_reset_counters, dim3(1), dim3(64), 0, s, counters, (int)CTR_OP, (int)CTR_TOTAL);
     clear_table(s, qf, g);
     return GQF_OK;
   }
@@ -121,29 +120,29 @@ static gqf_status launch_rebuild(hipStream_t s, DeviceState& ds, Arena& ar, cons
     GQF_LAUNCH((k_rb_splits<uint32_t, false>), dim3((p.nsub + 1 + 255) / 256), dim3(256), 0, s, A, na, na_dev, B, nb,
                p.nsub, splits, (const uint32_t*)nullptr);
   GQF_LAUNCH_BC(nknown * 4, na_dev, 4, (k_rb_agg<MODE, MERGE>), dim3(p.G), dim3(RB_THREADS), 0, s, A, na, na_dev, B,
-                nb, flags, splits, g.q, g.r, p.chunk, agg, ds.counters);
-  GQF_TRY(rb_scan(s, ar, agg, p.G, (uint64_t)g.S, ds.counters, nullptr));
+                nb, flags, splits, g.q, g.r, p.chunk, agg, counters);
+  GQF_TRY(rb_scan(s, ar, agg, p.G, (uint64_t)g.S, counters, nullptr));
   // sparse tables (N * 8 < S): clear once, then sub-tiles skip windows holding
   // only zeros; otherwise sub-tiles write every slot of their range and the
   // tail is cleared after.  With the count on the device, the kernels decide.
   const int precleared = na_dev ? -1 : (N * 8 < (uint64_t)g.S ? 1 : 0);
   const int pre_mode = na_dev ? 2 : 1, post_mode = na_dev ? 3 : 0;
   if (precleared != 0)
-    GQF_LAUNCH(k_sqf_clear_tail, dim3(1024), dim3(256), 0, s, qf.table, g.bpe, g.bytes, ds.counters, pre_mode, na_dev,
+    GQF_LAUNCH(k_sqf_clear_tail, dim3(1024), dim3(256), 0, s, qf.table, g.bpe, g.bytes, counters, pre_mode, na_dev,
                nadd, g.S);
 #define GQF_RB_WRITE(BPE_)                                                                                          \
   GQF_LAUNCH_BC(nknown * 4 + (double)g.bytes, na_dev, 4, (k_rb_write<MODE, MERGE, BPE_>), dim3(p.G),               \
                 dim3(RB_THREADS), 0, s, qf.table, g.S, g.q, g.r, A, na, na_dev, B, nb, flags, splits, p.chunk, agg, \
-                ds.counters, fix, fix_cnt, fix_more, ds.counters + CTR_MISC + 2, (uint32_t)(N + 1), precleared)
+                counters, fix, fix_cnt, fix_more, counters + CTR_MISC + 2, (uint32_t)(N + 1), precleared)
   if (g.bpe == 1) GQF_RB_WRITE(1);
   else if (g.bpe == 2) GQF_RB_WRITE(2);
   else if (g.bpe == 3) GQF_RB_WRITE(3);
   else GQF_RB_WRITE(4);
 #undef GQF_RB_WRITE
   if (precleared != 1)
-    GQF_LAUNCH(k_sqf_clear_tail, dim3(1024), dim3(256), 0, s, qf.table, g.bpe, g.bytes, ds.counters, post_mode, na_dev,
+    GQF_LAUNCH(k_sqf_clear_tail, dim3(1024), dim3(256), 0, s, qf.table, g.bpe, g.bytes, counters, post_mode, na_dev,
                nadd, g.S);
-  GQF_LAUNCH(k_sqf_fixup, dim3(p.nsub + 1), dim3(64), 0, s, qf.table, g.bpe, fix, fix_cnt, fix_more, ds.counters,
+  GQF_LAUNCH(k_sqf_fixup, dim3(p.nsub + 1), dim3(64), 0, s, qf.table, g.bpe, fix, fix_cnt, fix_more, counters,
              p.nsub, (uint32_t)(N + 1), na_dev, nadd);
   return GQF_OK;
 }
@@ -156,6 +155,20 @@ static gqf_status check_keys(int n, const void* keys) {
   return GQF_OK;
 }
 
+// The build's kernels: hash + sort the keys, then the canonical layout
+// rebuild, over a workspace arena and the device counters.  A layout that
+// does not fit raises the capacity flag in counters[CTR_ERR] before any
+// byte of the table is written.
+static size_t sqf_build_ws_bytes(uint32_t n) { return sort_workspace_bytes(n, 4, false) + rebuild_ws_bytes(n); }
+static gqf_status sqf_build_launch(hipStream_t s, Arena& ar, uint32_t* counters, const gqf_sqf& qf, const SqfGeom& g,
+                                  uint32_t un, const uint32_t* d_keys, int no_dup) {
+  uint32_t* sorted = nullptr;
+  GQF_TRY(radix_sort<uint32_t>(s, ar, d_keys, true, (g.q + g.r) | sqf_hash_flag(), nullptr, false, un, 0, g.q + g.r,
+                               &sorted, nullptr));
+  if (no_dup) return launch_rebuild<KEEP_UNIQUE, false>(s, counters, ar, qf, g, sorted, un, nullptr, nullptr, 0, nullptr);
+  return launch_rebuild<KEEP_ALL, false>(s, counters, ar, qf, g, sorted, un, nullptr, nullptr, 0, nullptr);
+}
+
 static gqf_status sqf_build_impl(const gqf_sqf& qf, int n, const uint32_t* d_keys, int no_dup, gqf_stream stream,
                                  float* ms) {
   GQF_TRY(sqf_check(qf));
@@ -165,13 +178,9 @@ static gqf_status sqf_build_impl(const gqf_sqf& qf, int n, const uint32_t* d_key
   GQF_TRY(cs.begin(stream));
   DeviceState& ds = *cs.ds;
   const uint32_t un = (uint32_t)n;
-  GQF_TRY(ws_reserve(ds, sort_workspace_bytes(un, 4, false) + rebuild_ws_bytes(un)));
+  GQF_TRY(ws_reserve(ds, sqf_build_ws_bytes(un)));
   Arena ar(ds.ws, ds.ws_cap);
-  uint32_t* sorted = nullptr;
-  GQF_TRY(radix_sort<uint32_t>(cs.stream, ds, ar, d_keys, true, (g.q + g.r) | sqf_hash_flag(), nullptr, false, un, 0, g.q + g.r, &sorted,
-                               nullptr));
-  if (no_dup) GQF_TRY((launch_rebuild<KEEP_UNIQUE, false>(cs.stream, ds, ar, qf, g, sorted, un, nullptr, nullptr, 0, nullptr)));
-  else GQF_TRY((launch_rebuild<KEEP_ALL, false>(cs.stream, ds, ar, qf, g, sorted, un, nullptr, nullptr, 0, nullptr)));
+  GQF_TRY(sqf_build_launch(cs.stream, ar, ds.counters, qf, g, un, d_keys, no_dup));
   return cs.end(ms, true);  // a layout that does not fit is detected before any write
 }
 
@@ -179,7 +188,7 @@ static gqf_status sqf_build_impl(const gqf_sqf& qf, int n, const uint32_t* d_key
 template <int MODE>
 static gqf_status sqf_merge_into(CallScope& cs, Arena& ar, const gqf_sqf& qf, const SqfGeom& g, const uint32_t* A,
                                  uint32_t m, const uint32_t* m_dev, const uint32_t* B, uint32_t k, float* ms) {
-  GQF_TRY((launch_rebuild<MODE, true>(cs.stream, *cs.ds, ar, qf, g, A, m, m_dev, B, k, nullptr)));
+  GQF_TRY((launch_rebuild<MODE, true>(cs.stream, cs.ds->counters, ar, qf, g, A, m, m_dev, B, k, nullptr)));
   return cs.end(ms, true);  // a layout that does not fit is detected before any write
 }
 
@@ -214,7 +223,7 @@ static gqf_status sqf_insert_local(CallScope& cs, Arena& ar, const gqf_sqf& qf, 
   uint4* agg = ar.take<uint4>(G);
   uint32_t* sctr = ar.take<uint32_t>(CTR_TOTAL);
   uint32_t* flags = ar.take<uint32_t>(4);
-  uint32_t* scratch = ar.take<uint32_t>(g.S);
+  uint32_t* region_tmp = ar.take<uint32_t>(g.S);
   if (ar.used > ar.cap) {
     set_error("internal: localized insert workspace overflow");
     return GQF_ERR_INVALID;
@@ -232,7 +241,7 @@ static gqf_status sqf_insert_local(CallScope& cs, Arena& ar, const gqf_sqf& qf, 
   GQF_HIP(hipMemcpyAsync(ds.h_pinned, flags, 4, hipMemcpyDeviceToHost, s));
   GQF_HIP(hipStreamSynchronize(s));
   if (ds.h_pinned[0] != 0) return GQF_OK;  // the full rebuild takes over; nothing was written
-  GQF_LAUNCH(k_pi_write<BPE>, kg, dim3(256), 0, s, qf.table, g.S, g.r, B, nb, s0, P, head, rend, scratch, flags);
+  GQF_LAUNCH(k_pi_write<BPE>, kg, dim3(256), 0, s, qf.table, g.S, g.r, B, nb, s0, P, head, rend, region_tmp, flags);
   *done = true;
   return GQF_OK;
 }
@@ -253,7 +262,7 @@ static gqf_status sqf_insert_impl(const gqf_sqf& qf, int n, const uint32_t* d_ke
   GQF_TRY(ws_reserve(ds, sort_workspace_bytes(un, 4, false) + std::max(full, local ? sqf_local_bytes(g, un) : 0)));
   Arena ar(ds.ws, ds.ws_cap);
   uint32_t* B = nullptr;
-  GQF_TRY(radix_sort<uint32_t>(cs.stream, ds, ar, d_keys, true, (g.q + g.r) | sqf_hash_flag(), nullptr, false, un, 0, g.q + g.r, &B,
+  GQF_TRY(radix_sort<uint32_t>(cs.stream, ar, d_keys, true, (g.q + g.r) | sqf_hash_flag(), nullptr, false, un, 0, g.q + g.r, &B,
                                nullptr));
   if (local) {
     const size_t mark = ar.used;
@@ -287,7 +296,7 @@ static gqf_status sqf_delete_impl(const gqf_sqf& qf, int n, const uint32_t* d_ke
                              rebuild_ws_bytes(g.S)));
   Arena ar(ds.ws, ds.ws_cap);
   uint32_t* D = nullptr;
-  GQF_TRY(radix_sort<uint32_t>(cs.stream, ds, ar, d_keys, true, (g.q + g.r) | sqf_hash_flag(), nullptr, false, un, 0, g.q + g.r, &D,
+  GQF_TRY(radix_sort<uint32_t>(cs.stream, ar, d_keys, true, (g.q + g.r) | sqf_hash_flag(), nullptr, false, un, 0, g.q + g.r, &D,
                                nullptr));
   uint32_t* A;
   const uint32_t* m_dev = nullptr;
@@ -299,7 +308,7 @@ static gqf_status sqf_delete_impl(const gqf_sqf& qf, int n, const uint32_t* d_ke
   }
   GQF_LAUNCH_BC(0.0, m_dev, 5, (k_delete_mark<uint32_t, false>), dim3((g.S + DM_TILE - 1) / DM_TILE), dim3(DM_THREADS),
                 0, cs.stream, A, g.S, m_dev, D, un, keep, (const uint32_t*)nullptr);
-  GQF_TRY((launch_rebuild<KEEP_FLAGS, false>(cs.stream, ds, ar, qf, g, A, g.S, m_dev, nullptr, 0, keep)));
+  GQF_TRY((launch_rebuild<KEEP_FLAGS, false>(cs.stream, ds.counters, ar, qf, g, A, g.S, m_dev, nullptr, 0, keep)));
   return cs.end(ms, true);  // deleting only shrinks the layout: it always fits
 }
 
@@ -392,7 +401,7 @@ gqf_status gqf_sqf_lookup(gqf_sqf qf, int sorted, int n, const uint32_t* d_keys,
       // order by the top 16 fingerprint bits: lookups then walk the table in order
       const uint32_t bits = g.q + g.r;
       const uint32_t lo = bits > 16 ? bits - 16 : 0;
-      GQF_TRY(radix_sort<uint32_t>(cs.stream, ds, ar, d_keys, true, bits | sqf_hash_flag(), nullptr, true, un, lo, bits, &fps, &idx));
+      GQF_TRY(radix_sort<uint32_t>(cs.stream, ar, d_keys, true, bits | sqf_hash_flag(), nullptr, true, un, lo, bits, &fps, &idx));
       const dim3 grid((un + 255) / 256);
       const double b = (double)un * (12 + 64);
       if (g.bpe == 1 && win) GQF_LAUNCH_B(b, (k_sqf_lookup_sorted<1, true>), grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, fps, idx, un, d_out);

@@ -389,8 +389,11 @@ __device__ __forceinline__ void rb_load(const uint32_t* __restrict__ A, const ui
   {
     const uint64_t g0 = (uint64_t)d0 + i0;
     uint32_t w[RB_ITEMS];
-    if (i0 + RB_ITEMS <= tn) {
-      // d0 is a multiple of RB_TILE and A is 256-byte aligned: 32-byte aligned runs
+    // 16-byte loads need A + g0 16-byte aligned: d0 and i0 are multiples of
+    // 8 entries, so that holds for every thread when it holds for A + d0
+    // (plain tables: A is 256-byte aligned; batches: a filter's segment of
+    // the batch key array may start at any entry)
+    if (i0 + RB_ITEMS <= tn && (reinterpret_cast<uintptr_t>(A + d0) & 15u) == 0) {
       const uint4* p = reinterpret_cast<const uint4*>(A + g0);
 #pragma unroll
       for (int k = 0; k < RB_ITEMS / 4; ++k) {

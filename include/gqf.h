@@ -178,7 +178,11 @@ gqf_status gqf_rsqf_insert(gqf_rsqf cqf, int n, const uint32_t* d_keys, int32_t*
 
 /* launchUnsortedLookups (RSQF.cu:540-563) when sorted == 0, launchLookups
  * (RSQF.cu:411-456) when sorted != 0: the LAST matching slot of the run,
- * or -1. */
+ * or -1.  The walk is the reference's and reads the 8-bit block offsets,
+ * which the format stores mod 256 (RSQF.cu offset++ on a uchar): in the
+ * blocks behind a run that pushes an offset past 255 a lookup can land on
+ * the wrong run and return -1 for a stored key, exactly as the reference
+ * does. */
 gqf_status gqf_rsqf_lookup(gqf_rsqf cqf, int sorted, int n, const uint32_t* d_keys,
                            int32_t* d_out, gqf_stream stream, float* ms);
 
@@ -219,8 +223,11 @@ gqf_status gqf_rsqf_load(gqf_rsqf cqf, const void* host_ref_layout);
  *                      few slots fewer than the GPU table, saving a table that
  *                      uses them fails with GQF_ERR_CAPACITY;
  *   GQF_IMAGE_RSQF     packed reference blocks (RSQF.cuh:24-30).
- * An SQF image records the calling thread's SQF hash at save time.  Loading
- * checks magic, version, geometry and checksum before touching the device. */
+ * An SQF image records the calling thread's SQF hash at save time; a load on
+ * a thread whose hash differs fails with GQF_ERR_INVALID (the table would
+ * answer that thread's queries with misses).  RSQF images record AP.  Loading
+ * checks magic, version, geometry, hash and checksum before touching the
+ * device. */
 enum gqf_image_layout { GQF_IMAGE_SQF_GPU = 1, GQF_IMAGE_SQF_CPU = 2, GQF_IMAGE_RSQF = 3 };
 /* Image size in bytes (0 for an unknown layout). */
 size_t gqf_image_bytes(int layout, unsigned int q, unsigned int r);
@@ -273,6 +280,11 @@ gqf_status gqf_sqf_batch_build(gqf_sqf_batch b, int use_graph, int* filter_statu
  * through a long cluster finishes by binary searches; results equal the
  * per-filter walk, also after a filter was changed since the batch build. */
 gqf_status gqf_sqf_batch_lookup(gqf_sqf_batch b, int use_graph, float* ms);
+/* The same lookups in one launch, every walk run to its end through the
+ * table (no window index): the per-filter calls' algorithm, batched.  The
+ * results are the same; the time is the like-for-like comparison with the
+ * per-filter calls. */
+gqf_status gqf_sqf_batch_lookup_walk(gqf_sqf_batch b, float* ms);
 gqf_status gqf_sqf_batch_info(gqf_sqf_batch b, gqf_batch_info* info);
 gqf_status gqf_sqf_batch_destroy(gqf_sqf_batch b);
 
@@ -382,6 +394,13 @@ gqf_status gqf_serve_run(int nfilters, size_t resident_bytes, gqf_request* reqs,
  * thread, as a lookup's key and answer); best device time of reps timed runs
  * after one warm-up.  Lookups are measured against n / best_ms sectors/s. */
 gqf_status gqf_gather_bench(size_t buffer_bytes, uint32_t n, int reps, float* best_ms);
+
+/* Streaming peak (no reference counterpart): best device times of a read, a
+ * copy and a write pass over buffers of `bytes` each (two are allocated),
+ * after one warm-up round; 16-byte loads and stores, each workgroup over one
+ * contiguous span.  Sized beyond the 256 MB Infinity Cache, the passes
+ * measure HBM: read bytes / read_ms, 2 * bytes / copy_ms, bytes / write_ms. */
+gqf_status gqf_stream_bench(size_t bytes, int reps, float* read_ms, float* copy_ms, float* write_ms);
 
 /* ============================================================ kernel timing
  * Per-kernel device time measured with HIP events on the launch stream (two

@@ -82,7 +82,8 @@ _lib = None
 def lib():
     global _lib
     if _lib is None:
-        path = os.path.join(HERE, "liboracle.so")
+        # GQF_ORACLE_LIB: another build of the same source (the UBSan test)
+        path = os.environ.get("GQF_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
         if not os.path.exists(path):
             build()
         _lib = C.CDLL(path)

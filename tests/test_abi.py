@@ -74,3 +74,23 @@ def test_sqf_hash_selector():
     with __import__("pytest").raises(G.GQFError):
         G.set_sqf_hash(7)
     assert G.sqf_hash() == G.HASH_AP
+
+
+def test_library_key_stream_is_the_reference_stream():
+    """gqf_mt19937_fill generates every bench input: it must reproduce the
+    reference drivers' init_genrand + genrand_int32 stream (golden vectors
+    written by the compiled mt19937ar.cpp), also after skipping outputs."""
+    import numpy as np
+
+    import gpuquotientfilters_amd as G
+    from conftest import golden
+
+    g = golden("mt19937.npz")
+    assert (G.mt19937(1, 4096) == g["seed1"]).all()
+    assert (G.mt19937(7, 1024) == g["seed7"]).all()
+    assert (G.mt19937(1, 1000, skip=3000) == g["seed1"][3000:4000]).all()
+    assert (G.mt19937(7, 1, skip=1023) == g["seed7"][1023:]).all()
+    # C++ standard known answer: the 10000th output of mt19937 from seed 5489
+    assert int(G.mt19937(5489, 1, skip=9999)[0]) == 4123659995
+    assert G.mt19937(3, 0).shape == (0,)
+    assert np.array_equal(G.mt19937(3, 5), G.mt19937(3, 7)[:5])

@@ -179,7 +179,17 @@ def test_long_runs_across_windows(q, r):
     base = np.concatenate([O.mt19937(21, 6000), np.full(5000, hot, np.uint32)])
     f = G.RSQF(q, r)
     f.bulk_build(dev(base))
-    assert (f.dump() == canon(base, q, r)).all()
+    t = f.dump()
+    assert (t == canon(base, q, r)).all()
+    # lookups follow the reference walk, which reads the wrapped offsets: in
+    # the blocks behind the long run it can land on the wrong run and miss a
+    # stored key.  The GPU answers must be the reference walk's, misses
+    # included (the oracle's lookup restates RSQF.cu:320-399 and is pinned to
+    # the reference's own tables).
+    probe = np.concatenate([O.mt19937(21, 6000), [hot], fresh_keys(23, 3000, base)]).astype(np.uint32)
+    want = O.rsqf_lookup_keys(t, q, r, probe)
+    assert (rsqf_lookup(f, probe) == want).all()
+    assert (rsqf_lookup(f, probe, sorted=True) == want).all()
     extra = np.concatenate([O.mt19937(22, 1500), np.full(300, hot, np.uint32)])
     f.insertGPU(dev(extra))
     allk = np.concatenate([base, extra])

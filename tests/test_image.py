@@ -53,3 +53,20 @@ def test_bad_images_are_refused():
     other = make_image(G.IMAGE_SQF_GPU, q + 1, r, bytes(O.sqf_table_bytes(q + 1, r)))
     b = np.frombuffer(other, dtype=np.uint8)
     assert G.lib().gqf_sqf_image_load(fake, b.ctypes.data, b.size) == G.ERR_INVALID  # q mismatch
+
+
+def test_hash_mismatch_is_refused_before_any_device_call():
+    """An image records the SQF hash its table was built with; loading it on a
+    thread that queries with the other hash would turn every stored key into a
+    miss, so the load fails (checked before the device is touched)."""
+    q, r = 6, 5
+    fake = G._SQF(q, r, 1, C.c_void_p(0x1000))
+    fnv = np.frombuffer(make_image(G.IMAGE_SQF_GPU, q, r, bytes(O.sqf_table_bytes(q, r)), hash_kind=G.HASH_FNV),
+                        dtype=np.uint8)
+    assert G.sqf_hash() == G.HASH_AP
+    assert G.lib().gqf_sqf_image_load(fake, fnv.ctypes.data, fnv.size) == G.ERR_INVALID
+    assert "hash" in G.lib().gqf_last_error().decode()
+    fake_r = G._RSQF(q, G.RSQF.calcNumBlocksGPU(q), C.c_void_p(0x1000), r)
+    rs = np.frombuffer(make_image(G.IMAGE_RSQF, q, r, bytes(G.lib().gqf_rsqf_ref_bytes(q, r)), hash_kind=G.HASH_FNV,
+                                  units=G.RSQF.calcNumBlocksGPU(q)), dtype=np.uint8)
+    assert G.lib().gqf_rsqf_image_load(fake_r, rs.ctypes.data, rs.size) == G.ERR_INVALID
