@@ -26,6 +26,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libgqf.so")
 
 NOT_FOUND = 0xFFFFFFFF
+NOALLOC_MAX_KEYS = 1 << 21  # GQF_NOALLOC_MAX_KEYS
 OK, ERR_INVALID, ERR_CAPACITY, ERR_HIP, ERR_NOMEM, ERR_UNSUPPORTED = range(6)
 BUILD_SEGMENTED_LAYOUTS, BUILD_PARALLEL_MERGING, BUILD_SEQUENTIAL_SHIFTS = 0, 1, 2
 
@@ -157,6 +158,7 @@ _SIGS = {
     "gqf_sqf_init": (_st, [C.POINTER(_SQF), C.c_uint, C.c_uint]),
     "gqf_sqf_destroy": (_st, [C.POINTER(_SQF)]),
     "gqf_sqf_bulk_build": (_st, [_SQF, C.c_int, C.c_int, _vp, C.c_int, _vp, _f]),
+    "gqf_sqf_bulk_build_noalloc": (_st, [_SQF, C.c_int, C.c_int, _vp, C.c_int, _vp, _f]),
     "gqf_sqf_insert": (_st, [_SQF, C.c_int, _vp, _vp, _f]),
     "gqf_sqf_insert_via_merge": (_st, [_SQF, _vp, C.c_int, _vp, C.c_int, C.c_int, _vp, _f]),
     "gqf_sqf_lookup": (_st, [_SQF, C.c_int, C.c_int, _vp, _vp, _vp, _f]),

@@ -14,7 +14,9 @@
 #include "gqf_sqf_dev.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cstring>
+#include <mutex>
 
 namespace gqf {
 
@@ -184,6 +186,36 @@ static gqf_status sqf_build_impl(const gqf_sqf& qf, int n, const uint32_t* d_key
   return cs.end(ms, true);  // a layout that does not fit is detected before any write
 }
 
+// ---------------------------------------------------------------- build without allocation
+// gqf_sqf_bulk_build_noalloc runs the build's kernels over `scratch`, a
+// fixed device array of the library (each GPU holds its own copy from the
+// code object's load until the process ends): the counters in its first 256
+// bytes, the sort buffers and rebuild states after them.  On the host the
+// array's name is only a stand-in for the symbol: the device address comes
+// from hipGetSymbolAddress, once per device, and only that address is handed
+// to kernels.
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)32 << 20;
+constexpr size_t NOALLOC_COUNTER_BYTES = 256;
+static_assert(CTR_TOTAL * sizeof(uint32_t) <= NOALLOC_COUNTER_BYTES, "counters fit their reserved bytes");
+static __device__ __align__(256) uint8_t scratch[NOALLOC_SCRATCH_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+struct NoallocDevice {
+  std::mutex mu;         // one call at a time per device: calls share the array
+  void* ptr = nullptr;   // device address of scratch on this device
+};
+static NoallocDevice g_noalloc[NOALLOC_MAX_DEVICES];
+
+static gqf_status noalloc_device(NoallocDevice** out) {
+  int dev = 0;
+  GQF_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) {
+    set_error("device %d is beyond the %d devices the no-allocation build supports", dev, NOALLOC_MAX_DEVICES);
+    return GQF_ERR_UNSUPPORTED;
+  }
+  *out = &g_noalloc[dev];
+  return GQF_OK;
+}
 // insert / insertViaMerge / merge: stored entries A merged with sorted B.
 template <int MODE>
 static gqf_status sqf_merge_into(CallScope& cs, Arena& ar, const gqf_sqf& qf, const SqfGeom& g, const uint32_t* A,
@@ -353,6 +385,53 @@ gqf_status gqf_sqf_bulk_build(gqf_sqf qf, int method, int n, const uint32_t* d_k
     return GQF_ERR_INVALID;
   }
   return sqf_build_impl(qf, n, d_keys, no_dup, stream, ms);
+}
+
+gqf_status gqf_sqf_bulk_build_noalloc(gqf_sqf qf, int method, int n, const uint32_t* d_keys, int no_dup,
+                                      gqf_stream stream, float* ms) {
+  if (method < GQF_BUILD_SEGMENTED_LAYOUTS || method > GQF_BUILD_SEQUENTIAL_SHIFTS) {
+    set_error("unknown build method %d", method);
+    return GQF_ERR_INVALID;
+  }
+  GQF_TRY(sqf_check(qf));
+  GQF_TRY(check_keys(n, d_keys));
+  if ((uint64_t)n > GQF_NOALLOC_MAX_KEYS) {
+    set_error("gqf_sqf_bulk_build_noalloc takes at most %u keys (n=%d); use gqf_sqf_bulk_build",
+              (unsigned)GQF_NOALLOC_MAX_KEYS, n);
+    return GQF_ERR_INVALID;
+  }
+  const uint32_t un = (uint32_t)n;
+  if (NOALLOC_COUNTER_BYTES + sqf_build_ws_bytes(un) > NOALLOC_SCRATCH_BYTES) {
+    set_error("internal: the no-allocation scratch (%zu bytes) is too small for %u keys", NOALLOC_SCRATCH_BYTES, un);
+    return GQF_ERR_INVALID;
+  }
+  const SqfGeom g = geom(qf);
+  NoallocDevice* nd = nullptr;
+  GQF_TRY(noalloc_device(&nd));
+  std::lock_guard<std::mutex> lk(nd->mu);
+  if (!nd->ptr) {
+    void* p = nullptr;
+    GQF_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    nd->ptr = p;
+  }
+  const auto t0 = std::chrono::steady_clock::now();
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  char* base = static_cast<char*>(nd->ptr);
+  uint32_t* counters = reinterpret_cast<uint32_t*>(base);
+  Arena ar(base + NOALLOC_COUNTER_BYTES, NOALLOC_SCRATCH_BYTES - NOALLOC_COUNTER_BYTES);
+  // the array keeps the previous call's contents: the counters start at zero
+  GQF_LAUNCH(k_reset_counters, dim3(1), dim3(64), 0, s, counters, 0, (int)CTR_TOTAL);
+  GQF_TRY(sqf_build_launch(s, ar, counters, qf, g, un, d_keys, no_dup));
+  GQF_HIP(hipGetLastError());
+  uint32_t err = 0;
+  GQF_HIP(hipMemcpyAsync(&err, counters + CTR_ERR, sizeof(err), hipMemcpyDeviceToHost, s));
+  GQF_HIP(hipStreamSynchronize(s));
+  if (ms) *ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  if (err & DEV_ERR_CAPACITY) {
+    set_error("capacity exceeded: the entries do not fit the table (table left unchanged)");
+    return GQF_ERR_CAPACITY;
+  }
+  return GQF_OK;
 }
 
 gqf_status gqf_sqf_insert(gqf_sqf qf, int n, const uint32_t* d_keys, gqf_stream stream, float* ms) {

@@ -111,6 +111,21 @@ gqf_status gqf_sqf_destroy(gqf_sqf* qf);
 gqf_status gqf_sqf_bulk_build(gqf_sqf qf, int method, int n, const uint32_t* d_keys, int no_dup,
                               gqf_stream stream, float* ms);
 
+/* No reference counterpart: gqf_sqf_bulk_build for callers that cannot
+ * allocate device memory (code under graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments and results (the same
+ * table, GQF_ERR_CAPACITY with the table unchanged); it calls no hipMalloc,
+ * hipFree or other allocating routine and needs no stream context: its
+ * sort buffers, rebuild states and counters live in a fixed 32 MiB device
+ * array of the library, one per GPU, held from the library's load until the
+ * process ends.  n <= GQF_NOALLOC_MAX_KEYS (larger n: GQF_ERR_INVALID, no
+ * device work).  Every call on a GPU shares that GPU's array: the library
+ * serialises calls per device (each blocks until its kernels finish), so
+ * they cannot overlap within a process.  *ms is the call's wall time. */
+#define GQF_NOALLOC_MAX_KEYS (1u << 21)
+gqf_status gqf_sqf_bulk_build_noalloc(gqf_sqf qf, int method, int n, const uint32_t* d_keys, int no_dup,
+                                      gqf_stream stream, float* ms);
+
 /* insert (quotientFilter.cu:533-634).  Deduplicating insert: a fingerprint
  * already stored is not added again. */
 gqf_status gqf_sqf_insert(gqf_sqf qf, int n, const uint32_t* d_keys, gqf_stream stream, float* ms);
