@@ -694,7 +694,11 @@ static gqf_status batch_create(SqfBatch* b, const gqf_sqf* filters, int count, i
   b->win = b->bpe <= 2;
   for (int f = 0; f < count; ++f) {
     const gqf_sqf& q = filters[f];
-    if (!q.table || q.rbits != b->r || q.bytesPerElement != b->bpe || (q.rbits + 3) % 8 != 0 || q.qbits < 1 ||
+    if (q.table && q.rbits >= 1 && (q.rbits + 3) % 8 != 0) {
+      set_error("filter %d: packed slot widths ((r+3)%%8 != 0) are not batched; use the per-filter calls", f);
+      return GQF_ERR_UNSUPPORTED;
+    }
+    if (!q.table || q.rbits != b->r || q.bytesPerElement != b->bpe || q.qbits < 1 ||
         q.qbits + q.rbits > 32 || q.qbits > 28) {
       set_error("filter %d: a batch needs initialised SQFs of one r (q + r <= 32)", f);
       return GQF_ERR_INVALID;

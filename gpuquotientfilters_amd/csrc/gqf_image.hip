@@ -6,7 +6,9 @@
 //   0  char[8] magic "GQFTABLE"
 //   8  u32 version (1)
 //  12  u32 layout   1 SQF GPU table   calcNumSlotsGPU bytes of big-endian
-//                                     (r+3)/8-byte elements (quotientFilter.cu:37-42, 151-170)
+//                                     (r+3)/8-byte elements (quotientFilter.cu:37-42, 151-170);
+//                                     for (r+3)%8 != 0 the device words of
+//                                     the packed table (gqf_sqf_table_bytes)
 //                   2 SQF CPU table   (r+3)-bit elements packed MSB-first
 //                                     into u32 words (quotientFilter.cpp:114-130)
 //                   3 RSQF blocks     packed {u8 offset; u64 occupieds;
@@ -75,13 +77,52 @@ size_t payload_bytes(int layout, unsigned q, unsigned r) {
   return 0;
 }
 
+bool is_packed(unsigned r) { return (r + 3) % 8 != 0; }
+
 gqf_status check_sqf_geometry(const gqf_sqf& qf) {
-  if (!qf.table || qf.qbits < 1 || qf.qbits > 28 || (qf.rbits + 3) % 8 != 0 || qf.qbits + qf.rbits > 32 ||
-      qf.bytesPerElement != (qf.rbits + 3) / 8) {
+  const unsigned r = qf.rbits;
+  const bool ok_bpe = is_packed(r) ? (r + 3 <= 16 && qf.bytesPerElement == 0) : qf.bytesPerElement == (r + 3) / 8;
+  if (!qf.table || qf.qbits < 1 || qf.qbits > 28 || r < 1 || qf.qbits + r > 32 || !ok_bpe) {
     set_error("not an initialised SQF");
     return GQF_ERR_INVALID;
   }
   return GQF_OK;
+}
+
+// Element of slot i of a GPU table held on the host: bpe big-endian bytes, or
+// (bpe == 0) (r+3)-bit fields packed MSB-first in the table's u32 words.
+uint32_t table_elem(const std::vector<uint8_t>& t, unsigned r, unsigned bpe, uint32_t i) {
+  if (bpe) {
+    uint32_t e = 0;
+    for (unsigned k = 0; k < bpe; ++k) e = (e << 8) | t[(size_t)i * bpe + k];
+    return e;
+  }
+  const unsigned w = r + 3;
+  const uint64_t bit = (uint64_t)i * w;
+  uint32_t e = 0;
+  for (unsigned k = 0; k < w; ++k) {
+    const uint64_t p = bit + k;
+    uint32_t word;
+    memcpy(&word, t.data() + 4 * (p / 32), 4);
+    e = (e << 1) | ((word >> (31 - p % 32)) & 1u);
+  }
+  return e;
+}
+void table_set(std::vector<uint8_t>& t, unsigned r, unsigned bpe, uint32_t i, uint32_t e) {
+  if (bpe) {
+    for (unsigned k = 0; k < bpe; ++k) t[(size_t)i * bpe + k] = (uint8_t)(e >> (8 * (bpe - 1 - k)));
+    return;
+  }
+  const unsigned w = r + 3;
+  const uint64_t bit = (uint64_t)i * w;
+  for (unsigned k = 0; k < w; ++k)
+    if ((e >> (w - 1 - k)) & 1u) {
+      const uint64_t p = bit + k;
+      uint32_t word;
+      memcpy(&word, t.data() + 4 * (p / 32), 4);
+      word |= 1u << (31 - p % 32);
+      memcpy(t.data() + 4 * (p / 32), &word, 4);
+    }
 }
 
 void write_header(uint8_t* h, int layout, unsigned q, unsigned r, int hash, const uint8_t* payload, size_t pbytes,
@@ -185,8 +226,7 @@ gqf_status gqf_sqf_image_save(gqf_sqf qf, int layout, void* image, size_t cap) {
   const uint64_t cap_slots = (uint64_t)nw * 32 / w;
   std::vector<uint32_t> words(nw, 0u);
   for (uint32_t i = 0; i < S; ++i) {
-    uint32_t e = 0;
-    for (unsigned k = 0; k < bpe; ++k) e = (e << 8) | t[(size_t)i * bpe + k];
+    const uint32_t e = table_elem(t, r, bpe, i);
     if (!e) continue;
     if (i >= cap_slots) {
       set_error("slot %u is used but the CPU layout holds %llu slots", i, (unsigned long long)cap_slots);
@@ -238,7 +278,7 @@ gqf_status gqf_sqf_image_load(gqf_sqf qf, const void* image, size_t bytes) {
       const uint64_t p = bit + k;
       e = (e << 1) | ((get32(payload + 4 * (p / 32)) >> (31 - p % 32)) & 1u);
     }
-    for (unsigned k = 0; k < bpe; ++k) t[(size_t)i * bpe + k] = (uint8_t)(e >> (8 * (bpe - 1 - k)));
+    table_set(t, r, bpe, i, e);
   }
   GQF_HIP(hipMemcpy(qf.table, t.data(), tb, hipMemcpyHostToDevice));
   return GQF_OK;

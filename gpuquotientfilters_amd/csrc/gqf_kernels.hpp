@@ -241,12 +241,10 @@ inline SortPlan sort_plan(uint32_t n) {
   return p;
 }
 
-// fp_out (HASH only, may be NULL): the fingerprints are also stored, so that
-// the pass's downsweep reads them instead of hashing again.
 template <typename K, bool HASH>
 __global__ __launch_bounds__(256) void k_rs_upsweep(const void* __restrict__ in, uint32_t n, uint32_t fp_bits,
                                                     uint32_t shift, uint32_t chunk, uint32_t G,
-                                                    uint32_t* __restrict__ hist, K* __restrict__ fp_out) {
+                                                    uint32_t* __restrict__ hist) {
   __shared__ uint32_t h[4 * 256];
   const int tid = threadIdx.x, wid = tid >> 6;
   for (int i = tid; i < 4 * 256; i += 256) h[i] = 0;
@@ -264,10 +262,7 @@ __global__ __launch_bounds__(256) void k_rs_upsweep(const void* __restrict__ in,
 #pragma unroll
     for (int i = 0; i < SORT_ITEMS; ++i) {
       const uint64_t idx = t0 + (uint64_t)i * 256 + tid;
-      if (idx < hi) {
-        atomicAdd(&h[wid * 256 + ((uint32_t)(k[i] >> shift) & 255u)], 1u);
-        if (HASH && fp_out) fp_out[idx] = k[i];
-      }
+      if (idx < hi) atomicAdd(&h[wid * 256 + ((uint32_t)(k[i] >> shift) & 255u)], 1u);
     }
   }
   __syncthreads();
@@ -726,20 +721,16 @@ gqf_status radix_sort(hipStream_t s, Arena& ar, const void* in, bool hash, uint3
   for (uint32_t p = 0; p < npass; ++p) {
     const uint32_t shift = lo_bit + 8 * p;
     bool first_hash = hash && p == 0;
-    if (first_hash && sizeof(K) == 4) {
-      // 32-bit fingerprints: hashed once, stored by the upsweep (into the
-      // buffer this pass's downsweep does not write) and read back by it
-      K* fps = kb[cur ^ 1];
-      GQF_LAUNCH_B((double)n * 8.0, (k_rs_upsweep<K, true>), dim3(plan.G), dim3(256), 0, s, src, n, fp_bits, shift,
-                   plan.chunk, plan.G, hist, fps);
-      src = fps;
-      first_hash = false;
-    } else if (first_hash) {
+    // the first pass hashes the raw keys twice, in its upsweep and in its
+    // downsweep: with the counting ranks the downsweep has ALU to spare, and
+    // that is faster than storing the fingerprints in the upsweep and reading
+    // them back (SQF build of 53.7 M keys: 1.17 vs 1.21 ms of kernels)
+    if (first_hash) {
       GQF_LAUNCH_B((double)n * 4.0, (k_rs_upsweep<K, true>), dim3(plan.G), dim3(256), 0, s, src, n, fp_bits, shift,
-                   plan.chunk, plan.G, hist, (K*)nullptr);
+                   plan.chunk, plan.G, hist);
     } else {
       GQF_LAUNCH_B((double)n * sizeof(K), (k_rs_upsweep<K, false>), dim3(plan.G), dim3(256), 0, s, src, n, fp_bits,
-                   shift, plan.chunk, plan.G, hist, (K*)nullptr);
+                   shift, plan.chunk, plan.G, hist);
     }
     const double kin = first_hash ? 4.0 : (double)sizeof(K);
     GQF_LAUNCH(k_rs_rowscan, dim3(256), dim3(1024), 0, s, hist, plan.G, dbase);

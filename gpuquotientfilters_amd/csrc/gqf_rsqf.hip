@@ -134,39 +134,73 @@ __device__ __forceinline__ uint32_t rsqf_lookup_one(const RsqfDev& f, uint32_t f
 // match (rsqf_lookup_one); the copies are the equal remainders ending there,
 // since remainders ascend inside a run.  A counting query over the multiset
 // the filter keeps (the reference has no counters, README.md:74).
-template <int R>
-__device__ __forceinline__ uint32_t rsqf_count_one(const RsqfDev& f, uint32_t fq, uint32_t fr) {
-  const uint32_t last = rsqf_lookup_one<R>(f, fq, fr);
-  if (last == RS_NOT_FOUND) return 0u;
-  uint32_t blockNum = last >> 6, slot = last & 63u, cnt = 1;
-  uint64_t ends = f.ends(blockNum);
-  for (;;) {  // step back while the previous slot is in the same run and equal
-    uint32_t pb = blockNum, ps = slot;
-    if (ps > 0) {
-      ps--;
-    } else {
-      if (pb == 0) break;
-      pb--;
-      ps = 63;
-    }
-    const uint64_t pends = pb == blockNum ? ends : f.ends(pb);
-    if (((pends >> ps) & 1ull) || (pb * 64 + ps) < fq) break;  // previous run, or before the quotient
-    if (f.get<R>(pb, ps) != fr) break;
-    cnt++;
-    blockNum = pb;
-    slot = ps;
-    ends = pends;
+// ---------------------------------------------------------------- counting query
+// Copies of a fingerprint, from run ranks rather than the 8-bit offsets: in a
+// valid RSQF the k-th run end (from slot 0) closes the run of the k-th
+// occupied quotient, so with P_occ / P_end, the per-block exclusive prefixes
+// of the occupied and run-end bits (k_rc_pop + a row scan), the run of fq is
+// found exactly however long the runs before it are:
+//   k      = P_occ[b] + #(occupied bits of block b up to fq),
+//   e      = slot of the k-th run end (binary search of P_end, then select),
+//   start  = max(fq, slot of the (k-1)-th run end + 1),
+// and the copies are the slots of [start, e] holding remainder fr.
+__global__ __launch_bounds__(256) void k_rc_pop(const uint64_t* __restrict__ meta, uint32_t nb,
+                                                uint32_t* __restrict__ cnt) {
+  const uint32_t b = blockIdx.x * 256u + threadIdx.x;
+  if (b >= nb) return;
+  cnt[2 * (size_t)b] = (uint32_t)__popcll(meta[4 * (size_t)b]);
+  cnt[2 * (size_t)b + 1] = (uint32_t)__popcll(meta[4 * (size_t)b + 1]);
+}
+
+// Slot of the k-th run end (k >= 1), searching blocks from b0 on; P: the
+// exclusive prefixes (occupied, run end) per block; tot_end: all run ends.
+__device__ __forceinline__ uint64_t rc_run_end(const RsqfDev& f, const uint32_t* __restrict__ P, uint32_t b0,
+                                               uint32_t k) {
+  uint32_t lo = b0, hi = f.nb;  // the block: the last with P_end < k
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (P[2 * (size_t)mid + 1] < k) lo = mid;
+    else hi = mid;
   }
-  return cnt;
+  return (uint64_t)lo * 64 + select_bit(f.ends(lo), k - P[2 * (size_t)lo + 1]);
 }
 
 template <int R>
-__global__ __launch_bounds__(256) void k_rsqf_count(RsqfDev f, uint32_t q, const uint32_t* __restrict__ keys,
-                                                    uint32_t n, uint32_t* __restrict__ out) {
+__device__ __forceinline__ uint32_t rsqf_count_exact(const RsqfDev& f, const uint32_t* __restrict__ P,
+                                                     uint32_t tot_end, uint32_t fq, uint32_t fr) {
+  const uint32_t b = fq >> 6, i = fq & 63u;
+  const uint64_t occ = f.occ(b);
+  if (!((occ >> i) & 1ull)) return 0u;
+  const uint32_t k = P[2 * (size_t)b] + (uint32_t)__popcll(occ & (i == 63 ? ~0ull : ((2ull << i) - 1ull)));
+  if (k > tot_end) return 0u;  // not a valid table: no run end for this quotient
+  const uint64_t e = rc_run_end(f, P, b, k);
+  uint64_t start = fq;
+  if (k > 1) {
+    // the previous run end lies before e: search from block 0 up to e's block
+    uint32_t lo = 0, hi = (uint32_t)(e >> 6) + 1;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (P[2 * (size_t)mid + 1] < k - 1) lo = mid;
+      else hi = mid;
+    }
+    const uint64_t pe = (uint64_t)lo * 64 + select_bit(f.ends(lo), k - 1 - P[2 * (size_t)lo + 1]);
+    if (pe + 1 > start) start = pe + 1;
+  }
+  uint32_t c = 0;
+  for (uint64_t s = start; s <= e; ++s) c += f.get<R>((uint32_t)(s >> 6), (uint32_t)(s & 63u)) == fr;
+  return c;
+}
+
+template <int R>
+__global__ __launch_bounds__(256) void k_rsqf_count(RsqfDev f, uint32_t q, const uint32_t* __restrict__ P,
+                                                    const uint32_t* __restrict__ totals,
+                                                    const uint32_t* __restrict__ keys, uint32_t n,
+                                                    uint32_t* __restrict__ out) {
   const uint64_t i = blockIdx.x * 256ull + threadIdx.x;
   if (i >= n) return;
   const uint64_t fp = fingerprint(keys[i], q + f.r);
-  out[i] = rsqf_count_one<R>(f, (uint32_t)((fp >> f.r) & mask64(q)), (uint32_t)(fp & ((1u << f.r) - 1u)));
+  out[i] = rsqf_count_exact<R>(f, P, totals[1], (uint32_t)((fp >> f.r) & mask64(q)),
+                               (uint32_t)(fp & ((1u << f.r) - 1u)));
 }
 
 template <int R>
@@ -176,16 +210,6 @@ __global__ __launch_bounds__(256) void k_rsqf_lookup(RsqfDev f, uint32_t q, cons
   if (i >= n) return;
   const uint64_t fp = fingerprint(keys[i], q + f.r);
   out[i] = rsqf_lookup_one<R>(f, (uint32_t)((fp >> f.r) & mask64(q)), (uint32_t)(fp & ((1u << f.r) - 1u)));
-}
-
-template <int R, typename K>
-__global__ __launch_bounds__(256) void k_rsqf_lookup_sorted(RsqfDev f, uint32_t q, const K* __restrict__ fps,
-                                                            const uint32_t* __restrict__ idx, uint32_t n,
-                                                            uint32_t* __restrict__ out) {
-  const uint64_t i = blockIdx.x * 256ull + threadIdx.x;
-  if (i >= n) return;
-  const uint64_t fp = (uint64_t)fps[i];
-  out[idx[i]] = rsqf_lookup_one<R>(f, (uint32_t)((fp >> f.r) & mask64(q)), (uint32_t)(fp & ((1u << f.r) - 1u)));
 }
 
 // ---------------------------------------------------------------- decode
@@ -1171,28 +1195,6 @@ static void launch_rlookup(hipStream_t s, const gqf_rsqf& f, uint32_t n, const u
   else GQF_LAUNCH_B((double)n * (8 + 128), k_rsqf_lookup<0>, grid, dim3(256), 0, s, d, f.qbits, keys, n, out);
 }
 
-template <typename K>
-static gqf_status rsqf_lookup_sorted_t(const gqf_rsqf& f, uint32_t n, const uint32_t* keys, uint32_t* out,
-                                       CallScope& cs) {
-  DeviceState& ds = *cs.ds;
-  const RsqfGeom g = rgeom(f);
-  GQF_TRY(ws_reserve(ds, sort_workspace_bytes(n, sizeof(K), true)));
-  Arena ar(ds.ws, ds.ws_cap);
-  K* fps;
-  uint32_t* idx;
-  // order by the top 16 fingerprint bits (quotient prefix): blocks are then
-  // visited in increasing order
-  const uint32_t lo = g.bits > 16 ? g.bits - 16 : 0;
-  GQF_TRY(radix_sort<K>(cs.stream, ar, keys, true, g.bits, nullptr, true, n, lo, g.bits, &fps, &idx));
-  const RsqfDev d = rdev(f);
-  const dim3 grid((n + 255) / 256);
-  if (f.rbits == 8)
-    GQF_LAUNCH_B((double)n * (sizeof(K) + 8 + 128), (k_rsqf_lookup_sorted<8, K>), grid, dim3(256), 0, cs.stream, d, f.qbits, fps, idx, n, out);
-  else
-    GQF_LAUNCH_B((double)n * (sizeof(K) + 8 + 128), (k_rsqf_lookup_sorted<0, K>), grid, dim3(256), 0, cs.stream, d, f.qbits, fps, idx, n, out);
-  return GQF_OK;
-}
-
 }  // namespace gqf
 
 using namespace gqf;
@@ -1267,9 +1269,10 @@ gqf_status gqf_rsqf_lookup(gqf_rsqf cqf, int sorted, int n, const uint32_t* d_ke
   GQF_TRY(cs.begin(stream));
   uint32_t* out = reinterpret_cast<uint32_t*>(d_out);
   if (n > 0) {
-    if (!sorted) launch_rlookup(cs.stream, cqf, (uint32_t)n, d_keys, out);
-    else if (cqf.qbits + cqf.rbits <= 32) GQF_TRY(rsqf_lookup_sorted_t<uint32_t>(cqf, (uint32_t)n, d_keys, out, cs));
-    else GQF_TRY(rsqf_lookup_sorted_t<uint64_t>(cqf, (uint32_t)n, d_keys, out, cs));
+    // sorted or not, the same per-key walk: sorting the keys first costs
+    // more than the locality it buys on MI355X (measured up to 1.6 GB tables)
+    (void)sorted;
+    launch_rlookup(cs.stream, cqf, (uint32_t)n, d_keys, out);
   }
   return cs.end(ms, false);
 }
@@ -1279,8 +1282,8 @@ size_t gqf_rsqf_workspace_bytes(gqf_rsqf cqf, int op, int n, int sorted) {
   const RsqfGeom g = rgeom(cqf);
   const bool wide = g.bits > 32;
   const uint32_t un = (uint32_t)n;
-  if (op == GQF_OP_LOOKUP)
-    return sorted && n > 0 ? sort_workspace_bytes(un, wide ? 8 : 4, true) : 0;
+  (void)sorted;
+  if (op == GQF_OP_LOOKUP) return 0;
   if (op == GQF_OP_INSERT) return wide ? rsqf_insert_ws<uint64_t>(g, un) : rsqf_insert_ws<uint32_t>(g, un);
   if (op == GQF_OP_DELETE) return wide ? rsqf_delete_ws<uint64_t>(g, un) : rsqf_delete_ws<uint32_t>(g, un);
   return 0;
@@ -1296,13 +1299,26 @@ gqf_status gqf_rsqf_count(gqf_rsqf cqf, int n, const uint32_t* d_keys, uint32_t*
   CallScope cs;
   GQF_TRY(cs.begin(stream));
   if (n > 0) {
+    DeviceState& ds = *cs.ds;
+    const uint32_t nb = cqf.numBlocks;
+    GQF_TRY(ws_reserve(ds, aligned_bytes((size_t)nb * 8) + 512 + scan_rows_bytes(nb, 2)));
+    Arena ar(ds.ws, ds.ws_cap);
+    uint32_t* P = ar.take<uint32_t>((size_t)nb * 2);
+    uint32_t* tot = ar.take<uint32_t>(2);
+    if (ar.used > ar.cap) {
+      set_error("internal: count workspace overflow");
+      return GQF_ERR_INVALID;
+    }
     const RsqfDev d = rdev(cqf);
+    GQF_LAUNCH_B((double)nb * 16 + (double)nb * 8, k_rc_pop, dim3((nb + 255) / 256), dim3(256), 0, cs.stream, d.meta,
+                 nb, P);
+    GQF_TRY(scan_rows<2>(cs.stream, ar, P, nb, tot));
     const dim3 grid(((uint32_t)n + 255) / 256);
     if (cqf.rbits == 8)
-      GQF_LAUNCH_B((double)n * (8 + 128), k_rsqf_count<8>, grid, dim3(256), 0, cs.stream, d, cqf.qbits, d_keys,
+      GQF_LAUNCH_B((double)n * (8 + 128), k_rsqf_count<8>, grid, dim3(256), 0, cs.stream, d, cqf.qbits, P, tot, d_keys,
                    (uint32_t)n, d_counts);
     else
-      GQF_LAUNCH_B((double)n * (8 + 128), k_rsqf_count<0>, grid, dim3(256), 0, cs.stream, d, cqf.qbits, d_keys,
+      GQF_LAUNCH_B((double)n * (8 + 128), k_rsqf_count<0>, grid, dim3(256), 0, cs.stream, d, cqf.qbits, P, tot, d_keys,
                    (uint32_t)n, d_counts);
   }
   return cs.end(ms, false);

@@ -21,18 +21,54 @@
 namespace gqf {
 
 // ---------------------------------------------------------------- pipelines
+// Packed slot widths ((r+3) % 8 != 0) run every pipeline on a wide copy of
+// the table in the workspace: 16-bit elements, the layout of a byte-aligned
+// 2-byte table (bpe and bytes below describe it), unpacked from the table
+// first where the operation reads it and packed back at the end.
 struct SqfGeom {
   uint32_t q, r, bpe, S;
   uint64_t bytes;
+  bool packed;
+  uint32_t pwords;  // packed widths: words of the caller's table
 };
 static SqfGeom geom(const gqf_sqf& qf) {
   SqfGeom g;
   g.q = qf.qbits;
   g.r = qf.rbits;
-  g.bpe = qf.bytesPerElement;
   g.S = sqf_slots(qf.qbits);
-  g.bytes = sqf_bytes(qf.qbits, qf.rbits);
+  g.packed = sqf_is_packed(qf.rbits);
+  g.bpe = g.packed ? 2u : qf.bytesPerElement;
+  g.bytes = g.packed ? 2ull * g.S : sqf_bytes(qf.qbits, qf.rbits);
+  g.pwords = g.packed ? (uint32_t)sqf_packed_words(qf.qbits, qf.rbits) : 0u;
   return g;
+}
+static size_t packed_ws_bytes(const SqfGeom& g) { return g.packed ? aligned_bytes(g.bytes) + 256 : 0; }
+
+// The table a pipeline works on: the caller's, or for packed widths a wide
+// copy in the workspace (unpacked from the table when `load`).
+static gqf_status work_table(hipStream_t s, Arena& ar, const gqf_sqf& qf, const SqfGeom& g, bool load,
+                             gqf_sqf* out) {
+  *out = qf;
+  if (!g.packed) return GQF_OK;
+  out->table = ar.take<uint8_t>(g.bytes);
+  out->bytesPerElement = 2;
+  if (ar.used > ar.cap) {
+    set_error("internal: wide-table workspace overflow");
+    return GQF_ERR_INVALID;
+  }
+  if (load)
+    GQF_LAUNCH_B((double)g.pwords * 4 + (double)g.bytes, k_sqf_unpack, dim3((g.S + 255) / 256), dim3(256), 0, s,
+                 reinterpret_cast<const uint32_t*>(qf.table), g.S, g.r + 3, reinterpret_cast<uint16_t*>(out->table));
+  return GQF_OK;
+}
+// Packs the wide copy back into the caller's table (packed widths only);
+// with ctr, not when the call raised the capacity flag.
+static void finish_table(hipStream_t s, const gqf_sqf& qf, const gqf_sqf& work, const SqfGeom& g,
+                         const uint32_t* ctr) {
+  if (!g.packed) return;
+  GQF_LAUNCH_B((double)g.bytes + (double)g.pwords * 4, k_sqf_pack, dim3((g.pwords + 255) / 256), dim3(256), 0, s,
+               reinterpret_cast<const uint16_t*>(work.table), g.S, g.r + 3, reinterpret_cast<uint32_t*>(qf.table),
+               g.pwords, ctr);
 }
 
 // Decodes the table into ascending fingerprints (at most S of them); their
@@ -180,9 +216,12 @@ static gqf_status sqf_build_impl(const gqf_sqf& qf, int n, const uint32_t* d_key
   GQF_TRY(cs.begin(stream));
   DeviceState& ds = *cs.ds;
   const uint32_t un = (uint32_t)n;
-  GQF_TRY(ws_reserve(ds, sqf_build_ws_bytes(un)));
+  GQF_TRY(ws_reserve(ds, sqf_build_ws_bytes(un) + packed_ws_bytes(g)));
   Arena ar(ds.ws, ds.ws_cap);
-  GQF_TRY(sqf_build_launch(cs.stream, ar, ds.counters, qf, g, un, d_keys, no_dup));
+  gqf_sqf wq;
+  GQF_TRY(work_table(cs.stream, ar, qf, g, false, &wq));
+  GQF_TRY(sqf_build_launch(cs.stream, ar, ds.counters, wq, g, un, d_keys, no_dup));
+  finish_table(cs.stream, qf, wq, g, ds.counters);
   return cs.end(ms, true);  // a layout that does not fit is detected before any write
 }
 
@@ -217,10 +256,13 @@ static gqf_status noalloc_device(NoallocDevice** out) {
   return GQF_OK;
 }
 // insert / insertViaMerge / merge: stored entries A merged with sorted B.
+// wq: the table worked on (the caller's qf, or its wide copy).
 template <int MODE>
-static gqf_status sqf_merge_into(CallScope& cs, Arena& ar, const gqf_sqf& qf, const SqfGeom& g, const uint32_t* A,
-                                 uint32_t m, const uint32_t* m_dev, const uint32_t* B, uint32_t k, float* ms) {
-  GQF_TRY((launch_rebuild<MODE, true>(cs.stream, cs.ds->counters, ar, qf, g, A, m, m_dev, B, k, nullptr)));
+static gqf_status sqf_merge_into(CallScope& cs, Arena& ar, const gqf_sqf& qf, const gqf_sqf& wq, const SqfGeom& g,
+                                 const uint32_t* A, uint32_t m, const uint32_t* m_dev, const uint32_t* B, uint32_t k,
+                                 float* ms) {
+  GQF_TRY((launch_rebuild<MODE, true>(cs.stream, cs.ds->counters, ar, wq, g, A, m, m_dev, B, k, nullptr)));
+  finish_table(cs.stream, qf, wq, g, cs.ds->counters);
   return cs.end(ms, true);  // a layout that does not fit is detected before any write
 }
 
@@ -288,9 +330,10 @@ static gqf_status sqf_insert_impl(const gqf_sqf& qf, int n, const uint32_t* d_ke
   DeviceState& ds = *cs.ds;
   const uint32_t un = (uint32_t)n;
   const int path = sqf_insert_path();
-  const bool local = mode == KEEP_TAGGED_INSERT && un > 0 && path != GQF_INSERT_REBUILD &&
+  // the localized path rewrites byte-aligned tables in place
+  const bool local = !g.packed && mode == KEEP_TAGGED_INSERT && un > 0 && path != GQF_INSERT_REBUILD &&
                      (path == GQF_INSERT_LOCAL || (un <= PI_AUTO_MAX && (uint64_t)un * PI_MIN_RATIO <= g.S));
-  const size_t full = sqf_decode_bytes(g) + rebuild_ws_bytes((uint64_t)g.S + un);
+  const size_t full = sqf_decode_bytes(g) + rebuild_ws_bytes((uint64_t)g.S + un) + packed_ws_bytes(g);
   GQF_TRY(ws_reserve(ds, sort_workspace_bytes(un, 4, false) + std::max(full, local ? sqf_local_bytes(g, un) : 0)));
   Arena ar(ds.ws, ds.ws_cap);
   uint32_t* B = nullptr;
@@ -307,13 +350,16 @@ static gqf_status sqf_insert_impl(const gqf_sqf& qf, int n, const uint32_t* d_ke
     if (done) return cs.end(ms, true);
     ar.used = mark;
   }
+  gqf_sqf wq;
+  GQF_TRY(work_table(cs.stream, ar, qf, g, true, &wq));
   uint32_t* A;
   const uint32_t* m_dev = nullptr;
-  GQF_TRY(sqf_decode(cs.stream, ar, qf, g, &A, &m_dev));
+  GQF_TRY(sqf_decode(cs.stream, ar, wq, g, &A, &m_dev));
   // at most S stored entries: grids sized for S + n, the count read on the device
-  if (mode == KEEP_TAGGED_INSERT) return sqf_merge_into<KEEP_TAGGED_INSERT>(cs, ar, qf, g, A, g.S, m_dev, B, un, ms);
-  if (mode == KEEP_UNIQUE) return sqf_merge_into<KEEP_UNIQUE>(cs, ar, qf, g, A, g.S, m_dev, B, un, ms);
-  return sqf_merge_into<KEEP_ALL>(cs, ar, qf, g, A, g.S, m_dev, B, un, ms);
+  if (mode == KEEP_TAGGED_INSERT)
+    return sqf_merge_into<KEEP_TAGGED_INSERT>(cs, ar, qf, wq, g, A, g.S, m_dev, B, un, ms);
+  if (mode == KEEP_UNIQUE) return sqf_merge_into<KEEP_UNIQUE>(cs, ar, qf, wq, g, A, g.S, m_dev, B, un, ms);
+  return sqf_merge_into<KEEP_ALL>(cs, ar, qf, wq, g, A, g.S, m_dev, B, un, ms);
 }
 
 static gqf_status sqf_delete_impl(const gqf_sqf& qf, int n, const uint32_t* d_keys, gqf_stream stream, float* ms) {
@@ -325,14 +371,16 @@ static gqf_status sqf_delete_impl(const gqf_sqf& qf, int n, const uint32_t* d_ke
   DeviceState& ds = *cs.ds;
   const uint32_t un = (uint32_t)n;
   GQF_TRY(ws_reserve(ds, sort_workspace_bytes(un, 4, false) + sqf_decode_bytes(g) + aligned_bytes(g.S + 1) +
-                             rebuild_ws_bytes(g.S)));
+                             rebuild_ws_bytes(g.S) + packed_ws_bytes(g)));
   Arena ar(ds.ws, ds.ws_cap);
   uint32_t* D = nullptr;
   GQF_TRY(radix_sort<uint32_t>(cs.stream, ar, d_keys, true, (g.q + g.r) | sqf_hash_flag(), nullptr, false, un, 0, g.q + g.r, &D,
                                nullptr));
+  gqf_sqf wq;
+  GQF_TRY(work_table(cs.stream, ar, qf, g, true, &wq));
   uint32_t* A;
   const uint32_t* m_dev = nullptr;
-  GQF_TRY(sqf_decode(cs.stream, ar, qf, g, &A, &m_dev));
+  GQF_TRY(sqf_decode(cs.stream, ar, wq, g, &A, &m_dev));
   uint8_t* keep = ar.take<uint8_t>((size_t)g.S + 1);
   if (ar.used > ar.cap) {
     set_error("internal: delete workspace overflow");
@@ -340,7 +388,8 @@ static gqf_status sqf_delete_impl(const gqf_sqf& qf, int n, const uint32_t* d_ke
   }
   GQF_LAUNCH_BC(0.0, m_dev, 5, (k_delete_mark<uint32_t, false>), dim3((g.S + DM_TILE - 1) / DM_TILE), dim3(DM_THREADS),
                 0, cs.stream, A, g.S, m_dev, D, un, keep, (const uint32_t*)nullptr);
-  GQF_TRY((launch_rebuild<KEEP_FLAGS, false>(cs.stream, ds.counters, ar, qf, g, A, g.S, m_dev, nullptr, 0, keep)));
+  GQF_TRY((launch_rebuild<KEEP_FLAGS, false>(cs.stream, ds.counters, ar, wq, g, A, g.S, m_dev, nullptr, 0, keep)));
+  finish_table(cs.stream, qf, wq, g, ds.counters);
   return cs.end(ms, true);  // deleting only shrinks the layout: it always fits
 }
 
@@ -358,7 +407,7 @@ gqf_status gqf_sqf_init(gqf_sqf* qf, unsigned int q, unsigned int r) {
   gqf_sqf t;
   t.qbits = q;
   t.rbits = r;
-  t.bytesPerElement = (r + 3) / 8;
+  t.bytesPerElement = sqf_is_packed(r) ? 0u : (r + 3) / 8;  // 0: packed (r+3)-bit elements
   t.table = reinterpret_cast<unsigned char*>(1);  // placeholder for the geometry check
   GQF_TRY(sqf_check(t));
   const size_t bytes = sqf_bytes(q, r);
@@ -401,11 +450,14 @@ gqf_status gqf_sqf_bulk_build_noalloc(gqf_sqf qf, int method, int n, const uint3
     return GQF_ERR_INVALID;
   }
   const uint32_t un = (uint32_t)n;
-  if (NOALLOC_COUNTER_BYTES + sqf_build_ws_bytes(un) > NOALLOC_SCRATCH_BYTES) {
-    set_error("internal: the no-allocation scratch (%zu bytes) is too small for %u keys", NOALLOC_SCRATCH_BYTES, un);
+  const SqfGeom g = geom(qf);
+  if (NOALLOC_COUNTER_BYTES + sqf_build_ws_bytes(un) + packed_ws_bytes(g) > NOALLOC_SCRATCH_BYTES) {
+    set_error(g.packed ? "the no-allocation scratch (%zu bytes) cannot hold %u keys plus the wide copy of this "
+                         "packed-width table; use gqf_sqf_bulk_build"
+                       : "internal: the no-allocation scratch (%zu bytes) is too small for %u keys",
+              NOALLOC_SCRATCH_BYTES, un);
     return GQF_ERR_INVALID;
   }
-  const SqfGeom g = geom(qf);
   NoallocDevice* nd = nullptr;
   GQF_TRY(noalloc_device(&nd));
   std::lock_guard<std::mutex> lk(nd->mu);
@@ -421,7 +473,10 @@ gqf_status gqf_sqf_bulk_build_noalloc(gqf_sqf qf, int method, int n, const uint3
   Arena ar(base + NOALLOC_COUNTER_BYTES, NOALLOC_SCRATCH_BYTES - NOALLOC_COUNTER_BYTES);
   // the array keeps the previous call's contents: the counters start at zero
   GQF_LAUNCH(k_reset_counters, dim3(1), dim3(64), 0, s, counters, 0, (int)CTR_TOTAL);
-  GQF_TRY(sqf_build_launch(s, ar, counters, qf, g, un, d_keys, no_dup));
+  gqf_sqf wq;
+  GQF_TRY(work_table(s, ar, qf, g, false, &wq));
+  GQF_TRY(sqf_build_launch(s, ar, counters, wq, g, un, d_keys, no_dup));
+  finish_table(s, qf, wq, g, counters);
   GQF_HIP(hipGetLastError());
   uint32_t err = 0;
   GQF_HIP(hipMemcpyAsync(&err, counters + CTR_ERR, sizeof(err), hipMemcpyDeviceToHost, s));
@@ -459,37 +514,24 @@ gqf_status gqf_sqf_lookup(gqf_sqf qf, int sorted, int n, const uint32_t* d_keys,
   const SqfGeom g = geom(qf);
   CallScope cs;
   GQF_TRY(cs.begin(stream));
-  DeviceState& ds = *cs.ds;
   const uint32_t un = (uint32_t)n;
   if (un) {
     // 64-slot windows need 16-byte aligned tables (hipMalloc gives 256)
     const bool win = (reinterpret_cast<uintptr_t>(qf.table) & 15u) == 0;
-    if (!sorted) {
-      const dim3 grid((un + 255) / 256);
-      const double b = (double)un * (8 + 64);
-      if (g.bpe == 1 && win) GQF_LAUNCH_B(b, (k_sqf_lookup<1, true>), grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, d_keys, un, d_out, sqf_hash_flag());
-      else if (g.bpe == 2 && win) GQF_LAUNCH_B(b, (k_sqf_lookup<2, true>), grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, d_keys, un, d_out, sqf_hash_flag());
-      else if (g.bpe == 1) GQF_LAUNCH_B(b, (k_sqf_lookup<1, false>), grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, d_keys, un, d_out, sqf_hash_flag());
-      else if (g.bpe == 2) GQF_LAUNCH_B(b, (k_sqf_lookup<2, false>), grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, d_keys, un, d_out, sqf_hash_flag());
-      else if (g.bpe == 3) GQF_LAUNCH_B(b, (k_sqf_lookup<3, false>), grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, d_keys, un, d_out, sqf_hash_flag());
-      else GQF_LAUNCH_B(b, (k_sqf_lookup<4, false>), grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, d_keys, un, d_out, sqf_hash_flag());
-    } else {
-      GQF_TRY(ws_reserve(ds, sort_workspace_bytes(un, 4, true)));
-      Arena ar(ds.ws, ds.ws_cap);
-      uint32_t *fps, *idx;
-      // order by the top 16 fingerprint bits: lookups then walk the table in order
-      const uint32_t bits = g.q + g.r;
-      const uint32_t lo = bits > 16 ? bits - 16 : 0;
-      GQF_TRY(radix_sort<uint32_t>(cs.stream, ar, d_keys, true, bits | sqf_hash_flag(), nullptr, true, un, lo, bits, &fps, &idx));
-      const dim3 grid((un + 255) / 256);
-      const double b = (double)un * (12 + 64);
-      if (g.bpe == 1 && win) GQF_LAUNCH_B(b, (k_sqf_lookup_sorted<1, true>), grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, fps, idx, un, d_out);
-      else if (g.bpe == 2 && win) GQF_LAUNCH_B(b, (k_sqf_lookup_sorted<2, true>), grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, fps, idx, un, d_out);
-      else if (g.bpe == 1) GQF_LAUNCH_B(b, (k_sqf_lookup_sorted<1, false>), grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, fps, idx, un, d_out);
-      else if (g.bpe == 2) GQF_LAUNCH_B(b, (k_sqf_lookup_sorted<2, false>), grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, fps, idx, un, d_out);
-      else if (g.bpe == 3) GQF_LAUNCH_B(b, (k_sqf_lookup_sorted<3, false>), grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, fps, idx, un, d_out);
-      else GQF_LAUNCH_B(b, (k_sqf_lookup_sorted<4, false>), grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, fps, idx, un, d_out);
-    }
+    // The sorted variant (launchSortedLookups) runs the same per-key walk
+    // without sorting the keys first: the results are the same by
+    // construction, and on MI355X sorting 2^24 keys costs more than the
+    // locality it buys (tables up to 147 MB; the RSQF up to 1.6 GB).
+    (void)sorted;
+    const dim3 grid((un + 255) / 256);
+    const double b = (double)un * (8 + 64);
+    if (g.packed) GQF_LAUNCH_B(b, k_sqf_lookup_packed, grid, dim3(256), 0, cs.stream, reinterpret_cast<const uint32_t*>(qf.table), g.S, g.q, g.r, d_keys, un, d_out, sqf_hash_flag());
+    else if (g.bpe == 1 && win) GQF_LAUNCH_B(b, (k_sqf_lookup<1, true>), grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, d_keys, un, d_out, sqf_hash_flag());
+    else if (g.bpe == 2 && win) GQF_LAUNCH_B(b, (k_sqf_lookup<2, true>), grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, d_keys, un, d_out, sqf_hash_flag());
+    else if (g.bpe == 1) GQF_LAUNCH_B(b, (k_sqf_lookup<1, false>), grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, d_keys, un, d_out, sqf_hash_flag());
+    else if (g.bpe == 2) GQF_LAUNCH_B(b, (k_sqf_lookup<2, false>), grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, d_keys, un, d_out, sqf_hash_flag());
+    else if (g.bpe == 3) GQF_LAUNCH_B(b, (k_sqf_lookup<3, false>), grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, d_keys, un, d_out, sqf_hash_flag());
+    else GQF_LAUNCH_B(b, (k_sqf_lookup<4, false>), grid, dim3(256), 0, cs.stream, qf.table, g.S, g.q, g.r, d_keys, un, d_out, sqf_hash_flag());
   }
   return cs.end(ms, false);
 }
@@ -505,16 +547,19 @@ gqf_status gqf_sqf_merge(gqf_sqf dst, gqf_sqf src, int no_dup, gqf_stream stream
   CallScope cs;
   GQF_TRY(cs.begin(stream));
   DeviceState& ds = *cs.ds;
-  GQF_TRY(ws_reserve(ds, 2 * sqf_decode_bytes(g) + rebuild_ws_bytes(2 * (uint64_t)g.S)));
+  GQF_TRY(ws_reserve(ds, 2 * sqf_decode_bytes(g) + rebuild_ws_bytes(2 * (uint64_t)g.S) + 2 * packed_ws_bytes(g)));
   Arena ar(ds.ws, ds.ws_cap);
+  gqf_sqf wd, ws;
+  GQF_TRY(work_table(cs.stream, ar, dst, g, true, &wd));
+  GQF_TRY(work_table(cs.stream, ar, src, g, true, &ws));
   uint32_t *A, *B;
   const uint32_t *m1_dev = nullptr, *m2_dev = nullptr;
-  GQF_TRY(sqf_decode(cs.stream, ar, dst, g, &A, &m1_dev));
-  GQF_TRY(sqf_decode(cs.stream, ar, src, g, &B, &m2_dev));
+  GQF_TRY(sqf_decode(cs.stream, ar, wd, g, &A, &m1_dev));
+  GQF_TRY(sqf_decode(cs.stream, ar, ws, g, &B, &m2_dev));
   uint32_t m2 = 0;  // the second operand's count sizes the merge on the host
   GQF_TRY(read_counter(cs.stream, ds, m2_dev, &m2));
-  if (no_dup) return sqf_merge_into<KEEP_UNIQUE>(cs, ar, dst, g, A, g.S, m1_dev, B, m2, ms);
-  return sqf_merge_into<KEEP_ALL>(cs, ar, dst, g, A, g.S, m1_dev, B, m2, ms);
+  if (no_dup) return sqf_merge_into<KEEP_UNIQUE>(cs, ar, dst, wd, g, A, g.S, m1_dev, B, m2, ms);
+  return sqf_merge_into<KEEP_ALL>(cs, ar, dst, wd, g, A, g.S, m1_dev, B, m2, ms);
 }
 
 }  // extern "C"

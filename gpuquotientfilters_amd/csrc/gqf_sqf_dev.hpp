@@ -11,10 +11,27 @@ constexpr uint32_t SQF_NOT_FOUND = 0xFFFFFFFFu;
 
 // numElements (quotientFilter.cu:174): (1 << q) * 1.1 in double, truncated.
 static inline uint32_t sqf_slots(unsigned q) { return (uint32_t)((double)(1u << q) * 1.1); }
-// calcNumSlotsGPU (quotientFilter.cu:37-42): bytes.
+
+// Slot widths with (r+3) % 8 != 0 (the reference GPU code requires
+// (r+3) % 8 == 0, README.md:32-33): the same S slots and the same elements,
+// (r+3) bits each, packed MSB-first into 32-bit words in the CPU_version's
+// order (getElement, quotientFilter.cpp:114-130: slot i at bits
+// [i(r+3), (i+1)(r+3)) counted from the most significant bit of word 0),
+// plus one trailing word so a 64-bit read at the last slot stays inside.
+// Such a filter has bytesPerElement == 0.  Supported up to 16-bit elements.
+__host__ __device__ __forceinline__ bool sqf_is_packed(unsigned r) { return (r + 3) % 8 != 0; }
+static inline size_t sqf_packed_words(unsigned q, unsigned r) {
+  return ((size_t)sqf_slots(q) * (r + 3) + 31) / 32 + 1;
+}
+// calcNumSlotsGPU (quotientFilter.cu:37-42): bytes; packed widths: the words.
 static inline size_t sqf_bytes(unsigned q, unsigned r) {
+  if (sqf_is_packed(r)) return 4 * sqf_packed_words(q, r);
   size_t tableBits = (size_t)(1u << q) * (r + 3);
   return (size_t)((double)(tableBits / 8) * 1.1);
+}
+static inline bool sqf_geometry_ok(unsigned q, unsigned r, unsigned bpe) {
+  if (q < 1 || r < 1 || q + r > 32 || q > 28) return false;
+  return sqf_is_packed(r) ? (r + 3 <= 16 && bpe == 0) : bpe == (r + 3) / 8;
 }
 
 [[maybe_unused]] static gqf_status sqf_check(const gqf_sqf& qf) {
@@ -22,13 +39,58 @@ static inline size_t sqf_bytes(unsigned q, unsigned r) {
     set_error("SQF table is NULL (call gqf_sqf_init)");
     return GQF_ERR_INVALID;
   }
-  if (qf.qbits < 1 || qf.rbits < 1 || (qf.rbits + 3) % 8 != 0 || qf.qbits + qf.rbits > 32 || qf.qbits > 28 ||
-      qf.bytesPerElement != (qf.rbits + 3) / 8) {
-    set_error("unsupported SQF geometry q=%u r=%u bytesPerElement=%u (need (r+3)%%8==0, q+r<=32, q<=28)", qf.qbits,
-              qf.rbits, qf.bytesPerElement);
+  if (!sqf_geometry_ok(qf.qbits, qf.rbits, qf.bytesPerElement)) {
+    set_error("unsupported SQF geometry q=%u r=%u bytesPerElement=%u (need q+r<=32, q<=28, r+3<=16 for packed "
+              "widths; bytesPerElement (r+3)/8, or 0 when (r+3)%%8 != 0)",
+              qf.qbits, qf.rbits, qf.bytesPerElement);
     return GQF_ERR_INVALID;
   }
   return GQF_OK;
+}
+
+// Element of slot i of a packed table of `width`-bit elements (0 past S).
+__host__ __device__ __forceinline__ uint32_t sqf_get_packed(const uint32_t* __restrict__ w, uint32_t S,
+                                                           uint32_t width, uint32_t i) {
+  if (i >= S) return 0u;
+  const uint64_t bit = (uint64_t)i * width;
+  const uint64_t j = bit >> 5;
+  const uint32_t sh = (uint32_t)(bit & 31u);
+  const uint64_t win = ((uint64_t)w[j] << 32) | w[j + 1];
+  return (uint32_t)(win >> (64u - sh - width)) & ((1u << width) - 1u);
+}
+
+// Packed table -> wide table (16-bit big-endian elements, the layout of a
+// byte-aligned 2-byte table), one slot per thread.
+[[maybe_unused]] static __global__ __launch_bounds__(256) void k_sqf_unpack(const uint32_t* __restrict__ packed,
+                                                                          uint32_t S, uint32_t width,
+                                                                          uint16_t* __restrict__ wide) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= S) return;
+  const uint32_t e = sqf_get_packed(packed, S, width, i);
+  wide[i] = (uint16_t)((e >> 8) | ((e & 0xFFu) << 8));
+}
+
+// Wide table -> packed words, one word per thread: the slots whose bits
+// overlap word j, shifted into place.  With ctr, nothing is written when the
+// call's capacity flag is set (the wide table then holds no result).
+[[maybe_unused]] static __global__ __launch_bounds__(256) void k_sqf_pack(const uint16_t* __restrict__ wide,
+                                                                        uint32_t S, uint32_t width,
+                                                                        uint32_t* __restrict__ packed,
+                                                                        uint32_t nwords,
+                                                                        const uint32_t* __restrict__ ctr) {
+  const uint32_t j = blockIdx.x * 256u + threadIdx.x;
+  if (j >= nwords) return;
+  if (ctr && (ctr[CTR_ERR] & DEV_ERR_CAPACITY)) return;
+  const uint64_t b0 = (uint64_t)j * 32;
+  uint32_t acc = 0;
+  for (uint64_t s = b0 / width; s < S && s * width < b0 + 32; ++s) {
+    const uint32_t x = wide[s];
+    const uint32_t e = ((x & 0xFFu) << 8) | (x >> 8);
+    // the element's most significant bit sits at bit offset s*width - b0 of the word
+    const int sh = 32 - (int)width - (int)((int64_t)(s * width) - (int64_t)b0);
+    acc |= sh >= 0 ? (e << sh) : (e >> -sh);
+  }
+  packed[j] = acc;
 }
 
 // ---------------------------------------------------------------- slot access
@@ -48,16 +110,17 @@ __host__ __device__ __forceinline__ uint32_t sqf_get(const uint8_t* __restrict__
 constexpr uint32_t SQF_GIVE_UP = 0xFFFFFFFEu;
 
 // findRunStartGPU (quotientFilter.cu:172-196) + run scan of hashAndLookUp
-// (quotientFilter.cu:333-356): slot of the first matching remainder.
+// (quotientFilter.cu:333-356): slot of the first matching remainder, over any
+// element accessor get(i) (0 past the last slot).
 // budget: at most 64 * budget slot steps, then SQF_GIVE_UP.
-template <int BPE>
-__host__ __device__ __forceinline__ uint32_t sqf_lookup_one(const uint8_t* __restrict__ t, uint32_t S, uint32_t fq,
-                                                           uint32_t fr, uint32_t budget = 0xFFFFFFFFu) {
-  if (!(sqf_get<BPE>(t, S, fq) & 4u)) return SQF_NOT_FOUND;
+template <class Get>
+__host__ __device__ __forceinline__ uint32_t sqf_walk(const Get& get, uint32_t S, uint32_t fq, uint32_t fr,
+                                                     uint32_t budget) {
+  if (!(get(fq) & 4u)) return SQF_NOT_FOUND;
   const uint64_t lim = (uint64_t)budget * 64;
   uint64_t steps = 0;
   uint32_t b = fq;
-  while (b > 0 && (sqf_get<BPE>(t, S, b) & 1u)) {
+  while (b > 0 && (get(b) & 1u)) {
     b--;
     if (++steps > lim) return SQF_GIVE_UP;
   }
@@ -66,22 +129,40 @@ __host__ __device__ __forceinline__ uint32_t sqf_lookup_one(const uint8_t* __res
     do {
       s++;
       if (++steps > lim) return SQF_GIVE_UP;
-    } while ((sqf_get<BPE>(t, S, s) & 2u) && s < S);
+    } while ((get(s) & 2u) && s < S);
     do {
       b++;
       if (++steps > lim) return SQF_GIVE_UP;
-    } while (!(sqf_get<BPE>(t, S, b) & 4u) && b < S);
+    } while (!(get(b) & 4u) && b < S);
   }
-  uint32_t e = sqf_get<BPE>(t, S, s);
+  uint32_t e = get(s);
   do {
     const uint32_t rem = e >> 3;
     if (rem == fr) return s;
     if (rem > fr) return SQF_NOT_FOUND;
     s++;
-    e = sqf_get<BPE>(t, S, s);
+    e = get(s);
     if (++steps > lim) return SQF_GIVE_UP;
   } while (e & 2u);
   return SQF_NOT_FOUND;
+}
+
+template <int BPE>
+struct SqfBytes {
+  const uint8_t* t;
+  uint32_t S;
+  __host__ __device__ __forceinline__ uint32_t operator()(uint32_t i) const { return sqf_get<BPE>(t, S, i); }
+};
+struct SqfPacked {
+  const uint32_t* w;
+  uint32_t S, width;
+  __host__ __device__ __forceinline__ uint32_t operator()(uint32_t i) const { return sqf_get_packed(w, S, width, i); }
+};
+
+template <int BPE>
+__host__ __device__ __forceinline__ uint32_t sqf_lookup_one(const uint8_t* __restrict__ t, uint32_t S, uint32_t fq,
+                                                           uint32_t fr, uint32_t budget = 0xFFFFFFFFu) {
+  return sqf_walk(SqfBytes<BPE>{t, S}, S, fq, fr, budget);
 }
 
 // The same walk over 64-slot windows (BPE 1 and 2): a window is loaded with
@@ -325,17 +406,14 @@ __global__ __launch_bounds__(256) void k_sqf_lookup(const uint8_t* __restrict__ 
   out[i] = sqf_lookup_fp<BPE, WIN>(t, S, (fp >> r) & (uint32_t)mask64(q), fp & ((1u << r) - 1u));
 }
 
-// Lookups in fingerprint order (sorted path): result scattered to the
-// original index.
-template <int BPE, bool WIN>
-__global__ __launch_bounds__(256) void k_sqf_lookup_sorted(const uint8_t* __restrict__ t, uint32_t S, uint32_t q,
-                                                           uint32_t r, const uint32_t* __restrict__ fps,
-                                                           const uint32_t* __restrict__ idx, uint32_t n,
-                                                           uint32_t* __restrict__ out) {
+// Lookups on a packed table (slot widths with (r+3) % 8 != 0): the same walk.
+[[maybe_unused]] static __global__ __launch_bounds__(256) void k_sqf_lookup_packed(
+    const uint32_t* __restrict__ t, uint32_t S, uint32_t q, uint32_t r, const uint32_t* __restrict__ keys, uint32_t n,
+    uint32_t* __restrict__ out, uint32_t hflag) {
   const uint64_t i = blockIdx.x * 256ull + threadIdx.x;
   if (i >= n) return;
-  const uint32_t fp = fps[i];
-  out[idx[i]] = sqf_lookup_fp<BPE, WIN>(t, S, (fp >> r) & (uint32_t)mask64(q), fp & ((1u << r) - 1u));
+  const uint32_t fp = fingerprint32(keys[i], q + r + hflag);
+  out[i] = sqf_walk(SqfPacked{t, S, r + 3}, S, (fp >> r) & (uint32_t)mask64(q), fp & ((1u << r) - 1u), 0xFFFFFFFFu);
 }
 
 // ---------------------------------------------------------------- fused rebuild

@@ -55,7 +55,14 @@ const char* gqf_version(void);
  * quotient_filter (quotientFilter.cuh:25-31) and the same table format:
  * floor(2^q * 1.1) slots of (r+3)/8 bytes, big-endian, element =
  * remainder<<3 | occupied<<2 | continuation<<1 | shifted.
- * Supported: (r + 3) % 8 == 0 (r = 5 or 13), q >= 1, q + r <= 32. */
+ * Supported: q >= 1, q <= 28, q + r <= 32, and either (r + 3) % 8 == 0
+ * (r = 5 or 13: the reference GPU format) or, beyond the reference GPU code
+ * (README.md:32-33), any r with r + 3 <= 16: the same S slots and elements,
+ * (r+3) bits each, packed MSB-first into 32-bit words in the CPU_version's
+ * order (getElement, quotientFilter.cpp:114-130), plus one trailing word;
+ * such a filter has bytesPerElement == 0.  Every SQF call accepts both
+ * layouts (batches: byte-aligned ones only), with the same results for the
+ * same q, r and keys. */
 typedef struct gqf_sqf {
   unsigned int qbits;
   unsigned int rbits;
@@ -141,7 +148,9 @@ gqf_status gqf_sqf_insert_via_merge(gqf_sqf qf, const uint32_t* d_old, int n_old
 /* launchUnsortedLookups (quotientFilter.cu:359-381) when sorted == 0,
  * launchSortedLookups (quotientFilter.cu:272-318) when sorted != 0.  Both
  * write, for each key, the slot of the first matching remainder of its run,
- * or GQF_NOT_FOUND. */
+ * or GQF_NOT_FOUND.  Both run the same per-key walk: the reference sorts the
+ * keys first for locality, which on MI355X costs more than it saves (the
+ * results do not depend on the order). */
 gqf_status gqf_sqf_lookup(gqf_sqf qf, int sorted, int n, const uint32_t* d_keys, uint32_t* d_out,
                           gqf_stream stream, float* ms);
 
@@ -193,7 +202,8 @@ gqf_status gqf_rsqf_insert(gqf_rsqf cqf, int n, const uint32_t* d_keys, int32_t*
 
 /* launchUnsortedLookups (RSQF.cu:540-563) when sorted == 0, launchLookups
  * (RSQF.cu:411-456) when sorted != 0: the LAST matching slot of the run,
- * or -1.  The walk is the reference's and reads the 8-bit block offsets,
+ * or -1.  Both run the same per-key walk (no sort first: on MI355X sorting
+ * costs more than its locality saves, measured up to 1.6 GB tables).  The walk is the reference's and reads the 8-bit block offsets,
  * which the format stores mod 256 (RSQF.cu offset++ on a uchar): in the
  * blocks behind a run that pushes an offset past 255 a lookup can land on
  * the wrong run and return -1 for a stored key, exactly as the reference
@@ -203,10 +213,12 @@ gqf_status gqf_rsqf_lookup(gqf_rsqf cqf, int sorted, int n, const uint32_t* d_ke
 
 /* Counting query (no reference counterpart; README.md:74 names counters as
  * missing): per key, the number of stored copies of its fingerprint -- the
- * RSQF here keeps a multiset, so the copies are the equal remainders that end
- * at the slot launchLookups returns.  Like the lookups, it reads the 8-bit
- * block offsets of the reference format, so it is exact while no run pushes
- * an offset past 255. */
+ * RSQF here keeps a multiset, so the copies are the slots of the key's run
+ * holding its remainder.  The run is found from run ranks (the k-th run end
+ * closes the run of the k-th occupied quotient; per-block prefix counts of
+ * both bit vectors are built first), not from the 8-bit block offsets, so
+ * counts are exact for runs of any length, also behind runs that push an
+ * offset past 255. */
 gqf_status gqf_rsqf_count(gqf_rsqf cqf, int n, const uint32_t* d_keys, uint32_t* d_counts, gqf_stream stream,
                           float* ms);
 
@@ -216,8 +228,8 @@ gqf_status gqf_rsqf_delete(gqf_rsqf cqf, int n, const uint32_t* d_keys, gqf_stre
                            float* ms);
 
 /* Device workspace (bytes) a call of `op` (gqf_op) over n keys reserves on
- * its stream's context; lookups only when sorted.  No reference counterpart
- * (the reference allocates per call). */
+ * its stream's context (lookups: none; `sorted` is kept for the signature).
+ * No reference counterpart (the reference allocates per call). */
 size_t gqf_rsqf_workspace_bytes(gqf_rsqf cqf, int op, int n, int sorted);
 
 /* Copy the table to host memory in the reference packed layout
@@ -366,7 +378,7 @@ typedef struct gqf_request {
 typedef struct gqf_serve_config {
   size_t hbm_budget;       /* bytes: tables + held workspaces + gather buffers */
   int coalesce;            /* see above */
-  int sorted_lookups;      /* lookups through the sorted path */
+  int sorted_lookups;      /* lookups through the sorted entry point (the same walk) */
   int max_batch;           /* keys per coalesced call; 0 = 2^24 */
 } gqf_serve_config;
 

@@ -25,6 +25,7 @@
 #include <climits>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <vector>
 
 #include "gqf.h"
@@ -179,8 +180,18 @@ inline void printQuotientFilterGPU(quotient_filter* qf) {
   std::vector<unsigned char> h(bytes);
   if (gqf_memcpy_dtoh(h.data(), qf->table, bytes) != GQF_OK) gqf_compat_detail::die("printQuotientFilterGPU");
   const unsigned int bpe = qf->bytesPerElement;
+  const unsigned int width = qf->rbits + 3;
   auto element = [&](size_t slot) {
     unsigned int e = 0;
+    if (bpe == 0) {  // packed (r+3)-bit elements, MSB-first in 32-bit words
+      for (unsigned int k = 0; k < width; ++k) {
+        const size_t p = slot * width + k;
+        unsigned int word;
+        std::memcpy(&word, h.data() + 4 * (p / 32), 4);
+        e = (e << 1) | ((word >> (31 - p % 32)) & 1u);
+      }
+      return e;
+    }
     for (unsigned int b = 0; b < bpe; ++b) e = (e << 8) | h[slot * bpe + b];
     return e;
   };

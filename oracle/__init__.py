@@ -149,6 +149,25 @@ def sqf_decode_words(words, q, r, nslots=None):
     return out
 
 
+def sqf_packed_words(q, r):
+    """Words of a packed-width GPU table: S (r+3)-bit slots, plus one word."""
+    return (sqf_num_slots(q) * (r + 3) + 31) // 32 + 1
+
+
+def sqf_pack(elems, q, r):
+    """Elements (one per slot) -> (r+3)-bit fields packed MSB-first into uint32
+    words, the CPU_version's bit order (getElement, quotientFilter.cpp:114-130):
+    slot i at bits [i(r+3), (i+1)(r+3)) from the top of word 0.  Inverse of
+    sqf_decode_words (checked against it and the reference's own words)."""
+    w = r + 3
+    e = np.ascontiguousarray(elems, dtype=np.uint32)
+    bits = ((e[:, None] >> np.arange(w - 1, -1, -1, dtype=np.uint32)) & 1).astype(np.uint8).ravel()
+    nwords = sqf_packed_words(q, r)
+    buf = np.zeros(nwords * 32, np.uint8)
+    buf[: len(bits)] = bits
+    return np.packbits(buf, bitorder="big").view(">u4").astype(np.uint32)
+
+
 def sqf_canon(fps, q, r, no_dup):
     fps = _u64(fps)
     t = np.zeros(sqf_table_bytes(q, r), dtype=np.uint8)

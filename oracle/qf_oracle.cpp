@@ -27,6 +27,9 @@ inline uint64_t low_mask64(unsigned n) { return n >= 64 ? ~0ull : ((1ull << n) -
 // ------------------------------------------------------------------ SQF
 // Slot codec of the GPU_version: (r+3)/8 bytes per slot, big-endian
 // (quotientFilter.cu:151-170).  element = rem<<3 | occupied<<2 | continuation<<1 | shifted.
+// For widths the GPU_version does not support ((r+3) % 8 != 0) the model
+// keeps the same elements in ceil((r+3)/8) bytes (a "wide" table); the
+// packed (r+3)-bit layout is compared against it element by element.
 struct Sqf {
   uint8_t* t;
   unsigned q, r, bpe;
@@ -53,7 +56,7 @@ Sqf mk_sqf(uint8_t* t, unsigned q, unsigned r) {
   f.t = t;
   f.q = q;
   f.r = r;
-  f.bpe = (r + 3) / 8;
+  f.bpe = (r + 3 + 7) / 8;
   f.S = orc_sqf_num_slots(q);
   return f;
 }
@@ -185,6 +188,8 @@ uint32_t orc_sqf_num_slots(unsigned q) { return (uint32_t)((double)(1u << q) * 1
 
 // calcNumSlotsGPU (quotientFilter.cu:37-42): a byte count despite the name.
 size_t orc_sqf_table_bytes(unsigned q, unsigned r) {
+  // widths the GPU_version does not support: S slots of ceil((r+3)/8) bytes
+  if ((r + 3) % 8 != 0) return (size_t)orc_sqf_num_slots(q) * ((r + 3 + 7) / 8);
   size_t tableBits = (size_t)(1u << q) * (r + 3);
   size_t tableSlots = tableBits / 8;
   return (size_t)((double)tableSlots * 1.1);
@@ -199,14 +204,14 @@ size_t orc_sqf_cpu_words(unsigned q, unsigned r) {
 }
 
 uint32_t orc_sqf_get(const uint8_t* table, unsigned r, uint32_t slot) {
-  unsigned bpe = (r + 3) / 8;
+  unsigned bpe = (r + 3 + 7) / 8;
   uint32_t e = table[(size_t)slot * bpe];
   for (unsigned k = 1; k < bpe; ++k) e = (e << 8) | table[(size_t)slot * bpe + k];
   return e;
 }
 
 void orc_sqf_set(uint8_t* table, unsigned r, uint32_t slot, uint32_t elem) {
-  unsigned bpe = (r + 3) / 8;
+  unsigned bpe = (r + 3 + 7) / 8;
   for (unsigned k = 0; k < bpe; ++k)
     table[(size_t)slot * bpe + k] = (uint8_t)(elem >> (8 * (bpe - 1 - k)));
 }

@@ -4,7 +4,8 @@ The reference CPU_version sources (quotientFilter.cpp, RSQF.cpp, mt19937ar.cpp)
 are compiled where they lie by ``oracle/Makefile`` (``make -C oracle ref``)
 into ``oracle/_ref/``; this script drives them through the C entry points in
 ``oracle/ref_*_wrap.cpp`` and stores inputs and outputs as ``.npz`` data.
-Run from the repository root:  ``python tests/golden/make_golden.py``.
+Run from the repository root:  ``python tests/golden/make_golden.py``
+(``--only sqf_packed`` writes just the packed-width SQF fixtures).
 """
 import os
 import sys
@@ -25,9 +26,30 @@ def save(name, **arrays):
     print("wrote", name, {k: v.shape for k, v in arrays.items()})
 
 
+# SQF widths the reference GPU code does not support ((r+3) % 8 != 0); the
+# CPU_version packs any width (quotientFilter.cpp:114-130)
+SQF_PACKED = [(12, 6, 0.7), (12, 7, 0.75), (12, 8, 0.8), (11, 9, 0.8), (11, 10, 0.85), (10, 11, 0.85), (10, 12, 0.9)]
+
+
+def sqf_fixture(q, r, alpha):
+    """CPU_version SQF: insertItem loop end state + mayContain (quotientFilter.cpp:202-325)."""
+    n = O.key_count(alpha, q)
+    ks = RefSQF.mt(1, n)
+    ref = RefSQF(q, r)
+    ins = ref.insert(ks)
+    fresh = RefSQF.mt(101, n)
+    lk = np.concatenate([ks, fresh])
+    save(f"sqf_fnv_q{q}_r{r}_a{alpha}.npz", q=np.uint32(q), r=np.uint32(r), keys=ks,
+         insert_ret=ins, words=ref.words(), lookup_keys=lk, lookup_ret=ref.lookup(lk).view(np.uint32))
+
+
 def main():
     O.build()
     assert RefSQF.available() and RefRSQF.available(), "oracle/_ref not built (needs the reference tree)"
+    if "--only" in sys.argv and sys.argv[sys.argv.index("--only") + 1] == "sqf_packed":
+        for q, r, alpha in SQF_PACKED:
+            sqf_fixture(q, r, alpha)
+        return
 
     # MT19937 stream of the reference generator (mt19937ar.cpp:57-70, genrand_int32)
     save("mt19937.npz", seed1=RefSQF.mt(1, 4096), seed7=RefSQF.mt(7, 1024))
@@ -51,15 +73,8 @@ def main():
     save("rank_select.npz", words=words, ks=ks, select=sel, rank=rank)
 
     # SQF CPU_version: insertItem loop end state + mayContain (quotientFilter.cpp:202-325)
-    for q, r, alpha in [(12, 5, 0.6), (14, 5, 0.9), (12, 13, 0.8)]:
-        n = O.key_count(alpha, q)
-        ks = RefSQF.mt(1, n)
-        ref = RefSQF(q, r)
-        ins = ref.insert(ks)
-        fresh = RefSQF.mt(101, n)
-        lk = np.concatenate([ks, fresh])
-        save(f"sqf_fnv_q{q}_r{r}_a{alpha}.npz", q=np.uint32(q), r=np.uint32(r), keys=ks,
-             insert_ret=ins, words=ref.words(), lookup_keys=lk, lookup_ret=ref.lookup(lk).view(np.uint32))
+    for q, r, alpha in [(12, 5, 0.6), (14, 5, 0.9), (12, 13, 0.8)] + SQF_PACKED:
+        sqf_fixture(q, r, alpha)
 
     # RSQF CPU_version: random-order inserts (history-dependent layout) and
     # ascending-fingerprint-order inserts (canonical layout), plus lookups

@@ -59,8 +59,16 @@ def test_invalid_geometry_rejected_without_gpu():
     import gpuquotientfilters_amd as G
 
     h = G._SQF()
-    # (r + 3) % 8 != 0 is rejected before any device call
-    assert G.lib().gqf_sqf_init(C.byref(h), 10, 6) == G.ERR_INVALID
+    # packed widths reach 16-bit elements; wider ones, q + r > 32 and q > 28
+    # are rejected before any device call
+    assert G.lib().gqf_sqf_init(C.byref(h), 10, 14) == G.ERR_INVALID
+    assert G.lib().gqf_sqf_init(C.byref(h), 25, 8) == G.ERR_INVALID
+    assert G.lib().gqf_sqf_init(C.byref(h), 29, 1) == G.ERR_INVALID
+    assert G.lib().gqf_sqf_init(C.byref(h), 10, 0) == G.ERR_INVALID
+    # sizes of packed tables: S (r+3)-bit slots in 32-bit words, plus one
+    for q, r in [(10, 6), (12, 8), (20, 9), (26, 4)]:
+        S = G.lib().gqf_sqf_num_slots(q)
+        assert G.lib().gqf_sqf_table_bytes(q, r) == 4 * ((S * (r + 3) + 31) // 32 + 1)
     h2 = G._RSQF()
     assert G.lib().gqf_rsqf_init(C.byref(h2), 10, 40) == G.ERR_INVALID
 

@@ -230,3 +230,28 @@ def test_batch_lookup_sees_tables_changed_after_the_build():
         got = out.to_host()[int(po[2]):int(po[3])]
         assert (got == O.sqf_lookup_keys(want, q, r, probe2)).all()
     b.destroy()
+
+
+def test_batch_at_the_benchmark_shape():
+    """160 filters with q drawn from 12..22 (the batched benchmark's range,
+    seed 7), r = 5, fill 0.3..0.9, Zipf(1.1) keys in Dup mode (capacity errors
+    included), 2^6..2^12 lookups each: one graph build and one graph lookup
+    launch against the per-filter calls for every filter, and the oracle for
+    every eighth filter."""
+    r = 5
+    qs, keys, probe = make_batch(160, 12, 22, r, seed=7, probes=(6, 12))
+    say("sqf batch at benchmark shape", len(qs), sum(map(len, keys)))
+    filters = [G.SQF(q, r) for q in qs]
+    dk, dp = dev(flat(keys)), dev(flat(probe))
+    out = G.DeviceArray.empty(len(flat(probe)))
+    b = G.SQFBatch(filters, dk, offsets(keys), dp, offsets(probe), out, no_dup=False)
+    ref_tables, ref_looks, ref_status = per_filter(qs, r, keys, probe, False)
+    for f in filters:
+        f.clear()
+    out.fill(0)
+    ms, st = b.build(use_graph=True)
+    assert st == ref_status
+    assert ref_status.count(G.OK) > 20 and ref_status.count(G.ERR_CAPACITY) > 20
+    b.lookup(use_graph=True)
+    check_batch(qs, r, keys, probe, False, filters, out, ref_tables, ref_looks, ref_status, oracle_every=8)
+    b.destroy()

@@ -219,23 +219,22 @@ def test_wide_insert_delete_equal_canonical(q, r, alpha):
     assert (f.dump() == canon(keys[n // 10:], q, r)).all()
 
 
-@pytest.mark.parametrize("q,r", [(14, 8), (13, 5), (20, 14)])
-def test_count_query(q, r):
+@pytest.mark.parametrize("q,r,hot", [(14, 8, 200), (13, 5, 200), (20, 14, 200), (14, 8, 5000), (15, 16, 3000)])
+def test_count_query(q, r, hot):
     """gqf_rsqf_count == multiplicity of each key's fingerprint in the stored
-    multiset (duplicates, a run of 200 copies of one key, misses; 34-bit
-    fingerprints at q=20 r=14).  Counts follow the reference lookup walk,
-    which reads the 8-bit block offsets: a run that pushes an offset past
-    255 breaks lookups (and counts) in the reference format, so the long
-    run stays below that."""
-    say("rsqf count", q, r)
-    n = O.key_count(0.7, q)
+    multiset (duplicates, a run of `hot` copies of one key, misses; 34-bit
+    fingerprints at q=20 r=14).  Counts follow run ranks, not the 8-bit
+    offsets: a run of thousands of copies pushes the offsets of the blocks
+    behind it past 255, and every count there stays exact."""
+    say("rsqf count", q, r, hot)
+    n = O.key_count(0.7 if hot <= 200 else 0.3, q)
     keys = O.mt19937(31, n)
     cand = O.mt19937(5, 1000)
-    hot = cand[(O.fingerprints(cand, q, r) >> np.uint64(r)) < (1 << (q - 2))][0]
-    stored = np.concatenate([keys, keys[:n // 5], keys[:n // 20], np.full(200, hot, np.uint32)])
+    hotk = cand[(O.fingerprints(cand, q, r) >> np.uint64(r)) < (1 << (q - 2))][0]
+    stored = np.concatenate([keys, keys[:n // 5], keys[:n // 20], np.full(hot, hotk, np.uint32)])
     f = G.RSQF(q, r)
     f.bulk_build(dev(stored))
-    probe = np.concatenate([keys[: n // 2], [hot], fresh_keys(32, 2000, stored)]).astype(np.uint32)
+    probe = np.concatenate([keys, [hotk], fresh_keys(32, 2000, stored)]).astype(np.uint32)
     out = G.DeviceArray.empty(len(probe))
     f.count(dev(probe), out)
     vals, cnt = np.unique(O.fingerprints(stored, q, r), return_counts=True)
@@ -243,7 +242,7 @@ def test_count_query(q, r):
     exp = np.array([want.get(int(x), 0) for x in O.fingerprints(probe, q, r)], np.uint32)
     got = out.to_host()
     assert (got == exp).all(), np.flatnonzero(got != exp)[:8]
-    assert got[n // 2] >= 200
+    assert got[n] >= hot
 
 
 @pytest.mark.parametrize("q,r,alpha", [(24, 10, 0.2), (26, 8, 0.05)])

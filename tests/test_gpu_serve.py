@@ -3,6 +3,8 @@ over resident RSQFs, served with coalescing on per-filter streams and again
 plainly (one call per request, in arrival order).  Every lookup result and
 every final table must be identical between the two, and equal to the oracle
 replaying the stream one request at a time."""
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -149,3 +151,62 @@ def test_tight_budget_serialises_and_rejects():
         assert (plan[i][3].to_host()[: len(want)] == want).all(), i
     for f in range(len(QS)):
         assert (fa[f].dump() == finals[f]).all(), f
+
+
+def test_wide_fingerprints_in_the_serving_loop():
+    """Filters at the serving configuration's sizes: q = 24 and 25 with r = 8
+    (32- and 33-bit fingerprints; the second takes the 64-bit decode path)
+    beside a small one, under the same mixed stream, coalesced and plain.
+    Every lookup and final table agrees between the two; the oracle replays
+    the stream: the small filter's lookups as they arrive, each big filter's
+    final table and a last lookup batch on it."""
+    r = 8
+    qs = [16, 24, 25]
+    say("serve wide fingerprints", qs, r)
+    fa, stored = make_set(qs, r, 0.5, seed=51)
+    fb, _ = make_set(qs, r, 0.5, seed=51)
+    n = 90
+    ra, plan_a, hold_a = make_stream(stored, n, seed=12, log2_sizes=(6, 14), mean_gap_us=60.0)
+    rb, plan_b, hold_b = make_stream(stored, n, seed=12, log2_sizes=(6, 14), mean_gap_us=60.0)
+    # a last lookup batch on every filter, after everything else has arrived
+    rng = np.random.default_rng(13)
+    tail = []
+    for f in range(len(qs)):
+        keys = np.concatenate([rng.choice(stored[f], 3000), rng.integers(0, 2 ** 32, 3000, dtype=np.uint32)])
+        tail.append((f, keys.astype(np.uint32)))
+    extra_a = (G.Request * (n + len(tail)))()
+    extra_b = (G.Request * (n + len(tail)))()
+    for src, dst in ((ra, extra_a), (rb, extra_b)):
+        C_size = C.sizeof(G.Request)
+        C.memmove(C.addressof(dst), C.addressof(src), n * C_size)
+    t_end = max(x.arrive_us for x in ra) + 50.0
+    outs = []
+    for j, (f, keys) in enumerate(tail):
+        d = dev(keys)
+        oa, ob = G.DeviceArray.empty(len(keys)), G.DeviceArray.empty(len(keys))
+        hold_a += [d, oa, ob]
+        for req, o in ((extra_a[n + j], oa), (extra_b[n + j], ob)):
+            req.filter, req.op, req.n = f, G.OP_LOOKUP, len(keys)
+            req.d_keys, req.d_out, req.arrive_us = d.ptr, o.ptr, t_end + j
+        outs.append((oa, ob))
+    sa = G.serve(fa, extra_a, coalesce=True)
+    sb = G.serve(fb, extra_b, coalesce=False)
+    assert all(x.status == G.OK for x in extra_a) and all(x.status == G.OK for x in extra_b)
+    assert sa.requests == sb.requests == n + len(tail)
+    for i, ((f, op, keys, oa), (_, _, _, ob)) in enumerate(zip(plan_a, plan_b)):
+        if op == G.OP_LOOKUP and len(keys):
+            assert (oa.to_host()[: len(keys)] == ob.to_host()[: len(keys)]).all(), i
+    # the oracle: small filter lookups on arrival (the replay checks filter 0
+    # only), final tables and the last lookups everywhere
+    small_plan = [(f, op, keys, o) if f == 0 or op != G.OP_LOOKUP else (f, op, np.zeros(0, np.uint32), o)
+                  for (f, op, keys, o) in plan_a]
+    expect, finals = replay_oracle(qs, r, stored, small_plan, [x.status for x in ra[:n]])
+    for i, want in expect.items():
+        assert (plan_a[i][3].to_host()[: len(want)] == want).all(), i
+    for f in range(len(qs)):
+        da = fa[f].dump()
+        assert (da == fb[f].dump()).all(), f
+        assert (da == finals[f]).all(), f
+        keys = tail[f][1]
+        want = O.rsqf_lookup_keys(finals[f], qs[f], r, keys)
+        assert (outs[f][0].to_host() == want).all() and (outs[f][1].to_host() == want).all(), f

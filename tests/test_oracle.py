@@ -112,7 +112,13 @@ def test_sqf_geometry(orc):
     assert orc.rsqf_table_bytes(26, 8) == 84934737
 
 
-@pytest.mark.parametrize("name", ["sqf_fnv_q12_r5_a0.6.npz", "sqf_fnv_q14_r5_a0.9.npz", "sqf_fnv_q12_r13_a0.8.npz"])
+SQF_PACKED_FIXTURES = ["sqf_fnv_q12_r6_a0.7.npz", "sqf_fnv_q12_r7_a0.75.npz", "sqf_fnv_q12_r8_a0.8.npz",
+                       "sqf_fnv_q11_r9_a0.8.npz", "sqf_fnv_q11_r10_a0.85.npz", "sqf_fnv_q10_r11_a0.85.npz",
+                       "sqf_fnv_q10_r12_a0.9.npz"]
+
+
+@pytest.mark.parametrize("name", ["sqf_fnv_q12_r5_a0.6.npz", "sqf_fnv_q14_r5_a0.9.npz", "sqf_fnv_q12_r13_a0.8.npz"]
+                         + SQF_PACKED_FIXTURES)
 def test_sqf_oracle_matches_reference_cpu(orc, name):
     g = golden(name)
     q, r = int(g["q"]), int(g["r"])
@@ -129,6 +135,24 @@ def test_sqf_oracle_matches_reference_cpu(orc, name):
     # (3) lookups == reference mayContain
     out = orc.sqf_lookup_keys(can, q, r, g["lookup_keys"], 1)
     assert (out == g["lookup_ret"]).all()
+
+
+@pytest.mark.parametrize("name", SQF_PACKED_FIXTURES)
+def test_sqf_packer_reproduces_reference_words(orc, name):
+    """The packed layout of widths (r+3) % 8 != 0 is the CPU_version's word
+    layout over the GPU's S slots: packing the oracle's canonical elements
+    gives the reference's own words (then zeros: the GPU table is a few words
+    longer), and unpacking any words gives back their elements."""
+    g = golden(name)
+    q, r = int(g["q"]), int(g["r"])
+    ref = g["words"].astype(np.uint32)
+    st, can = orc.sqf_canon(orc.fingerprints(g["keys"], q, r, 1), q, r, True)
+    assert st == orc.OK
+    packed = orc.sqf_pack(orc.sqf_decode(can, q, r), q, r)
+    assert len(packed) == orc.sqf_packed_words(q, r) >= len(ref)
+    assert (packed[: len(ref)] == ref).all() and not packed[len(ref):].any()
+    rnd = orc.mt19937(7, orc.sqf_num_slots(q)) & np.uint32((1 << (r + 3)) - 1)
+    assert (orc.sqf_decode_words(orc.sqf_pack(rnd, q, r), q, r) == rnd).all()
 
 
 def test_sqf_delete_restatement_is_canonical(orc):
