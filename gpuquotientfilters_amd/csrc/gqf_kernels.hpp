@@ -119,6 +119,21 @@ __device__ __forceinline__ uint32_t block256_excl_scan(uint32_t x, uint32_t* s_t
   return pre + inc - x;
 }
 
+// Exclusive scan of one uint32 per thread over a block of T threads (T a
+// multiple of 64); s_tmp: T/64 words.  Contains two barriers.
+template <int T>
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t x, uint32_t* s_tmp) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint32_t inc = wave_incl_add(x);
+  if (lane == 63) s_tmp[w] = inc;
+  __syncthreads();
+  uint32_t pre = 0;
+#pragma unroll
+  for (int i = 0; i < T / 64; ++i) pre += (i < w) ? s_tmp[i] : 0u;
+  __syncthreads();
+  return pre + inc - x;
+}
+
 // Same for uint64 (used with packed counters).
 __device__ __forceinline__ uint64_t block256_excl_scan64(uint64_t x, uint64_t* s_tmp4, uint64_t* total) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -235,7 +250,8 @@ inline SortPlan sort_plan(uint32_t n) {
     return p;
   }
   const uint32_t G0 = tiles < 2048 ? tiles : 2048u;
-  const uint32_t per = (tiles + G0 - 1) / G0;
+  // an even number of 4096-key tiles: whole 8192-key downsweep sub-tiles
+  const uint32_t per = ((tiles + G0 - 1) / G0 + 1) & ~1u;
   p.chunk = per * SORT_TILE;
   p.G = (uint32_t)(((uint64_t)n + p.chunk - 1) / p.chunk);
   return p;
@@ -398,8 +414,8 @@ __device__ __forceinline__ void ts_load_vals(const uint32_t* __restrict__ in_val
 
 // Step 1 of a counting pass over the nibble at bit sh: the entries' ranks
 // within the thread (4 bits each, entry i at bits 4i of *rk) and the thread's
-// 16 counts, written to s_cnt[j*256 + t].
-template <typename K>
+// 16 counts, written to s_cnt[j*T + t] (T threads in the block).
+template <int T, typename K>
 __device__ __forceinline__ void ts_count(const K (&k)[SORT_ITEMS], uint32_t sh, uint64_t& rk, uint16_t* s_cnt) {
   uint64_t h = 0;
   uint32_t last = 0, r = 0;
@@ -417,7 +433,7 @@ __device__ __forceinline__ void ts_count(const K (&k)[SORT_ITEMS], uint32_t sh, 
 #pragma unroll
   for (int j = 0; j < 16; ++j) {
     const uint32_t c = full ? (j == (int)last ? 16u : 0u) : (uint32_t)(h >> (4 * j)) & 15u;
-    s_cnt[j * SORT_THREADS + t] = (uint16_t)c;
+    s_cnt[j * T + t] = (uint16_t)c;
   }
 }
 
@@ -425,6 +441,7 @@ __device__ __forceinline__ void ts_count(const K (&k)[SORT_ITEMS], uint32_t sh, 
 // scanned alongside in the high half of a 32-bit word: both halves stay below
 // 2^16) and writes exclusive prefixes back.  Returns the exclusive prefix of
 // `extra` over the block.  Contains two barriers.
+template <int T>
 __device__ __forceinline__ uint32_t ts_scan(uint16_t* s_cnt, uint32_t extra, uint32_t* s_tmp) {
   const uint32_t t = threadIdx.x;
   uint4* row = reinterpret_cast<uint4*>(s_cnt + 16 * t);
@@ -433,7 +450,7 @@ __device__ __forceinline__ uint32_t ts_scan(uint16_t* s_cnt, uint32_t extra, uin
   uint32_t sum = 0;
 #pragma unroll
   for (int j = 0; j < 8; ++j) sum += (w[j] & 0xFFFFu) + (w[j] >> 16);
-  const uint32_t ex = block256_excl_scan(sum | (extra << 16), s_tmp, nullptr);
+  const uint32_t ex = block_excl_scan<T>(sum | (extra << 16), s_tmp);
   uint32_t run = ex & 0xFFFFu;
 #pragma unroll
   for (int j = 0; j < 8; ++j) {
@@ -449,14 +466,14 @@ __device__ __forceinline__ uint32_t ts_scan(uint16_t* s_cnt, uint32_t extra, uin
 }
 
 // Step 3: every entry to its position (keys and values, swizzled).
-template <typename K, bool VALS>
+template <int T, typename K, bool VALS>
 __device__ __forceinline__ void ts_scatter(const K (&k)[SORT_ITEMS], const uint32_t (&v)[SORT_ITEMS], uint32_t sh,
                                            uint64_t rk, const uint16_t* s_cnt, K* s_keys, uint32_t* s_vals) {
   const uint32_t t = threadIdx.x;
 #pragma unroll
   for (int i = 0; i < SORT_ITEMS; ++i) {
     const uint32_t d = (uint32_t)(k[i] >> sh) & 15u;
-    const uint32_t pos = (uint32_t)s_cnt[d * SORT_THREADS + t] + ((uint32_t)(rk >> (4 * i)) & 15u);
+    const uint32_t pos = (uint32_t)s_cnt[d * T + t] + ((uint32_t)(rk >> (4 * i)) & 15u);
     s_keys[ts_swz<sizeof(K)>(pos)] = k[i];
     if (VALS) s_vals[ts_swz<4>(pos)] = v[i];
   }
@@ -500,28 +517,44 @@ __device__ __forceinline__ void ts_read_blocked(const K* s_keys, const uint32_t*
 // sorts each by the digit in LDS (the tile sort above) and writes every
 // digit's run to the digit base + the chunk's offset + the runs of earlier
 // sub-tiles.  The sub-tile's digit counts come from LDS atomics on per-wave
-// histograms during the first counting pass.  Occupancy: the 32-bit
-// key-only sort takes 30 KB of LDS and 128 VGPRs (four workgroups per CU;
-// at five its loop-invariant addresses spill); with values or 64-bit keys,
-// 46 KB (three per CU), both: 62 KB (two).
+// histograms during the first counting pass.
+//   Sub-tiles hold 16 keys per thread, T threads: 32-bit keys without
+// values take 512 threads (8192 keys), so a digit's run per sub-tile is 32
+// keys, one 128-byte line -- runs of 16 keys cap the pattern at 3.7 TB/s,
+// runs of 32 at 4.8 (a probe of the bare pattern); 64-bit keys reach whole
+// lines with 256 threads.
+//   Occupancy: 32-bit keys, 512 threads: 58 KB of LDS and 128 VGPRs, two
+// workgroups (16 waves) per CU; 64-bit keys or values, 256 threads: 46 KB,
+// three; both: 62 KB, two.
+template <typename K, bool VALS>
+constexpr int ds_threads() { return (sizeof(K) == 4 && !VALS) ? 512 : 256; }
+
+// (the second bound is waves per SIMD: workgroups per CU times T/256)
 template <typename K, bool VALS, bool HASH>
-__global__ __launch_bounds__(256, sizeof(K) == 4 ? (VALS ? 3 : 4) : (VALS ? 2 : 3)) void k_rs_downsweep(
-    const void* __restrict__ in_keys, const uint32_t* __restrict__ in_vals, K* __restrict__ out_keys,
-    uint32_t* __restrict__ out_vals, uint32_t n, uint32_t fp_bits, uint32_t shift, uint32_t chunk, uint32_t G,
-    const uint32_t* __restrict__ hist, const uint32_t* __restrict__ digit_totals) {
-  __shared__ __align__(16) K s_keys[SORT_TILE];
-  __shared__ __align__(16) uint32_t s_vals[VALS ? SORT_TILE : 4];
-  __shared__ __align__(16) uint16_t s_cnt[16 * SORT_THREADS];
-  __shared__ uint32_t s_hist[4 * 256];
+__global__ __launch_bounds__((ds_threads<K, VALS>()), (sizeof(K) == 4 ? (VALS ? 3 : 4) : (VALS ? 2 : 3))) void
+k_rs_downsweep(const void* __restrict__ in_keys, const uint32_t* __restrict__ in_vals, K* __restrict__ out_keys,
+               uint32_t* __restrict__ out_vals, uint32_t n, uint32_t fp_bits, uint32_t shift, uint32_t chunk,
+               uint32_t G, const uint32_t* __restrict__ hist, const uint32_t* __restrict__ digit_totals) {
+  constexpr int T = ds_threads<K, VALS>();
+  constexpr int TILE = T * SORT_ITEMS;
+  constexpr int W = T / 64;
+  __shared__ __align__(16) K s_keys[TILE];
+  __shared__ __align__(16) uint32_t s_vals[VALS ? TILE : 4];
+  __shared__ __align__(16) uint16_t s_cnt[16 * T];
+  __shared__ uint32_t s_hist[W * 256];
   __shared__ uint32_t s_run[256];
   __shared__ uint32_t s_base[256];
-  __shared__ uint32_t s_tmp[4];
+  __shared__ uint32_t s_tmp[W];
   const uint32_t tid = threadIdx.x, wid = tid >> 6;
+  const bool dig = tid < 256;  // the threads that own a digit
 #pragma unroll
-  for (int w = 0; w < 4; ++w) s_hist[w * 256 + tid] = 0;
+  for (int w = 0; w < W * 256 / T; ++w) s_hist[w * T + tid] = 0;
   // digit bases: exclusive scan of the 256 digit totals (every workgroup
   // does it; it saves a launch), plus this chunk's offset
-  s_run[tid] = block256_excl_scan(digit_totals[tid], s_tmp, nullptr) + hist[(uint64_t)tid * G + blockIdx.x];
+  {
+    const uint32_t base = block_excl_scan<T>(dig ? digit_totals[tid] : 0u, s_tmp);
+    if (dig) s_run[tid] = base + hist[(uint64_t)tid * G + blockIdx.x];
+  }
   const uint64_t c0 = (uint64_t)blockIdx.x * chunk;
   const uint64_t c1 = c0 + chunk < n ? c0 + chunk : (uint64_t)n;
   const bool kvec = (reinterpret_cast<uintptr_t>(in_keys) & 15u) == 0;
@@ -530,47 +563,49 @@ __global__ __launch_bounds__(256, sizeof(K) == 4 ? (VALS ? 3 : 4) : (VALS ? 2 : 
   uint32_t v[SORT_ITEMS];
   uint64_t rk;
   if (c0 < c1) {
-    const uint32_t tn = (uint32_t)(c1 - c0 < (uint64_t)SORT_TILE ? c1 - c0 : (uint64_t)SORT_TILE);
+    const uint32_t tn = (uint32_t)(c1 - c0 < (uint64_t)TILE ? c1 - c0 : (uint64_t)TILE);
     ts_load_keys<K, HASH>(in_keys, c0, tn, fp_bits, kvec, k);
     if (VALS) ts_load_vals(in_vals, c0, tn, vvec, v);
   }
-  for (uint64_t t0 = c0; t0 < c1; t0 += SORT_TILE) {
-    const uint32_t tile_n = (uint32_t)(c1 - t0 < (uint64_t)SORT_TILE ? c1 - t0 : (uint64_t)SORT_TILE);
+  for (uint64_t t0 = c0; t0 < c1; t0 += TILE) {
+    const uint32_t tile_n = (uint32_t)(c1 - t0 < (uint64_t)TILE ? c1 - t0 : (uint64_t)TILE);
     // pass 1 (low nibble) and the sub-tile's digit histogram
-    ts_count(k, shift, rk, s_cnt);
+    ts_count<T>(k, shift, rk, s_cnt);
 #pragma unroll
     for (int i = 0; i < SORT_ITEMS; ++i) atomicAdd(&s_hist[wid * 256 + ((uint32_t)(k[i] >> shift) & 255u)], 1u);
     __syncthreads();
     uint32_t total = 0;
+    if (dig) {
 #pragma unroll
-    for (int w = 0; w < 4; ++w) {
-      total += s_hist[w * 256 + tid];
-      s_hist[w * 256 + tid] = 0;  // the next sub-tile's counts start after several barriers
+      for (int w = 0; w < W; ++w) {
+        total += s_hist[w * 256 + tid];
+        s_hist[w * 256 + tid] = 0;  // the next sub-tile's counts start after several barriers
+      }
     }
-    const uint32_t lstart = ts_scan(s_cnt, total, s_tmp);
-    s_base[tid] = s_run[tid] - lstart;  // output of sorted position p with digit tid: s_base + p
+    const uint32_t lstart = ts_scan<T>(s_cnt, total, s_tmp);
+    if (dig) s_base[tid] = s_run[tid] - lstart;  // output of sorted position p with digit tid: s_base + p
     __syncthreads();
-    ts_scatter<K, VALS>(k, v, shift, rk, s_cnt, s_keys, s_vals);
+    ts_scatter<T, K, VALS>(k, v, shift, rk, s_cnt, s_keys, s_vals);
     __syncthreads();
     ts_read_blocked<K, VALS>(s_keys, s_vals, k, v);
     // pass 2 (high nibble)
-    ts_count(k, shift + 4, rk, s_cnt);
+    ts_count<T>(k, shift + 4, rk, s_cnt);
     __syncthreads();
-    (void)ts_scan(s_cnt, 0u, s_tmp);
+    (void)ts_scan<T>(s_cnt, 0u, s_tmp);
     __syncthreads();
-    ts_scatter<K, VALS>(k, v, shift + 4, rk, s_cnt, s_keys, s_vals);
+    ts_scatter<T, K, VALS>(k, v, shift + 4, rk, s_cnt, s_keys, s_vals);
     __syncthreads();
     // the next sub-tile's loads go out before this one's stores
-    const uint64_t tn0 = t0 + SORT_TILE;
+    const uint64_t tn0 = t0 + TILE;
     if (tn0 < c1) {
-      const uint32_t tn = (uint32_t)(c1 - tn0 < (uint64_t)SORT_TILE ? c1 - tn0 : (uint64_t)SORT_TILE);
+      const uint32_t tn = (uint32_t)(c1 - tn0 < (uint64_t)TILE ? c1 - tn0 : (uint64_t)TILE);
       ts_load_keys<K, HASH>(in_keys, tn0, tn, fp_bits, kvec, k);
       if (VALS) ts_load_vals(in_vals, tn0, tn, vvec, v);
     }
     // runs of equal digits to their outputs; consecutive lanes, consecutive positions
 #pragma unroll 4
     for (int j = 0; j < SORT_ITEMS; ++j) {
-      const uint32_t p = tid + (uint32_t)j * SORT_THREADS;
+      const uint32_t p = tid + (uint32_t)j * T;
       if (p < tile_n) {
         const K key = s_keys[ts_swz<sizeof(K)>(p)];
         const uint32_t gpos = s_base[(uint32_t)(key >> shift) & 255u] + p;
@@ -578,7 +613,7 @@ __global__ __launch_bounds__(256, sizeof(K) == 4 ? (VALS ? 3 : 4) : (VALS ? 2 : 
         if (VALS) out_vals[gpos] = s_vals[ts_swz<4>(p)];
       }
     }
-    s_run[tid] += total;
+    if (dig) s_run[tid] += total;
     // no barrier: the next sub-tile writes s_base and s_keys only after its
     // first barrier, which every thread reaches after these reads
   }
@@ -740,17 +775,17 @@ gqf_status radix_sort(hipStream_t s, Arena& ar, const void* in, bool hash, uint3
     const double pass_bytes = (double)n * (kin + sizeof(K) + (with_vals ? ((vsrc ? 4.0 : 0.0) + 4.0) : 0.0));
     if (with_vals) {
       if (first_hash)
-        GQF_LAUNCH_B(pass_bytes, (k_rs_downsweep<K, true, true>), dim3(plan.G), dim3(SORT_THREADS), 0, s, src, vsrc,
+        GQF_LAUNCH_B(pass_bytes, (k_rs_downsweep<K, true, true>), dim3(plan.G), dim3(ds_threads<K, true>()), 0, s, src, vsrc,
                      dst, vdst, n, fp_bits, shift, plan.chunk, plan.G, hist, dbase);
       else
-        GQF_LAUNCH_B(pass_bytes, (k_rs_downsweep<K, true, false>), dim3(plan.G), dim3(SORT_THREADS), 0, s, src, vsrc,
+        GQF_LAUNCH_B(pass_bytes, (k_rs_downsweep<K, true, false>), dim3(plan.G), dim3(ds_threads<K, true>()), 0, s, src, vsrc,
                      dst, vdst, n, fp_bits, shift, plan.chunk, plan.G, hist, dbase);
     } else {
       if (first_hash)
-        GQF_LAUNCH_B(pass_bytes, (k_rs_downsweep<K, false, true>), dim3(plan.G), dim3(SORT_THREADS), 0, s, src,
+        GQF_LAUNCH_B(pass_bytes, (k_rs_downsweep<K, false, true>), dim3(plan.G), dim3(ds_threads<K, false>()), 0, s, src,
                      nullptr, dst, nullptr, n, fp_bits, shift, plan.chunk, plan.G, hist, dbase);
       else
-        GQF_LAUNCH_B(pass_bytes, (k_rs_downsweep<K, false, false>), dim3(plan.G), dim3(SORT_THREADS), 0, s, src,
+        GQF_LAUNCH_B(pass_bytes, (k_rs_downsweep<K, false, false>), dim3(plan.G), dim3(ds_threads<K, false>()), 0, s, src,
                      nullptr, dst, nullptr, n, fp_bits, shift, plan.chunk, plan.G, hist, dbase);
     }
     src = dst;
