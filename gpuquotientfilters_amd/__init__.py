@@ -128,6 +128,8 @@ _SIGS = {
     "gqf_sqf_get_hash": (C.c_int, []),
     "gqf_sqf_set_insert_path": (_st, [C.c_int]),
     "gqf_sqf_get_insert_path": (C.c_int, []),
+    "gqf_rsqf_set_insert_path": (_st, [C.c_int]),
+    "gqf_rsqf_get_insert_path": (C.c_int, []),
     "gqf_stream_create": (_st, [C.POINTER(_vp)]),
     "gqf_stream_destroy": (_st, [_vp]),
     "gqf_stream_workspace_bytes": (_st, [_vp, C.POINTER(C.c_size_t)]),
@@ -250,6 +252,18 @@ def set_sqf_insert_path(path):
 
 def sqf_insert_path():
     return lib().gqf_sqf_get_insert_path()
+
+
+def set_rsqf_insert_path(path):
+    """How this thread's RSQF inserts update the table: INSERT_AUTO (default)
+    or INSERT_LOCAL (in place, window by window, falling back to the rebuild
+    where a window cannot be rewritten alone), INSERT_REBUILD (decode and
+    rebuild every slot).  The resulting tables are the same."""
+    check(lib().gqf_rsqf_set_insert_path(int(path)))
+
+
+def rsqf_insert_path():
+    return lib().gqf_rsqf_get_insert_path()
 
 
 def sqf_hash():

@@ -106,26 +106,29 @@ static inline RbPlan rb_plan(uint64_t N, uint32_t min_per = 1) {
   return p;
 }
 
-// Block-wide scan of the per-thread states: *ex_out = exclusive prefix of the
-// calling thread within the block, returns the block aggregate.  s_wave:
-// RB_WAVES + 1 entries (wave aggregates, then their prefixes and the total).
-__device__ __forceinline__ RS rs_block(RS a, RS* ex_out, uint4* s_wave) {
+// Block-wide scan of the per-thread states (NW waves): *ex_out = exclusive
+// prefix of the calling thread within the block, returns the block
+// aggregate.  s_wave: NW + 1 entries (wave aggregates, then their prefixes
+// and the total).  Two barriers.
+template <int NW>
+__device__ __forceinline__ RS rs_block_n(RS a, RS* ex_out, uint4* s_wave) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   RS wt;
   const RS ex = rs_wave_excl(a, &wt);
   if (lane == 0) s_wave[wid] = rs_pack(wt);
   __syncthreads();
   if (wid == 0) {
-    const RS x = lane < RB_WAVES ? rs_unpack(s_wave[lane]) : rs_none();
+    const RS x = lane < NW ? rs_unpack(s_wave[lane]) : rs_none();
     RS tot;
     const RS xe = rs_wave_excl(x, &tot);
-    if (lane < RB_WAVES) s_wave[lane] = rs_pack(xe);
-    if (lane == 0) s_wave[RB_WAVES] = rs_pack(tot);
+    if (lane < NW) s_wave[lane] = rs_pack(xe);
+    if (lane == 0) s_wave[NW] = rs_pack(tot);
   }
   __syncthreads();
   *ex_out = rs_combine(rs_unpack(s_wave[wid]), ex);
-  return rs_unpack(s_wave[RB_WAVES]);
+  return rs_unpack(s_wave[NW]);
 }
+__device__ __forceinline__ RS rs_block(RS a, RS* ex_out, uint4* s_wave) { return rs_block_n<RB_WAVES>(a, ex_out, s_wave); }
 
 // Exclusive scan of the G chunk states (G <= RB_MAX_CHUNKS) in two launches
 // of ceil(G / 1024) workgroups, one state per thread with coalesced loads:

@@ -958,6 +958,487 @@ __global__ __launch_bounds__(64) void k_rr_tail(uint8_t* __restrict__ base, cons
     meta[4 * (size_t)j + 2] = (L - 64 * j + 1) & 0xFFu;
 }
 
+// ---------------------------------------------------------------- windowed insert
+// An insert that rewrites the table window by window in place instead of
+// decoding it to a sorted array and rebuilding.  Window w is RD_BLOCKS
+// blocks: the quotients and the slots [4096 w, 4096 w + 4096).
+//   E_w  the stored entries of the window's quotients (the runs of its
+//        occupied quotients, which may end in window w + 1);
+//   B_w  the batch's fingerprints with those quotients (B is sorted);
+//   M_w  their stable merge, stored entries first on ties (as the rebuild's
+//        merge of the decoded table with the batch).
+// The canonical layout of the whole is the location scan over the windows'
+// merged entries in window order, so:
+//   k_rd_pop + scan  occupied and run-end counts before each window: d_w =
+//                    the runs of earlier quotients still open at its first
+//                    slot (their ends are its first d_w run ends);
+//   k_wm_bounds      each window's range of B;
+//   k_wm_sum         each window's scan state, from its entries' quotient
+//                    counts (run ends and occupieds only) and B_w's;
+//   rb_scan          the state before each window: its incoming location;
+//   k_wm_check       the blocks each window hands over (below);
+//   k_wm_save        ... copied aside and cleared;
+//   k_wm_write       each window decodes E_w, merges B_w, lays M_w out on top
+//                    of its incoming state and writes its blocks.
+// Decoding follows the rule of k_rd_emit: image slot s lies in run
+// k = 1 + (run ends before s); the window's j-th run (k = d_w + j) is that
+// of its j-th occupied quotient Q, and s holds an entry iff Q <= s.
+//   In place, the first blocks of window x can be read and written by two
+// workgroups: window x - 1's new entries land there (new incoming location)
+// and its old runs ended there (old incoming location).  Below
+// h_x = max(old, new incoming location of x) -- at most WM_SIDE blocks --
+// k_wm_save copies the blocks' run ends and remainders aside and clears
+// them; k_wm_write reads those blocks from the copy and ORs into them, while
+// every other block has one reader and one writer, its own window, which
+// stores it whole.  Occupieds and offsets belong to their window alone.
+//   A window with more than WM_CAP merged entries, a run end that is not in
+// its two-window image, or more than WM_SIDE blocks to hand over sends the
+// call to the decode-and-rebuild path: the flag ctr[CTR_MISC] is raised
+// before anything is written, and every writer checks it.
+constexpr uint32_t WM_THREADS = RB_THREADS;        // k_wm_write: 16 image slots per thread
+constexpr uint32_t WM_ITEMS = 16;                  // merged entries per thread
+constexpr uint32_t WM_CAP = WM_THREADS * WM_ITEMS;  // 8192 merged entries per window
+constexpr uint32_t WM_IMG = 2 * RD_BLOCKS;          // image: windows w and w + 1 (128 blocks)
+constexpr uint32_t WM_SIDE = 8;                     // blocks a window hands over at most (512 slots)
+constexpr uint32_t WS_THREADS = 256;                // k_wm_sum: 16 quotients, 32 image slots per thread
+
+// n copies of quotient fq appended to a scan state (rs_push n times).
+__device__ __forceinline__ void rs_push_run(RS& s, uint32_t fq, uint32_t n) {
+  s.lp = add_sat(s.lp > fq ? s.lp : fq, n);
+  s.c += n;
+  s.fq1 = fq + 1u;
+}
+
+// Entries per window-relative quotient, 16 bits each (a window that is not
+// flagged holds fewer than 2^16 entries of one quotient).
+__device__ __forceinline__ void wm_cnt_add(uint32_t* s_cnt, uint32_t p, uint32_t n) {
+  atomicAdd(&s_cnt[p >> 1], n << (16 * (p & 1)));
+}
+__device__ __forceinline__ uint32_t wm_cnt_get(const uint32_t* s_cnt, uint32_t p) {
+  return (s_cnt[p >> 1] >> (16 * (p & 1))) & 0xFFFFu;
+}
+
+// Each window's range of B: bstart[x] = first fingerprint with quotient >=
+// 4096 x (x = 0 .. W).  Also resets the counters the insert uses.
+template <typename K>
+__global__ __launch_bounds__(256) void k_wm_bounds(const K* __restrict__ B, uint32_t n, uint32_t r, uint32_t W,
+                                                   uint32_t* __restrict__ bstart, uint32_t* __restrict__ ctr) {
+  if (blockIdx.x == 0 && threadIdx.x >= CTR_OP && threadIdx.x < CTR_TOTAL) ctr[threadIdx.x] = 0u;
+  const uint32_t x = blockIdx.x * 256 + threadIdx.x;
+  if (x > W) return;
+  const uint64_t key = ((uint64_t)x * RD_SLOTS) << r;
+  uint32_t lo = 0, hi = n;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if ((uint64_t)B[mid] < key) lo = mid + 1;
+    else hi = mid;
+  }
+  bstart[x] = lo;
+}
+
+// Window w's scan state (agg[w]) from the entry counts of its quotients;
+// wold[w] = its old incoming location (1 + the slot of the d_w-th run end, 0
+// when d_w = 0).  Thread t: quotients [16 t, 16 t + 16), image slots
+// [32 t, 32 t + 32).  The run ends at or after the window's first slot are
+// ranked (s_rend[i] = slot of the (d_w + i)-th, i = 0 .. O), so the j-th
+// occupied quotient p holds the slots [max(p, s_rend[j-1] + 1), s_rend[j]]
+// (the decode rule of k_rd_emit); the batch adds its quotient counts.
+// wp: exclusive occupied / run-end prefixes.
+template <typename K>
+__global__ __launch_bounds__(WS_THREADS) void k_wm_sum(RsqfDev f, const uint32_t* __restrict__ wp,
+                                                       const K* __restrict__ B, const uint32_t* __restrict__ bstart,
+                                                       uint4* __restrict__ agg, uint32_t* __restrict__ wold,
+                                                       uint32_t* __restrict__ ctr) {
+  __shared__ uint32_t s_cnt[RD_SLOTS / 2];   // batch entries per quotient, 16 bits each
+  __shared__ int16_t s_rend[RD_SLOTS + 1];   // image slot of the (d + i)-th run end; -1: before the window
+  __shared__ uint32_t s_tmp[WS_THREADS / 64];
+  __shared__ uint4 s_wave[WS_THREADS / 64 + 1];
+  const uint32_t w = blockIdx.x, t = threadIdx.x;
+  const uint64_t bw = (uint64_t)w * RD_BLOCKS;
+  const uint64_t qb = bw + (t >> 2), eb = bw + (t >> 1);
+  const uint32_t o16 = qb < f.nb ? (uint32_t)(f.occ((uint32_t)qb) >> (16 * (t & 3))) & 0xFFFFu : 0u;
+  const uint32_t e32 = eb < f.nb ? (uint32_t)(f.ends((uint32_t)eb) >> (32 * (t & 1))) : 0u;
+  const uint32_t b0 = bstart[w], b1 = bstart[w + 1];
+  const uint32_t d = wp[2 * (size_t)w] - wp[2 * (size_t)w + 1];
+  for (uint32_t i = t; i < RD_SLOTS / 2; i += WS_THREADS) s_cnt[i] = 0u;
+  if (t == 0) s_rend[0] = -1;  // d = 0: the previous run ended before the window
+  // occupied ranks (high half) and run-end ranks (low half): at most 4096 and 8192
+  const uint32_t pk = block_excl_scan<WS_THREADS>(((uint32_t)__popc(o16) << 16) | (uint32_t)__popc(e32), s_tmp);
+  uint32_t tot = 0;
+#pragma unroll
+  for (int i = 0; i < (int)(WS_THREADS / 64); ++i) tot += s_tmp[i];
+  const uint32_t O = tot >> 16, E = tot & 0xFFFFu;
+  {
+    uint32_t k = pk & 0xFFFFu;  // run ends before this thread's first slot
+    for (uint32_t x = e32; x; x &= x - 1u) {
+      ++k;  // rank of this run end
+      if (k >= d && k - d <= O) s_rend[k - d] = (int16_t)(32 * t + (uint32_t)__builtin_ctz(x));
+    }
+  }
+  if (b1 - b0 <= WM_CAP) {
+    const K kbase = (K)((uint64_t)w * RD_SLOTS) << f.r;
+    for (uint32_t i = b0 + t; i < b1; i += WS_THREADS) wm_cnt_add(s_cnt, (uint32_t)((B[i] - kbase) >> f.r), 1u);
+  }
+  __syncthreads();
+  const bool bad = d + O > E;  // a run end the image does not hold
+  RS x = rs_none();
+  uint32_t j = pk >> 16;  // occupied quotients before this thread's first one
+#pragma unroll
+  for (uint32_t i = 0; i < 16; ++i) {
+    const uint32_t p = 16 * t + i;
+    uint32_t n = wm_cnt_get(s_cnt, p);
+    if ((o16 >> i) & 1u) {
+      ++j;
+      if (!bad) {
+        const int32_t e = s_rend[j], pe = s_rend[j - 1];
+        const int32_t st = (int32_t)p > pe + 1 ? (int32_t)p : pe + 1;
+        if (e >= st) n += (uint32_t)(e - st + 1);
+      }
+    }
+    if (n) rs_push_run(x, w * RD_SLOTS + p, n);
+  }
+  RS ex;
+  const RS all = rs_block_n<WS_THREADS / 64>(x, &ex, s_wave);
+  if (t == 0) {
+    agg[w] = rs_pack(all);
+    wold[w] = d && !bad ? w * RD_SLOTS + (uint32_t)(s_rend[0] + 1) : 0u;
+    if (bad || all.c > WM_CAP || b1 - b0 > WM_CAP) ctr[CTR_MISC] = 1u;
+  }
+}
+
+// The handover bound of every window (wsave), from the scanned states (each
+// window's exclusive prefix: lp = its new incoming location) and wold; more
+// than WM_SIDE blocks to hand over raises the fallback flag.
+__global__ __launch_bounds__(256) void k_wm_check(const uint4* __restrict__ pre, const uint32_t* __restrict__ wold,
+                                                  uint32_t W, uint32_t* __restrict__ wsave,
+                                                  uint32_t* __restrict__ ctr) {
+  const uint32_t x = blockIdx.x * 256 + threadIdx.x;
+  if (x >= W) return;
+  const uint32_t lp = rs_unpack(pre[x]).lp, o = wold[x];
+  const uint64_t h = lp > o ? lp : o;
+  if (h > (uint64_t)x * RD_SLOTS + 64 * WM_SIDE) ctr[CTR_MISC] = 1u;
+  wsave[x] = (uint32_t)h;
+}
+
+// Blocks of window w below its handover bound: run ends and remainders to
+// the side copy (WM_SIDE blocks per window), then cleared in the table.
+__global__ __launch_bounds__(64) void k_wm_save(uint8_t* __restrict__ base, uint32_t nb, uint32_t r,
+                                                const uint32_t* __restrict__ wsave, uint64_t* __restrict__ side_end,
+                                                uint32_t* __restrict__ side_rem, const uint32_t* __restrict__ ctr) {
+  if (ctr[CTR_ERR] || ctr[CTR_MISC]) return;
+  const uint32_t w = blockIdx.x, t = threadIdx.x;
+  const uint64_t s0 = (uint64_t)w * RD_SLOTS, h = wsave[w];
+  if (h <= s0) return;
+  const uint64_t bw = (uint64_t)w * RD_BLOCKS;
+  uint64_t nsv = (h - s0 + 63) / 64;  // <= WM_SIDE: k_wm_check
+  if (nsv > WM_SIDE) nsv = WM_SIDE;
+  if (bw + nsv > nb) nsv = nb - bw;
+  uint64_t* meta = reinterpret_cast<uint64_t*>(base);
+  uint32_t* remg = reinterpret_cast<uint32_t*>(base + rsqf_meta_bytes(nb));
+  const uint32_t rw = 2 * r;
+  for (uint32_t i = t; i < nsv; i += 64) {
+    side_end[(uint64_t)w * WM_SIDE + i] = meta[4 * (bw + i) + 1];
+    meta[4 * (bw + i) + 1] = 0ull;
+  }
+  for (uint32_t i = t; i < nsv * rw; i += 64) {
+    side_rem[(uint64_t)w * WM_SIDE * rw + i] = remg[bw * rw + i];
+    remg[bw * rw + i] = 0u;
+  }
+}
+
+// Image-relative slot s's remainder in an image of remainder words (RW per block).
+template <int R, uint32_t RW>
+__device__ __forceinline__ uint32_t wm_get_rem(const uint32_t* s_rem, uint32_t s, uint32_t rr) {
+  if (R == 8) return reinterpret_cast<const uint8_t*>(s_rem)[s];
+  if (R == 16) return reinterpret_cast<const uint16_t*>(s_rem)[s];
+  const uint32_t bit = (s & 63) * rr;
+  const uint32_t* blk = s_rem + (s >> 6) * RW;
+  const uint32_t lo = blk[bit >> 5] >> (bit & 31);
+  const uint32_t hi = (bit & 31) + rr > 32 ? blk[(bit >> 5) + 1] << (32 - (bit & 31)) : 0u;
+  return (lo | hi) & ((1u << rr) - 1u);
+}
+template <int R, uint32_t RW>
+__device__ __forceinline__ void wm_put_rem(uint32_t* s_rem, uint32_t s, uint32_t rr, uint32_t v) {
+  if (R == 8) {
+    reinterpret_cast<uint8_t*>(s_rem)[s] = (uint8_t)v;
+  } else if (R == 16) {
+    reinterpret_cast<uint16_t*>(s_rem)[s] = (uint16_t)v;
+  } else {
+    const uint32_t bit = (s & 63) * rr;
+    uint32_t* blk = s_rem + (s >> 6) * RW;
+    atomicOr(&blk[bit >> 5], v << (bit & 31));
+    if ((bit & 31) + rr > 32) atomicOr(&blk[(bit >> 5) + 1], v >> (32 - (bit & 31)));
+  }
+}
+
+// Entries of a sorted LDS array at most v.
+__device__ __forceinline__ uint32_t wm_rank_le(const uint32_t* a, uint32_t n, uint32_t v) {
+  uint32_t lo = 0, hi = n;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (a[mid] <= v) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Window w: decode E_w (saved blocks from the side copy), merge B_w, lay M_w
+// out from the window's incoming state and write the window's blocks (plus
+// ORs into the saved blocks of window w + 1 its entries reach).  Entries are
+// held window-relative: (quotient - 4096 w) << r | remainder.  Every input
+// load goes out at the start: table and side words are both read for the
+// blocks that may be saved, and chosen once the bounds are in.
+template <typename K, int R>
+__global__ __launch_bounds__(WM_THREADS, (R == 8 ? 6 : 4)) void k_wm_write(uint8_t* __restrict__ base, uint32_t nb, uint32_t r,
+                                                            uint32_t W, const uint32_t* __restrict__ wp,
+                                                            const K* __restrict__ B,
+                                                            const uint32_t* __restrict__ bstart,
+                                                            const uint4* __restrict__ pre,
+                                                            const uint32_t* __restrict__ wsave,
+                                                            const uint64_t* __restrict__ side_end,
+                                                            const uint32_t* __restrict__ side_rem,
+                                                            const uint32_t* __restrict__ ctr) {
+  if (ctr[CTR_ERR] || ctr[CTR_MISC]) return;
+  constexpr uint32_t RW = R ? 2 * (uint32_t)R : 32;  // remainder words per image block
+  constexpr uint32_t RL = (WM_IMG * RW + WM_THREADS - 1) / WM_THREADS;
+  __shared__ __align__(16) uint32_t s_rem[WM_IMG * RW];
+  __shared__ __align__(16) uint32_t s_m[WM_CAP];  // E_w from the front, B_w's low words at the back
+  __shared__ uint32_t s_u[RD_SLOTS / 2];          // occupied list; merge bitmap; neighbours' quotients
+  __shared__ uint32_t s_end[WM_IMG * 2];
+  __shared__ uint32_t s_occ[RD_BLOCKS * 2];
+  __shared__ int32_t s_L[RD_BLOCKS];
+  __shared__ uint32_t s_tmp[WM_THREADS / 64];
+  __shared__ uint4 s_wave[RB_WAVES + 1];
+  const uint32_t w = blockIdx.x, t = threadIdx.x, lane = t & 63, wid = t >> 6;
+  const uint32_t rr = R ? (uint32_t)R : r, rw = 2 * rr;
+  uint64_t* meta = reinterpret_cast<uint64_t*>(base);
+  uint32_t* remg = reinterpret_cast<uint32_t*>(base + rsqf_meta_bytes(nb));
+  const uint64_t bw = (uint64_t)w * RD_BLOCKS;
+  const bool next = w + 1 < W;
+  // ---- loads: bounds and states, then table and side words, then the batch
+  const uint64_t h0 = wsave[w], h1 = next ? wsave[w + 1] : 0u;
+  const uint32_t b0 = bstart[w], nB = bstart[w + 1] - b0;
+  const uint32_t d = wp[2 * (size_t)w] - wp[2 * (size_t)w + 1];
+  const RS X = rs_unpack(pre[w]);
+  const uint32_t ib = t >> 2, sh = 16 * (t & 3);  // this thread's image slots [16 t, 16 t + 16): block ib
+  uint64_t om = 0, et = 0, es = 0;
+  if (bw + ib < nb) {
+    if (ib < RD_BLOCKS) {
+      om = meta[4 * (bw + ib)];
+      et = meta[4 * (bw + ib) + 1];
+      if (ib < WM_SIDE) es = side_end[(uint64_t)w * WM_SIDE + ib];
+    } else if (next && ib - RD_BLOCKS < WM_SIDE) {
+      es = side_end[(uint64_t)(w + 1) * WM_SIDE + (ib - RD_BLOCKS)];
+    }
+  }
+  uint32_t tv[RL], sv[RL];
+#pragma unroll
+  for (uint32_t k = 0; k < RL; ++k) {
+    const uint32_t i = t + k * WM_THREADS, j = i / rw;
+    tv[k] = 0u;
+    sv[k] = 0u;
+    if (i < WM_IMG * rw && bw + j < nb) {
+      if (j < RD_BLOCKS) {
+        tv[k] = remg[bw * rw + i];
+        if (j < WM_SIDE) sv[k] = side_rem[(uint64_t)w * WM_SIDE * rw + i];
+      } else if (next && j - RD_BLOCKS < WM_SIDE) {
+        sv[k] = side_rem[(uint64_t)(w + 1) * WM_SIDE * rw + (i - RD_BLOCKS * rw)];
+      }
+    }
+  }
+  // the batch's low words straight to LDS (the difference to the window base fits them)
+  const uint32_t nBc = nB < WM_CAP ? nB : WM_CAP;
+  {
+    const uint32_t* B32 = reinterpret_cast<const uint32_t*>(B);
+    constexpr uint32_t KW = sizeof(K) / 4;
+    for (uint32_t k = 0; k * WM_THREADS < nBc; ++k) {
+      const uint32_t bs = (k * RB_WAVES + wid) * 64;
+      if (bs + lane < nBc)
+        __builtin_amdgcn_global_load_lds(B32 + KW * ((uint64_t)b0 + bs + lane), s_m + (WM_CAP - nBc) + bs, 4, 0, 0);
+    }
+  }
+  auto saved = [&](uint32_t i) { return 64 * (bw + i) < (i < RD_BLOCKS ? h0 : h1); };
+#pragma unroll
+  for (uint32_t k = 0; k < RL; ++k) {
+    const uint32_t i = t + k * WM_THREADS;
+    if (i < WM_IMG * rw) {
+      const uint32_t j = i / rw;
+      s_rem[j * RW + (i - j * rw)] = saved(j) ? sv[k] : tv[k];
+    }
+  }
+  const uint32_t o16 = ib < RD_BLOCKS && t < 256 ? (uint32_t)(om >> sh) & 0xFFFFu : 0u;
+  const uint32_t e16 = (uint32_t)((saved(ib) ? es : et) >> sh) & 0xFFFFu;
+  // ---- decode: occupied ranks and run-end ranks, the occupied list, the entries
+  const uint32_t pk = block_excl_scan<WM_THREADS>(((uint32_t)__popc(o16) << 16) | (uint32_t)__popc(e16), s_tmp);
+  uint32_t tot = 0;
+#pragma unroll
+  for (int i = 0; i < (int)(WM_THREADS / 64); ++i) tot += s_tmp[i];
+  const uint32_t O = tot >> 16;
+  uint16_t* s_list = reinterpret_cast<uint16_t*>(s_u);
+  {
+    uint32_t g = pk >> 16;
+    for (uint32_t x = o16; x; x &= x - 1u) s_list[g++] = (uint16_t)(16 * t + (uint32_t)__builtin_ctz(x));
+  }
+  __syncthreads();
+  uint32_t qv[16];
+  uint32_t m = 0;
+  {
+    uint32_t k = pk & 0xFFFFu;
+#pragma unroll
+    for (uint32_t i = 0; i < 16; ++i) {
+      const uint32_t s = 16 * t + i;
+      const uint32_t Q = (k >= d && k - d < O) ? s_list[k - d] : 0xFFFFu;
+      qv[i] = Q;
+      if (Q <= s) m |= 1u << i;
+      k += (e16 >> i) & 1u;
+    }
+  }
+  const uint32_t posE = block_excl_scan<WM_THREADS>((uint32_t)__popc(m), s_tmp);
+  uint32_t nE = 0;
+#pragma unroll
+  for (int i = 0; i < (int)(WM_THREADS / 64); ++i) nE += s_tmp[i];
+  if (nE + nB > WM_CAP) {  // unreachable: k_wm_sum counted the same entries and raised the flag
+    rb_stage_wait();
+    return;
+  }
+  const uint32_t nM = nE + nB;
+  {
+    uint32_t p = posE;
+#pragma unroll
+    for (uint32_t i = 0; i < 16; ++i)
+      if ((m >> i) & 1u) s_m[p++] = (qv[i] << rr) | wm_get_rem<R, RW>(s_rem, 16 * t + i, rr);
+  }
+  // ---- stable merge: each batch entry marks its output position in a bitmap
+  const uint32_t kb32 = (uint32_t)((uint64_t)w * RD_SLOTS << rr);
+  const uint32_t* sB = s_m + (WM_CAP - nB);
+  uint32_t* s_bmp = s_u;
+  uint32_t* s_bpre = s_u + WM_CAP / 32;
+  rb_stage_wait();  // the batch words are in LDS
+  __syncthreads();  // every entry is written, the occupied list is no longer read
+  uint32_t v[WM_ITEMS];
+  const uint32_t i0 = WM_ITEMS * t;
+  if (nB) {
+    if (t < WM_CAP / 32) s_bmp[t] = 0u;
+    __syncthreads();
+    for (uint32_t j = t; j < nB; j += WM_THREADS) {
+      const uint32_t p = j + wm_rank_le(s_m, nE, sB[j] - kb32);
+      atomicOr(&s_bmp[p >> 5], 1u << (p & 31));
+    }
+    __syncthreads();
+    if (t < 64) {
+      uint32_t c[4], sum = 0;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        c[i] = (uint32_t)__popc(s_bmp[4 * t + i]);
+        sum += c[i];
+      }
+      uint32_t run = wave_incl_add(sum) - sum;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        s_bpre[4 * t + i] = run;
+        run += c[i];
+      }
+    }
+    __syncthreads();
+    const uint32_t wd = s_bmp[i0 >> 5];
+    uint32_t bi = s_bpre[i0 >> 5] + (uint32_t)__popc(wd & ((1u << (i0 & 31)) - 1u));
+    uint32_t ai = i0 - bi;
+    const uint32_t bits = wd >> (i0 & 31);
+    // branch-free: one LDS read per entry at the chosen index
+#pragma unroll
+    for (uint32_t j = 0; j < WM_ITEMS; ++j) {
+      const uint32_t isb = (bits >> j) & 1u;
+      const uint32_t at = isb ? (WM_CAP - nB) + bi : ai;
+      v[j] = i0 + j < nM ? s_m[at < WM_CAP ? at : 0u] - (isb ? kb32 : 0u) : 0u;
+      bi += isb;
+      ai += isb ^ 1u;
+    }
+  } else {
+#pragma unroll
+    for (uint32_t j = 0; j < WM_ITEMS; ++j) v[j] = i0 + j < nM ? s_m[i0 + j] : 0u;
+  }
+  const uint32_t cnt = nM > i0 ? (nM - i0 < WM_ITEMS ? nM - i0 : WM_ITEMS) : 0u;
+  // ---- locations: this thread's entries on top of the window's incoming state
+  RS a = rs_none();
+#pragma unroll
+  for (uint32_t j = 0; j < WM_ITEMS; ++j)
+    if (j < cnt) rs_push(a, w * RD_SLOTS + (v[j] >> rr));
+  __syncthreads();  // the bitmap is no longer read: the neighbours' quotients take its place
+  uint16_t* s_q0 = reinterpret_cast<uint16_t*>(s_u);
+  uint16_t* s_q1 = s_q0 + WM_THREADS;
+  if (cnt) {
+    s_q0[t] = (uint16_t)(v[0] >> rr);
+    s_q1[t] = (uint16_t)(v[cnt - 1] >> rr);
+  }
+  for (uint32_t i = t; i < WM_IMG * RW; i += WM_THREADS) s_rem[i] = 0u;
+  if (t < WM_IMG * 2) s_end[t] = 0u;
+  if (t < RD_BLOCKS * 2) s_occ[t] = 0u;
+  RS ex;
+  (void)rs_block(a, &ex, s_wave);  // two barriers: the cleared images and the neighbours are visible after it
+  const RS me = rs_combine(X, ex);
+  const uint32_t s0 = w * RD_SLOTS;
+  const uint32_t rmask = (1u << rr) - 1u;
+  constexpr uint32_t NONE = RD_SLOTS + 64;  // past every window-relative quotient
+  int32_t prev = (int32_t)me.lp - 1;
+  uint32_t pq = i0 > 0 && cnt ? s_q1[t - 1] : NONE;  // quotient of the entry before
+#pragma unroll
+  for (uint32_t j = 0; j < WM_ITEMS; ++j) {
+    if (j < cnt) {
+      const uint32_t fqr = v[j] >> rr;
+      const int32_t fq = (int32_t)(s0 + fqr);
+      const int32_t lc = fq > prev + 1 ? fq : prev + 1;
+      const uint32_t rel = (uint32_t)lc - s0;  // < 4096 + 64 WM_SIDE: k_wm_check
+      const uint32_t nq = j + 1 < cnt ? v[j + 1] >> rr : (i0 + j + 1 < nM ? s_q0[t + 1] : NONE);
+      if (rel < WM_IMG * 64) {
+        wm_put_rem<R, RW>(s_rem, rel, rr, v[j] & rmask);
+        if (fqr != pq) atomicOr(&s_occ[fqr >> 5], 1u << (fqr & 31));
+        if (nq != fqr) {
+          atomicOr(&s_end[rel >> 5], 1u << (rel & 31));
+          // blocks whose first quotient lies in (fqr, nq]: the last entry before them is this one
+          const uint32_t jb1 = (nq >> 6) < RD_BLOCKS - 1 ? (nq >> 6) : RD_BLOCKS - 1;
+          for (uint32_t jb = (fqr >> 6) + 1; jb <= jb1; ++jb) s_L[jb] = lc;
+        }
+      }
+      prev = lc;
+      pq = fqr;
+    }
+  }
+  if (t == 0) {
+    // blocks up to the first quotient: the last entry of earlier windows
+    const uint32_t f0 = nM ? v[0] >> rr : NONE;
+    const uint32_t jb1 = (f0 >> 6) < RD_BLOCKS - 1 ? (f0 >> 6) : RD_BLOCKS - 1;
+    for (uint32_t jb = 0; jb <= jb1; ++jb) s_L[jb] = (int32_t)X.lp - 1;
+  }
+  __syncthreads();
+  // ---- stores: whole words where the window owns the block, ORs into saved blocks
+  if (t < RD_BLOCKS && bw + t < nb) {
+    const uint64_t b = bw + t;
+    meta[4 * b] = ((uint64_t)s_occ[2 * t + 1] << 32) | s_occ[2 * t];
+    const int64_t L = s_L[t], sb = 64 * (int64_t)b;
+    meta[4 * b + 2] = L >= sb ? (uint64_t)((L - sb + 1) & 0xFF) : 0ull;
+  }
+  if (t < WM_IMG && bw + t < nb) {
+    const uint64_t b = bw + t;
+    const uint64_t e = ((uint64_t)s_end[2 * t + 1] << 32) | s_end[2 * t];
+    if (saved(t)) {
+      if (e) atomicOr((unsigned long long*)&meta[4 * b + 1], (unsigned long long)e);
+    } else if (t < RD_BLOCKS) {
+      meta[4 * b + 1] = e;
+    }
+  }
+  for (uint32_t i = t; i < WM_IMG * rw; i += WM_THREADS) {
+    const uint32_t j = i / rw;
+    const uint32_t x = s_rem[j * RW + (i - j * rw)];
+    if (bw + j < nb) {
+      if (saved(j)) {
+        if (x) atomicOr(&remg[bw * rw + i], x);
+      } else if (j < RD_BLOCKS) {
+        remg[bw * rw + i] = x;
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------- pipelines
 struct RsqfGeom {
   uint32_t q, r, nb, bits;
@@ -1152,6 +1633,70 @@ static gqf_status rsqf_insert_t(const gqf_rsqf& f, uint32_t n, const uint32_t* k
   return rsqf_rebuild<K, KEEP_ALL, true>(cs.stream, ds, ar, f, A, (uint32_t)g.cap, m_dev, B, n, nullptr, hb);
 }
 
+// Workspace of the windowed insert (beside the batch's sort).
+static size_t rsqf_win_ws(const RsqfGeom& g) {
+  const uint32_t W = rd_windows(g.nb);
+  return aligned_bytes((size_t)W * 8) + 256 + scan_rows_bytes(W, 2) + aligned_bytes(((size_t)W + 1) * 4) +
+         aligned_bytes((size_t)W * 16) + 2 * aligned_bytes((size_t)W * 4) + aligned_bytes((size_t)W * WM_SIDE * 8) +
+         aligned_bytes((size_t)W * WM_SIDE * 8 * g.r) + aligned_bytes(((W + RBS_PER - 1) / RBS_PER) * 16) + 1024;
+}
+// Tables the windowed insert takes: one scan state per window.
+static bool rsqf_win_fits(const RsqfGeom& g) { return rd_windows(g.nb) <= RB_MAX_CHUNKS; }
+
+// The windowed insert of the sorted fingerprints B (see "windowed insert").
+// Leaves ctr[CTR_MISC] != 0 when a window needs the rebuild instead; the
+// table is then untouched.
+template <typename K>
+static gqf_status rsqf_insert_win(hipStream_t s, DeviceState& ds, Arena& ar, const gqf_rsqf& f, const K* B,
+                                  uint32_t n) {
+  const RsqfGeom g = rgeom(f);
+  const RsqfDev d = rdev(f);
+  const uint32_t W = rd_windows(g.nb);
+  uint32_t* wp = ar.take<uint32_t>((size_t)W * 2);
+  uint32_t* tot = ar.take<uint32_t>(4);
+  uint32_t* bstart = ar.take<uint32_t>((size_t)W + 1);
+  uint4* agg = ar.take<uint4>(W);
+  uint32_t* wold = ar.take<uint32_t>(W);
+  uint32_t* wsave = ar.take<uint32_t>(W);
+  uint64_t* side_end = ar.take<uint64_t>((size_t)W * WM_SIDE);
+  uint32_t* side_rem = ar.take<uint32_t>((size_t)W * WM_SIDE * 2 * g.r);
+  if (ar.used > ar.cap) {
+    set_error("internal: windowed insert workspace overflow");
+    return GQF_ERR_INVALID;
+  }
+  const double meta = (double)g.nb * 16;  // occupieds + run ends
+  GQF_LAUNCH_B(meta, k_rd_pop, dim3((W + 3) / 4), dim3(256), 0, s, d.meta, g.nb, W, wp, (uint32_t*)nullptr, 0u);
+  GQF_TRY(scan_rows<2>(s, ar, wp, W, tot));
+  GQF_LAUNCH((k_wm_bounds<K>), dim3((W + 1 + 255) / 256), dim3(256), 0, s, B, n, g.r, W, bstart, ds.counters);
+  GQF_LAUNCH_B(meta * 1.5 + (double)n * sizeof(K), (k_wm_sum<K>), dim3(W), dim3(WS_THREADS), 0, s, d, wp, B, bstart,
+               agg, wold, ds.counters);
+  GQF_TRY(rb_scan(s, ar, agg, W, g.cap, ds.counters, nullptr));
+  GQF_LAUNCH(k_wm_check, dim3((W + 255) / 256), dim3(256), 0, s, agg, wold, W, wsave, ds.counters);
+  GQF_LAUNCH(k_wm_save, dim3(W), dim3(64), 0, s, reinterpret_cast<uint8_t*>(f.blocks), g.nb, g.r, wsave, side_end,
+             side_rem, ds.counters);
+  // the table read once and written once (plus the batch)
+  const double wbytes = 2.0 * (double)rsqf_dev_bytes(g.q, g.r) + (double)n * sizeof(K);
+#define GQF_WM_WRITE(R_)                                                                                          \
+  GQF_LAUNCH_B(wbytes, (k_wm_write<K, R_>), dim3(W), dim3(WM_THREADS), 0, s, reinterpret_cast<uint8_t*>(f.blocks), \
+               g.nb, g.r, W, wp, B, bstart, agg, wsave, side_end, side_rem, ds.counters)
+  if (g.r == 8) GQF_WM_WRITE(8);
+  else if (g.r == 16) GQF_WM_WRITE(16);
+  else GQF_WM_WRITE(0);
+#undef GQF_WM_WRITE
+  return GQF_OK;
+}
+
+template <typename K>
+static gqf_status rsqf_insert_win_t(const gqf_rsqf& f, uint32_t n, const uint32_t* keys, CallScope& cs) {
+  DeviceState& ds = *cs.ds;
+  const RsqfGeom g = rgeom(f);
+  GQF_TRY(ws_reserve(ds, sort_workspace_bytes(n, sizeof(K), false) + rsqf_win_ws(g)));
+  Arena ar(ds.ws, ds.ws_cap);
+  K* B;
+  GQF_TRY(radix_sort<K>(cs.stream, ar, keys, true, g.bits, nullptr, false, n, 0, g.bits, &B, nullptr));
+  return rsqf_insert_win<K>(cs.stream, ds, ar, f, B, n);
+}
+
 template <typename K>
 static size_t rsqf_delete_ws(const RsqfGeom& g, uint32_t n) {
   return rsqf_decode_ws(g, sizeof(K)) + sort_workspace_bytes(n, sizeof(K), false) + aligned_bytes(g.cap + 1) +
@@ -1241,12 +1786,30 @@ gqf_status gqf_rsqf_bulk_build(gqf_rsqf cqf, int n, const uint32_t* d_keys, gqf_
 
 gqf_status gqf_rsqf_insert(gqf_rsqf cqf, int n, const uint32_t* d_keys, int32_t* d_out, gqf_stream stream, float* ms) {
   GQF_TRY(rsqf_args(cqf, n, d_keys));
+  const bool wide = cqf.qbits + cqf.rbits > 32;
+  float ms_win = 0.f;
+  if (n > 0 && rsqf_insert_path() != GQF_INSERT_REBUILD && rsqf_win_fits(rgeom(cqf))) {
+    CallScope cs;
+    GQF_TRY(cs.begin(stream));
+    if (!wide) GQF_TRY(rsqf_insert_win_t<uint32_t>(cqf, (uint32_t)n, d_keys, cs));
+    else GQF_TRY(rsqf_insert_win_t<uint64_t>(cqf, (uint32_t)n, d_keys, cs));
+    if (d_out) launch_rlookup(cs.stream, cqf, (uint32_t)n, d_keys, reinterpret_cast<uint32_t*>(d_out));
+    const gqf_status st = cs.end(&ms_win, true);
+    if (st != GQF_OK && st != GQF_ERR_CAPACITY) return st;
+    // a window the windowed path cannot take: nothing was written, rebuild
+    if (cs.ds->h_pinned[CTR_MISC] == 0) {
+      if (ms) *ms = ms_win;
+      return st;
+    }
+  }
   CallScope cs;
   GQF_TRY(cs.begin(stream));
-  if (cqf.qbits + cqf.rbits <= 32) GQF_TRY(rsqf_insert_t<uint32_t>(cqf, (uint32_t)n, d_keys, cs));
+  if (!wide) GQF_TRY(rsqf_insert_t<uint32_t>(cqf, (uint32_t)n, d_keys, cs));
   else GQF_TRY(rsqf_insert_t<uint64_t>(cqf, (uint32_t)n, d_keys, cs));
   if (d_out && n > 0) launch_rlookup(cs.stream, cqf, (uint32_t)n, d_keys, reinterpret_cast<uint32_t*>(d_out));
-  return cs.end(ms, true);
+  const gqf_status st = cs.end(ms, true);
+  if (ms) *ms += ms_win;
+  return st;
 }
 
 gqf_status gqf_rsqf_delete(gqf_rsqf cqf, int n, const uint32_t* d_keys, gqf_stream stream, float* ms) {

@@ -20,6 +20,8 @@ static thread_local int t_sqf_hash = GQF_HASH_AP;
 uint32_t sqf_hash_flag() { return t_sqf_hash == GQF_HASH_FNV ? 0x100u : 0u; }
 static thread_local int t_sqf_insert_path = GQF_INSERT_AUTO;
 int sqf_insert_path() { return t_sqf_insert_path; }
+static thread_local int t_rsqf_insert_path = GQF_INSERT_AUTO;
+int rsqf_insert_path() { return t_rsqf_insert_path; }
 
 void set_error(const char* fmt, ...) {
   va_list ap;
@@ -290,6 +292,17 @@ gqf_status gqf_sqf_set_insert_path(int path) {
 }
 
 int gqf_sqf_get_insert_path(void) { return t_sqf_insert_path; }
+
+gqf_status gqf_rsqf_set_insert_path(int path) {
+  if (path != GQF_INSERT_AUTO && path != GQF_INSERT_REBUILD && path != GQF_INSERT_LOCAL) {
+    set_error("unknown insert path %d (GQF_INSERT_AUTO, _REBUILD or _LOCAL)", path);
+    return GQF_ERR_INVALID;
+  }
+  t_rsqf_insert_path = path;
+  return GQF_OK;
+}
+
+int gqf_rsqf_get_insert_path(void) { return t_rsqf_insert_path; }
 
 gqf_status gqf_device_count(int* count) {
   if (!count) return GQF_ERR_INVALID;

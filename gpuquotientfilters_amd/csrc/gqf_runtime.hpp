@@ -34,6 +34,7 @@ gqf_status hip_fail(hipError_t e, const char* what, const char* file, int line);
 uint32_t sqf_hash_flag();
 // SQF insert path of the calling thread (gqf_sqf_set_insert_path).
 int sqf_insert_path();
+int rsqf_insert_path();
 
 // ---------------------------------------------------------------- device state
 // Small device counters used by the kernels (index constants below).

@@ -93,6 +93,17 @@ int gqf_sqf_get_hash(void);
 enum gqf_insert_path { GQF_INSERT_AUTO = 0, GQF_INSERT_REBUILD = 1, GQF_INSERT_LOCAL = 2 };
 gqf_status gqf_sqf_set_insert_path(int path);
 int gqf_sqf_get_insert_path(void);
+/* The same choice for this thread's RSQF inserts (no reference counterpart;
+ * the results are equal, only the work differs):
+ *   AUTO, LOCAL  rewrite the table in place, window by window (4096 slots:
+ *            each window's stored entries decoded from its run ends, merged
+ *            with its batch entries, laid out from its incoming location);
+ *            a window of more than 8192 merged entries, or runs crossing a
+ *            whole window, sends the call to the rebuild before any write;
+ *            tables of more than 2^16 windows (q > 27) always rebuild;
+ *   REBUILD  decode the table, merge the sorted batch, rebuild every slot. */
+gqf_status gqf_rsqf_set_insert_path(int path);
+int gqf_rsqf_get_insert_path(void);
 
 enum gqf_sqf_build_method {
   GQF_BUILD_SEGMENTED_LAYOUTS = 0,  /* bulkBuildSegmentedLayouts (quotientFilter.cu:1446) */
