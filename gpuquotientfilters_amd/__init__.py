@@ -173,6 +173,7 @@ _SIGS = {
     "gqf_rsqf_destroy": (_st, [C.POINTER(_RSQF)]),
     "gqf_rsqf_bulk_build": (_st, [_RSQF, C.c_int, _vp, _vp, _f]),
     "gqf_rsqf_insert": (_st, [_RSQF, C.c_int, _vp, _vp, _vp, _f]),
+    "gqf_rsqf_insert_report": (_st, [_RSQF, C.c_int, _vp, _vp, C.POINTER(C.c_uint32), _vp, _f]),
     "gqf_rsqf_lookup": (_st, [_RSQF, C.c_int, C.c_int, _vp, _vp, _vp, _f]),
     "gqf_rsqf_delete": (_st, [_RSQF, C.c_int, _vp, _vp, _f]),
     "gqf_rsqf_count": (_st, [_RSQF, C.c_int, _vp, _vp, _vp, _f]),
@@ -500,6 +501,16 @@ class RSQF:
         return self._ms(lib().gqf_rsqf_insert, self.h, _n(keys, n), _ptr(keys), _ptr(out), stream)
 
     insert = insertGPU
+
+    def insert_report(self, keys, out=None, n=None, stream=None):
+        """insertGPU with the reference's per-item failures: a batch that does
+        not fit is inserted in part, out gets -1 for each key left out.
+        Returns (ms, number of keys left out)."""
+        failed = C.c_uint32(0)
+        ms = C.c_float(0)
+        check(lib().gqf_rsqf_insert_report(self.h, _n(keys, n), _ptr(keys), _ptr(out), C.byref(failed), stream,
+                                           C.byref(ms)))
+        return ms.value, failed.value
 
     def lookup(self, keys, out, n=None, sorted=True, stream=None):
         return self._ms(lib().gqf_rsqf_lookup, self.h, int(bool(sorted)), _n(keys, n), _ptr(keys), _ptr(out), stream)

@@ -184,6 +184,7 @@ static_assert(RB_MAX_CHUNKS <= 64 * RBS_PER, "the apply kernel combines at most 
       const RS all = rs_combine(carry, tot);
       if (total) *total = rs_pack(all);
       ctr[CTR_COUNT] = all.c;
+      ctr[CTR_MISC + 1] = all.lp;                                 // ... unclamped (saturating)
       ctr[CTR_MISC + 3] = (uint32_t)(all.lp < S ? all.lp : S);  // the slot after the last entry
       if (all.lp > S) ctr[CTR_ERR] |= DEV_ERR_CAPACITY;
     }

@@ -18,6 +18,8 @@
 
 #include <type_traits>
 
+#include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <vector>
 
@@ -1733,6 +1735,13 @@ static gqf_status rsqf_delete_t(const gqf_rsqf& f, uint32_t n, const uint32_t* k
   return rsqf_rebuild<K, KEEP_FLAGS, false>(cs.stream, ds, ar, f, A, mcap, m_dev, (const K*)nullptr, 0u, keep, hb);
 }
 
+// Keys left out of a partial insert: fingerprint >= cut -> -1.
+__global__ __launch_bounds__(256) void k_rk_mark(const uint32_t* __restrict__ keys, uint32_t n, uint32_t bits,
+                                                 uint64_t cut, uint32_t* __restrict__ out) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n && fingerprint(keys[i], bits) >= cut) out[i] = RS_NOT_FOUND;
+}
+
 static void launch_rlookup(hipStream_t s, const gqf_rsqf& f, uint32_t n, const uint32_t* keys, uint32_t* out) {
   const RsqfDev d = rdev(f);
   const dim3 grid((n + 255) / 256);
@@ -1784,7 +1793,10 @@ gqf_status gqf_rsqf_bulk_build(gqf_rsqf cqf, int n, const uint32_t* d_keys, gqf_
   return cs.end(ms, true);
 }
 
-gqf_status gqf_rsqf_insert(gqf_rsqf cqf, int n, const uint32_t* d_keys, int32_t* d_out, gqf_stream stream, float* ms) {
+// insertGPU as gqf_rsqf_insert; *lp_end (may be NULL) receives 1 + the last
+// location of the merged layout (saturating), also when it does not fit.
+static gqf_status rsqf_insert_call(gqf_rsqf cqf, int n, const uint32_t* d_keys, int32_t* d_out, gqf_stream stream,
+                                   float* ms, uint32_t* lp_end) {
   GQF_TRY(rsqf_args(cqf, n, d_keys));
   const bool wide = cqf.qbits + cqf.rbits > 32;
   float ms_win = 0.f;
@@ -1799,6 +1811,7 @@ gqf_status gqf_rsqf_insert(gqf_rsqf cqf, int n, const uint32_t* d_keys, int32_t*
     // a window the windowed path cannot take: nothing was written, rebuild
     if (cs.ds->h_pinned[CTR_MISC] == 0) {
       if (ms) *ms = ms_win;
+      if (lp_end) *lp_end = cs.ds->h_pinned[CTR_MISC + 1];
       return st;
     }
   }
@@ -1809,7 +1822,67 @@ gqf_status gqf_rsqf_insert(gqf_rsqf cqf, int n, const uint32_t* d_keys, int32_t*
   if (d_out && n > 0) launch_rlookup(cs.stream, cqf, (uint32_t)n, d_keys, reinterpret_cast<uint32_t*>(d_out));
   const gqf_status st = cs.end(ms, true);
   if (ms) *ms += ms_win;
+  if (lp_end) *lp_end = n > 0 ? cs.ds->h_pinned[CTR_MISC + 1] : 0u;
   return st;
+}
+
+gqf_status gqf_rsqf_insert(gqf_rsqf cqf, int n, const uint32_t* d_keys, int32_t* d_out, gqf_stream stream, float* ms) {
+  return rsqf_insert_call(cqf, n, d_keys, d_out, stream, ms, nullptr);
+}
+
+gqf_status gqf_rsqf_insert_report(gqf_rsqf cqf, int n, const uint32_t* d_keys, int32_t* d_out, uint32_t* n_failed,
+                                  gqf_stream stream, float* ms) {
+  if (n_failed) *n_failed = 0;
+  uint32_t lp = 0;
+  float t0 = 0.f;
+  const gqf_status st = rsqf_insert_call(cqf, n, d_keys, d_out, stream, &t0, &lp);
+  if (ms) *ms = t0;
+  if (st != GQF_ERR_CAPACITY) return st;
+  // The batch does not fit (nothing was written).  With e = 1 + the last
+  // location of the full merge, leaving out the k = e - S largest new
+  // fingerprints moves every earlier entry back by k and leaves only stored
+  // entries after them, so the rest fits; the cut is rounded down to whole
+  // fingerprints (every copy of a fingerprint shares its fate).
+  const RsqfGeom g = rgeom(cqf);
+  const auto t_start = std::chrono::steady_clock::now();
+  std::vector<uint32_t> hk((size_t)n);
+  GQF_HIP(hipMemcpy(hk.data(), d_keys, (size_t)n * 4, hipMemcpyDeviceToHost));
+  std::vector<uint64_t> fp((size_t)n);
+  for (int i = 0; i < n; ++i) fp[(size_t)i] = fingerprint(hk[(size_t)i], g.bits);
+  const uint64_t k64 = (uint64_t)lp > g.cap ? (uint64_t)lp - g.cap : 1u;
+  const size_t keep0 = k64 >= (uint64_t)n ? 0 : (size_t)n - (size_t)k64;
+  std::vector<uint64_t> srt(fp);
+  std::nth_element(srt.begin(), srt.begin() + (ptrdiff_t)keep0, srt.end());
+  const uint64_t cut = srt[keep0];  // keys with fingerprints >= cut are left out
+  std::vector<uint32_t> kept;
+  kept.reserve(keep0);
+  for (int i = 0; i < n; ++i)
+    if (fp[(size_t)i] < cut) kept.push_back(hk[(size_t)i]);
+  if (n_failed) *n_failed = (uint32_t)((size_t)n - kept.size());
+  uint32_t* dk = nullptr;
+  if (!kept.empty()) {
+    GQF_HIP(hipMalloc((void**)&dk, kept.size() * 4));
+    if (hipMemcpy(dk, kept.data(), kept.size() * 4, hipMemcpyHostToDevice) != hipSuccess) {
+      (void)hipFree(dk);
+      set_error("hipMemcpy of the kept keys failed");
+      return GQF_ERR_HIP;
+    }
+    float t1 = 0.f;
+    const gqf_status s2 = rsqf_insert_call(cqf, (int)kept.size(), dk, nullptr, stream, &t1, nullptr);
+    (void)hipFree(dk);
+    if (s2 != GQF_OK) return s2;
+  }
+  if (d_out) {
+    CallScope cs;
+    GQF_TRY(cs.begin(stream));
+    launch_rlookup(cs.stream, cqf, (uint32_t)n, d_keys, reinterpret_cast<uint32_t*>(d_out));
+    GQF_LAUNCH(k_rk_mark, dim3(((uint32_t)n + 255) / 256), dim3(256), 0, cs.stream, d_keys, (uint32_t)n, g.bits, cut,
+               reinterpret_cast<uint32_t*>(d_out));
+    GQF_TRY(cs.end(nullptr, false));
+  }
+  if (ms)
+    *ms = t0 + std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+  return GQF_OK;
 }
 
 gqf_status gqf_rsqf_delete(gqf_rsqf cqf, int n, const uint32_t* d_keys, gqf_stream stream, float* ms) {

@@ -211,6 +211,21 @@ gqf_status gqf_rsqf_bulk_build(gqf_rsqf cqf, int n, const uint32_t* d_keys, gqf_
 gqf_status gqf_rsqf_insert(gqf_rsqf cqf, int n, const uint32_t* d_keys, int32_t* d_out,
                            gqf_stream stream, float* ms);
 
+/* insertGPU with the reference's per-item failure report (RSQF.cu:745,753,
+ * 816,860 write -1 for an item that cannot be placed; testGPURSQF.cu:71-80
+ * counts them).  A batch that fits is inserted as by gqf_rsqf_insert.  One
+ * that does not is inserted in part: with e = 1 + the last location of the
+ * whole merge and S = 64 numBlocks slots, the keys whose fingerprints are
+ * among the e - S largest of the batch -- every copy of the smallest such
+ * fingerprint included -- are left out, the rest is inserted (it fits:
+ * every entry before them moves back by e - S, only stored entries follow
+ * them).  d_out (may be NULL): slots as gqf_rsqf_insert, -1 for the keys
+ * left out; *n_failed (may be NULL): their count.  GQF_OK in both cases.
+ * Which items fail in the reference depends on thread timing; this rule is
+ * deterministic. */
+gqf_status gqf_rsqf_insert_report(gqf_rsqf cqf, int n, const uint32_t* d_keys, int32_t* d_out, uint32_t* n_failed,
+                                  gqf_stream stream, float* ms);
+
 /* launchUnsortedLookups (RSQF.cu:540-563) when sorted == 0, launchLookups
  * (RSQF.cu:411-456) when sorted != 0: the LAST matching slot of the run,
  * or -1.  Both run the same per-key walk (no sort first: on MI355X sorting

@@ -262,11 +262,13 @@ inline float launchUnsortedLookups(countingQuotientFilterGPU cqf, int numValues,
 }
 
 // insertGPU (RSQF.cu:882-988): d_returnValues receives each key's slot in
-// the resulting (canonical) table.
+// the resulting (canonical) table, or -1 for a key that did not fit (the
+// batch is then inserted in part, as the reference's insert leaves it).
 inline float insertGPU(countingQuotientFilterGPU cqf, int numValues, unsigned int* d_insertValues,
                        int* d_returnValues) {
   float ms = 0.f;
-  const gqf_status s = gqf_rsqf_insert(gqf_from(cqf), numValues, d_insertValues, d_returnValues, nullptr, &ms);
+  const gqf_status s =
+      gqf_rsqf_insert_report(gqf_from(cqf), numValues, d_insertValues, d_returnValues, nullptr, nullptr, &ms);
   return gqf_compat_detail::ms_or_fail(s, ms, "insertGPU");
 }
 

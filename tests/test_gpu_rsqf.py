@@ -266,3 +266,45 @@ def test_wide_decode_bounds_crossed_by_a_long_run(q, r, alpha):
     f.deleteGPU(dev(np.concatenate([np.full(3000, hot, np.uint32), rnd[:20000]])))
     rest = np.concatenate([rnd[20000:], extra[:50000], np.full(3500, hot, np.uint32)])
     assert (f.dump() == canon(rest, q, r)).all()
+
+
+@pytest.mark.parametrize("q,r,fill,extra", [(10, 8, 0.9, 400), (12, 5, 0.95, 300), (18, 16, 0.97, 12000)])
+def test_insert_report_places_what_fits(q, r, fill, extra):
+    """The reference's insertGPU writes -1 for an item it cannot place and
+    keeps the rest (RSQF.cu:745-860).  gqf_rsqf_insert_report: with e = 1 +
+    the last location of the whole merge and S slots, the keys with the e - S
+    largest fingerprints (all copies of the cut fingerprint) are left out,
+    the rest is inserted, and the table is the canonical layout of what was
+    placed.  A batch that fits behaves as the plain insert."""
+    say("rsqf insert report", q, r, fill, extra)
+    base = O.mt19937(91, O.key_count(fill, q))
+    batch = np.concatenate([fresh_keys(92, extra, base), base[:20]])  # duplicates of stored keys too
+    S = O.rsqf_num_blocks(q) * 64
+    f = G.RSQF(q, r)
+    f.bulk_build(dev(base))
+    out = G.DeviceArray.empty(len(batch))
+    _, failed = f.insert_report(dev(batch), out)
+    # expected cut, from the canonical layout of everything
+    allfp = np.sort(np.concatenate([O.fingerprints(base, q, r), O.fingerprints(batch, q, r)]))
+    fq = (allfp >> np.uint64(r)).astype(np.int64)
+    i = np.arange(len(fq))
+    e = int((np.maximum.accumulate(fq - i) + i)[-1]) + 1
+    assert e > S, "the case must overflow"
+    bf = O.fingerprints(batch, q, r)
+    cut = np.sort(bf)[len(batch) - (e - S)]
+    keep = bf < cut
+    assert failed == int((~keep).sum()) > 0
+    t = canon(np.concatenate([base, batch[keep]]), q, r)
+    assert (f.dump() == t).all()
+    got = out.to_host().view(np.int32)
+    assert (got[~keep] == -1).all()
+    assert (got[keep] == O.rsqf_lookup_keys(t, q, r, batch[keep]).view(np.int32)).all()
+    # a batch that fits: plain insert, nothing left out
+    g2 = G.RSQF(q, r)
+    g2.bulk_build(dev(base[: len(base) // 2]))
+    out2 = G.DeviceArray.empty(50)
+    _, failed2 = g2.insert_report(dev(base[-50:]), out2)
+    assert failed2 == 0
+    t2 = canon(np.concatenate([base[: len(base) // 2], base[-50:]]), q, r)
+    assert (g2.dump() == t2).all()
+    assert (out2.to_host() == O.rsqf_lookup_keys(t2, q, r, base[-50:])).all()
