@@ -57,6 +57,24 @@ __host__ __device__ __forceinline__ uint64_t fingerprint(uint32_t key, unsigned 
   return fmix64(((uint64_t)lo << 32) | key) & mask64(bits);
 }
 
+// Position of the k-th set bit of x (k >= 1, 0-based position), found by
+// halving with popcounts; 64 if x has fewer than k set bits.
+__host__ __device__ __forceinline__ uint32_t select_bit64(uint64_t x, uint32_t k) {
+  if (k == 0 || k > (uint32_t)__builtin_popcountll(x)) return 64u;
+  uint32_t pos = 0;
+#pragma unroll
+  for (uint32_t half = 32; half >= 1; half >>= 1) {
+    const uint64_t lowm = half == 64 ? ~0ull : ((1ull << half) - 1ull);
+    const uint32_t c = (uint32_t)__builtin_popcountll(x & lowm);
+    if (k > c) {
+      k -= c;
+      x >>= half;
+      pos += half;
+    }
+  }
+  return pos;
+}
+
 // ---------------------------------------------------------------- wave64 scans
 // Inclusive scans across the 64 lanes of a wave with DPP moves: row_shr 1, 2,
 // 4, 8 inside each 16-lane row, then row_bcast:15 (rows 1, 3) and

@@ -270,7 +270,7 @@ static gqf_status sqf_merge_into(CallScope& cs, Arena& ar, const gqf_sqf& qf, co
 static size_t sqf_local_bytes(const SqfGeom& g, uint32_t nb) {
   const uint64_t R = (g.S + PI_REGION - 1) / PI_REGION, W = ((uint64_t)g.S + 63) / 64;
   const uint64_t G = ((uint64_t)nb + RB_TILE - 1) / RB_TILE;
-  return aligned_bytes(W) + aligned_bytes((R + 1) * 4) + 256 + 4 * aligned_bytes((uint64_t)nb * 4) +
+  return aligned_bytes(W * 2) + aligned_bytes((R + 1) * 4) + 256 + 5 * aligned_bytes((uint64_t)nb * 4) +
          aligned_bytes(nb) + aligned_bytes(G * 16) + aligned_bytes(CTR_TOTAL * 4) + 256 + aligned_bytes((uint64_t)g.S * 4) +
          aligned_bytes(((G + RBS_PER - 1) / RBS_PER) * 16) + 2048 + scan_rows_bytes((uint32_t)R + 1, 1);
 }
@@ -286,12 +286,13 @@ static gqf_status sqf_insert_local(CallScope& cs, Arena& ar, const gqf_sqf& qf, 
   DeviceState& ds = *cs.ds;
   *done = false;
   const uint32_t R = (g.S + PI_REGION - 1) / PI_REGION, W = (g.S + 63) / 64, G = (nb + RB_TILE - 1) / RB_TILE;
-  uint8_t* wcnt = ar.take<uint8_t>(W);
+  uint16_t* wpre = ar.take<uint16_t>(W);
   uint32_t* rcnt = ar.take<uint32_t>(R + 1);
   uint32_t* tot = ar.take<uint32_t>(4);
   uint32_t* E = ar.take<uint32_t>(nb);
   uint32_t* s0 = ar.take<uint32_t>(nb);
   uint32_t* P = ar.take<uint32_t>(nb);
+  uint32_t* J = ar.take<uint32_t>(nb);
   uint32_t* rend = ar.take<uint32_t>(nb);
   uint8_t* head = ar.take<uint8_t>(nb);
   uint4* agg = ar.take<uint4>(G);
@@ -304,13 +305,13 @@ static gqf_status sqf_insert_local(CallScope& cs, Arena& ar, const gqf_sqf& qf, 
   }
   const dim3 kg((nb + 255) / 256);
   GQF_LAUNCH(k_pi_init, dim3(1), dim3(64), 0, s, rcnt + R, flags);
-  GQF_LAUNCH_B((double)g.bytes, k_pi_count<BPE>, dim3(R), dim3(256), 0, s, qf.table, g.S, wcnt, rcnt);
+  GQF_LAUNCH_B((double)g.bytes, k_pi_count<BPE>, dim3(R), dim3(256), 0, s, qf.table, g.S, wpre, rcnt);
   GQF_TRY(scan_rows<1>(s, ar, rcnt, R + 1, tot));
-  GQF_LAUNCH(k_pi_keys<BPE>, kg, dim3(256), 0, s, qf.table, g.S, g.q, g.r, B, nb, wcnt, rcnt, E, s0, flags);
+  GQF_LAUNCH(k_pi_keys<BPE>, kg, dim3(256), 0, s, qf.table, g.S, g.q, g.r, B, nb, wpre, rcnt, E, s0, flags);
   GQF_LAUNCH(k_pi_jagg, dim3(G), dim3(RB_THREADS), 0, s, E, nb, agg);
   GQF_TRY(rb_scan(s, ar, agg, G, ~0ull, sctr, nullptr));
-  GQF_LAUNCH(k_pi_select<BPE>, dim3(G), dim3(RB_THREADS), 0, s, qf.table, g.S, E, s0, nb, agg, wcnt, rcnt, R, P, head,
-             flags);
+  GQF_LAUNCH(k_pi_ranks, dim3(G), dim3(RB_THREADS), 0, s, E, nb, agg, rcnt, R, J, flags);
+  GQF_LAUNCH(k_pi_pos<BPE>, kg, dim3(256), 0, s, qf.table, g.S, g.q, g.r, B, J, s0, nb, wpre, rcnt, R, P, head);
   GQF_LAUNCH(k_pi_regions, kg, dim3(256), 0, s, s0, P, head, nb, rend, flags);
   GQF_HIP(hipMemcpyAsync(ds.h_pinned, flags, 4, hipMemcpyDeviceToHost, s));
   GQF_HIP(hipStreamSynchronize(s));

@@ -3,6 +3,8 @@
 // entry points (gqf_sqf.hip) and the batched ones (gqf_batch.hip).
 #pragma once
 
+#include <type_traits>
+
 #include "gqf_rebuild.hpp"
 
 namespace gqf {
@@ -1168,14 +1170,17 @@ __global__ __launch_bounds__(256) void k_sqf_decode(const uint8_t* __restrict__ 
 // so every region is decoded, merged with its keys and re-encoded in place,
 // each by one thread.  A key's slot past the last empty slot, or a region
 // longer than PI_LEN_CAP, sends the call to the full rebuild instead (flags).
+// E(x) and the j-th empty slot come from per-region and per-64-slot-window
+// prefixes plus one 64-slot bitmask, so each key costs a few loads.
 constexpr uint32_t PI_REGION = 4096;        // slots per counting region (64 windows of 64)
 constexpr uint32_t PI_LEN_CAP = 1u << 16;   // longest region a thread rewrites
 constexpr uint32_t PI_SCAN_CAP = 1u << 16;  // longest backward scan to a boundary
 // Automatic choice: batches of at most PI_AUTO_MAX keys and S / 256 of the
-// table.  Per-region threads are latency-bound (about 2.5 ns per key on
-// MI355X against a fixed ~0.3 ms), so the rebuild wins from about 2^11 keys
-// on a q=26 table.
-constexpr uint32_t PI_AUTO_MAX = 1024;
+// table.  Measured on a q=26 r=5 table (ms per call, localized / rebuild):
+// at fill 0.45, 2^14 keys 0.43 / 0.57, 2^16 0.43 / 0.55, 2^18 0.50 / 0.55;
+// at fill 0.75, 2^16 0.64 / 0.63, 2^17 0.68 / 0.62 -- its regions grow with
+// the fill, and one thread rewrites each.
+constexpr uint32_t PI_AUTO_MAX = 65536;
 constexpr uint32_t PI_MIN_RATIO = 256;
 
 template <int BPE>
@@ -1185,9 +1190,10 @@ __device__ __forceinline__ void sqf_put(uint8_t* __restrict__ t, uint32_t i, uin
   for (int b = 0; b < BPE; ++b) p[b] = (uint8_t)(e >> (8 * (BPE - 1 - b)));
 }
 
-// Empty slots per 64-slot window (wcnt, one byte each) and per region (rcnt).
+// Empty slots per region (rcnt) and, per 64-slot window, the empty slots of
+// its region before it (wpre, 16 bits).
 template <int BPE>
-__global__ __launch_bounds__(256) void k_pi_count(const uint8_t* __restrict__ t, uint32_t S, uint8_t* __restrict__ wcnt,
+__global__ __launch_bounds__(256) void k_pi_count(const uint8_t* __restrict__ t, uint32_t S, uint16_t* __restrict__ wpre,
                                                   uint32_t* __restrict__ rcnt) {
   __shared__ uint32_t s_tmp4[4];
   const uint32_t s0 = blockIdx.x * PI_REGION + threadIdx.x * 16;
@@ -1196,64 +1202,87 @@ __global__ __launch_bounds__(256) void k_pi_count(const uint8_t* __restrict__ t,
   uint32_t c = 0;
 #pragma unroll
   for (int j = 0; j < 16; ++j) c += (s0 + (uint32_t)j < S && (e[j] & 7u) == 0) ? 1u : 0u;
-  uint32_t wsum = c + (uint32_t)__shfl_xor((int)c, 1);
-  wsum += (uint32_t)__shfl_xor((int)wsum, 2);  // the 4 threads of a window
-  if ((threadIdx.x & 3) == 0 && s0 < S) wcnt[s0 >> 6] = (uint8_t)wsum;
   uint32_t tot;
-  (void)block256_excl_scan(c, s_tmp4, &tot);
+  const uint32_t pre = block256_excl_scan(c, s_tmp4, &tot);
+  if ((threadIdx.x & 3) == 0 && s0 < S) wpre[s0 >> 6] = (uint16_t)pre;  // the window's first thread
   if (threadIdx.x == 0) rcnt[blockIdx.x] = tot;
 }
 
-// Empty slots before slot h (ER: exclusive region prefix of rcnt).
+// Bit j set: slot 64 w + j is empty (*empty) / a boundary -- empty or
+// unshifted, so that no run crosses it (*bound).  Slots past S: neither.
 template <int BPE>
-__device__ uint32_t pi_empties_before(const uint8_t* __restrict__ t, uint32_t S, const uint8_t* __restrict__ wcnt,
-                                      const uint32_t* __restrict__ ER, uint32_t h) {
-  const uint32_t reg = h / PI_REGION, w = h >> 6;
-  uint32_t e = ER[reg];
-  for (uint32_t x = reg * (PI_REGION / 64); x < w; ++x) e += wcnt[x];
-  for (uint32_t s = w << 6; s < h; ++s) e += (sqf_get<BPE>(t, S, s) & 7u) == 0 ? 1u : 0u;
-  return e;
+__device__ __forceinline__ void pi_masks64(const uint8_t* __restrict__ t, uint32_t S, uint32_t w, uint64_t* empty,
+                                           uint64_t* bound) {
+  uint64_t em = 0, bd = 0;
+#pragma unroll
+  for (uint32_t c = 0; c < 4; ++c) {
+    uint32_t e[16];
+    sqf_load16<BPE>(t, S, 64 * w + 16 * c, e);
+#pragma unroll
+    for (uint32_t j = 0; j < 16; ++j) {
+      if (64 * w + 16 * c + j < S) {
+        em |= (uint64_t)((e[j] & 7u) == 0) << (16 * c + j);
+        bd |= (uint64_t)((e[j] & 1u) == 0) << (16 * c + j);
+      }
+    }
+  }
+  *empty = em;
+  *bound = bd;
 }
 
-// The j-th empty slot (0-based; j < ER[R], the total).
+// Empty slots before window w (ER: exclusive region prefix of rcnt).
+__device__ __forceinline__ uint32_t pi_empties_before_window(const uint16_t* __restrict__ wpre,
+                                                             const uint32_t* __restrict__ ER, uint32_t w) {
+  return ER[w / (PI_REGION / 64)] + wpre[w];
+}
+
+// The j-th empty slot (0-based), known to lie at or after slot h: windows
+// are walked forward from h's (usually it is there or in the next one).
 template <int BPE>
-__device__ uint32_t pi_select_empty(const uint8_t* __restrict__ t, uint32_t S, const uint8_t* __restrict__ wcnt,
-                                    const uint32_t* __restrict__ ER, uint32_t R, uint32_t j) {
-  uint32_t lo = 0, hi = R;  // ER[lo] <= j < ER[hi]
-  while (hi - lo > 1) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (ER[mid] <= j) lo = mid;
-    else hi = mid;
-  }
-  uint32_t e = ER[lo], w = lo * (PI_REGION / 64);
+__device__ uint32_t pi_select_empty(const uint8_t* __restrict__ t, uint32_t S, const uint16_t* __restrict__ wpre,
+                                    const uint32_t* __restrict__ ER, uint32_t h, uint32_t j) {
   const uint32_t wend = (S + 63) / 64;
-  while (w + 1 < wend && e + wcnt[w] <= j) e += wcnt[w++];
-  const uint32_t end = w * 64 + 64 < S ? w * 64 + 64 : S;
-  for (uint32_t s = w * 64; s < end; ++s)
-    if ((sqf_get<BPE>(t, S, s) & 7u) == 0 && e++ == j) return s;
+  uint32_t w = h >> 6;
+  if (w >= wend) return S;
+  uint32_t g = pi_empties_before_window(wpre, ER, w);
+  for (; w < wend; ++w) {
+    uint64_t em, bd;
+    pi_masks64<BPE>(t, S, w, &em, &bd);
+    const uint32_t c = (uint32_t)__popcll(em);
+    if (g + c > j) return 64 * w + select_bit64(em, j - g + 1);
+    g += c;
+  }
   return S;  // not reached for a consistent count
 }
 
-// Per key: empties before its quotient, and its region's boundary.
+// Per key: empties before its quotient, and its region's boundary (the last
+// boundary slot at or before it).
 template <int BPE>
 __global__ __launch_bounds__(256) void k_pi_keys(const uint8_t* __restrict__ t, uint32_t S, uint32_t q, uint32_t r,
                                                  const uint32_t* __restrict__ B, uint32_t nb,
-                                                 const uint8_t* __restrict__ wcnt, const uint32_t* __restrict__ ER,
+                                                 const uint16_t* __restrict__ wpre, const uint32_t* __restrict__ ER,
                                                  uint32_t* __restrict__ E, uint32_t* __restrict__ s0,
                                                  uint32_t* __restrict__ flags) {
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= nb) return;
   const uint32_t h = (B[i] >> r) & (uint32_t)mask64(q);
-  E[i] = pi_empties_before<BPE>(t, S, wcnt, ER, h);
-  uint32_t x = h, steps = 0;
-  while (x > 0 && (sqf_get<BPE>(t, S, x) & 1u)) {
-    --x;
-    if (++steps > PI_SCAN_CAP) {
+  uint32_t w = h >> 6;
+  uint64_t em, bd;
+  pi_masks64<BPE>(t, S, w, &em, &bd);
+  const uint64_t below = (1ull << (h & 63)) - 1ull;
+  E[i] = pi_empties_before_window(wpre, ER, w) + (uint32_t)__popcll(em & below);
+  uint64_t m = bd & (below | (1ull << (h & 63)));  // boundaries at or before h
+  uint32_t steps = 0;
+  while (!m && w > 0) {
+    --w;
+    pi_masks64<BPE>(t, S, w, &em, &bd);
+    m = bd;
+    if (++steps * 64 > PI_SCAN_CAP) {
       atomicOr(flags, 1u);
       break;
     }
   }
-  s0[i] = x;
+  s0[i] = m ? 64 * w + 63 - (uint32_t)__clzll(m) : 0u;
 }
 
 // Chunk states of the slot-rank recurrence (RB_TILE keys per workgroup, as
@@ -1271,19 +1300,15 @@ __global__ __launch_bounds__(256) void k_pi_keys(const uint8_t* __restrict__ t, 
   if (threadIdx.x == 0) agg[blockIdx.x] = rs_pack(sub);
 }
 
-// Each key's slot p_i (the j_i-th empty slot) and region heads:
-// head_i = (i == 0 or s0_i > p_{i-1}).
-template <int BPE>
-__global__ __launch_bounds__(RB_THREADS) void k_pi_select(const uint8_t* __restrict__ t, uint32_t S,
-                                                          const uint32_t* __restrict__ E,
-                                                          const uint32_t* __restrict__ s0, uint32_t nb,
-                                                          const uint4* __restrict__ pre,
-                                                          const uint8_t* __restrict__ wcnt,
-                                                          const uint32_t* __restrict__ ER, uint32_t R,
-                                                          uint32_t* __restrict__ P, uint8_t* __restrict__ head,
-                                                          uint32_t* __restrict__ flags) {
+// Each key's empty rank j_i = max(E_i, j_{i-1} + 1), from the chunk prefixes
+// of the rank scan (RB_TILE keys per workgroup).
+[[maybe_unused]] static __global__ __launch_bounds__(RB_THREADS) void k_pi_ranks(const uint32_t* __restrict__ E,
+                                                                                 uint32_t nb,
+                                                                                 const uint4* __restrict__ pre,
+                                                                                 const uint32_t* __restrict__ ER,
+                                                                                 uint32_t R, uint32_t* __restrict__ J,
+                                                                                 uint32_t* __restrict__ flags) {
   __shared__ uint4 s_wave[RB_WAVES + 1];
-  __shared__ uint32_t s_last[RB_THREADS];
   const uint32_t tid = threadIdx.x, i0 = blockIdx.x * RB_TILE + tid * RB_ITEMS;
   const uint32_t total = ER[R];
   uint32_t e[RB_ITEMS];
@@ -1295,40 +1320,39 @@ __global__ __launch_bounds__(RB_THREADS) void k_pi_select(const uint8_t* __restr
   }
   RS ex;
   (void)rs_block(a, &ex, s_wave);
-  const RS X = rs_unpack(pre[blockIdx.x]);
-  const RS me = rs_combine(X, ex);
-  uint32_t lp = me.lp, p[RB_ITEMS];
+  uint32_t lp = rs_combine(rs_unpack(pre[blockIdx.x]), ex).lp;
 #pragma unroll
   for (int k = 0; k < RB_ITEMS; ++k) {
-    p[k] = S;
     if (i0 + k < nb) {
       lp = lp + 1u > e[k] + 1u ? lp + 1u : e[k] + 1u;
-      const uint32_t j = lp - 1u;
-      if (j >= total) atomicOr(flags, 2u);  // no empty slot left: the full rebuild decides
-      else p[k] = pi_select_empty<BPE>(t, S, wcnt, ER, R, j);
-      P[i0 + k] = p[k];
+      if (lp - 1u >= total) atomicOr(flags, 2u);  // no empty slot left: the full rebuild decides
+      J[i0 + k] = lp - 1u;
     }
   }
-  // the key before this thread's first: the previous thread's last, or (thread
-  // 0) the previous chunk's last key, whose rank is the chunk prefix's lp - 1
-  s_last[tid] = p[RB_ITEMS - 1];
-  __syncthreads();
-  uint32_t prevp;
-  if (tid > 0) {
-    prevp = s_last[tid - 1];
-  } else if (X.c > 0 && X.lp - 1u < total) {
-    prevp = pi_select_empty<BPE>(t, S, wcnt, ER, R, X.lp - 1u);
-  } else {
-    prevp = 0xFFFFFFFFu;
+}
+
+// One thread per key: its slot p_i (the j_i-th empty slot, at or after its
+// quotient) and whether it heads a region (i == 0 or s0_i > p_{i-1}; the
+// previous key's slot is selected again here rather than waited for).
+template <int BPE>
+__global__ __launch_bounds__(256) void k_pi_pos(const uint8_t* __restrict__ t, uint32_t S, uint32_t q, uint32_t r,
+                                                const uint32_t* __restrict__ B, const uint32_t* __restrict__ J,
+                                                const uint32_t* __restrict__ s0, uint32_t nb,
+                                                const uint16_t* __restrict__ wpre, const uint32_t* __restrict__ ER,
+                                                uint32_t R, uint32_t* __restrict__ P, uint8_t* __restrict__ head) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= nb) return;
+  const uint32_t total = ER[R], mq = (uint32_t)mask64(q);
+  const uint32_t j = J[i];
+  const uint32_t p = j < total ? pi_select_empty<BPE>(t, S, wpre, ER, (B[i] >> r) & mq, j) : S;
+  P[i] = p;
+  bool h = true;
+  if (i > 0) {
+    const uint32_t jp = J[i - 1];
+    const uint32_t pp = jp < total ? pi_select_empty<BPE>(t, S, wpre, ER, (B[i - 1] >> r) & mq, jp) : S;
+    h = s0[i] > pp;
   }
-#pragma unroll
-  for (int k = 0; k < RB_ITEMS; ++k) {
-    if (i0 + k < nb) {
-      const bool h = (i0 + k == 0) || (prevp == 0xFFFFFFFFu) || s0[i0 + k] > prevp;
-      head[i0 + k] = h ? 1 : 0;
-      prevp = p[k];
-    }
-  }
+  head[i] = h ? 1 : 0;
 }
 
 // Region of each head key: its keys [i, rend_i) and a length check.
@@ -1353,7 +1377,10 @@ __global__ __launch_bounds__(RB_THREADS) void k_pi_select(const uint8_t* __restr
 // One thread per region [s0_i, P[rend_i - 1]]: decode its entries into the
 // slot-indexed scratch, merge the region's keys (insert rule: a key equal
 // to the entry before it is dropped), write every slot of the region (elements
-// and zeros), then the occupied bits of its run heads.
+// and zeros), then the occupied bits of its run heads.  (One workgroup per
+// 4096-slot window rewriting its regions from an LDS copy was measured
+// slower: the windows run in rounds, each as long as its longest region,
+// where one thread per key keeps every region in flight at once.)
 template <int BPE>
 __global__ __launch_bounds__(256) void k_pi_write(uint8_t* __restrict__ t, uint32_t S, uint32_t r,
                                                   const uint32_t* __restrict__ B, uint32_t nb,

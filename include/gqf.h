@@ -83,7 +83,7 @@ int gqf_sqf_get_hash(void);
 /* How gqf_sqf_insert (insert, quotientFilter.cu:1184) of the calling thread
  * updates the table.  No reference counterpart: the results are the same
  * either way; only the work differs.
- *   AUTO     localized for batches of at most 1024 keys (and S/256), else
+ *   AUTO     localized for batches of at most 65536 keys (and S/256), else
  *            the rebuild;
  *   REBUILD  decode the table, merge the sorted batch, rebuild every slot;
  *   LOCAL    rewrite only the regions the batch changes, between the last
