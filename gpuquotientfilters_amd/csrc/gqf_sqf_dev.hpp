@@ -510,51 +510,21 @@ __device__ __forceinline__ void rb_load(const uint32_t* __restrict__ A, const ui
 // workgroup works on the previous sub-tile; the entries just before the
 // ranges (the first thread's previous entry) go to prev2[0] (A) and
 // prev2[1] (B).  rb_stage_wait() and a barrier make a buffer readable.
-//   Each range is copied from the 16-byte boundary at or below its start, in
-// 16-byte pieces (one per lane, 64 consecutive pieces per wave and step);
-// pieces end at the 16-byte boundary at or above the range's end, but never
-// past the last whole piece of the array (n words), whose few remaining
-// words go by 4-byte loads.  A's copy starts at word 0 of the buffer, B's at
-// rb_stage_boff (A's pieces, rounded up); the buffer holds at most
-// RB_TILE + 12 words.
-constexpr uint32_t RB_STAGE = RB_TILE + 16;  // words per stage buffer
-__device__ __forceinline__ uint32_t rb_stage_boff(uint32_t a0, uint32_t la) { return (la + (a0 & 3u) + 3u) & ~3u; }
-
-// Words [s0, s0 + len) of src (n words) from the piece at s0 & ~3: pieces
-// [0, npc) are whole, *t0 = the first word left for 4-byte loads.
-__device__ __forceinline__ uint32_t rb_stage_pieces(uint32_t n, uint32_t s0, uint32_t len, uint32_t* t0) {
-  const uint32_t al = s0 & ~3u, end = s0 + len;
-  const uint32_t up = (end + 3u) & ~3u, whole = n & ~3u;
-  const uint32_t pe = up < whole ? up : whole;
-  *t0 = pe > al ? pe : al;
-  return len && pe > al ? (pe - al) / 4 : 0u;
-}
-
 __device__ __forceinline__ void rb_stage_issue(const uint32_t* __restrict__ A, const uint32_t* __restrict__ B,
-                                               uint32_t na, uint32_t nb, uint32_t a0, uint32_t a1, uint32_t d0,
-                                               uint32_t tn, uint32_t* buf, uint32_t* prev2) {
-  const uint32_t la = a1 - a0, b0 = d0 - a0, lb = tn - la;
+                                               uint32_t a0, uint32_t a1, uint32_t d0, uint32_t tn, uint32_t* buf,
+                                               uint32_t* prev2) {
+  const uint32_t la = a1 - a0, b0 = d0 - a0;
   const uint32_t wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  // B's pieces follow A's in the buffer (A's region is whole pieces), so one
-  // index runs over both: piece p < pA is A's, the rest B's
-  const uint32_t boff = rb_stage_boff(a0, la), pA = boff / 4;
-  uint32_t tA, tB;
-  const uint32_t nA = rb_stage_pieces(na, a0, la, &tA), nB = rb_stage_pieces(nb, b0, lb, &tB);
-  const uint32_t* srcA = A + (a0 & ~3u);
-  const uint32_t* srcB = B + (b0 & ~3u);
 #pragma unroll
-  for (uint32_t k = 0; k < (RB_STAGE / 4 + RB_THREADS - 1) / RB_THREADS; ++k) {
-    const uint32_t bp = (k * RB_WAVES + wid) * 64, p = bp + lane;  // 64 consecutive pieces per wave and step
-    const bool inA = p < pA;
-    if (inA ? p < nA : p - pA < nB)
-      __builtin_amdgcn_global_load_lds(inA ? srcA + 4 * p : srcB + 4 * (p - pA), buf + 4 * bp, 16, 0, 0);
+  for (int k = 0; k < RB_ITEMS; ++k) {
+    // wave wid, step k: the 64 consecutive positions from base (lane l -> buf[base + l])
+    const uint32_t base = ((uint32_t)k * RB_WAVES + wid) * 64;
+    const uint32_t pos = base + lane;
+    if (pos < tn) {
+      const uint32_t* src = pos < la ? A + (a0 + pos) : B + (b0 + (pos - la));
+      __builtin_amdgcn_global_load_lds(src, buf + base, 4, 0, 0);
+    }
   }
-  // the words past the arrays' last whole pieces (at most 3 each)
-  const uint32_t eA = a0 + la, eB = b0 + lb;
-  if (wid == 0 && la && tA < eA && lane < eA - tA)
-    __builtin_amdgcn_global_load_lds(A + tA + lane, buf + (tA - (a0 & ~3u)), 4, 0, 0);
-  if (wid == 1 && lb && tB < eB && lane < eB - tB)
-    __builtin_amdgcn_global_load_lds(B + tB + lane, buf + boff + (tB - (b0 & ~3u)), 4, 0, 0);
   if (wid == 0 && lane < 2 && (lane == 0 ? a0 > 0 : b0 > 0))
     __builtin_amdgcn_global_load_lds(lane == 0 ? A + (a0 - 1) : B + (b0 - 1), prev2, 4, 0, 0);
 }
@@ -570,8 +540,8 @@ __device__ __forceinline__ void rb_merge_staged(const uint32_t* buf, const uint3
                                                 uint32_t d0, uint32_t tn, uint32_t* v, uint32_t& tags,
                                                 uint32_t* s_bmp, uint32_t* s_wpre) {
   const uint32_t b0 = d0 - a0, la = a1 - a0, lb = tn - la;
-  const uint32_t* sA = buf + (a0 & 3u);
-  const uint32_t* sB = buf + rb_stage_boff(a0, la) + (b0 & 3u);
+  const uint32_t* sA = buf;
+  const uint32_t* sB = buf + la;
   for (uint32_t j = threadIdx.x; j < lb; j += RB_THREADS) {
     const uint32_t x = sB[j];
     uint32_t lo = 0, hi = la;
@@ -661,7 +631,7 @@ __global__ __launch_bounds__(RB_THREADS, 8) void k_rb_agg(const uint32_t* __rest
                                                 uint4* __restrict__ agg, uint32_t* __restrict__ ctr) {
   // the rebuild's counters start at zero (nothing reads them before the scan)
   if (blockIdx.x == 0 && threadIdx.x >= CTR_OP && threadIdx.x < CTR_TOTAL) ctr[threadIdx.x] = 0u;
-  __shared__ __align__(16) uint32_t s_stage[MERGE ? 2 : 1][MERGE ? RB_STAGE : 1];
+  __shared__ uint32_t s_stage[MERGE ? 2 : 1][MERGE ? RB_TILE : 1];
   __shared__ uint32_t s_prev[2][2];
   __shared__ uint32_t s_bmp[MERGE ? RB_TILE / 32 : 1], s_wpre[MERGE ? RB_TILE / 32 : 1];
   __shared__ uint4 s_wave[RB_WAVES + 1];
@@ -681,8 +651,8 @@ __global__ __launch_bounds__(RB_THREADS, 8) void k_rb_agg(const uint32_t* __rest
     ca1 = splits[u0 + 1];
     if (c0 + RB_TILE < c1) na1 = splits[u0 + 2];
     na0 = ca1;
-    rb_stage_issue(A, B, na, nb, ca0, ca1, (uint32_t)c0, (uint32_t)(c1 - c0 < RB_TILE ? c1 - c0 : RB_TILE),
-                   s_stage[0], s_prev[0]);
+    rb_stage_issue(A, B, ca0, ca1, (uint32_t)c0, (uint32_t)(c1 - c0 < RB_TILE ? c1 - c0 : RB_TILE), s_stage[0],
+                   s_prev[0]);
   }
   int buf = 0;
   for (uint64_t d0 = c0; d0 < c1; d0 += RB_TILE) {
@@ -697,7 +667,7 @@ __global__ __launch_bounds__(RB_THREADS, 8) void k_rb_agg(const uint32_t* __rest
       if (threadIdx.x < RB_TILE / 32) s_bmp[threadIdx.x] = 0u;
       __syncthreads();
       if (dn < c1)
-        rb_stage_issue(A, B, na, nb, na0, na1, (uint32_t)dn, (uint32_t)(c1 - dn < RB_TILE ? c1 - dn : RB_TILE),
+        rb_stage_issue(A, B, na0, na1, (uint32_t)dn, (uint32_t)(c1 - dn < RB_TILE ? c1 - dn : RB_TILE),
                        s_stage[buf ^ 1], s_prev[buf ^ 1]);
       const uint32_t nn1 = dn + RB_TILE < c1 ? splits[u + 3] : 0u;
       rb_merge_staged(s_stage[buf], s_prev[buf], ca0, ca1, (uint32_t)d0, tn, v, tags, s_bmp, s_wpre);
@@ -874,7 +844,7 @@ __global__ __launch_bounds__(RB_THREADS, 6) void k_rb_write(uint8_t* __restrict_
                                                   uint32_t* __restrict__ fix_more_n, uint32_t fix_cap,
                                                   int precleared) {
   if (ctr[CTR_ERR]) return;  // capacity error: the table stays as it was
-  __shared__ __align__(16) uint32_t s_stage[MERGE ? 2 : 1][MERGE ? RB_STAGE : 1];
+  __shared__ uint32_t s_stage[MERGE ? 2 : 1][MERGE ? RB_TILE : 1];
   __shared__ uint32_t s_prev[2][2];
   __shared__ uint32_t s_bmp[MERGE ? RB_TILE / 32 : 1], s_wpre[MERGE ? RB_TILE / 32 : 1];
   __shared__ __align__(16) uint32_t s_win[rb_win_words<BPE>()];
@@ -897,8 +867,8 @@ __global__ __launch_bounds__(RB_THREADS, 6) void k_rb_write(uint8_t* __restrict_
     ca1 = splits[u0 + 1];
     if (c0 + RB_TILE < c1) na1 = splits[u0 + 2];
     na0 = ca1;
-    rb_stage_issue(A, B, na, nb, ca0, ca1, (uint32_t)c0, (uint32_t)(c1 - c0 < RB_TILE ? c1 - c0 : RB_TILE),
-                   s_stage[0], s_prev[0]);
+    rb_stage_issue(A, B, ca0, ca1, (uint32_t)c0, (uint32_t)(c1 - c0 < RB_TILE ? c1 - c0 : RB_TILE), s_stage[0],
+                   s_prev[0]);
   }
   int buf = 0;
   for (uint64_t d0 = c0; d0 < c1; d0 += RB_TILE) {
@@ -913,7 +883,7 @@ __global__ __launch_bounds__(RB_THREADS, 6) void k_rb_write(uint8_t* __restrict_
       if (threadIdx.x < RB_TILE / 32) s_bmp[threadIdx.x] = 0u;
       __syncthreads();
       if (dn < c1)
-        rb_stage_issue(A, B, na, nb, na0, na1, (uint32_t)dn, (uint32_t)(c1 - dn < RB_TILE ? c1 - dn : RB_TILE),
+        rb_stage_issue(A, B, na0, na1, (uint32_t)dn, (uint32_t)(c1 - dn < RB_TILE ? c1 - dn : RB_TILE),
                        s_stage[buf ^ 1], s_prev[buf ^ 1]);
       const uint32_t nn1 = dn + RB_TILE < c1 ? splits[u + 3] : 0u;
       rb_merge_staged(s_stage[buf], s_prev[buf], ca0, ca1, (uint32_t)d0, tn, v, tags, s_bmp, s_wpre);
