@@ -1112,7 +1112,8 @@ __global__ __launch_bounds__(WS_THREADS) void k_wm_sum(RsqfDev f, const uint32_t
 // window's exclusive prefix: lp = its new incoming location) and wold; more
 // than WM_SIDE blocks to hand over raises the fallback flag.
 __global__ __launch_bounds__(256) void k_wm_check(const uint4* __restrict__ pre, const uint32_t* __restrict__ wold,
-                                                  uint32_t W, uint32_t* __restrict__ wsave,
+                                                  const uint32_t* __restrict__ bstart, uint32_t W,
+                                                  uint32_t* __restrict__ wsave, uint8_t* __restrict__ wclean,
                                                   uint32_t* __restrict__ ctr) {
   const uint32_t x = blockIdx.x * 256 + threadIdx.x;
   if (x >= W) return;
@@ -1120,15 +1121,30 @@ __global__ __launch_bounds__(256) void k_wm_check(const uint4* __restrict__ pre,
   const uint64_t h = lp > o ? lp : o;
   if (h > (uint64_t)x * RD_SLOTS + 64 * WM_SIDE) ctr[CTR_MISC] = 1u;
   wsave[x] = (uint32_t)h;
+  // no batch entry and the same incoming location: the canonical layout of
+  // the window's slots is the one stored (its outgoing location is unchanged
+  // too), so the window is not rewritten -- unless its runs reach into the
+  // next window, whose cleared blocks would then lose them
+  const uint32_t s0 = x * RD_SLOTS;
+  bool clean = bstart[x + 1] == bstart[x] && (lp > s0 ? lp : s0) == (o > s0 ? o : s0);
+  if (clean && x + 1 < W) {
+    const uint32_t lp1 = rs_unpack(pre[x + 1]).lp, o1 = wold[x + 1];
+    clean = (lp1 > o1 ? lp1 : o1) <= s0 + RD_SLOTS;
+  }
+  wclean[x] = clean ? 1 : 0;
 }
 
 // Blocks of window w below its handover bound: run ends and remainders to
 // the side copy (WM_SIDE blocks per window), then cleared in the table.
 __global__ __launch_bounds__(64) void k_wm_save(uint8_t* __restrict__ base, uint32_t nb, uint32_t r,
-                                                const uint32_t* __restrict__ wsave, uint64_t* __restrict__ side_end,
+                                                const uint32_t* __restrict__ wsave,
+                                                const uint8_t* __restrict__ wclean, uint64_t* __restrict__ side_end,
                                                 uint32_t* __restrict__ side_rem, const uint32_t* __restrict__ ctr) {
   if (ctr[CTR_ERR] || ctr[CTR_MISC]) return;
   const uint32_t w = blockIdx.x, t = threadIdx.x;
+  // a window that is not rewritten keeps its blocks: its neighbour reads the
+  // copy and ORs in the same bits it finds there
+  const bool keep = wclean[w] != 0;
   const uint64_t s0 = (uint64_t)w * RD_SLOTS, h = wsave[w];
   if (h <= s0) return;
   const uint64_t bw = (uint64_t)w * RD_BLOCKS;
@@ -1140,11 +1156,11 @@ __global__ __launch_bounds__(64) void k_wm_save(uint8_t* __restrict__ base, uint
   const uint32_t rw = 2 * r;
   for (uint32_t i = t; i < nsv; i += 64) {
     side_end[(uint64_t)w * WM_SIDE + i] = meta[4 * (bw + i) + 1];
-    meta[4 * (bw + i) + 1] = 0ull;
+    if (!keep) meta[4 * (bw + i) + 1] = 0ull;
   }
   for (uint32_t i = t; i < nsv * rw; i += 64) {
     side_rem[(uint64_t)w * WM_SIDE * rw + i] = remg[bw * rw + i];
-    remg[bw * rw + i] = 0u;
+    if (!keep) remg[bw * rw + i] = 0u;
   }
 }
 
@@ -1197,10 +1213,11 @@ __global__ __launch_bounds__(WM_THREADS, (R == 8 ? 6 : 4)) void k_wm_write(uint8
                                                             const uint32_t* __restrict__ bstart,
                                                             const uint4* __restrict__ pre,
                                                             const uint32_t* __restrict__ wsave,
+                                                            const uint8_t* __restrict__ wclean,
                                                             const uint64_t* __restrict__ side_end,
                                                             const uint32_t* __restrict__ side_rem,
                                                             const uint32_t* __restrict__ ctr) {
-  if (ctr[CTR_ERR] || ctr[CTR_MISC]) return;
+  if (ctr[CTR_ERR] || ctr[CTR_MISC] || wclean[blockIdx.x]) return;
   constexpr uint32_t RW = R ? 2 * (uint32_t)R : 32;  // remainder words per image block
   constexpr uint32_t RL = (WM_IMG * RW + WM_THREADS - 1) / WM_THREADS;
   __shared__ __align__(16) uint32_t s_rem[WM_IMG * RW];
@@ -1639,7 +1656,8 @@ static gqf_status rsqf_insert_t(const gqf_rsqf& f, uint32_t n, const uint32_t* k
 static size_t rsqf_win_ws(const RsqfGeom& g) {
   const uint32_t W = rd_windows(g.nb);
   return aligned_bytes((size_t)W * 8) + 256 + scan_rows_bytes(W, 2) + aligned_bytes(((size_t)W + 1) * 4) +
-         aligned_bytes((size_t)W * 16) + 2 * aligned_bytes((size_t)W * 4) + aligned_bytes((size_t)W * WM_SIDE * 8) +
+         aligned_bytes((size_t)W * 16) + 2 * aligned_bytes((size_t)W * 4) + aligned_bytes(W) +
+         aligned_bytes((size_t)W * WM_SIDE * 8) +
          aligned_bytes((size_t)W * WM_SIDE * 8 * g.r) + aligned_bytes(((W + RBS_PER - 1) / RBS_PER) * 16) + 1024;
 }
 // Tables the windowed insert takes: one scan state per window.
@@ -1660,6 +1678,7 @@ static gqf_status rsqf_insert_win(hipStream_t s, DeviceState& ds, Arena& ar, con
   uint4* agg = ar.take<uint4>(W);
   uint32_t* wold = ar.take<uint32_t>(W);
   uint32_t* wsave = ar.take<uint32_t>(W);
+  uint8_t* wclean = ar.take<uint8_t>(W);
   uint64_t* side_end = ar.take<uint64_t>((size_t)W * WM_SIDE);
   uint32_t* side_rem = ar.take<uint32_t>((size_t)W * WM_SIDE * 2 * g.r);
   if (ar.used > ar.cap) {
@@ -1673,14 +1692,14 @@ static gqf_status rsqf_insert_win(hipStream_t s, DeviceState& ds, Arena& ar, con
   GQF_LAUNCH_B(meta * 1.5 + (double)n * sizeof(K), (k_wm_sum<K>), dim3(W), dim3(WS_THREADS), 0, s, d, wp, B, bstart,
                agg, wold, ds.counters);
   GQF_TRY(rb_scan(s, ar, agg, W, g.cap, ds.counters, nullptr));
-  GQF_LAUNCH(k_wm_check, dim3((W + 255) / 256), dim3(256), 0, s, agg, wold, W, wsave, ds.counters);
-  GQF_LAUNCH(k_wm_save, dim3(W), dim3(64), 0, s, reinterpret_cast<uint8_t*>(f.blocks), g.nb, g.r, wsave, side_end,
-             side_rem, ds.counters);
+  GQF_LAUNCH(k_wm_check, dim3((W + 255) / 256), dim3(256), 0, s, agg, wold, bstart, W, wsave, wclean, ds.counters);
+  GQF_LAUNCH(k_wm_save, dim3(W), dim3(64), 0, s, reinterpret_cast<uint8_t*>(f.blocks), g.nb, g.r, wsave, wclean,
+             side_end, side_rem, ds.counters);
   // the table read once and written once (plus the batch)
   const double wbytes = 2.0 * (double)rsqf_dev_bytes(g.q, g.r) + (double)n * sizeof(K);
 #define GQF_WM_WRITE(R_)                                                                                          \
   GQF_LAUNCH_B(wbytes, (k_wm_write<K, R_>), dim3(W), dim3(WM_THREADS), 0, s, reinterpret_cast<uint8_t*>(f.blocks), \
-               g.nb, g.r, W, wp, B, bstart, agg, wsave, side_end, side_rem, ds.counters)
+               g.nb, g.r, W, wp, B, bstart, agg, wsave, wclean, side_end, side_rem, ds.counters)
   if (g.r == 8) GQF_WM_WRITE(8);
   else if (g.r == 16) GQF_WM_WRITE(16);
   else GQF_WM_WRITE(0);

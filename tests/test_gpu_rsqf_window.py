@@ -210,3 +210,22 @@ def test_window_insert_on_reference_tables():
                 finally:
                     G.set_rsqf_insert_path(G.INSERT_AUTO)
             assert (out[0] == out[1]).all(), np.flatnonzero(out[0] != out[1])[:8]
+
+
+@pytest.mark.parametrize("q,r,alpha", [(20, 8, 0.5), (21, 13, 0.85), (19, 5, 0.95)])
+def test_window_insert_small_batches_skip_untouched_windows(q, r, alpha):
+    """Batches far smaller than the table leave most windows untouched: those
+    keep their bytes (no batch entry, same incoming location, no runs into
+    the next window) while their neighbours are rewritten around them --
+    including a neighbour that ORs its runs into the untouched window's first
+    blocks.  Tables equal the rebuild's and the canonical layout after every
+    batch, 1 to 2000 keys."""
+    say("rsqf window small batches", q, r, alpha)
+    n0 = O.key_count(alpha, q)
+    keys = O.mt19937(83, n0 + 4000)
+    sizes = [1, 2, 7, 50, 300, 2000]
+    cuts = np.cumsum([n0] + sizes)
+    batches = [keys[a:b] for a, b in zip(cuts[:-1], cuts[1:])]
+    tables, status = _both(q, r, keys[:n0], batches)
+    assert all(s == G.OK for s in status)
+    assert (tables[-1] == canon(keys[:cuts[-1]], q, r)).all()
