@@ -304,9 +304,11 @@ __global__ __launch_bounds__(256) void k_rs_upsweep(const void* __restrict__ in,
 }
 
 // Row d (one workgroup per digit): exclusive scan of the G chunk counts in
-// place; the row total goes to totals[d].
+// place; the row total goes to totals[d].  crowd (may be null): a word of
+// host-coherent memory set to 1 when a row total exceeds `limit`.
 [[maybe_unused]] static __global__ __launch_bounds__(1024) void k_rs_rowscan(uint32_t* __restrict__ hist, uint32_t G,
-                                                            uint32_t* __restrict__ totals) {
+                                                            uint32_t* __restrict__ totals,
+                                                            uint32_t* __restrict__ crowd, uint32_t limit) {
   __shared__ uint32_t s_w[16];
   __shared__ uint32_t s_carry;
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
@@ -331,7 +333,10 @@ __global__ __launch_bounds__(256) void k_rs_upsweep(const void* __restrict__ in,
     if (tid == 0) s_carry = carry + tot;
     __syncthreads();
   }
-  if (tid == 0) totals[blockIdx.x] = s_carry;
+  if (tid == 0) {
+    totals[blockIdx.x] = s_carry;
+    if (crowd && s_carry > limit) __hip_atomic_store(crowd, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
 }
 
 // ---------------------------------------------------------------- LDS tile sort
@@ -717,6 +722,71 @@ __global__ __launch_bounds__(SORT_THREADS) void k_rs_small(const void* __restric
   }
 }
 
+// ---------------------------------------------------------------- segment sort
+// A sort of SORT_TILE+1 .. SEG_MAX_N keys without values takes two steps in
+// place of one chunked pass per 8 bits: one chunked pass on the top 8 bits
+// of the key range (a stable partition into 256 segments of equal top
+// digit), then k_rs_seg, one workgroup per segment, sorts the bits below in
+// LDS with every 4-bit counting pass of the tile sort back to back.  That is
+// 4 launches instead of 12 (31-bit fingerprints) or 15 (34-bit ones), which
+// is what a sort of 2^16..2^20 keys is bound by.
+//   Uniform keys fill a segment to SEG_MAX_N / 256 = half a tile on average.
+// Skewed keys can crowd one top digit (the AP hash of small integers puts
+// most of them under a few top-bit values): the pass's row scan raises a
+// host-visible flag when a segment exceeds SEG_TILE, the host reads it while
+// the partition pass runs, and then sorts the partitioned keys with the
+// chunked passes instead.
+constexpr int SEG_THREADS = 512;
+constexpr int SEG_TILE = SEG_THREADS * SORT_ITEMS;  // 8192 keys
+constexpr uint32_t SEG_MAX_N = 256u * 4096u;
+
+// keys: the partitioned array, each segment sorted in place by its bits
+// [0, nbits) (nbits = the top digit's shift); digit_totals: the 256 segment
+// sizes, each at most SEG_TILE.
+template <typename K>
+__global__ __launch_bounds__(SEG_THREADS) void k_rs_seg(K* __restrict__ keys, const uint32_t* __restrict__ digit_totals,
+                                                        uint32_t nbits) {
+  constexpr int T = SEG_THREADS, W = T / 64;
+  __shared__ __align__(16) K s_keys[SEG_TILE];
+  __shared__ __align__(16) uint16_t s_cnt[16 * T];
+  __shared__ uint32_t s_tmp[W];
+  __shared__ uint32_t s_seg[2];
+  const uint32_t tid = threadIdx.x;
+  {
+    const uint32_t c = tid < 256 ? digit_totals[tid] : 0u;
+    const uint32_t pre = block_excl_scan<T>(c, s_tmp);
+    if (tid == blockIdx.x) {
+      s_seg[0] = pre;
+      s_seg[1] = c;
+    }
+    __syncthreads();
+  }
+  const uint32_t c0 = s_seg[0], m = s_seg[1];
+  // the whole workgroup (s_seg is shared); a segment above SEG_TILE is not
+  // launched here (the host sorts crowded keys with the chunked passes)
+  if (m <= 1 || m > (uint32_t)SEG_TILE) return;
+  K k[SORT_ITEMS];
+  const uint32_t v[SORT_ITEMS] = {};  // no values
+  K* seg = keys + c0;
+  ts_load_keys<K, false>(seg, 0, m, 0u, (reinterpret_cast<uintptr_t>(seg) & 15u) == 0, k);
+  const uint32_t np = (nbits + 3) / 4;
+  for (uint32_t j = 0; j < np; ++j) {
+    uint64_t rk;
+    ts_count<T>(k, 4 * j, rk, s_cnt);
+    __syncthreads();
+    (void)ts_scan<T>(s_cnt, 0u, s_tmp);
+    __syncthreads();
+    ts_scatter<T, K, false>(k, v, 4 * j, rk, s_cnt, s_keys, nullptr);
+    __syncthreads();
+    // the next pass writes s_cnt, and s_keys only after its first barrier
+    if (j + 1 < np) {
+      uint32_t unused[SORT_ITEMS];
+      ts_read_blocked<K, false>(s_keys, nullptr, k, unused);
+    }
+  }
+  for (uint32_t p = tid; p < m; p += T) seg[p] = s_keys[ts_swz<sizeof(K)>(p)];
+}
+
 inline size_t sort_workspace_bytes(uint64_t n, size_t key_bytes, bool vals) {
   const SortPlan p = sort_plan((uint32_t)n);
   return 2 * aligned_bytes(n * key_bytes) + (vals ? 2 * aligned_bytes(n * 4) : 0) +
@@ -727,10 +797,12 @@ inline size_t sort_workspace_bytes(uint64_t n, size_t key_bytes, bool vals) {
 // input is raw uint32 keys, fingerprinted (fp_bits) as they are read in the
 // first pass.  vals: nullptr + with_vals -> the original index is carried.
 // Returns the buffers holding the result through *keys_out / *vals_out.
+// With the stream's context (ds), a keys-only sort of at most SEG_MAX_N keys
+// takes the partition + segment sort (it waits for the partition's row scan).
 template <typename K>
 gqf_status radix_sort(hipStream_t s, Arena& ar, const void* in, bool hash, uint32_t fp_bits,
                       const uint32_t* in_vals, bool with_vals, uint32_t n, uint32_t lo_bit, uint32_t hi_bit,
-                      K** keys_out, uint32_t** vals_out) {
+                      K** keys_out, uint32_t** vals_out, DeviceState* ds = nullptr) {
   K* kb[2] = {ar.take<K>(n ? n : 1), ar.take<K>(n ? n : 1)};
   uint32_t* vb[2] = {nullptr, nullptr};
   if (with_vals) {
@@ -771,6 +843,39 @@ gqf_status radix_sort(hipStream_t s, Arena& ar, const void* in, bool hash, uint3
   const void* src = in;
   const uint32_t* vsrc = in_vals;
   int cur = 0;
+  if (ds && !with_vals && lo_bit == 0 && hi_bit >= 16 && npass >= 2 && n <= SEG_MAX_N) {
+    // the top 8 bits in one chunked pass, then every segment in LDS
+    const uint32_t shift = hi_bit - 8;
+    *reinterpret_cast<volatile uint32_t*>(ds->h_crowd) = 0u;
+    if (hash) {
+      GQF_LAUNCH_B((double)n * 4.0, (k_rs_upsweep<K, true>), dim3(plan.G), dim3(256), 0, s, in, n, fp_bits, shift,
+                   plan.chunk, plan.G, hist);
+    } else {
+      GQF_LAUNCH_B((double)n * sizeof(K), (k_rs_upsweep<K, false>), dim3(plan.G), dim3(256), 0, s, in, n, fp_bits,
+                   shift, plan.chunk, plan.G, hist);
+    }
+    GQF_LAUNCH(k_rs_rowscan, dim3(256), dim3(1024), 0, s, hist, plan.G, dbase, ds->d_crowd, (uint32_t)SEG_TILE);
+    GQF_HIP(hipEventRecord(ds->ev_crowd, s));
+    const double pass_bytes = (double)n * ((hash ? 4.0 : (double)sizeof(K)) + sizeof(K));
+    if (hash)
+      GQF_LAUNCH_B(pass_bytes, (k_rs_downsweep<K, false, true>), dim3(plan.G), dim3(ds_threads<K, false>()), 0, s, in,
+                   nullptr, kb[0], nullptr, n, fp_bits, shift, plan.chunk, plan.G, hist, dbase);
+    else
+      GQF_LAUNCH_B(pass_bytes, (k_rs_downsweep<K, false, false>), dim3(plan.G), dim3(ds_threads<K, false>()), 0, s, in,
+                   nullptr, kb[0], nullptr, n, fp_bits, shift, plan.chunk, plan.G, hist, dbase);
+    // the host reads the flag while the partition pass runs
+    GQF_HIP(hipEventSynchronize(ds->ev_crowd));
+    if (*reinterpret_cast<volatile uint32_t*>(ds->h_crowd) == 0u) {
+      GQF_LAUNCH_B((double)n * 2.0 * sizeof(K), (k_rs_seg<K>), dim3(256), dim3(SEG_THREADS), 0, s, kb[0], dbase, shift);
+      *keys_out = kb[0];
+      if (vals_out) *vals_out = nullptr;
+      return GQF_OK;
+    }
+    // a crowded segment: every chunked pass over the partitioned fingerprints
+    src = kb[0];
+    cur = 1;
+    hash = false;
+  }
   for (uint32_t p = 0; p < npass; ++p) {
     const uint32_t shift = lo_bit + 8 * p;
     bool first_hash = hash && p == 0;
@@ -786,7 +891,7 @@ gqf_status radix_sort(hipStream_t s, Arena& ar, const void* in, bool hash, uint3
                    shift, plan.chunk, plan.G, hist);
     }
     const double kin = first_hash ? 4.0 : (double)sizeof(K);
-    GQF_LAUNCH(k_rs_rowscan, dim3(256), dim3(1024), 0, s, hist, plan.G, dbase);
+    GQF_LAUNCH(k_rs_rowscan, dim3(256), dim3(1024), 0, s, hist, plan.G, dbase, (uint32_t*)nullptr, 0u);
     K* dst = kb[cur];
     uint32_t* vdst = vb[cur];
     // read keys (+ values) once, write them once

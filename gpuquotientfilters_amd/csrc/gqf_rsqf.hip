@@ -1627,7 +1627,7 @@ static gqf_status rsqf_build_t(const gqf_rsqf& f, uint32_t n, const uint32_t* ke
   GQF_TRY(ws_reserve(ds, sort_workspace_bytes(n, sizeof(K), false) + rsqf_rebuild_ws(n)));
   Arena ar(ds.ws, ds.ws_cap);
   K* sorted;
-  GQF_TRY(radix_sort<K>(cs.stream, ar, keys, true, g.bits, nullptr, false, n, 0, g.bits, &sorted, nullptr));
+  GQF_TRY(radix_sort<K>(cs.stream, ar, keys, true, g.bits, nullptr, false, n, 0, g.bits, &sorted, nullptr, cs.ds));
   return rsqf_rebuild<K, KEEP_ALL, false>(cs.stream, ds, ar, f, sorted, n, nullptr, (const K*)nullptr, 0u, nullptr);
 }
 
@@ -1647,7 +1647,7 @@ static gqf_status rsqf_insert_t(const gqf_rsqf& f, uint32_t n, const uint32_t* k
   const uint32_t* hb = nullptr;
   GQF_TRY(rsqf_decode<K>(cs.stream, ar, f, &A, &m_dev, &hb));
   K* B;
-  GQF_TRY(radix_sort<K>(cs.stream, ar, keys, true, g.bits, nullptr, false, n, 0, g.bits, &B, nullptr));
+  GQF_TRY(radix_sort<K>(cs.stream, ar, keys, true, g.bits, nullptr, false, n, 0, g.bits, &B, nullptr, cs.ds));
   // at most cap stored entries: the grids are sized for cap + n
   return rsqf_rebuild<K, KEEP_ALL, true>(cs.stream, ds, ar, f, A, (uint32_t)g.cap, m_dev, B, n, nullptr, hb);
 }
@@ -1714,7 +1714,7 @@ static gqf_status rsqf_insert_win_t(const gqf_rsqf& f, uint32_t n, const uint32_
   GQF_TRY(ws_reserve(ds, sort_workspace_bytes(n, sizeof(K), false) + rsqf_win_ws(g)));
   Arena ar(ds.ws, ds.ws_cap);
   K* B;
-  GQF_TRY(radix_sort<K>(cs.stream, ar, keys, true, g.bits, nullptr, false, n, 0, g.bits, &B, nullptr));
+  GQF_TRY(radix_sort<K>(cs.stream, ar, keys, true, g.bits, nullptr, false, n, 0, g.bits, &B, nullptr, cs.ds));
   return rsqf_insert_win<K>(cs.stream, ds, ar, f, B, n);
 }
 
@@ -1735,7 +1735,7 @@ static gqf_status rsqf_delete_t(const gqf_rsqf& f, uint32_t n, const uint32_t* k
   const uint32_t* hb = nullptr;
   GQF_TRY(rsqf_decode<K>(cs.stream, ar, f, &A, &m_dev, &hb));
   K* D;
-  GQF_TRY(radix_sort<K>(cs.stream, ar, keys, true, g.bits, nullptr, false, n, 0, g.bits, &D, nullptr));
+  GQF_TRY(radix_sort<K>(cs.stream, ar, keys, true, g.bits, nullptr, false, n, 0, g.bits, &D, nullptr, cs.ds));
   const uint32_t mcap = (uint32_t)g.cap;  // the grids cover the capacity; kernels read the count
   uint8_t* keep = ar.take<uint8_t>((size_t)mcap + 1);
   if (ar.used > ar.cap) {

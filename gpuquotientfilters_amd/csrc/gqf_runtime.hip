@@ -60,6 +60,9 @@ gqf_status device_state(hipStream_t stream, DeviceState** out) {
   GQF_HIP(hipHostMalloc((void**)&ds->h_pinned, CTR_TOTAL * sizeof(uint32_t), hipHostMallocDefault));
   GQF_HIP(hipEventCreate(&ds->ev0));
   GQF_HIP(hipEventCreate(&ds->ev1));
+  GQF_HIP(hipHostMalloc((void**)&ds->h_crowd, 64, hipHostMallocMapped | hipHostMallocCoherent));
+  GQF_HIP(hipHostGetDevicePointer((void**)&ds->d_crowd, ds->h_crowd, 0));
+  GQF_HIP(hipEventCreateWithFlags(&ds->ev_crowd, hipEventDisableTiming));
   GQF_HIP(hipDeviceSynchronize());
   ds->device = dev;
   ds->stream = stream;
@@ -101,6 +104,8 @@ static gqf_status release_state(hipStream_t stream) {
     GQF_HIP(hipHostFree(ds->h_pinned));
     GQF_HIP(hipEventDestroy(ds->ev0));
     GQF_HIP(hipEventDestroy(ds->ev1));
+    GQF_HIP(hipHostFree(ds->h_crowd));
+    GQF_HIP(hipEventDestroy(ds->ev_crowd));
     for (size_t i = 0; i < ds->pool_size; ++i) (void)hipEventDestroy(ds->pool[i]);
     free(ds->pool);
     free(ds->pending);

@@ -71,6 +71,12 @@ struct DeviceState {
   uint32_t* counters = nullptr;     // CTR_TOTAL device counters
   uint32_t* h_pinned = nullptr;     // pinned host mirror of the counters
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // sort probe: a word of host-coherent pinned memory (host and device views)
+  // that a sort's row scan raises when its top-digit segments are too large
+  // for the segment sort, and the event recorded after that scan
+  uint32_t* h_crowd = nullptr;
+  uint32_t* d_crowd = nullptr;
+  hipEvent_t ev_crowd = nullptr;
   // per-launch timing (gqf_profile_enable): event pool and launches to resolve
   hipEvent_t* pool = nullptr;
   size_t pool_size = 0, pool_used = 0;

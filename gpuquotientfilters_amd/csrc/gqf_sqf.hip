@@ -198,11 +198,12 @@ static gqf_status check_keys(int n, const void* keys) {
 // does not fit raises the capacity flag in counters[CTR_ERR] before any
 // byte of the table is written.
 static size_t sqf_build_ws_bytes(uint32_t n) { return sort_workspace_bytes(n, 4, false) + rebuild_ws_bytes(n); }
+// ds: the stream's context (null: the sort takes its chunked passes only).
 static gqf_status sqf_build_launch(hipStream_t s, Arena& ar, uint32_t* counters, const gqf_sqf& qf, const SqfGeom& g,
-                                  uint32_t un, const uint32_t* d_keys, int no_dup) {
+                                  uint32_t un, const uint32_t* d_keys, int no_dup, DeviceState* ds) {
   uint32_t* sorted = nullptr;
   GQF_TRY(radix_sort<uint32_t>(s, ar, d_keys, true, (g.q + g.r) | sqf_hash_flag(), nullptr, false, un, 0, g.q + g.r,
-                               &sorted, nullptr));
+                               &sorted, nullptr, ds));
   if (no_dup) return launch_rebuild<KEEP_UNIQUE, false>(s, counters, ar, qf, g, sorted, un, nullptr, nullptr, 0, nullptr);
   return launch_rebuild<KEEP_ALL, false>(s, counters, ar, qf, g, sorted, un, nullptr, nullptr, 0, nullptr);
 }
@@ -220,7 +221,7 @@ static gqf_status sqf_build_impl(const gqf_sqf& qf, int n, const uint32_t* d_key
   Arena ar(ds.ws, ds.ws_cap);
   gqf_sqf wq;
   GQF_TRY(work_table(cs.stream, ar, qf, g, false, &wq));
-  GQF_TRY(sqf_build_launch(cs.stream, ar, ds.counters, wq, g, un, d_keys, no_dup));
+  GQF_TRY(sqf_build_launch(cs.stream, ar, ds.counters, wq, g, un, d_keys, no_dup, &ds));
   finish_table(cs.stream, qf, wq, g, ds.counters);
   return cs.end(ms, true);  // a layout that does not fit is detected before any write
 }
@@ -339,7 +340,7 @@ static gqf_status sqf_insert_impl(const gqf_sqf& qf, int n, const uint32_t* d_ke
   Arena ar(ds.ws, ds.ws_cap);
   uint32_t* B = nullptr;
   GQF_TRY(radix_sort<uint32_t>(cs.stream, ar, d_keys, true, (g.q + g.r) | sqf_hash_flag(), nullptr, false, un, 0, g.q + g.r, &B,
-                               nullptr));
+                               nullptr, &ds));
   if (local) {
     const size_t mark = ar.used;
     bool done = false;
@@ -376,7 +377,7 @@ static gqf_status sqf_delete_impl(const gqf_sqf& qf, int n, const uint32_t* d_ke
   Arena ar(ds.ws, ds.ws_cap);
   uint32_t* D = nullptr;
   GQF_TRY(radix_sort<uint32_t>(cs.stream, ar, d_keys, true, (g.q + g.r) | sqf_hash_flag(), nullptr, false, un, 0, g.q + g.r, &D,
-                               nullptr));
+                               nullptr, &ds));
   gqf_sqf wq;
   GQF_TRY(work_table(cs.stream, ar, qf, g, true, &wq));
   uint32_t* A;
@@ -476,7 +477,7 @@ gqf_status gqf_sqf_bulk_build_noalloc(gqf_sqf qf, int method, int n, const uint3
   GQF_LAUNCH(k_reset_counters, dim3(1), dim3(64), 0, s, counters, 0, (int)CTR_TOTAL);
   gqf_sqf wq;
   GQF_TRY(work_table(s, ar, qf, g, false, &wq));
-  GQF_TRY(sqf_build_launch(s, ar, counters, wq, g, un, d_keys, no_dup));
+  GQF_TRY(sqf_build_launch(s, ar, counters, wq, g, un, d_keys, no_dup, nullptr));
   finish_table(s, qf, wq, g, counters);
   GQF_HIP(hipGetLastError());
   uint32_t err = 0;

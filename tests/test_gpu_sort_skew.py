@@ -2,7 +2,10 @@
 sort (builds, inserts, sorted lookups), bit-exact against the oracle:
 
   * sizes at the bounds of the sort's forms (4096 keys and fewer: one
-    workgroup; 4097 on: chunked LSD passes) and around 2^18;
+    workgroup; 4097 to 2^20 keys: a partition on the top 8 bits, then one
+    workgroup per segment, or the chunked LSD passes when a segment holds
+    more than 8192 keys; above 2^20: the chunked LSD passes) and around
+    2^18;
   * one key repeated thousands of times (a digit value holding most of a
     chunk);
   * thousands of distinct keys whose fingerprints share their top 8 bits,
