@@ -997,11 +997,15 @@ __global__ __launch_bounds__(64) void k_rr_tail(uint8_t* __restrict__ base, cons
 // its two-window image, or more than WM_SIDE blocks to hand over sends the
 // call to the decode-and-rebuild path: the flag ctr[CTR_MISC] is raised
 // before anything is written, and every writer checks it.
-constexpr uint32_t WM_THREADS = RB_THREADS;        // k_wm_write: 16 image slots per thread
-constexpr uint32_t WM_ITEMS = 16;                  // merged entries per thread
-constexpr uint32_t WM_CAP = WM_THREADS * WM_ITEMS;  // 8192 merged entries per window
-constexpr uint32_t WM_IMG = 2 * RD_BLOCKS;          // image: windows w and w + 1 (128 blocks)
 constexpr uint32_t WM_SIDE = 8;                     // blocks a window hands over at most (512 slots)
+constexpr uint32_t WM_WAVES = 8;                    // k_wm_write: 16 image slots per thread
+constexpr uint32_t WM_THREADS = 64 * WM_WAVES;
+constexpr uint32_t WM_ITEMS = 16;                   // merged entries per thread
+constexpr uint32_t WM_CAP = WM_THREADS * WM_ITEMS;  // 8192 merged entries per window
+// image: windows w and w + 1 (128 blocks, 16 slots per thread); the
+// window's entries end before block RD_BLOCKS + WM_SIDE (k_wm_check)
+constexpr uint32_t WM_IMG = WM_THREADS / 4;
+static_assert(WM_IMG >= RD_BLOCKS + WM_SIDE, "the image holds every slot a window's entries reach");
 constexpr uint32_t WS_THREADS = 256;                // k_wm_sum: 16 quotients, 32 image slots per thread
 
 // n copies of quotient fq appended to a scan state (rs_push n times).
@@ -1227,7 +1231,7 @@ __global__ __launch_bounds__(WM_THREADS, (R == 8 ? 6 : 4)) void k_wm_write(uint8
   __shared__ uint32_t s_occ[RD_BLOCKS * 2];
   __shared__ int32_t s_L[RD_BLOCKS];
   __shared__ uint32_t s_tmp[WM_THREADS / 64];
-  __shared__ uint4 s_wave[RB_WAVES + 1];
+  __shared__ uint4 s_wave[WM_WAVES + 1];
   const uint32_t w = blockIdx.x, t = threadIdx.x, lane = t & 63, wid = t >> 6;
   const uint32_t rr = R ? (uint32_t)R : r, rw = 2 * rr;
   uint64_t* meta = reinterpret_cast<uint64_t*>(base);
@@ -1271,7 +1275,7 @@ __global__ __launch_bounds__(WM_THREADS, (R == 8 ? 6 : 4)) void k_wm_write(uint8
     const uint32_t* B32 = reinterpret_cast<const uint32_t*>(B);
     constexpr uint32_t KW = sizeof(K) / 4;
     for (uint32_t k = 0; k * WM_THREADS < nBc; ++k) {
-      const uint32_t bs = (k * RB_WAVES + wid) * 64;
+      const uint32_t bs = (k * WM_WAVES + wid) * 64;
       if (bs + lane < nBc)
         __builtin_amdgcn_global_load_lds(B32 + KW * ((uint64_t)b0 + bs + lane), s_m + (WM_CAP - nBc) + bs, 4, 0, 0);
     }
@@ -1301,7 +1305,7 @@ __global__ __launch_bounds__(WM_THREADS, (R == 8 ? 6 : 4)) void k_wm_write(uint8
   __syncthreads();
   uint32_t qv[16];
   uint32_t m = 0;
-  {
+  if ((pk & 0xFFFFu) < d + O) {  // else every slot of this thread lies after the window's last run
     uint32_t k = pk & 0xFFFFu;
 #pragma unroll
     for (uint32_t i = 0; i < 16; ++i) {
@@ -1344,17 +1348,18 @@ __global__ __launch_bounds__(WM_THREADS, (R == 8 ? 6 : 4)) void k_wm_write(uint8
       atomicOr(&s_bmp[p >> 5], 1u << (p & 31));
     }
     __syncthreads();
+    static_assert(WM_CAP / 32 <= 256, "one wave scans the bitmap, 4 words per lane");
     if (t < 64) {
       uint32_t c[4], sum = 0;
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        c[i] = (uint32_t)__popc(s_bmp[4 * t + i]);
+        c[i] = 4 * t + i < WM_CAP / 32 ? (uint32_t)__popc(s_bmp[4 * t + i]) : 0u;
         sum += c[i];
       }
       uint32_t run = wave_incl_add(sum) - sum;
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        s_bpre[4 * t + i] = run;
+        if (4 * t + i < WM_CAP / 32) s_bpre[4 * t + i] = run;
         run += c[i];
       }
     }
@@ -1363,14 +1368,19 @@ __global__ __launch_bounds__(WM_THREADS, (R == 8 ? 6 : 4)) void k_wm_write(uint8
     uint32_t bi = s_bpre[i0 >> 5] + (uint32_t)__popc(wd & ((1u << (i0 & 31)) - 1u));
     uint32_t ai = i0 - bi;
     const uint32_t bits = wd >> (i0 & 31);
-    // branch-free: one LDS read per entry at the chosen index
+    if (i0 < nM) {
+      // branch-free: one LDS read per entry at the chosen index
 #pragma unroll
-    for (uint32_t j = 0; j < WM_ITEMS; ++j) {
-      const uint32_t isb = (bits >> j) & 1u;
-      const uint32_t at = isb ? (WM_CAP - nB) + bi : ai;
-      v[j] = i0 + j < nM ? s_m[at < WM_CAP ? at : 0u] - (isb ? kb32 : 0u) : 0u;
-      bi += isb;
-      ai += isb ^ 1u;
+      for (uint32_t j = 0; j < WM_ITEMS; ++j) {
+        const uint32_t isb = (bits >> j) & 1u;
+        const uint32_t at = isb ? (WM_CAP - nB) + bi : ai;
+        v[j] = i0 + j < nM ? s_m[at < WM_CAP ? at : 0u] - (isb ? kb32 : 0u) : 0u;
+        bi += isb;
+        ai += isb ^ 1u;
+      }
+    } else {
+#pragma unroll
+      for (uint32_t j = 0; j < WM_ITEMS; ++j) v[j] = 0u;
     }
   } else {
 #pragma unroll
@@ -1393,7 +1403,7 @@ __global__ __launch_bounds__(WM_THREADS, (R == 8 ? 6 : 4)) void k_wm_write(uint8
   if (t < WM_IMG * 2) s_end[t] = 0u;
   if (t < RD_BLOCKS * 2) s_occ[t] = 0u;
   RS ex;
-  (void)rs_block(a, &ex, s_wave);  // two barriers: the cleared images and the neighbours are visible after it
+  (void)rs_block_n<WM_WAVES>(a, &ex, s_wave);  // two barriers: the cleared images and the neighbours are visible after it
   const RS me = rs_combine(X, ex);
   const uint32_t s0 = w * RD_SLOTS;
   const uint32_t rmask = (1u << rr) - 1u;
