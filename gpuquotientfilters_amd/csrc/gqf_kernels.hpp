@@ -552,36 +552,22 @@ __device__ __forceinline__ void ts_read_blocked(const K* s_keys, const uint32_t*
 template <typename K, bool VALS>
 constexpr int ds_threads() { return (sizeof(K) == 4 && !VALS) ? 512 : 256; }
 
-// (the second bound is waves per SIMD: workgroups per CU times T/256)
-template <typename K, bool VALS, bool HASH>
-__global__ __launch_bounds__((ds_threads<K, VALS>()), (sizeof(K) == 4 ? (VALS ? 3 : 4) : (VALS ? 2 : 3))) void
-k_rs_downsweep(const void* __restrict__ in_keys, const uint32_t* __restrict__ in_vals, K* __restrict__ out_keys,
-               uint32_t* __restrict__ out_vals, uint32_t n, uint32_t fp_bits, uint32_t shift, uint32_t chunk,
-               uint32_t G, const uint32_t* __restrict__ hist, const uint32_t* __restrict__ digit_totals) {
-  constexpr int T = ds_threads<K, VALS>();
+// The downsweep of one chunk [c0, c1): sub-tile by sub-tile, sorted by the
+// digit in LDS and written to s_run[digit] + the runs of earlier sub-tiles
+// (s_run: the chunk's output base of every digit; s_hist: all zeros).
+template <typename K, bool VALS, bool HASH, int T>
+__device__ __forceinline__ void ds_chunk(const void* __restrict__ in_keys, const uint32_t* __restrict__ in_vals,
+                                         K* __restrict__ out_keys, uint32_t* __restrict__ out_vals, uint32_t fp_bits,
+                                         uint32_t shift, uint64_t c0, uint64_t c1, K* s_keys, uint32_t* s_vals,
+                                         uint16_t* s_cnt, uint32_t* s_hist, uint32_t* s_run, uint32_t* s_base,
+                                         uint32_t* s_tmp) {
   constexpr int TILE = T * SORT_ITEMS;
   constexpr int W = T / 64;
-  __shared__ __align__(16) K s_keys[TILE];
-  __shared__ __align__(16) uint32_t s_vals[VALS ? TILE : 4];
-  __shared__ __align__(16) uint16_t s_cnt[16 * T];
-  __shared__ uint32_t s_hist[W * 256];
-  __shared__ uint32_t s_run[256];
-  __shared__ uint32_t s_base[256];
-  __shared__ uint32_t s_tmp[W];
   const uint32_t tid = threadIdx.x, wid = tid >> 6;
-  const bool dig = tid < 256;  // the threads that own a digit
-#pragma unroll
-  for (int w = 0; w < W * 256 / T; ++w) s_hist[w * T + tid] = 0;
-  // digit bases: exclusive scan of the 256 digit totals (every workgroup
-  // does it; it saves a launch), plus this chunk's offset
-  {
-    const uint32_t base = block_excl_scan<T>(dig ? digit_totals[tid] : 0u, s_tmp);
-    if (dig) s_run[tid] = base + hist[(uint64_t)tid * G + blockIdx.x];
-  }
-  const uint64_t c0 = (uint64_t)blockIdx.x * chunk;
-  const uint64_t c1 = c0 + chunk < n ? c0 + chunk : (uint64_t)n;
-  const bool kvec = (reinterpret_cast<uintptr_t>(in_keys) & 15u) == 0;
-  const bool vvec = (reinterpret_cast<uintptr_t>(in_vals) & 15u) == 0;
+  const bool dig = tid < 256;
+  const size_t kin = (HASH || sizeof(K) == 4) ? 4 : sizeof(K);
+  const bool kvec = ((reinterpret_cast<uintptr_t>(in_keys) + c0 * kin) & 15u) == 0;
+  const bool vvec = ((reinterpret_cast<uintptr_t>(in_vals) + c0 * 4) & 15u) == 0;
   K k[SORT_ITEMS];
   uint32_t v[SORT_ITEMS];
   uint64_t rk;
@@ -640,6 +626,37 @@ k_rs_downsweep(const void* __restrict__ in_keys, const uint32_t* __restrict__ in
     // no barrier: the next sub-tile writes s_base and s_keys only after its
     // first barrier, which every thread reaches after these reads
   }
+}
+
+// (the second bound is waves per SIMD: workgroups per CU times T/256)
+template <typename K, bool VALS, bool HASH>
+__global__ __launch_bounds__((ds_threads<K, VALS>()), (sizeof(K) == 4 ? (VALS ? 3 : 4) : (VALS ? 2 : 3))) void
+k_rs_downsweep(const void* __restrict__ in_keys, const uint32_t* __restrict__ in_vals, K* __restrict__ out_keys,
+               uint32_t* __restrict__ out_vals, uint32_t n, uint32_t fp_bits, uint32_t shift, uint32_t chunk,
+               uint32_t G, const uint32_t* __restrict__ hist, const uint32_t* __restrict__ digit_totals) {
+  constexpr int T = ds_threads<K, VALS>();
+  constexpr int W = T / 64;
+  __shared__ __align__(16) K s_keys[T * SORT_ITEMS];
+  __shared__ __align__(16) uint32_t s_vals[VALS ? T * SORT_ITEMS : 4];
+  __shared__ __align__(16) uint16_t s_cnt[16 * T];
+  __shared__ uint32_t s_hist[W * 256];
+  __shared__ uint32_t s_run[256];
+  __shared__ uint32_t s_base[256];
+  __shared__ uint32_t s_tmp[W];
+  const uint32_t tid = threadIdx.x;
+  const bool dig = tid < 256;  // the threads that own a digit
+#pragma unroll
+  for (int w = 0; w < W * 256 / T; ++w) s_hist[w * T + tid] = 0;
+  // digit bases: exclusive scan of the 256 digit totals (every workgroup
+  // does it; it saves a launch), plus this chunk's offset
+  {
+    const uint32_t base = block_excl_scan<T>(dig ? digit_totals[tid] : 0u, s_tmp);
+    if (dig) s_run[tid] = base + hist[(uint64_t)tid * G + blockIdx.x];
+  }
+  const uint64_t c0 = (uint64_t)blockIdx.x * chunk;
+  const uint64_t c1 = c0 + chunk < n ? c0 + chunk : (uint64_t)n;
+  ds_chunk<K, VALS, HASH, T>(in_keys, in_vals, out_keys, out_vals, fp_bits, shift, c0, c1, s_keys, s_vals, s_cnt,
+                             s_hist, s_run, s_base, s_tmp);
 }
 
 // Every pass of a sort of at most SORT_TILE keys in one workgroup: the keys
@@ -723,36 +740,197 @@ __global__ __launch_bounds__(SORT_THREADS) void k_rs_small(const void* __restric
 }
 
 // ---------------------------------------------------------------- segment sort
-// A sort of SORT_TILE+1 .. SEG_MAX_N keys without values takes two steps in
-// place of one chunked pass per 8 bits: one chunked pass on the top 8 bits
-// of the key range (a stable partition into 256 segments of equal top
-// digit), then k_rs_seg, one workgroup per segment, sorts the bits below in
-// LDS with every 4-bit counting pass of the tile sort back to back.  That is
-// 4 launches instead of 12 (31-bit fingerprints) or 15 (34-bit ones), which
-// is what a sort of 2^16..2^20 keys is bound by.
-//   Uniform keys fill a segment to SEG_MAX_N / 256 = half a tile on average.
-// Skewed keys can crowd one top digit (the AP hash of small integers puts
-// most of them under a few top-bit values): the pass's row scan raises a
-// host-visible flag when a segment exceeds SEG_TILE, the host reads it while
-// the partition pass runs, and then sorts the partitioned keys with the
-// chunked passes instead.
+// A keys-only sort with the stream's context replaces the 8-bit passes of
+// the chunked sort by a partition on the top bits and one LDS sort per group
+// of keys:
+//   * 4097 .. 256 * SEG_GROUP_AVG keys: one chunked pass on the top 8 bits
+//     of the key range (a stable partition into 256 segments), then
+//     k_rs_seg, one workgroup per segment, sorts the bits below in LDS with
+//     every 4-bit counting pass of the tile sort back to back.  4 launches
+//     instead of 12 (31-bit fingerprints) or 15 (34-bit ones).
+//   * more keys: a second chunked pass over the segments on the next b2 bits
+//     (k_rs_upsweep_seg, k_rs_segbase, k_rs_downsweep_seg: each chunk lies in
+//     one segment, and the digit bases count within the segment), so that
+//     256 << b2 groups of at most ~7600 keys on average remain for k_rs_seg.
+//     Two chunked passes and one LDS pass instead of 4 (or 5) chunked passes.
+// Within a group every key shares the bits above the sorted range, so the
+// LDS sort holds the low 32 bits only (also for 64-bit keys).
+//   Skewed keys can crowd a group (the AP hash puts most small integer keys
+// under a few top-bit values): the row scan of the last partition raises a
+// host-visible flag when a group exceeds SEG_TILE, the host reads it while
+// the partition's downsweep runs, and then sorts the partitioned keys with
+// the chunked passes instead.
 constexpr int SEG_THREADS = 512;
 constexpr int SEG_TILE = SEG_THREADS * SORT_ITEMS;  // 8192 keys
-constexpr uint32_t SEG_MAX_N = 256u * 4096u;
+// Average group size the partitions aim below: the largest of 256 << b2
+// groups of uniform keys stays under SEG_TILE (7600 + 5 sigma < 8192).
+constexpr uint32_t SEG_GROUP_AVG = 7600;
 
-// keys: the partitioned array, each segment sorted in place by its bits
-// [0, nbits) (nbits = the top digit's shift); digit_totals: the 256 segment
-// sizes, each at most SEG_TILE.
+// b2 of a sort of n keys: the fewest bits for 256 << b2 groups of at most
+// SEG_GROUP_AVG keys on average (0: the top-digit segments are the groups;
+// 9: too many keys for one more 8-bit pass).
+inline uint32_t seg_bits2(uint64_t n) {
+  uint32_t b = 0;
+  while (b <= 8 && n > ((uint64_t)256u << b) * SEG_GROUP_AVG) ++b;
+  return b;
+}
+
+// Chunk x of a pass over the 256 segments of a partitioned array (sizes
+// totals[0..255], in order), each cut into chunks of `chunk` keys: its range
+// [*c0, *c1) and its segment (256 and an empty range past the last chunk).
+// s_a, s_b: 257 words of LDS each (segment starts, chunk prefixes).
+template <int T>
+__device__ __forceinline__ uint32_t seg_chunk(const uint32_t* __restrict__ totals, uint32_t chunk, uint32_t x,
+                                              uint64_t* c0, uint64_t* c1, uint32_t* s_a, uint32_t* s_b,
+                                              uint32_t* s_tmp) {
+  const uint32_t t = threadIdx.x;
+  const uint32_t m = t < 256 ? totals[t] : 0u;
+  const uint32_t nc = (m + chunk - 1) / chunk;
+  const uint32_t st = block_excl_scan<T>(m, s_tmp);
+  const uint32_t cp = block_excl_scan<T>(nc, s_tmp);
+  if (t < 256) {
+    s_a[t] = st;
+    s_b[t] = cp;
+    if (t == 255) {
+      s_a[256] = st + m;
+      s_b[256] = cp + nc;
+    }
+  }
+  __syncthreads();
+  if (x >= s_b[256]) {
+    *c0 = *c1 = 0;
+    return 256;
+  }
+  uint32_t lo = 0, hi = 256;  // the last segment whose first chunk is at or before x
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (s_b[mid] <= x) lo = mid;
+    else hi = mid;
+  }
+  // uniform over the workgroup: kept in scalar registers
+  lo = __builtin_amdgcn_readfirstlane(lo);
+  const uint32_t a0 = __builtin_amdgcn_readfirstlane(s_a[lo]), a1 = __builtin_amdgcn_readfirstlane(s_a[lo + 1]);
+  const uint32_t b0 = __builtin_amdgcn_readfirstlane(s_b[lo]);
+  *c0 = a0 + (uint64_t)(x - b0) * chunk;
+  *c1 = *c0 + chunk < a1 ? *c0 + chunk : (uint64_t)a1;
+  return lo;
+}
+
+// Digit histogram of chunk x (blockIdx.x) of the segmented pass, to
+// hist2[x * 256 + digit] (zeros past the last chunk).
+// cb: the chunk's bounds [cb[2x], cb[2x+1]) for k_rs_downsweep_seg.
+template <typename K>
+__global__ __launch_bounds__(256) void k_rs_upsweep_seg(const K* __restrict__ in, const uint32_t* __restrict__ totals,
+                                                        uint32_t chunk, uint32_t shift, uint32_t* __restrict__ hist2,
+                                                        uint32_t* __restrict__ cb) {
+  __shared__ uint32_t h[4 * 256];
+  __shared__ uint32_t s_a[257], s_b[257], s_tmp[4];
+  const int tid = threadIdx.x, wid = tid >> 6;
+  for (int i = tid; i < 4 * 256; i += 256) h[i] = 0;
+  uint64_t lo, hi;
+  (void)seg_chunk<256>(totals, chunk, blockIdx.x, &lo, &hi, s_a, s_b, s_tmp);
+  if (tid == 0) {
+    cb[2 * (uint64_t)blockIdx.x] = (uint32_t)lo;
+    cb[2 * (uint64_t)blockIdx.x + 1] = (uint32_t)hi;
+  }
+  for (uint64_t t0 = lo; t0 < hi; t0 += SORT_TILE) {
+    K k[SORT_ITEMS];
+#pragma unroll
+    for (int i = 0; i < SORT_ITEMS; ++i) {
+      const uint64_t idx = t0 + (uint64_t)i * 256 + tid;
+      k[i] = idx < hi ? in[idx] : (K)0;
+    }
+#pragma unroll
+    for (int i = 0; i < SORT_ITEMS; ++i) {
+      const uint64_t idx = t0 + (uint64_t)i * 256 + tid;
+      if (idx < hi) atomicAdd(&h[wid * 256 + ((uint32_t)(k[i] >> shift) & 255u)], 1u);
+    }
+  }
+  __syncthreads();
+  hist2[(uint64_t)blockIdx.x * 256 + tid] = h[tid] + h[256 + tid] + h[512 + tid] + h[768 + tid];
+}
+
+// Segment s (one workgroup; thread d = digit): the digit totals over the
+// segment's chunks, the output base of every (chunk, digit) for
+// k_rs_downsweep_seg, and the group table of k_rs_seg.  Group s * 2^b2 + j
+// is digit d = (the segment's low 8 - b2 bits) << b2 | j: the keys of a
+// segment share the high bits of the digit.  crowd (host-coherent): raised
+// when a group exceeds `limit`.
+[[maybe_unused]] static __global__ __launch_bounds__(256) void k_rs_segbase(const uint32_t* __restrict__ totals, uint32_t chunk,
+                                                    const uint32_t* __restrict__ hist2, uint32_t* __restrict__ base2,
+                                                    uint32_t b2, uint32_t* __restrict__ gstart,
+                                                    uint32_t* __restrict__ gsize, uint32_t* __restrict__ crowd,
+                                                    uint32_t limit) {
+  __shared__ uint32_t s_a[257], s_b[257], s_tmp[4];
+  const uint32_t sg = blockIdx.x, d = threadIdx.x;
+  uint64_t c0, c1;
+  (void)seg_chunk<256>(totals, chunk, 0u, &c0, &c1, s_a, s_b, s_tmp);  // the tables only
+  const uint32_t x0 = s_b[sg], x1 = s_b[sg + 1], start = s_a[sg];
+  uint32_t tot = 0;
+  for (uint32_t x = x0; x < x1; ++x) tot += hist2[(uint64_t)x * 256 + d];
+  const uint32_t dpre = block_excl_scan<256>(tot, s_tmp);
+  uint32_t run = start + dpre;
+  for (uint32_t x = x0; x < x1; ++x) {
+    const uint32_t c = hist2[(uint64_t)x * 256 + d];
+    base2[(uint64_t)x * 256 + d] = run;
+    run += c;
+  }
+  const uint32_t lowb = 8u - b2;
+  if ((d >> b2) == (sg & ((1u << lowb) - 1u))) {
+    const uint32_t g = (sg << b2) | (d & ((1u << b2) - 1u));
+    gstart[g] = start + dpre;
+    gsize[g] = tot;
+    if (tot > limit) __hip_atomic_store(crowd, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
+
+// The downsweep of the segmented pass: chunk x (blockIdx.x) of its segment,
+// with its digit bases from k_rs_segbase.
+template <typename K>
+__global__ __launch_bounds__((ds_threads<K, false>()), (sizeof(K) == 4 ? 4 : 3)) void k_rs_downsweep_seg(
+    const K* __restrict__ in, K* __restrict__ out, const uint32_t* __restrict__ cb, uint32_t shift,
+    const uint32_t* __restrict__ base2) {
+  constexpr int T = ds_threads<K, false>();
+  constexpr int W = T / 64;
+  __shared__ __align__(16) K s_keys[T * SORT_ITEMS];
+  __shared__ __align__(16) uint32_t s_vals[4];
+  __shared__ __align__(16) uint16_t s_cnt[16 * T];
+  __shared__ uint32_t s_hist[W * 256];
+  __shared__ uint32_t s_run[256];
+  __shared__ uint32_t s_base[256];
+  __shared__ uint32_t s_tmp[W];
+  const uint32_t tid = threadIdx.x;
+  // past the last chunk the range is empty (no early return: it costs the
+  // chunk loop 13 spilled registers)
+  const uint64_t c0 = cb[2 * (uint64_t)blockIdx.x], c1 = cb[2 * (uint64_t)blockIdx.x + 1];
+#pragma unroll
+  for (int w = 0; w < W * 256 / T; ++w) s_hist[w * T + tid] = 0;
+  if (tid < 256) s_run[tid] = base2[(uint64_t)blockIdx.x * 256 + tid];
+  __syncthreads();
+  ds_chunk<K, false, false, T>(in, nullptr, out, nullptr, 0u, shift, c0, c1, s_keys, s_vals, s_cnt, s_hist, s_run,
+                               s_base, s_tmp);
+}
+
+// One group per workgroup, sorted in place by its bits [0, nbits) (nbits <=
+// 32; the bits above are equal within the group).  Group = (gstart, gsize)
+// [blockIdx.x], or without a table the top-digit segment blockIdx.x of
+// digit_totals.  A group above SEG_TILE is never launched here (the host
+// sorts crowded keys with the chunked passes) and is left alone.
 template <typename K>
 __global__ __launch_bounds__(SEG_THREADS) void k_rs_seg(K* __restrict__ keys, const uint32_t* __restrict__ digit_totals,
-                                                        uint32_t nbits) {
+                                                        const uint32_t* __restrict__ gstart,
+                                                        const uint32_t* __restrict__ gsize, uint32_t nbits) {
   constexpr int T = SEG_THREADS, W = T / 64;
-  __shared__ __align__(16) K s_keys[SEG_TILE];
+  __shared__ __align__(16) uint32_t s_keys[SEG_TILE];
   __shared__ __align__(16) uint16_t s_cnt[16 * T];
   __shared__ uint32_t s_tmp[W];
   __shared__ uint32_t s_seg[2];
   const uint32_t tid = threadIdx.x;
-  {
+  uint32_t c0, m;
+  if (gsize) {
+    c0 = gstart[blockIdx.x];
+    m = gsize[blockIdx.x];
+  } else {
     const uint32_t c = tid < 256 ? digit_totals[tid] : 0u;
     const uint32_t pre = block_excl_scan<T>(c, s_tmp);
     if (tid == blockIdx.x) {
@@ -760,15 +938,22 @@ __global__ __launch_bounds__(SEG_THREADS) void k_rs_seg(K* __restrict__ keys, co
       s_seg[1] = c;
     }
     __syncthreads();
+    c0 = s_seg[0];
+    m = s_seg[1];
   }
-  const uint32_t c0 = s_seg[0], m = s_seg[1];
-  // the whole workgroup (s_seg is shared); a segment above SEG_TILE is not
-  // launched here (the host sorts crowded keys with the chunked passes)
-  if (m <= 1 || m > (uint32_t)SEG_TILE) return;
-  K k[SORT_ITEMS];
-  const uint32_t v[SORT_ITEMS] = {};  // no values
+  if (m <= 1 || m > (uint32_t)SEG_TILE) return;  // the whole workgroup
   K* seg = keys + c0;
-  ts_load_keys<K, false>(seg, 0, m, 0u, (reinterpret_cast<uintptr_t>(seg) & 15u) == 0, k);
+  const uint32_t lmask = nbits >= 32 ? 0xFFFFFFFFu : (1u << nbits) - 1u;
+  const K high = seg[0] & ~(K)lmask;
+  uint32_t k[SORT_ITEMS];
+  const uint32_t v[SORT_ITEMS] = {};  // no values
+  {
+    K kk[SORT_ITEMS];
+    ts_load_keys<K, false>(seg, 0, m, 0u, (reinterpret_cast<uintptr_t>(seg) & 15u) == 0, kk);
+    const uint32_t i0 = tid * SORT_ITEMS;
+#pragma unroll
+    for (int i = 0; i < SORT_ITEMS; ++i) k[i] = i0 + i < m ? (uint32_t)kk[i] & lmask : 0xFFFFFFFFu;
+  }
   const uint32_t np = (nbits + 3) / 4;
   for (uint32_t j = 0; j < np; ++j) {
     uint64_t rk;
@@ -776,29 +961,42 @@ __global__ __launch_bounds__(SEG_THREADS) void k_rs_seg(K* __restrict__ keys, co
     __syncthreads();
     (void)ts_scan<T>(s_cnt, 0u, s_tmp);
     __syncthreads();
-    ts_scatter<T, K, false>(k, v, 4 * j, rk, s_cnt, s_keys, nullptr);
+    ts_scatter<T, uint32_t, false>(k, v, 4 * j, rk, s_cnt, s_keys, nullptr);
     __syncthreads();
     // the next pass writes s_cnt, and s_keys only after its first barrier
     if (j + 1 < np) {
       uint32_t unused[SORT_ITEMS];
-      ts_read_blocked<K, false>(s_keys, nullptr, k, unused);
+      ts_read_blocked<uint32_t, false>(s_keys, nullptr, k, unused);
     }
   }
-  for (uint32_t p = tid; p < m; p += T) seg[p] = s_keys[ts_swz<sizeof(K)>(p)];
+  for (uint32_t p = tid; p < m; p += T) seg[p] = high | (K)s_keys[ts_swz<4>(p)];
 }
 
-inline size_t sort_workspace_bytes(uint64_t n, size_t key_bytes, bool vals) {
+// Chunks of the segmented pass over n keys (each segment adds at most one
+// partial chunk).
+inline uint32_t seg_chunks2(uint64_t n, uint32_t chunk) { return (uint32_t)((n + chunk - 1) / chunk) + 256u; }
+
+// seg: room for the segmented pass of a keys-only sort of more than
+// 256 * SEG_GROUP_AVG keys (radix_sort with a stream context).
+inline size_t sort_workspace_bytes(uint64_t n, size_t key_bytes, bool vals, bool seg = true) {
   const SortPlan p = sort_plan((uint32_t)n);
-  return 2 * aligned_bytes(n * key_bytes) + (vals ? 2 * aligned_bytes(n * 4) : 0) +
-         aligned_bytes((size_t)p.G * 256 * 4) + aligned_bytes(256 * 4) + 1024;
+  size_t b = 2 * aligned_bytes(n * key_bytes) + (vals ? 2 * aligned_bytes(n * 4) : 0) +
+             aligned_bytes((size_t)p.G * 256 * 4) + aligned_bytes(256 * 4) + 1024;
+  if (seg && !vals) {
+    const uint32_t b2 = seg_bits2(n);
+    if (b2 >= 1 && b2 <= 8)
+      b += 2 * aligned_bytes((size_t)seg_chunks2(n, p.chunk) * 256 * 4) + 2 * aligned_bytes(((size_t)256 << b2) * 4) +
+           aligned_bytes((size_t)seg_chunks2(n, p.chunk) * 2 * 4);
+  }
+  return b;
 }
 
 // LSD radix sort of n keys over bits [lo_bit, hi_bit).  When hash is set, the
 // input is raw uint32 keys, fingerprinted (fp_bits) as they are read in the
 // first pass.  vals: nullptr + with_vals -> the original index is carried.
 // Returns the buffers holding the result through *keys_out / *vals_out.
-// With the stream's context (ds), a keys-only sort of at most SEG_MAX_N keys
-// takes the partition + segment sort (it waits for the partition's row scan).
+// With the stream's context (ds), a keys-only sort partitions on its top
+// bits and sorts each group in LDS (segment sort, above).
 template <typename K>
 gqf_status radix_sort(hipStream_t s, Arena& ar, const void* in, bool hash, uint32_t fp_bits,
                       const uint32_t* in_vals, bool with_vals, uint32_t n, uint32_t lo_bit, uint32_t hi_bit,
@@ -843,8 +1041,25 @@ gqf_status radix_sort(hipStream_t s, Arena& ar, const void* in, bool hash, uint3
   const void* src = in;
   const uint32_t* vsrc = in_vals;
   int cur = 0;
-  if (ds && !with_vals && lo_bit == 0 && hi_bit >= 16 && npass >= 2 && n <= SEG_MAX_N) {
-    // the top 8 bits in one chunked pass, then every segment in LDS
+  const uint32_t b2 = seg_bits2(n);
+  const bool seg1 = ds && !with_vals && lo_bit == 0 && b2 == 0 && hi_bit >= 16 && hi_bit - 8 <= 32;
+  const bool seg2 = ds && !with_vals && lo_bit == 0 && b2 >= 1 && b2 <= 8 && hi_bit >= 12 + b2 &&
+                    hi_bit - 8 - b2 <= 32;
+  uint32_t *hist2 = nullptr, *base2 = nullptr, *gstart = nullptr, *gsize = nullptr, *cb = nullptr;
+  const uint32_t G2 = seg2 ? seg_chunks2(n, plan.chunk) : 0u;
+  if (seg2) {
+    cb = ar.take<uint32_t>((size_t)G2 * 2);
+    hist2 = ar.take<uint32_t>((size_t)G2 * 256);
+    base2 = ar.take<uint32_t>((size_t)G2 * 256);
+    gstart = ar.take<uint32_t>((size_t)256 << b2);
+    gsize = ar.take<uint32_t>((size_t)256 << b2);
+    if (ar.used > ar.cap) {
+      set_error("internal: sort workspace overflow");
+      return GQF_ERR_INVALID;
+    }
+  }
+  if (seg1 || seg2) {
+    // the top 8 bits in one chunked pass
     const uint32_t shift = hi_bit - 8;
     *reinterpret_cast<volatile uint32_t*>(ds->h_crowd) = 0u;
     if (hash) {
@@ -854,8 +1069,9 @@ gqf_status radix_sort(hipStream_t s, Arena& ar, const void* in, bool hash, uint3
       GQF_LAUNCH_B((double)n * sizeof(K), (k_rs_upsweep<K, false>), dim3(plan.G), dim3(256), 0, s, in, n, fp_bits,
                    shift, plan.chunk, plan.G, hist);
     }
-    GQF_LAUNCH(k_rs_rowscan, dim3(256), dim3(1024), 0, s, hist, plan.G, dbase, ds->d_crowd, (uint32_t)SEG_TILE);
-    GQF_HIP(hipEventRecord(ds->ev_crowd, s));
+    GQF_LAUNCH(k_rs_rowscan, dim3(256), dim3(1024), 0, s, hist, plan.G, dbase, seg1 ? ds->d_crowd : (uint32_t*)nullptr,
+               (uint32_t)SEG_TILE);
+    if (seg1) GQF_HIP(hipEventRecord(ds->ev_crowd, s));
     const double pass_bytes = (double)n * ((hash ? 4.0 : (double)sizeof(K)) + sizeof(K));
     if (hash)
       GQF_LAUNCH_B(pass_bytes, (k_rs_downsweep<K, false, true>), dim3(plan.G), dim3(ds_threads<K, false>()), 0, s, in,
@@ -863,17 +1079,37 @@ gqf_status radix_sort(hipStream_t s, Arena& ar, const void* in, bool hash, uint3
     else
       GQF_LAUNCH_B(pass_bytes, (k_rs_downsweep<K, false, false>), dim3(plan.G), dim3(ds_threads<K, false>()), 0, s, in,
                    nullptr, kb[0], nullptr, n, fp_bits, shift, plan.chunk, plan.G, hist, dbase);
-    // the host reads the flag while the partition pass runs
+    K* out = kb[0];
+    uint32_t nbits = shift;
+    if (seg2) {
+      // the next b2 bits within every segment
+      const uint32_t shift2 = hi_bit - 8 - b2;
+      GQF_LAUNCH_B((double)n * sizeof(K), (k_rs_upsweep_seg<K>), dim3(G2), dim3(256), 0, s, kb[0], dbase, plan.chunk,
+                   shift2, hist2, cb);
+      GQF_LAUNCH(k_rs_segbase, dim3(256), dim3(256), 0, s, dbase, plan.chunk, hist2, base2, b2, gstart, gsize,
+                 ds->d_crowd, (uint32_t)SEG_TILE);
+      GQF_HIP(hipEventRecord(ds->ev_crowd, s));
+      GQF_LAUNCH_B((double)n * 2.0 * sizeof(K), (k_rs_downsweep_seg<K>), dim3(G2), dim3(ds_threads<K, false>()), 0, s,
+                   kb[0], kb[1], cb, shift2, base2);
+      out = kb[1];
+      nbits = shift2;
+    }
+    // the host reads the flag while the partition's downsweep runs
     GQF_HIP(hipEventSynchronize(ds->ev_crowd));
     if (*reinterpret_cast<volatile uint32_t*>(ds->h_crowd) == 0u) {
-      GQF_LAUNCH_B((double)n * 2.0 * sizeof(K), (k_rs_seg<K>), dim3(256), dim3(SEG_THREADS), 0, s, kb[0], dbase, shift);
-      *keys_out = kb[0];
+      if (seg2)
+        GQF_LAUNCH_B((double)n * 2.0 * sizeof(K), (k_rs_seg<K>), dim3(256u << b2), dim3(SEG_THREADS), 0, s, out,
+                     (const uint32_t*)nullptr, gstart, gsize, nbits);
+      else
+        GQF_LAUNCH_B((double)n * 2.0 * sizeof(K), (k_rs_seg<K>), dim3(256), dim3(SEG_THREADS), 0, s, out, dbase,
+                     (const uint32_t*)nullptr, (const uint32_t*)nullptr, nbits);
+      *keys_out = out;
       if (vals_out) *vals_out = nullptr;
       return GQF_OK;
     }
-    // a crowded segment: every chunked pass over the partitioned fingerprints
-    src = kb[0];
-    cur = 1;
+    // a crowded group: every chunked pass over the partitioned fingerprints
+    src = out;
+    cur = out == kb[0] ? 1 : 0;
     hash = false;
   }
   for (uint32_t p = 0; p < npass; ++p) {

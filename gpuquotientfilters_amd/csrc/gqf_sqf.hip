@@ -197,7 +197,10 @@ static gqf_status check_keys(int n, const void* keys) {
 // rebuild, over a workspace arena and the device counters.  A layout that
 // does not fit raises the capacity flag in counters[CTR_ERR] before any
 // byte of the table is written.
-static size_t sqf_build_ws_bytes(uint32_t n) { return sort_workspace_bytes(n, 4, false) + rebuild_ws_bytes(n); }
+// seg: the sort may take its segmented pass (a call with a stream context).
+static size_t sqf_build_ws_bytes(uint32_t n, bool seg) {
+  return sort_workspace_bytes(n, 4, false, seg) + rebuild_ws_bytes(n);
+}
 // ds: the stream's context (null: the sort takes its chunked passes only).
 static gqf_status sqf_build_launch(hipStream_t s, Arena& ar, uint32_t* counters, const gqf_sqf& qf, const SqfGeom& g,
                                   uint32_t un, const uint32_t* d_keys, int no_dup, DeviceState* ds) {
@@ -217,7 +220,7 @@ static gqf_status sqf_build_impl(const gqf_sqf& qf, int n, const uint32_t* d_key
   GQF_TRY(cs.begin(stream));
   DeviceState& ds = *cs.ds;
   const uint32_t un = (uint32_t)n;
-  GQF_TRY(ws_reserve(ds, sqf_build_ws_bytes(un) + packed_ws_bytes(g)));
+  GQF_TRY(ws_reserve(ds, sqf_build_ws_bytes(un, true) + packed_ws_bytes(g)));
   Arena ar(ds.ws, ds.ws_cap);
   gqf_sqf wq;
   GQF_TRY(work_table(cs.stream, ar, qf, g, false, &wq));
@@ -453,7 +456,7 @@ gqf_status gqf_sqf_bulk_build_noalloc(gqf_sqf qf, int method, int n, const uint3
   }
   const uint32_t un = (uint32_t)n;
   const SqfGeom g = geom(qf);
-  if (NOALLOC_COUNTER_BYTES + sqf_build_ws_bytes(un) + packed_ws_bytes(g) > NOALLOC_SCRATCH_BYTES) {
+  if (NOALLOC_COUNTER_BYTES + sqf_build_ws_bytes(un, false) + packed_ws_bytes(g) > NOALLOC_SCRATCH_BYTES) {
     set_error(g.packed ? "the no-allocation scratch (%zu bytes) cannot hold %u keys plus the wide copy of this "
                          "packed-width table; use gqf_sqf_bulk_build"
                        : "internal: the no-allocation scratch (%zu bytes) is too small for %u keys",

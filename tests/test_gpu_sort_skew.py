@@ -199,3 +199,61 @@ def bench_zipf(n, s=1.1, universe=1 << 28, seed=7):
         z = rng.zipf(s, 2 * (n - len(out)) + 16).astype(np.uint64)
         out = np.concatenate([out, z[z <= universe]])
     return (out[:n] - 1).astype(np.uint32)
+
+
+# The partition on the top 8 bits alone serves up to 256 * 7600 keys; larger
+# sorts partition again within every segment (2^b2 groups each) before the
+# LDS sort of each group.
+SEG1_MAX = 256 * 7600
+
+
+@pytest.mark.parametrize("n", [SEG1_MAX, SEG1_MAX + 1, 2600000])
+def test_sqf_build_at_the_second_partition(n):
+    """Sizes at the bound of the second partition, 28-bit fingerprints."""
+    q, r = 23, 5
+    say("sqf build second partition", n)
+    keys = O.mt19937(71, n)
+    f = G.SQF(q, r)
+    f.bulk_build(dev(keys), NoDuplicates=False)
+    assert (f.table_bytes() == sqf_canon(keys, q, r, False)).all()
+
+
+@pytest.mark.parametrize("q,r,n", [(24, 10, 2600000), (23, 11, 4200000)])
+def test_rsqf_build_at_the_second_partition(q, r, n):
+    """34-bit fingerprints: the groups sort their low 32 bits."""
+    say("rsqf build second partition", q, r, n)
+    keys = O.mt19937(72, n)
+    f = G.RSQF(q, r)
+    f.bulk_build(dev(keys))
+    assert (f.dump() == _rsqf_canon(keys, q, r)).all()
+
+
+@pytest.mark.parametrize("kind", ["hot", "crowded", "zipf"])
+def test_sqf_build_skewed_past_the_second_partition(kind):
+    """Skewed sorts of more than 256 * 7600 keys: one key 20001 times, 12000
+    distinct keys under one top-8-bit value, Zipf ranks as keys."""
+    q, r = 24, 5
+    n = 2300000
+    say("sqf skewed second partition", kind)
+    if kind == "hot":
+        keys = _with_hot(73, n, 20000)
+    elif kind == "crowded":
+        keys = _crowded(74, q, r, 12000, n - 12000)
+    else:
+        keys = bench_zipf(n)
+    f = G.SQF(q, r)
+    f.bulk_build(dev(keys), NoDuplicates=True)
+    assert (f.table_bytes() == sqf_canon(keys, q, r, True)).all()
+
+
+def test_rsqf_insert_and_delete_batches_past_the_second_partition():
+    q, r = 24, 10
+    say("rsqf insert/delete second partition")
+    base = O.mt19937(75, 4000000)
+    f = G.RSQF(q, r)
+    f.bulk_build(dev(base[:1500000]))
+    f.insertGPU(dev(base[1500000:]))
+    assert (f.dump() == _rsqf_canon(base, q, r)).all()
+    gone = base[:2200000]
+    f.deleteGPU(dev(gone))
+    assert (f.dump() == _rsqf_canon(base[2200000:], q, r)).all()
