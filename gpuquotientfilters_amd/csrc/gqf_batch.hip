@@ -559,7 +559,8 @@ static void batch_build_launch(SqfBatch& b) {
     else
       GQF_LAUNCH_B((double)b.N * 4, k_bs_upsweep<false>, dim3(b.G), dim3(256), 0, s, src, b.d_ch, b.d_flt, shift,
                    b.G, b.d_hist);
-    GQF_LAUNCH(k_rs_rowscan, dim3(256), dim3(1024), 0, s, b.d_hist, b.G, b.d_tot, (uint32_t*)nullptr, 0u);
+    GQF_LAUNCH(k_rs_rowscan, dim3(256), dim3(1024), 0, s, b.d_hist, b.G, b.d_tot, (uint32_t*)nullptr, 0u,
+               (uint32_t*)nullptr);
     GQF_LAUNCH(k_bs_digitbase, dim3(F), dim3(256), 0, s, b.d_hist, b.d_tot, b.d_fchunk, b.G, b.d_dbase);
     if (p == 0)
       GQF_LAUNCH_B((double)b.N * 8, k_bs_downsweep<true>, dim3(b.G), dim3(256), 0, s, src, dst, b.d_ch, b.d_flt,

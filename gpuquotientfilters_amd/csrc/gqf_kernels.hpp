@@ -305,13 +305,16 @@ __global__ __launch_bounds__(256) void k_rs_upsweep(const void* __restrict__ in,
 
 // Row d (one workgroup per digit): exclusive scan of the G chunk counts in
 // place; the row total goes to totals[d].  crowd (may be null): a word of
-// host-coherent memory set to 1 when a row total exceeds `limit`.
+// host-coherent memory set to 1 when a row total exceeds `limit`.  clear
+// (may be null): a word set to zero.
 [[maybe_unused]] static __global__ __launch_bounds__(1024) void k_rs_rowscan(uint32_t* __restrict__ hist, uint32_t G,
                                                             uint32_t* __restrict__ totals,
-                                                            uint32_t* __restrict__ crowd, uint32_t limit) {
+                                                            uint32_t* __restrict__ crowd, uint32_t limit,
+                                                            uint32_t* __restrict__ clear) {
   __shared__ uint32_t s_w[16];
   __shared__ uint32_t s_carry;
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  if (clear && blockIdx.x == 0 && tid == 0) *clear = 0u;
   uint32_t* row = hist + (uint64_t)blockIdx.x * G;
   if (tid == 0) s_carry = 0;
   __syncthreads();
@@ -364,20 +367,29 @@ __device__ __forceinline__ uint32_t ts_swz(uint32_t p) {
   return p ^ (((p >> 5) & 7u) << 1);
 }
 
+// 16 bytes from a 4-byte aligned address: one global_load_dwordx4 (gfx950
+// multi-dword loads need dword alignment only), so chunks that start at any
+// key still load whole groups.
+__device__ __forceinline__ uint4 ld16_dw(const uint32_t* p) {
+  uint4 x;
+  __builtin_memcpy(&x, p, 16);
+  return x;
+}
+
 // Thread t's 16 consecutive entries [16t, 16t+16) of the sub-tile at t0
-// (tile_n of them valid; the rest become all-ones padding).  vec: the array
-// is 16-byte aligned, so whole groups load as 16-byte words.
+// (tile_n of them valid; the rest become all-ones padding); whole groups
+// load as 16-byte words.
 template <typename K, bool HASH>
 __device__ __forceinline__ void ts_load_keys(const void* __restrict__ in, uint64_t t0, uint32_t tile_n,
-                                             uint32_t fp_bits, bool vec, K (&k)[SORT_ITEMS]) {
+                                             uint32_t fp_bits, K (&k)[SORT_ITEMS]) {
   const uint32_t i0 = threadIdx.x * SORT_ITEMS;
   if (HASH || sizeof(K) == 4) {
     const uint32_t* p = reinterpret_cast<const uint32_t*>(in) + t0 + i0;
     uint32_t w[SORT_ITEMS];
-    if (vec && i0 + SORT_ITEMS <= tile_n) {
+    if (i0 + SORT_ITEMS <= tile_n) {
 #pragma unroll
       for (int c = 0; c < SORT_ITEMS / 4; ++c) {
-        const uint4 x = reinterpret_cast<const uint4*>(p)[c];
+        const uint4 x = ld16_dw(p + 4 * c);
         w[4 * c] = x.x;
         w[4 * c + 1] = x.y;
         w[4 * c + 2] = x.z;
@@ -396,10 +408,10 @@ __device__ __forceinline__ void ts_load_keys(const void* __restrict__ in, uint64
     }
   } else {
     const K* p = reinterpret_cast<const K*>(in) + t0 + i0;
-    if (vec && i0 + SORT_ITEMS <= tile_n) {
+    if (i0 + SORT_ITEMS <= tile_n) {
 #pragma unroll
       for (int c = 0; c < SORT_ITEMS / 2; ++c) {
-        const uint4 x = reinterpret_cast<const uint4*>(p)[c];
+        const uint4 x = ld16_dw(reinterpret_cast<const uint32_t*>(p + 2 * c));
         k[2 * c] = (K)(((uint64_t)x.y << 32) | x.x);
         k[2 * c + 1] = (K)(((uint64_t)x.w << 32) | x.z);
       }
@@ -412,7 +424,7 @@ __device__ __forceinline__ void ts_load_keys(const void* __restrict__ in, uint64
 
 // Values of the same positions; without an input array, the input index.
 __device__ __forceinline__ void ts_load_vals(const uint32_t* __restrict__ in_vals, uint64_t t0, uint32_t tile_n,
-                                             bool vec, uint32_t (&v)[SORT_ITEMS]) {
+                                             uint32_t (&v)[SORT_ITEMS]) {
   const uint32_t i0 = threadIdx.x * SORT_ITEMS;
   if (!in_vals) {
 #pragma unroll
@@ -420,10 +432,10 @@ __device__ __forceinline__ void ts_load_vals(const uint32_t* __restrict__ in_val
     return;
   }
   const uint32_t* p = in_vals + t0 + i0;
-  if (vec && i0 + SORT_ITEMS <= tile_n) {
+  if (i0 + SORT_ITEMS <= tile_n) {
 #pragma unroll
     for (int c = 0; c < SORT_ITEMS / 4; ++c) {
-      const uint4 x = reinterpret_cast<const uint4*>(p)[c];
+      const uint4 x = ld16_dw(p + 4 * c);
       v[4 * c] = x.x;
       v[4 * c + 1] = x.y;
       v[4 * c + 2] = x.z;
@@ -565,16 +577,13 @@ __device__ __forceinline__ void ds_chunk(const void* __restrict__ in_keys, const
   constexpr int W = T / 64;
   const uint32_t tid = threadIdx.x, wid = tid >> 6;
   const bool dig = tid < 256;
-  const size_t kin = (HASH || sizeof(K) == 4) ? 4 : sizeof(K);
-  const bool kvec = ((reinterpret_cast<uintptr_t>(in_keys) + c0 * kin) & 15u) == 0;
-  const bool vvec = ((reinterpret_cast<uintptr_t>(in_vals) + c0 * 4) & 15u) == 0;
   K k[SORT_ITEMS];
   uint32_t v[SORT_ITEMS];
   uint64_t rk;
   if (c0 < c1) {
     const uint32_t tn = (uint32_t)(c1 - c0 < (uint64_t)TILE ? c1 - c0 : (uint64_t)TILE);
-    ts_load_keys<K, HASH>(in_keys, c0, tn, fp_bits, kvec, k);
-    if (VALS) ts_load_vals(in_vals, c0, tn, vvec, v);
+    ts_load_keys<K, HASH>(in_keys, c0, tn, fp_bits, k);
+    if (VALS) ts_load_vals(in_vals, c0, tn, v);
   }
   for (uint64_t t0 = c0; t0 < c1; t0 += TILE) {
     const uint32_t tile_n = (uint32_t)(c1 - t0 < (uint64_t)TILE ? c1 - t0 : (uint64_t)TILE);
@@ -608,8 +617,8 @@ __device__ __forceinline__ void ds_chunk(const void* __restrict__ in_keys, const
     const uint64_t tn0 = t0 + TILE;
     if (tn0 < c1) {
       const uint32_t tn = (uint32_t)(c1 - tn0 < (uint64_t)TILE ? c1 - tn0 : (uint64_t)TILE);
-      ts_load_keys<K, HASH>(in_keys, tn0, tn, fp_bits, kvec, k);
-      if (VALS) ts_load_vals(in_vals, tn0, tn, vvec, v);
+      ts_load_keys<K, HASH>(in_keys, tn0, tn, fp_bits, k);
+      if (VALS) ts_load_vals(in_vals, tn0, tn, v);
     }
     // runs of equal digits to their outputs; consecutive lanes, consecutive positions
 #pragma unroll 4
@@ -762,6 +771,10 @@ __global__ __launch_bounds__(SORT_THREADS) void k_rs_small(const void* __restric
 // the chunked passes instead.
 constexpr int SEG_THREADS = 512;
 constexpr int SEG_TILE = SEG_THREADS * SORT_ITEMS;  // 8192 keys
+constexpr uint32_t SEG_BB = 11;                     // bucket bits of the LDS group sort
+constexpr uint32_t SEG_NBK = 1u << SEG_BB;          // its buckets (16-bit counters)
+constexpr uint32_t SEG_RANK_MAX = 32;               // largest bucket ranked key by key
+constexpr uint32_t SEG_SLOW_GRID = 256;             // workgroups walking the list of groups k_rs_seg left
 // Average group size the partitions aim below: the largest of 256 << b2
 // groups of uniform keys stays under SEG_TILE (7600 + 5 sigma < 8192).
 constexpr uint32_t SEG_GROUP_AVG = 7600;
@@ -911,45 +924,42 @@ __global__ __launch_bounds__((ds_threads<K, false>()), (sizeof(K) == 4 ? 4 : 3))
                                s_base, s_tmp);
 }
 
-// One group per workgroup, sorted in place by its bits [0, nbits) (nbits <=
-// 32; the bits above are equal within the group).  Group = (gstart, gsize)
-// [blockIdx.x], or without a table the top-digit segment blockIdx.x of
-// digit_totals.  A group above SEG_TILE is never launched here (the host
-// sorts crowded keys with the chunked passes) and is left alone.
-template <typename K>
-__global__ __launch_bounds__(SEG_THREADS) void k_rs_seg(K* __restrict__ keys, const uint32_t* __restrict__ digit_totals,
-                                                        const uint32_t* __restrict__ gstart,
-                                                        const uint32_t* __restrict__ gsize, uint32_t nbits) {
-  constexpr int T = SEG_THREADS, W = T / 64;
-  __shared__ __align__(16) uint32_t s_keys[SEG_TILE];
-  __shared__ __align__(16) uint16_t s_cnt[16 * T];
-  __shared__ uint32_t s_tmp[W];
-  __shared__ uint32_t s_seg[2];
-  const uint32_t tid = threadIdx.x;
-  uint32_t c0, m;
+// Group g of a segment sort: [*c0, *c0 + *m) from the table (gstart, gsize),
+// or without a table the top-digit segment g of digit_totals (block scan).
+template <int T>
+__device__ __forceinline__ void seg_group(uint32_t g, const uint32_t* __restrict__ digit_totals,
+                                          const uint32_t* __restrict__ gstart, const uint32_t* __restrict__ gsize,
+                                          uint32_t* c0, uint32_t* m, uint32_t* s_tmp, uint32_t* s_seg) {
   if (gsize) {
-    c0 = gstart[blockIdx.x];
-    m = gsize[blockIdx.x];
-  } else {
-    const uint32_t c = tid < 256 ? digit_totals[tid] : 0u;
-    const uint32_t pre = block_excl_scan<T>(c, s_tmp);
-    if (tid == blockIdx.x) {
-      s_seg[0] = pre;
-      s_seg[1] = c;
-    }
-    __syncthreads();
-    c0 = s_seg[0];
-    m = s_seg[1];
+    *c0 = gstart[g];
+    *m = gsize[g];
+    return;
   }
-  if (m <= 1 || m > (uint32_t)SEG_TILE) return;  // the whole workgroup
-  K* seg = keys + c0;
+  const uint32_t c = threadIdx.x < 256 ? digit_totals[threadIdx.x] : 0u;
+  const uint32_t pre = block_excl_scan<T>(c, s_tmp);
+  if (threadIdx.x == g) {
+    s_seg[0] = pre;
+    s_seg[1] = c;
+  }
+  __syncthreads();
+  *c0 = s_seg[0];
+  *m = s_seg[1];
+  __syncthreads();  // s_seg is read before it is written again
+}
+
+// A group of m <= SEG_TILE keys sorted in place by its bits [0, nbits) with
+// the 4-bit counting passes of the tile sort, back to back in LDS.
+template <typename K, int T>
+__device__ __forceinline__ void seg_counting_sort(K* __restrict__ seg, uint32_t m, uint32_t nbits, uint32_t* s_keys,
+                                                  uint16_t* s_cnt, uint32_t* s_tmp) {
+  const uint32_t tid = threadIdx.x;
   const uint32_t lmask = nbits >= 32 ? 0xFFFFFFFFu : (1u << nbits) - 1u;
+  const uint32_t v[SORT_ITEMS] = {};  // no values
   const K high = seg[0] & ~(K)lmask;
   uint32_t k[SORT_ITEMS];
-  const uint32_t v[SORT_ITEMS] = {};  // no values
   {
     K kk[SORT_ITEMS];
-    ts_load_keys<K, false>(seg, 0, m, 0u, (reinterpret_cast<uintptr_t>(seg) & 15u) == 0, kk);
+    ts_load_keys<K, false>(seg, 0, m, 0u, kk);
     const uint32_t i0 = tid * SORT_ITEMS;
 #pragma unroll
     for (int i = 0; i < SORT_ITEMS; ++i) k[i] = i0 + i < m ? (uint32_t)kk[i] & lmask : 0xFFFFFFFFu;
@@ -972,6 +982,131 @@ __global__ __launch_bounds__(SEG_THREADS) void k_rs_seg(K* __restrict__ keys, co
   for (uint32_t p = tid; p < m; p += T) seg[p] = high | (K)s_keys[ts_swz<4>(p)];
 }
 
+// One group per workgroup, sorted in place by its bits [0, nbits) (nbits <=
+// 32; the bits above are equal within the group): the keys' top SEG_BB bits
+// of the range pick one of SEG_NBK buckets (LDS counters; the scanned starts
+// then serve as cursors that place every key in its bucket), and every key's
+// place is its bucket's start plus the bucket's smaller keys.  With hashed
+// keys a bucket holds ~3 keys.  A group with a bucket of more than
+// SEG_RANK_MAX keys (many copies of a few keys) takes the 4-bit counting
+// passes instead: in the same workgroup (INLINE: grids of one workgroup per
+// CU, where occupancy does not matter), or appended to `slow` (count, then
+// group ids) for k_rs_seg_slow, which keeps this kernel at 40 registers and
+// four workgroups per CU.  A group above SEG_TILE is never launched here
+// (the host sorts crowded keys with the chunked passes) and is left alone.
+template <typename K, bool INLINE>
+__global__ __launch_bounds__(SEG_THREADS) void k_rs_seg(K* __restrict__ keys, const uint32_t* __restrict__ digit_totals,
+                                                        const uint32_t* __restrict__ gstart,
+                                                        const uint32_t* __restrict__ gsize, uint32_t nbits,
+                                                        uint32_t* __restrict__ slow) {
+  constexpr int T = SEG_THREADS, W = T / 64;
+  __shared__ __align__(16) uint32_t s_keys[SEG_TILE];
+  __shared__ __align__(16) uint16_t s_cnt[INLINE ? 16 * T : 8];  // the counting passes' count table
+  __shared__ __align__(16) uint32_t s_c[SEG_NBK / 2];  // bucket counters / cursors, 16 bits each
+  __shared__ uint32_t s_tmp[W];
+  __shared__ uint32_t s_seg[2];
+  const uint32_t tid = threadIdx.x;
+  uint32_t c0, m;
+  seg_group<T>(blockIdx.x, digit_totals, gstart, gsize, &c0, &m, s_tmp, s_seg);
+  if (m <= 1 || m > (uint32_t)SEG_TILE) return;  // the whole workgroup
+  K* seg = keys + c0;
+  const uint32_t lmask = nbits >= 32 ? 0xFFFFFFFFu : (1u << nbits) - 1u;
+  const uint32_t i0 = tid * SORT_ITEMS;
+  const uint32_t nreal = m > i0 ? (m - i0 < SORT_ITEMS ? m - i0 : SORT_ITEMS) : 0u;
+  const uint32_t bsh = nbits > SEG_BB ? nbits - SEG_BB : 0u;
+  uint32_t k[SORT_ITEMS];
+  {
+    K kk[SORT_ITEMS];
+    ts_load_keys<K, false>(seg, 0, m, 0u, kk);
+#pragma unroll
+    for (int i = 0; i < SORT_ITEMS; ++i) k[i] = (uint32_t)kk[i] & lmask;
+  }
+  const K high = seg[0] & ~(K)lmask;
+  for (uint32_t i = tid; i < SEG_NBK / 2; i += T) s_c[i] = 0u;
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < SORT_ITEMS; ++i) {
+    if ((uint32_t)i < nreal) {
+      const uint32_t b = k[i] >> bsh;
+      atomicAdd(&s_c[b >> 1], 1u << (16 * (b & 1u)));
+    }
+  }
+  __syncthreads();
+  {
+    constexpr uint32_t PT = SEG_NBK / 2 / T;  // counter words per thread
+    static_assert(PT == 2, "4 bucket counters per thread");
+    uint2* c2 = reinterpret_cast<uint2*>(s_c) + tid;
+    const uint2 a = *c2;
+    const uint32_t w[PT] = {a.x, a.y};
+    uint32_t sum = 0, mx = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < PT; ++i) {
+      const uint32_t lo = w[i] & 0xFFFFu, hi = w[i] >> 16;
+      sum += lo + hi;
+      mx = lo > mx ? lo : mx;
+      mx = hi > mx ? hi : mx;
+    }
+    uint32_t run = block_excl_scan<T>(sum, s_tmp);
+    uint32_t st[PT];
+#pragma unroll
+    for (uint32_t i = 0; i < PT; ++i) {
+      const uint32_t lo = run;
+      run += w[i] & 0xFFFFu;
+      st[i] = lo | (run << 16);
+      run += w[i] >> 16;
+    }
+    *c2 = make_uint2(st[0], st[1]);
+    if (__syncthreads_or(mx > SEG_RANK_MAX)) {  // also orders the bucket starts
+      if (INLINE) seg_counting_sort<K, T>(seg, m, nbits, s_keys, s_cnt, s_tmp);
+      else if (tid == 0) slow[1 + atomicAdd(&slow[0], 1u)] = blockIdx.x;
+      return;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < SORT_ITEMS; ++i) {
+    if ((uint32_t)i < nreal) {
+      const uint32_t b = k[i] >> bsh, hs = 16 * (b & 1u);
+      s_keys[(atomicAdd(&s_c[b >> 1], 1u << hs) >> hs) & 0xFFFFu] = k[i];
+    }
+  }
+  __syncthreads();
+  // counter b now holds the end of bucket b
+  const uint16_t* s_c16 = reinterpret_cast<const uint16_t*>(s_c);
+  for (uint32_t p = tid; p < m; p += T) {
+    const uint32_t x = s_keys[p], b = x >> bsh;
+    const uint32_t st = b ? s_c16[b - 1] : 0u, en = s_c16[b];
+    uint32_t rank = st;
+    for (uint32_t j = st; j < en; ++j) {
+      const uint32_t y = s_keys[j];
+      rank += (y < x || (y == x && j < p)) ? 1u : 0u;
+    }
+    seg[rank] = high | (K)x;  // within the group's lines: merged in L2
+  }
+}
+
+// The groups k_rs_seg left (slow[0] of them, ids after it): the 4-bit
+// counting passes of the tile sort, back to back in LDS.  A fixed grid walks
+// the list (empty: every workgroup returns at once).
+template <typename K>
+__global__ __launch_bounds__(SEG_THREADS) void k_rs_seg_slow(K* __restrict__ keys,
+                                                             const uint32_t* __restrict__ digit_totals,
+                                                             const uint32_t* __restrict__ gstart,
+                                                             const uint32_t* __restrict__ gsize, uint32_t nbits,
+                                                             const uint32_t* __restrict__ slow) {
+  constexpr int T = SEG_THREADS, W = T / 64;
+  __shared__ __align__(16) uint32_t s_keys[SEG_TILE];
+  __shared__ __align__(16) uint16_t s_cnt[16 * T];
+  __shared__ uint32_t s_tmp[W];
+  __shared__ uint32_t s_seg[2];
+  const uint32_t ns = slow[0];
+  for (uint32_t q = blockIdx.x; q < ns; q += gridDim.x) {
+    uint32_t c0, m;
+    seg_group<T>(slow[1 + q], digit_totals, gstart, gsize, &c0, &m, s_tmp, s_seg);
+    seg_counting_sort<K, T>(keys + c0, m, nbits, s_keys, s_cnt, s_tmp);
+    __syncthreads();  // s_keys is read before the next group writes it
+  }
+}
+
 // Chunks of the segmented pass over n keys (each segment adds at most one
 // partial chunk).
 inline uint32_t seg_chunks2(uint64_t n, uint32_t chunk) { return (uint32_t)((n + chunk - 1) / chunk) + 256u; }
@@ -986,7 +1121,7 @@ inline size_t sort_workspace_bytes(uint64_t n, size_t key_bytes, bool vals, bool
     const uint32_t b2 = seg_bits2(n);
     if (b2 >= 1 && b2 <= 8)
       b += 2 * aligned_bytes((size_t)seg_chunks2(n, p.chunk) * 256 * 4) + 2 * aligned_bytes(((size_t)256 << b2) * 4) +
-           aligned_bytes((size_t)seg_chunks2(n, p.chunk) * 2 * 4);
+           aligned_bytes((size_t)seg_chunks2(n, p.chunk) * 2 * 4) + aligned_bytes((1 + ((size_t)256 << b2)) * 4);
   }
   return b;
 }
@@ -1045,9 +1180,11 @@ gqf_status radix_sort(hipStream_t s, Arena& ar, const void* in, bool hash, uint3
   const bool seg1 = ds && !with_vals && lo_bit == 0 && b2 == 0 && hi_bit >= 16 && hi_bit - 8 <= 32;
   const bool seg2 = ds && !with_vals && lo_bit == 0 && b2 >= 1 && b2 <= 8 && hi_bit >= 12 + b2 &&
                     hi_bit - 8 - b2 <= 32;
-  uint32_t *hist2 = nullptr, *base2 = nullptr, *gstart = nullptr, *gsize = nullptr, *cb = nullptr;
+  uint32_t *hist2 = nullptr, *base2 = nullptr, *gstart = nullptr, *gsize = nullptr, *cb = nullptr, *slow = nullptr;
   const uint32_t G2 = seg2 ? seg_chunks2(n, plan.chunk) : 0u;
+  const uint32_t ngroups = 256u << b2;
   if (seg2) {
+    slow = ar.take<uint32_t>(1 + ngroups);
     cb = ar.take<uint32_t>((size_t)G2 * 2);
     hist2 = ar.take<uint32_t>((size_t)G2 * 256);
     base2 = ar.take<uint32_t>((size_t)G2 * 256);
@@ -1070,7 +1207,7 @@ gqf_status radix_sort(hipStream_t s, Arena& ar, const void* in, bool hash, uint3
                    shift, plan.chunk, plan.G, hist);
     }
     GQF_LAUNCH(k_rs_rowscan, dim3(256), dim3(1024), 0, s, hist, plan.G, dbase, seg1 ? ds->d_crowd : (uint32_t*)nullptr,
-               (uint32_t)SEG_TILE);
+               (uint32_t)SEG_TILE, slow);
     if (seg1) GQF_HIP(hipEventRecord(ds->ev_crowd, s));
     const double pass_bytes = (double)n * ((hash ? 4.0 : (double)sizeof(K)) + sizeof(K));
     if (hash)
@@ -1097,12 +1234,15 @@ gqf_status radix_sort(hipStream_t s, Arena& ar, const void* in, bool hash, uint3
     // the host reads the flag while the partition's downsweep runs
     GQF_HIP(hipEventSynchronize(ds->ev_crowd));
     if (*reinterpret_cast<volatile uint32_t*>(ds->h_crowd) == 0u) {
-      if (seg2)
-        GQF_LAUNCH_B((double)n * 2.0 * sizeof(K), (k_rs_seg<K>), dim3(256u << b2), dim3(SEG_THREADS), 0, s, out,
-                     (const uint32_t*)nullptr, gstart, gsize, nbits);
-      else
-        GQF_LAUNCH_B((double)n * 2.0 * sizeof(K), (k_rs_seg<K>), dim3(256), dim3(SEG_THREADS), 0, s, out, dbase,
-                     (const uint32_t*)nullptr, (const uint32_t*)nullptr, nbits);
+      if (seg2) {
+        GQF_LAUNCH_B((double)n * 2.0 * sizeof(K), (k_rs_seg<K, false>), dim3(ngroups), dim3(SEG_THREADS), 0, s, out,
+                     (const uint32_t*)nullptr, gstart, gsize, nbits, slow);
+        GQF_LAUNCH(k_rs_seg_slow<K>, dim3(SEG_SLOW_GRID), dim3(SEG_THREADS), 0, s, out, (const uint32_t*)nullptr,
+                   gstart, gsize, nbits, (const uint32_t*)slow);
+      } else {
+        GQF_LAUNCH_B((double)n * 2.0 * sizeof(K), (k_rs_seg<K, true>), dim3(256), dim3(SEG_THREADS), 0, s, out, dbase,
+                     (const uint32_t*)nullptr, (const uint32_t*)nullptr, nbits, (uint32_t*)nullptr);
+      }
       *keys_out = out;
       if (vals_out) *vals_out = nullptr;
       return GQF_OK;
@@ -1127,7 +1267,8 @@ gqf_status radix_sort(hipStream_t s, Arena& ar, const void* in, bool hash, uint3
                    shift, plan.chunk, plan.G, hist);
     }
     const double kin = first_hash ? 4.0 : (double)sizeof(K);
-    GQF_LAUNCH(k_rs_rowscan, dim3(256), dim3(1024), 0, s, hist, plan.G, dbase, (uint32_t*)nullptr, 0u);
+    GQF_LAUNCH(k_rs_rowscan, dim3(256), dim3(1024), 0, s, hist, plan.G, dbase, (uint32_t*)nullptr, 0u,
+               (uint32_t*)nullptr);
     K* dst = kb[cur];
     uint32_t* vdst = vb[cur];
     // read keys (+ values) once, write them once
