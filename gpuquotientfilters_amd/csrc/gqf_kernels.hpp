@@ -771,7 +771,7 @@ __global__ __launch_bounds__(SORT_THREADS) void k_rs_small(const void* __restric
 // the chunked passes instead.
 constexpr int SEG_THREADS = 512;
 constexpr int SEG_TILE = SEG_THREADS * SORT_ITEMS;  // 8192 keys
-constexpr uint32_t SEG_BB = 11;                     // bucket bits of the LDS group sort
+constexpr uint32_t SEG_BB = 13;                     // bucket bits of the LDS group sort
 constexpr uint32_t SEG_NBK = 1u << SEG_BB;          // its buckets (16-bit counters)
 constexpr uint32_t SEG_RANK_MAX = 32;               // largest bucket ranked key by key
 constexpr uint32_t SEG_SLOW_GRID = 256;             // workgroups walking the list of groups k_rs_seg left
@@ -1034,10 +1034,17 @@ __global__ __launch_bounds__(SEG_THREADS) void k_rs_seg(K* __restrict__ keys, co
   __syncthreads();
   {
     constexpr uint32_t PT = SEG_NBK / 2 / T;  // counter words per thread
-    static_assert(PT == 2, "4 bucket counters per thread");
-    uint2* c2 = reinterpret_cast<uint2*>(s_c) + tid;
-    const uint2 a = *c2;
-    const uint32_t w[PT] = {a.x, a.y};
+    static_assert(PT % 4 == 0, "whole 16-byte rows of counter words per thread");
+    uint4* c4 = reinterpret_cast<uint4*>(s_c) + tid * (PT / 4);
+    uint32_t w[PT];
+#pragma unroll
+    for (uint32_t i = 0; i < PT / 4; ++i) {
+      const uint4 a = c4[i];
+      w[4 * i] = a.x;
+      w[4 * i + 1] = a.y;
+      w[4 * i + 2] = a.z;
+      w[4 * i + 3] = a.w;
+    }
     uint32_t sum = 0, mx = 0;
 #pragma unroll
     for (uint32_t i = 0; i < PT; ++i) {
@@ -1055,7 +1062,8 @@ __global__ __launch_bounds__(SEG_THREADS) void k_rs_seg(K* __restrict__ keys, co
       st[i] = lo | (run << 16);
       run += w[i] >> 16;
     }
-    *c2 = make_uint2(st[0], st[1]);
+#pragma unroll
+    for (uint32_t i = 0; i < PT / 4; ++i) c4[i] = make_uint4(st[4 * i], st[4 * i + 1], st[4 * i + 2], st[4 * i + 3]);
     if (__syncthreads_or(mx > SEG_RANK_MAX)) {  // also orders the bucket starts
       if (INLINE) seg_counting_sort<K, T>(seg, m, nbits, s_keys, s_cnt, s_tmp);
       else if (tid == 0) slow[1 + atomicAdd(&slow[0], 1u)] = blockIdx.x;
