@@ -1139,7 +1139,9 @@ inline size_t sort_workspace_bytes(uint64_t n, size_t key_bytes, bool vals, bool
 // first pass.  vals: nullptr + with_vals -> the original index is carried.
 // Returns the buffers holding the result through *keys_out / *vals_out.
 // With the stream's context (ds), a keys-only sort partitions on its top
-// bits and sorts each group in LDS (segment sort, above).
+// bits and sorts each group in LDS (segment sort, above); after a crowded
+// probe the next 4, 8, ... 256 sorts on the stream skip straight to the
+// chunked passes.
 template <typename K>
 gqf_status radix_sort(hipStream_t s, Arena& ar, const void* in, bool hash, uint32_t fp_bits,
                       const uint32_t* in_vals, bool with_vals, uint32_t n, uint32_t lo_bit, uint32_t hi_bit,
@@ -1185,9 +1187,14 @@ gqf_status radix_sort(hipStream_t s, Arena& ar, const void* in, bool hash, uint3
   const uint32_t* vsrc = in_vals;
   int cur = 0;
   const uint32_t b2 = seg_bits2(n);
-  const bool seg1 = ds && !with_vals && lo_bit == 0 && b2 == 0 && hi_bit >= 16 && hi_bit - 8 <= 32;
-  const bool seg2 = ds && !with_vals && lo_bit == 0 && b2 >= 1 && b2 <= 8 && hi_bit >= 12 + b2 &&
-                    hi_bit - 8 - b2 <= 32;
+  constexpr int cw = sizeof(K) == 8 ? 1 : 0;
+  bool probe = ds && !with_vals && lo_bit == 0;
+  if (probe && ds->crowd_skip[cw]) {  // recent sorts of this width on this stream were crowded
+    --ds->crowd_skip[cw];
+    probe = false;
+  }
+  const bool seg1 = probe && b2 == 0 && hi_bit >= 16 && hi_bit - 8 <= 32;
+  const bool seg2 = probe && b2 >= 1 && b2 <= 8 && hi_bit >= 12 + b2 && hi_bit - 8 - b2 <= 32;
   uint32_t *hist2 = nullptr, *base2 = nullptr, *gstart = nullptr, *gsize = nullptr, *cb = nullptr, *slow = nullptr;
   const uint32_t G2 = seg2 ? seg_chunks2(n, plan.chunk) : 0u;
   const uint32_t ngroups = 256u << b2;
@@ -1241,7 +1248,11 @@ gqf_status radix_sort(hipStream_t s, Arena& ar, const void* in, bool hash, uint3
     }
     // the host reads the flag while the partition's downsweep runs
     GQF_HIP(hipEventSynchronize(ds->ev_crowd));
-    if (*reinterpret_cast<volatile uint32_t*>(ds->h_crowd) == 0u) {
+    const bool crowded = *reinterpret_cast<volatile uint32_t*>(ds->h_crowd) != 0u;
+    uint32_t& bo = ds->crowd_backoff[cw];
+    bo = crowded ? (bo ? (bo < 256 ? 2 * bo : 256) : 4) : 0;
+    ds->crowd_skip[cw] = bo;
+    if (!crowded) {
       if (seg2) {
         GQF_LAUNCH_B((double)n * 2.0 * sizeof(K), (k_rs_seg<K, false>), dim3(ngroups), dim3(SEG_THREADS), 0, s, out,
                      (const uint32_t*)nullptr, gstart, gsize, nbits, slow);

@@ -77,6 +77,11 @@ struct DeviceState {
   uint32_t* h_crowd = nullptr;
   uint32_t* d_crowd = nullptr;
   hipEvent_t ev_crowd = nullptr;
+  // after a crowded sort the next crowd_skip sorts take the chunked passes
+  // directly; crowd_backoff doubles with every crowded probe in a row.  One
+  // pair per key width: 32-bit (SQF, AP hash) and 64-bit (wide) fingerprints
+  // crowd differently.
+  uint32_t crowd_skip[2] = {0, 0}, crowd_backoff[2] = {0, 0};
   // per-launch timing (gqf_profile_enable): event pool and launches to resolve
   hipEvent_t* pool = nullptr;
   size_t pool_size = 0, pool_used = 0;

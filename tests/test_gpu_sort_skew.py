@@ -48,11 +48,11 @@ def _with_hot(seed, n, hot_copies):
     (14, 13, 14000, 9000),        # 16-bit slots
 ])
 @pytest.mark.parametrize("no_dup", [True, False])
-def test_sqf_build_sizes_and_hot_keys(q, r, n, hot, no_dup):
+def test_sqf_build_sizes_and_hot_keys(q, r, n, hot, no_dup, fresh_stream):
     say("sqf sort build", q, r, n, hot, no_dup)
     keys = _with_hot(41, n, hot)
     f = G.SQF(q, r)
-    f.bulk_build(dev(keys), NoDuplicates=no_dup)
+    f.bulk_build(dev(keys), NoDuplicates=no_dup, stream=fresh_stream)
     want = sqf_canon(keys, q, r, no_dup)
     got = f.table_bytes()
     assert (got == want).all(), np.flatnonzero(got != want)[:8]
@@ -207,24 +207,33 @@ def bench_zipf(n, s=1.1, universe=1 << 28, seed=7):
 SEG1_MAX = 256 * 7600
 
 
+@pytest.fixture
+def fresh_stream():
+    """A stream of its own: after crowded sorts a stream's next sorts skip
+    the partition, so tests of the partition path start on a new one."""
+    st = G.Stream()
+    yield st
+    st.destroy()
+
+
 @pytest.mark.parametrize("n", [SEG1_MAX, SEG1_MAX + 1, 2600000])
-def test_sqf_build_at_the_second_partition(n):
+def test_sqf_build_at_the_second_partition(n, fresh_stream):
     """Sizes at the bound of the second partition, 28-bit fingerprints."""
     q, r = 23, 5
     say("sqf build second partition", n)
     keys = O.mt19937(71, n)
     f = G.SQF(q, r)
-    f.bulk_build(dev(keys), NoDuplicates=False)
+    f.bulk_build(dev(keys), NoDuplicates=False, stream=fresh_stream)
     assert (f.table_bytes() == sqf_canon(keys, q, r, False)).all()
 
 
 @pytest.mark.parametrize("q,r,n", [(24, 10, 2600000), (23, 11, 4200000)])
-def test_rsqf_build_at_the_second_partition(q, r, n):
+def test_rsqf_build_at_the_second_partition(q, r, n, fresh_stream):
     """34-bit fingerprints: the groups sort their low 32 bits."""
     say("rsqf build second partition", q, r, n)
     keys = O.mt19937(72, n)
     f = G.RSQF(q, r)
-    f.bulk_build(dev(keys))
+    f.bulk_build(dev(keys), stream=fresh_stream)
     assert (f.dump() == _rsqf_canon(keys, q, r)).all()
 
 
@@ -246,14 +255,29 @@ def test_sqf_build_skewed_past_the_second_partition(kind):
     assert (f.table_bytes() == sqf_canon(keys, q, r, True)).all()
 
 
-def test_rsqf_insert_and_delete_batches_past_the_second_partition():
+def test_rsqf_insert_and_delete_batches_past_the_second_partition(fresh_stream):
     q, r = 24, 10
     say("rsqf insert/delete second partition")
     base = O.mt19937(75, 4000000)
     f = G.RSQF(q, r)
-    f.bulk_build(dev(base[:1500000]))
-    f.insertGPU(dev(base[1500000:]))
+    f.bulk_build(dev(base[:1500000]), stream=fresh_stream)
+    f.insertGPU(dev(base[1500000:]), stream=fresh_stream)
     assert (f.dump() == _rsqf_canon(base, q, r)).all()
     gone = base[:2200000]
-    f.deleteGPU(dev(gone))
+    f.deleteGPU(dev(gone), stream=fresh_stream)
     assert (f.dump() == _rsqf_canon(base[2200000:], q, r)).all()
+
+
+def test_crowded_then_uniform_sorts_on_one_stream(fresh_stream):
+    """After a crowded sort the stream's next sorts take the chunked passes
+    for a while, then probe the partition again: every table stays exact."""
+    q, r = 20, 5
+    say("crowded then uniform on one stream")
+    crowded = bench_zipf(1 << 17)
+    f = G.SQF(q, r)
+    f.bulk_build(dev(crowded), NoDuplicates=True, stream=fresh_stream)
+    assert (f.table_bytes() == sqf_canon(crowded, q, r, True)).all()
+    for i in range(7):
+        keys = O.mt19937(80 + i, 200000)
+        f.bulk_build(dev(keys), NoDuplicates=True, stream=fresh_stream)
+        assert (f.table_bytes() == sqf_canon(keys, q, r, True)).all()
