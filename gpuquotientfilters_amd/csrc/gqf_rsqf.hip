@@ -1229,7 +1229,6 @@ __global__ __launch_bounds__(WM_THREADS, (R == 8 ? 6 : 4)) void k_wm_write(uint8
   __shared__ uint32_t s_u[RD_SLOTS / 2];          // occupied list; merge bitmap; neighbours' quotients
   __shared__ uint32_t s_end[WM_IMG * 2];
   __shared__ uint32_t s_occ[RD_BLOCKS * 2];
-  __shared__ int32_t s_L[RD_BLOCKS];
   __shared__ uint32_t s_tmp[WM_THREADS / 64];
   __shared__ uint4 s_wave[WM_WAVES + 1];
   const uint32_t w = blockIdx.x, t = threadIdx.x, lane = t & 63, wid = t >> 6;
@@ -1392,60 +1391,89 @@ __global__ __launch_bounds__(WM_THREADS, (R == 8 ? 6 : 4)) void k_wm_write(uint8
 #pragma unroll
   for (uint32_t j = 0; j < WM_ITEMS; ++j)
     if (j < cnt) rs_push(a, w * RD_SLOTS + (v[j] >> rr));
-  __syncthreads();  // the bitmap is no longer read: the neighbours' quotients take its place
-  uint16_t* s_q0 = reinterpret_cast<uint16_t*>(s_u);
-  uint16_t* s_q1 = s_q0 + WM_THREADS;
-  if (cnt) {
-    s_q0[t] = (uint16_t)(v[0] >> rr);
-    s_q1[t] = (uint16_t)(v[cnt - 1] >> rr);
-  }
+  __syncthreads();  // the bitmap is no longer read: the first quotients take its place
+  uint16_t* s_q0 = reinterpret_cast<uint16_t*>(s_u);  // each thread's first quotient
+  if (cnt) s_q0[t] = (uint16_t)(v[0] >> rr);
   for (uint32_t i = t; i < WM_IMG * RW; i += WM_THREADS) s_rem[i] = 0u;
   if (t < WM_IMG * 2) s_end[t] = 0u;
-  if (t < RD_BLOCKS * 2) s_occ[t] = 0u;
+  // occupieds after the insert: the stored ones, and the batch's quotients (below)
+  if (t < 4 * RD_BLOCKS) reinterpret_cast<uint16_t*>(s_occ)[t] = (uint16_t)o16;
   RS ex;
   (void)rs_block_n<WM_WAVES>(a, &ex, s_wave);  // two barriers: the cleared images and the neighbours are visible after it
   const RS me = rs_combine(X, ex);
+  for (uint32_t j = t; j < nB; j += WM_THREADS) {
+    const uint32_t q = (sB[j] - kb32) >> rr;
+    atomicOr(&s_occ[q >> 5], 1u << (q & 31));
+  }
   const uint32_t s0 = w * RD_SLOTS;
   const uint32_t rmask = (1u << rr) - 1u;
   constexpr uint32_t NONE = RD_SLOTS + 64;  // past every window-relative quotient
-  int32_t prev = (int32_t)me.lp - 1;
-  uint32_t pq = i0 > 0 && cnt ? s_q1[t - 1] : NONE;  // quotient of the entry before
+  // quotient of the entry after this thread's last one
+  const uint32_t nq_last = i0 + cnt < nM ? s_q0[t + 1] : NONE;
+  // every entry at its location: remainder, and the run end where the next
+  // entry's quotient differs (whole threads: no count checks)
+  auto place = [&](auto full) {
+    constexpr bool FULL = decltype(full)::value;
+    int32_t prev = (int32_t)me.lp - 1;
 #pragma unroll
-  for (uint32_t j = 0; j < WM_ITEMS; ++j) {
-    if (j < cnt) {
-      const uint32_t fqr = v[j] >> rr;
-      const int32_t fq = (int32_t)(s0 + fqr);
-      const int32_t lc = fq > prev + 1 ? fq : prev + 1;
-      const uint32_t rel = (uint32_t)lc - s0;  // < 4096 + 64 WM_SIDE: k_wm_check
-      const uint32_t nq = j + 1 < cnt ? v[j + 1] >> rr : (i0 + j + 1 < nM ? s_q0[t + 1] : NONE);
-      if (rel < WM_IMG * 64) {
-        wm_put_rem<R, RW>(s_rem, rel, rr, v[j] & rmask);
-        if (fqr != pq) atomicOr(&s_occ[fqr >> 5], 1u << (fqr & 31));
-        if (nq != fqr) {
-          atomicOr(&s_end[rel >> 5], 1u << (rel & 31));
-          // blocks whose first quotient lies in (fqr, nq]: the last entry before them is this one
-          const uint32_t jb1 = (nq >> 6) < RD_BLOCKS - 1 ? (nq >> 6) : RD_BLOCKS - 1;
-          for (uint32_t jb = (fqr >> 6) + 1; jb <= jb1; ++jb) s_L[jb] = lc;
+    for (uint32_t j = 0; j < WM_ITEMS; ++j) {
+      if (FULL || j < cnt) {
+        const uint32_t fqr = v[j] >> rr;
+        const int32_t fq = (int32_t)(s0 + fqr);
+        const int32_t lc = fq > prev + 1 ? fq : prev + 1;
+        const uint32_t rel = (uint32_t)lc - s0;  // < 4096 + 64 WM_SIDE: k_wm_check
+        const uint32_t nq = j + 1 < WM_ITEMS && (FULL || j + 1 < cnt) ? v[j + 1] >> rr : nq_last;
+        if (rel < WM_IMG * 64) {
+          wm_put_rem<R, RW>(s_rem, rel, rr, v[j] & rmask);
+          if (nq != fqr) atomicOr(&s_end[rel >> 5], 1u << (rel & 31));
         }
+        prev = lc;
       }
-      prev = lc;
-      pq = fqr;
+    }
+  };
+  if (cnt == WM_ITEMS) place(std::true_type{});
+  else if (cnt) place(std::false_type{});
+  __syncthreads();
+  // ---- block offsets (one wave): the last entry before block b's first
+  // quotient ends the run of the window's k-th occupied quotient, k = its
+  // occupieds before the block -- the k-th run end of the window in s_end --
+  // or, for k = 0, is the last entry of earlier windows
+  if (t < RD_BLOCKS) {
+    static_assert(WM_IMG * 2 == 4 * RD_BLOCKS, "four run-end words per lane");
+    uint32_t es = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < 4; ++i) es += (uint32_t)__popc(s_end[4 * t + i]);
+    const uint32_t ep = wave_incl_add(es) - es;  // run ends before word 4 t
+    const uint32_t oc = (uint32_t)__popc(s_occ[2 * t]) + (uint32_t)__popc(s_occ[2 * t + 1]);
+    const uint32_t k = wave_incl_add(oc) - oc;
+    uint32_t lo = 0, hi = RD_BLOCKS;  // the last word group whose run ends before it are fewer than k
+#pragma unroll
+    for (int it = 0; it < 6; ++it) {
+      const uint32_t mid = (lo + hi) >> 1;
+      const uint32_t em = (uint32_t)__shfl((int)ep, (int)mid);
+      if (em < k) lo = mid;
+      else hi = mid;
+    }
+    uint32_t rest = k - (uint32_t)__shfl((int)ep, (int)lo), word = 4 * lo + 3;
+    bool found = false;
+#pragma unroll
+    for (uint32_t i = 0; i < 4; ++i) {
+      const uint32_t c = (uint32_t)__popc(s_end[4 * lo + i]);
+      if (!found && rest <= c) {
+        word = 4 * lo + i;
+        found = true;
+      } else if (!found) {
+        rest -= c;
+      }
+    }
+    const int64_t L = k ? (int64_t)s0 + 32 * word + select_bit64(s_end[word], rest) : (int64_t)X.lp - 1;
+    if (bw + t < nb) {
+      const int64_t sb = 64 * (int64_t)(bw + t);
+      meta[4 * (bw + t) + 2] = L >= sb ? (uint64_t)((L - sb + 1) & 0xFF) : 0ull;
     }
   }
-  if (t == 0) {
-    // blocks up to the first quotient: the last entry of earlier windows
-    const uint32_t f0 = nM ? v[0] >> rr : NONE;
-    const uint32_t jb1 = (f0 >> 6) < RD_BLOCKS - 1 ? (f0 >> 6) : RD_BLOCKS - 1;
-    for (uint32_t jb = 0; jb <= jb1; ++jb) s_L[jb] = (int32_t)X.lp - 1;
-  }
-  __syncthreads();
   // ---- stores: whole words where the window owns the block, ORs into saved blocks
-  if (t < RD_BLOCKS && bw + t < nb) {
-    const uint64_t b = bw + t;
-    meta[4 * b] = ((uint64_t)s_occ[2 * t + 1] << 32) | s_occ[2 * t];
-    const int64_t L = s_L[t], sb = 64 * (int64_t)b;
-    meta[4 * b + 2] = L >= sb ? (uint64_t)((L - sb + 1) & 0xFF) : 0ull;
-  }
+  if (t < RD_BLOCKS && bw + t < nb) meta[4 * (bw + t)] = ((uint64_t)s_occ[2 * t + 1] << 32) | s_occ[2 * t];
   if (t < WM_IMG && bw + t < nb) {
     const uint64_t b = bw + t;
     const uint64_t e = ((uint64_t)s_end[2 * t + 1] << 32) | s_end[2 * t];
