@@ -1055,7 +1055,7 @@ __global__ __launch_bounds__(WS_THREADS) void k_wm_sum(RsqfDev f, const uint32_t
                                                        const K* __restrict__ B, const uint32_t* __restrict__ bstart,
                                                        uint4* __restrict__ agg, uint32_t* __restrict__ wold,
                                                        uint32_t* __restrict__ ctr) {
-  __shared__ uint32_t s_cnt[RD_SLOTS / 2];   // batch entries per quotient, 16 bits each
+  __shared__ __align__(16) uint32_t s_cnt[RD_SLOTS / 2];  // batch entries per quotient, 16 bits each
   __shared__ int16_t s_rend[RD_SLOTS + 1];   // image slot of the (d + i)-th run end; -1: before the window
   __shared__ uint32_t s_tmp[WS_THREADS / 64];
   __shared__ uint4 s_wave[WS_THREADS / 64 + 1];
@@ -1078,7 +1078,7 @@ __global__ __launch_bounds__(WS_THREADS) void k_wm_sum(RsqfDev f, const uint32_t
     uint32_t k = pk & 0xFFFFu;  // run ends before this thread's first slot
     for (uint32_t x = e32; x; x &= x - 1u) {
       ++k;  // rank of this run end
-      if (k >= d && k - d <= O) s_rend[k - d] = (int16_t)(32 * t + (uint32_t)__builtin_ctz(x));
+      if (k - d <= O) s_rend[k - d] = (int16_t)(32 * t + (uint32_t)__builtin_ctz(x));  // k < d wraps past O
     }
   }
   if (b1 - b0 <= WM_CAP) {
@@ -1089,16 +1089,25 @@ __global__ __launch_bounds__(WS_THREADS) void k_wm_sum(RsqfDev f, const uint32_t
   const bool bad = d + O > E;  // a run end the image does not hold
   RS x = rs_none();
   uint32_t j = pk >> 16;  // occupied quotients before this thread's first one
+  // this thread's 16 batch counts in two 16-byte reads
+  uint32_t bc[8];
+  {
+    const uint4* c4 = reinterpret_cast<const uint4*>(s_cnt) + 2 * t;
+    const uint4 lo = c4[0], hi = c4[1];
+    bc[0] = lo.x, bc[1] = lo.y, bc[2] = lo.z, bc[3] = lo.w, bc[4] = hi.x, bc[5] = hi.y, bc[6] = hi.z, bc[7] = hi.w;
+  }
+  int32_t pe = bad ? 0 : s_rend[j];  // end of the run before this thread's first occupied quotient
 #pragma unroll
   for (uint32_t i = 0; i < 16; ++i) {
     const uint32_t p = 16 * t + i;
-    uint32_t n = wm_cnt_get(s_cnt, p);
+    uint32_t n = (bc[i >> 1] >> (16 * (i & 1))) & 0xFFFFu;
     if ((o16 >> i) & 1u) {
       ++j;
       if (!bad) {
-        const int32_t e = s_rend[j], pe = s_rend[j - 1];
+        const int32_t e = s_rend[j];
         const int32_t st = (int32_t)p > pe + 1 ? (int32_t)p : pe + 1;
         if (e >= st) n += (uint32_t)(e - st + 1);
+        pe = e;
       }
     }
     if (n) rs_push_run(x, w * RD_SLOTS + p, n);
