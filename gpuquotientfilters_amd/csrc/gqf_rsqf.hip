@@ -1318,7 +1318,7 @@ __global__ __launch_bounds__(WM_THREADS, (R == 8 ? 6 : 4)) void k_wm_write(uint8
 #pragma unroll
     for (uint32_t i = 0; i < 16; ++i) {
       const uint32_t s = 16 * t + i;
-      const uint32_t Q = (k >= d && k - d < O) ? s_list[k - d] : 0xFFFFu;
+      const uint32_t Q = k - d < O ? s_list[k - d] : 0xFFFFu;  // k < d wraps past O
       qv[i] = Q;
       if (Q <= s) m |= 1u << i;
       k += (e16 >> i) & 1u;
