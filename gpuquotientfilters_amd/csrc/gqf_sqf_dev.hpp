@@ -1008,19 +1008,47 @@ __device__ __forceinline__ void sqf_load16(const uint8_t* __restrict__ t, uint32
   }
 }
 
+// Entries, run starts and occupieds among the 4 one-byte slots of a table
+// word (byte b = slot 4k + b), all at once: per byte, bit 2 = occupied,
+// bit 1 = continuation, any of bits 0..2 = holds an entry.
+__device__ __forceinline__ void sqf_flags4(uint32_t w, uint32_t& ci, uint32_t& cr, uint32_t& co) {
+  const uint32_t f = w & 0x07070707u;
+  const uint32_t nz = (f | (f >> 1) | (f >> 2)) & 0x01010101u;
+  ci += (uint32_t)__popc(nz);
+  cr += (uint32_t)__popc(nz & ~(w >> 1));
+  co += (uint32_t)__popc(w & 0x04040404u);
+}
+
+// The 16 one-byte slots [s, s+16) as 4 words when whole and aligned.
+__device__ __forceinline__ bool sqf_words16(const uint8_t* __restrict__ t, uint32_t S, uint32_t s, uint32_t (&w)[4]) {
+  if (s + 16 > S || (reinterpret_cast<uintptr_t>(t) & 15u) != 0) return false;
+  const uint4 x = *reinterpret_cast<const uint4*>(t + s);
+  w[0] = x.x;
+  w[1] = x.y;
+  w[2] = x.z;
+  w[3] = x.w;
+  return true;
+}
+
 template <int BPE>
 __global__ __launch_bounds__(256) void k_sqf_count(const uint8_t* __restrict__ t, uint32_t S,
                                                    uint32_t* __restrict__ counts) {
   __shared__ uint64_t s_tmp[4];
   const uint32_t s0 = blockIdx.x * DC_SLOTS + threadIdx.x * 16;
-  uint32_t e[16];
-  sqf_load16<BPE>(t, S, s0, e);
   uint32_t ci = 0, cr = 0, co = 0;
+  uint32_t w[4];
+  if (BPE == 1 && sqf_words16(t, S, s0, w)) {
 #pragma unroll
-  for (int j = 0; j < 16; ++j) {
-    ci += (e[j] & 7u) != 0;
-    cr += ((e[j] & 7u) != 0) && !(e[j] & 2u);
-    co += (e[j] & 4u) != 0;
+    for (int k = 0; k < 4; ++k) sqf_flags4(w[k], ci, cr, co);
+  } else {
+    uint32_t e[16];
+    sqf_load16<BPE>(t, S, s0, e);
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      ci += (e[j] & 7u) != 0;
+      cr += ((e[j] & 7u) != 0) && !(e[j] & 2u);
+      co += (e[j] & 4u) != 0;
+    }
   }
   uint64_t tot;
   block256_excl_scan64((uint64_t)ci | ((uint64_t)cr << 21) | ((uint64_t)co << 42), s_tmp, &tot);
@@ -1064,13 +1092,23 @@ __global__ __launch_bounds__(256) void k_sqf_decode(const uint8_t* __restrict__ 
   const uint32_t s0 = blockIdx.x * DC_SLOTS;
   const uint32_t bI = bases[blockIdx.x * 3 + 0], bR = bases[blockIdx.x * 3 + 1], bO = bases[blockIdx.x * 3 + 2];
   uint32_t e[16];
-  sqf_load16<BPE>(t, S, s0 + tid * 16, e);
   uint32_t ci = 0, cr = 0, co = 0;
+  uint32_t w[4];
+  if (BPE == 1 && sqf_words16(t, S, s0 + tid * 16, w)) {
 #pragma unroll
-  for (int j = 0; j < 16; ++j) {
-    ci += (e[j] & 7u) != 0;
-    cr += ((e[j] & 7u) != 0) && !(e[j] & 2u);
-    co += (e[j] & 4u) != 0;
+    for (int k = 0; k < 4; ++k) {
+      sqf_flags4(w[k], ci, cr, co);
+#pragma unroll
+      for (int b = 0; b < 4; ++b) e[4 * k + b] = (w[k] >> (8 * b)) & 0xFFu;
+    }
+  } else {
+    sqf_load16<BPE>(t, S, s0 + tid * 16, e);
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      ci += (e[j] & 7u) != 0;
+      cr += ((e[j] & 7u) != 0) && !(e[j] & 2u);
+      co += (e[j] & 4u) != 0;
+    }
   }
   uint64_t btot;
   const uint64_t ex = block256_excl_scan64((uint64_t)ci | ((uint64_t)cr << 21) | ((uint64_t)co << 42), s_tmp, &btot);
