@@ -1060,9 +1060,11 @@ __global__ __launch_bounds__(256) void k_sqf_count(const uint8_t* __restrict__ t
 }
 
 // Quotient of the entry at slot idx by walking its cluster (extractQuotients,
-// quotientFilter.cu:1580-1599); fallback for very long pending chains.
+// quotientFilter.cu:1580-1599); fallback for very long pending chains, kept
+// out of line so that the decode's unrolled entry loop holds one call, not
+// sixteen copies of the walk.
 template <int BPE>
-__device__ uint32_t sqf_walk_quotient(const uint8_t* __restrict__ t, uint32_t S, uint32_t idx) {
+__device__ __noinline__ uint32_t sqf_walk_quotient(const uint8_t* __restrict__ t, uint32_t S, uint32_t idx) {
   uint32_t b = idx;
   do {
     b--;
