@@ -579,19 +579,16 @@ __device__ __forceinline__ void rb_merge_staged(const uint32_t* buf, const uint3
     v[0] = 0u;
   }
   const uint32_t bits = w >> (i0 & 31);
-  tags = 0;
+  // branch-free: one LDS read per entry, from B (buf + la) or A
+  tags = i0 + RB_ITEMS <= tn ? bits & ((1u << RB_ITEMS) - 1u)
+                             : (i0 < tn ? bits & ((1u << (tn - i0)) - 1u) : 0u);
 #pragma unroll
   for (int j = 0; j < RB_ITEMS; ++j) {
-    uint32_t x = 0;
-    if (i0 + j < tn) {
-      if ((bits >> j) & 1u) {
-        x = sB[bi++];
-        tags |= 1u << j;
-      } else {
-        x = sA[ai++];
-      }
-    }
-    v[1 + j] = x;
+    const uint32_t isb = (bits >> j) & 1u;
+    const uint32_t at = isb ? la + bi : ai;
+    v[1 + j] = i0 + j < tn ? buf[at] : 0u;
+    bi += isb;
+    ai += isb ^ 1u;
   }
 }
 
