@@ -510,6 +510,8 @@ __device__ __forceinline__ void rb_load(const uint32_t* __restrict__ A, const ui
 // workgroup works on the previous sub-tile; the entries just before the
 // ranges (the first thread's previous entry) go to prev2[0] (A) and
 // prev2[1] (B).  rb_stage_wait() and a barrier make a buffer readable.
+// FULL: tn == RB_TILE (every position is in range, no per-entry tests).
+template <bool FULL = false>
 __device__ __forceinline__ void rb_stage_issue(const uint32_t* __restrict__ A, const uint32_t* __restrict__ B,
                                                uint32_t a0, uint32_t a1, uint32_t d0, uint32_t tn, uint32_t* buf,
                                                uint32_t* prev2) {
@@ -520,7 +522,7 @@ __device__ __forceinline__ void rb_stage_issue(const uint32_t* __restrict__ A, c
     // wave wid, step k: the 64 consecutive positions from base (lane l -> buf[base + l])
     const uint32_t base = ((uint32_t)k * RB_WAVES + wid) * 64;
     const uint32_t pos = base + lane;
-    if (pos < tn) {
+    if (FULL || pos < tn) {
       const uint32_t* src = pos < la ? A + (a0 + pos) : B + (b0 + (pos - la));
       __builtin_amdgcn_global_load_lds(src, buf + base, 4, 0, 0);
     }
@@ -529,13 +531,21 @@ __device__ __forceinline__ void rb_stage_issue(const uint32_t* __restrict__ A, c
     __builtin_amdgcn_global_load_lds(lane == 0 ? A + (a0 - 1) : B + (b0 - 1), prev2, 4, 0, 0);
 }
 
+// rb_stage_issue, with the whole-tile form for whole tiles.
+__device__ __forceinline__ void rb_stage(const uint32_t* __restrict__ A, const uint32_t* __restrict__ B, uint32_t a0,
+                                         uint32_t a1, uint32_t d0, uint32_t tn, uint32_t* buf, uint32_t* prev2) {
+  if (tn == RB_TILE) rb_stage_issue<true>(A, B, a0, a1, d0, tn, buf, prev2);
+  else rb_stage_issue<false>(A, B, a0, a1, d0, tn, buf, prev2);
+}
+
 // This thread's RB_ITEMS merged entries of a staged sub-tile (as rb_load;
 // tags bit j: entry j came from B).  A batch adds few entries to a sub-tile,
 // so instead of a merge-path search per thread every B entry finds its
 // output position (its index + the A entries <= it: A first on ties) and
 // marks it in the bitmap s_bmp (RB_TILE bits, zero on entry); a prefix count
 // of the bitmap words (s_wpre) gives each thread its A and B offsets.  Every
-// thread of the block calls it (two barriers inside).
+// thread of the block calls it (two barriers inside).  FULL: tn == RB_TILE.
+template <bool FULL = false>
 __device__ __forceinline__ void rb_merge_staged(const uint32_t* buf, const uint32_t* prev2, uint32_t a0, uint32_t a1,
                                                 uint32_t d0, uint32_t tn, uint32_t* v, uint32_t& tags,
                                                 uint32_t* s_bmp, uint32_t* s_wpre) {
@@ -572,7 +582,7 @@ __device__ __forceinline__ void rb_merge_staged(const uint32_t* buf, const uint3
     const bool hasA = a0 > 0, hasB = b0 > 0;
     const uint32_t pA = hasA ? prev2[0] : 0u, pB = hasB ? prev2[1] : 0u;
     v[0] = (hasB && (!hasA || pA <= pB)) ? pB : pA;
-  } else if (i0 <= tn) {
+  } else if (FULL || i0 <= tn) {
     const bool pb = (s_bmp[(i0 - 1) >> 5] >> ((i0 - 1) & 31)) & 1u;
     v[0] = pb ? sB[bi - 1] : sA[ai - 1];
   } else {
@@ -580,20 +590,21 @@ __device__ __forceinline__ void rb_merge_staged(const uint32_t* buf, const uint3
   }
   const uint32_t bits = w >> (i0 & 31);
   // branch-free: one LDS read per entry, from B (buf + la) or A
-  tags = i0 + RB_ITEMS <= tn ? bits & ((1u << RB_ITEMS) - 1u)
+  tags = FULL || i0 + RB_ITEMS <= tn ? bits & ((1u << RB_ITEMS) - 1u)
                              : (i0 < tn ? bits & ((1u << (tn - i0)) - 1u) : 0u);
 #pragma unroll
   for (int j = 0; j < RB_ITEMS; ++j) {
     const uint32_t isb = (bits >> j) & 1u;
     const uint32_t at = isb ? la + bi : ai;
-    v[1 + j] = i0 + j < tn ? buf[at] : 0u;
+    v[1 + j] = FULL || i0 + j < tn ? buf[at] : 0u;
     bi += isb;
     ai += isb ^ 1u;
   }
 }
 
-// Keep rule of this thread's entries (bit j of the result) and their scan state.
-template <int MODE>
+// Keep rule of this thread's entries (bit j of the result) and their scan
+// state.  FULL: tn == RB_TILE.
+template <int MODE, bool FULL = false>
 __device__ __forceinline__ uint32_t rb_classify(const uint32_t* v, uint32_t tags, uint32_t fl, uint32_t d0,
                                                 uint32_t tn, uint32_t r, uint64_t mq, RS& agg) {
   uint32_t keepbits = 0;
@@ -602,7 +613,7 @@ __device__ __forceinline__ uint32_t rb_classify(const uint32_t* v, uint32_t tags
 #pragma unroll
   for (int j = 0; j < RB_ITEMS; ++j) {
     const uint32_t i = i0 + j;
-    if (i < tn) {
+    if (FULL || i < tn) {
       const uint32_t fp = v[1 + j], pfp = v[j];
       const bool first = (d0 + i) == 0;
       bool kp;
@@ -648,7 +659,7 @@ __global__ __launch_bounds__(RB_THREADS, 8) void k_rb_agg(const uint32_t* __rest
     ca1 = splits[u0 + 1];
     if (c0 + RB_TILE < c1) na1 = splits[u0 + 2];
     na0 = ca1;
-    rb_stage_issue(A, B, ca0, ca1, (uint32_t)c0, (uint32_t)(c1 - c0 < RB_TILE ? c1 - c0 : RB_TILE), s_stage[0],
+    rb_stage(A, B, ca0, ca1, (uint32_t)c0, (uint32_t)(c1 - c0 < RB_TILE ? c1 - c0 : RB_TILE), s_stage[0],
                    s_prev[0]);
   }
   int buf = 0;
@@ -664,10 +675,13 @@ __global__ __launch_bounds__(RB_THREADS, 8) void k_rb_agg(const uint32_t* __rest
       if (threadIdx.x < RB_TILE / 32) s_bmp[threadIdx.x] = 0u;
       __syncthreads();
       if (dn < c1)
-        rb_stage_issue(A, B, na0, na1, (uint32_t)dn, (uint32_t)(c1 - dn < RB_TILE ? c1 - dn : RB_TILE),
+        rb_stage(A, B, na0, na1, (uint32_t)dn, (uint32_t)(c1 - dn < RB_TILE ? c1 - dn : RB_TILE),
                        s_stage[buf ^ 1], s_prev[buf ^ 1]);
       const uint32_t nn1 = dn + RB_TILE < c1 ? splits[u + 3] : 0u;
-      rb_merge_staged(s_stage[buf], s_prev[buf], ca0, ca1, (uint32_t)d0, tn, v, tags, s_bmp, s_wpre);
+      if (tn == RB_TILE)
+        rb_merge_staged<true>(s_stage[buf], s_prev[buf], ca0, ca1, (uint32_t)d0, tn, v, tags, s_bmp, s_wpre);
+      else
+        rb_merge_staged<false>(s_stage[buf], s_prev[buf], ca0, ca1, (uint32_t)d0, tn, v, tags, s_bmp, s_wpre);
       ca0 = na0;
       ca1 = na1;
       na0 = na1;
@@ -677,7 +691,8 @@ __global__ __launch_bounds__(RB_THREADS, 8) void k_rb_agg(const uint32_t* __rest
       rb_load<MODE>(A, flags, (uint32_t)d0, tn, v, tags, fl);
     }
     RS a;
-    (void)rb_classify<MODE>(v, tags, fl, (uint32_t)d0, tn, r, mq, a);
+    if (tn == RB_TILE) (void)rb_classify<MODE, true>(v, tags, fl, (uint32_t)d0, tn, r, mq, a);
+    else (void)rb_classify<MODE, false>(v, tags, fl, (uint32_t)d0, tn, r, mq, a);
     RS ex;
     const RS sub = rs_block(a, &ex, s_wave);
     X = rs_combine(X, sub);
@@ -864,7 +879,7 @@ __global__ __launch_bounds__(RB_THREADS, 6) void k_rb_write(uint8_t* __restrict_
     ca1 = splits[u0 + 1];
     if (c0 + RB_TILE < c1) na1 = splits[u0 + 2];
     na0 = ca1;
-    rb_stage_issue(A, B, ca0, ca1, (uint32_t)c0, (uint32_t)(c1 - c0 < RB_TILE ? c1 - c0 : RB_TILE), s_stage[0],
+    rb_stage(A, B, ca0, ca1, (uint32_t)c0, (uint32_t)(c1 - c0 < RB_TILE ? c1 - c0 : RB_TILE), s_stage[0],
                    s_prev[0]);
   }
   int buf = 0;
@@ -880,10 +895,13 @@ __global__ __launch_bounds__(RB_THREADS, 6) void k_rb_write(uint8_t* __restrict_
       if (threadIdx.x < RB_TILE / 32) s_bmp[threadIdx.x] = 0u;
       __syncthreads();
       if (dn < c1)
-        rb_stage_issue(A, B, na0, na1, (uint32_t)dn, (uint32_t)(c1 - dn < RB_TILE ? c1 - dn : RB_TILE),
+        rb_stage(A, B, na0, na1, (uint32_t)dn, (uint32_t)(c1 - dn < RB_TILE ? c1 - dn : RB_TILE),
                        s_stage[buf ^ 1], s_prev[buf ^ 1]);
       const uint32_t nn1 = dn + RB_TILE < c1 ? splits[u + 3] : 0u;
-      rb_merge_staged(s_stage[buf], s_prev[buf], ca0, ca1, (uint32_t)d0, tn, v, tags, s_bmp, s_wpre);
+      if (tn == RB_TILE)
+        rb_merge_staged<true>(s_stage[buf], s_prev[buf], ca0, ca1, (uint32_t)d0, tn, v, tags, s_bmp, s_wpre);
+      else
+        rb_merge_staged<false>(s_stage[buf], s_prev[buf], ca0, ca1, (uint32_t)d0, tn, v, tags, s_bmp, s_wpre);
       ca0 = na0;
       ca1 = na1;
       na0 = na1;
@@ -893,7 +911,8 @@ __global__ __launch_bounds__(RB_THREADS, 6) void k_rb_write(uint8_t* __restrict_
       rb_load<MODE>(A, flags, (uint32_t)d0, tn, v, tags, fl);
     }
     RS a;
-    const uint32_t keepbits = rb_classify<MODE>(v, tags, fl, (uint32_t)d0, tn, r, mq, a);
+    const uint32_t keepbits = tn == RB_TILE ? rb_classify<MODE, true>(v, tags, fl, (uint32_t)d0, tn, r, mq, a)
+                                            : rb_classify<MODE, false>(v, tags, fl, (uint32_t)d0, tn, r, mq, a);
     X = rb_emit<BPE>(t, q, r, v, keepbits, a, X, u, s_wave, s_win, &s_nfix, fix, fix_cnt, fix_more,
                      fix_more_n, fix_cap, precleared);
     __syncthreads();
