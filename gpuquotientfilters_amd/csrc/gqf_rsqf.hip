@@ -634,7 +634,8 @@ __device__ __forceinline__ void rr_load(const FpArray<K, PK>& A, const uint8_t* 
   }
 }
 
-template <typename K, int MODE>
+// FULL: tn == RB_TILE (no per-entry range tests).
+template <typename K, int MODE, bool FULL = false>
 __device__ __forceinline__ uint32_t rr_classify(const K* v, uint32_t fl, uint32_t tn, uint32_t r, uint32_t mq,
                                                 RS& agg) {
   uint32_t keepbits = 0;
@@ -642,7 +643,7 @@ __device__ __forceinline__ uint32_t rr_classify(const K* v, uint32_t fl, uint32_
   const uint32_t i0 = (uint32_t)threadIdx.x * RB_ITEMS;
 #pragma unroll
   for (int j = 0; j < RB_ITEMS; ++j) {
-    if (i0 + j < tn && (MODE == KEEP_ALL || ((fl >> j) & 1u))) {
+    if ((FULL || i0 + j < tn) && (MODE == KEEP_ALL || ((fl >> j) & 1u))) {
       keepbits |= 1u << j;
       rs_push(agg, (uint32_t)(v[j] >> r) & mq);
     }
@@ -707,7 +708,8 @@ __global__ __launch_bounds__(RB_THREADS, 8) void k_rr_agg(const K* __restrict__ 
       rr_load<K, MODE, PK>(A, flags, (uint32_t)d0, tn, v, fl);
     }
     RS a;
-    (void)rr_classify<K, MODE>(v, fl, tn, r, mq, a);
+    if (tn == RB_TILE) (void)rr_classify<K, MODE, true>(v, fl, tn, r, mq, a);
+    else (void)rr_classify<K, MODE, false>(v, fl, tn, r, mq, a);
     RS ex;
     X = rs_combine(X, rs_block(a, &ex, s_wave));
     __syncthreads();
@@ -838,7 +840,8 @@ __global__ __launch_bounds__(RB_THREADS, 6) void k_rr_write(uint8_t* __restrict_
       rr_load<K, MODE, PK>(A, flags, (uint32_t)d0, tn, v, fl);
     }
     RS a;
-    const uint32_t keepbits = rr_classify<K, MODE>(v, fl, tn, rr, mq, a);
+    const uint32_t keepbits = tn == RB_TILE ? rr_classify<K, MODE, true>(v, fl, tn, rr, mq, a)
+                                            : rr_classify<K, MODE, false>(v, fl, tn, rr, mq, a);
     RS ex;
     const RS sub = rs_block(a, &ex, s_wave);
     const RS me = rs_combine(X, ex);

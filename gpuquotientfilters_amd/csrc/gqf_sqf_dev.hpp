@@ -1174,7 +1174,8 @@ __global__ __launch_bounds__(256) void k_sqf_decode(const uint8_t* __restrict__ 
     } else {
       fq = sqf_walk_quotient<BPE>(t, S, s0 + tid * 16 + j);
     }
-    s_out[xi] = (K)(((uint64_t)fq << r) | (el >> 3));
+    if constexpr (sizeof(K) == 4) s_out[xi] = (fq << r) | (el >> 3);  // q + r <= 32
+    else s_out[xi] = (K)(((uint64_t)fq << r) | (el >> 3));
     xi++;
   }
   __syncthreads();
