@@ -1404,6 +1404,35 @@ __device__ __forceinline__ uint32_t upper_bound_g(const K* __restrict__ a, uint3
   return lo;
 }
 
+// The first index in [0, n) whose element is not below x (UPPER: is above
+// x), or n, by an 8-ary search over the first 8 lanes of a wave: every lane
+// of the wave calls it and gets the result.  Each round probes 8 points
+// that cut [lo, hi] into 9 parts, so the chain of dependent loads is
+// log9(n) long instead of log2(n).
+template <bool UPPER, typename K, typename AA>
+__device__ __forceinline__ uint32_t wave8_bound(AA a, uint32_t n, K x) {
+  const uint32_t j = threadIdx.x & 63u;
+  uint32_t lo = 0, hi = n;  // the answer lies in [lo, hi]
+  while (hi - lo >= 9) {
+    const uint64_t span = hi - lo;
+    bool before = false;  // the probe lies before the answer
+    if (j < 8) {
+      const K y = a[lo + (uint32_t)(span * (j + 1) / 9)];  // in (lo, hi)
+      before = UPPER ? !(x < y) : (y < x);
+    }
+    const uint32_t c = (uint32_t)__popcll(__ballot(before));  // probes before the answer
+    const uint32_t nlo = c ? lo + (uint32_t)(span * c / 9) + 1 : lo;
+    hi = c < 8 ? lo + (uint32_t)(span * (c + 1) / 9) : hi;
+    lo = nlo;
+  }
+  bool before = false;
+  if (j < 8 && lo + j < hi) {
+    const K y = a[lo + j];
+    before = UPPER ? !(x < y) : (y < x);
+  }
+  return lo + (uint32_t)__popcll(__ballot(before));
+}
+
 // First index of the equal run of A that contains i (galloping search back).
 template <typename AA>
 __device__ __forceinline__ uint32_t run_first(AA A, uint32_t i) {
@@ -1440,8 +1469,14 @@ __global__ __launch_bounds__(DM_THREADS) void k_delete_mark(const K* __restrict_
   const uint32_t t0 = blockIdx.x * (uint32_t)DM_TILE;
   if (t0 >= na) return;
   const uint32_t t1 = t0 + DM_TILE < na ? t0 + DM_TILE : na;
-  if (tid == 0) s_rng[0] = lower_bound_g(D, 0u, nd, A[t0]);
-  if (tid == 64) s_rng[1] = upper_bound_g(D, 0u, nd, A[t1 - 1]);
+  // the slice of D this tile can match (waves 0 and 1: uniform branches)
+  if (tid < 64) {
+    const uint32_t b = wave8_bound<false>(D, nd, A[t0]);
+    if (tid == 0) s_rng[0] = b;
+  } else if (tid < 128) {
+    const uint32_t b = wave8_bound<true>(D, nd, A[t1 - 1]);
+    if (tid == 64) s_rng[1] = b;
+  }
   // this thread's entries
   const uint32_t i0 = t0 + (uint32_t)tid * DM_ITEMS;
   K v[DM_ITEMS];
