@@ -1633,7 +1633,7 @@ static gqf_status rsqf_rebuild_t(hipStream_t s, DeviceState& ds, Arena& ar, cons
   const double nknown = na_dev ? (double)(MERGE ? nbk : 0u) : (double)N;  // entries the host knows of
   const double kper = na_dev ? (double)sizeof(K) : per;
   if (MERGE)
-    GQF_LAUNCH((k_rb_splits<K, PK>), dim3((p.nsub + 1 + 255) / 256), dim3(256), 0, s, A, na, na_dev, B, nbk, p.nsub,
+    GQF_LAUNCH((k_rb_splits<K, PK>), dim3(((p.nsub + 1) * RB_SPLIT_LANES + 255) / 256), dim3(256), 0, s, A, na, na_dev, B, nbk, p.nsub,
                splits, hb);
   GQF_LAUNCH_BC(nknown * kper, na_dev, per, (k_rr_agg<K, MODE, MERGE, PK>), dim3(p.G), dim3(RB_THREADS), 0, s, A,
                 na, na_dev, B, nbk, flags, splits, g.q, g.r, p.chunk, agg, hb, ds.counters);

@@ -155,7 +155,7 @@ static gqf_status launch_rebuild(hipStream_t s, uint32_t* counters, Arena& ar, c
   const uint32_t nadd = MERGE ? nb : 0u;
   const double nknown = na_dev ? (double)nadd : (double)N;  // entries the host knows of; the rest: 4 B each on the device
   if (MERGE)
-    GQF_LAUNCH((k_rb_splits<uint32_t, false>), dim3((p.nsub + 1 + 255) / 256), dim3(256), 0, s, A, na, na_dev, B, nb,
+    GQF_LAUNCH((k_rb_splits<uint32_t, false>), dim3(((p.nsub + 1) * RB_SPLIT_LANES + 255) / 256), dim3(256), 0, s, A, na, na_dev, B, nb,
                p.nsub, splits, (const uint32_t*)nullptr);
   GQF_LAUNCH_BC(nknown * 4, na_dev, 4, (k_rb_agg<MODE, MERGE>), dim3(p.G), dim3(RB_THREADS), 0, s, A, na, na_dev, B,
                 nb, flags, splits, g.q, g.r, p.chunk, agg, counters);
