@@ -1355,6 +1355,30 @@ __device__ __forceinline__ FpArray<K, PK> fp_array(const void* p, const uint32_t
   return a;
 }
 
+// The first index in [lo, hi) where below(m) fails (below holds, then
+// fails), or hi: an 8-ary search by a group of 8 lanes (threadIdx.x % 8 is
+// the lane's place in its group).  Each round probes 8 points that cut the
+// range into 9 parts, so the chain of dependent loads is log9 of the range
+// long instead of log2.  The groups of a wave may search different ranges;
+// every lane of the wave calls it (ballots inside).
+template <typename P>
+__device__ __forceinline__ uint32_t group8_search(uint32_t lo, uint32_t hi, P below) {
+  const uint32_t j = threadIdx.x & 7u, sh = threadIdx.x & 56u;  // sh: the group's first lane in the wave
+  while (__any(hi - lo >= 9)) {
+    const uint64_t span = hi - lo;
+    const bool active = span >= 9;
+    const bool b = active && below(lo + (uint32_t)(span * (j + 1) / 9));  // a point in (lo, hi)
+    const uint32_t c = (uint32_t)__popcll((__ballot(b) >> sh) & 0xFFull);  // probes where below holds
+    if (active) {
+      const uint32_t nlo = c ? lo + (uint32_t)(span * c / 9) + 1 : lo;
+      hi = c < 8 ? lo + (uint32_t)(span * (c + 1) / 9) : hi;
+      lo = nlo;
+    }
+  }
+  const bool b = lo + j < hi && below(lo + j);
+  return lo + (uint32_t)__popcll((__ballot(b) >> sh) & 0xFFull);
+}
+
 // ====================================================================== merge
 // Number of A entries among the first d outputs of the stable merge (A first
 // on ties).  A and B: pointers or FpArray views.

@@ -73,12 +73,10 @@ __device__ __forceinline__ RS rs_wave_excl(RS x, RS* total) {
 __device__ __forceinline__ void rb_stage_wait() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
 // Merge-path split (A entries among the first u*RB_TILE outputs) of every
-// sub-tile boundary, RB_SPLIT_LANES lanes per split: an 8-ary search (each
-// round probes 8 points that cut the range into 9 parts), so the chain of
-// dependent loads is log9 of the range long instead of log2.  The predicate
-// A[m] <= B[d - 1 - m] holds below the split and fails from it on (A first
-// on ties).  na_dev (may be NULL): A's count on the device, overriding na.
-// PK: A is packed (FpArray) with its bounds at hb.
+// sub-tile boundary, RB_SPLIT_LANES lanes per split (group8_search): the
+// predicate A[m] <= B[d - 1 - m] holds below the split and fails from it on
+// (A first on ties).  na_dev (may be NULL): A's count on the device,
+// overriding na.  PK: A is packed (FpArray) with its bounds at hb.
 constexpr uint32_t RB_SPLIT_LANES = 8;
 template <typename K, bool PK>
 __global__ __launch_bounds__(256) void k_rb_splits(const K* __restrict__ A, uint32_t na,
@@ -86,29 +84,15 @@ __global__ __launch_bounds__(256) void k_rb_splits(const K* __restrict__ A, uint
                                                    uint32_t nb, uint32_t nsub, uint32_t* __restrict__ splits,
                                                    const uint32_t* __restrict__ hb) {
   const uint32_t gt = blockIdx.x * 256 + threadIdx.x;
-  const uint32_t u = gt / RB_SPLIT_LANES, j = gt % RB_SPLIT_LANES;
-  const uint32_t sh = (threadIdx.x & 63u) & ~(RB_SPLIT_LANES - 1u);  // this group's ballot bits
+  const uint32_t u = gt / RB_SPLIT_LANES;
   if (na_dev) na = *na_dev;
   const auto AA = fp_array<K, PK>(A, hb);
   const uint64_t N = (uint64_t)na + nb;
   // every lane stays for the ballots; groups past nsub search an empty range
   const uint32_t d = u <= nsub ? (uint32_t)((uint64_t)u * RB_TILE < N ? (uint64_t)u * RB_TILE : N) : 0u;
-  uint32_t lo = d > nb ? d - nb : 0u, hi = d < na ? d : na;
-  auto below = [&](uint32_t m) { return AA[m] <= B[d - 1 - m]; };  // m in [lo, hi)
-  while (__any(hi - lo >= 9)) {
-    const uint64_t span = hi - lo;
-    const bool active = span >= 9;
-    const bool b = active && below(lo + (uint32_t)(span * (j + 1) / 9));
-    const uint32_t c = (uint32_t)__popcll((__ballot(b) >> sh) & 0xFFull);  // probes below the split
-    if (active) {
-      const uint32_t nlo = c ? lo + (uint32_t)(span * c / 9) + 1 : lo;
-      hi = c < 8 ? lo + (uint32_t)(span * (c + 1) / 9) : hi;
-      lo = nlo;
-    }
-  }
-  const bool b = lo + j < hi && below(lo + j);
-  const uint32_t c = (uint32_t)__popcll((__ballot(b) >> sh) & 0xFFull);
-  if (j == 0 && u <= nsub) splits[u] = lo + c;
+  const uint32_t lo = d > nb ? d - nb : 0u, hi = d < na ? d : na;
+  const uint32_t a = group8_search(lo, hi, [&](uint32_t m) { return AA[m] <= B[d - 1 - m]; });
+  if (gt % RB_SPLIT_LANES == 0 && u <= nsub) splits[u] = a;
 }
 
 struct RbPlan {

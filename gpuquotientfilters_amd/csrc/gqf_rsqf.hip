@@ -1028,21 +1028,19 @@ __device__ __forceinline__ uint32_t wm_cnt_get(const uint32_t* s_cnt, uint32_t p
 }
 
 // Each window's range of B: bstart[x] = first fingerprint with quotient >=
-// 4096 x (x = 0 .. W).  Also resets the counters the insert uses.
+// RD_SLOTS x (x = 0 .. W), 8 lanes per window (group8_search).  Also resets
+// the counters the insert uses.
+constexpr uint32_t WB_LANES = 8;
 template <typename K>
 __global__ __launch_bounds__(256) void k_wm_bounds(const K* __restrict__ B, uint32_t n, uint32_t r, uint32_t W,
                                                    uint32_t* __restrict__ bstart, uint32_t* __restrict__ ctr) {
   if (blockIdx.x == 0 && threadIdx.x >= CTR_OP && threadIdx.x < CTR_TOTAL) ctr[threadIdx.x] = 0u;
-  const uint32_t x = blockIdx.x * 256 + threadIdx.x;
-  if (x > W) return;
+  const uint32_t gt = blockIdx.x * 256 + threadIdx.x;
+  const uint32_t x = gt / WB_LANES;
+  // every lane stays for the ballots; groups past W search an empty range
   const uint64_t key = ((uint64_t)x * RD_SLOTS) << r;
-  uint32_t lo = 0, hi = n;
-  while (lo < hi) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if ((uint64_t)B[mid] < key) lo = mid + 1;
-    else hi = mid;
-  }
-  bstart[x] = lo;
+  const uint32_t b = group8_search(0u, x <= W ? n : 0u, [&](uint32_t m) { return (uint64_t)B[m] < key; });
+  if (gt % WB_LANES == 0 && x <= W) bstart[x] = b;
 }
 
 // Window w's scan state (agg[w]) from the entry counts of its quotients;
@@ -1738,7 +1736,8 @@ static gqf_status rsqf_insert_win(hipStream_t s, DeviceState& ds, Arena& ar, con
   const double meta = (double)g.nb * 16;  // occupieds + run ends
   GQF_LAUNCH_B(meta, k_rd_pop, dim3((W + 3) / 4), dim3(256), 0, s, d.meta, g.nb, W, wp, (uint32_t*)nullptr, 0u);
   GQF_TRY(scan_rows<2>(s, ar, wp, W, tot));
-  GQF_LAUNCH((k_wm_bounds<K>), dim3((W + 1 + 255) / 256), dim3(256), 0, s, B, n, g.r, W, bstart, ds.counters);
+  GQF_LAUNCH((k_wm_bounds<K>), dim3(((W + 1) * WB_LANES + 255) / 256), dim3(256), 0, s, B, n, g.r, W, bstart,
+             ds.counters);
   GQF_LAUNCH_B(meta * 1.5 + (double)n * sizeof(K), (k_wm_sum<K>), dim3(W), dim3(WS_THREADS), 0, s, d, wp, B, bstart,
                agg, wold, ds.counters);
   GQF_TRY(rb_scan(s, ar, agg, W, g.cap, ds.counters, nullptr));
