@@ -516,14 +516,22 @@ __device__ __forceinline__ void rb_stage_issue(const uint32_t* __restrict__ A, c
                                                uint32_t a0, uint32_t a1, uint32_t d0, uint32_t tn, uint32_t* buf,
                                                uint32_t* prev2) {
   const uint32_t la = a1 - a0, b0 = d0 - a0;
-  const uint32_t wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // wid is uniform in the wave: scalar, so the LDS destinations are too
+  const uint32_t wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  // this lane's sources for step 0, as addresses (the B one is used only
+  // where pos >= la); each step moves RB_WAVES * 64 positions on, so the
+  // steps need no table bases
+  const uint32_t pos0 = wid * 64 + lane;
+  const uintptr_t pa = reinterpret_cast<uintptr_t>(A) + 4 * ((uintptr_t)a0 + pos0);
+  const uintptr_t pb = reinterpret_cast<uintptr_t>(B) + 4 * ((uintptr_t)b0 + pos0) - 4 * (uintptr_t)la;
 #pragma unroll
   for (int k = 0; k < RB_ITEMS; ++k) {
     // wave wid, step k: the 64 consecutive positions from base (lane l -> buf[base + l])
     const uint32_t base = ((uint32_t)k * RB_WAVES + wid) * 64;
     const uint32_t pos = base + lane;
     if (FULL || pos < tn) {
-      const uint32_t* src = pos < la ? A + (a0 + pos) : B + (b0 + (pos - la));
+      const uintptr_t step = (uintptr_t)k * RB_WAVES * 64 * 4;
+      const uint32_t* src = reinterpret_cast<const uint32_t*>((pos < la ? pa : pb) + step);
       __builtin_amdgcn_global_load_lds(src, buf + base, 4, 0, 0);
     }
   }
